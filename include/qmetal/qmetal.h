@@ -1,0 +1,156 @@
+/* qmetal.h -- stable C ABI of the MI355X statevector simulator.
+ *
+ * Drop-in for the reference ABI (include/qmetal/qmetal.h of matiasrodlo/qmetal):
+ * the same 27 entry points, types, gate codes and version macros, with the
+ * same conventions. Everything after the "appended" marker is new and only
+ * ever appended to.
+ *
+ * Convention: return 0 on success and -1 on failure, with qm_last_error()
+ * holding the message for the calling thread. Every entry point catches;
+ * nothing throws across the ABI. Handle constructors return NULL on failure.
+ *
+ * Index order is LSB-first: qubit q occupies bit q of the basis index.
+ * Amplitudes are complex64 stored interleaved (re, im, re, im, ...), resident
+ * in device memory (HBM) on the current HIP device.
+ */
+
+#ifndef QMETAL_H
+#define QMETAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QMETAL_VERSION_MAJOR 0
+#define QMETAL_VERSION_MINOR 1
+#define QMETAL_VERSION_PATCH 0
+
+/* Gate codes. ABI: append only, never renumber (reference qmetal.h:28-36). */
+typedef enum {
+  QM_X = 0, QM_Y = 1, QM_Z = 2, QM_H = 3,
+  QM_S = 4, QM_SDG = 5, QM_T = 6, QM_TDG = 7,
+  QM_RX = 8, QM_RY = 9, QM_RZ = 10, QM_P = 11,
+  QM_CX = 12, QM_CY = 13, QM_CZ = 14, QM_SWAP = 15,
+  QM_CP = 16,
+  QM_CCX = 17,
+  /* appended: general one-qubit unitary U3(theta, phi, lambda) =
+   * [[cos(t/2), -e^{i l} sin(t/2)], [e^{i p} sin(t/2), e^{i(p+l)} cos(t/2)]] */
+  QM_U3 = 18
+} qm_gate_kind;
+
+typedef struct qm_simulator_s *qm_simulator;
+typedef struct qm_circuit_s *qm_circuit;
+
+/* A Hamiltonian as parallel arrays: one coefficient and three Pauli masks per
+ * term (reference qmetal.h:41-50). */
+typedef struct {
+  uint32_t num_terms;
+  const double *coeffs;
+  const uint64_t *x_masks;
+  const uint64_t *y_masks;
+  const uint64_t *z_masks;
+} qm_hamiltonian;
+
+/* --- diagnostics (reference qmetal.h:54-60) -------------------------------- */
+
+void qm_version(int *major, int *minor, int *patch);
+const char *qm_last_error(void);          /* thread-local, valid until next call */
+int qm_is_available(void);                /* 1 if a HIP device exists */
+const char *qm_device_name(void);
+uint64_t qm_max_buffer_length(void);      /* device memory in bytes */
+uint64_t qm_recommended_working_set(void);/* free device memory in bytes */
+uint32_t qm_max_qubits(void);             /* 34 on MI355X */
+
+/* --- circuits (reference qmetal.h:64-73) ----------------------------------- */
+
+qm_circuit qm_circuit_create(uint32_t num_qubits);
+void qm_circuit_destroy(qm_circuit c);
+int qm_circuit_add(qm_circuit c, qm_gate_kind kind, const uint32_t *qubits,
+                   uint32_t num_qubits, const double *params,
+                   uint32_t num_params);
+size_t qm_circuit_size(qm_circuit c);
+
+/* Number of GPU passes the circuit lowers to. With fuse = 0 this equals the
+ * gate count; with fuse = 1 it is what execution will actually cost. */
+size_t qm_circuit_plan_size(qm_circuit c, int fuse);
+
+/* --- simulator (reference qmetal.h:77-107) --------------------------------- */
+
+qm_simulator qm_create(uint32_t num_qubits);
+void qm_destroy(qm_simulator s);
+
+int qm_reset(qm_simulator s);
+int qm_run(qm_simulator s, qm_circuit c);         /* fused */
+int qm_run_unfused(qm_simulator s, qm_circuit c); /* ablation baseline */
+int qm_synchronize(qm_simulator s);
+
+uint32_t qm_num_qubits(qm_simulator s);
+uint64_t qm_dim(qm_simulator s);
+uint64_t qm_dispatch_count(qm_simulator s);
+
+/* Host copy of the interleaved state, refreshed by each call, valid until the
+ * next mutation or call. Length is 2 * qm_dim(). Returns NULL on error. */
+const float *qm_amplitudes(qm_simulator s);
+
+int qm_norm(qm_simulator s, double *out);
+int qm_expectation_pauli(qm_simulator s, uint64_t x_mask, uint64_t y_mask,
+                         uint64_t z_mask, double *out);
+int qm_expectation(qm_simulator s, const qm_hamiltonian *h, double *out);
+
+/* Draws `shots` basis-state indices without collapsing the state. `out` must
+ * hold `shots` entries. Identical seeds give identical samples. */
+int qm_sample(qm_simulator s, size_t shots, uint64_t seed, uint64_t *out);
+
+/* Adjoint gradient. `out_grad` must hold qm_circuit_size(c) entries; gates
+ * carrying no parameter get 0. Fails if a parameterised gate has no native
+ * generator (only RX, RY, RZ do). */
+int qm_energy_and_gradient(qm_simulator s, qm_circuit c,
+                           const qm_hamiltonian *h, double *out_energy,
+                           double *out_grad);
+
+/* === appended ============================================================ */
+
+/* Fusion switches (reference include/qmetal/plan.h:76-94). qm_run uses
+ * {1, 1, 1, 4, 0}; qm_run_unfused uses {0, 0, 0, 1, 0}. block_bits >= 2
+ * enables the LDS-staged Blocked1q pass (2..12). */
+typedef struct {
+  int collapse_1q_windows;
+  int fuse_diagonals;
+  int recognize_zz;
+  uint32_t group_1q;
+  uint32_t block_bits;
+} qm_fusion_options;
+
+int qm_run_options(qm_simulator s, qm_circuit c, const qm_fusion_options *o);
+
+/* Read-only view of a built plan. Op kinds: 0 Dense1q, 1 Dense1qGroup,
+ * 2 Blocked1q, 3 Controlled1q, 4 DiagonalLayer, 5 Swap, 6 CCX. */
+typedef struct qm_plan_s *qm_plan;
+typedef struct {
+  uint32_t kind;
+  uint32_t qubits[3];
+  double matrix[8];       /* Dense1q / Controlled1q, row-major (re, im) */
+  double global_phase;    /* DiagonalLayer */
+  uint32_t block_bits;    /* Blocked1q */
+  size_t num_terms;       /* DiagonalLayer */
+  size_t num_group;       /* Dense1qGroup / Blocked1q wires */
+} qm_plan_op;
+
+qm_plan qm_plan_build(qm_circuit c, const qm_fusion_options *o);
+void qm_plan_destroy(qm_plan p);
+size_t qm_plan_size(qm_plan p);
+int qm_plan_get(qm_plan p, size_t k, qm_plan_op *out);
+/* Arrays of op k, each may be NULL: group_qubits[num_group],
+ * group_matrices[8 * num_group], masks[num_terms], angles[num_terms]. */
+int qm_plan_get_arrays(qm_plan p, size_t k, uint32_t *group_qubits,
+                       double *group_matrices, uint64_t *masks,
+                       double *angles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QMETAL_H */

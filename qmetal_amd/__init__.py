@@ -1,0 +1,391 @@
+"""Python bindings for the MI355X statevector simulator.
+
+A ctypes wrapper over the C ABI in include/qmetal/qmetal.h, with the classes
+and methods of the reference's Python package (python/qmetal/__init__.py):
+``Circuit``, ``Simulator``, ``Hamiltonian``, ``QMetalError``, ``version``,
+``is_available``, ``device_name``, ``max_qubits``. ``import qmetal_amd as
+qmetal`` is a drop-in for that path.
+
+    import qmetal_amd as qmetal
+    c = qmetal.Circuit(4)
+    c.h(0)
+    for i in range(3):
+        c.cx(i, i + 1)
+    sim = qmetal.Simulator(4)
+    sim.run(c)
+    print(sim.counts(1000))
+
+Differences from the reference, all forced by discrete HBM or appended:
+``amplitudes()`` returns a host copy (numpy complex64) refreshed per call, not
+a zero-copy view; ``max_qubits()`` is 34; ``Circuit.u3`` and
+``Simulator.run(..., options=...)`` / ``Circuit.plan(...)`` are appended.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_float,
+                    c_int, c_size_t, c_uint32, c_uint64, c_void_p)
+
+__all__ = ["Circuit", "Simulator", "Hamiltonian", "QMetalError", "FusionOptions",
+           "version", "is_available", "device_name", "max_qubits", "library_path"]
+
+
+class QMetalError(RuntimeError):
+    """Raised when the native library reports a failure."""
+
+
+def _find_library() -> str:
+    env = os.environ.get("QMETAL_LIBRARY")
+    if env:
+        return env
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, "libqmetal.so")
+    if os.path.exists(path):
+        return path
+    raise QMetalError("libqmetal.so not found. Build it with `python -m qmetal_amd.build`, "
+                      "or point QMETAL_LIBRARY at it.")
+
+
+_path = _find_library()
+_lib = ctypes.CDLL(_path)
+
+
+def library_path() -> str:
+    return _path
+
+
+def _decl(name, restype, *argtypes):
+    fn = getattr(_lib, name)
+    fn.restype = restype
+    fn.argtypes = list(argtypes)
+    return fn
+
+
+_u64p, _f64p, _u32p = POINTER(c_uint64), POINTER(c_double), POINTER(c_uint32)
+
+
+class _Hamiltonian(Structure):
+    _fields_ = [("num_terms", c_uint32), ("coeffs", _f64p), ("x_masks", _u64p),
+                ("y_masks", _u64p), ("z_masks", _u64p)]
+
+
+class FusionOptions(Structure):
+    """qm_fusion_options: the reference's FusionOptions (plan.h:76-94)."""
+    _fields_ = [("collapse_1q_windows", c_int), ("fuse_diagonals", c_int),
+                ("recognize_zz", c_int), ("group_1q", c_uint32), ("block_bits", c_uint32)]
+
+    @classmethod
+    def default(cls):
+        return cls(1, 1, 1, 4, 0)
+
+    @classmethod
+    def none(cls):
+        return cls(0, 0, 0, 1, 0)
+
+    @classmethod
+    def blocked(cls, bits: int):
+        return cls(1, 1, 1, 4, int(bits))
+
+
+class _PlanOp(Structure):
+    _fields_ = [("kind", c_uint32), ("qubits", c_uint32 * 3), ("matrix", c_double * 8),
+                ("global_phase", c_double), ("block_bits", c_uint32),
+                ("num_terms", c_size_t), ("num_group", c_size_t)]
+
+
+_version = _decl("qm_version", None, POINTER(c_int), POINTER(c_int), POINTER(c_int))
+_last_error = _decl("qm_last_error", c_char_p)
+_is_available = _decl("qm_is_available", c_int)
+_device_name = _decl("qm_device_name", c_char_p)
+_max_qubits = _decl("qm_max_qubits", c_uint32)
+_max_buffer = _decl("qm_max_buffer_length", c_uint64)
+
+_circuit_create = _decl("qm_circuit_create", c_void_p, c_uint32)
+_circuit_destroy = _decl("qm_circuit_destroy", None, c_void_p)
+_circuit_add = _decl("qm_circuit_add", c_int, c_void_p, c_int, _u32p, c_uint32, _f64p, c_uint32)
+_circuit_size = _decl("qm_circuit_size", c_size_t, c_void_p)
+_plan_size = _decl("qm_circuit_plan_size", c_size_t, c_void_p, c_int)
+
+_create = _decl("qm_create", c_void_p, c_uint32)
+_destroy = _decl("qm_destroy", None, c_void_p)
+_reset = _decl("qm_reset", c_int, c_void_p)
+_run = _decl("qm_run", c_int, c_void_p, c_void_p)
+_run_unfused = _decl("qm_run_unfused", c_int, c_void_p, c_void_p)
+_run_options = _decl("qm_run_options", c_int, c_void_p, c_void_p, POINTER(FusionOptions))
+_sync = _decl("qm_synchronize", c_int, c_void_p)
+_num_qubits = _decl("qm_num_qubits", c_uint32, c_void_p)
+_dim = _decl("qm_dim", c_uint64, c_void_p)
+_dispatch_count = _decl("qm_dispatch_count", c_uint64, c_void_p)
+_amplitudes = _decl("qm_amplitudes", POINTER(c_float), c_void_p)
+_norm = _decl("qm_norm", c_int, c_void_p, _f64p)
+_expect_pauli = _decl("qm_expectation_pauli", c_int, c_void_p, c_uint64, c_uint64, c_uint64, _f64p)
+_expect = _decl("qm_expectation", c_int, c_void_p, POINTER(_Hamiltonian), _f64p)
+_sample = _decl("qm_sample", c_int, c_void_p, c_size_t, c_uint64, _u64p)
+_energy_grad = _decl("qm_energy_and_gradient", c_int, c_void_p, c_void_p,
+                     POINTER(_Hamiltonian), _f64p, _f64p)
+
+_plan_build = _decl("qm_plan_build", c_void_p, c_void_p, POINTER(FusionOptions))
+_plan_destroy = _decl("qm_plan_destroy", None, c_void_p)
+_plan_nops = _decl("qm_plan_size", c_size_t, c_void_p)
+_plan_get = _decl("qm_plan_get", c_int, c_void_p, c_size_t, POINTER(_PlanOp))
+_plan_arrays = _decl("qm_plan_get_arrays", c_int, c_void_p, c_size_t, _u32p, _f64p, _u64p, _f64p)
+
+
+def _err() -> str:
+    return _last_error().decode("utf-8", "replace")
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise QMetalError(_err())
+
+
+# Gate codes: qm_gate_kind in qmetal.h.
+_X, _Y, _Z, _H, _S, _SDG, _T, _TDG = range(8)
+_RX, _RY, _RZ, _P = 8, 9, 10, 11
+_CX, _CY, _CZ, _SWAP, _CP, _CCX = 12, 13, 14, 15, 16, 17
+_U3 = 18
+
+PLAN_KINDS = ("Dense1q", "Dense1qGroup", "Blocked1q", "Controlled1q", "DiagonalLayer",
+              "Swap", "CCX")
+
+
+def version() -> tuple[int, int, int]:
+    a, b, c = c_int(), c_int(), c_int()
+    _version(byref(a), byref(b), byref(c))
+    return a.value, b.value, c.value
+
+
+def is_available() -> bool:
+    return bool(_is_available())
+
+
+def device_name() -> str:
+    return _device_name().decode()
+
+
+def max_qubits() -> int:
+    return int(_max_qubits())
+
+
+class Hamiltonian:
+    """A real-weighted sum of Pauli strings, given as bitmasks."""
+
+    def __init__(self):
+        self.terms: list[tuple[float, int, int, int]] = []
+
+    def add(self, coeff: float, x_mask: int = 0, y_mask: int = 0,
+            z_mask: int = 0) -> "Hamiltonian":
+        self.terms.append((float(coeff), int(x_mask), int(y_mask), int(z_mask)))
+        return self
+
+    def add_string(self, coeff: float, s: str) -> "Hamiltonian":
+        """`s` is "IXYZ"-style, leftmost character is qubit 0."""
+        x = y = z = 0
+        for q, ch in enumerate(s.upper()):
+            bit = 1 << q
+            if ch == "X":
+                x |= bit
+            elif ch == "Y":
+                y |= bit
+            elif ch == "Z":
+                z |= bit
+            elif ch != "I":
+                raise ValueError(f"bad Pauli character {ch!r}")
+        return self.add(coeff, x, y, z)
+
+    def _c(self):
+        n = len(self.terms)
+        coeffs = (c_double * n)(*[t[0] for t in self.terms])
+        xs = (c_uint64 * n)(*[t[1] for t in self.terms])
+        ys = (c_uint64 * n)(*[t[2] for t in self.terms])
+        zs = (c_uint64 * n)(*[t[3] for t in self.terms])
+        h = _Hamiltonian(n, coeffs, xs, ys, zs)
+        h._keep = (coeffs, xs, ys, zs)  # the arrays must outlive the struct
+        return h
+
+
+class Circuit:
+    """A gate stream. `gates` mirrors what was added: (kind, qubits, params)."""
+
+    def __init__(self, num_qubits: int):
+        self.num_qubits = int(num_qubits)
+        self.gates: list[tuple[int, tuple[int, ...], tuple[float, ...]]] = []
+        self._h = _circuit_create(self.num_qubits)
+        if not self._h:
+            raise QMetalError(_err())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _circuit_destroy(self._h)
+            self._h = None
+
+    def __len__(self) -> int:
+        return int(_circuit_size(self._h))
+
+    def add(self, kind: int, qubits, params=()):
+        qubits = tuple(int(q) for q in qubits)
+        params = tuple(float(p) for p in params)
+        qs = (c_uint32 * len(qubits))(*qubits)
+        ps = (c_double * len(params))(*params) if params else None
+        _check(_circuit_add(self._h, int(kind), qs, len(qubits), ps, len(params)))
+        self.gates.append((int(kind), qubits, params))
+        return self
+
+    _add = add
+
+    # Single-qubit
+    def x(self, q): return self._add(_X, (q,))
+    def y(self, q): return self._add(_Y, (q,))
+    def z(self, q): return self._add(_Z, (q,))
+    def h(self, q): return self._add(_H, (q,))
+    def s(self, q): return self._add(_S, (q,))
+    def sdg(self, q): return self._add(_SDG, (q,))
+    def t(self, q): return self._add(_T, (q,))
+    def tdg(self, q): return self._add(_TDG, (q,))
+    def rx(self, q, theta): return self._add(_RX, (q,), (theta,))
+    def ry(self, q, theta): return self._add(_RY, (q,), (theta,))
+    def rz(self, q, theta): return self._add(_RZ, (q,), (theta,))
+    def p(self, q, theta): return self._add(_P, (q,), (theta,))
+    def u3(self, q, theta, phi, lam): return self._add(_U3, (q,), (theta, phi, lam))
+
+    # Multi-qubit
+    def cx(self, c, t): return self._add(_CX, (c, t))
+    def cy(self, c, t): return self._add(_CY, (c, t))
+    def cz(self, a, b): return self._add(_CZ, (a, b))
+    def swap(self, a, b): return self._add(_SWAP, (a, b))
+    def cp(self, a, b, theta): return self._add(_CP, (a, b), (theta,))
+    def ccx(self, c0, c1, t): return self._add(_CCX, (c0, c1, t))
+
+    def plan_size(self, fuse: bool = True) -> int:
+        """GPU passes this circuit lowers to -- what execution actually costs."""
+        return int(_plan_size(self._h, 1 if fuse else 0))
+
+    def plan(self, options: FusionOptions | None = None) -> list[dict]:
+        """The lowered plan as a list of dicts (kind name, wires, matrices,
+        masks, angles, global phase), as qm_plan_build returns it."""
+        opts = options if options is not None else FusionOptions.default()
+        p = _plan_build(self._h, byref(opts))
+        if not p:
+            raise QMetalError(_err())
+        try:
+            out = []
+            for k in range(int(_plan_nops(p))):
+                op = _PlanOp()
+                _check(_plan_get(p, k, byref(op)))
+                ng, nt = int(op.num_group), int(op.num_terms)
+                gq = (c_uint32 * max(ng, 1))()
+                gm = (c_double * max(8 * ng, 1))()
+                mk = (c_uint64 * max(nt, 1))()
+                an = (c_double * max(nt, 1))()
+                _check(_plan_arrays(p, k, gq, gm, mk, an))
+                m = list(op.matrix)
+                out.append({
+                    "kind": PLAN_KINDS[op.kind],
+                    "kind_id": int(op.kind),
+                    "qubits": list(op.qubits),
+                    "matrix": [complex(m[2 * i], m[2 * i + 1]) for i in range(4)],
+                    "group_qubits": list(gq)[:ng],
+                    "group_matrices": [complex(gm[2 * i], gm[2 * i + 1]) for i in range(4 * ng)],
+                    "masks": list(mk)[:nt],
+                    "angles": list(an)[:nt],
+                    "global_phase": float(op.global_phase),
+                    "block_bits": int(op.block_bits),
+                })
+            return out
+        finally:
+            _plan_destroy(p)
+
+
+class Simulator:
+    def __init__(self, num_qubits: int):
+        self._h = _create(int(num_qubits))
+        if not self._h:
+            raise QMetalError(_err())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _destroy(self._h)
+            self._h = None
+
+    def close(self):
+        self.__del__()
+
+    def __enter__(self): return self
+    def __exit__(self, *exc): self.__del__()
+
+    @property
+    def num_qubits(self) -> int: return int(_num_qubits(self._h))
+    @property
+    def dim(self) -> int: return int(_dim(self._h))
+    @property
+    def dispatch_count(self) -> int: return int(_dispatch_count(self._h))
+
+    def reset(self): _check(_reset(self._h)); return self
+
+    def run(self, circuit: Circuit, fuse: bool = True, options: FusionOptions | None = None):
+        if options is not None:
+            _check(_run_options(self._h, circuit._h, byref(options)))
+        else:
+            _check((_run if fuse else _run_unfused)(self._h, circuit._h))
+        return self
+
+    def synchronize(self): _check(_sync(self._h)); return self
+
+    def amplitudes(self):
+        """Host copy of the state as numpy complex64 (a list without numpy)."""
+        p = _amplitudes(self._h)
+        if not p:
+            raise QMetalError(_err())
+        n = self.dim
+        try:
+            import numpy as np
+        except ImportError:
+            return [complex(p[2 * i], p[2 * i + 1]) for i in range(n)]
+        buf = np.ctypeslib.as_array(p, shape=(2 * n,))
+        return buf.view(np.complex64).copy()
+
+    def norm(self) -> float:
+        out = c_double()
+        _check(_norm(self._h, byref(out)))
+        return out.value
+
+    def expectation(self, observable) -> float:
+        out = c_double()
+        if isinstance(observable, Hamiltonian):
+            h = observable._c()
+        elif isinstance(observable, str):
+            h = Hamiltonian().add_string(1.0, observable)._c()
+        else:
+            raise TypeError("observable must be a Hamiltonian or a Pauli string")
+        _check(_expect(self._h, byref(h), byref(out)))
+        return out.value
+
+    def expectation_pauli(self, x: int = 0, y: int = 0, z: int = 0) -> float:
+        out = c_double()
+        _check(_expect_pauli(self._h, int(x), int(y), int(z), byref(out)))
+        return out.value
+
+    def sample(self, shots: int, seed: int = 0):
+        buf = (c_uint64 * int(shots))()
+        _check(_sample(self._h, int(shots), int(seed), buf))
+        return list(buf)
+
+    def counts(self, shots: int, seed: int = 0) -> dict[str, int]:
+        """Bitstring counts, little-endian: qubit 0 is the rightmost character."""
+        n = self.num_qubits
+        out: dict[str, int] = {}
+        for v in self.sample(shots, seed):
+            key = format(v, f"0{n}b")
+            out[key] = out.get(key, 0) + 1
+        return dict(sorted(out.items(), key=lambda kv: -kv[1]))
+
+    def energy_and_gradient(self, circuit: Circuit, hamiltonian: Hamiltonian):
+        """Adjoint gradient: one entry per gate, zero where unparameterised."""
+        h = hamiltonian._c()
+        energy = c_double()
+        grad = (c_double * max(len(circuit), 1))()
+        _check(_energy_grad(self._h, circuit._h, byref(h), byref(energy), grad))
+        return energy.value, list(grad)[:len(circuit)]

@@ -1,0 +1,150 @@
+// kernels.h -- launch interface of the gfx950 statevector kernels.
+//
+// Every launcher enqueues on the given stream and returns immediately; none
+// synchronises. Argument structs travel by value as kernel arguments.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qmamd {
+
+// One 2x2 applied inside a gather tile: `lbit` is the tile-local bit it acts
+// on, `cmask` the tile-local bits that must all be 1 for it to apply
+// (controls that live inside the tile). Matrix row-major, (re, im) pairs.
+struct Slot {
+  uint8_t lbit;
+  uint8_t cmask;
+  uint8_t pad[2];
+  float m[8];
+};
+
+// Gather layout. Thread T owns the 2^NL amplitudes
+//   deposit(T) | fixed | sum_k bit_k(r) << rpos[k] | e,   r < 2^(NL-1), e < 2,
+// where deposit() inserts zero bits at the ascending positions ins[], which
+// are {0} + rpos[] + the fixed-one bits.
+struct Layout {
+  uint64_t nthreads;
+  uint64_t fixed;
+  uint32_t nins;
+  uint8_t ins[12];
+  uint8_t rpos[8];
+};
+
+struct GatherArgs {
+  float2 *a;   // state
+  float2 *b;   // optional second state receiving the same op (or nullptr)
+  Layout lay;
+  uint32_t nslots;
+  Slot slots[4];
+};
+
+// Diagonal phase layer after host-side classification, in turns (1 = 2 pi).
+// Tile-local bit k (k = 0 is physical bit 0, k >= 1 is rpos[k-1]) carries
+// linear coefficient beta0[k] and pair coefficient gamma[k][j]; terms on
+// non-local bits are read from `terms`:
+//   kind 0: linear, phi += ang if bit p of the base index is set
+//   kind 1: pair,   phi += ang if both bits of mask are set in the base
+//   kind 2: cross,  beta[k] += ang if bit p of the base index is set
+struct DiagTerm {
+  uint64_t mask;
+  double ang;
+  uint32_t kind;
+  uint32_t k;
+};
+
+struct DiagArgs {
+  float2 *a;
+  Layout lay;
+  double constant;
+  double beta0[4];
+  double gamma[4][4];
+  const DiagTerm *terms;
+  uint32_t nterms;
+};
+
+// General diagonal (masks of any weight), one amplitude pair per thread.
+struct DiagGenericArgs {
+  float2 *a;
+  uint64_t dim;
+  const uint64_t *masks;
+  const double *angles;  // turns
+  uint32_t nterms;
+  double global;         // turns
+};
+
+struct BlockedArgs {
+  float2 *a;
+  uint64_t nblocks;
+  uint32_t bits;
+  uint32_t count;
+  const Slot *gates;  // lbit = physical wire (< bits)
+};
+
+void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim);
+void launch_zero(hipStream_t s, float2 *a, uint64_t dim);
+void launch_gather(hipStream_t s, int nl, const GatherArgs &g);
+void launch_diag(hipStream_t s, int nl, const DiagArgs &d);
+void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d);
+void launch_blocked(hipStream_t s, const BlockedArgs &b);
+
+// FP64 reductions. Each writes per-block partials into `partials` (capacity
+// kRedBlocks doubles or double2) and then reduces them in a fixed order into
+// out[0] (and out[1]), so results are run-to-run deterministic.
+constexpr int kRedBlocks = 2048;
+constexpr int kRedThreads = 256;
+
+// out[0] = sum |a_i|^2
+void launch_abs2(hipStream_t s, const float2 *a, uint64_t dim, double *partials,
+                 double *out);
+// out[0..1] = sum_i s(j) conj(bra_i) ket_j,  j = i ^ flip,
+// s(j) = (-1)^popcount(j & sign). bra == ket gives the Pauli expectation sum.
+void launch_pauli(hipStream_t s, const float2 *bra, const float2 *ket,
+                  uint64_t dim, uint64_t flip, uint64_t sign, double *partials,
+                  double *out);
+
+// lam_i += sum_t coef_t * s_t(j) * psi_j over terms sharing one flip (j = i ^
+// flip); coefs are complex (c_t * i^ny_t) in double. Terms: sign masks and
+// coefficients, up to kAccumTerms per launch.
+constexpr int kAccumTerms = 32;
+struct AccumArgs {
+  float2 *lam;
+  const float2 *psi;
+  uint64_t dim;
+  uint64_t flip;
+  uint32_t nterms;
+  uint64_t sign[kAccumTerms];
+  double cre[kAccumTerms];
+  double cim[kAccumTerms];
+};
+void launch_pauli_accum(hipStream_t s, const AccumArgs &a);
+
+// Adjoint step for a rotation exp(-i t P / 2), P in {X, Y, Z} on wire q:
+// accumulates <lam|P|psi> (FP64, per-block partials reduced into
+// out[2*slot], out[2*slot+1]) and then applies the inverse rotation to both
+// states, in one pass over both.
+struct AdjArgs {
+  float2 *psi;
+  float2 *lam;
+  uint64_t dim;
+  uint32_t q;
+  uint32_t axis;  // 0 = X, 1 = Y, 2 = Z
+  float m[8];     // inverse rotation, row-major
+};
+void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,
+                        double *out2);
+
+// Sampling: FP64 sums of |a|^2 over 1024-amplitude blocks, an inclusive FP64
+// prefix over them, and per-shot resolution (splitmix64 stream, binary search
+// over blocks, in-block walk clamped to the last non-zero amplitude).
+constexpr uint64_t kSampleBlock = 1024;
+void launch_block_sums(hipStream_t s, const float2 *a, uint64_t dim,
+                       double *sums);
+void launch_prefix(hipStream_t s, double *v, uint64_t count, double *chunk);
+void launch_shots(hipStream_t s, const float2 *a, uint64_t dim,
+                  const double *cdf, uint64_t nblocks, uint64_t shots,
+                  uint64_t seed, uint64_t *out);
+
+}  // namespace qmamd

@@ -1,0 +1,651 @@
+// kernels.hip -- hand-written gfx950 kernels for in-place complex64
+// statevector passes. Wave64 throughout; every index is 64-bit so n = 34
+// (2^34 amplitudes, 128 GiB) is addressable; launches cap the grid and loop.
+//
+// Kernel families
+//   k_gather<NL>   2x2 slots (optionally controlled) applied to the 2^NL
+//                  amplitudes a thread owns: Dense1q, Group, Controlled1q,
+//                  Swap (three CX slots), CCX, and the gate-by-gate path.
+//                  Bit 0 always lives inside one 16-byte float4 load, so every
+//                  load and store is a full dwordx4; thread-index bits go to
+//                  the lowest free positions so neighbouring lanes touch
+//                  neighbouring 16-byte words.
+//   k_diag<NL>     phase layer with the phase written as
+//                  const + sum beta_k x_k + sum gamma_kj x_k x_j over the
+//                  thread's local bits after the base index has been folded in.
+//   k_blocked      contiguous 2^b tile staged in LDS, gates applied in order.
+//   reductions     FP64 accumulation from the first add; per-block partials
+//                  reduced by one block in a fixed order (deterministic).
+//   sampling       FP64 block sums, FP64 prefix, per-shot resolution on GPU.
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace qmamd {
+namespace {
+
+constexpr int kT = 256;             // threads per workgroup
+constexpr uint64_t kMaxGrid = 1u << 20;
+
+inline unsigned grid_for(uint64_t work) {
+  uint64_t g = (work + kT - 1) / kT;
+  if (g < 1) g = 1;
+  if (g > kMaxGrid) g = kMaxGrid;
+  return unsigned(g);
+}
+
+__device__ __forceinline__ uint64_t deposit(uint64_t t, const Layout &L) {
+  for (uint32_t k = 0; k < L.nins; k++) {
+    const uint32_t p = L.ins[k];
+    const uint64_t low = t & ((1ull << p) - 1ull);
+    t = ((t >> p) << (p + 1)) | low;
+  }
+  return t | L.fixed;
+}
+
+template <int NL>
+__device__ __forceinline__ void offsets(const Layout &L, uint64_t (&off)[1 << (NL - 1)]) {
+#pragma unroll
+  for (int r = 0; r < (1 << (NL - 1)); r++) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int k = 0; k < NL - 1; k++)
+      if ((r >> k) & 1) o |= 1ull << L.rpos[k];
+    off[r] = o;
+  }
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// x' = m00 x + m01 y, y' = m10 x + m11 y
+__device__ __forceinline__ void rot(const float *m, float2 &x, float2 &y) {
+  const float2 u = x, v = y;
+  x.x = m[0] * u.x - m[1] * u.y + m[2] * v.x - m[3] * v.y;
+  x.y = m[0] * u.y + m[1] * u.x + m[2] * v.y + m[3] * v.x;
+  y.x = m[4] * u.x - m[5] * u.y + m[6] * v.x - m[7] * v.y;
+  y.y = m[4] * u.y + m[5] * u.x + m[6] * v.y + m[7] * v.x;
+}
+
+template <int NL, int LB>
+__device__ __forceinline__ void apply_slot(float2 (&x)[1 << NL], const Slot &s) {
+  const uint32_t cm = s.cmask;
+#pragma unroll
+  for (int l = 0; l < (1 << NL); l++) {
+    if (l & (1 << LB)) continue;
+    if ((uint32_t(l) & cm) != cm) continue;
+    rot(s.m, x[l], x[l | (1 << LB)]);
+  }
+}
+
+template <int NL>
+__device__ __forceinline__ void apply_any(float2 (&x)[1 << NL], const Slot &s) {
+  switch (s.lbit) {
+    case 0: apply_slot<NL, 0>(x, s); break;
+    case 1: if constexpr (NL > 1) apply_slot<NL, 1>(x, s); break;
+    case 2: if constexpr (NL > 2) apply_slot<NL, 2>(x, s); break;
+    case 3: if constexpr (NL > 3) apply_slot<NL, 3>(x, s); break;
+    case 4: if constexpr (NL > 4) apply_slot<NL, 4>(x, s); break;
+    default: break;
+  }
+}
+
+template <int NL>
+__device__ __forceinline__ void gather_one(float2 *__restrict__ a, uint64_t base,
+                                           const uint64_t (&off)[1 << (NL - 1)],
+                                           const GatherArgs &g) {
+  float2 x[1 << NL];
+#pragma unroll
+  for (int r = 0; r < (1 << (NL - 1)); r++) {
+    const float4 v = *reinterpret_cast<const float4 *>(a + (base | off[r]));
+    x[2 * r] = make_float2(v.x, v.y);
+    x[2 * r + 1] = make_float2(v.z, v.w);
+  }
+  for (uint32_t k = 0; k < g.nslots; k++) apply_any<NL>(x, g.slots[k]);
+#pragma unroll
+  for (int r = 0; r < (1 << (NL - 1)); r++)
+    *reinterpret_cast<float4 *>(a + (base | off[r])) =
+        make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
+}
+
+template <int NL>
+__global__ __launch_bounds__(kT) void k_gather(GatherArgs g) {
+  uint64_t off[1 << (NL - 1)];
+  offsets<NL>(g.lay, off);
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  for (uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x; t < g.lay.nthreads;
+       t += stride) {
+    const uint64_t base = deposit(t, g.lay);
+    gather_one<NL>(g.a, base, off, g);
+    if (g.b) gather_one<NL>(g.b, base, off, g);
+  }
+}
+
+template <int NL>
+__global__ __launch_bounds__(kT) void k_diag(DiagArgs d) {
+  uint64_t off[1 << (NL - 1)];
+  offsets<NL>(d.lay, off);
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  for (uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x; t < d.lay.nthreads;
+       t += stride) {
+    const uint64_t base = deposit(t, d.lay);
+    double phi = d.constant;
+    double beta[4] = {d.beta0[0], d.beta0[1], d.beta0[2], d.beta0[3]};
+    for (uint32_t k = 0; k < d.nterms; k++) {
+      const DiagTerm tm = d.terms[k];
+      const bool hit = (base & tm.mask) == tm.mask;
+      if (tm.kind == 2) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (hit && tm.k == uint32_t(j)) beta[j] += tm.ang;
+      } else if (hit) {
+        phi += tm.ang;
+      }
+    }
+    double ph[1 << NL];
+    ph[0] = phi;
+#pragma unroll
+    for (int h = 0; h < NL; h++) {
+#pragma unroll
+      for (int l = 0; l < (1 << h); l++) {
+        double inc = beta[h];
+#pragma unroll
+        for (int k = 0; k < h; k++)
+          if ((l >> k) & 1) inc += d.gamma[k][h];
+        ph[l | (1 << h)] = ph[l] + inc;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < (1 << (NL - 1)); r++) {
+      float4 *p = reinterpret_cast<float4 *>(d.a + (base | off[r]));
+      float4 v = *p;
+      float s0, c0, s1, c1;
+      sincospif(float(2.0 * (ph[2 * r] - rint(ph[2 * r]))), &s0, &c0);
+      sincospif(float(2.0 * (ph[2 * r + 1] - rint(ph[2 * r + 1]))), &s1, &c1);
+      const float2 a0 = cmul(make_float2(v.x, v.y), make_float2(c0, s0));
+      const float2 a1 = cmul(make_float2(v.z, v.w), make_float2(c1, s1));
+      *p = make_float4(a0.x, a0.y, a1.x, a1.y);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_diag_generic(DiagGenericArgs d) {
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  for (uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x; i < d.dim;
+       i += stride) {
+    double ph = d.global;
+    for (uint32_t k = 0; k < d.nterms; k++)
+      if ((i & d.masks[k]) == d.masks[k]) ph += d.angles[k];
+    float s, c;
+    sincospif(float(2.0 * (ph - rint(ph))), &s, &c);
+    d.a[i] = cmul(d.a[i], make_float2(c, s));
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_blocked(BlockedArgs b) {
+  extern __shared__ float2 tile[];
+  const uint32_t block = 1u << b.bits, pairs = block >> 1;
+  for (uint64_t blk = blockIdx.x; blk < b.nblocks; blk += gridDim.x) {
+    float2 *g = b.a + blk * block;
+    for (uint32_t i = threadIdx.x; i < block; i += kT) tile[i] = g[i];
+    __syncthreads();
+    for (uint32_t k = 0; k < b.count; k++) {
+      const Slot s = b.gates[k];
+      const uint32_t q = s.lbit, stride = 1u << q;
+      for (uint32_t t = threadIdx.x; t < pairs; t += kT) {
+        const uint32_t i = ((t >> q) << (q + 1)) | (t & (stride - 1));
+        rot(s.m, tile[i], tile[i | stride]);
+      }
+      __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < block; i += kT) g[i] = tile[i];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_fill_basis(float2 *a, uint64_t dim) {
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  float4 *v = reinterpret_cast<float4 *>(a);
+  for (uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x; i < dim / 2; i += stride)
+    v[i] = make_float4(i == 0 ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kT) void k_zero(float2 *a, uint64_t dim) {
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  float4 *v = reinterpret_cast<float4 *>(a);
+  for (uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x; i < dim / 2; i += stride)
+    v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// --- FP64 reductions --------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Sum over the workgroup; thread 0 gets the total. Fixed order.
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kT / 64; k++) t += sh[k];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kT) void k_abs2(const float2 *a, uint64_t dim, double *part) {
+  __shared__ double sh[kT / 64];
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  const float4 *v = reinterpret_cast<const float4 *>(a);
+  double acc = 0.0;
+  for (uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x; i < dim / 2; i += stride) {
+    const float4 x = v[i];
+    acc += double(x.x) * x.x + double(x.y) * x.y + double(x.z) * x.z + double(x.w) * x.w;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__device__ __forceinline__ double sgn(uint64_t j, uint64_t sign) {
+  return (__popcll(j & sign) & 1) ? -1.0 : 1.0;
+}
+
+// acc += s(j) conj(x) y
+__device__ __forceinline__ void acc_cj(double &re, double &im, float2 x, float2 y, double s) {
+  re += s * (double(x.x) * y.x + double(x.y) * y.y);
+  im += s * (double(x.x) * y.y - double(x.y) * y.x);
+}
+
+// sum_i s(i ^ flip) conj(bra_i) ket_{i ^ flip}. Each amplitude is read once:
+// a thread takes an even index i with the flip's top bit clear together with
+// its partners, and adds both directions of the pairing.
+__global__ __launch_bounds__(kT) void k_pauli(const float2 *bra, const float2 *ket, uint64_t dim,
+                                              uint64_t flip, uint64_t sign, double *part) {
+  __shared__ double sh[kT / 64];
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  double re = 0.0, im = 0.0;
+  if (flip == 0) {
+    for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < dim / 2; u += stride) {
+      const uint64_t i = 2 * u;
+      const float4 b = *reinterpret_cast<const float4 *>(bra + i);
+      const float4 k = *reinterpret_cast<const float4 *>(ket + i);
+      acc_cj(re, im, make_float2(b.x, b.y), make_float2(k.x, k.y), sgn(i, sign));
+      acc_cj(re, im, make_float2(b.z, b.w), make_float2(k.z, k.w), sgn(i + 1, sign));
+    }
+  } else if (flip == 1) {
+    for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < dim / 2; u += stride) {
+      const uint64_t i = 2 * u;
+      const float4 b = *reinterpret_cast<const float4 *>(bra + i);
+      const float4 k = *reinterpret_cast<const float4 *>(ket + i);
+      // i -> j = i + 1, and i + 1 -> i
+      acc_cj(re, im, make_float2(b.x, b.y), make_float2(k.z, k.w), sgn(i + 1, sign));
+      acc_cj(re, im, make_float2(b.z, b.w), make_float2(k.x, k.y), sgn(i, sign));
+    }
+  } else {
+    const uint32_t h = 63 - __clzll(flip);  // top bit of flip, >= 1
+    const uint64_t fe = flip & ~1ull;
+    const bool odd = flip & 1ull;
+    for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < dim / 4; u += stride) {
+      // i: even, bit h clear
+      uint64_t t = u;
+      t = ((t >> (h - 1)) << h) | (t & ((1ull << (h - 1)) - 1ull));
+      const uint64_t i = t << 1;
+      const uint64_t p = i ^ fe;  // even partner base, bit h set
+      const float4 bi = *reinterpret_cast<const float4 *>(bra + i);
+      const float4 ki = *reinterpret_cast<const float4 *>(ket + i);
+      const float4 bp = *reinterpret_cast<const float4 *>(bra + p);
+      const float4 kp = *reinterpret_cast<const float4 *>(ket + p);
+      const float2 bi0 = make_float2(bi.x, bi.y), bi1 = make_float2(bi.z, bi.w);
+      const float2 ki0 = make_float2(ki.x, ki.y), ki1 = make_float2(ki.z, ki.w);
+      const float2 bp0 = make_float2(bp.x, bp.y), bp1 = make_float2(bp.z, bp.w);
+      const float2 kp0 = make_float2(kp.x, kp.y), kp1 = make_float2(kp.z, kp.w);
+      // partner of i is p (+1 if flip is odd); of i+1 is p+1 (-1 if odd)
+      const float2 kj0 = odd ? kp1 : kp0, kj1 = odd ? kp0 : kp1;
+      const float2 bj0 = odd ? bp1 : bp0, bj1 = odd ? bp0 : bp1;
+      const uint64_t j0 = i ^ flip, j1 = (i + 1) ^ flip;
+      acc_cj(re, im, bi0, kj0, sgn(j0, sign));
+      acc_cj(re, im, bi1, kj1, sgn(j1, sign));
+      acc_cj(re, im, bj0, ki0, sgn(i, sign));
+      acc_cj(re, im, bj1, ki1, sgn(i + 1, sign));
+    }
+  }
+  const double tr = block_sum(re, sh);
+  const double ti = block_sum(im, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = tr;
+    part[2 * blockIdx.x + 1] = ti;
+  }
+}
+
+// Reduce `count` partials with component stride `comps` into out[0..comps).
+__global__ __launch_bounds__(kT) void k_finish(const double *part, int count, int comps,
+                                               double *out) {
+  __shared__ double sh[kT / 64];
+  for (int c = 0; c < comps; c++) {
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < count; k += kT) acc += part[comps * k + c];
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[c] = t;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_pauli_accum(AccumArgs p) {
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  const uint64_t fe = p.flip & ~1ull;
+  const bool odd = p.flip & 1ull;
+  for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < p.dim / 2; u += stride) {
+    const uint64_t i = 2 * u, src = i ^ fe;
+    const float4 s = *reinterpret_cast<const float4 *>(p.psi + src);
+    float4 *l = reinterpret_cast<float4 *>(p.lam + i);
+    float4 lv = *l;
+    const float2 y0 = odd ? make_float2(s.z, s.w) : make_float2(s.x, s.y);
+    const float2 y1 = odd ? make_float2(s.x, s.y) : make_float2(s.z, s.w);
+    const uint64_t j0 = i ^ p.flip, j1 = (i + 1) ^ p.flip;
+    double c0r = 0, c0i = 0, c1r = 0, c1i = 0;
+    for (uint32_t k = 0; k < p.nterms; k++) {
+      const double s0 = sgn(j0, p.sign[k]), s1 = sgn(j1, p.sign[k]);
+      c0r += s0 * p.cre[k];
+      c0i += s0 * p.cim[k];
+      c1r += s1 * p.cre[k];
+      c1i += s1 * p.cim[k];
+    }
+    lv.x += float(c0r * y0.x - c0i * y0.y);
+    lv.y += float(c0r * y0.y + c0i * y0.x);
+    lv.z += float(c1r * y1.x - c1i * y1.y);
+    lv.w += float(c1r * y1.y + c1i * y1.x);
+    *l = lv;
+  }
+}
+
+// Fused adjoint step for a rotation on wire q: overlap <lam|P|psi> in FP64
+// (P = X, Y or Z on q, taken on the pair (i0, i1 = i0 | 2^q)), then the
+// inverse rotation on both states.
+__device__ __forceinline__ void adj_pair(const AdjArgs &a, float2 &p0, float2 &p1,
+                                         float2 &l0, float2 &l1, double &re, double &im) {
+  if (a.axis == 0) {         // X: conj(l0) p1 + conj(l1) p0
+    acc_cj(re, im, l0, p1, 1.0);
+    acc_cj(re, im, l1, p0, 1.0);
+  } else if (a.axis == 1) {  // Y (before i^ny): -conj(l0) p1 + conj(l1) p0
+    acc_cj(re, im, l0, p1, -1.0);
+    acc_cj(re, im, l1, p0, 1.0);
+  } else {                   // Z: conj(l0) p0 - conj(l1) p1
+    acc_cj(re, im, l0, p0, 1.0);
+    acc_cj(re, im, l1, p1, -1.0);
+  }
+  rot(a.m, p0, p1);
+  rot(a.m, l0, l1);
+}
+
+__global__ __launch_bounds__(kT) void k_adjoint_rot(AdjArgs a, double *part) {
+  __shared__ double sh[kT / 64];
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  double re = 0.0, im = 0.0;
+  if (a.q == 0) {
+    for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < a.dim / 2; u += stride) {
+      float4 *pp = reinterpret_cast<float4 *>(a.psi + 2 * u);
+      float4 *lp = reinterpret_cast<float4 *>(a.lam + 2 * u);
+      const float4 pv = *pp, lv = *lp;
+      float2 p0 = make_float2(pv.x, pv.y), p1 = make_float2(pv.z, pv.w);
+      float2 l0 = make_float2(lv.x, lv.y), l1 = make_float2(lv.z, lv.w);
+      adj_pair(a, p0, p1, l0, l1, re, im);
+      *pp = make_float4(p0.x, p0.y, p1.x, p1.y);
+      *lp = make_float4(l0.x, l0.y, l1.x, l1.y);
+    }
+  } else {
+    const uint32_t q = a.q;
+    const uint64_t bit = 1ull << q;
+    for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < a.dim / 4; u += stride) {
+      const uint64_t t = ((u >> (q - 1)) << q) | (u & ((1ull << (q - 1)) - 1ull));
+      const uint64_t i = t << 1;
+      float4 *pa = reinterpret_cast<float4 *>(a.psi + i);
+      float4 *pb = reinterpret_cast<float4 *>(a.psi + (i | bit));
+      float4 *la = reinterpret_cast<float4 *>(a.lam + i);
+      float4 *lb = reinterpret_cast<float4 *>(a.lam + (i | bit));
+      const float4 pav = *pa, pbv = *pb, lav = *la, lbv = *lb;
+      float2 p0 = make_float2(pav.x, pav.y), p1 = make_float2(pbv.x, pbv.y);
+      float2 l0 = make_float2(lav.x, lav.y), l1 = make_float2(lbv.x, lbv.y);
+      float2 p2 = make_float2(pav.z, pav.w), p3 = make_float2(pbv.z, pbv.w);
+      float2 l2 = make_float2(lav.z, lav.w), l3 = make_float2(lbv.z, lbv.w);
+      adj_pair(a, p0, p1, l0, l1, re, im);
+      adj_pair(a, p2, p3, l2, l3, re, im);
+      *pa = make_float4(p0.x, p0.y, p2.x, p2.y);
+      *pb = make_float4(p1.x, p1.y, p3.x, p3.y);
+      *la = make_float4(l0.x, l0.y, l2.x, l2.y);
+      *lb = make_float4(l1.x, l1.y, l3.x, l3.y);
+    }
+  }
+  const double tr = block_sum(re, sh);
+  const double ti = block_sum(im, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = tr;
+    part[2 * blockIdx.x + 1] = ti;
+  }
+}
+
+// --- sampling ---------------------------------------------------------------
+
+// One wave per 1024-amplitude block: each lane sums 16 amplitudes in FP64.
+__global__ __launch_bounds__(kT) void k_block_sums(const float2 *a, uint64_t dim, double *sums,
+                                                   uint64_t nblocks) {
+  const uint64_t waves = uint64_t(gridDim.x) * (kT / 64);
+  const int lane = threadIdx.x & 63;
+  for (uint64_t b = uint64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6); b < nblocks;
+       b += waves) {
+    double acc = 0.0;
+    const uint64_t start = b * kSampleBlock;
+    for (int k = 0; k < 16; k++) {
+      const uint64_t i = start + uint64_t(k) * 64 + lane;
+      if (i < dim) {
+        const float2 x = a[i];
+        acc += double(x.x) * x.x + double(x.y) * x.y;
+      }
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) sums[b] = acc;
+  }
+}
+
+// Inclusive scan of each 1024-element chunk in place; chunk totals out.
+__global__ __launch_bounds__(kT) void k_scan_chunks(double *v, uint64_t count, double *chunk) {
+  __shared__ double sh[kT];
+  const uint64_t base = uint64_t(blockIdx.x) * 1024;
+  double x[4];
+  double run = 0.0;
+  for (int k = 0; k < 4; k++) {
+    const uint64_t i = base + threadIdx.x * 4 + k;
+    run += (i < count) ? v[i] : 0.0;
+    x[k] = run;
+  }
+  sh[threadIdx.x] = run;
+  __syncthreads();
+  // Hillis-Steele over the 256 per-thread totals.
+  for (int o = 1; o < kT; o <<= 1) {
+    const double add = threadIdx.x >= o ? sh[threadIdx.x - o] : 0.0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const double before = threadIdx.x ? sh[threadIdx.x - 1] : 0.0;
+  for (int k = 0; k < 4; k++) {
+    const uint64_t i = base + threadIdx.x * 4 + k;
+    if (i < count) v[i] = x[k] + before;
+  }
+  if (threadIdx.x == kT - 1) chunk[blockIdx.x] = sh[kT - 1];
+}
+
+// Exclusive prefix of the chunk totals by one workgroup, sequential per
+// thread segment.
+__global__ __launch_bounds__(kT) void k_scan_totals(double *chunk, uint64_t nchunks) {
+  __shared__ double sh[kT];
+  const uint64_t per = (nchunks + kT - 1) / kT;
+  const uint64_t lo = threadIdx.x * per;
+  double s = 0.0;
+  for (uint64_t i = lo; i < lo + per && i < nchunks; i++) s += chunk[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double run = 0.0;
+    for (int k = 0; k < kT; k++) {
+      const double t = sh[k];
+      sh[k] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  double run = sh[threadIdx.x];
+  for (uint64_t i = lo; i < lo + per && i < nchunks; i++) {
+    const double t = chunk[i];
+    chunk[i] = run;
+    run += t;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_add_offsets(double *v, uint64_t count, const double *chunk) {
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  for (uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x; i < count; i += stride)
+    v[i] += chunk[i / 1024];
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Shot k uses the (k+1)-th splitmix64 output from `seed`: the generator's
+// state after k+1 steps is seed + (k+1) * golden, so shots are independent.
+__global__ __launch_bounds__(kT) void k_shots(const float2 *a, uint64_t dim, const double *cdf,
+                                              uint64_t nblocks, uint64_t shots, uint64_t seed,
+                                              uint64_t *out) {
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  const double total = cdf[nblocks - 1];
+  for (uint64_t k = uint64_t(blockIdx.x) * kT + threadIdx.x; k < shots; k += stride) {
+    if (!(total > 0.0)) {
+      out[k] = 0;
+      continue;
+    }
+    const uint64_t r = mix64(seed + (k + 1) * 0x9E3779B97F4A7C15ull);
+    const double u = double(r >> 11) * (1.0 / 9007199254740992.0) * total;
+    uint64_t lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+    }
+    const double rem = u - (lo > 0 ? cdf[lo - 1] : 0.0);
+    const uint64_t start = lo * kSampleBlock;
+    const uint64_t end = start + kSampleBlock < dim ? start + kSampleBlock : dim;
+    double acc = 0.0;
+    int64_t last = -1;
+    uint64_t chosen = end - 1;
+    bool found = false;
+    for (uint64_t i = start; i < end; i++) {
+      const float2 x = a[i];
+      const double pr = double(x.x) * x.x + double(x.y) * x.y;
+      if (pr > 0.0) last = int64_t(i);
+      acc += pr;
+      if (acc > rem) {
+        chosen = i;
+        found = true;
+        break;
+      }
+    }
+    if (!found) chosen = last >= 0 ? uint64_t(last) : end - 1;
+    out[k] = chosen;
+  }
+}
+
+}  // namespace
+
+// --- launchers --------------------------------------------------------------
+
+void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim) {
+  k_fill_basis<<<grid_for(dim / 2), kT, 0, s>>>(a, dim);
+}
+
+void launch_zero(hipStream_t s, float2 *a, uint64_t dim) {
+  k_zero<<<grid_for(dim / 2), kT, 0, s>>>(a, dim);
+}
+
+void launch_gather(hipStream_t s, int nl, const GatherArgs &g) {
+  const unsigned grid = grid_for(g.lay.nthreads);
+  switch (nl) {
+    case 1: k_gather<1><<<grid, kT, 0, s>>>(g); break;
+    case 2: k_gather<2><<<grid, kT, 0, s>>>(g); break;
+    case 3: k_gather<3><<<grid, kT, 0, s>>>(g); break;
+    case 4: k_gather<4><<<grid, kT, 0, s>>>(g); break;
+    case 5: k_gather<5><<<grid, kT, 0, s>>>(g); break;
+    default: break;
+  }
+}
+
+void launch_diag(hipStream_t s, int nl, const DiagArgs &d) {
+  const unsigned grid = grid_for(d.lay.nthreads);
+  switch (nl) {
+    case 1: k_diag<1><<<grid, kT, 0, s>>>(d); break;
+    case 2: k_diag<2><<<grid, kT, 0, s>>>(d); break;
+    case 3: k_diag<3><<<grid, kT, 0, s>>>(d); break;
+    case 4: k_diag<4><<<grid, kT, 0, s>>>(d); break;
+    default: break;
+  }
+}
+
+void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d) {
+  k_diag_generic<<<grid_for(d.dim), kT, 0, s>>>(d);
+}
+
+void launch_blocked(hipStream_t s, const BlockedArgs &b) {
+  const uint64_t g = b.nblocks < kMaxGrid ? b.nblocks : kMaxGrid;
+  k_blocked<<<unsigned(g), kT, sizeof(float2) << b.bits, s>>>(b);
+}
+
+void launch_abs2(hipStream_t s, const float2 *a, uint64_t dim, double *partials,
+                 double *out) {
+  k_abs2<<<kRedBlocks, kT, 0, s>>>(a, dim, partials);
+  k_finish<<<1, kT, 0, s>>>(partials, kRedBlocks, 1, out);
+}
+
+void launch_pauli(hipStream_t s, const float2 *bra, const float2 *ket,
+                  uint64_t dim, uint64_t flip, uint64_t sign, double *partials,
+                  double *out) {
+  k_pauli<<<kRedBlocks, kT, 0, s>>>(bra, ket, dim, flip, sign, partials);
+  k_finish<<<1, kT, 0, s>>>(partials, kRedBlocks, 2, out);
+}
+
+void launch_pauli_accum(hipStream_t s, const AccumArgs &a) {
+  k_pauli_accum<<<grid_for(a.dim / 2), kT, 0, s>>>(a);
+}
+
+void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,
+                        double *out2) {
+  k_adjoint_rot<<<kRedBlocks, kT, 0, s>>>(a, partials);
+  k_finish<<<1, kT, 0, s>>>(partials, kRedBlocks, 2, out2);
+}
+
+void launch_block_sums(hipStream_t s, const float2 *a, uint64_t dim, double *sums) {
+  const uint64_t nb = (dim + kSampleBlock - 1) / kSampleBlock;
+  k_block_sums<<<grid_for(nb * 64), kT, 0, s>>>(a, dim, sums, nb);
+}
+
+void launch_prefix(hipStream_t s, double *v, uint64_t count, double *chunk) {
+  const uint64_t nchunks = (count + 1023) / 1024;
+  k_scan_chunks<<<unsigned(nchunks), kT, 0, s>>>(v, count, chunk);
+  if (nchunks > 1) {
+    k_scan_totals<<<1, kT, 0, s>>>(chunk, nchunks);
+    k_add_offsets<<<grid_for(count), kT, 0, s>>>(v, count, chunk);
+  }
+}
+
+void launch_shots(hipStream_t s, const float2 *a, uint64_t dim,
+                  const double *cdf, uint64_t nblocks, uint64_t shots,
+                  uint64_t seed, uint64_t *out) {
+  k_shots<<<grid_for(shots), kT, 0, s>>>(a, dim, cdf, nblocks, shots, seed, out);
+}
+
+}  // namespace qmamd

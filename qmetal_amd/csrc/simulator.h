@@ -1,0 +1,68 @@
+// simulator.h -- the MI355X statevector executor behind the C ABI.
+//
+// One Simulator owns one HBM allocation of 2^n complex64 amplitudes on the
+// current HIP device and one stream of default priority. Plan ops are
+// enqueued asynchronously, one kernel per op; reads, reductions and samples
+// synchronise. No HIP types appear here so the C ABI compiles as plain C++.
+
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "core.h"
+
+namespace qmamd {
+
+// 2^34 complex64 amplitudes is one 128 GiB allocation, within the 288 GB of
+// HBM on one MI355X. The adjoint gradient needs a second state of the same
+// size.
+constexpr uint32_t kMaxQubits = 34;
+
+class Simulator {
+ public:
+  explicit Simulator(uint32_t n);
+  ~Simulator();
+  Simulator(const Simulator &) = delete;
+  Simulator &operator=(const Simulator &) = delete;
+
+  void reset();
+  void run(const Circuit &c, const FuseOpts &o);
+  void run_plan(const Plan &p);
+  void apply_gate(const Gate &g);
+  void synchronize();
+
+  uint32_t num_qubits() const;
+  uint64_t dim() const;
+  uint64_t dispatch_count() const;
+
+  // Host copy of the interleaved complex64 state, refreshed on every call and
+  // valid until the next call or mutation.
+  const float *amplitudes();
+
+  double abs2sum();
+  double norm();
+  double expectation(const Pauli &p);
+  double expectation(const Hamiltonian &h);
+  std::vector<uint64_t> sample(size_t shots, uint64_t seed);
+
+  std::pair<double, std::vector<double>> energy_and_gradient(
+      const Circuit &c, const Hamiltonian &h);
+
+  // Device address of the state (for callers that share device memory).
+  void *device_state();
+
+  static bool available();
+  static std::string device_name();
+  static uint64_t total_memory();
+  static uint64_t free_memory();
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace qmamd

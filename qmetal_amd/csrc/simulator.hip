@@ -1,0 +1,841 @@
+// simulator.hip -- host executor: HBM state, stream, per-op launch geometry,
+// FP64 reductions, GPU-side sampling and the adjoint sweep.
+//
+// Host-side counterpart of the reference executor (src/metal/simulator.mm);
+// each function cites the part of it whose behaviour it keeps.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "kernels.h"
+#include "simulator.h"
+
+namespace qmamd {
+namespace {
+
+void check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    fail(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Controls and diagonal masks at or above this bit are handled as fixed-one
+// bits (the amplitudes with the bit clear are never read): a run of 2^4
+// complex64 is 128 bytes, a whole cache line.
+constexpr uint32_t kFixedMin = 4;
+
+// Amplitudes a thread owns by default: 2^4, i.e. eight 16-byte loads in
+// flight per thread.
+constexpr int kLocalBits = 4;
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+int current_device() {
+  int d = 0;
+  check(hipGetDevice(&d), "hipGetDevice");
+  return d;
+}
+
+void narrow(const cd *m, float *out) {
+  for (int k = 0; k < 4; k++) {
+    out[2 * k] = float(m[k].real());
+    out[2 * k + 1] = float(m[k].imag());
+  }
+}
+
+// Angle in turns, reduced to [-0.5, 0.5].
+double turns(double rad) {
+  const double t = rad / kTwoPi;
+  return t - std::rint(t);
+}
+
+// Builds a gather layout: `reg` are the physical positions of tile-local bits
+// 1..k (bit 0 is always local), padded with the highest free positions up to
+// `want` local bits in total.
+struct LayoutBuilder {
+  uint32_t n;
+  std::vector<uint32_t> reg;
+  uint64_t fixed = 0;
+
+  int local_of(uint32_t pos) const {
+    if (pos == 0) return 0;
+    for (size_t k = 0; k < reg.size(); k++)
+      if (reg[k] == pos) return int(k) + 1;
+    return -1;
+  }
+  int add(uint32_t pos) {
+    int l = local_of(pos);
+    if (l >= 0) return l;
+    reg.push_back(pos);
+    return int(reg.size());
+  }
+  Layout finish(int want, int *nl_out) {
+    const uint32_t nfixed = uint32_t(__builtin_popcountll(fixed));
+    int nl = int(reg.size()) + 1;
+    for (uint32_t p = n; p-- > 1 && nl < want;) {
+      if (local_of(p) >= 0 || ((fixed >> p) & 1)) continue;
+      if (uint32_t(nl) + nfixed >= n) break;
+      reg.push_back(p);
+      nl++;
+    }
+    Layout L{};
+    std::vector<uint32_t> ins = reg;
+    ins.push_back(0);
+    for (uint32_t p = 0; p < 64; p++)
+      if ((fixed >> p) & 1) ins.push_back(p);
+    std::sort(ins.begin(), ins.end());
+    if (ins.size() > sizeof(L.ins) || reg.size() > sizeof(L.rpos))
+      fail("internal: layout too wide");
+    L.nins = uint32_t(ins.size());
+    for (size_t k = 0; k < ins.size(); k++) L.ins[k] = uint8_t(ins[k]);
+    for (size_t k = 0; k < reg.size(); k++) L.rpos[k] = uint8_t(reg[k]);
+    L.fixed = fixed;
+    L.nthreads = (uint64_t(1) << n) >> ins.size();
+    *nl_out = nl;
+    return L;
+  }
+};
+
+Slot make_slot(int lbit, uint32_t cmask, const float *m8) {
+  Slot s{};
+  s.lbit = uint8_t(lbit);
+  s.cmask = uint8_t(cmask);
+  std::memcpy(s.m, m8, sizeof(s.m));
+  return s;
+}
+
+const float kX[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+
+}  // namespace
+
+struct Simulator::Impl {
+  uint32_t n = 0;
+  uint64_t dim = 0;
+  int device = 0;
+  uint64_t dispatches = 0;
+  hipStream_t stream = nullptr;
+  float2 *state = nullptr;
+  float2 *costate = nullptr;
+
+  // FP64 reduction partials and result slots.
+  double *partials = nullptr;
+  double *results = nullptr;
+  size_t result_cap = 0;
+  double *host_results = nullptr;  // pinned
+  size_t host_result_cap = 0;
+
+  // Per-plan parameter block (diagonal terms, blocked gate lists), staged in
+  // pinned memory and copied once per run ahead of the plan's kernels.
+  uint8_t *arena = nullptr;
+  size_t arena_cap = 0;
+  uint8_t *staging = nullptr;
+  size_t staging_cap = 0;
+  hipEvent_t staging_done = nullptr;
+  bool staging_pending = false;
+
+  // Sampling scratch.
+  double *cdf = nullptr;
+  size_t cdf_cap = 0;
+  double *chunks = nullptr;
+  size_t chunks_cap = 0;
+  uint64_t *shots_dev = nullptr;
+  size_t shots_cap = 0;
+
+  std::vector<float> mirror;
+
+  void use() { check(hipSetDevice(device), "hipSetDevice"); }
+
+  void launched() { check(hipGetLastError(), "kernel launch"); }
+
+  template <typename T>
+  void grow(T *&p, size_t &cap, size_t need, const char *what) {
+    if (need <= cap) return;
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (p) check(hipFree(p), "hipFree");
+    p = nullptr;
+    cap = 0;
+    size_t c = std::max<size_t>(need, 64);
+    check(hipMalloc(reinterpret_cast<void **>(&p), c * sizeof(T)), what);
+    cap = c;
+  }
+
+  double *result_slots(size_t count) {
+    grow(results, result_cap, count, "hipMalloc(results)");
+    return results;
+  }
+
+  // Copies `count` result doubles to the host and waits.
+  const double *fetch_results(size_t count) {
+    if (count > host_result_cap) {
+      if (host_results) check(hipHostFree(host_results), "hipHostFree");
+      host_results = nullptr;
+      host_result_cap = 0;
+      check(hipHostMalloc(reinterpret_cast<void **>(&host_results),
+                          std::max<size_t>(count, 64) * sizeof(double)),
+            "hipHostMalloc(results)");
+      host_result_cap = std::max<size_t>(count, 64);
+    }
+    check(hipMemcpyAsync(host_results, results, count * sizeof(double),
+                         hipMemcpyDeviceToHost, stream),
+          "hipMemcpyAsync(results)");
+    check(hipStreamSynchronize(stream), "GPU error");
+    return host_results;
+  }
+
+  // Reserves `bytes` of host staging for the next plan upload.
+  uint8_t *stage(size_t bytes) {
+    if (staging_pending) {
+      check(hipEventSynchronize(staging_done), "hipEventSynchronize");
+      staging_pending = false;
+    }
+    if (bytes > staging_cap) {
+      if (staging) check(hipHostFree(staging), "hipHostFree");
+      staging = nullptr;
+      staging_cap = 0;
+      size_t c = std::max<size_t>(bytes, 1 << 16);
+      check(hipHostMalloc(reinterpret_cast<void **>(&staging), c), "hipHostMalloc(staging)");
+      staging_cap = c;
+    }
+    grow(arena, arena_cap, bytes, "hipMalloc(arena)");
+    return staging;
+  }
+
+  void upload(size_t bytes) {
+    if (!bytes) return;
+    check(hipMemcpyAsync(arena, staging, bytes, hipMemcpyHostToDevice, stream),
+          "hipMemcpyAsync(params)");
+    check(hipEventRecord(staging_done, stream), "hipEventRecord");
+    staging_pending = true;
+  }
+
+  void gather(LayoutBuilder &lb, const std::vector<Slot> &slots, float2 *second,
+              int want = kLocalBits) {
+    GatherArgs g{};
+    g.a = state;
+    g.b = second;
+    int nl = 0;
+    g.lay = lb.finish(std::max<int>(want, int(lb.reg.size()) + 1), &nl);
+    if (slots.size() > 4) fail("internal: too many slots");
+    g.nslots = uint32_t(slots.size());
+    for (size_t k = 0; k < slots.size(); k++) g.slots[k] = slots[k];
+    if (nl < 1 || nl > 5) fail("internal: gather width");
+    launch_gather(stream, nl, g);
+    launched();
+  }
+
+  // One 2x2 on `target`, with controls; controls at or above kFixedMin become
+  // fixed-one bits so only their half of the state is touched.
+  void controlled(const std::vector<uint32_t> &ctrls, uint32_t target,
+                  const float *m8, float2 *second) {
+    LayoutBuilder lb{n};
+    const int lt = lb.add(target);
+    uint32_t cm = 0;
+    for (uint32_t c : ctrls) {
+      if (c >= kFixedMin) lb.fixed |= 1ull << c;
+      else cm |= 1u << lb.add(c);
+    }
+    gather(lb, {make_slot(lt, cm, m8)}, second);
+  }
+
+  void swap(uint32_t a, uint32_t b, float2 *second) {
+    LayoutBuilder lb{n};
+    const int la = lb.add(a), lbb = lb.add(b);
+    gather(lb,
+           {make_slot(lbb, 1u << la, kX), make_slot(la, 1u << lbb, kX),
+            make_slot(lbb, 1u << la, kX)},
+           second);
+  }
+
+  void group(const std::vector<uint32_t> &wires, const std::vector<cd> &mats,
+             float2 *second) {
+    LayoutBuilder lb{n};
+    std::vector<Slot> slots;
+    for (size_t k = 0; k < wires.size(); k++) {
+      float m8[8];
+      narrow(&mats[4 * k], m8);
+      slots.push_back(make_slot(lb.add(wires[k]), 0, m8));
+    }
+    gather(lb, slots, second);
+  }
+
+  size_t diag_bytes(const PlanOp &op) const {
+    return op.masks.size() * std::max(sizeof(DiagTerm), sizeof(uint64_t) + sizeof(double));
+  }
+
+  // Prepared launch of a parameterised op whose parameters sit in the arena.
+  struct Pending {
+    int type = 0;  // 1 diagonal, 2 generic diagonal, 3 blocked
+    int nl = 0;
+    DiagArgs d{};
+    DiagGenericArgs g{};
+    BlockedArgs b{};
+  };
+
+  // Phase layer (reference kernels.metal:203-218 semantics, FP64 phase).
+  // Terms are folded on the host into const + sum beta_k x_k +
+  // sum gamma_kj x_k x_j over the tile-local bits; terms on other bits go to
+  // the arena for the per-thread fold.
+  void prep_diagonal(const PlanOp &op, size_t off, Pending &pl) {
+    const size_t T = op.masks.size();
+    bool low_weight = true;
+    uint64_t common = ~0ull;
+    for (uint64_t m : op.masks) {
+      const int w = __builtin_popcountll(m);
+      if (w < 1 || w > 2) low_weight = false;
+      common &= m;
+    }
+    if (!low_weight) {
+      uint64_t *mk = reinterpret_cast<uint64_t *>(staging + off);
+      double *an = reinterpret_cast<double *>(staging + off + T * sizeof(uint64_t));
+      for (size_t t = 0; t < T; t++) {
+        mk[t] = op.masks[t];
+        an[t] = turns(op.angles[t]);
+      }
+      pl.type = 2;
+      pl.g.a = state;
+      pl.g.dim = dim;
+      pl.g.masks = reinterpret_cast<const uint64_t *>(arena + off);
+      pl.g.angles = reinterpret_cast<const double *>(arena + off + T * sizeof(uint64_t));
+      pl.g.nterms = uint32_t(T);
+      pl.g.global = turns(op.global);
+      return;
+    }
+    LayoutBuilder lb{n};
+    // With no global phase, amplitudes missing any common mask bit keep
+    // phase 0 and are never touched.
+    if (turns(op.global) == 0.0) lb.fixed = common & ~((1ull << kFixedMin) - 1ull);
+    DiagArgs &d = pl.d;
+    d = DiagArgs{};
+    d.a = state;
+    d.lay = lb.finish(kLocalBits, &pl.nl);
+    if (pl.nl > 4) fail("internal: diagonal width");
+    double constant = op.global;
+    // (kind, mask, k) -> angle, merged.
+    std::map<std::tuple<uint32_t, uint64_t, uint32_t>, double> terms;
+    for (size_t t = 0; t < T; t++) {
+      const uint64_t m = op.masks[t] & ~lb.fixed;
+      const double a = op.angles[t];
+      if (m == 0) {
+        constant += a;
+        continue;
+      }
+      const uint32_t p0 = uint32_t(__builtin_ctzll(m));
+      const int l0 = lb.local_of(p0);
+      if (__builtin_popcountll(m) == 1) {
+        if (l0 >= 0) d.beta0[l0] += a;
+        else terms[{0u, m, 0u}] += a;
+        continue;
+      }
+      const uint32_t p1 = 63u - uint32_t(__builtin_clzll(m));
+      const int l1 = lb.local_of(p1);
+      if (l0 >= 0 && l1 >= 0) d.gamma[std::min(l0, l1)][std::max(l0, l1)] += a;
+      else if (l0 >= 0) terms[{2u, 1ull << p1, uint32_t(l0)}] += a;
+      else if (l1 >= 0) terms[{2u, 1ull << p0, uint32_t(l1)}] += a;
+      else terms[{1u, m, 0u}] += a;
+    }
+    d.constant = turns(constant);
+    for (auto &b : d.beta0) b = turns(b);
+    for (auto &row : d.gamma)
+      for (auto &g : row) g = turns(g);
+    DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off);
+    size_t k = 0;
+    for (const auto &kv : terms) {
+      dt[k].kind = std::get<0>(kv.first);
+      dt[k].mask = std::get<1>(kv.first);
+      dt[k].k = std::get<2>(kv.first);
+      dt[k].ang = turns(kv.second);
+      k++;
+    }
+    d.terms = reinterpret_cast<const DiagTerm *>(arena + off);
+    d.nterms = uint32_t(k);
+    pl.type = 1;
+  }
+
+  // Reference simulator.mm:464-499 (block_bits 2..12, wires below it).
+  void prep_blocked(const PlanOp &op, size_t off, Pending &pl) {
+    const uint32_t b = op.block_bits;
+    if (b < 2 || b > 12) fail("block_bits must be 2..12");
+    if (b > n) fail("block larger than the state");
+    Slot *sl = reinterpret_cast<Slot *>(staging + off);
+    for (size_t k = 0; k < op.wires.size(); k++) {
+      if (op.wires[k] >= b) fail("blocked gate outside the block");
+      float m8[8];
+      narrow(&op.mats[4 * k], m8);
+      sl[k] = make_slot(int(op.wires[k]), 0, m8);
+    }
+    pl.type = 3;
+    pl.b.a = state;
+    pl.b.nblocks = dim >> b;
+    pl.b.bits = b;
+    pl.b.count = uint32_t(op.wires.size());
+    pl.b.gates = reinterpret_cast<const Slot *>(arena + off);
+  }
+
+  void launch_pending(const Pending &pl) {
+    if (pl.type == 1) launch_diag(stream, pl.nl, pl.d);
+    else if (pl.type == 2) launch_diag_generic(stream, pl.g);
+    else if (pl.type == 3) launch_blocked(stream, pl.b);
+    launched();
+  }
+
+  size_t param_bytes(const PlanOp &op) const {
+    size_t b = 0;
+    if (op.kind == OpKind::Diagonal) b = diag_bytes(op);
+    if (op.kind == OpKind::Blocked) b = op.wires.size() * sizeof(Slot);
+    return (b + 255) & ~size_t(255);
+  }
+
+  void op(const PlanOp &p, const Pending *pl) {
+    switch (p.kind) {
+      case OpKind::Dense1q: {
+        float m8[8];
+        narrow(p.m, m8);
+        controlled({}, p.q[0], m8, nullptr);
+        break;
+      }
+      case OpKind::Group:
+        if (p.wires.size() < 2 || p.wires.size() > 4)
+          fail("group size must be 2..4");
+        group(p.wires, p.mats, nullptr);
+        break;
+      case OpKind::Blocked:
+        launch_pending(*pl);
+        break;
+      case OpKind::Controlled: {
+        float m8[8];
+        narrow(p.m, m8);
+        controlled({p.q[0]}, p.q[1], m8, nullptr);
+        break;
+      }
+      case OpKind::Diagonal:
+        if (p.masks.empty()) return;  // no dispatch, as the reference
+        launch_pending(*pl);
+        break;
+      case OpKind::Swap:
+        swap(p.q[0], p.q[1], nullptr);
+        break;
+      case OpKind::CCX:
+        controlled({p.q[0], p.q[1]}, p.q[2], kX, nullptr);
+        break;
+    }
+    dispatches++;
+  }
+
+  // Gate path: one gate on the state (and, for the adjoint, the same gate on
+  // the co-state in the same pass). Reference simulator.mm:348-434.
+  void gate(const Gate &g, float2 *second) {
+    for (int k = 0; k < g.nq; k++)
+      if (g.q[k] >= n) fail("qubit index out of range");
+    if (g.nq == 1) {
+      cd m[4];
+      gate_matrix(g.kind, g.p, m);
+      float m8[8];
+      narrow(m, m8);
+      controlled({}, g.q[0], m8, second);
+      return;
+    }
+    if (g.nq == 2) {
+      if (g.q[0] == g.q[1]) fail("2-qubit gate on identical wires");
+      switch (g.kind) {
+        case K_CX:
+        case K_CY: {
+          cd m[4];
+          gate_matrix(g.kind == K_CX ? K_X : K_Y, nullptr, m);
+          float m8[8];
+          narrow(m, m8);
+          controlled({g.q[0]}, g.q[1], m8, second);
+          return;
+        }
+        case K_CZ:
+        case K_CP: {
+          const cd ph = g.kind == K_CZ ? cd(-1, 0) : cd(std::cos(g.p[0]), std::sin(g.p[0]));
+          const cd m[4] = {1.0, 0.0, 0.0, ph};
+          float m8[8];
+          narrow(m, m8);
+          controlled({g.q[0]}, g.q[1], m8, second);
+          return;
+        }
+        case K_SWAP:
+          swap(g.q[0], g.q[1], second);
+          return;
+        default:
+          break;
+      }
+    }
+    if (g.nq == 3 && g.kind == K_CCX) {
+      if (g.q[0] == g.q[1] || g.q[0] == g.q[2] || g.q[1] == g.q[2])
+        fail("CCX on repeated wires");
+      controlled({g.q[0], g.q[1]}, g.q[2], kX, second);
+      return;
+    }
+    fail(std::string("unsupported gate: ") + kind_name(g.kind));
+  }
+
+  double pauli_sum(const float2 *bra, const float2 *ket, const Pauli &p,
+                   double *slot) {
+    launch_pauli(stream, bra, ket, dim, p.x | p.y, p.y | p.z, partials, slot);
+    launched();
+    return 0.0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+
+bool Simulator::available() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return count > 0;
+}
+
+std::string Simulator::device_name() {
+  if (!available()) return "none";
+  hipDeviceProp_t prop;
+  check(hipGetDeviceProperties(&prop, current_device()), "hipGetDeviceProperties");
+  return std::string(prop.name) + " (" + prop.gcnArchName + ")";
+}
+
+uint64_t Simulator::total_memory() {
+  if (!available()) return 0;
+  size_t fr = 0, tot = 0;
+  check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+  return tot;
+}
+
+uint64_t Simulator::free_memory() {
+  if (!available()) return 0;
+  size_t fr = 0, tot = 0;
+  check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+  return fr;
+}
+
+// Reference simulator.mm:272-326: validate n, preflight the allocation,
+// allocate, write |0...0>.
+Simulator::Simulator(uint32_t n) : impl_(new Impl) {
+  if (n == 0 || n > kMaxQubits)
+    fail("Simulator: num_qubits must be 1.." + std::to_string(kMaxQubits));
+  if (!available()) fail("Simulator: no HIP device");
+  Impl *s = impl_.get();
+  s->n = n;
+  s->dim = uint64_t(1) << n;
+  s->device = current_device();
+  const uint64_t bytes = s->dim * 8ull;
+  if (const char *cap = getenv("QMETAL_ALLOC_CAP_BYTES")) {
+    const uint64_t c = strtoull(cap, nullptr, 10);
+    if (c && bytes > c)
+      fail(std::to_string(n) + " qubits needs " + std::to_string(bytes) +
+           " bytes, past QMETAL_ALLOC_CAP_BYTES=" + std::to_string(c));
+  }
+  const uint64_t freeb = free_memory();
+  if (bytes > freeb)
+    fail(std::to_string(n) + " qubits needs " + std::to_string(bytes >> 30) +
+         " GiB, only " + std::to_string(freeb >> 30) + " GiB of device memory free");
+  check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+  check(hipEventCreateWithFlags(&s->staging_done, hipEventDisableTiming), "hipEventCreate");
+  if (hipMalloc(reinterpret_cast<void **>(&s->state), bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    s->state = nullptr;
+    fail("Simulator: state allocation failed (" + std::to_string(bytes >> 30) + " GiB)");
+  }
+  check(hipMalloc(reinterpret_cast<void **>(&s->partials), 2 * kRedBlocks * sizeof(double)),
+        "hipMalloc(partials)");
+  reset();
+}
+
+Simulator::~Simulator() {
+  Impl *s = impl_.get();
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  hipFree(s->state);
+  hipFree(s->costate);
+  hipFree(s->partials);
+  hipFree(s->results);
+  hipFree(s->arena);
+  hipFree(s->cdf);
+  hipFree(s->chunks);
+  hipFree(s->shots_dev);
+  if (s->staging) hipHostFree(s->staging);
+  if (s->host_results) hipHostFree(s->host_results);
+  if (s->staging_done) hipEventDestroy(s->staging_done);
+  if (s->stream) hipStreamDestroy(s->stream);
+}
+
+uint32_t Simulator::num_qubits() const { return impl_->n; }
+uint64_t Simulator::dim() const { return impl_->dim; }
+uint64_t Simulator::dispatch_count() const { return impl_->dispatches; }
+void *Simulator::device_state() { return impl_->state; }
+
+void Simulator::reset() {
+  Impl *s = impl_.get();
+  s->use();
+  launch_fill_basis(s->stream, s->state, s->dim);
+  s->launched();
+  s->dispatches++;
+}
+
+void Simulator::run(const Circuit &c, const FuseOpts &o) {
+  if (c.n != impl_->n) fail("circuit/simulator size mismatch");
+  check_circuit(c);
+  run_plan(build_plan(c, o));
+}
+
+void Simulator::run_plan(const Plan &p) {
+  Impl *s = impl_.get();
+  if (p.n != s->n) fail("plan/simulator size mismatch");
+  s->use();
+  // Every parameterised op writes its region of one staging block, which is
+  // uploaded once, ahead of the plan's kernels on the same stream.
+  std::vector<size_t> offs(p.ops.size());
+  size_t total = 0;
+  for (size_t k = 0; k < p.ops.size(); k++) {
+    offs[k] = total;
+    total += s->param_bytes(p.ops[k]);
+  }
+  std::vector<Impl::Pending> pend(p.ops.size());
+  if (total) {
+    s->stage(total);
+    for (size_t k = 0; k < p.ops.size(); k++) {
+      const PlanOp &op = p.ops[k];
+      if (op.kind == OpKind::Diagonal && !op.masks.empty())
+        s->prep_diagonal(op, offs[k], pend[k]);
+      else if (op.kind == OpKind::Blocked)
+        s->prep_blocked(op, offs[k], pend[k]);
+    }
+    s->upload(total);
+  }
+  for (size_t k = 0; k < p.ops.size(); k++) s->op(p.ops[k], &pend[k]);
+}
+
+void Simulator::apply_gate(const Gate &g) {
+  Impl *s = impl_.get();
+  s->use();
+  s->gate(g, nullptr);
+  s->dispatches++;
+}
+
+void Simulator::synchronize() {
+  Impl *s = impl_.get();
+  s->use();
+  check(hipStreamSynchronize(s->stream), "GPU error");
+}
+
+const float *Simulator::amplitudes() {
+  Impl *s = impl_.get();
+  s->use();
+  s->mirror.resize(2 * s->dim);
+  check(hipMemcpyAsync(s->mirror.data(), s->state, s->dim * 8, hipMemcpyDeviceToHost, s->stream),
+        "hipMemcpyAsync(amplitudes)");
+  check(hipStreamSynchronize(s->stream), "GPU error");
+  return s->mirror.data();
+}
+
+// Reference simulator.mm:615-627, with FP64 accumulation on the GPU.
+double Simulator::abs2sum() {
+  Impl *s = impl_.get();
+  s->use();
+  double *slot = s->result_slots(1);
+  launch_abs2(s->stream, s->state, s->dim, s->partials, slot);
+  s->launched();
+  s->dispatches++;
+  return s->fetch_results(1)[0];
+}
+
+double Simulator::norm() { return std::sqrt(abs2sum()); }
+
+namespace {
+// Real part of i^ny * (re + i im) (reference simulator.mm:653-660).
+double real_ipow(uint32_t ny, double re, double im) {
+  switch (ny & 3u) {
+    case 0: return re;
+    case 1: return -im;
+    case 2: return -re;
+    default: return im;
+  }
+}
+}  // namespace
+
+double Simulator::expectation(const Pauli &p) {
+  Hamiltonian h;
+  h.push_back(Term{1.0, p});
+  return expectation(h);
+}
+
+// One FP64 Pauli reduction per term (reference simulator.mm:629-667), all
+// enqueued before a single read-back.
+double Simulator::expectation(const Hamiltonian &h) {
+  Impl *s = impl_.get();
+  for (const Term &t : h)
+    if (!t.p.valid(s->n)) fail("expectation: invalid Pauli string for this width");
+  if (h.empty()) return 0.0;
+  s->use();
+  double *slots = s->result_slots(2 * h.size());
+  for (size_t k = 0; k < h.size(); k++) {
+    s->pauli_sum(s->state, s->state, h[k].p, slots + 2 * k);
+    s->dispatches++;
+  }
+  const double *r = s->fetch_results(2 * h.size());
+  double total = 0.0;
+  for (size_t k = 0; k < h.size(); k++)
+    total += h[k].c * real_ipow(uint32_t(__builtin_popcountll(h[k].p.y)), r[2 * k], r[2 * k + 1]);
+  return total;
+}
+
+// Reference simulator.mm:677-763, resolved on the GPU: FP64 block sums,
+// FP64 prefix, then one thread per shot (splitmix64, binary search over
+// blocks, in-block walk clamped to the last non-zero amplitude).
+std::vector<uint64_t> Simulator::sample(size_t shots, uint64_t seed) {
+  Impl *s = impl_.get();
+  std::vector<uint64_t> out(shots);
+  if (shots == 0) return out;
+  s->use();
+  const uint64_t nb = (s->dim + kSampleBlock - 1) / kSampleBlock;
+  s->grow(s->cdf, s->cdf_cap, nb, "hipMalloc(cdf)");
+  s->grow(s->chunks, s->chunks_cap, (nb + 1023) / 1024, "hipMalloc(chunks)");
+  s->grow(s->shots_dev, s->shots_cap, shots, "hipMalloc(shots)");
+  launch_block_sums(s->stream, s->state, s->dim, s->cdf);
+  s->launched();
+  launch_prefix(s->stream, s->cdf, nb, s->chunks);
+  s->launched();
+  launch_shots(s->stream, s->state, s->dim, s->cdf, nb, shots, seed, s->shots_dev);
+  s->launched();
+  s->dispatches++;
+  check(hipMemcpyAsync(out.data(), s->shots_dev, shots * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                       s->stream),
+        "hipMemcpyAsync(shots)");
+  check(hipStreamSynchronize(s->stream), "GPU error");
+  return out;
+}
+
+namespace {
+// U^dagger (reference simulator.mm:786-799); U3(t,p,l)^dagger = U3(-t,-l,-p).
+Gate inverse(const Gate &g) {
+  Gate v = g;
+  switch (g.kind) {
+    case K_RX: case K_RY: case K_RZ: case K_P: case K_CP: v.p[0] = -g.p[0]; break;
+    case K_S: v.kind = K_SDG; break;
+    case K_SDG: v.kind = K_S; break;
+    case K_T: v.kind = K_TDG; break;
+    case K_TDG: v.kind = K_T; break;
+    case K_U3: v.p[0] = -g.p[0]; v.p[1] = -g.p[2]; v.p[2] = -g.p[1]; break;
+    default: break;
+  }
+  return v;
+}
+
+bool is_rotation(uint8_t k) { return k == K_RX || k == K_RY || k == K_RZ; }
+bool has_parameter(const Gate &g) {
+  return g.np > 0 || g.kind == K_RX || g.kind == K_RY || g.kind == K_RZ ||
+         g.kind == K_P || g.kind == K_CP || g.kind == K_U3;
+}
+}  // namespace
+
+// Adjoint gradient (reference simulator.mm:814-963). The forward pass runs
+// the fused plan (only the final state is needed); the co-state is built
+// with one pass per distinct Pauli flip; E = Re<psi|lambda> is an FP64
+// reduction; the backward sweep fuses, per rotation, the FP64 overlap
+// <lambda|P|psi> with the undo of that rotation on both states, and undoes
+// every other gate on both states in one gather pass. Overlaps stay on the
+// device until one read-back at the end.
+std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
+    const Circuit &c, const Hamiltonian &h) {
+  Impl *s = impl_.get();
+  if (c.n != s->n) fail("circuit/simulator size mismatch");
+  for (const Gate &g : c.gates)
+    if (has_parameter(g) && !is_rotation(g.kind))
+      fail(std::string("gradient: ") + kind_name(g.kind) +
+           " carries a parameter but has no native generator; only rx, ry and rz are "
+           "differentiable");
+  check_circuit(c);
+  for (const Term &t : h)
+    if (!t.p.valid(s->n)) fail("gradient: invalid Pauli term");
+  s->use();
+  if (!s->costate) {
+    const uint64_t bytes = s->dim * 8ull;
+    if (bytes > free_memory()) fail("gradient: co-state exceeds free device memory");
+    if (hipMalloc(reinterpret_cast<void **>(&s->costate), bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      s->costate = nullptr;
+      fail("gradient: co-state allocation failed");
+    }
+  }
+  reset();
+  run_plan(build_plan(c, FuseOpts{}));
+
+  // lambda = H psi, terms grouped by flip mask.
+  launch_zero(s->stream, s->costate, s->dim);
+  s->launched();
+  std::map<uint64_t, std::vector<size_t>> by_flip;
+  for (size_t k = 0; k < h.size(); k++) by_flip[h[k].p.x | h[k].p.y].push_back(k);
+  for (const auto &kv : by_flip) {
+    const std::vector<size_t> &ts = kv.second;
+    for (size_t b = 0; b < ts.size(); b += kAccumTerms) {
+      AccumArgs a{};
+      a.lam = s->costate;
+      a.psi = s->state;
+      a.dim = s->dim;
+      a.flip = kv.first;
+      for (size_t j = b; j < ts.size() && j < b + kAccumTerms; j++) {
+        const Term &t = h[ts[j]];
+        const uint32_t ny = uint32_t(__builtin_popcountll(t.p.y));
+        double re = t.c, im = 0.0;  // c * i^ny
+        switch (ny & 3u) {
+          case 1: re = 0.0; im = t.c; break;
+          case 2: re = -t.c; break;
+          case 3: re = 0.0; im = -t.c; break;
+          default: break;
+        }
+        a.sign[a.nterms] = t.p.y | t.p.z;
+        a.cre[a.nterms] = re;
+        a.cim[a.nterms] = im;
+        a.nterms++;
+      }
+      launch_pauli_accum(s->stream, a);
+      s->launched();
+      s->dispatches++;
+    }
+  }
+  const size_t G = c.gates.size();
+  double *slots = s->result_slots(2 * (G + 1));
+  // E = Re<psi|lambda>
+  s->pauli_sum(s->state, s->costate, Pauli{}, slots + 2 * G);
+  s->dispatches++;
+
+  for (size_t k = G; k-- > 0;) {
+    const Gate &g = c.gates[k];
+    const Gate inv = inverse(g);
+    if (is_rotation(g.kind)) {
+      cd m[4];
+      gate_matrix(inv.kind, inv.p, m);
+      AdjArgs a{};
+      a.psi = s->state;
+      a.lam = s->costate;
+      a.dim = s->dim;
+      a.q = g.q[0];
+      a.axis = g.kind == K_RX ? 0 : g.kind == K_RY ? 1 : 2;
+      narrow(m, a.m);
+      launch_adjoint_rot(s->stream, a, s->partials, slots + 2 * k);
+      s->launched();
+    } else {
+      s->gate(inv, s->costate);
+    }
+    s->dispatches++;
+  }
+  const double *r = s->fetch_results(2 * (G + 1));
+  std::vector<double> grad(G, 0.0);
+  for (size_t k = 0; k < G; k++) {
+    const Gate &g = c.gates[k];
+    if (!is_rotation(g.kind)) continue;
+    // grad = Im(i^ny S): ny = 1 for RY, 0 for RX/RZ.
+    grad[k] = g.kind == K_RY ? r[2 * k] : r[2 * k + 1];
+  }
+  return {r[2 * G], grad};
+}
+
+}  // namespace qmamd
