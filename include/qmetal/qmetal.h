@@ -149,6 +149,48 @@ int qm_plan_get_arrays(qm_plan p, size_t k, uint32_t *group_qubits,
                        double *group_matrices, uint64_t *masks,
                        double *angles);
 
+/* Adjoint energy and gradient with device-resident arguments, for framework
+ * tensors (ROCm PyTorch): `params` holds one double per RX/RY/RZ gate in
+ * circuit order and replaces those gates' angles; `out_energy` (1 double) and
+ * `out_grad` (num_params doubles, one per rotation) are device pointers.
+ * `stream` is the caller's hipStream_t (or NULL for the null stream): work
+ * already queued there completes before the parameters are read, and work
+ * queued there afterwards sees the results. */
+int qm_energy_and_gradient_device(qm_simulator s, qm_circuit c,
+                                  const qm_hamiltonian *h, const double *params,
+                                  size_t num_params, double *out_energy,
+                                  double *out_grad, void *stream);
+
+/* Batched adjoint: many independent circuits (widths 1..20, each with its own
+ * Hamiltonian) solved for energy and per-gate gradient in one persistent,
+ * graph-captured launch. prepare compiles and uploads the programs (circuits
+ * and Hamiltonians are borrowed for the call); run replays the launch and
+ * copies out out_energy[count] and out_grad (qm_circuit_size(c_i) entries per
+ * instance, back to back, 0 for gates without a parameter); out_ms gets the
+ * device time of the launch. autotune times the launch configurations on the
+ * prepared batch and keeps the fastest (workgroup size, persistent
+ * workgroups per CU, widest instance kept in LDS). */
+typedef struct qm_batch_s *qm_batch;
+qm_batch qm_batch_create(void);
+void qm_batch_destroy(qm_batch b);
+int qm_batch_prepare(qm_batch b, size_t count, const qm_circuit *circuits,
+                     const qm_hamiltonian *hams);
+int qm_batch_autotune(qm_batch b, uint32_t *threads, uint32_t *per_cu,
+                      uint32_t *lds_bits);
+int qm_batch_set_config(qm_batch b, uint32_t threads, uint32_t per_cu,
+                        uint32_t lds_bits);
+int qm_batch_run(qm_batch b, double *out_energy, double *out_grad,
+                 double *out_ms);
+
+/* Device timing of plan-op kernels, measured with events on the simulator's
+ * own stream around each launch. qm_read_timing synchronises, reports every
+ * op timed since the previous read (up to `cap` records: plan-op kind, device
+ * milliseconds, algorithmic bytes = one read and one write of each amplitude
+ * the op can change) and sets *count to the number timed. */
+int qm_set_timing(qm_simulator s, int enable);
+int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
+                   size_t cap, size_t *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,7 +28,7 @@ import os
 from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_float,
                     c_int, c_size_t, c_uint32, c_uint64, c_void_p)
 
-__all__ = ["Circuit", "Simulator", "Hamiltonian", "QMetalError", "FusionOptions",
+__all__ = ["Circuit", "Simulator", "Hamiltonian", "QMetalError", "FusionOptions", "Batch",
            "version", "is_available", "device_name", "max_qubits", "library_path"]
 
 
@@ -125,6 +125,18 @@ _expect = _decl("qm_expectation", c_int, c_void_p, POINTER(_Hamiltonian), _f64p)
 _sample = _decl("qm_sample", c_int, c_void_p, c_size_t, c_uint64, _u64p)
 _energy_grad = _decl("qm_energy_and_gradient", c_int, c_void_p, c_void_p,
                      POINTER(_Hamiltonian), _f64p, _f64p)
+
+_set_timing = _decl("qm_set_timing", c_int, c_void_p, c_int)
+_read_timing = _decl("qm_read_timing", c_int, c_void_p, _u32p, _f64p, _f64p, c_size_t,
+                     POINTER(c_size_t))
+
+_batch_create = _decl("qm_batch_create", c_void_p)
+_batch_destroy = _decl("qm_batch_destroy", None, c_void_p)
+_batch_prepare = _decl("qm_batch_prepare", c_int, c_void_p, c_size_t, POINTER(c_void_p),
+                       POINTER(_Hamiltonian))
+_batch_autotune = _decl("qm_batch_autotune", c_int, c_void_p, _u32p, _u32p, _u32p)
+_batch_set_config = _decl("qm_batch_set_config", c_int, c_void_p, c_uint32, c_uint32, c_uint32)
+_batch_run = _decl("qm_batch_run", c_int, c_void_p, _f64p, _f64p, _f64p)
 
 _plan_build = _decl("qm_plan_build", c_void_p, c_void_p, POINTER(FusionOptions))
 _plan_destroy = _decl("qm_plan_destroy", None, c_void_p)
@@ -382,6 +394,21 @@ class Simulator:
             out[key] = out.get(key, 0) + 1
         return dict(sorted(out.items(), key=lambda kv: -kv[1]))
 
+    def set_timing(self, enable: bool = True):
+        """Record device time of every plan-op kernel (events on the
+        simulator's stream)."""
+        _check(_set_timing(self._h, 1 if enable else 0))
+        return self
+
+    def read_timing(self, cap: int = 1 << 16):
+        """[(op kind name, device ms, algorithmic bytes)] since the last read."""
+        kinds = (c_uint32 * cap)()
+        ms = (c_double * cap)()
+        by = (c_double * cap)()
+        count = c_size_t()
+        _check(_read_timing(self._h, kinds, ms, by, cap, byref(count)))
+        return [(PLAN_KINDS[kinds[k]], ms[k], by[k]) for k in range(min(count.value, cap))]
+
     def energy_and_gradient(self, circuit: Circuit, hamiltonian: Hamiltonian):
         """Adjoint gradient: one entry per gate, zero where unparameterised."""
         h = hamiltonian._c()
@@ -389,3 +416,53 @@ class Simulator:
         grad = (c_double * max(len(circuit), 1))()
         _check(_energy_grad(self._h, circuit._h, byref(h), byref(energy), grad))
         return energy.value, list(grad)[:len(circuit)]
+
+
+class Batch:
+    """Many small circuits' energies and adjoint gradients in one persistent,
+    graph-captured launch (qm_batch_*)."""
+
+    def __init__(self):
+        self._h = _batch_create()
+        if not self._h:
+            raise QMetalError(_err())
+        self._sizes: list[int] = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _batch_destroy(self._h)
+            self._h = None
+
+    def close(self):
+        self.__del__()
+
+    def prepare(self, circuits, hamiltonians):
+        n = len(circuits)
+        cs = (c_void_p * n)(*[c._h for c in circuits])
+        hs_py = [h._c() for h in hamiltonians]
+        hs = (_Hamiltonian * n)(*hs_py)
+        _check(_batch_prepare(self._h, n, cs, hs))
+        self._sizes = [len(c) for c in circuits]
+        return self
+
+    def autotune(self):
+        t, p, b = c_uint32(), c_uint32(), c_uint32()
+        _check(_batch_autotune(self._h, byref(t), byref(p), byref(b)))
+        return {"threads": t.value, "per_cu": p.value, "lds_bits": b.value}
+
+    def set_config(self, threads: int, per_cu: int, lds_bits: int):
+        _check(_batch_set_config(self._h, threads, per_cu, lds_bits))
+        return self
+
+    def run(self):
+        """(energies, per-instance gradient lists, device ms)."""
+        import numpy as np
+        e = np.zeros(max(len(self._sizes), 1))
+        g = np.zeros(max(sum(self._sizes), 1))
+        ms = c_double()
+        _check(_batch_run(self._h, e.ctypes.data_as(_f64p), g.ctypes.data_as(_f64p), byref(ms)))
+        grads, off = [], 0
+        for k in self._sizes:
+            grads.append(g[off:off + k].copy())
+            off += k
+        return e[:len(self._sizes)].copy(), grads, ms.value

@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libqmetal.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("QMETAL_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "simulator.hip"]
+HIP_SOURCES = ["kernels.hip", "simulator.hip", "batch.hip"]
 CXX_SOURCES = ["gates.cpp", "plan.cpp", "capi.cpp"]
 
 HIPCC_FLAGS = [

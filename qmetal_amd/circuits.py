@@ -199,3 +199,17 @@ def to_hamiltonian(terms):
     for c, x, y, z in terms:
         h.add(c, x, y, z)
     return h
+
+
+def qaoa_instances(count: int = 4096, p: int = 6, lo: int = 10, hi: int = 20,
+                   seed: int = 0x0A0A):
+    """Batched QAOA-ring workload: per instance n ~ U{lo..hi},
+    gamma_k ~ U[0, pi], beta_k ~ U[0, pi/2], all from splitmix64(seed)."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = lo + int(rng.next_u64() % (hi - lo + 1))
+        gam = [rng.uniform(0, math.pi) for _ in range(p)]
+        bet = [rng.uniform(0, math.pi / 2) for _ in range(p)]
+        out.append((n, gam, bet))
+    return out

@@ -1,0 +1,57 @@
+// batch.h -- many small circuits' energies and adjoint gradients in one
+// persistent launch.
+//
+// Each instance is compiled on the host into a short program (forward plan,
+// co-state build, energy, backward sweep). A persistent kernel of a few
+// workgroups per CU pulls instances from an atomic queue, largest first, and
+// runs each program to completion inside one workgroup: both states in LDS
+// when they fit, otherwise in that workgroup's slot of an HBM scratch buffer.
+// The launch (queue reset + persistent kernel) is captured once in a
+// hipGraph and replayed; block size, workgroups per CU and the LDS cut-off are
+// picked by timing the candidates on the batch itself.
+
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "core.h"
+
+namespace qmamd {
+
+struct BatchConfig {
+  uint32_t threads = 512;   // workgroup size
+  uint32_t per_cu = 1;      // persistent workgroups per CU
+  uint32_t lds_bits = 12;   // instances with n <= lds_bits run in LDS
+};
+
+class Batch {
+ public:
+  Batch();
+  ~Batch();
+  Batch(const Batch &) = delete;
+  Batch &operator=(const Batch &) = delete;
+
+  // Compile the instances and upload their programs. Widths 1..kBatchMaxQubits.
+  void prepare(const std::vector<const Circuit *> &circuits,
+               const std::vector<const Hamiltonian *> &hams);
+  // Pick the launch configuration by timing each candidate (twice) on the
+  // prepared batch; returns the chosen one.
+  BatchConfig autotune();
+  void set_config(const BatchConfig &c);
+  BatchConfig config() const;
+  // One replay of the captured launch; results copied back.
+  void run(double *energies, double *grads);
+  // Device milliseconds of the last run (events around the graph).
+  double last_ms() const;
+  size_t total_gates() const;
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+constexpr uint32_t kBatchMaxQubits = 20;
+
+}  // namespace qmamd

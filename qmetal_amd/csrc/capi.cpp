@@ -9,6 +9,7 @@
 #include <new>
 #include <string>
 
+#include "batch.h"
 #include "core.h"
 #include "simulator.h"
 
@@ -71,6 +72,9 @@ struct qm_simulator_s {
 };
 struct qm_plan_s {
   Plan p;
+};
+struct qm_batch_s {
+  Batch *b;
 };
 
 extern "C" {
@@ -295,6 +299,126 @@ int qm_energy_and_gradient(qm_simulator s, qm_circuit c, const qm_hamiltonian *h
     if (out_energy) *out_energy = eg.first;
     if (!eg.second.empty())
       std::memcpy(out_grad, eg.second.data(), eg.second.size() * sizeof(double));
+  });
+}
+
+int qm_energy_and_gradient_device(qm_simulator s, qm_circuit c, const qm_hamiltonian *h,
+                                  const double *params, size_t num_params,
+                                  double *out_energy, double *out_grad, void *stream) {
+  if (!s || !c || !h || !out_energy || (num_params && (!params || !out_grad))) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] {
+    s->s->energy_and_gradient_device(c->c, to_hamiltonian(h), params, num_params, out_energy,
+                                     out_grad, stream);
+  });
+}
+
+qm_batch qm_batch_create(void) {
+  auto *h = new (std::nothrow) qm_batch_s;
+  if (!h) {
+    set_error("out of memory");
+    return nullptr;
+  }
+  h->b = nullptr;
+  try {
+    if (!Simulator::available()) fail("batch: no HIP device");
+    h->b = new Batch();
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+void qm_batch_destroy(qm_batch b) {
+  if (!b) return;
+  delete b->b;
+  delete b;
+}
+
+int qm_batch_prepare(qm_batch b, size_t count, const qm_circuit *circuits,
+                     const qm_hamiltonian *hams) {
+  if (!b || (count && (!circuits || !hams))) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] {
+    std::vector<Hamiltonian> hs(count);
+    std::vector<const Circuit *> cs(count);
+    std::vector<const Hamiltonian *> hp(count);
+    for (size_t i = 0; i < count; i++) {
+      if (!circuits[i]) fail("null circuit in batch");
+      cs[i] = &circuits[i]->c;
+      hs[i] = to_hamiltonian(&hams[i]);
+      hp[i] = &hs[i];
+    }
+    b->b->prepare(cs, hp);
+  });
+}
+
+int qm_batch_autotune(qm_batch b, uint32_t *threads, uint32_t *per_cu, uint32_t *lds_bits) {
+  if (!b) {
+    set_error("null batch");
+    return -1;
+  }
+  return guard([&] {
+    BatchConfig c = b->b->autotune();
+    if (threads) *threads = c.threads;
+    if (per_cu) *per_cu = c.per_cu;
+    if (lds_bits) *lds_bits = c.lds_bits;
+  });
+}
+
+int qm_batch_set_config(qm_batch b, uint32_t threads, uint32_t per_cu, uint32_t lds_bits) {
+  if (!b) {
+    set_error("null batch");
+    return -1;
+  }
+  return guard([&] {
+    BatchConfig c;
+    c.threads = threads;
+    c.per_cu = per_cu;
+    c.lds_bits = lds_bits;
+    b->b->set_config(c);
+  });
+}
+
+int qm_batch_run(qm_batch b, double *out_energy, double *out_grad, double *out_ms) {
+  if (!b) {
+    set_error("null batch");
+    return -1;
+  }
+  return guard([&] {
+    b->b->run(out_energy, out_grad);
+    if (out_ms) *out_ms = b->b->last_ms();
+  });
+}
+
+int qm_set_timing(qm_simulator s, int enable) {
+  if (!s) {
+    set_error("null simulator");
+    return -1;
+  }
+  return guard([&] { s->s->set_timing(enable != 0); });
+}
+
+int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
+                   size_t cap, size_t *count) {
+  if (!s || !count) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] {
+    auto t = s->s->read_timing();
+    *count = t.size();
+    for (size_t k = 0; k < t.size() && k < cap; k++) {
+      if (kinds) kinds[k] = t[k].kind;
+      if (ms) ms[k] = t[k].ms;
+      if (bytes) bytes[k] = t[k].bytes;
+    }
   });
 }
 

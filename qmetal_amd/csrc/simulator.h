@@ -22,6 +22,13 @@ namespace qmamd {
 // size.
 constexpr uint32_t kMaxQubits = 34;
 
+// Device time of one plan-op kernel and its algorithmic bytes.
+struct OpTime {
+  uint32_t kind;
+  double ms;
+  double bytes;
+};
+
 class Simulator {
  public:
   explicit Simulator(uint32_t n);
@@ -51,6 +58,18 @@ class Simulator {
 
   std::pair<double, std::vector<double>> energy_and_gradient(
       const Circuit &c, const Hamiltonian &h);
+
+  // Same, with parameters (one per RX/RY/RZ, circuit order) read from and
+  // energy / gradient (one per parameter) written to device memory; ordered
+  // after and before work on `caller_stream` (a hipStream_t, may be null).
+  void energy_and_gradient_device(const Circuit &c, const Hamiltonian &h,
+                                  const double *params, size_t nparams,
+                                  double *out_energy, double *out_grad,
+                                  void *caller_stream);
+
+  // Per-op timing with events on the simulator's stream (off by default).
+  void set_timing(bool on);
+  std::vector<OpTime> read_timing();
 
   // Device address of the state (for callers that share device memory).
   void *device_state();

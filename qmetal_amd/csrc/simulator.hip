@@ -390,7 +390,68 @@ struct Simulator::Impl {
     return (b + 255) & ~size_t(255);
   }
 
+  // Algorithmic bytes of a plan op: one read and one write of every
+  // amplitude it can change (8 bytes each).
+  double op_bytes(const PlanOp &p) const {
+    const double A = double(dim);
+    switch (p.kind) {
+      case OpKind::Dense1q:
+      case OpKind::Group:
+      case OpKind::Blocked:
+        return 16.0 * A;
+      case OpKind::Controlled:
+      case OpKind::Swap:
+        return 8.0 * A;
+      case OpKind::CCX:
+        return 4.0 * A;
+      case OpKind::Diagonal: {
+        if (turns(p.global) != 0.0) return 16.0 * A;
+        uint64_t common = ~0ull;
+        for (uint64_t m : p.masks) common &= m;
+        return 16.0 * A / double(uint64_t(1) << __builtin_popcountll(common));
+      }
+    }
+    return 0.0;
+  }
+
+  // Optional per-op device timing: events recorded on this stream around
+  // each plan-op launch.
+  struct OpTiming {
+    uint32_t kind;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  bool timing = false;
+  std::vector<OpTiming> timings;
+  std::vector<hipEvent_t> event_pool;
+
+  hipEvent_t take_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+
   void op(const PlanOp &p, const Pending *pl) {
+    if (p.kind == OpKind::Diagonal && p.masks.empty()) return;  // no dispatch
+    hipEvent_t a = nullptr;
+    if (timing) {
+      a = take_event();
+      check(hipEventRecord(a, stream), "hipEventRecord");
+    }
+    op_launch(p, pl);
+    if (timing) {
+      hipEvent_t b = take_event();
+      check(hipEventRecord(b, stream), "hipEventRecord");
+      timings.push_back(OpTiming{uint32_t(p.kind), op_bytes(p), a, b});
+    }
+  }
+
+  void op_launch(const PlanOp &p, const Pending *pl) {
     switch (p.kind) {
       case OpKind::Dense1q: {
         float m8[8];
@@ -413,7 +474,6 @@ struct Simulator::Impl {
         break;
       }
       case OpKind::Diagonal:
-        if (p.masks.empty()) return;  // no dispatch, as the reference
         launch_pending(*pl);
         break;
       case OpKind::Swap:
@@ -564,6 +624,11 @@ Simulator::~Simulator() {
   if (s->staging) hipHostFree(s->staging);
   if (s->host_results) hipHostFree(s->host_results);
   if (s->staging_done) hipEventDestroy(s->staging_done);
+  for (auto &t : s->timings) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
   if (s->stream) hipStreamDestroy(s->stream);
 }
 
@@ -611,6 +676,24 @@ void Simulator::run_plan(const Plan &p) {
     s->upload(total);
   }
   for (size_t k = 0; k < p.ops.size(); k++) s->op(p.ops[k], &pend[k]);
+}
+
+void Simulator::set_timing(bool on) { impl_->timing = on; }
+
+std::vector<OpTime> Simulator::read_timing() {
+  Impl *s = impl_.get();
+  s->use();
+  check(hipStreamSynchronize(s->stream), "GPU error");
+  std::vector<OpTime> out;
+  for (const auto &t : s->timings) {
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, t.a, t.b), "hipEventElapsedTime");
+    out.push_back(OpTime{t.kind, double(ms), t.bytes});
+    s->event_pool.push_back(t.a);
+    s->event_pool.push_back(t.b);
+  }
+  s->timings.clear();
+  return out;
 }
 
 void Simulator::apply_gate(const Gate &g) {
@@ -836,6 +919,58 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
     grad[k] = g.kind == K_RY ? r[2 * k] : r[2 * k + 1];
   }
   return {r[2 * G], grad};
+}
+
+// Device-resident variant for framework tensors: parameters are read from
+// device memory (one per RX/RY/RZ, circuit order), energy and gradient are
+// written to device memory, and the caller's stream is ordered before and
+// after this work through events.
+void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &h,
+                                           const double *params, size_t nparams,
+                                           double *out_energy, double *out_grad,
+                                           void *caller_stream) {
+  Impl *s = impl_.get();
+  s->use();
+  hipStream_t cs = static_cast<hipStream_t>(caller_stream);
+  hipEvent_t ev;
+  check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+  struct EventGuard {
+    hipEvent_t e;
+    ~EventGuard() { (void)hipEventDestroy(e); }
+  } guard{ev};
+  check(hipEventRecord(ev, cs), "hipEventRecord(caller)");
+  check(hipStreamWaitEvent(s->stream, ev, 0), "hipStreamWaitEvent");
+  std::vector<size_t> where;
+  for (size_t k = 0; k < c.gates.size(); k++)
+    if (is_rotation(c.gates[k].kind)) where.push_back(k);
+  if (where.size() != nparams)
+    fail("gradient: " + std::to_string(nparams) + " parameters given, circuit has " +
+         std::to_string(where.size()) + " rotations");
+  std::vector<double> host(nparams + 1);
+  if (nparams) {
+    check(hipMemcpyAsync(host.data(), params, nparams * sizeof(double), hipMemcpyDeviceToHost,
+                         s->stream),
+          "hipMemcpyAsync(params)");
+    check(hipStreamSynchronize(s->stream), "GPU error");
+  }
+  Circuit cc = c;
+  for (size_t j = 0; j < nparams; j++) cc.gates[where[j]].p[0] = host[j];
+  auto eg = energy_and_gradient(cc, h);
+  // Results leave through pinned staging so the copies stay asynchronous.
+  uint8_t *st = s->stage((nparams + 1) * sizeof(double));
+  double *hv = reinterpret_cast<double *>(st);
+  hv[0] = eg.first;
+  for (size_t j = 0; j < nparams; j++) hv[1 + j] = eg.second[where[j]];
+  check(hipMemcpyAsync(out_energy, hv, sizeof(double), hipMemcpyHostToDevice, s->stream),
+        "hipMemcpyAsync(energy)");
+  if (nparams)
+    check(hipMemcpyAsync(out_grad, hv + 1, nparams * sizeof(double), hipMemcpyHostToDevice,
+                         s->stream),
+          "hipMemcpyAsync(grad)");
+  check(hipEventRecord(s->staging_done, s->stream), "hipEventRecord");
+  s->staging_pending = true;
+  check(hipEventRecord(ev, s->stream), "hipEventRecord(done)");
+  check(hipStreamWaitEvent(cs, ev, 0), "hipStreamWaitEvent(caller)");
 }
 
 }  // namespace qmamd

@@ -1,0 +1,85 @@
+"""Batched persistent launch against the one-circuit path and the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import qmetal_amd as qm
+from qmetal_amd import circuits as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_instances():
+    """Uneven widths, every gate kind the adjoint accepts, X/Y/Z terms."""
+    out = []
+    rng = C.SplitMix64(31)
+    for k in range(40):
+        n = 3 + k % 12
+        g = C.hardware_efficient(n, 2, seed=k)
+        g += [(C.H, (0,), ()), (C.CZ, (0, n - 1), ()), (C.S, (1,), ()), (C.T, (2 % n,), ())]
+        if n >= 3:
+            g += [(C.CCX, (0, 1, 2), ()), (C.SWAP, (0, n - 1), ()), (C.CY, (n - 1, 1), ())]
+        g += [(C.RX, (q,), (rng.uniform(0, 6.28),)) for q in range(n)]
+        h = C.tfim_ring_hamiltonian(n) + [(0.25, 0, 1, 0), (0.1, 0, 3 % (1 << n), 1 << (n - 1))]
+        out.append((n, g, h))
+    for n in (15, 16):  # wider than the LDS cut-off: HBM scratch path
+        g = C.qaoa_ring(n, 2, [0.3, 0.5], [0.2, 0.9])
+        out.append((n, g, C.ring_zz_hamiltonian(n)))
+    return out
+
+
+def plain(n, g, h):
+    with qm.Simulator(n) as s:
+        e, gr = s.energy_and_gradient(C.to_circuit(n, g), C.to_hamiltonian(h))
+    return e, np.array(gr)
+
+
+def test_batch_matches_plain_path_and_oracle():
+    inst = mixed_instances()
+    circs = [C.to_circuit(n, g) for n, g, _ in inst]
+    hams = [C.to_hamiltonian(h) for _, _, h in inst]
+    b = qm.Batch().prepare(circs, hams)
+    for cfg in ((256, 1, 12), (1024, 1, 13), (512, 2, 10)):
+        try:
+            b.set_config(*cfg)
+        except qm.QMetalError:
+            continue  # LDS cut-off beyond this device
+        e, grads, ms = b.run()
+        for i, (n, g, h) in enumerate(inst):
+            e0, g0 = plain(n, g, h)
+            assert abs(e[i] - e0) <= 1e-5, (cfg, i)
+            assert np.abs(grads[i] - g0).max() <= 1e-5, (cfg, i)
+    for i in (0, 7, 13, 29):
+        n, g, h = inst[i]
+        want = O.param_shift(n, g, h)
+        assert np.abs(grads[i] - want).max() <= 1e-5, i
+        assert abs(e[i] - O.expect_h(O.run(n, g), h)) <= 1e-5
+
+
+def test_batch_graph_replays_are_identical():
+    inst = C.qaoa_instances(64, lo=6, hi=14, seed=5)
+    circs = [C.to_circuit(n, C.qaoa_ring(n, 6, ga, be)) for n, ga, be in inst]
+    hams = [C.to_hamiltonian(C.ring_zz_hamiltonian(n)) for n, _, _ in inst]
+    b = qm.Batch().prepare(circs, hams)
+    cfg = b.autotune()
+    print("autotuned:", cfg)
+    e1, g1, _ = b.run()
+    e2, g2, _ = b.run()
+    assert np.array_equal(e1, e2)
+    assert all(np.array_equal(x, y) for x, y in zip(g1, g2))
+    for i in range(0, 64, 9):
+        n, ga, be = inst[i]
+        e0, g0 = plain(n, C.qaoa_ring(n, 6, ga, be), C.ring_zz_hamiltonian(n))
+        assert abs(e1[i] - e0) <= 1e-5 and np.abs(g1[i] - g0).max() <= 1e-5
+
+
+def test_batch_fails_closed():
+    b = qm.Batch()
+    bad = C.to_circuit(3, [(C.H, (0,), ()), (C.CP, (0, 1), (0.4,))])
+    with pytest.raises(qm.QMetalError, match="differentiable"):
+        b.prepare([bad], [C.to_hamiltonian(C.ring_zz_hamiltonian(3))])
+    wide = C.to_circuit(21, [(C.H, (0,), ())])
+    with pytest.raises(qm.QMetalError, match="width"):
+        b.prepare([wide], [qm.Hamiltonian()])

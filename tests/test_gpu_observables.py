@@ -1,0 +1,194 @@
+"""FP64 reductions, sampling and the adjoint gradient on the GPU against the
+complex128 oracle and closed forms (reference tests/test_gpu.mm:258-480 and
+tests/test_python.py:34-99, restated)."""
+import math
+
+import numpy as np
+import pytest
+
+import qmetal_amd as qm
+from qmetal_amd import circuits as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def sim_with(n, gates):
+    s = qm.Simulator(n)
+    s.run(C.to_circuit(n, gates))
+    return s
+
+
+def test_expectation_closed_forms_on_ghz():
+    n = 8
+    with sim_with(n, C.ghz(n)) as s:
+        assert abs(s.expectation_pauli(z=1)) <= 1e-5
+        assert abs(s.expectation_pauli(z=0b11) - 1) <= 1e-5
+        assert abs(s.expectation_pauli(x=1)) <= 1e-5
+        assert abs(s.expectation_pauli(x=(1 << n) - 1) - 1) <= 1e-5
+        assert abs(s.expectation_pauli() - 1) <= 1e-5
+        assert abs(s.expectation("Z" + "I" * (n - 1))) <= 1e-5
+        assert abs(s.expectation("ZZ" + "I" * (n - 2)) - 1) <= 1e-5
+    with qm.Simulator(6) as s:
+        for q in range(6):
+            assert abs(s.expectation_pauli(z=1 << q) - 1) <= 1e-6
+
+
+def test_y_sign_closed_forms():
+    with sim_with(1, [(C.H, (0,), ()), (C.S, (0,), ())]) as s:
+        assert abs(s.expectation("Y") - 1) <= 1e-6
+        assert abs(s.expectation("X")) <= 1e-6
+    with sim_with(1, [(C.H, (0,), ()), (C.SDG, (0,), ())]) as s:
+        assert abs(s.expectation("Y") + 1) <= 1e-6
+    with sim_with(2, [(C.H, (0,), ()), (C.CX, (0, 1), ())]) as s:
+        assert abs(s.expectation("YY") + 1) <= 1e-6
+        assert abs(s.expectation("XX") - 1) <= 1e-6
+        assert abs(s.expectation("ZZ") - 1) <= 1e-6
+
+
+def test_random_paulis_against_oracle():
+    for n in (10, 17):
+        gates = C.random_circuit(n, 4, seed=0xABC)
+        ref = O.run(n, gates)
+        rng = C.SplitMix64(7)
+        with sim_with(n, gates) as s:
+            for _ in range(24):
+                x = y = z = 0
+                for q in range(n):
+                    r = rng.next_u64() % 4
+                    x |= (r == 1) << q
+                    y |= (r == 2) << q
+                    z |= (r == 3) << q
+                want = O.expect(ref, x, y, z)
+                got = s.expectation_pauli(x, y, z)
+                assert abs(want - got) <= 1e-5, (n, x, y, z, want, got)
+
+
+def test_hamiltonian_sum_and_norm():
+    n = 8
+    gates = C.hardware_efficient(n, 2, seed=0xFEED)
+    ref = O.run(n, gates)
+    h = qm.Hamiltonian()
+    terms = []
+    for q in range(n - 1):
+        terms += [(1.0, 0, 0, 3 << q), (0.3, 1 << q, 0, 0)]
+    for t in terms:
+        h.add(*t)
+    with sim_with(n, gates) as s:
+        assert abs(O.expect_h(ref, terms) - s.expectation(h)) <= 1e-4
+        assert abs(s.norm() - 1.0) <= 1e-5
+
+
+def test_norm_is_fp64_accurate_at_scale():
+    """A uniform state of 2^24 amplitudes: the FP64 sum of 2^24 equal terms
+    is exact to round-off, where an FP32 running sum would drift."""
+    n = 24
+    with sim_with(n, [(C.H, (q,), ()) for q in range(n)]) as s:
+        amp = np.float32(1 / math.sqrt(1 << n))
+        want = math.sqrt((1 << n) * float(amp) ** 2)
+        assert abs(s.norm() - want) <= 1e-12
+
+
+def test_sampling_ghz_poles_and_determinism():
+    n = 6
+    with sim_with(n, C.ghz(n)) as s:
+        shots = s.sample(20000, 12345)
+        assert set(shots) <= {0, (1 << n) - 1}
+        frac = shots.count(0) / len(shots)
+        assert abs(frac - 0.5) < 0.02
+        counts = s.counts(4000, seed=7)
+        assert set(counts) == {"0" * n, "1" * n}
+        assert abs(counts["0" * n] / 4000 - 0.5) < 0.03
+    with sim_with(5, C.qft(5)) as s:
+        a, b, c = s.sample(500, 999), s.sample(500, 999), s.sample(500, 1000)
+        assert a == b and a != c
+
+
+def test_sampling_total_variation():
+    n = 6
+    gates = C.random_circuit(n, 3, seed=0x1234)
+    probs = np.abs(O.run(n, gates)) ** 2
+    with sim_with(n, gates) as s:
+        draw = np.array(s.sample(200000, 2024))
+    hist = np.bincount(draw, minlength=1 << n) / len(draw)
+    tv = 0.5 * np.abs(hist - probs).sum()
+    print(f"total variation {tv:.4f}")
+    assert tv < 0.02
+
+
+def test_sampling_matches_oracle_procedure_on_gpu_amplitudes():
+    """Given the GPU's own amplitudes, the oracle's two-level procedure picks
+    the same index for every shot except ones on a CDF boundary."""
+    for n, gates in ((11, C.qaoa_ring(11, 2)), (14, C.brickwork(14, 4)), (20, C.qft(20))):
+        with sim_with(n, gates) as s:
+            shots = np.array(s.sample(20000, 99), dtype=np.uint64)
+            amps = s.amplitudes().astype(np.complex128)
+        want = O.sample(amps, 20000, 99)
+        mism = int(np.sum(shots != want))
+        print(f"n={n}: {mism} of 20000 shots differ from the oracle procedure")
+        assert mism <= 2
+
+
+def test_ghz20_invariants_and_1e5_shots():
+    n = 20
+    with sim_with(n, C.ghz(n)) as s:
+        assert abs(s.norm() - 1) <= 1e-5
+        for q in range(n):
+            assert abs(s.expectation_pauli(z=1 << q)) <= 1e-5
+        assert abs(s.expectation_pauli(z=1 | (1 << (n - 1))) - 1) <= 1e-5
+        shots = set(s.sample(100000, 7))
+        assert shots == {0, (1 << n) - 1}
+
+
+def tfim6():
+    n = 6
+    h = [(1.0, 0, 0, 3 << q) for q in range(n - 1)] + [(0.4, 1 << q, 0, 0) for q in range(n)]
+    h.append((0.25, 0, 1, 0))  # a Y term, so an i^ny sign error cannot hide
+    return n, h
+
+
+@pytest.mark.parametrize("name", ["hea_d2", "tfim_s3", "qaoa_p2"])
+def test_gradient_against_parameter_shift(name):
+    n, terms = tfim6()
+    gates = {"hea_d2": C.hardware_efficient(n, 2, seed=0xC0FFEE),
+             "tfim_s3": C.tfim_trotter(n, 3), "qaoa_p2": C.qaoa_ring(n, 2)}[name]
+    c = C.to_circuit(n, gates)
+    with qm.Simulator(n) as s:
+        e, g = s.energy_and_gradient(c, C.to_hamiltonian(terms))
+    assert abs(e - O.expect_h(O.run(n, gates), terms)) <= 1e-5
+    want = O.param_shift(n, gates, terms)
+    worst = float(np.abs(np.array(g) - want).max())
+    print(f"{name}: max |adjoint - shift| = {worst:.3e}")
+    assert worst <= 1e-5
+    for k, (kind, _, _) in enumerate(gates):
+        if kind not in (C.RX, C.RY, C.RZ):
+            assert g[k] == 0.0
+
+
+def test_gradient_ry_closed_form():
+    for t in (0.0, 0.3, 1.1, 2.7, -0.9, 0.4, 1.3, -0.7):
+        with qm.Simulator(2) as s:
+            e, g = s.energy_and_gradient(C.to_circuit(2, [(C.RY, (0,), (t,))]),
+                                         qm.Hamiltonian().add_string(1.0, "ZI"))
+        assert abs(e - math.cos(t)) <= 1e-6 and abs(g[0] + math.sin(t)) <= 1e-6
+
+
+def test_gradient_fails_closed():
+    bad = qm.Circuit(2)
+    bad.h(0).cp(0, 1, 0.4)
+    with qm.Simulator(2) as s:
+        with pytest.raises(qm.QMetalError, match="differentiable"):
+            s.energy_and_gradient(bad, qm.Hamiltonian().add_string(1.0, "ZI"))
+
+
+def test_gradient_hea_n12_against_oracle_adjoint():
+    """Wider circuit, wires past the fixed-bit threshold, TFIM ring: GPU
+    adjoint vs the complex128 adjoint."""
+    n = 12
+    gates = C.hardware_efficient(n, 3, seed=0x4EA)
+    terms = C.tfim_ring_hamiltonian(n)
+    with qm.Simulator(n) as s:
+        e, g = s.energy_and_gradient(C.to_circuit(n, gates), C.to_hamiltonian(terms))
+    e_ref, g_ref = O.adjoint(n, gates, terms)
+    assert abs(e - e_ref) <= 1e-5
+    assert np.abs(np.array(g) - g_ref).max() <= 1e-5
