@@ -65,7 +65,20 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="also time the unfused plan")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
+                    help="only run a torch device copy of 2^N complex64 (8 * 2^N bytes read "
+                         "and written) three times: the known-size kernel that calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(1 << args.copy_probe, dtype=torch.complex64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy probe: 3 x {8 << args.copy_probe} bytes read and written", flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)

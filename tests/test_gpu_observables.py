@@ -80,13 +80,17 @@ def test_hamiltonian_sum_and_norm():
 
 
 def test_norm_is_fp64_accurate_at_scale():
-    """A uniform state of 2^24 amplitudes: the FP64 sum of 2^24 equal terms
-    is exact to round-off, where an FP32 running sum would drift."""
+    """The GPU norm of 2^24 complex64 amplitudes equals the FP64 norm of the
+    same amplitudes to 1e-12 (an FP32 accumulation would be off by ~1e-7)."""
     n = 24
-    with sim_with(n, [(C.H, (q,), ()) for q in range(n)]) as s:
-        amp = np.float32(1 / math.sqrt(1 << n))
-        want = math.sqrt((1 << n) * float(amp) ** 2)
+    with sim_with(n, C.brickwork(n, 2, seed=4)) as s:
+        a = s.amplitudes()
+        want = math.sqrt(float(np.sum(a.real.astype(np.float64) ** 2 + a.imag.astype(np.float64) ** 2)))
         assert abs(s.norm() - want) <= 1e-12
+        z = s.expectation_pauli(z=1 << 5)
+        zz = np.where((np.arange(1 << n) >> 5) & 1, -1.0, 1.0)
+        want_z = float(np.sum(zz * (a.real.astype(np.float64) ** 2 + a.imag.astype(np.float64) ** 2)))
+        assert abs(z - want_z) <= 1e-12
 
 
 def test_sampling_ghz_poles_and_determinism():

@@ -106,7 +106,7 @@ def test_frozen_set_against_oracle(name):
     for n in (6, 10, 14, 20):
         gates = FROZEN[name](n)
         want = O.run(n, gates)
-        for opt in ("fused", "none", "blocked8"):
+        for opt in ("fused", "none", "blocked8") if n >= 8 else ("fused", "none"):
             c = C.to_circuit(n, gates)
             with qm.Simulator(n) as s:
                 s.run(c, options=FusionOptions(*FUSION[opt]))
@@ -122,7 +122,9 @@ def test_reference_streams_against_reference_amplitudes():
     11, under every FusionOptions, against the reference oracle's amplitudes."""
     for key, d in load_golden("frozen_small.json.gz").items():
         gates, want = golden_gates(d), golden_amps(d)
-        for opt in FUSION:
+        # A block wider than the state is rejected, as by the reference
+        # executor (src/metal/simulator.mm:468).
+        for opt in [o for o in FUSION if FUSION[o][4] <= d["n"]]:
             got = gpu_state(d["n"], gates, options=FusionOptions(*FUSION[opt]))
             assert np.abs(got - want).max() <= TOL_DEEP, (key, opt)
 
@@ -130,7 +132,7 @@ def test_reference_streams_against_reference_amplitudes():
 def test_fuzz_streams_against_reference_amplitudes():
     for k, d in enumerate(load_golden("fuzz.json.gz")):
         gates, want = golden_gates(d), golden_amps(d)
-        for opt in ("fused", "none", "blocked8"):
+        for opt in ("fused", "none", "diag_only", "win_only"):
             got = gpu_state(d["n"], gates, options=FusionOptions(*FUSION[opt]))
             assert np.abs(got - want).max() <= TOL1 * 4, (k, opt)
 
@@ -197,12 +199,25 @@ def test_fails_closed():
         c = qm.Circuit(4)
         with pytest.raises(qm.QMetalError, match="size mismatch"):
             s.run(c)
-        c = qm.Circuit(3)
-        c.h(0)
+    with qm.Simulator(5) as s:
+        # H on five wires lowers to Group(0..3) + Dense1q(4): a blockable run
+        c = C.to_circuit(5, [(C.H, (q,), ()) for q in range(5)])
         with pytest.raises(qm.QMetalError, match="block_bits"):
-            s.run(c.h(1), options=FusionOptions.blocked(13))
+            s.run(c, options=FusionOptions.blocked(13))
+    with qm.Simulator(6) as s:
+        with pytest.raises(qm.QMetalError, match="larger than the state"):
+            s.run(C.to_circuit(6, [(C.H, (q,), ()) for q in range(5)]),
+                  options=FusionOptions.blocked(8))
     with pytest.raises(qm.QMetalError):
         qm.Simulator(qm.max_qubits() + 1)
+
+
+def test_blocked_plans_at_width():
+    """Blocked1q (LDS-staged) plans where the block fits: b = 8, 10, 12."""
+    for n, b in ((10, 8), (14, 10), (16, 12)):
+        for gates in (C.tfim_trotter(n, 4), C.brickwork(n, 4), C.hardware_efficient(n, 3)):
+            d = diff(n, gates, options=FusionOptions.blocked(b))
+            assert d <= TOL_DEEP, (n, b, d)
 
 
 def test_dispatch_count():
