@@ -84,6 +84,8 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
+    # No torch here: use the ROCm runtime libqmetal.so was linked against.
+    os.environ.setdefault("QMETAL_HIP_RUNTIME", "system")
     import qmetal_amd as qm
     from qmetal_amd import circuits as C
 

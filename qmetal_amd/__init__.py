@@ -48,6 +48,32 @@ def _find_library() -> str:
                       "or point QMETAL_LIBRARY at it.")
 
 
+def _share_hip_runtime():
+    """One process holds one HIP runtime. PyTorch-ROCm resolves its HIP
+    runtime from its own lib directory, so when torch is installed its
+    runtime is loaded first and libqmetal.so binds to the same one (matching
+    soname) whether torch is imported before or after this package; device
+    pointers and streams are then shared. QMETAL_HIP_RUNTIME=system keeps the
+    runtime libqmetal.so was linked against (ROCm's), =torch requires torch's.
+    """
+    mode = os.environ.get("QMETAL_HIP_RUNTIME", "auto")
+    if mode == "system":
+        return None
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    locs = list(spec.submodule_search_locations or []) if spec else []
+    hip = os.path.join(locs[0], "lib", "libamdhip64.so") if locs else ""
+    if not hip or not os.path.exists(hip):
+        if mode == "torch":
+            raise QMetalError("QMETAL_HIP_RUNTIME=torch but torch's HIP runtime was not found")
+        return None
+    return ctypes.CDLL(hip, mode=ctypes.RTLD_GLOBAL)
+
+
+_hip_runtime = _share_hip_runtime()
 _path = _find_library()
 _lib = ctypes.CDLL(_path)
 

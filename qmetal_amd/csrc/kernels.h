@@ -31,6 +31,11 @@ struct Layout {
   uint32_t nins;
   uint8_t ins[12];
   uint8_t rpos[8];
+  // 1 when positions 1..3 are thread-index bits: then eight neighbouring lanes
+  // cover one whole 128-byte line and the accesses can bypass the caches
+  // (nontemporal); otherwise a line is split across load instructions and
+  // must be merged in L2.
+  uint32_t streaming;
 };
 
 struct GatherArgs {
@@ -61,8 +66,9 @@ struct DiagArgs {
   double constant;
   double beta0[4];
   double gamma[4][4];
-  const DiagTerm *terms;
+  const DiagTerm *terms;  // [0, nuniform): uniform in a workgroup
   uint32_t nterms;
+  uint32_t nuniform;
 };
 
 // General diagonal (masks of any weight), one amplitude pair per thread.

@@ -92,25 +92,46 @@ __device__ __forceinline__ void apply_any(float2 (&x)[1 << NL], const Slot &s) {
   }
 }
 
-template <int NL>
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__device__ __forceinline__ float4 ld4(const float2 *p) {
+  if constexpr (NT) {
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+  return *reinterpret_cast<const float4 *>(p);
+}
+
+template <bool NT>
+__device__ __forceinline__ void st4(float2 *p, float4 v) {
+  if constexpr (NT) {
+    const v4f w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<v4f *>(p));
+  } else {
+    *reinterpret_cast<float4 *>(p) = v;
+  }
+}
+
+template <int NL, bool NT>
 __device__ __forceinline__ void gather_one(float2 *__restrict__ a, uint64_t base,
                                            const uint64_t (&off)[1 << (NL - 1)],
                                            const GatherArgs &g) {
   float2 x[1 << NL];
 #pragma unroll
   for (int r = 0; r < (1 << (NL - 1)); r++) {
-    const float4 v = *reinterpret_cast<const float4 *>(a + (base | off[r]));
+    const float4 v = ld4<NT>(a + (base | off[r]));
     x[2 * r] = make_float2(v.x, v.y);
     x[2 * r + 1] = make_float2(v.z, v.w);
   }
   for (uint32_t k = 0; k < g.nslots; k++) apply_any<NL>(x, g.slots[k]);
 #pragma unroll
   for (int r = 0; r < (1 << (NL - 1)); r++)
-    *reinterpret_cast<float4 *>(a + (base | off[r])) =
-        make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
+    st4<NT>(a + (base | off[r]),
+            make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
 }
 
-template <int NL>
+template <int NL, bool NT>
 __global__ __launch_bounds__(kT) void k_gather(GatherArgs g) {
   uint64_t off[1 << (NL - 1)];
   offsets<NL>(g.lay, off);
@@ -118,22 +139,63 @@ __global__ __launch_bounds__(kT) void k_gather(GatherArgs g) {
   for (uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x; t < g.lay.nthreads;
        t += stride) {
     const uint64_t base = deposit(t, g.lay);
-    gather_one<NL>(g.a, base, off, g);
-    if (g.b) gather_one<NL>(g.b, base, off, g);
+    gather_one<NL, NT>(g.a, base, off, g);
+    if (g.b) gather_one<NL, NT>(g.b, base, off, g);
   }
 }
 
-template <int NL>
+// sin and cos of 2 pi t for a phase t in turns: t is reduced to [-1/2, 1/2]
+// in FP64, then the hardware transcendental unit, which takes its argument in
+// turns, evaluates both (v_sin_f32 / v_cos_f32, no range reduction needed).
+__device__ __forceinline__ void turn_sincos(double t, float &s, float &c) {
+  const float r = float(t - rint(t));
+  s = __builtin_amdgcn_sinf(r);
+  c = __builtin_amdgcn_cosf(r);
+}
+
+// Sum over the wave; every lane gets the total (butterfly).
+__device__ __forceinline__ double wave_allsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int NL, bool NT>
 __global__ __launch_bounds__(kT) void k_diag(DiagArgs d) {
   uint64_t off[1 << (NL - 1)];
   offsets<NL>(d.lay, off);
   const uint64_t stride = uint64_t(gridDim.x) * kT;
-  for (uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x; t < d.lay.nthreads;
-       t += stride) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t0 = uint64_t(blockIdx.x) * kT; t0 < d.lay.nthreads; t0 += stride) {
+    // Workgroup-uniform terms (they read only base-index bits that are the
+    // same for every thread of this iteration): the lanes of each wave take
+    // one term each and the wave sums the hits.
+    const uint64_t ub = deposit(t0, d.lay);
+    double phi = 0.0;
+    double beta[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = lane; k < d.nuniform; k += 64) {
+      const DiagTerm tm = d.terms[k];
+      if ((ub & tm.mask) != tm.mask) continue;
+      if (tm.kind == 2) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (tm.k == uint32_t(j)) beta[j] += tm.ang;
+      } else {
+        phi += tm.ang;
+      }
+    }
+    if (d.nuniform) {
+      phi = wave_allsum(phi);
+#pragma unroll
+      for (int j = 0; j < NL; j++) beta[j] = wave_allsum(beta[j]);
+    }
+    phi += d.constant;
+#pragma unroll
+    for (int j = 0; j < 4; j++) beta[j] += d.beta0[j];
+    const uint64_t t = t0 + threadIdx.x;
+    if (t >= d.lay.nthreads) continue;
     const uint64_t base = deposit(t, d.lay);
-    double phi = d.constant;
-    double beta[4] = {d.beta0[0], d.beta0[1], d.beta0[2], d.beta0[3]};
-    for (uint32_t k = 0; k < d.nterms; k++) {
+    for (uint32_t k = d.nuniform; k < d.nterms; k++) {
       const DiagTerm tm = d.terms[k];
       const bool hit = (base & tm.mask) == tm.mask;
       if (tm.kind == 2) {
@@ -157,17 +219,22 @@ __global__ __launch_bounds__(kT) void k_diag(DiagArgs d) {
         ph[l | (1 << h)] = ph[l] + inc;
       }
     }
+    // All loads are issued before any store so every 16-byte load of the
+    // thread is in flight at once.
+    float4 v[1 << (NL - 1)];
+#pragma unroll
+    for (int r = 0; r < (1 << (NL - 1)); r++) v[r] = ld4<NT>(d.a + (base | off[r]));
 #pragma unroll
     for (int r = 0; r < (1 << (NL - 1)); r++) {
-      float4 *p = reinterpret_cast<float4 *>(d.a + (base | off[r]));
-      float4 v = *p;
       float s0, c0, s1, c1;
-      sincospif(float(2.0 * (ph[2 * r] - rint(ph[2 * r]))), &s0, &c0);
-      sincospif(float(2.0 * (ph[2 * r + 1] - rint(ph[2 * r + 1]))), &s1, &c1);
-      const float2 a0 = cmul(make_float2(v.x, v.y), make_float2(c0, s0));
-      const float2 a1 = cmul(make_float2(v.z, v.w), make_float2(c1, s1));
-      *p = make_float4(a0.x, a0.y, a1.x, a1.y);
+      turn_sincos(ph[2 * r], s0, c0);
+      turn_sincos(ph[2 * r + 1], s1, c1);
+      const float2 a0 = cmul(make_float2(v[r].x, v[r].y), make_float2(c0, s0));
+      const float2 a1 = cmul(make_float2(v[r].z, v[r].w), make_float2(c1, s1));
+      v[r] = make_float4(a0.x, a0.y, a1.x, a1.y);
     }
+#pragma unroll
+    for (int r = 0; r < (1 << (NL - 1)); r++) st4<NT>(d.a + (base | off[r]), v[r]);
   }
 }
 
@@ -179,7 +246,7 @@ __global__ __launch_bounds__(kT) void k_diag_generic(DiagGenericArgs d) {
     for (uint32_t k = 0; k < d.nterms; k++)
       if ((i & d.masks[k]) == d.masks[k]) ph += d.angles[k];
     float s, c;
-    sincospif(float(2.0 * (ph - rint(ph))), &s, &c);
+    turn_sincos(ph, s, c);
     d.a[i] = cmul(d.a[i], make_float2(c, s));
   }
 }
@@ -573,27 +640,41 @@ void launch_zero(hipStream_t s, float2 *a, uint64_t dim) {
   k_zero<<<grid_for(dim / 2), kT, 0, s>>>(a, dim);
 }
 
-void launch_gather(hipStream_t s, int nl, const GatherArgs &g) {
+template <bool NT>
+void launch_gather_t(hipStream_t s, int nl, const GatherArgs &g) {
   const unsigned grid = grid_for(g.lay.nthreads);
   switch (nl) {
-    case 1: k_gather<1><<<grid, kT, 0, s>>>(g); break;
-    case 2: k_gather<2><<<grid, kT, 0, s>>>(g); break;
-    case 3: k_gather<3><<<grid, kT, 0, s>>>(g); break;
-    case 4: k_gather<4><<<grid, kT, 0, s>>>(g); break;
-    case 5: k_gather<5><<<grid, kT, 0, s>>>(g); break;
+    case 1: k_gather<1, NT><<<grid, kT, 0, s>>>(g); break;
+    case 2: k_gather<2, NT><<<grid, kT, 0, s>>>(g); break;
+    case 3: k_gather<3, NT><<<grid, kT, 0, s>>>(g); break;
+    case 4: k_gather<4, NT><<<grid, kT, 0, s>>>(g); break;
+    case 5: k_gather<5, NT><<<grid, kT, 0, s>>>(g); break;
+    default: break;
+  }
+}
+
+void launch_gather(hipStream_t s, int nl, const GatherArgs &g) {
+  if (g.lay.streaming) launch_gather_t<true>(s, nl, g);
+  else launch_gather_t<false>(s, nl, g);
+}
+
+
+
+template <bool NT>
+void launch_diag_t(hipStream_t s, int nl, const DiagArgs &d) {
+  const unsigned grid = grid_for(d.lay.nthreads);
+  switch (nl) {
+    case 1: k_diag<1, NT><<<grid, kT, 0, s>>>(d); break;
+    case 2: k_diag<2, NT><<<grid, kT, 0, s>>>(d); break;
+    case 3: k_diag<3, NT><<<grid, kT, 0, s>>>(d); break;
+    case 4: k_diag<4, NT><<<grid, kT, 0, s>>>(d); break;
     default: break;
   }
 }
 
 void launch_diag(hipStream_t s, int nl, const DiagArgs &d) {
-  const unsigned grid = grid_for(d.lay.nthreads);
-  switch (nl) {
-    case 1: k_diag<1><<<grid, kT, 0, s>>>(d); break;
-    case 2: k_diag<2><<<grid, kT, 0, s>>>(d); break;
-    case 3: k_diag<3><<<grid, kT, 0, s>>>(d); break;
-    case 4: k_diag<4><<<grid, kT, 0, s>>>(d); break;
-    default: break;
-  }
+  if (d.lay.streaming) launch_diag_t<true>(s, nl, d);
+  else launch_diag_t<false>(s, nl, d);
 }
 
 void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d) {

@@ -29,6 +29,15 @@ void check(hipError_t e, const char *what) {
 // complex64 is 128 bytes, a whole cache line.
 constexpr uint32_t kFixedMin = 4;
 
+// Phase layers skip amplitudes down to 128-byte runs (bit 4), a whole line:
+// skipping 64-byte halves saves nothing, HBM moves full lines.
+constexpr uint32_t kDiagFixedMin = 4;
+
+// Thread-index bits of one workgroup (256 threads): these land on the lowest
+// free positions; every other bit of the base index is uniform across the
+// workgroup for one iteration of the grid-stride loop.
+constexpr uint32_t kLaneBits = 8;
+
 // Amplitudes a thread owns by default: 2^4, i.e. eight 16-byte loads in
 // flight per thread.
 constexpr int kLocalBits = 4;
@@ -96,6 +105,10 @@ struct LayoutBuilder {
     for (size_t k = 0; k < reg.size(); k++) L.rpos[k] = uint8_t(reg[k]);
     L.fixed = fixed;
     L.nthreads = (uint64_t(1) << n) >> ins.size();
+    L.streaming = 1;
+    for (uint32_t p : ins)
+      if (p >= 1 && p <= 3) L.streaming = 0;
+    if (n < 12) L.streaming = 0;
     *nl_out = nl;
     return L;
   }
@@ -308,7 +321,7 @@ struct Simulator::Impl {
     LayoutBuilder lb{n};
     // With no global phase, amplitudes missing any common mask bit keep
     // phase 0 and are never touched.
-    if (turns(op.global) == 0.0) lb.fixed = common & ~((1ull << kFixedMin) - 1ull);
+    if (turns(op.global) == 0.0) lb.fixed = common & ~((1ull << kDiagFixedMin) - 1ull);
     DiagArgs &d = pl.d;
     d = DiagArgs{};
     d.a = state;
@@ -342,14 +355,34 @@ struct Simulator::Impl {
     for (auto &b : d.beta0) b = turns(b);
     for (auto &row : d.gamma)
       for (auto &g : row) g = turns(g);
+    // Positions of the workgroup's thread-index bits: the lowest positions
+    // the layout leaves free.
+    uint64_t lane = 0;
+    {
+      uint64_t used = d.lay.fixed | 1ull;
+      for (uint32_t k = 0; k < d.lay.nins; k++) used |= 1ull << d.lay.ins[k];
+      uint32_t got = 0;
+      for (uint32_t p = 0; p < n && got < kLaneBits; p++)
+        if (!((used >> p) & 1)) {
+          lane |= 1ull << p;
+          got++;
+        }
+    }
+    // Terms that do not read a thread-index bit are uniform across the
+    // workgroup and go first; the kernel evaluates them with scalar branches.
     DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off);
     size_t k = 0;
-    for (const auto &kv : terms) {
-      dt[k].kind = std::get<0>(kv.first);
-      dt[k].mask = std::get<1>(kv.first);
-      dt[k].k = std::get<2>(kv.first);
-      dt[k].ang = turns(kv.second);
-      k++;
+    for (int pass = 0; pass < 2; pass++) {
+      for (const auto &kv : terms) {
+        const bool uniform = (std::get<1>(kv.first) & lane) == 0;
+        if (uniform != (pass == 0)) continue;
+        dt[k].kind = std::get<0>(kv.first);
+        dt[k].mask = std::get<1>(kv.first);
+        dt[k].k = std::get<2>(kv.first);
+        dt[k].ang = turns(kv.second);
+        k++;
+      }
+      if (pass == 0) d.nuniform = uint32_t(k);
     }
     d.terms = reinterpret_cast<const DiagTerm *>(arena + off);
     d.nterms = uint32_t(k);
