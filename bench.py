@@ -46,6 +46,65 @@ def cpu_baseline(n_cpu: int, n_gpu: int, threads: int):
     return dt, dt * scale
 
 
+def batched_qaoa(qm, C, count: int, repeats: int, plain_count: int):
+    """4096 QAOA-ring p=6 instances (n ~ U{10..20}, gamma ~ U[0, pi],
+    beta ~ U[0, pi/2], splitmix64): energy <sum Z_i Z_i+1> and the full
+    adjoint gradient of every instance from one persistent graph launch,
+    against the same instances solved one circuit per launch."""
+    import numpy as np
+    inst = C.qaoa_instances(count)
+    t0 = time.perf_counter()
+    circs = [C.to_circuit(n, C.qaoa_ring(n, 6, g, b)) for n, g, b in inst]
+    hams = [C.to_hamiltonian(C.ring_zz_hamiltonian(n)) for n, _, _ in inst]
+    t_build = time.perf_counter() - t0
+    b = qm.Batch()
+    t0 = time.perf_counter()
+    b.prepare(circs, hams)
+    t_prep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cfg = b.autotune()
+    t_tune = time.perf_counter() - t0
+    e, g, _ = b.run()
+    dev_ms, wall = [], []
+    for _ in range(repeats):
+        w0 = time.perf_counter()
+        e, g, ms = b.run()
+        wall.append(time.perf_counter() - w0)
+        dev_ms.append(ms)
+    batch_s = min(wall)
+    # One circuit per launch: qm_energy_and_gradient per instance, one
+    # simulator per width (allocated up front, outside the timing).
+    sims = {n: qm.Simulator(n) for n in sorted({n for n, _, _ in inst})}
+    k = min(plain_count, count)
+    worst = 0.0
+    p0 = time.perf_counter()
+    for i in range(k):
+        n = inst[i][0]
+        e0, g0 = sims[n].energy_and_gradient(circs[i], hams[i])
+        worst = max(worst, abs(e0 - e[i]), float(np.abs(np.array(g0) - g[i]).max()))
+    plain_s = (time.perf_counter() - p0) * count / k
+    for s in sims.values():
+        s.close()
+    params = sum(12 * n for n, _, _ in inst)
+    return {
+        "workload": f"qaoa_ring_p6_x{count}_n10-20",
+        "instances": count,
+        "batched_s": round(batch_s, 4),
+        "batched_device_ms": round(min(dev_ms), 3),
+        "batched_grads_per_s": round(count / batch_s, 1),
+        "batched_param_derivs_per_s": round(params / batch_s, 1),
+        "plain_s": round(plain_s, 3),
+        "plain_grads_per_s": round(count / plain_s, 1),
+        "plain_sample": k,
+        "speedup": round(plain_s / batch_s, 2),
+        "max_abs_diff_vs_plain": worst,
+        "autotuned": cfg,
+        "host_build_s": round(t_build, 2),
+        "prepare_s": round(t_prep, 2),
+        "autotune_s": round(t_tune, 2),
+    }
+
+
 def load_traffic(workload: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -65,6 +124,11 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="also time the unfused plan")
+    ap.add_argument("--batch-count", type=int, default=4096,
+                    help="batched QAOA instances (0 skips the batched measurement)")
+    ap.add_argument("--batch-repeats", type=int, default=3)
+    ap.add_argument("--plain-count", type=int, default=4096,
+                    help="instances solved one circuit per launch for the comparison")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only run a torch device copy of 2^N complex64 (8 * 2^N bytes read "
                          "and written) three times: the known-size kernel that calibrates the "
@@ -150,6 +214,17 @@ def main() -> int:
         print(f"# unfused: {unfused:.3f} s/circuit", file=sys.stderr, flush=True)
     sim.close()
 
+    batched = None
+    if args.batch_count:
+        try:
+            batched = batched_qaoa(qm, C, args.batch_count, args.batch_repeats, args.plain_count)
+            print(f"# batched QAOA x{args.batch_count}: {batched['batched_s']:.3f} s vs plain "
+                  f"{batched['plain_s']:.2f} s ({batched['speedup']}x), max diff "
+                  f"{batched['max_abs_diff_vs_plain']:.2e}, config {batched['autotuned']}",
+                  file=sys.stderr, flush=True)
+        except Exception as exc:  # report, do not lose the headline line
+            batched = {"error": str(exc)}
+
     cpu = None
     if not args.no_cpu:
         dt, extrap = cpu_baseline(args.cpu_n, n, args.cpu_threads)
@@ -188,6 +263,8 @@ def main() -> int:
     }
     if unfused is not None:
         out["unfused_s_per_circuit"] = round(unfused, 4)
+    if batched is not None:
+        out["batched_adjoint"] = batched
     print(json.dumps(out), flush=True)
     return 0
 

@@ -21,7 +21,7 @@
 namespace qmamd {
 
 struct BatchConfig {
-  uint32_t threads = 512;   // workgroup size
+  uint32_t threads = 512;   // workgroup size (64..512)
   uint32_t per_cu = 1;      // persistent workgroups per CU
   uint32_t lds_bits = 12;   // instances with n <= lds_bits run in LDS
 };

@@ -6,12 +6,24 @@
 //                               control mask), SWAP, DIAG (FP64 phase terms)
 //   ACCUM per Pauli flip        lam (+)= sum_t c_t i^ny_t s_t(j) psi_j
 //   ENERGY                      E = Re<psi|lam>, FP64 block reduction
-//   backward, last gate first   ROT: FP64 <lam|P|psi> then the inverse
-//                               rotation on both states; other gates: the
-//                               inverse on both states
+//   backward sweep              segments of the gate stream, last first:
+//     ROTGROUP  up to 3 one-qubit gates on distinct wires: for each, the FP64
+//               overlap <lam|P|psi> (rotations) then its inverse, on both
+//               states held in registers
+//     DIAGRUN   a run of diagonal blocks (diagonal gates and CX.RZ.CX
+//               triples): the FP64 overlaps Im<lam|Z...Z|psi> of its
+//               rotations, then one inverse phase pass on both states
+//     CTRL/SWAP any other gate: its inverse on both states
+//
 // The semantics are those of the single-circuit adjoint (reference
-// src/metal/simulator.mm:814-963); results are per gate, 0 for gates without
-// a parameter.
+// src/metal/simulator.mm:814-963): grad[k] = Im<lam_k|P_k|psi_k> with both
+// states after gate k and every later gate undone. Gates that commute can be
+// undone in any order without changing these overlaps -- <U'lam|P|U'psi> =
+// <lam|P|psi> whenever U' commutes with P -- which is what lets a segment
+// take all its overlaps at one point: one-qubit gates on distinct wires
+// commute, diagonal blocks commute, and CX(a,b) Z_b CX(a,b) = Z_a Z_b gives
+// the generator of the rotation inside a CX.RZ.CX triple. Results are per
+// gate, 0 for gates without a parameter.
 
 #include <hip/hip_runtime.h>
 
@@ -30,13 +42,21 @@ void check(hipError_t e, const char *what) {
   if (e != hipSuccess) fail(std::string("batch: ") + what + ": " + hipGetErrorString(e));
 }
 
-enum : uint8_t { B_INIT = 0, B_GROUP, B_CTRL, B_SWAP, B_DIAG, B_ACCUM, B_ENERGY, B_ROT };
+enum : uint8_t {
+  B_INIT = 0, B_GROUP, B_CTRL, B_SWAP, B_DIAG, B_ACCUM, B_ENERGY, B_ROTGROUP, B_DIAGRUN
+};
 constexpr uint8_t F_BOTH = 1, F_ASSIGN = 2;
+constexpr uint32_t kNoAxis = 3;
+constexpr int kOverlapChunk = 8;
+// Wires per ROTGROUP: both states' 2^3 amplitudes and 6 FP64 accumulators
+// stay in registers within the 128 VGPRs a 1024-thread workgroup allows.
+constexpr size_t kRotGroup = 3;
 
 struct BOp {
-  uint8_t code, flags, k, axis;
+  uint8_t code, flags, k, pad;
   uint8_t w[4];
-  uint32_t a, b;
+  uint32_t a, b;  // group: matrices offset; diag/accum/diagrun: term offset, count
+  uint32_t c, d;  // rotgroup: aux offset; diagrun: overlap offset, count
   uint64_t mask;
   float m[8];
 };
@@ -56,6 +76,7 @@ struct BatchArgs {
   const uint32_t *order;
   const BTerm *terms;
   const float *mats;
+  const uint32_t *aux;
   uint32_t count;
   uint32_t lds_bits;
   uint64_t slot;
@@ -65,7 +86,8 @@ struct BatchArgs {
   double *grad;
 };
 
-constexpr int kMaxThreads = 1024;
+constexpr int kMaxThreads = 512;
+constexpr int kWaves = kMaxThreads / 64;
 
 __device__ __forceinline__ uint32_t ins0(uint32_t x, uint32_t p) {
   return ((x >> p) << (p + 1)) | (x & ((1u << p) - 1u));
@@ -83,28 +105,31 @@ __device__ __forceinline__ double sgn(uint32_t j, uint64_t sign) {
   return (__popcll(uint64_t(j) & sign) & 1) ? -1.0 : 1.0;
 }
 
-// Workgroup sum of two doubles; the totals are valid in thread 0.
-__device__ void block_sum2(double &a, double &b, double (*red)[kMaxThreads / 64]) {
+// Workgroup sums of N doubles; totals valid in thread 0.
+template <int N>
+__device__ void block_sum(double (&v)[N], double (*red)[kWaves]) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    a += __shfl_down(a, o, 64);
-    b += __shfl_down(b, o, 64);
-  }
+  for (int i = 0; i < N; i++)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_down(v[i], o, 64);
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[0][w] = a;
-    red[1][w] = b;
-  }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int i = 0; i < N; i++) red[i][w] = v[i];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double s0 = 0.0, s1 = 0.0;
-    for (int k = 0; k < int(blockDim.x >> 6); k++) {
-      s0 += red[0][k];
-      s1 += red[1][k];
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      double s = 0.0;
+      for (int k = 0; k < int(blockDim.x >> 6); k++) s += red[i][k];
+      v[i] = s;
     }
-    a = s0;
-    b = s1;
-  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void acc_cj(double &re, double &im, float2 x, float2 y, double s) {
+  re += s * (double(x.x) * y.x + double(x.y) * y.y);
+  im += s * (double(x.x) * y.y - double(x.y) * y.x);
 }
 
 template <int K>
@@ -147,6 +172,82 @@ __device__ void group_any(float2 *s, uint32_t dim, const BOp &op, const float *m
   }
 }
 
+// Up to three one-qubit gates on distinct wires, on both states: for each
+// wire, the overlap <lam|P|psi> of its rotation (if any), then its inverse.
+// aux holds (axis, grad slot) per wire; axis 0/1/2 = X/Y/Z, kNoAxis = none.
+template <int K>
+__device__ void rotgroup(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
+                         const float *mats, const uint32_t *aux, double *out,
+                         double (*red)[kWaves]) {
+  double acc[2 * K];
+#pragma unroll
+  for (int i = 0; i < 2 * K; i++) acc[i] = 0.0;
+  const uint32_t units = dim >> K;
+  for (uint32_t u = threadIdx.x; u < units; u += blockDim.x) {
+    uint32_t base = u;
+#pragma unroll
+    for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
+    float2 p[1 << K], l[1 << K];
+    uint32_t idx[1 << K];
+#pragma unroll
+    for (int t = 0; t < (1 << K); t++) {
+      uint32_t o = base;
+#pragma unroll
+      for (int j = 0; j < K; j++)
+        if ((t >> j) & 1) o |= 1u << op.w[j];
+      idx[t] = o;
+      p[t] = psi[o];
+      l[t] = lam[o];
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const uint32_t axis = aux[op.c + 2 * j];
+#pragma unroll
+      for (int t = 0; t < (1 << K); t++) {
+        if ((t >> j) & 1) continue;
+        const int t1 = t | (1 << j);
+        if (axis == 0) {         // X
+          acc_cj(acc[2 * j], acc[2 * j + 1], l[t], p[t1], 1.0);
+          acc_cj(acc[2 * j], acc[2 * j + 1], l[t1], p[t], 1.0);
+        } else if (axis == 1) {  // Y, before its factor i
+          acc_cj(acc[2 * j], acc[2 * j + 1], l[t], p[t1], -1.0);
+          acc_cj(acc[2 * j], acc[2 * j + 1], l[t1], p[t], 1.0);
+        } else if (axis == 2) {  // Z
+          acc_cj(acc[2 * j], acc[2 * j + 1], l[t], p[t], 1.0);
+          acc_cj(acc[2 * j], acc[2 * j + 1], l[t1], p[t1], -1.0);
+        }
+        const float *m = mats + op.a + 8 * j;
+        rot2(m, p[t], p[t1]);
+        rot2(m, l[t], l[t1]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < (1 << K); t++) {
+      psi[idx[t]] = p[t];
+      lam[idx[t]] = l[t];
+    }
+  }
+  block_sum<2 * K>(acc, red);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const uint32_t axis = aux[op.c + 2 * j];
+      // dE/dt = Im(i^ny S): ny = 1 for Y.
+      if (axis != kNoAxis) out[aux[op.c + 2 * j + 1]] = axis == 1 ? acc[2 * j] : acc[2 * j + 1];
+    }
+}
+
+__device__ void rotgroup_any(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
+                             const float *mats, const uint32_t *aux, double *out,
+                             double (*red)[kWaves]) {
+  switch (op.k) {
+    case 1: rotgroup<1>(psi, lam, dim, op, mats, aux, out, red); break;
+    case 2: rotgroup<2>(psi, lam, dim, op, mats, aux, out, red); break;
+    case 3: rotgroup<3>(psi, lam, dim, op, mats, aux, out, red); break;
+    default: break;
+  }
+}
+
 __device__ void ctrl(float2 *s, uint32_t dim, const BOp &op) {
   const uint32_t t = op.w[0], bit = 1u << t, cm = uint32_t(op.mask);
   for (uint32_t u = threadIdx.x; u < dim / 2; u += blockDim.x) {
@@ -170,17 +271,53 @@ __device__ void swap2(float2 *s, uint32_t dim, const BOp &op) {
   }
 }
 
+__device__ __forceinline__ float2 phase_of(uint32_t i, const BTerm *terms, uint32_t a,
+                                           uint32_t b) {
+  double ph = 0.0;
+  for (uint32_t t = a; t < a + b; t++) {
+    const uint64_t m = terms[t].mask;
+    if ((uint64_t(i) & m) == m) ph += terms[t].x;
+  }
+  const float r = float(ph - rint(ph));
+  return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+}
+
+__device__ __forceinline__ float2 cmulf(float2 v, float2 e) {
+  return make_float2(v.x * e.x - v.y * e.y, v.x * e.y + v.y * e.x);
+}
+
 __device__ void diag(float2 *s, uint32_t dim, const BOp &op, const BTerm *terms) {
-  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) {
-    double ph = 0.0;
-    for (uint32_t t = op.a; t < op.a + op.b; t++) {
-      const uint64_t m = terms[t].mask;
-      if ((uint64_t(i) & m) == m) ph += terms[t].x;
+  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x)
+    s[i] = cmulf(s[i], phase_of(i, terms, op.a, op.b));
+}
+
+// Overlaps Im<lam|Z_m|psi> for each Z-string mask of the run (in chunks of
+// kOverlapChunk), then the run's inverse phase on both states.
+__device__ void diagrun(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
+                        const BTerm *terms, double *out, double (*red)[kWaves]) {
+  for (uint32_t c0 = 0; c0 < op.d; c0 += kOverlapChunk) {
+    double acc[kOverlapChunk];
+    uint64_t zm[kOverlapChunk];
+#pragma unroll
+    for (int r = 0; r < kOverlapChunk; r++) {
+      acc[r] = 0.0;
+      zm[r] = c0 + r < op.d ? terms[op.c + c0 + r].mask : 0;
     }
-    float sn, cs;
-    sincospif(float(2.0 * (ph - rint(ph))), &sn, &cs);
-    const float2 v = s[i];
-    s[i] = make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+    for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) {
+      const float2 l = lam[i], p = psi[i];
+      const double w = double(l.x) * p.y - double(l.y) * p.x;  // Im(conj(l) p)
+#pragma unroll
+      for (int r = 0; r < kOverlapChunk; r++) acc[r] += sgn(i, zm[r]) * w;
+    }
+    block_sum<kOverlapChunk>(acc, red);
+    if (threadIdx.x == 0)
+      for (uint32_t r = 0; r < kOverlapChunk && c0 + r < op.d; r++)
+        out[uint32_t(terms[op.c + c0 + r].x)] = acc[r];
+  }
+  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) {
+    const float2 e = phase_of(i, terms, op.a, op.b);
+    psi[i] = cmulf(psi[i], e);
+    lam[i] = cmulf(lam[i], e);
   }
 }
 
@@ -206,52 +343,18 @@ __device__ void accum(const float2 *psi, float2 *lam, uint32_t dim, const BOp &o
   }
 }
 
-__device__ __forceinline__ void acc_cj(double &re, double &im, float2 x, float2 y, double s) {
-  re += s * (double(x.x) * y.x + double(x.y) * y.y);
-  im += s * (double(x.x) * y.y - double(x.y) * y.x);
-}
-
 __device__ void energy(const float2 *psi, const float2 *lam, uint32_t dim, const BOp &op,
-                       double *out, double (*red)[kMaxThreads / 64]) {
-  double re = 0.0, im = 0.0;
-  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) acc_cj(re, im, psi[i], lam[i], 1.0);
-  block_sum2(re, im, red);
-  if (threadIdx.x == 0) out[op.a] = re;
-}
-
-__device__ void rot_grad(float2 *psi, float2 *lam, uint32_t dim, const BOp &op, double *out,
-                         double (*red)[kMaxThreads / 64]) {
-  const uint32_t q = op.w[0], bit = 1u << q;
-  double re = 0.0, im = 0.0;
-  for (uint32_t u = threadIdx.x; u < dim / 2; u += blockDim.x) {
-    const uint32_t i0 = ins0(u, q), i1 = i0 | bit;
-    float2 p0 = psi[i0], p1 = psi[i1], l0 = lam[i0], l1 = lam[i1];
-    if (op.axis == 0) {
-      acc_cj(re, im, l0, p1, 1.0);
-      acc_cj(re, im, l1, p0, 1.0);
-    } else if (op.axis == 1) {
-      acc_cj(re, im, l0, p1, -1.0);
-      acc_cj(re, im, l1, p0, 1.0);
-    } else {
-      acc_cj(re, im, l0, p0, 1.0);
-      acc_cj(re, im, l1, p1, -1.0);
-    }
-    rot2(op.m, p0, p1);
-    rot2(op.m, l0, l1);
-    psi[i0] = p0;
-    psi[i1] = p1;
-    lam[i0] = l0;
-    lam[i1] = l1;
-  }
-  block_sum2(re, im, red);
-  // dE/dt = Im(i^ny S): ny = 1 for Y.
-  if (threadIdx.x == 0) out[op.a] = op.axis == 1 ? re : im;
+                       double *out, double (*red)[kWaves]) {
+  double v[2] = {0.0, 0.0};
+  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) acc_cj(v[0], v[1], psi[i], lam[i], 1.0);
+  block_sum<2>(v, red);
+  if (threadIdx.x == 0) out[op.a] = v[0];
 }
 
 __global__ __launch_bounds__(kMaxThreads) void k_batch(BatchArgs A) {
   extern __shared__ float2 lds[];
   __shared__ uint32_t s_idx;
-  __shared__ double red[2][kMaxThreads / 64];
+  __shared__ double red[kOverlapChunk][kWaves];
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_idx = atomicAdd(A.counter, 1u);
@@ -296,8 +399,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_batch(BatchArgs A) {
         case B_ENERGY:
           energy(psi, lam, dim, op, A.energy, red);
           break;
-        case B_ROT:
-          rot_grad(psi, lam, dim, op, A.grad, red);
+        case B_ROTGROUP:
+          rotgroup_any(psi, lam, dim, op, A.mats, A.aux, A.grad, red);
+          break;
+        case B_DIAGRUN:
+          diagrun(psi, lam, dim, op, A.terms, A.grad, red);
           break;
         default:
           break;
@@ -342,10 +448,19 @@ Gate inverse_gate(const Gate &g) {
 
 bool is_rot(uint8_t k) { return k == K_RX || k == K_RY || k == K_RZ; }
 
+// CX(a,b) RZ_b(t) CX(a,b) at position i.
+bool zz_triple(const Circuit &c, size_t i) {
+  if (i + 2 >= c.gates.size()) return false;
+  const Gate &x = c.gates[i], &r = c.gates[i + 1], &y = c.gates[i + 2];
+  return x.kind == K_CX && y.kind == K_CX && r.kind == K_RZ && x.q[0] == y.q[0] &&
+         x.q[1] == y.q[1] && r.q[0] == x.q[1];
+}
+
 struct Program {
   std::vector<BOp> ops;
   std::vector<BTerm> terms;
   std::vector<float> mats;
+  std::vector<uint32_t> aux;
 
   BOp &add(uint8_t code, uint8_t flags = 0) {
     BOp o{};
@@ -439,6 +554,76 @@ struct Program {
         break;
     }
   }
+
+  // Backward segment: one-qubit gates [i, j) of the forward stream, any
+  // wires. Each wire's gates are undone last-first; the wires' remaining
+  // last gates form a layer on distinct wires, issued as ROTGROUPs of <= 3.
+  void window_backward(const Circuit &c, size_t i, size_t j, uint32_t grad_base) {
+    std::map<uint32_t, std::vector<size_t>> per_wire;
+    for (size_t k = i; k < j; k++) per_wire[c.gates[k].q[0]].push_back(k);
+    for (;;) {
+      std::vector<size_t> layer;
+      for (auto &kv : per_wire)
+        if (!kv.second.empty()) {
+          layer.push_back(kv.second.back());
+          kv.second.pop_back();
+        }
+      if (layer.empty()) break;
+      for (size_t b = 0; b < layer.size(); b += kRotGroup) {
+        BOp &o = add(B_ROTGROUP, F_BOTH);
+        const size_t take = std::min<size_t>(kRotGroup, layer.size() - b);
+        o.k = uint8_t(take);
+        o.a = uint32_t(mats.size());
+        o.c = uint32_t(aux.size());
+        for (size_t t = 0; t < take; t++) {
+          const Gate &g = c.gates[layer[b + t]];
+          const Gate inv = inverse_gate(g);
+          o.w[t] = uint8_t(g.q[0]);
+          cd m[4];
+          gate_matrix(inv.kind, inv.p, m);
+          float f[8];
+          narrow(m, f);
+          mats.insert(mats.end(), f, f + 8);
+          aux.push_back(g.kind == K_RX ? 0u : g.kind == K_RY ? 1u : g.kind == K_RZ ? 2u : kNoAxis);
+          aux.push_back(grad_base + uint32_t(layer[b + t]));
+        }
+      }
+    }
+  }
+
+  // Backward segment: a run of diagonal blocks. Inverse phase terms (all
+  // angles and the global phase negated) and one Z-string overlap per
+  // rotation: Z_q for a diagonal RZ_q, Z_a Z_b for the RZ of a CX.RZ.CX.
+  void diagrun_backward(const Circuit &c, const std::vector<std::pair<size_t, size_t>> &blocks,
+                        uint32_t grad_base) {
+    std::vector<uint64_t> masks;
+    std::vector<double> angles;
+    double global = 0.0;
+    std::vector<std::pair<uint64_t, uint32_t>> overlaps;
+    for (const auto &bl : blocks) {
+      const size_t k = bl.first;
+      if (bl.second == 3) {
+        const uint64_t ma = 1ull << c.gates[k].q[0], mb = 1ull << c.gates[k].q[1];
+        const double t = c.gates[k + 1].p[0];
+        masks.insert(masks.end(), {ma, mb, ma | mb});
+        angles.insert(angles.end(), {t, t, -2.0 * t});
+        global += -t / 2.0;
+        overlaps.push_back({ma | mb, grad_base + uint32_t(k + 1)});
+      } else {
+        global += diagonal_term(c.gates[k], masks, angles);
+        if (c.gates[k].kind == K_RZ)
+          overlaps.push_back({1ull << c.gates[k].q[0], grad_base + uint32_t(k)});
+      }
+    }
+    BOp &o = add(B_DIAGRUN, F_BOTH);
+    o.a = uint32_t(terms.size());
+    terms.push_back(BTerm{0, turns(-global), 0.0});
+    for (size_t t = 0; t < masks.size(); t++) terms.push_back(BTerm{masks[t], turns(-angles[t]), 0.0});
+    o.b = uint32_t(terms.size()) - o.a;
+    o.c = uint32_t(terms.size());
+    for (const auto &ov : overlaps) terms.push_back(BTerm{ov.first, double(ov.second), 0.0});
+    o.d = uint32_t(overlaps.size());
+  }
 };
 
 void compile(const Circuit &c, const Hamiltonian &h, uint32_t energy_slot, uint32_t grad_base,
@@ -478,23 +663,47 @@ void compile(const Circuit &c, const Hamiltonian &h, uint32_t energy_slot, uint3
       }
       pg.terms.push_back(BTerm{t->p.y | t->p.z, re, im});
     }
-    pg.ops[pg.ops.size() - 1].b = uint32_t(pg.terms.size()) - pg.ops.back().a;
+    pg.ops.back().b = uint32_t(pg.terms.size()) - pg.ops.back().a;
   }
   pg.add(B_ENERGY).a = energy_slot;
-  for (size_t k = c.gates.size(); k-- > 0;) {
-    const Gate &g = c.gates[k];
-    const Gate inv = inverse_gate(g);
-    if (is_rot(g.kind)) {
-      BOp &o = pg.add(B_ROT, F_BOTH);
-      o.axis = g.kind == K_RX ? 0 : g.kind == K_RY ? 1 : 2;
-      o.w[0] = uint8_t(g.q[0]);
-      o.a = grad_base + uint32_t(k);
-      cd m[4];
-      gate_matrix(inv.kind, inv.p, m);
-      narrow(m, o.m);
+
+  // Segment the forward stream: runs of diagonal blocks that contain a
+  // multi-qubit block, windows of one-qubit gates, single other gates.
+  struct Seg {
+    int kind;  // 0 window [i, j), 1 diagonal run, 2 single gate i
+    size_t i, j;
+    std::vector<std::pair<size_t, size_t>> blocks;
+  };
+  std::vector<Seg> segs;
+  const size_t G = c.gates.size();
+  auto block_len = [&](size_t k) -> size_t {
+    if (zz_triple(c, k)) return 3;
+    return is_diagonal(c.gates[k]) ? 1 : 0;
+  };
+  for (size_t i = 0; i < G;) {
+    const Gate &g = c.gates[i];
+    if (zz_triple(c, i) || (g.nq == 2 && is_diagonal(g))) {
+      Seg s{1, i, i, {}};
+      size_t j = i;
+      for (size_t l; j < G && (l = block_len(j)) != 0; j += l) s.blocks.push_back({j, l});
+      s.j = j;
+      segs.push_back(std::move(s));
+      i = j;
+    } else if (g.nq == 1) {
+      size_t j = i;
+      while (j < G && c.gates[j].nq == 1) j++;
+      segs.push_back(Seg{0, i, j, {}});
+      i = j;
     } else {
-      pg.gate(inv, F_BOTH);
+      segs.push_back(Seg{2, i, i + 1, {}});
+      i++;
     }
+  }
+  for (size_t s = segs.size(); s-- > 0;) {
+    const Seg &sg = segs[s];
+    if (sg.kind == 0) pg.window_backward(c, sg.i, sg.j, grad_base);
+    else if (sg.kind == 1) pg.diagrun_backward(c, sg.blocks, grad_base);
+    else pg.gate(inverse_gate(c.gates[sg.i]), F_BOTH);
   }
 }
 
@@ -518,8 +727,11 @@ struct Batch::Impl {
   BOp *ops = nullptr;
   BInst *insts = nullptr;
   uint32_t *order = nullptr;
+  uint32_t *sample_order = nullptr;
+  uint32_t sample_count = 0;
   BTerm *terms = nullptr;
   float *mats = nullptr;
+  uint32_t *aux = nullptr;
   float2 *scratch = nullptr;
   size_t scratch_wgs = 0;
   uint64_t slot = 0;
@@ -538,15 +750,19 @@ struct Batch::Impl {
     hipFree(ops);
     hipFree(insts);
     hipFree(order);
+    hipFree(sample_order);
     hipFree(terms);
     hipFree(mats);
+    hipFree(aux);
     hipFree(energy);
     hipFree(grad);
     ops = nullptr;
     insts = nullptr;
     order = nullptr;
+    sample_order = nullptr;
     terms = nullptr;
     mats = nullptr;
+    aux = nullptr;
     energy = nullptr;
     grad = nullptr;
   }
@@ -557,6 +773,7 @@ struct Batch::Impl {
     const uint64_t need_slot = max_n > cfg.lds_bits ? (uint64_t(1) << max_n) : 0;
     if (need_slot == 0) return;
     if (scratch && slot >= need_slot && scratch_wgs >= wgs) return;
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     if (scratch) check(hipFree(scratch), "hipFree");
     scratch = nullptr;
     slot = need_slot;
@@ -565,49 +782,63 @@ struct Batch::Impl {
           "hipMalloc(scratch)");
   }
 
-  void capture() {
-    if (exec) {
-      check(hipGraphExecDestroy(exec), "hipGraphExecDestroy");
-      exec = nullptr;
-    }
-    const uint32_t wgs = uint32_t(cus) * cfg.per_cu;
-    ensure_scratch(wgs);
+  BatchArgs args(bool sample) {
     BatchArgs A{};
     A.ops = ops;
     A.insts = insts;
-    A.order = order;
+    A.order = sample ? sample_order : order;
     A.terms = terms;
     A.mats = mats;
-    A.count = count;
+    A.aux = aux;
+    A.count = sample ? sample_count : count;
     A.lds_bits = cfg.lds_bits;
     A.slot = slot;
     A.scratch = scratch;
     A.counter = counter;
     A.energy = energy;
     A.grad = grad;
+    return A;
+  }
+
+  void enqueue(bool sample) {
+    const uint32_t wgs = uint32_t(cus) * cfg.per_cu;
+    ensure_scratch(wgs);
     const size_t lds = lds_bytes(cfg);
     check(hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
           "hipFuncSetAttribute");
+    k_reset<<<1, 64, 0, stream>>>(counter);
+    k_batch<<<wgs, cfg.threads, lds, stream>>>(args(sample));
+  }
+
+  void capture() {
+    if (exec) {
+      check(hipGraphExecDestroy(exec), "hipGraphExecDestroy");
+      exec = nullptr;
+    }
+    ensure_scratch(uint32_t(cus) * cfg.per_cu);
     hipGraph_t graph;
     check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    k_reset<<<1, 64, 0, stream>>>(counter);
-    k_batch<<<wgs, cfg.threads, lds, stream>>>(A);
+    enqueue(false);
     check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     check(hipGraphDestroy(graph), "hipGraphDestroy");
     captured = true;
   }
 
-  void launch() {
-    if (!captured) capture();
+  double timed(bool sample) {
     check(hipEventRecord(ev0, stream), "hipEventRecord");
-    check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+    if (sample) {
+      enqueue(true);
+    } else {
+      if (!captured) capture();
+      check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+    }
     check(hipEventRecord(ev1, stream), "hipEventRecord");
     check(hipStreamSynchronize(stream), "GPU error");
     float t = 0.0f;
     check(hipEventElapsedTime(&t, ev0, ev1), "hipEventElapsedTime");
-    ms = t;
+    return t;
   }
 };
 
@@ -625,6 +856,7 @@ Batch::Batch() : impl_(new Impl) {
     b->max_lds = size_t(optin);
   (void)hipGetLastError();
   b->cfg.lds_bits = b->max_lds >= (size_t(2) << 13 << 3) ? 13 : 12;
+  b->cfg.threads = 512;
   check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
   check(hipEventCreate(&b->ev0), "hipEventCreate");
   check(hipEventCreate(&b->ev1), "hipEventCreate");
@@ -653,7 +885,7 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
   std::vector<BInst> insts;
   std::vector<double> cost;
   uint32_t grad_base = 0;
-  b->max_n = 0;
+  uint32_t max_n = 0;
   for (size_t i = 0; i < circuits.size(); i++) {
     BInst in{};
     in.n = circuits[i]->n;
@@ -663,7 +895,7 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
     insts.push_back(in);
     cost.push_back(double(in.op_end - in.op_begin) * double(uint64_t(1) << in.n));
     grad_base += uint32_t(circuits[i]->gates.size());
-    b->max_n = std::max(b->max_n, in.n);
+    max_n = std::max(max_n, in.n);
   }
   // Largest first: the persistent workgroups then drain a queue whose tail
   // is made of the cheapest instances.
@@ -671,14 +903,21 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(),
                    [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+  // Every 16th instance of that order: the autotuner's sample.
+  std::vector<uint32_t> sample;
+  for (size_t k = 0; k < order.size(); k += 16) sample.push_back(order[k]);
   check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
   b->free_all();
+  b->max_n = max_n;
   upload(b->ops, pg.ops);
   upload(b->insts, insts);
   upload(b->order, order);
+  upload(b->sample_order, sample);
   upload(b->terms, pg.terms);
   upload(b->mats, pg.mats);
+  upload(b->aux, pg.aux);
   b->count = uint32_t(insts.size());
+  b->sample_count = uint32_t(sample.size());
   b->ngrad = grad_base;
   check(hipMalloc(reinterpret_cast<void **>(&b->energy), std::max<size_t>(1, b->count) * 8),
         "hipMalloc(energy)");
@@ -692,7 +931,7 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
 void Batch::set_config(const BatchConfig &c) {
   Impl *b = impl_.get();
   if (c.threads < 64 || c.threads > uint32_t(kMaxThreads) || c.threads % 64)
-    fail("batch: threads must be a multiple of 64 up to 1024");
+    fail("batch: threads must be a multiple of 64 up to 512");
   if (c.per_cu < 1 || c.per_cu > 4) fail("batch: per_cu must be 1..4");
   if (c.lds_bits > 14 || b->lds_bytes(c) > b->max_lds) fail("batch: LDS cut-off too large");
   b->cfg = c;
@@ -701,26 +940,30 @@ void Batch::set_config(const BatchConfig &c) {
 
 BatchConfig Batch::config() const { return impl_->cfg; }
 
+// Times each candidate twice on the sample (every 16th instance by cost) and
+// keeps the fastest; the full batch is then captured with it.
 BatchConfig Batch::autotune() {
   Impl *b = impl_.get();
   if (!b->count) return b->cfg;
+  check(hipSetDevice(b->device), "hipSetDevice");
   BatchConfig best = b->cfg;
   double best_ms = 1e300;
-  for (uint32_t bits : {10u, 11u, 12u, 13u}) {
+  for (uint32_t bits : {11u, 12u, 13u}) {
     BatchConfig c;
     c.lds_bits = bits;
     if (b->lds_bytes(c) > b->max_lds) continue;
-    for (uint32_t per : {1u, 2u, 4u}) {
+    for (uint32_t per : {1u, 2u}) {
       // All persistent workgroups of a CU must fit its 160 KiB of LDS.
       if (per * b->lds_bytes(c) > (size_t(160) << 10)) continue;
-      for (uint32_t th : {256u, 512u, 1024u}) {
+      for (uint32_t th : {256u, 512u}) {
+        if (per * th > 2048) continue;
         c.per_cu = per;
         c.threads = th;
         set_config(c);
-        b->launch();
-        b->launch();
-        if (b->ms < best_ms) {
-          best_ms = b->ms;
+        b->timed(true);
+        const double ms = b->timed(true);
+        if (ms < best_ms) {
+          best_ms = ms;
           best = c;
         }
       }
@@ -734,7 +977,7 @@ void Batch::run(double *energies, double *grads) {
   Impl *b = impl_.get();
   check(hipSetDevice(b->device), "hipSetDevice");
   if (!b->count) return;
-  b->launch();
+  b->ms = b->timed(false);
   if (energies)
     check(hipMemcpy(energies, b->energy, b->count * sizeof(double), hipMemcpyDeviceToHost),
           "hipMemcpy(energy)");

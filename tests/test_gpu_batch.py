@@ -41,7 +41,7 @@ def test_batch_matches_plain_path_and_oracle():
     circs = [C.to_circuit(n, g) for n, g, _ in inst]
     hams = [C.to_hamiltonian(h) for _, _, h in inst]
     b = qm.Batch().prepare(circs, hams)
-    for cfg in ((256, 1, 12), (1024, 1, 13), (512, 2, 10)):
+    for cfg in ((256, 1, 12), (512, 1, 13), (512, 2, 10)):
         try:
             b.set_config(*cfg)
         except qm.QMetalError:
