@@ -105,6 +105,158 @@ def batched_qaoa(qm, C, count: int, repeats: int, plain_count: int):
     }
 
 
+def oracle_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def suite(qm, C, emit):
+    """Every configuration of the measurement plan, one JSON object each:
+    GHZ n=20, QFT scale sweep (fused vs unfused, alternated), brickwork,
+    HEA adjoint gradients. Correctness is checked against the complex128
+    oracle where it fits (n <= 26) and against invariants above."""
+    import numpy as np
+    from oracle import oracle as O
+    th = oracle_threads()
+
+    def best_of(fn, reps):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    # --- GHZ n=20 ---------------------------------------------------------
+    n = 20
+    g = C.to_circuit(n, C.ghz(n))
+    with qm.Simulator(n) as s:
+        def run_ghz():
+            s.reset()
+            s.run(g)
+            s.synchronize()
+        t = best_of(run_ghz, 5)
+        amps = s.amplitudes().astype(np.complex128)
+        zq = max(abs(s.expectation_pauli(z=1 << q)) for q in range(n))
+        zz = s.expectation_pauli(z=1 | (1 << (n - 1)))
+        shots = s.sample(100000, 7)
+        norm = s.norm()
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    ref = O.run(n, C.ghz(n))
+    t_cpu = time.perf_counter() - t0
+    emit({"config": "ghz_n20", "gpu_s": t, "cpu_oracle_s_1thread": t_cpu,
+          "max_abs_diff_vs_oracle": float(np.abs(amps - ref).max()), "norm": norm,
+          "max_abs_z_q": zq, "z0_z19": zz, "shots": 100000,
+          "distinct_outcomes": sorted(set(shots)), "plan_ops": g.plan_size(True)})
+
+    # --- QFT scale sweep: fused vs unfused, alternated --------------------
+    for n in (26, 30, 32, 33, 34):
+        qft = C.to_circuit(n, C.qft(n))
+        with qm.Simulator(n) as s:
+            s.run(C.to_circuit(n, C.product_state_prep(n, seed=2024)))
+            s.run(qft)
+            s.synchronize()
+            times = {True: [], False: []}
+            for fuse in (True, False, True, False):
+                t0 = time.perf_counter()
+                s.run(qft, fuse=fuse)
+                s.synchronize()
+                times[fuse].append(time.perf_counter() - t0)
+            s.reset()
+            s.run(qft)
+            norm0 = s.norm()
+            z0 = max(abs(s.expectation_pauli(z=1 << q)) for q in range(n))
+            diff = None
+            if n == 26:
+                s.reset()
+                prep = C.product_state_prep(n, seed=2024)
+                s.run(C.to_circuit(n, prep))
+                s.run(qft)
+                got = s.amplitudes().astype(np.complex128)
+                O.set_threads(th)
+                ref = O.run(n, prep + C.qft(n))
+                O.set_threads(1)
+                diff = float(np.abs(got - ref).max())
+        tf, tu = min(times[True]), min(times[False])
+        emit({"config": f"qft_n{n}", "fused_s": tf, "unfused_s": tu, "speedup": tu / tf,
+              "plan_ops_fused": qft.plan_size(True), "plan_ops_unfused": qft.plan_size(False),
+              "hbm_GBps_algorithmic": qft_bytes(n) / tf / 1e9,
+              "pct_hbm_peak": 100 * qft_bytes(n) / tf / 1e9 / HBM_PEAK_GBS,
+              "qft_zero_norm": norm0, "qft_zero_max_abs_z_q": z0,
+              "max_abs_diff_vs_oracle_product_state": diff})
+
+    # --- brickwork depth 10 (Haar U3 per wire, alternating CX layers) -----
+    for n in (26, 32):
+        gates = C.brickwork(n, 10)
+        c = C.to_circuit(n, gates)
+        with qm.Simulator(n) as s:
+            s.run(c)
+            s.synchronize()
+            def run_bw():
+                s.reset()
+                s.run(c)
+                s.synchronize()
+            t = best_of(run_bw, 2)
+            norm = s.norm()
+            diff = None
+            if n == 26:
+                got = s.amplitudes().astype(np.complex128)
+                O.set_threads(th)
+                ref = O.run(n, gates)
+                O.set_threads(1)
+                diff = float(np.abs(got - ref).max())
+        bytes_ = 0.0
+        for op in c.plan():
+            k = op["kind"]
+            bytes_ += (16 if k in ("Dense1q", "Dense1qGroup", "Blocked1q", "DiagonalLayer")
+                       else 8 if k in ("Controlled1q", "Swap") else 4) * float(1 << n)
+        emit({"config": f"brickwork_n{n}_d10", "gates": len(c), "plan_ops": c.plan_size(True),
+              "gpu_s": t, "hbm_GBps_algorithmic": bytes_ / t / 1e9, "norm": norm,
+              "max_abs_diff_vs_oracle": diff})
+
+    # --- HEA depth 3 adjoint, TFIM ring -----------------------------------
+    for n in (20, 26, 32):
+        P = 6 * n
+        rng = C.SplitMix64(0x4EA + n)
+        params = [rng.uniform(0, 2 * math.pi) for _ in range(P)]
+        gates = C.hardware_efficient(n, 3, params=params)
+        terms = C.tfim_ring_hamiltonian(n)
+        c, h = C.to_circuit(n, gates), C.to_hamiltonian(terms)
+        rot = [k for k, (kind, _, _) in enumerate(gates) if kind in (8, 9, 10)]
+        with qm.Simulator(n) as s:
+            s.energy_and_gradient(c, h)
+            t0 = time.perf_counter()
+            e, gr = s.energy_and_gradient(c, h)
+            t = time.perf_counter() - t0
+            gr = np.array(gr)
+            # central differences (h = 1e-3) on a few parameters, on the GPU
+            fd_err = 0.0
+            for j in rot[:: max(1, len(rot) // 4)][:4]:
+                vals = []
+                for d in (1e-3, -1e-3):
+                    g2 = list(gates)
+                    kind, q, p = g2[j]
+                    g2[j] = (kind, q, (p[0] + d,))
+                    s.reset()
+                    s.run(C.to_circuit(n, g2))
+                    vals.append(s.expectation(h))
+                fd_err = max(fd_err, abs((vals[0] - vals[1]) / 2e-3 - gr[j]))
+        rec = {"config": f"hea_d3_adjoint_n{n}", "gates": len(gates), "params": P,
+               "energy": e, "gpu_s": t, "grads_per_s": 1.0 / t,
+               "max_abs_err_vs_central_diff_h1e-3": fd_err}
+        if n == 20:
+            O.set_threads(th)
+            t0 = time.perf_counter()
+            want = O.param_shift(n, gates, terms)
+            rec["cpu_param_shift_s"] = time.perf_counter() - t0
+            e_ref = O.expect_h(O.run(n, gates), terms)
+            O.set_threads(1)
+            rec["max_abs_err_vs_fp64_param_shift"] = float(np.abs(gr - want).max())
+            rec["energy_err_vs_fp64"] = abs(e - e_ref)
+            rec["cpu_threads"] = th
+        emit(rec)
+
+
 def load_traffic(workload: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -129,6 +281,8 @@ def main() -> int:
     ap.add_argument("--batch-repeats", type=int, default=3)
     ap.add_argument("--plain-count", type=int, default=4096,
                     help="instances solved one circuit per launch for the comparison")
+    ap.add_argument("--suite", action="store_true",
+                    help="measure every configuration (one JSON line each) instead of the headline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only run a torch device copy of 2^N complex64 (8 * 2^N bytes read "
                          "and written) three times: the known-size kernel that calibrates the "
@@ -152,6 +306,11 @@ def main() -> int:
     os.environ.setdefault("QMETAL_HIP_RUNTIME", "system")
     import qmetal_amd as qm
     from qmetal_amd import circuits as C
+
+    if args.suite:
+        print(f"# device {qm.device_name()}  suite", file=sys.stderr, flush=True)
+        suite(qm, C, lambda rec: print(json.dumps(rec), flush=True))
+        return 0
 
     n = args.n
     workload = f"qft_n{n}_product_state"
