@@ -5,8 +5,9 @@ events on the simulator's stream and the complex128 CPU oracle timed beside it.
 
     python bench.py [--steps K] [--warmup W] [--n 33] [--cpu-n 24] [--no-cpu]
 
-One step is one qm_run of QFT(n) (n(n+1)/2 gates, 2n-1 fused passes) on the
-resident state. Inputs are synthetic: RY(theta_q), RZ(phi_q) on every wire with
+One step is one qm_run of QFT(n) (n(n+1)/2 gates, a plan of 2n-1 fused ops,
+which the executor merges into about n/4 read-modify-write passes over HBM;
+--no-merge launches one kernel per plan op) on the resident state. Inputs are synthetic: RY(theta_q), RZ(phi_q) on every wire with
 theta, phi ~ U[0, 2 pi) from splitmix64(2024), applied before timing.
 """
 from __future__ import annotations
@@ -276,6 +277,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="also time the unfused plan")
+    ap.add_argument("--no-merge", action="store_true",
+                    help="one kernel launch per plan op (executor-level pass merging off)")
     ap.add_argument("--batch-count", type=int, default=4096,
                     help="batched QAOA instances (0 skips the batched measurement)")
     ap.add_argument("--batch-repeats", type=int, default=3)
@@ -316,9 +319,11 @@ def main() -> int:
     workload = f"qft_n{n}_product_state"
     print(f"# device {qm.device_name()}  workload {workload}", file=sys.stderr, flush=True)
     sim = qm.Simulator(n)
+    sim.set_pass_merging(not args.no_merge)
     sim.run(C.to_circuit(n, C.product_state_prep(n, seed=2024)))
     qft = C.to_circuit(n, C.qft(n))
     plan_ops = qft.plan_size(True)
+    launches = plan_ops if args.no_merge else max(qft.passes()) + 1
     for _ in range(args.warmup):
         sim.run(qft)
     sim.synchronize()
@@ -361,6 +366,19 @@ def main() -> int:
     worst_z = max(abs(sim.expectation_pauli(z=1 << q)) for q in range(0, n, max(1, n // 8)))
     norm0 = sim.norm()
     print(f"# QFT|0>: norm {norm0:.8f}  max|<Z_q>| {worst_z:.2e}", file=sys.stderr, flush=True)
+
+    # The same plan, one launch per plan op, for comparison (one circuit).
+    op_by_op = None
+    if not args.no_merge:
+        sim.set_pass_merging(False)
+        sim.run(qft)
+        sim.synchronize()
+        o0 = time.perf_counter()
+        sim.run(qft)
+        sim.synchronize()
+        op_by_op = time.perf_counter() - o0
+        sim.set_pass_merging(True)
+        print(f"# one launch per plan op: {op_by_op:.3f} s/circuit", file=sys.stderr, flush=True)
 
     unfused = None
     if args.unfused:
@@ -409,7 +427,8 @@ def main() -> int:
         "dtype": "complex64",
         "data": "synthetic",
         "config": {"workload": workload, "n": n, "gates": n * (n + 1) // 2,
-                   "plan_ops": plan_ops, "fused": True},
+                   "plan_ops": plan_ops, "fused": True, "launches": launches,
+                   "pass_merging": not args.no_merge},
         "hbm_GBps_algorithmic": round(alg / s_per / 1e9, 1),
         "pct_hbm_peak": round(100 * alg / s_per / 1e9 / HBM_PEAK_GBS, 1),
         "invariants": {"norm_after_steps": norm_after, "qft_zero_norm": norm0,
@@ -420,6 +439,8 @@ def main() -> int:
                      "traffic": load_traffic(workload)},
         "cpu_baseline": cpu,
     }
+    if op_by_op is not None:
+        out["op_by_op_s_per_circuit"] = round(op_by_op, 4)
     if unfused is not None:
         out["unfused_s_per_circuit"] = round(unfused, 4)
     if batched is not None:

@@ -191,6 +191,20 @@ int qm_set_timing(qm_simulator s, int enable);
 int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
                    size_t cap, size_t *count);
 
+/* Executor-level pass merging below the plan (no counterpart in the
+ * reference, whose executor encodes one dispatch per plan op,
+ * src/metal/simulator.mm:436-590). Consecutive plan ops whose moving wires fit
+ * one thread's register tile (bit 0 plus up to 4 wires) share one kernel and
+ * one read-modify-write pass over the state; phase layers join any pass.
+ * The plan and qm_circuit_plan_size are unchanged; qm_dispatch_count counts
+ * the launches actually made. On by default; qm_set_pass_merging(s, 0)
+ * restores one launch per plan op. A timing record of kind 7 is one merged
+ * pass. qm_plan_passes reports the grouping qm_run would use: pass_of_op
+ * (qm_plan_size entries, may be NULL) gets each op's pass index (UINT32_MAX
+ * for an op that launches nothing), *num_passes the number of launches. */
+int qm_set_pass_merging(qm_simulator s, int enable);
+int qm_plan_passes(qm_plan p, uint32_t *pass_of_op, size_t *num_passes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -169,6 +169,8 @@ _plan_destroy = _decl("qm_plan_destroy", None, c_void_p)
 _plan_nops = _decl("qm_plan_size", c_size_t, c_void_p)
 _plan_get = _decl("qm_plan_get", c_int, c_void_p, c_size_t, POINTER(_PlanOp))
 _plan_arrays = _decl("qm_plan_get_arrays", c_int, c_void_p, c_size_t, _u32p, _f64p, _u64p, _f64p)
+_plan_passes = _decl("qm_plan_passes", c_int, c_void_p, _u32p, POINTER(c_size_t))
+_set_pass_merging = _decl("qm_set_pass_merging", c_int, c_void_p, c_int)
 
 
 def _err() -> str:
@@ -188,6 +190,8 @@ _U3 = 18
 
 PLAN_KINDS = ("Dense1q", "Dense1qGroup", "Blocked1q", "Controlled1q", "DiagonalLayer",
               "Swap", "CCX")
+# Timing records: plan-op kinds, then a merged pass of several plan ops.
+TIMING_KINDS = PLAN_KINDS + ("Pass",)
 
 
 def version() -> tuple[int, int, int]:
@@ -300,6 +304,23 @@ class Circuit:
     def plan_size(self, fuse: bool = True) -> int:
         """GPU passes this circuit lowers to -- what execution actually costs."""
         return int(_plan_size(self._h, 1 if fuse else 0))
+
+    def passes(self, options: FusionOptions | None = None) -> list[int]:
+        """Pass index of every plan op under executor-level pass merging (-1
+        for an op that launches nothing); max + 1 is the number of launches
+        qm_run makes for this circuit."""
+        opts = options if options is not None else FusionOptions.default()
+        p = _plan_build(self._h, byref(opts))
+        if not p:
+            raise QMetalError(_err())
+        try:
+            n = int(_plan_nops(p))
+            out = (c_uint32 * max(n, 1))()
+            count = c_size_t()
+            _check(_plan_passes(p, out, byref(count)))
+            return [-1 if out[k] == 0xFFFFFFFF else int(out[k]) for k in range(n)]
+        finally:
+            _plan_destroy(p)
 
     def plan(self, options: FusionOptions | None = None) -> list[dict]:
         """The lowered plan as a list of dicts (kind name, wires, matrices,
@@ -420,6 +441,12 @@ class Simulator:
             out[key] = out.get(key, 0) + 1
         return dict(sorted(out.items(), key=lambda kv: -kv[1]))
 
+    def set_pass_merging(self, enable: bool = True):
+        """Merge consecutive plan ops into shared HBM passes (default on);
+        off gives one kernel launch per plan op."""
+        _check(_set_pass_merging(self._h, 1 if enable else 0))
+        return self
+
     def set_timing(self, enable: bool = True):
         """Record device time of every plan-op kernel (events on the
         simulator's stream)."""
@@ -433,7 +460,7 @@ class Simulator:
         by = (c_double * cap)()
         count = c_size_t()
         _check(_read_timing(self._h, kinds, ms, by, cap, byref(count)))
-        return [(PLAN_KINDS[kinds[k]], ms[k], by[k]) for k in range(min(count.value, cap))]
+        return [(TIMING_KINDS[kinds[k]], ms[k], by[k]) for k in range(min(count.value, cap))]
 
     def energy_and_gradient(self, circuit: Circuit, hamiltonian: Hamiltonian):
         """Adjoint gradient: one entry per gate, zero where unparameterised."""

@@ -22,7 +22,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("QMETAL_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "simulator.hip", "batch.hip"]
-CXX_SOURCES = ["gates.cpp", "plan.cpp", "capi.cpp"]
+CXX_SOURCES = ["gates.cpp", "plan.cpp", "schedule.cpp", "capi.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -11,6 +11,7 @@
 
 #include "batch.h"
 #include "core.h"
+#include "schedule.h"
 #include "simulator.h"
 
 using namespace qmamd;
@@ -490,6 +491,30 @@ int qm_plan_get_arrays(qm_plan p, size_t k, uint32_t *gq, double *gm,
       if (angles) angles[i] = op.angles[i];
     }
   });
+}
+
+int qm_plan_passes(qm_plan p, uint32_t *pass_of_op, size_t *num_passes) {
+  if (!p || !num_passes) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] {
+    const std::vector<Pass> passes = schedule_passes(p->p);
+    if (pass_of_op)
+      for (size_t k = 0; k < p->p.ops.size(); k++) pass_of_op[k] = UINT32_MAX;
+    for (size_t i = 0; i < passes.size(); i++)
+      for (uint32_t k : passes[i].ops)
+        if (pass_of_op) pass_of_op[k] = uint32_t(i);
+    *num_passes = passes.size();
+  });
+}
+
+int qm_set_pass_merging(qm_simulator s, int enable) {
+  if (!s) {
+    set_error("null simulator");
+    return -1;
+  }
+  return guard([&] { s->s->set_pass_merging(enable != 0); });
 }
 
 }  // extern "C"

@@ -89,12 +89,47 @@ struct BlockedArgs {
   const Slot *gates;  // lbit = physical wire (< bits)
 };
 
+// One op of a merged pass (several plan ops applied to the 2^NL amplitudes a
+// thread holds between one load and one store of them).
+//   type 0, a 2x2 slot: applies to tile-local bit `lbit` where the tile-local
+//     bits `cmask` are 1 and the thread's base index has every bit of `gmask`.
+//   type 1, a phase layer: phase in turns =
+//       constant + sum_k beta_k x_k + sum_{k<j} gamma_kj x_k x_j
+//     over tile-local bits after terms[term0 .. term0 + nterms) are folded in
+//     from the base index (the first `nuniform` read only bits that are the
+//     same across the workgroup); applied to local combinations containing
+//     `cmask`, in threads whose base has `gmask` (the others keep phase 0).
+constexpr int kPassMaxNL = 5;
+struct PassOp {
+  uint32_t type;
+  uint32_t lbit;
+  uint32_t cmask;
+  uint32_t term0;
+  uint32_t nterms;
+  uint32_t nuniform;
+  uint64_t gmask;
+  float m[8];
+  double constant;
+  double beta0[kPassMaxNL];
+  double gamma[kPassMaxNL][kPassMaxNL];
+};
+
+struct PassArgs {
+  float2 *a;
+  Layout lay;
+  const PassOp *ops;
+  const DiagTerm *terms;
+  uint32_t nops;
+};
+
 void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim);
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim);
 void launch_gather(hipStream_t s, int nl, const GatherArgs &g);
 void launch_diag(hipStream_t s, int nl, const DiagArgs &d);
 void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d);
 void launch_blocked(hipStream_t s, const BlockedArgs &b);
+// nl in {4, 5}.
+void launch_pass(hipStream_t s, int nl, const PassArgs &p);
 
 // FP64 reductions. Each writes per-block partials into `partials` (capacity
 // kRedBlocks doubles or double2) and then reduces them in a fixed order into

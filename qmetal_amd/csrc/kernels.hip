@@ -70,26 +70,32 @@ __device__ __forceinline__ void rot(const float *m, float2 &x, float2 &y) {
 }
 
 template <int NL, int LB>
-__device__ __forceinline__ void apply_slot(float2 (&x)[1 << NL], const Slot &s) {
-  const uint32_t cm = s.cmask;
+__device__ __forceinline__ void apply_slot(float2 (&x)[1 << NL], uint32_t cm, const float *m) {
 #pragma unroll
   for (int l = 0; l < (1 << NL); l++) {
     if (l & (1 << LB)) continue;
     if ((uint32_t(l) & cm) != cm) continue;
-    rot(s.m, x[l], x[l | (1 << LB)]);
+    rot(m, x[l], x[l | (1 << LB)]);
+  }
+}
+
+// 2x2 `m` on tile-local bit `lbit` wherever the tile-local bits `cm` are 1.
+template <int NL>
+__device__ __forceinline__ void apply_any(float2 (&x)[1 << NL], uint32_t lbit, uint32_t cm,
+                                          const float *m) {
+  switch (lbit) {
+    case 0: apply_slot<NL, 0>(x, cm, m); break;
+    case 1: if constexpr (NL > 1) apply_slot<NL, 1>(x, cm, m); break;
+    case 2: if constexpr (NL > 2) apply_slot<NL, 2>(x, cm, m); break;
+    case 3: if constexpr (NL > 3) apply_slot<NL, 3>(x, cm, m); break;
+    case 4: if constexpr (NL > 4) apply_slot<NL, 4>(x, cm, m); break;
+    default: break;
   }
 }
 
 template <int NL>
 __device__ __forceinline__ void apply_any(float2 (&x)[1 << NL], const Slot &s) {
-  switch (s.lbit) {
-    case 0: apply_slot<NL, 0>(x, s); break;
-    case 1: if constexpr (NL > 1) apply_slot<NL, 1>(x, s); break;
-    case 2: if constexpr (NL > 2) apply_slot<NL, 2>(x, s); break;
-    case 3: if constexpr (NL > 3) apply_slot<NL, 3>(x, s); break;
-    case 4: if constexpr (NL > 4) apply_slot<NL, 4>(x, s); break;
-    default: break;
-  }
+  apply_any<NL>(x, s.lbit, s.cmask, s.m);
 }
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -248,6 +254,105 @@ __global__ __launch_bounds__(kT) void k_diag_generic(DiagGenericArgs d) {
     float s, c;
     turn_sincos(ph, s, c);
     d.a[i] = cmul(d.a[i], make_float2(c, s));
+  }
+}
+
+// Merged pass: the thread's 2^NL amplitudes are loaded once, every op of the
+// pass is applied to them in plan order, and they are stored once. Ops are
+// workgroup-uniform; their descriptors come from device memory.
+template <int NL, bool NT>
+__global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
+  uint64_t off[1 << (NL - 1)];
+  offsets<NL>(p.lay, off);
+  const PassOp *__restrict__ ops = p.ops;
+  const DiagTerm *__restrict__ terms = p.terms;
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t0 = uint64_t(blockIdx.x) * kT; t0 < p.lay.nthreads; t0 += stride) {
+    const uint64_t ub = deposit(t0, p.lay);
+    const uint64_t t = t0 + threadIdx.x;
+    const bool active = t < p.lay.nthreads;
+    const uint64_t base = active ? deposit(t, p.lay) : ub;
+    float2 x[1 << NL];
+    if (active) {
+      float4 v[1 << (NL - 1)];
+#pragma unroll
+      for (int r = 0; r < (1 << (NL - 1)); r++) v[r] = ld4<NT>(p.a + (base | off[r]));
+#pragma unroll
+      for (int r = 0; r < (1 << (NL - 1)); r++) {
+        x[2 * r] = make_float2(v[r].x, v[r].y);
+        x[2 * r + 1] = make_float2(v[r].z, v[r].w);
+      }
+    }
+    for (uint32_t o = 0; o < p.nops; o++) {
+      const PassOp &op = ops[o];
+      const bool on = active && (base & op.gmask) == op.gmask;
+      if (op.type == 0) {
+        if (on) apply_any<NL>(x, op.lbit, op.cmask, op.m);
+        continue;
+      }
+      // Phase layer. Workgroup-uniform terms: one term per lane, summed over
+      // the wave (every lane takes part, active or not).
+      double phi = 0.0;
+      double beta[NL];
+#pragma unroll
+      for (int j = 0; j < NL; j++) beta[j] = 0.0;
+      const DiagTerm *tl = terms + op.term0;
+      for (uint32_t k = lane; k < op.nuniform; k += 64) {
+        const DiagTerm tm = tl[k];
+        if ((ub & tm.mask) != tm.mask) continue;
+        if (tm.kind == 2) {
+#pragma unroll
+          for (int j = 0; j < NL; j++)
+            if (tm.k == uint32_t(j)) beta[j] += tm.ang;
+        } else {
+          phi += tm.ang;
+        }
+      }
+      if (op.nuniform) {
+        phi = wave_allsum(phi);
+#pragma unroll
+        for (int j = 0; j < NL; j++) beta[j] = wave_allsum(beta[j]);
+      }
+      if (!on) continue;
+      for (uint32_t k = op.nuniform; k < op.nterms; k++) {
+        const DiagTerm tm = tl[k];
+        const bool hit = (base & tm.mask) == tm.mask;
+        if (tm.kind == 2) {
+#pragma unroll
+          for (int j = 0; j < NL; j++)
+            if (hit && tm.k == uint32_t(j)) beta[j] += tm.ang;
+        } else if (hit) {
+          phi += tm.ang;
+        }
+      }
+      phi += op.constant;
+#pragma unroll
+      for (int j = 0; j < NL; j++) beta[j] += op.beta0[j];
+      const uint32_t cm = op.cmask;
+#pragma unroll
+      for (int l = 0; l < (1 << NL); l++) {
+        if ((uint32_t(l) & cm) != cm) continue;
+        double ph = phi;
+#pragma unroll
+        for (int h = 0; h < NL; h++) {
+          if (!((l >> h) & 1)) continue;
+          ph += beta[h];
+#pragma unroll
+          for (int k = 0; k < h; k++)
+            if ((l >> k) & 1) ph += op.gamma[k][h];
+        }
+        float s, c;
+        turn_sincos(ph, s, c);
+        x[l] = cmul(x[l], make_float2(c, s));
+      }
+    }
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < (1 << (NL - 1)); r++)
+        st4<NT>(p.a + (base | off[r]),
+                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
+    }
   }
 }
 
@@ -675,6 +780,21 @@ void launch_diag_t(hipStream_t s, int nl, const DiagArgs &d) {
 void launch_diag(hipStream_t s, int nl, const DiagArgs &d) {
   if (d.lay.streaming) launch_diag_t<true>(s, nl, d);
   else launch_diag_t<false>(s, nl, d);
+}
+
+template <bool NT>
+void launch_pass_t(hipStream_t s, int nl, const PassArgs &p) {
+  const unsigned grid = grid_for(p.lay.nthreads);
+  switch (nl) {
+    case 4: k_pass<4, NT><<<grid, kT, 0, s>>>(p); break;
+    case 5: k_pass<5, NT><<<grid, kT, 0, s>>>(p); break;
+    default: break;
+  }
+}
+
+void launch_pass(hipStream_t s, int nl, const PassArgs &p) {
+  if (p.lay.streaming) launch_pass_t<true>(s, nl, p);
+  else launch_pass_t<false>(s, nl, p);
 }
 
 void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d) {

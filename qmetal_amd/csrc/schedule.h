@@ -1,0 +1,55 @@
+// schedule.h -- executor-level scheduling below the Plan: consecutive plan
+// ops are merged into one read-modify-write pass over HBM when the wires they
+// move fit in one thread's register tile.
+//
+// A merged pass holds 2^NL amplitudes per thread: index bit 0 (inside one
+// 16-byte load) plus up to kPassMaxReg register wires. Every dense 2x2 of the
+// pass acts on one of those wires; controls on any other bit become
+// predicates on the thread's base index; diagonal layers are elementwise and
+// need no register wire at all. Bits that every op of the pass requires to be
+// 1 (controls, common diagonal mask bits, at or above bit kPassFixedMin) are
+// fixed: the pass never reads amplitudes with such a bit clear.
+//
+// The Plan itself (and qm_circuit_plan_size) is unchanged; only the number of
+// kernel launches and HBM passes drops. Pure C++ so the grouping is tested on
+// the CPU.
+
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "core.h"
+
+namespace qmamd {
+
+constexpr uint32_t kPassMaxReg = 4;     // register wires beyond bit 0 (NL <= 5)
+constexpr uint32_t kPassFixedMin = 4;   // 2^4 complex64 = one 128-byte line
+constexpr uint32_t kPassMinQubits = 5;  // below this a plan runs op by op
+constexpr uint32_t kPassMaxOps = 16;    // plan ops per merged pass
+
+struct Pass {
+  bool merged = false;         // false: ops[0] alone through its own kernel
+  std::vector<uint32_t> ops;   // plan-op indices, in plan order
+  std::vector<uint32_t> reg;   // register wires (merged), first use order
+  uint64_t fixed = 0;          // fixed-one bits (merged)
+};
+
+// Whether plan op `op` can join a merged pass.
+bool pass_mergeable(const PlanOp &op);
+// Wires (other than 0) that must be register-local for `op`.
+std::vector<uint32_t> pass_wires(const PlanOp &op);
+// Bits `op` alone requires to be 1 (controls and common diagonal mask bits at
+// or above kPassFixedMin; a diagonal with a non-zero global phase has none).
+uint64_t pass_fixed(const PlanOp &op);
+// HBM bytes of one pass over a 2^n state with fixed bits `fixed`.
+double pass_bytes(uint32_t n, uint64_t fixed);
+// HBM bytes `op` moves when it runs alone through its own kernel.
+double op_alone_bytes(uint32_t n, const PlanOp &op);
+
+// Greedy grouping in plan order: an op joins the open pass when its wires
+// still fit and the merged pass moves no more bytes than the open pass and
+// the op would separately.
+std::vector<Pass> schedule_passes(const Plan &p);
+
+}  // namespace qmamd

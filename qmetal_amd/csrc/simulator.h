@@ -2,7 +2,8 @@
 //
 // One Simulator owns one HBM allocation of 2^n complex64 amplitudes on the
 // current HIP device and one stream of default priority. Plan ops are
-// enqueued asynchronously, one kernel per op; reads, reductions and samples
+// enqueued asynchronously, consecutive ops sharing one kernel (one HBM pass)
+// where their wires fit a register tile; reads, reductions and samples
 // synchronise. No HIP types appear here so the C ABI compiles as plain C++.
 
 #pragma once
@@ -22,7 +23,9 @@ namespace qmamd {
 // size.
 constexpr uint32_t kMaxQubits = 34;
 
-// Device time of one plan-op kernel and its algorithmic bytes.
+// Device time of one launch and its algorithmic bytes: a plan op (kind
+// 0..6) or a merged pass (kind 7, bytes of one pass over the amplitudes it
+// reads).
 struct OpTime {
   uint32_t kind;
   double ms;
@@ -39,6 +42,10 @@ class Simulator {
   void reset();
   void run(const Circuit &c, const FuseOpts &o);
   void run_plan(const Plan &p);
+  // Merging of consecutive plan ops into shared HBM passes (on by default).
+  // Off: one kernel launch per plan op.
+  void set_pass_merging(bool on);
+  bool pass_merging() const;
   void apply_gate(const Gate &g);
   void synchronize();
 

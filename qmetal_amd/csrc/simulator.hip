@@ -14,6 +14,7 @@
 #include <tuple>
 
 #include "kernels.h"
+#include "schedule.h"
 #include "simulator.h"
 
 namespace qmamd {
@@ -43,6 +44,9 @@ constexpr uint32_t kLaneBits = 8;
 constexpr int kLocalBits = 4;
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// Timing record kind of a merged pass (plan-op kinds are 0..6).
+constexpr uint32_t kTimingPass = 7;
 
 int current_device() {
   int d = 0;
@@ -280,6 +284,216 @@ struct Simulator::Impl {
     return op.masks.size() * std::max(sizeof(DiagTerm), sizeof(uint64_t) + sizeof(double));
   }
 
+  // Positions of a workgroup's thread-index bits under layout L: the lowest
+  // positions the layout leaves free. Every other bit of a thread's base index
+  // is the same across the workgroup for one iteration of the grid-stride loop.
+  uint64_t lane_bits(const Layout &L) const {
+    uint64_t used = L.fixed | 1ull, lane = 0;
+    for (uint32_t k = 0; k < L.nins; k++) used |= 1ull << L.ins[k];
+    uint32_t got = 0;
+    for (uint32_t p = 0; p < n && got < kLaneBits; p++)
+      if (!((used >> p) & 1)) {
+        lane |= 1ull << p;
+        got++;
+      }
+    return lane;
+  }
+
+  // Phase-layer terms relative to a layout: weight-1 and weight-2 masks are
+  // folded into const + sum beta_k x_k + sum gamma_kj x_k x_j over the
+  // tile-local bits (`local_of`), bits in `fixed` count as 1, and the terms
+  // that read base-index bits are written to `dt`, workgroup-uniform ones
+  // first. Returns the number of terms written; angles in turns.
+  template <int D, typename LocalOf>
+  size_t fold_terms(const PlanOp &op, uint64_t fixed, uint64_t lane, LocalOf local_of,
+                    double &constant, double (&beta0)[D], double (&gamma)[D][D],
+                    DiagTerm *dt, uint32_t &nuniform) const {
+    double c = op.global;
+    std::map<std::tuple<uint32_t, uint64_t, uint32_t>, double> terms;
+    for (size_t t = 0; t < op.masks.size(); t++) {
+      const uint64_t m = op.masks[t] & ~fixed;
+      const double a = op.angles[t];
+      if (m == 0) {
+        c += a;
+        continue;
+      }
+      const uint32_t p0 = uint32_t(__builtin_ctzll(m));
+      const int l0 = local_of(p0);
+      if (__builtin_popcountll(m) == 1) {
+        if (l0 >= 0) beta0[l0] += a;
+        else terms[{0u, m, 0u}] += a;
+        continue;
+      }
+      const uint32_t p1 = 63u - uint32_t(__builtin_clzll(m));
+      const int l1 = local_of(p1);
+      if (l0 >= 0 && l1 >= 0) gamma[std::min(l0, l1)][std::max(l0, l1)] += a;
+      else if (l0 >= 0) terms[{2u, 1ull << p1, uint32_t(l0)}] += a;
+      else if (l1 >= 0) terms[{2u, 1ull << p0, uint32_t(l1)}] += a;
+      else terms[{1u, m, 0u}] += a;
+    }
+    constant = turns(c);
+    for (int k = 0; k < D; k++) {
+      beta0[k] = turns(beta0[k]);
+      for (int j = 0; j < D; j++) gamma[k][j] = turns(gamma[k][j]);
+    }
+    size_t k = 0;
+    for (int pass = 0; pass < 2; pass++) {
+      for (const auto &kv : terms) {
+        const bool uniform = (std::get<1>(kv.first) & lane) == 0;
+        if (uniform != (pass == 0)) continue;
+        dt[k].kind = std::get<0>(kv.first);
+        dt[k].mask = std::get<1>(kv.first);
+        dt[k].k = std::get<2>(kv.first);
+        dt[k].ang = turns(kv.second);
+        k++;
+      }
+      if (pass == 0) nuniform = uint32_t(k);
+    }
+    return k;
+  }
+
+  // --- merged passes ---------------------------------------------------------
+
+  struct PassLaunch {
+    int nl = 0;
+    PassArgs a{};
+    double bytes = 0.0;
+  };
+
+  static size_t pass_slots(const PlanOp &op) {
+    switch (op.kind) {
+      case OpKind::Group: return op.wires.size();
+      case OpKind::Swap: return 3;
+      default: return 1;
+    }
+  }
+
+  size_t pass_param_bytes(const Plan &p, const Pass &ps) const {
+    size_t b = 0;
+    for (uint32_t k : ps.ops) {
+      b += pass_slots(p.ops[k]) * sizeof(PassOp);
+      if (p.ops[k].kind == OpKind::Diagonal) b += p.ops[k].masks.size() * sizeof(DiagTerm);
+    }
+    return (b + 255) & ~size_t(255);
+  }
+
+  // Lays out a merged pass: register wires first, fixed bits, fill bits at
+  // the top; then one PassOp per 2x2 slot or phase layer, in plan order.
+  void prep_pass(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
+    LayoutBuilder lb{n};
+    lb.fixed = ps.fixed;
+    for (uint32_t q : ps.reg) lb.add(q);
+    const int want = lb.reg.size() + 1 <= 4 ? 4 : 5;
+    pl.a.lay = lb.finish(want, &pl.nl);
+    if (pl.nl != want) fail("internal: pass width");
+    const uint64_t lane = lane_bits(pl.a.lay);
+    size_t nops = 0;
+    for (uint32_t k : ps.ops) nops += pass_slots(p.ops[k]);
+    PassOp *po = reinterpret_cast<PassOp *>(staging + off);
+    DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off + nops * sizeof(PassOp));
+    size_t i = 0, nt = 0;
+    auto local = [&](uint32_t q) {
+      const int l = lb.local_of(q);
+      if (l < 0) fail("internal: pass wire not local");
+      return uint32_t(l);
+    };
+    // A control: nothing if fixed (always 1 here), a tile-local bit, or a
+    // bit of the thread's base index.
+    auto control = [&](uint32_t c, PassOp &o) {
+      if ((lb.fixed >> c) & 1) return;
+      const int l = lb.local_of(c);
+      if (l >= 0) o.cmask |= 1u << l;
+      else o.gmask |= 1ull << c;
+    };
+    auto slot = [&](uint32_t target, const float *m8) -> PassOp & {
+      PassOp &o = po[i++];
+      o = PassOp{};
+      o.type = 0;
+      o.lbit = local(target);
+      std::memcpy(o.m, m8, sizeof(o.m));
+      return o;
+    };
+    for (uint32_t k : ps.ops) {
+      const PlanOp &op = p.ops[k];
+      float m8[8];
+      switch (op.kind) {
+        case OpKind::Dense1q:
+          narrow(op.m, m8);
+          slot(op.q[0], m8);
+          break;
+        case OpKind::Group:
+          for (size_t w = 0; w < op.wires.size(); w++) {
+            narrow(&op.mats[4 * w], m8);
+            slot(op.wires[w], m8);
+          }
+          break;
+        case OpKind::Controlled: {
+          narrow(op.m, m8);
+          PassOp &o = slot(op.q[1], m8);
+          control(op.q[0], o);
+          break;
+        }
+        case OpKind::Swap: {
+          const uint32_t a = op.q[0], b = op.q[1];
+          control(a, slot(b, kX));
+          control(b, slot(a, kX));
+          control(a, slot(b, kX));
+          break;
+        }
+        case OpKind::CCX: {
+          PassOp &o = slot(op.q[2], kX);
+          control(op.q[0], o);
+          control(op.q[1], o);
+          break;
+        }
+        case OpKind::Diagonal: {
+          PassOp &o = po[i++];
+          o = PassOp{};
+          o.type = 1;
+          // With no global phase an amplitude lacking a bit common to every
+          // mask keeps phase 0: only combinations holding those bits change.
+          uint64_t common = ~0ull;
+          for (uint64_t m : op.masks) common &= m;
+          if (turns(op.global) == 0.0) {
+            for (uint32_t q = 0; q < 64; q++)
+              if (((common & ~lb.fixed) >> q) & 1) control(q, o);
+          }
+          o.term0 = uint32_t(nt);
+          o.nterms = uint32_t(fold_terms(
+              op, lb.fixed, lane, [&](uint32_t q) { return lb.local_of(q); }, o.constant,
+              o.beta0, o.gamma, dt + nt, o.nuniform));
+          nt += o.nterms;
+          break;
+        }
+        case OpKind::Blocked:
+          fail("internal: blocked op in a merged pass");
+      }
+    }
+    pl.a.a = state;
+    pl.a.ops = reinterpret_cast<const PassOp *>(arena + off);
+    pl.a.terms = reinterpret_cast<const DiagTerm *>(arena + off + nops * sizeof(PassOp));
+    pl.a.nops = uint32_t(nops);
+    pl.bytes = pass_bytes(n, ps.fixed);
+  }
+
+  void pass_launch(const PassLaunch &pl) {
+    hipEvent_t a = nullptr;
+    if (timing) {
+      a = take_event();
+      check(hipEventRecord(a, stream), "hipEventRecord");
+    }
+    launch_pass(stream, pl.nl, pl.a);
+    launched();
+    dispatches++;
+    if (timing) {
+      hipEvent_t b = take_event();
+      check(hipEventRecord(b, stream), "hipEventRecord");
+      timings.push_back(OpTiming{kTimingPass, pl.bytes, a, b});
+    }
+  }
+
+  bool merge = true;
+
   // Prepared launch of a parameterised op whose parameters sit in the arena.
   struct Pending {
     int type = 0;  // 1 diagonal, 2 generic diagonal, 3 blocked
@@ -327,65 +541,11 @@ struct Simulator::Impl {
     d.a = state;
     d.lay = lb.finish(kLocalBits, &pl.nl);
     if (pl.nl > 4) fail("internal: diagonal width");
-    double constant = op.global;
-    // (kind, mask, k) -> angle, merged.
-    std::map<std::tuple<uint32_t, uint64_t, uint32_t>, double> terms;
-    for (size_t t = 0; t < T; t++) {
-      const uint64_t m = op.masks[t] & ~lb.fixed;
-      const double a = op.angles[t];
-      if (m == 0) {
-        constant += a;
-        continue;
-      }
-      const uint32_t p0 = uint32_t(__builtin_ctzll(m));
-      const int l0 = lb.local_of(p0);
-      if (__builtin_popcountll(m) == 1) {
-        if (l0 >= 0) d.beta0[l0] += a;
-        else terms[{0u, m, 0u}] += a;
-        continue;
-      }
-      const uint32_t p1 = 63u - uint32_t(__builtin_clzll(m));
-      const int l1 = lb.local_of(p1);
-      if (l0 >= 0 && l1 >= 0) d.gamma[std::min(l0, l1)][std::max(l0, l1)] += a;
-      else if (l0 >= 0) terms[{2u, 1ull << p1, uint32_t(l0)}] += a;
-      else if (l1 >= 0) terms[{2u, 1ull << p0, uint32_t(l1)}] += a;
-      else terms[{1u, m, 0u}] += a;
-    }
-    d.constant = turns(constant);
-    for (auto &b : d.beta0) b = turns(b);
-    for (auto &row : d.gamma)
-      for (auto &g : row) g = turns(g);
-    // Positions of the workgroup's thread-index bits: the lowest positions
-    // the layout leaves free.
-    uint64_t lane = 0;
-    {
-      uint64_t used = d.lay.fixed | 1ull;
-      for (uint32_t k = 0; k < d.lay.nins; k++) used |= 1ull << d.lay.ins[k];
-      uint32_t got = 0;
-      for (uint32_t p = 0; p < n && got < kLaneBits; p++)
-        if (!((used >> p) & 1)) {
-          lane |= 1ull << p;
-          got++;
-        }
-    }
-    // Terms that do not read a thread-index bit are uniform across the
-    // workgroup and go first; the kernel evaluates them with scalar branches.
-    DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off);
-    size_t k = 0;
-    for (int pass = 0; pass < 2; pass++) {
-      for (const auto &kv : terms) {
-        const bool uniform = (std::get<1>(kv.first) & lane) == 0;
-        if (uniform != (pass == 0)) continue;
-        dt[k].kind = std::get<0>(kv.first);
-        dt[k].mask = std::get<1>(kv.first);
-        dt[k].k = std::get<2>(kv.first);
-        dt[k].ang = turns(kv.second);
-        k++;
-      }
-      if (pass == 0) d.nuniform = uint32_t(k);
-    }
+    d.nterms = uint32_t(fold_terms(
+        op, lb.fixed, lane_bits(d.lay), [&](uint32_t q) { return lb.local_of(q); },
+        d.constant, d.beta0, d.gamma, reinterpret_cast<DiagTerm *>(staging + off),
+        d.nuniform));
     d.terms = reinterpret_cast<const DiagTerm *>(arena + off);
-    d.nterms = uint32_t(k);
     pl.type = 1;
   }
 
@@ -688,19 +848,37 @@ void Simulator::run_plan(const Plan &p) {
   Impl *s = impl_.get();
   if (p.n != s->n) fail("plan/simulator size mismatch");
   s->use();
-  // Every parameterised op writes its region of one staging block, which is
-  // uploaded once, ahead of the plan's kernels on the same stream.
-  std::vector<size_t> offs(p.ops.size());
-  size_t total = 0;
-  for (size_t k = 0; k < p.ops.size(); k++) {
-    offs[k] = total;
-    total += s->param_bytes(p.ops[k]);
+  // Executor-level grouping below the plan: consecutive ops that fit one
+  // register tile share one HBM pass (or each op runs alone).
+  std::vector<Pass> passes;
+  if (s->merge) {
+    passes = schedule_passes(p);
+  } else {
+    for (uint32_t k = 0; k < p.ops.size(); k++) {
+      Pass ps;
+      ps.ops.push_back(k);
+      passes.push_back(std::move(ps));
+    }
   }
-  std::vector<Impl::Pending> pend(p.ops.size());
+  // Every parameterised op or pass writes its region of one staging block,
+  // which is uploaded once, ahead of the plan's kernels on the same stream.
+  std::vector<size_t> offs(passes.size());
+  size_t total = 0;
+  for (size_t k = 0; k < passes.size(); k++) {
+    offs[k] = total;
+    total += passes[k].merged ? s->pass_param_bytes(p, passes[k])
+                              : s->param_bytes(p.ops[passes[k].ops[0]]);
+  }
+  std::vector<Impl::Pending> pend(passes.size());
+  std::vector<Impl::PassLaunch> merged(passes.size());
   if (total) {
     s->stage(total);
-    for (size_t k = 0; k < p.ops.size(); k++) {
-      const PlanOp &op = p.ops[k];
+    for (size_t k = 0; k < passes.size(); k++) {
+      if (passes[k].merged) {
+        s->prep_pass(p, passes[k], offs[k], merged[k]);
+        continue;
+      }
+      const PlanOp &op = p.ops[passes[k].ops[0]];
       if (op.kind == OpKind::Diagonal && !op.masks.empty())
         s->prep_diagonal(op, offs[k], pend[k]);
       else if (op.kind == OpKind::Blocked)
@@ -708,8 +886,14 @@ void Simulator::run_plan(const Plan &p) {
     }
     s->upload(total);
   }
-  for (size_t k = 0; k < p.ops.size(); k++) s->op(p.ops[k], &pend[k]);
+  for (size_t k = 0; k < passes.size(); k++) {
+    if (passes[k].merged) s->pass_launch(merged[k]);
+    else s->op(p.ops[passes[k].ops[0]], &pend[k]);
+  }
 }
+
+void Simulator::set_pass_merging(bool on) { impl_->merge = on; }
+bool Simulator::pass_merging() const { return impl_->merge; }
 
 void Simulator::set_timing(bool on) { impl_->timing = on; }
 

@@ -222,6 +222,7 @@ def test_blocked_plans_at_width():
 
 def test_dispatch_count():
     with qm.Simulator(8) as s:
+        s.set_pass_merging(False)  # one launch per plan op
         before = s.dispatch_count
         assert before == 1  # the constructor's reset
         c = C.to_circuit(8, C.ghz(8))

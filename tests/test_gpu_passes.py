@@ -1,0 +1,152 @@
+"""Merged HBM passes on the GPU: several plan ops applied between one load and
+one store of each amplitude. Checked against the complex128 oracle and against
+the same plan run one launch per op; launch accounting follows the grouping.
+
+Tolerances: 1e-5 max amplitude difference after one gate (complex64 floor),
+1e-4 for deep circuits; merged vs op-by-op 2e-6 (same FP32 arithmetic, phases
+summed in another FP64 order).
+"""
+import numpy as np
+import pytest
+
+import qmetal_amd as qm
+from qmetal_amd import FusionOptions
+from qmetal_amd import circuits as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL1 = 1e-5
+TOL_DEEP = 1e-4
+TOL_SAME = 2e-6
+
+ONE = (C.X, C.Y, C.Z, C.H, C.S, C.SDG, C.T, C.TDG)
+ROT = (C.RX, C.RY, C.RZ, C.P)
+
+
+def mixed_stream(n, count, seed):
+    """Every gate kind on wires drawn uniformly (splitmix64), so controls and
+    targets land on register, thread-index, fixed and fill bits alike."""
+    r = C.SplitMix64(seed)
+    gates = [(C.H, (q,), ()) for q in range(n)]
+
+    def wires(k):
+        out = []
+        while len(out) < k:
+            q = r.next_u64() % n
+            if q not in out:
+                out.append(q)
+        return tuple(out)
+
+    for _ in range(count):
+        u = r.next_u64() % 16
+        if u < 3:
+            gates.append((ONE[r.next_u64() % len(ONE)], wires(1), ()))
+        elif u < 6:
+            gates.append((ROT[r.next_u64() % len(ROT)], wires(1), (r.uniform(-3.2, 3.2),)))
+        elif u < 7:
+            gates.append((C.U3, wires(1), tuple(r.uniform(-3.2, 3.2) for _ in range(3))))
+        elif u < 10:
+            gates.append(((C.CX, C.CY, C.CZ, C.SWAP)[r.next_u64() % 4], wires(2), ()))
+        elif u < 13:
+            gates.append((C.CP, wires(2), (r.uniform(-3.2, 3.2),)))
+        elif u < 14:
+            gates.append((C.CCX, wires(3), ()))
+        else:  # a ZZ coupling, recognised as a diagonal run
+            a, b = wires(2)
+            gates += C.zz(a, b, r.uniform(-3.2, 3.2))
+    return gates
+
+
+def run(n, gates, merge=True, options=None, prep=None):
+    with qm.Simulator(n) as s:
+        s.set_pass_merging(merge)
+        if prep:
+            s.run(C.to_circuit(n, prep))
+        before = s.dispatch_count
+        s.run(C.to_circuit(n, gates), options=options)
+        s.synchronize()
+        launches = s.dispatch_count - before
+        return s.amplitudes().astype(np.complex128), s.norm(), launches
+
+
+def test_mixed_streams_against_oracle():
+    for n in (5, 6, 8, 9, 11, 13):
+        for seed in range(6):
+            gates = mixed_stream(n, 40 + 10 * seed, 1000 * n + seed)
+            want = O.run(n, gates)
+            got, norm, _ = run(n, gates)
+            ref, _, _ = run(n, gates, merge=False)
+            d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
+            assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP, (n, seed, d)
+            assert dm <= TOL_SAME, (n, seed, dm)
+
+
+@pytest.mark.parametrize("name,n", [
+    ("qft", 14), ("qft", 20), ("ghz", 20), ("qaoa", 16), ("tfim", 14), ("grover", 12),
+    ("hea", 14), ("brickwork", 16), ("random", 12)])
+def test_configurations_merged_vs_op_by_op(name, n):
+    gates = {
+        "qft": lambda: C.qft(n), "ghz": lambda: C.ghz(n), "qaoa": lambda: C.qaoa_ring(n, 6),
+        "tfim": lambda: C.tfim_trotter(n, 10), "grover": lambda: C.grover_proxy(n, 4),
+        "hea": lambda: C.hardware_efficient(n, 3), "brickwork": lambda: C.brickwork(n, 10),
+        "random": lambda: C.random_circuit(n, 10),
+    }[name]()
+    prep = C.product_state_prep(n, seed=99)
+    want = O.run(n, prep + gates)
+    c = C.to_circuit(n, gates)
+    for opt in ("fused", "none", "diag_only", "win_only"):
+        options = FusionOptions(*{"fused": (1, 1, 1, 4, 0), "none": (0, 0, 0, 1, 0),
+                                  "diag_only": (0, 1, 1, 1, 0),
+                                  "win_only": (1, 0, 0, 4, 0)}[opt])
+        got, norm, launches = run(n, gates, options=options, prep=prep)
+        ref, _, ref_launches = run(n, gates, merge=False, options=options, prep=prep)
+        d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
+        passes = c.passes(options)
+        print(f"{name:9s} n={n:2d} {opt:9s} plan={len(passes):4d} launches={launches:4d} "
+              f"diff={d:.2e} vs op-by-op {dm:.2e}")
+        assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP, (opt, d)
+        assert dm <= TOL_SAME, (opt, dm)
+        assert launches == max(passes) + 1
+        assert ref_launches == sum(1 for k, op in enumerate(c.plan(options))
+                                   if not (op["kind"] == "DiagonalLayer" and not op["masks"]))
+
+
+def test_dispatch_count_follows_grouping():
+    """One launch per plan op with merging off (the reference's accounting,
+    tests/test_gpu.mm:247-256); one per merged pass with it on."""
+    c = C.to_circuit(8, C.ghz(8))
+    with qm.Simulator(8) as s:
+        s.set_pass_merging(False)
+        before = s.dispatch_count
+        s.run(c)
+        s.synchronize()
+        assert s.dispatch_count - before == len(c) == c.plan_size(True)
+        s.set_pass_merging(True)
+        before = s.dispatch_count
+        s.run(c)
+        s.synchronize()
+        assert s.dispatch_count - before == max(c.passes()) + 1 < len(c)
+
+
+def test_qft_24_merged_vs_op_by_op():
+    n = 24
+    prep = C.product_state_prep(n, seed=2024)
+    got, norm, launches = run(n, C.qft(n), prep=prep)
+    ref, _, ref_launches = run(n, C.qft(n), merge=False, prep=prep)
+    dm = np.abs(got - ref).max()
+    print(f"QFT24: {launches} launches vs {ref_launches}; merged vs op-by-op {dm:.2e}")
+    assert launches == 6 and ref_launches == 2 * n - 1
+    assert dm <= TOL_SAME and abs(norm - 1) <= TOL1
+
+
+def test_qft_zero_state_is_uniform_merged():
+    n = 28
+    with qm.Simulator(n) as s:
+        s.run(C.to_circuit(n, C.qft(n)))
+        zs = [abs(s.expectation_pauli(z=1 << q)) for q in (0, 1, 5, 13, 20, 27)]
+        xs = [s.expectation_pauli(x=1 << q) for q in (0, 9, 27)]
+        norm = s.norm()
+    assert abs(norm - 1) <= TOL1
+    assert max(zs) <= TOL1
+    assert all(abs(x - 1) <= TOL1 for x in xs)

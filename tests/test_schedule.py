@@ -1,0 +1,106 @@
+"""Executor-level pass merging: the grouping of plan ops into HBM passes
+(host logic, no GPU). The plan itself is unchanged; these tests pin which
+ops share a pass and the constraints every pass must meet."""
+import pytest
+
+from conftest import FUSION, golden_gates, load_golden
+from qmetal_amd import FusionOptions
+from qmetal_amd import circuits as C
+
+MAX_REG = 4  # register wires beyond bit 0 in one pass
+
+
+def moving_wires(op):
+    """Wires (other than 0) a plan op needs in the thread's registers."""
+    k, q = op["kind"], op["qubits"]
+    if k == "Dense1q":
+        w = {q[0]}
+    elif k == "Dense1qGroup":
+        w = set(op["group_qubits"])
+    elif k == "Controlled1q":
+        w = {q[1]}
+    elif k == "Swap":
+        w = {q[0], q[1]}
+    elif k == "CCX":
+        w = {q[2]}
+    else:
+        w = set()
+    return w - {0}
+
+
+def check_grouping(circ, options=None):
+    plan = circ.plan(options)
+    passes = circ.passes(options)
+    assert len(passes) == len(plan)
+    seen = [p for p in passes if p >= 0]
+    # passes are contiguous runs in plan order, numbered from 0
+    assert seen == sorted(seen)
+    if seen:
+        assert sorted(set(seen)) == list(range(max(seen) + 1))
+    for k, op in enumerate(plan):
+        if op["kind"] == "DiagonalLayer" and not op["masks"]:
+            assert passes[k] == -1
+        else:
+            assert passes[k] >= 0
+    groups = {}
+    for k, p in enumerate(passes):
+        if p >= 0:
+            groups.setdefault(p, []).append(plan[k])
+    for ops in groups.values():
+        if len(ops) > 1:
+            wires = set().union(*(moving_wires(o) for o in ops))
+            assert len(wires) <= MAX_REG, wires
+            assert all(o["kind"] != "Blocked1q" for o in ops)
+    return max(seen) + 1 if seen else 0
+
+
+def test_qft_pass_counts():
+    """QFT: H on wire j then the CP ladder onto j, for j = n-1 .. 0. Four H's
+    and their ladders share a pass (the last pass also takes wire 0)."""
+    for n, want in ((12, 3), (26, 7), (30, 8), (32, 8), (33, 8), (34, 9)):
+        c = C.to_circuit(n, C.qft(n))
+        assert c.plan_size(True) == 2 * n - 1
+        assert check_grouping(c) == want, n
+
+
+def test_ghz_pass_count():
+    # H(0), CX(0,1), ..., CX(18,19): four new targets per pass
+    assert check_grouping(C.to_circuit(20, C.ghz(20))) == 5
+
+
+def test_narrow_states_run_op_by_op():
+    for n in (2, 3, 4):
+        c = C.to_circuit(n, C.qft(n))
+        assert c.passes() == list(range(c.plan_size(True)))
+
+
+def test_blocked_ops_stay_alone():
+    c = C.to_circuit(12, C.brickwork(12, 4))
+    opts = FusionOptions(*FUSION["blocked8"])
+    plan, passes = c.plan(opts), c.passes(opts)
+    for k, op in enumerate(plan):
+        if op["kind"] == "Blocked1q":
+            assert passes.count(passes[k]) == 1
+    check_grouping(c, opts)
+
+
+@pytest.mark.parametrize("opt", sorted(FUSION))
+def test_grouping_invariants_on_streams(opt):
+    """Every fuzz stream (n = 3..7) and the configuration circuits under every
+    fusion setting: contiguous passes, each within the register budget."""
+    opts = FusionOptions(*FUSION[opt])
+    for d in load_golden("fuzz.json.gz"):
+        if FUSION[opt][4] <= d["n"]:
+            check_grouping(C.to_circuit(d["n"], golden_gates(d)), opts)
+    for n, gates in ((14, C.brickwork(14, 6)), (12, C.qaoa_ring(12, 6)),
+                     (13, C.hardware_efficient(13, 3)), (16, C.tfim_trotter(16, 5)),
+                     (10, C.grover_proxy(10, 4))):
+        if FUSION[opt][4] <= n:
+            check_grouping(C.to_circuit(n, gates), opts)
+
+
+def test_merging_reduces_launches():
+    for n, gates in ((20, C.brickwork(20, 10)), (16, C.qaoa_ring(16, 6)),
+                     (18, C.hardware_efficient(18, 3))):
+        c = C.to_circuit(n, gates)
+        assert check_grouping(c) < c.plan_size(True)
