@@ -43,6 +43,8 @@ def _run(cmd):
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
+    # The pass kernel's slot statements are generated text.
+    _run([sys.executable, os.path.join(CSRC, "gen_slot_asm.py")])
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
     jobs = []
     for src in HIP_SOURCES:

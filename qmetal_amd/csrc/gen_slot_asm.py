@@ -1,0 +1,137 @@
+"""Writes slot_asm.inc: one inline-assembly statement per 2x2 slot op of a
+merged pass, for tiles of 2^NL amplitudes (NL = 4, 5).
+
+    python qmetal_amd/csrc/gen_slot_asm.py      (build.py runs it)
+
+Why assembly: a pass applies a runtime list of ops to the tile held in
+registers. Written in C++, every op kind or target bit is a separate code path
+that redefines the tile, and the compiler then keeps a second copy of the
+whole tile across the op loop (double the VGPRs, half the occupancy). One
+statement whose outputs are tied to its inputs defines every amplitude exactly
+once per op, whatever section runs inside it.
+
+Section `sel` = lbit + NL * kind:
+  kind 0  real matrix (m00, m01, m10, m11 = m[0], m[2], m[4], m[6]), all pairs
+  kind 1  complex matrix, all pairs
+  kind 2  complex matrix, only the pairs whose bit is set in `pm` (pair p is
+          the p-th index l with bit lbit clear, ascending)
+Any other `sel` leaves the tile unchanged. The matrix is per lane (VGPRs).
+"""
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def bfly_real(a, b):
+    ra, ia, rb, ib = f"%[r{a}]", f"%[i{a}]", f"%[r{b}]", f"%[i{b}]"
+    return [
+        f"v_mul_f32 %[t0], %[m1r], {rb}",
+        f"v_fmac_f32 %[t0], %[m0r], {ra}",
+        f"v_mul_f32 %[t1], %[m1r], {ib}",
+        f"v_fmac_f32 %[t1], %[m0r], {ia}",
+        f"v_mul_f32 {rb}, %[m3r], {rb}",
+        f"v_fmac_f32 {rb}, %[m2r], {ra}",
+        f"v_mul_f32 {ib}, %[m3r], {ib}",
+        f"v_fmac_f32 {ib}, %[m2r], {ia}",
+        f"v_mov_b32 {ra}, %[t0]",
+        f"v_mov_b32 {ia}, %[t1]",
+    ]
+
+
+def bfly_cplx(a, b):
+    ra, ia, rb, ib = f"%[r{a}]", f"%[i{a}]", f"%[r{b}]", f"%[i{b}]"
+    return [
+        # a'.x = m00r ax - m00i ay + m01r bx - m01i by
+        f"v_mul_f32 %[t0], %[m0r], {ra}",
+        f"v_fma_f32 %[t0], -%[m0i], {ia}, %[t0]",
+        f"v_fmac_f32 %[t0], %[m1r], {rb}",
+        f"v_fma_f32 %[t0], -%[m1i], {ib}, %[t0]",
+        # a'.y = m00r ay + m00i ax + m01r by + m01i bx
+        f"v_mul_f32 %[t1], %[m0r], {ia}",
+        f"v_fmac_f32 %[t1], %[m0i], {ra}",
+        f"v_fmac_f32 %[t1], %[m1r], {ib}",
+        f"v_fmac_f32 %[t1], %[m1i], {rb}",
+        # b'.x = m10r ax - m10i ay + m11r bx - m11i by
+        f"v_mul_f32 %[t2], %[m2r], {ra}",
+        f"v_fma_f32 %[t2], -%[m2i], {ia}, %[t2]",
+        f"v_fmac_f32 %[t2], %[m3r], {rb}",
+        f"v_fma_f32 %[t2], -%[m3i], {ib}, %[t2]",
+        # b'.y = m10r ay + m10i ax + m11r by + m11i bx, in place (last reader)
+        f"v_mul_f32 {ib}, %[m3r], {ib}",
+        f"v_fmac_f32 {ib}, %[m2r], {ia}",
+        f"v_fmac_f32 {ib}, %[m2i], {ra}",
+        f"v_fmac_f32 {ib}, %[m3i], {rb}",
+        f"v_mov_b32 {ra}, %[t0]",
+        f"v_mov_b32 {ia}, %[t1]",
+        f"v_mov_b32 {rb}, %[t2]",
+    ]
+
+
+def pairs(nl, h):
+    return [(l, l | (1 << h)) for l in range(1 << nl) if not (l >> h) & 1]
+
+
+def statement(nl):
+    lines = []
+    nsec = 3 * nl
+    for sec in range(nsec):
+        lines += [f"s_cmp_eq_u32 %[sel], {sec}", f"s_cbranch_scc1 .Lqs%=_{sec}"]
+    lines.append("s_branch .Lqs%=_end")
+    for sec in range(nsec):
+        h, kind = sec % nl, sec // nl
+        lines.append(f".Lqs%=_{sec}:")
+        for p, (a, b) in enumerate(pairs(nl, h)):
+            if kind == 0:
+                lines += bfly_real(a, b)
+            elif kind == 1:
+                lines += bfly_cplx(a, b)
+            else:
+                lines += [f"s_bitcmp1_b32 %[pm], {p}", f"s_cbranch_scc0 .Lqs%=_{sec}_{p}"]
+                lines += bfly_cplx(a, b)
+                lines.append(f".Lqs%=_{sec}_{p}:")
+        lines.append("s_branch .Lqs%=_end")
+    lines.append(".Lqs%=_end:")
+    body = "\n".join(f'      "{ln}\\n"' for ln in lines)
+    n = 1 << nl
+    outs = ",\n".join(
+        f'        [r{k}] "+v"(x[{k}].x), [i{k}] "+v"(x[{k}].y)' for k in range(n))
+    return f"""template <>
+__device__ __forceinline__ void slot_asm<{nl}>(float2 (&x)[{n}], uint32_t sel, uint32_t pm,
+                                             const float (&m)[8]) {{
+  float t0, t1, t2;
+  asm(
+{body}
+      : {outs.strip()},
+        [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2)
+      : [m0r] "v"(m[0]), [m0i] "v"(m[1]), [m1r] "v"(m[2]), [m1i] "v"(m[3]),
+        [m2r] "v"(m[4]), [m2i] "v"(m[5]), [m3r] "v"(m[6]), [m3i] "v"(m[7]),
+        [sel] "s"(sel), [pm] "s"(pm)
+      : "scc");
+}}
+"""
+
+
+def main():
+    out = [
+        "// slot_asm.inc -- generated by gen_slot_asm.py; do not edit.",
+        "// One inline-assembly statement per 2x2 slot op of a merged pass (see the",
+        "// generator for the section layout and why the op is written this way).",
+        "",
+        "template <int NL>",
+        "__device__ __forceinline__ void slot_asm(float2 (&x)[1 << NL], uint32_t sel, uint32_t pm,",
+        "                                         const float (&m)[8]);",
+        "",
+    ]
+    for nl in (4, 5):
+        out.append(statement(nl))
+    text = "\n".join(out)
+    path = os.path.join(HERE, "slot_asm.inc")
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as f:
+            f.write(text)
+    return path
+
+
+if __name__ == "__main__":
+    print(main())

@@ -24,13 +24,15 @@ struct Slot {
 // Gather layout. Thread T owns the 2^NL amplitudes
 //   deposit(T) | fixed | sum_k bit_k(r) << rpos[k] | e,   r < 2^(NL-1), e < 2,
 // where deposit() inserts zero bits at the ascending positions ins[], which
-// are {0} + rpos[] + the fixed-one bits.
+// are {0} + rpos[] + the fixed-one bits; hm[k] = ~(2^ins[k] - 1) (inserting a
+// zero at p turns t into t + (t & hm)).
 struct Layout {
   uint64_t nthreads;
   uint64_t fixed;
   uint32_t nins;
   uint8_t ins[12];
   uint8_t rpos[8];
+  uint64_t hm[12];
   // 1 when positions 1..3 are thread-index bits: then eight neighbouring lanes
   // cover one whole 128-byte line and the accesses can bypass the caches
   // (nontemporal); otherwise a line is split across load instructions and
@@ -92,26 +94,37 @@ struct BlockedArgs {
 // One op of a merged pass (several plan ops applied to the 2^NL amplitudes a
 // thread holds between one load and one store of them).
 //   type 0, a 2x2 slot: applies to tile-local bit `lbit` where the tile-local
-//     bits `cmask` are 1 and the thread's base index has every bit of `gmask`.
-//   type 1, a phase layer: phase in turns =
-//       constant + sum_k beta_k x_k + sum_{k<j} gamma_kj x_k x_j
-//     over tile-local bits after terms[term0 .. term0 + nterms) are folded in
-//     from the base index (the first `nuniform` read only bits that are the
-//     same across the workgroup); applied to local combinations containing
-//     `cmask`, in threads whose base has `gmask` (the others keep phase 0).
+//     bits `cmask` are 1 and the thread's base index has every bit of `gmask`;
+//     type 2 is the same with a real matrix (imaginary parts zero).
+//   type 1, a phase layer: amplitude l (tile-local combination) is multiplied
+//     by table[l] * exp(2 pi i (phi + sum_{k in l} beta_k)), where table holds
+//     everything that does not depend on the thread (constant, local pair
+//     terms, linear terms of local bits outside `bmask`) and phi / beta_k
+//     (k in `bmask`, starting from beta0[k]) are FP64 sums, in turns, of the
+//     terms[term0 .. term0 + nterms) that read the thread's base index: the
+//     first `nuniform` read only bits that are the same across the workgroup
+//     (`umask` bit 0 / bit 1+k: they feed phi / beta_k). `has_phi`: some term
+//     feeds phi. Applied to local combinations containing `cmask`, in threads
+//     whose base has `gmask` (the others keep phase 0).
 constexpr int kPassMaxNL = 5;
+enum PassOpType : uint32_t { kPassSlot = 0, kPassPhase = 1, kPassRealSlot = 2 };
 struct PassOp {
   uint32_t type;
   uint32_t lbit;
   uint32_t cmask;
+  uint32_t sel;   // slot: code section, lbit + NL * (0 real, 1 complex, 2 masked)
+  uint32_t pm;    // slot, masked: pair p (p-th local index with lbit clear) applies
   uint32_t term0;
   uint32_t nterms;
   uint32_t nuniform;
+  uint32_t umask;
+  uint32_t bmask;
+  uint32_t has_phi;
+  uint32_t pad[3];
   uint64_t gmask;
   float m[8];
-  double constant;
   double beta0[kPassMaxNL];
-  double gamma[kPassMaxNL][kPassMaxNL];
+  float table[2 << kPassMaxNL];  // (re, im) per local combination
 };
 
 struct PassArgs {
@@ -122,6 +135,36 @@ struct PassArgs {
   uint32_t nops;
 };
 
+// A merged pass of chain shape: step i applies an uncontrolled 2x2 to
+// tile-local bit b_i (b_i = i + 1 for i < K, then bit 0 if the chain ends on
+// wire 0), then multiplies every local combination l with bit b_i set by
+// exp(2 pi i (beta + sum_{k in l, k != b_i} gamma_k)). Every mask of such a
+// phase layer holds b_i, so its local part is linear in the other local bits,
+// which split into a low group (the lowest two) and a high group (the rest,
+// up to two): lo[c] and hi[c] (c = 1..3, the group's combination) are the
+// products of exp(2 pi i gamma_k) over the group, rounded to float. beta
+// (turns) = beta0 + the step's base-index terms that hold in the thread's
+// base index. A step without a phase layer has unit factors and no terms.
+// Real chains use m[0], m[2], m[4], m[6].
+struct ChainStep {
+  float m[8];
+  float lo[6];
+  float hi[6];
+  double beta0;
+};
+
+// The base-index terms of all steps in two flat lists (DiagTerm.k = step):
+// `uni` reads only workgroup-uniform bits, `lane` the rest.
+struct ChainArgs {
+  float2 *a;
+  Layout lay;
+  const ChainStep *steps;
+  const DiagTerm *uni;
+  const DiagTerm *lane;
+  uint32_t nuni;
+  uint32_t nlane;
+};
+
 void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim);
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim);
 void launch_gather(hipStream_t s, int nl, const GatherArgs &g);
@@ -130,6 +173,8 @@ void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d);
 void launch_blocked(hipStream_t s, const BlockedArgs &b);
 // nl in {4, 5}.
 void launch_pass(hipStream_t s, int nl, const PassArgs &p);
+// nl = 5 with k = 4, or nl = 4 with k in 1..3; `zero`: a last step on bit 0.
+void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const ChainArgs &c);
 
 // FP64 reductions. Each writes per-block partials into `partials` (capacity
 // kRedBlocks doubles or double2) and then reduces them in a fixed order into

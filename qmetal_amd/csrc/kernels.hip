@@ -35,13 +35,38 @@ inline unsigned grid_for(uint64_t work) {
   return unsigned(g);
 }
 
-__device__ __forceinline__ uint64_t deposit(uint64_t t, const Layout &L) {
-  for (uint32_t k = 0; k < L.nins; k++) {
-    const uint32_t p = L.ins[k];
-    const uint64_t low = t & ((1ull << p) - 1ull);
-    t = ((t >> p) << (p + 1)) | low;
-  }
-  return t | L.fixed;
+// Thread index -> base index without the fixed bits: zero bits inserted at
+// the layout's positions, lowest first.
+__device__ __forceinline__ uint64_t scatter(uint64_t t, const Layout &L) {
+  for (uint32_t k = 0; k < L.nins; k++) t += t & L.hm[k];
+  return t;
+}
+
+// Base index of thread-index value t0 + tid with t0 a multiple of the
+// workgroup size: scatter is a bit placement, so the workgroup-uniform part
+// scatter(t0) | fixed and the per-thread part scatter(tid) are disjoint. The
+// per-thread part stays below 2^24 (the workgroup's index bits land on the
+// lowest free positions), so a thread addresses its amplitudes as a uniform
+// 64-bit pointer plus one 32-bit byte offset.
+struct TileAddr {
+  uint32_t lane_bytes;  // 8 * scatter(tid)
+  uint64_t lane_bits;   // scatter(tid)
+};
+
+__device__ __forceinline__ TileAddr tile_addr(const Layout &L) {
+  TileAddr a;
+  a.lane_bits = scatter(threadIdx.x, L);
+  a.lane_bytes = uint32_t(a.lane_bits) * 8u;
+  return a;
+}
+
+// Amplitude pair (ub | o) + the thread's part, as a 16-byte address.
+__device__ __forceinline__ const float2 *pair_ptr(const float2 *a, uint64_t ubo,
+                                                  uint32_t lane_bytes) {
+  return reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(a + ubo) + lane_bytes);
+}
+__device__ __forceinline__ float2 *pair_ptr(float2 *a, uint64_t ubo, uint32_t lane_bytes) {
+  return reinterpret_cast<float2 *>(reinterpret_cast<char *>(a + ubo) + lane_bytes);
 }
 
 template <int NL>
@@ -68,6 +93,25 @@ __device__ __forceinline__ void rot(const float *m, float2 &x, float2 &y) {
   y.x = m[4] * u.x - m[5] * u.y + m[6] * v.x - m[7] * v.y;
   y.y = m[4] * u.y + m[5] * u.x + m[6] * v.y + m[7] * v.x;
 }
+
+// a *= f in place (f per lane) unless `skip` (workgroup-uniform, in an SGPR)
+// is set; the branch is inside the statement, so a is defined once either way.
+__device__ __forceinline__ void cmul_inplace(float2 &a, float2 f, uint32_t skip) {
+  float t;
+  asm("s_cmp_lg_u32 %[skip], 0\n\t"
+      "s_cbranch_scc1 .Lqm%=\n\t"
+      "v_mul_f32 %[t], %[ax], %[fy]\n\t"
+      "v_fmac_f32 %[t], %[ay], %[fx]\n\t"
+      "v_mul_f32 %[ax], %[ax], %[fx]\n\t"
+      "v_fma_f32 %[ax], -%[ay], %[fy], %[ax]\n\t"
+      "v_mov_b32 %[ay], %[t]\n"
+      ".Lqm%=:"
+      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t)
+      : [fx] "v"(f.x), [fy] "v"(f.y), [skip] "s"(skip)
+      : "scc");
+}
+
+#include "slot_asm.inc"
 
 template <int NL, int LB>
 __device__ __forceinline__ void apply_slot(float2 (&x)[1 << NL], uint32_t cm, const float *m) {
@@ -120,20 +164,21 @@ __device__ __forceinline__ void st4(float2 *p, float4 v) {
 }
 
 template <int NL, bool NT>
-__device__ __forceinline__ void gather_one(float2 *__restrict__ a, uint64_t base,
+__device__ __forceinline__ void gather_one(float2 *__restrict__ a, uint64_t ub,
+                                           uint32_t lane_bytes,
                                            const uint64_t (&off)[1 << (NL - 1)],
                                            const GatherArgs &g) {
   float2 x[1 << NL];
 #pragma unroll
   for (int r = 0; r < (1 << (NL - 1)); r++) {
-    const float4 v = ld4<NT>(a + (base | off[r]));
+    const float4 v = ld4<NT>(pair_ptr(a, ub | off[r], lane_bytes));
     x[2 * r] = make_float2(v.x, v.y);
     x[2 * r + 1] = make_float2(v.z, v.w);
   }
   for (uint32_t k = 0; k < g.nslots; k++) apply_any<NL>(x, g.slots[k]);
 #pragma unroll
   for (int r = 0; r < (1 << (NL - 1)); r++)
-    st4<NT>(a + (base | off[r]),
+    st4<NT>(pair_ptr(a, ub | off[r], lane_bytes),
             make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
 }
 
@@ -141,12 +186,13 @@ template <int NL, bool NT>
 __global__ __launch_bounds__(kT) void k_gather(GatherArgs g) {
   uint64_t off[1 << (NL - 1)];
   offsets<NL>(g.lay, off);
+  const TileAddr ta = tile_addr(g.lay);
   const uint64_t stride = uint64_t(gridDim.x) * kT;
-  for (uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x; t < g.lay.nthreads;
-       t += stride) {
-    const uint64_t base = deposit(t, g.lay);
-    gather_one<NL, NT>(g.a, base, off, g);
-    if (g.b) gather_one<NL, NT>(g.b, base, off, g);
+  for (uint64_t t0 = uint64_t(blockIdx.x) * kT; t0 < g.lay.nthreads; t0 += stride) {
+    if (t0 + threadIdx.x >= g.lay.nthreads) break;
+    const uint64_t ub = scatter(t0, g.lay) | g.lay.fixed;
+    gather_one<NL, NT>(g.a, ub, ta.lane_bytes, off, g);
+    if (g.b) gather_one<NL, NT>(g.b, ub, ta.lane_bytes, off, g);
   }
 }
 
@@ -170,13 +216,14 @@ template <int NL, bool NT>
 __global__ __launch_bounds__(kT) void k_diag(DiagArgs d) {
   uint64_t off[1 << (NL - 1)];
   offsets<NL>(d.lay, off);
+  const TileAddr ta = tile_addr(d.lay);
   const uint64_t stride = uint64_t(gridDim.x) * kT;
   const uint32_t lane = threadIdx.x & 63;
   for (uint64_t t0 = uint64_t(blockIdx.x) * kT; t0 < d.lay.nthreads; t0 += stride) {
     // Workgroup-uniform terms (they read only base-index bits that are the
     // same for every thread of this iteration): the lanes of each wave take
     // one term each and the wave sums the hits.
-    const uint64_t ub = deposit(t0, d.lay);
+    const uint64_t ub = scatter(t0, d.lay) | d.lay.fixed;
     double phi = 0.0;
     double beta[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t k = lane; k < d.nuniform; k += 64) {
@@ -198,9 +245,8 @@ __global__ __launch_bounds__(kT) void k_diag(DiagArgs d) {
     phi += d.constant;
 #pragma unroll
     for (int j = 0; j < 4; j++) beta[j] += d.beta0[j];
-    const uint64_t t = t0 + threadIdx.x;
-    if (t >= d.lay.nthreads) continue;
-    const uint64_t base = deposit(t, d.lay);
+    if (t0 + threadIdx.x >= d.lay.nthreads) continue;
+    const uint64_t base = ub | ta.lane_bits;
     for (uint32_t k = d.nuniform; k < d.nterms; k++) {
       const DiagTerm tm = d.terms[k];
       const bool hit = (base & tm.mask) == tm.mask;
@@ -229,7 +275,7 @@ __global__ __launch_bounds__(kT) void k_diag(DiagArgs d) {
     // thread is in flight at once.
     float4 v[1 << (NL - 1)];
 #pragma unroll
-    for (int r = 0; r < (1 << (NL - 1)); r++) v[r] = ld4<NT>(d.a + (base | off[r]));
+    for (int r = 0; r < (1 << (NL - 1)); r++) v[r] = ld4<NT>(pair_ptr(d.a, ub | off[r], ta.lane_bytes));
 #pragma unroll
     for (int r = 0; r < (1 << (NL - 1)); r++) {
       float s0, c0, s1, c1;
@@ -240,7 +286,7 @@ __global__ __launch_bounds__(kT) void k_diag(DiagArgs d) {
       v[r] = make_float4(a0.x, a0.y, a1.x, a1.y);
     }
 #pragma unroll
-    for (int r = 0; r < (1 << (NL - 1)); r++) st4<NT>(d.a + (base | off[r]), v[r]);
+    for (int r = 0; r < (1 << (NL - 1)); r++) st4<NT>(pair_ptr(d.a, ub | off[r], ta.lane_bytes), v[r]);
   }
 }
 
@@ -257,100 +303,268 @@ __global__ __launch_bounds__(kT) void k_diag_generic(DiagGenericArgs d) {
   }
 }
 
+// Read-only kernel inputs seen through the constant address space: uniform
+// loads through such a pointer become scalar loads into SGPRs (the compiler
+// need not prove that the state stores of the kernel do not alias them).
+template <typename T>
+using const_ptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ const_ptr<T> as_const(const T *p) {
+  return (const_ptr<T>)p;
+}
+
 // Merged pass: the thread's 2^NL amplitudes are loaded once, every op of the
 // pass is applied to them in plan order, and they are stored once. Ops are
-// workgroup-uniform; their descriptors come from device memory.
+// workgroup-uniform; their descriptors are read with scalar loads.
 template <int NL, bool NT>
 __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
   uint64_t off[1 << (NL - 1)];
   offsets<NL>(p.lay, off);
-  const PassOp *__restrict__ ops = p.ops;
-  const DiagTerm *__restrict__ terms = p.terms;
+  const const_ptr<PassOp> ops = as_const(p.ops);
+  const const_ptr<DiagTerm> terms = as_const(p.terms);
+  const TileAddr ta = tile_addr(p.lay);
   const uint64_t stride = uint64_t(gridDim.x) * kT;
   const uint32_t lane = threadIdx.x & 63;
   for (uint64_t t0 = uint64_t(blockIdx.x) * kT; t0 < p.lay.nthreads; t0 += stride) {
-    const uint64_t ub = deposit(t0, p.lay);
-    const uint64_t t = t0 + threadIdx.x;
-    const bool active = t < p.lay.nthreads;
-    const uint64_t base = active ? deposit(t, p.lay) : ub;
+    const uint64_t ub = scatter(t0, p.lay) | p.lay.fixed;
+    const bool active = t0 + threadIdx.x < p.lay.nthreads;
+    const uint64_t base = ub | ta.lane_bits;
+    // No lane-divergent control flow between the load and the store (it
+    // would make the compiler keep a second copy of the amplitudes): lanes
+    // past the end compute on a valid tile they never store, and a control on
+    // a base-index bit picks the identity per lane instead of branching.
     float2 x[1 << NL];
-    if (active) {
+    {
+      // Lanes past the end read the workgroup's first tile (in bounds),
+      // compute on it and drop it.
+      const uint32_t lb = active ? ta.lane_bytes : 0u;
       float4 v[1 << (NL - 1)];
 #pragma unroll
-      for (int r = 0; r < (1 << (NL - 1)); r++) v[r] = ld4<NT>(p.a + (base | off[r]));
+      for (int r = 0; r < (1 << (NL - 1)); r++) v[r] = ld4<NT>(pair_ptr(p.a, ub | off[r], lb));
 #pragma unroll
       for (int r = 0; r < (1 << (NL - 1)); r++) {
         x[2 * r] = make_float2(v[r].x, v[r].y);
         x[2 * r + 1] = make_float2(v[r].z, v[r].w);
       }
     }
+    // Every op runs both statements that write the tile, the 2x2 slot and
+    // the per-combination phase multiply, each of which skips itself inside
+    // (a branch around either would be a second definition of the tile).
     for (uint32_t o = 0; o < p.nops; o++) {
-      const PassOp &op = ops[o];
-      const bool on = active && (base & op.gmask) == op.gmask;
-      if (op.type == 0) {
-        if (on) apply_any<NL>(x, op.lbit, op.cmask, op.m);
-        continue;
+      const bool is_phase = ops[o].type == kPassPhase;
+      const uint64_t gmask = ops[o].gmask;
+      const uint32_t cm = ops[o].cmask;
+      const bool gon = (base & gmask) == gmask;
+      {
+        // A control on a base-index bit: the identity where this lane's
+        // base lacks it. A phase op's section code matches nothing.
+        float m[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const float mi = ops[o].m[i];
+          m[i] = gon ? mi : (i == 0 || i == 6 ? 1.0f : 0.0f);
+        }
+        slot_asm<NL>(x, __builtin_amdgcn_readfirstlane(ops[o].sel),
+                     __builtin_amdgcn_readfirstlane(ops[o].pm), m);
       }
-      // Phase layer. Workgroup-uniform terms: one term per lane, summed over
-      // the wave (every lane takes part, active or not).
-      double phi = 0.0;
-      double beta[NL];
+      // Phase factors (temporaries; computed for a phase op only).
+      const uint32_t bmask = ops[o].bmask;
+      const bool has_phi = ops[o].has_phi != 0;
+      float2 e[NL];
 #pragma unroll
-      for (int j = 0; j < NL; j++) beta[j] = 0.0;
-      const DiagTerm *tl = terms + op.term0;
-      for (uint32_t k = lane; k < op.nuniform; k += 64) {
-        const DiagTerm tm = tl[k];
-        if ((ub & tm.mask) != tm.mask) continue;
-        if (tm.kind == 2) {
+      for (int j = 0; j < NL; j++) e[j] = make_float2(1.0f, 0.0f);
+      float2 ephi = make_float2(1.0f, 0.0f);
+      if (is_phase) {
+        // Workgroup-uniform terms: one term per lane, summed over the wave
+        // (every lane takes part, active or not).
+        const uint32_t nuniform = ops[o].nuniform, nterms = ops[o].nterms;
+        const uint32_t umask = ops[o].umask;
+        const const_ptr<DiagTerm> tl = terms + ops[o].term0;
+        double phi = 0.0;
+        double beta[NL];
 #pragma unroll
-          for (int j = 0; j < NL; j++)
-            if (tm.k == uint32_t(j)) beta[j] += tm.ang;
-        } else {
-          phi += tm.ang;
+        for (int j = 0; j < NL; j++) beta[j] = 0.0;
+        for (uint32_t k0 = 0; k0 < nuniform; k0 += 64) {
+          // Branch-free: lanes past the end read the last term and add zero.
+          const uint32_t k = k0 + lane;
+          const uint32_t kc = k < nuniform ? k : nuniform - 1;
+          const uint64_t mask = tl[kc].mask;
+          const double add = (k < nuniform && (ub & mask) == mask) ? double(tl[kc].ang) : 0.0;
+          const uint32_t kind = tl[kc].kind, kk = tl[kc].k;
+          phi += kind != 2 ? add : 0.0;
+#pragma unroll
+          for (int j = 0; j < NL; j++) beta[j] += (kind == 2 && kk == uint32_t(j)) ? add : 0.0;
+        }
+        if (umask & 1u) phi = wave_allsum(phi);
+#pragma unroll
+        for (int j = 0; j < NL; j++)
+          if ((umask >> (j + 1)) & 1u) beta[j] = wave_allsum(beta[j]);
+        for (uint32_t k = nuniform; k < nterms; k++) {
+          const uint64_t mask = tl[k].mask;
+          const double add = (base & mask) == mask ? double(tl[k].ang) : 0.0;
+          const uint32_t kind = tl[k].kind, kk = tl[k].k;
+          if (kind == 2) {
+#pragma unroll
+            for (int j = 0; j < NL; j++) beta[j] += kk == uint32_t(j) ? add : 0.0;
+          } else {
+            phi += add;
+          }
+        }
+        // One FP64-reduced sin/cos per thread-dependent sum; each amplitude
+        // then takes a table entry times the factors of its local bits.
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+          if ((bmask >> j) & 1u) {
+            float sn, cs;
+            turn_sincos(beta[j] + ops[o].beta0[j], sn, cs);
+            e[j] = make_float2(cs, sn);
+          }
+        }
+        if (has_phi) {
+          float sn, cs;
+          turn_sincos(phi, sn, cs);
+          ephi = make_float2(cs, sn);
         }
       }
-      if (op.nuniform) {
-        phi = wave_allsum(phi);
-#pragma unroll
-        for (int j = 0; j < NL; j++) beta[j] = wave_allsum(beta[j]);
-      }
-      if (!on) continue;
-      for (uint32_t k = op.nuniform; k < op.nterms; k++) {
-        const DiagTerm tm = tl[k];
-        const bool hit = (base & tm.mask) == tm.mask;
-        if (tm.kind == 2) {
-#pragma unroll
-          for (int j = 0; j < NL; j++)
-            if (hit && tm.k == uint32_t(j)) beta[j] += tm.ang;
-        } else if (hit) {
-          phi += tm.ang;
-        }
-      }
-      phi += op.constant;
-#pragma unroll
-      for (int j = 0; j < NL; j++) beta[j] += op.beta0[j];
-      const uint32_t cm = op.cmask;
 #pragma unroll
       for (int l = 0; l < (1 << NL); l++) {
-        if ((uint32_t(l) & cm) != cm) continue;
-        double ph = phi;
+        float2 f = make_float2(ops[o].table[2 * l], ops[o].table[2 * l + 1]);
+        if (has_phi) f = cmul(f, ephi);
 #pragma unroll
-        for (int h = 0; h < NL; h++) {
-          if (!((l >> h) & 1)) continue;
-          ph += beta[h];
-#pragma unroll
-          for (int k = 0; k < h; k++)
-            if ((l >> k) & 1) ph += op.gamma[k][h];
-        }
-        float s, c;
-        turn_sincos(ph, s, c);
-        x[l] = cmul(x[l], make_float2(c, s));
+        for (int h = 0; h < NL; h++)
+          if (((l >> h) & 1) && ((bmask >> h) & 1u)) f = cmul(f, e[h]);
+        // A thread whose base lacks a bit every term needs keeps phase 0.
+        if (gmask) f = gon ? f : make_float2(1.0f, 0.0f);
+        // Slot ops, and combinations without the local bits every term
+        // needs, are left alone.
+        cmul_inplace(x[l], f,
+                     __builtin_amdgcn_readfirstlane(!is_phase || (uint32_t(l) & cm) != cm));
       }
     }
     if (active) {
 #pragma unroll
       for (int r = 0; r < (1 << (NL - 1)); r++)
-        st4<NT>(p.a + (base | off[r]),
+        st4<NT>(pair_ptr(p.a, ub | off[r], ta.lane_bytes),
+                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
+    }
+  }
+}
+
+// x' = m00 x + m01 y, y' = m10 x + m11 y for a real 2x2 {m00, m01, m10, m11}.
+__device__ __forceinline__ void rot_real(float m00, float m01, float m10, float m11, float2 &x,
+                                         float2 &y) {
+  const float2 u = x, v = y;
+  x.x = m00 * u.x + m01 * v.x;
+  x.y = m00 * u.y + m01 * v.y;
+  y.x = m10 * u.x + m11 * v.x;
+  y.y = m10 * u.y + m11 * v.y;
+}
+
+// Chain-shaped merged pass (ChainStep): every step's local bit, and which
+// combinations its phase touches, are compile-time, so the step sequence is
+// straight-line code. The loads of the workgroup-uniform terms are issued
+// ahead of the amplitude loads so their latency overlaps.
+template <int NL, bool NT, int K, bool Z, bool REAL>
+__global__ __launch_bounds__(kT) void k_chain(ChainArgs p) {
+  constexpr int S = K + (Z ? 1 : 0);
+  static_assert(S <= NL && K < NL, "chain longer than the tile");
+  uint64_t off[1 << (NL - 1)];
+  offsets<NL>(p.lay, off);
+  const const_ptr<ChainStep> steps = as_const(p.steps);
+  const const_ptr<DiagTerm> uni = as_const(p.uni);
+  const const_ptr<DiagTerm> lane_terms = as_const(p.lane);
+  const TileAddr ta = tile_addr(p.lay);
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t0 = uint64_t(blockIdx.x) * kT; t0 < p.lay.nthreads; t0 += stride) {
+    const uint64_t ub = scatter(t0, p.lay) | p.lay.fixed;
+    const bool active = t0 + threadIdx.x < p.lay.nthreads;
+    const uint64_t base = ub | ta.lane_bits;
+    // Workgroup-uniform terms, one per lane (branch-free: lanes past the end
+    // read the last term and add zero).
+    double bu[S];
+#pragma unroll
+    for (int i = 0; i < S; i++) bu[i] = 0.0;
+    for (uint32_t k0 = 0; k0 < p.nuni; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      const uint32_t kc = k < p.nuni ? k : p.nuni - 1;
+      const uint64_t mask = uni[kc].mask;
+      const double add = (k < p.nuni && (ub & mask) == mask) ? double(uni[kc].ang) : 0.0;
+      const uint32_t st = uni[kc].k;
+#pragma unroll
+      for (int i = 0; i < S; i++) bu[i] += st == uint32_t(i) ? add : 0.0;
+    }
+    float2 x[1 << NL];
+    {
+      // Lanes past the end read the workgroup's first tile and never store.
+      const uint32_t lb = active ? ta.lane_bytes : 0u;
+#pragma unroll
+      for (int r = 0; r < (1 << (NL - 1)); r++) {
+        const float4 v = ld4<NT>(pair_ptr(p.a, ub | off[r], lb));
+        x[2 * r] = make_float2(v.x, v.y);
+        x[2 * r + 1] = make_float2(v.z, v.w);
+      }
+    }
+    // Per-thread factor of each step: exp(2 pi i beta).
+    double b[S];
+#pragma unroll
+    for (int i = 0; i < S; i++) b[i] = wave_allsum(bu[i]) + steps[i].beta0;
+    for (uint32_t k = 0; k < p.nlane; k++) {
+      const uint64_t mask = lane_terms[k].mask;
+      const double add = (base & mask) == mask ? double(lane_terms[k].ang) : 0.0;
+      const uint32_t st = lane_terms[k].k;
+#pragma unroll
+      for (int i = 0; i < S; i++)
+        if (st == uint32_t(i)) b[i] += add;
+    }
+    float2 e[S];
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+      float sn, cs;
+      turn_sincos(b[i], sn, cs);
+      e[i] = make_float2(cs, sn);
+    }
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+      const int bit = i < K ? i + 1 : 0;
+      // The step's constants are read through an index the compiler cannot
+      // see before the previous step has updated the tile; otherwise it
+      // hoists every step's scalar loads to the top and spills SGPRs.
+      uint32_t si = uint32_t(i);
+      asm volatile("" : "+s"(si) : "v"(x[0].x));
+      const ChainStep __attribute__((address_space(4))) &st = steps[si];
+      {
+        float m[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) m[j] = st.m[j];
+#pragma unroll
+        for (int l = 0; l < (1 << NL); l++) {
+          if (l & (1 << bit)) continue;
+          if constexpr (REAL) rot_real(m[0], m[2], m[4], m[6], x[l], x[l | (1 << bit)]);
+          else rot(m, x[l], x[l | (1 << bit)]);
+        }
+      }
+      // Phase on the combinations holding the step's bit: the other local
+      // bits (ascending) form c; its low two bits pick lo, the rest hi.
+      float2 a[4];
+      a[0] = e[i];
+#pragma unroll
+      for (int h = 1; h < 4; h++)
+        a[h] = cmul(e[i], make_float2(st.hi[2 * h - 2], st.hi[2 * h - 1]));
+#pragma unroll
+      for (int l = 0; l < (1 << NL); l++) {
+        if (!(l & (1 << bit))) continue;
+        const int c = (l & ((1 << bit) - 1)) | ((l >> (bit + 1)) << bit);
+        const int lo = c & 3, hi = c >> 2;
+        if (lo) x[l] = cmul(x[l], make_float2(st.lo[2 * lo - 2], st.lo[2 * lo - 1]));
+        x[l] = cmul(x[l], a[hi]);
+      }
+    }
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < (1 << (NL - 1)); r++)
+        st4<NT>(pair_ptr(p.a, ub | off[r], ta.lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
     }
   }
@@ -795,6 +1009,33 @@ void launch_pass_t(hipStream_t s, int nl, const PassArgs &p) {
 void launch_pass(hipStream_t s, int nl, const PassArgs &p) {
   if (p.lay.streaming) launch_pass_t<true>(s, nl, p);
   else launch_pass_t<false>(s, nl, p);
+}
+
+template <int NL, int K, bool Z>
+void launch_chain_k(hipStream_t s, bool nt, bool real, const ChainArgs &c) {
+  const unsigned grid = grid_for(c.lay.nthreads);
+  if (nt && real) k_chain<NL, true, K, Z, true><<<grid, kT, 0, s>>>(c);
+  else if (nt) k_chain<NL, true, K, Z, false><<<grid, kT, 0, s>>>(c);
+  else if (real) k_chain<NL, false, K, Z, true><<<grid, kT, 0, s>>>(c);
+  else k_chain<NL, false, K, Z, false><<<grid, kT, 0, s>>>(c);
+}
+
+void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const ChainArgs &c) {
+  const bool nt = c.lay.streaming != 0;
+  if (nl == 5 && k == 4) {
+    if (zero) launch_chain_k<5, 4, true>(s, nt, real, c);
+    else launch_chain_k<5, 4, false>(s, nt, real, c);
+  } else if (nl == 4 && k >= 1 && k <= 3) {
+    switch (k * 2 + (zero ? 1 : 0)) {
+      case 2: launch_chain_k<4, 1, false>(s, nt, real, c); break;
+      case 3: launch_chain_k<4, 1, true>(s, nt, real, c); break;
+      case 4: launch_chain_k<4, 2, false>(s, nt, real, c); break;
+      case 5: launch_chain_k<4, 2, true>(s, nt, real, c); break;
+      case 6: launch_chain_k<4, 3, false>(s, nt, real, c); break;
+      case 7: launch_chain_k<4, 3, true>(s, nt, real, c); break;
+      default: break;
+    }
+  }
 }
 
 void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d) {

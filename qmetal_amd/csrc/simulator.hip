@@ -105,7 +105,10 @@ struct LayoutBuilder {
     if (ins.size() > sizeof(L.ins) || reg.size() > sizeof(L.rpos))
       fail("internal: layout too wide");
     L.nins = uint32_t(ins.size());
-    for (size_t k = 0; k < ins.size(); k++) L.ins[k] = uint8_t(ins[k]);
+    for (size_t k = 0; k < ins.size(); k++) {
+      L.ins[k] = uint8_t(ins[k]);
+      L.hm[k] = ~((uint64_t(1) << ins[k]) - 1);
+    }
     for (size_t k = 0; k < reg.size(); k++) L.rpos[k] = uint8_t(reg[k]);
     L.fixed = fixed;
     L.nthreads = (uint64_t(1) << n) >> ins.size();
@@ -358,6 +361,18 @@ struct Simulator::Impl {
     int nl = 0;
     PassArgs a{};
     double bytes = 0.0;
+    // Chain-shaped pass (straight-line kernel) instead of the generic one.
+    bool chain = false;
+    int chain_k = 0;
+    bool chain_zero = false;
+    bool chain_real = false;
+    ChainArgs c{};
+  };
+
+  // Local coefficients of a phase op of a merged pass (turns).
+  struct PhaseCoef {
+    double beta0[kPassMaxNL] = {};
+    double gamma[kPassMaxNL][kPassMaxNL] = {};
   };
 
   static size_t pass_slots(const PlanOp &op) {
@@ -392,6 +407,7 @@ struct Simulator::Impl {
     PassOp *po = reinterpret_cast<PassOp *>(staging + off);
     DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off + nops * sizeof(PassOp));
     size_t i = 0, nt = 0;
+    std::vector<PhaseCoef> coef(nops);
     auto local = [&](uint32_t q) {
       const int l = lb.local_of(q);
       if (l < 0) fail("internal: pass wire not local");
@@ -408,7 +424,8 @@ struct Simulator::Impl {
     auto slot = [&](uint32_t target, const float *m8) -> PassOp & {
       PassOp &o = po[i++];
       o = PassOp{};
-      o.type = 0;
+      const bool real = m8[1] == 0.0f && m8[3] == 0.0f && m8[5] == 0.0f && m8[7] == 0.0f;
+      o.type = real ? kPassRealSlot : kPassSlot;
       o.lbit = local(target);
       std::memcpy(o.m, m8, sizeof(o.m));
       return o;
@@ -447,9 +464,10 @@ struct Simulator::Impl {
           break;
         }
         case OpKind::Diagonal: {
+          PhaseCoef &pc = coef[i];
           PassOp &o = po[i++];
           o = PassOp{};
-          o.type = 1;
+          o.type = kPassPhase;
           // With no global phase an amplitude lacking a bit common to every
           // mask keeps phase 0: only combinations holding those bits change.
           uint64_t common = ~0ull;
@@ -459,9 +477,37 @@ struct Simulator::Impl {
               if (((common & ~lb.fixed) >> q) & 1) control(q, o);
           }
           o.term0 = uint32_t(nt);
+          double constant = 0.0;
+          double(&beta0)[kPassMaxNL] = pc.beta0;
+          double(&gamma)[kPassMaxNL][kPassMaxNL] = pc.gamma;
           o.nterms = uint32_t(fold_terms(
-              op, lb.fixed, lane, [&](uint32_t q) { return lb.local_of(q); }, o.constant,
-              o.beta0, o.gamma, dt + nt, o.nuniform));
+              op, lb.fixed, lane, [&](uint32_t q) { return lb.local_of(q); }, constant, beta0,
+              gamma, dt + nt, o.nuniform));
+          // Which sums the base-index terms feed; the workgroup-uniform ones
+          // are reduced over the wave only where they feed something.
+          for (uint32_t u = 0; u < o.nterms; u++) {
+            const DiagTerm &tm = dt[nt + u];
+            const uint32_t bit = tm.kind == 2 ? 2u << tm.k : 1u;
+            if (u < o.nuniform) o.umask |= bit;
+            if (tm.kind == 2) o.bmask |= 1u << tm.k;
+            else o.has_phi = 1;
+          }
+          // Everything that does not depend on the thread, per local
+          // combination: exp(2 pi i (constant + sum beta0 + sum gamma)).
+          for (uint32_t h = 0; h < uint32_t(kPassMaxNL); h++)
+            if ((o.bmask >> h) & 1) o.beta0[h] = beta0[h];
+          for (uint32_t l = 0; l < (1u << pl.nl); l++) {
+            double th = constant;
+            for (int h = 0; h < pl.nl; h++) {
+              if (!((l >> h) & 1)) continue;
+              if (!((o.bmask >> h) & 1)) th += beta0[h];
+              for (int k = 0; k < h; k++)
+                if ((l >> k) & 1) th += gamma[k][h];
+            }
+            th -= std::rint(th);
+            o.table[2 * l] = float(std::cos(kTwoPi * th));
+            o.table[2 * l + 1] = float(std::sin(kTwoPi * th));
+          }
           nt += o.nterms;
           break;
         }
@@ -469,11 +515,129 @@ struct Simulator::Impl {
           fail("internal: blocked op in a merged pass");
       }
     }
+    // Code section of each slot: its local bit and kind (real over all
+    // pairs, complex over all pairs, complex over the pairs whose local
+    // controls are set); a phase op selects no section.
+    for (size_t j = 0; j < nops; j++) {
+      PassOp &o = po[j];
+      if (o.type == kPassPhase) {
+        o.sel = ~0u;
+        continue;
+      }
+      const uint32_t kind = o.cmask ? 2u : (o.type == kPassRealSlot ? 0u : 1u);
+      o.sel = o.lbit + uint32_t(pl.nl) * kind;
+      o.pm = 0;
+      uint32_t pidx = 0;
+      for (uint32_t l = 0; l < (1u << pl.nl); l++) {
+        if ((l >> o.lbit) & 1) continue;
+        if ((l & o.cmask) == o.cmask) o.pm |= 1u << pidx;
+        pidx++;
+      }
+    }
     pl.a.a = state;
     pl.a.ops = reinterpret_cast<const PassOp *>(arena + off);
     pl.a.terms = reinterpret_cast<const DiagTerm *>(arena + off + nops * sizeof(PassOp));
     pl.a.nops = uint32_t(nops);
     pl.bytes = pass_bytes(n, ps.fixed);
+    to_chain(po, dt, coef, nops, nt, off, pl);
+  }
+
+  // A pass of chain shape -- uncontrolled 2x2s on local bits 1, 2, ..., K in
+  // order (optionally one more on bit 0 last), each followed by at most one
+  // phase layer that only touches combinations holding that step's bit --
+  // is rewritten in place as ChainSteps for the straight-line kernel.
+  void to_chain(const PassOp *po, const DiagTerm *dt, const std::vector<PhaseCoef> &coef,
+                size_t nops, size_t nt, size_t off, PassLaunch &pl) {
+    struct Step {
+      int slot, phase;
+    };
+    std::vector<Step> st;
+    bool real = true;
+    for (size_t j = 0; j < nops; j++) {
+      const PassOp &o = po[j];
+      if (o.type != kPassPhase) {
+        if (o.cmask || o.gmask) return;
+        if (o.type != kPassRealSlot) real = false;
+        st.push_back(Step{int(j), -1});
+        continue;
+      }
+      if (st.empty() || st.back().phase >= 0) return;
+      const uint32_t b = po[st.back().slot].lbit;
+      if (o.gmask || o.has_phi || o.cmask != (1u << b) || (o.bmask & ~(1u << b))) return;
+      st.back().phase = int(j);
+    }
+    const int S = int(st.size());
+    int K = 0;
+    while (K < S && po[st[K].slot].lbit == uint32_t(K + 1)) K++;
+    const bool zero = K < S;
+    if (zero && (K != S - 1 || po[st[K].slot].lbit != 0)) return;
+    if (!((pl.nl == 5 && K == 4) || (pl.nl == 4 && K >= 1 && K <= 3))) return;
+    // Copy what is read before the region is overwritten.
+    const std::vector<PassOp> ops(po, po + nops);
+    const std::vector<DiagTerm> terms(dt, dt + nt);
+    ChainStep *cs = reinterpret_cast<ChainStep *>(staging + off);
+    DiagTerm *uni = reinterpret_cast<DiagTerm *>(staging + off + size_t(S) * sizeof(ChainStep));
+    size_t nuni = 0;
+    for (int i = 0; i < S; i++) {
+      const PassOp &o = ops[st[i].phase >= 0 ? st[i].phase : 0];
+      if (st[i].phase >= 0)
+        for (uint32_t u = 0; u < o.nuniform; u++) {
+          uni[nuni] = terms[o.term0 + u];
+          uni[nuni++].k = uint32_t(i);
+        }
+    }
+    DiagTerm *lanes = uni + nuni;
+    size_t nlane = 0;
+    for (int i = 0; i < S; i++) {
+      const PassOp &o = ops[st[i].phase >= 0 ? st[i].phase : 0];
+      if (st[i].phase >= 0)
+        for (uint32_t u = o.nuniform; u < o.nterms; u++) {
+          lanes[nlane] = terms[o.term0 + u];
+          lanes[nlane++].k = uint32_t(i);
+        }
+    }
+    for (int i = 0; i < S; i++) {
+      ChainStep &c = cs[i];
+      c = ChainStep{};
+      std::memcpy(c.m, ops[st[i].slot].m, sizeof(c.m));
+      const int b = int(ops[st[i].slot].lbit);
+      // Other local bits, ascending: the lowest two form the low group.
+      int other[kPassMaxNL], no = 0;
+      for (int k = 0; k < pl.nl; k++)
+        if (k != b) other[no++] = k;
+      double g[kPassMaxNL] = {};
+      if (st[i].phase >= 0) {
+        const PhaseCoef &pc = coef[size_t(st[i].phase)];
+        c.beta0 = pc.beta0[b];
+        for (int j = 0; j < no; j++)
+          g[j] = pc.gamma[std::min(b, other[j])][std::max(b, other[j])];
+      }
+      for (int cc = 1; cc < 4; cc++) {
+        double tl = 0.0, th = 0.0;
+        for (int j = 0; j < 2; j++)
+          if ((cc >> j) & 1) {
+            tl += g[j];
+            if (2 + j < no) th += g[2 + j];
+          }
+        tl -= std::rint(tl);
+        th -= std::rint(th);
+        c.lo[2 * cc - 2] = float(std::cos(kTwoPi * tl));
+        c.lo[2 * cc - 1] = float(std::sin(kTwoPi * tl));
+        c.hi[2 * cc - 2] = float(std::cos(kTwoPi * th));
+        c.hi[2 * cc - 1] = float(std::sin(kTwoPi * th));
+      }
+    }
+    pl.chain = true;
+    pl.chain_k = K;
+    pl.chain_zero = zero;
+    pl.chain_real = real;
+    pl.c.a = state;
+    pl.c.lay = pl.a.lay;
+    pl.c.steps = reinterpret_cast<const ChainStep *>(arena + off);
+    pl.c.uni = reinterpret_cast<const DiagTerm *>(arena + off + size_t(S) * sizeof(ChainStep));
+    pl.c.lane = pl.c.uni + nuni;
+    pl.c.nuni = uint32_t(nuni);
+    pl.c.nlane = uint32_t(nlane);
   }
 
   void pass_launch(const PassLaunch &pl) {
@@ -482,7 +646,8 @@ struct Simulator::Impl {
       a = take_event();
       check(hipEventRecord(a, stream), "hipEventRecord");
     }
-    launch_pass(stream, pl.nl, pl.a);
+    if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);
+    else launch_pass(stream, pl.nl, pl.a);
     launched();
     dispatches++;
     if (timing) {

@@ -153,19 +153,29 @@ def test_drift_with_depth():
 
 def test_launch_grouping_invariance():
     """Splitting a circuit into several qm_run calls at plan-op boundaries
-    changes no arithmetic: the result is bit-identical (the reference's
-    flush-cadence test, test_gpu.mm:189-222)."""
+    changes no arithmetic with one launch per plan op: the result is
+    bit-identical (the reference's flush-cadence test, test_gpu.mm:189-222).
+    With executor-level pass merging the cut also moves pass boundaries, so
+    phases are rounded in another order: equal to 2e-6."""
     n = 12
     gates = C.tfim_trotter(n, 30)
-    whole = gpu_state(n, gates)
     # cut after each RX layer: RX layers end a window, so plans concatenate
     layer = 3 * (n - 1) + n
-    for cut in (1, 2, 7):
+    for merge, tol in ((False, 0.0), (True, 2e-6)):
         with qm.Simulator(n) as s:
-            for a in range(0, 30, cut):
-                s.run(C.to_circuit(n, gates[a * layer:(a + cut) * layer]))
-            got = s.amplitudes().astype(np.complex128)
-        assert np.array_equal(got, whole), cut
+            s.set_pass_merging(merge)
+            s.run(C.to_circuit(n, gates))
+            whole = s.amplitudes().astype(np.complex128)
+        for cut in (1, 2, 7):
+            with qm.Simulator(n) as s:
+                s.set_pass_merging(merge)
+                for a in range(0, 30, cut):
+                    s.run(C.to_circuit(n, gates[a * layer:(a + cut) * layer]))
+                got = s.amplitudes().astype(np.complex128)
+            if merge:
+                assert np.abs(got - whole).max() <= tol, cut
+            else:
+                assert np.array_equal(got, whole), cut
 
 
 def test_qft_26_against_oracle():
