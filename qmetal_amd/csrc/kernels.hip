@@ -113,6 +113,78 @@ __device__ __forceinline__ void cmul_inplace(float2 &a, float2 f, uint32_t skip)
 
 #include "slot_asm.inc"
 
+// In-place updates with tied operands for straight-line code (the chain
+// kernel): every amplitude keeps its registers, so the compiler cannot hold
+// a step's results and its inputs at the same time. The matrix / factor
+// operands are workgroup-uniform (SGPRs) unless the name ends in _v.
+__device__ __forceinline__ void bfly_real_s(float m00, float m01, float m10, float m11,
+                                            float2 &a, float2 &b) {
+  float t0, t1;
+  asm("v_mul_f32 %[t0], %[m01], %[bx]\n\t"
+      "v_fmac_f32 %[t0], %[m00], %[ax]\n\t"
+      "v_mul_f32 %[t1], %[m01], %[by]\n\t"
+      "v_fmac_f32 %[t1], %[m00], %[ay]\n\t"
+      "v_mul_f32 %[bx], %[m11], %[bx]\n\t"
+      "v_fmac_f32 %[bx], %[m10], %[ax]\n\t"
+      "v_mul_f32 %[by], %[m11], %[by]\n\t"
+      "v_fmac_f32 %[by], %[m10], %[ay]\n\t"
+      "v_mov_b32 %[ax], %[t0]\n\t"
+      "v_mov_b32 %[ay], %[t1]"
+      : [ax] "+v"(a.x), [ay] "+v"(a.y), [bx] "+v"(b.x), [by] "+v"(b.y), [t0] "=&v"(t0),
+        [t1] "=&v"(t1)
+      : [m00] "s"(m00), [m01] "s"(m01), [m10] "s"(m10), [m11] "s"(m11));
+}
+
+__device__ __forceinline__ void bfly_cplx_s(const float *m, float2 &a, float2 &b) {
+  float t0, t1, t2;
+  asm("v_mul_f32 %[t0], %[m0r], %[ax]\n\t"
+      "v_fma_f32 %[t0], -%[m0i], %[ay], %[t0]\n\t"
+      "v_fmac_f32 %[t0], %[m1r], %[bx]\n\t"
+      "v_fma_f32 %[t0], -%[m1i], %[by], %[t0]\n\t"
+      "v_mul_f32 %[t1], %[m0r], %[ay]\n\t"
+      "v_fmac_f32 %[t1], %[m0i], %[ax]\n\t"
+      "v_fmac_f32 %[t1], %[m1r], %[by]\n\t"
+      "v_fmac_f32 %[t1], %[m1i], %[bx]\n\t"
+      "v_mul_f32 %[t2], %[m2r], %[ax]\n\t"
+      "v_fma_f32 %[t2], -%[m2i], %[ay], %[t2]\n\t"
+      "v_fmac_f32 %[t2], %[m3r], %[bx]\n\t"
+      "v_fma_f32 %[t2], -%[m3i], %[by], %[t2]\n\t"
+      "v_mul_f32 %[by], %[m3r], %[by]\n\t"
+      "v_fmac_f32 %[by], %[m2r], %[ay]\n\t"
+      "v_fmac_f32 %[by], %[m2i], %[ax]\n\t"
+      "v_fmac_f32 %[by], %[m3i], %[bx]\n\t"
+      "v_mov_b32 %[ax], %[t0]\n\t"
+      "v_mov_b32 %[ay], %[t1]\n\t"
+      "v_mov_b32 %[bx], %[t2]"
+      : [ax] "+v"(a.x), [ay] "+v"(a.y), [bx] "+v"(b.x), [by] "+v"(b.y), [t0] "=&v"(t0),
+        [t1] "=&v"(t1), [t2] "=&v"(t2)
+      : [m0r] "s"(m[0]), [m0i] "s"(m[1]), [m1r] "s"(m[2]), [m1i] "s"(m[3]), [m2r] "s"(m[4]),
+        [m2i] "s"(m[5]), [m3r] "s"(m[6]), [m3i] "s"(m[7]));
+}
+
+// a *= f in place.
+__device__ __forceinline__ void cmul_s(float2 &a, float fx, float fy) {
+  float t;
+  asm("v_mul_f32 %[t], %[fy], %[ax]\n\t"
+      "v_fmac_f32 %[t], %[fx], %[ay]\n\t"
+      "v_mul_f32 %[ax], %[fx], %[ax]\n\t"
+      "v_fma_f32 %[ax], -%[fy], %[ay], %[ax]\n\t"
+      "v_mov_b32 %[ay], %[t]"
+      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t)
+      : [fx] "s"(fx), [fy] "s"(fy));
+}
+
+__device__ __forceinline__ void cmul_v(float2 &a, float2 f) {
+  float t;
+  asm("v_mul_f32 %[t], %[fy], %[ax]\n\t"
+      "v_fmac_f32 %[t], %[fx], %[ay]\n\t"
+      "v_mul_f32 %[ax], %[fx], %[ax]\n\t"
+      "v_fma_f32 %[ax], -%[fy], %[ay], %[ax]\n\t"
+      "v_mov_b32 %[ay], %[t]"
+      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t)
+      : [fx] "v"(f.x), [fy] "v"(f.y));
+}
+
 template <int NL, int LB>
 __device__ __forceinline__ void apply_slot(float2 (&x)[1 << NL], uint32_t cm, const float *m) {
 #pragma unroll
@@ -451,16 +523,6 @@ __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
   }
 }
 
-// x' = m00 x + m01 y, y' = m10 x + m11 y for a real 2x2 {m00, m01, m10, m11}.
-__device__ __forceinline__ void rot_real(float m00, float m01, float m10, float m11, float2 &x,
-                                         float2 &y) {
-  const float2 u = x, v = y;
-  x.x = m00 * u.x + m01 * v.x;
-  x.y = m00 * u.y + m01 * v.y;
-  y.x = m10 * u.x + m11 * v.x;
-  y.y = m10 * u.y + m11 * v.y;
-}
-
 // Chain-shaped merged pass (ChainStep): every step's local bit, and which
 // combinations its phase touches, are compile-time, so the step sequence is
 // straight-line code. The loads of the workgroup-uniform terms are issued
@@ -541,8 +603,8 @@ __global__ __launch_bounds__(kT) void k_chain(ChainArgs p) {
 #pragma unroll
         for (int l = 0; l < (1 << NL); l++) {
           if (l & (1 << bit)) continue;
-          if constexpr (REAL) rot_real(m[0], m[2], m[4], m[6], x[l], x[l | (1 << bit)]);
-          else rot(m, x[l], x[l | (1 << bit)]);
+          if constexpr (REAL) bfly_real_s(m[0], m[2], m[4], m[6], x[l], x[l | (1 << bit)]);
+          else bfly_cplx_s(m, x[l], x[l | (1 << bit)]);
         }
       }
       // Phase on the combinations holding the step's bit: the other local
@@ -557,8 +619,8 @@ __global__ __launch_bounds__(kT) void k_chain(ChainArgs p) {
         if (!(l & (1 << bit))) continue;
         const int c = (l & ((1 << bit) - 1)) | ((l >> (bit + 1)) << bit);
         const int lo = c & 3, hi = c >> 2;
-        if (lo) x[l] = cmul(x[l], make_float2(st.lo[2 * lo - 2], st.lo[2 * lo - 1]));
-        x[l] = cmul(x[l], a[hi]);
+        if (lo) cmul_s(x[l], st.lo[2 * lo - 2], st.lo[2 * lo - 1]);
+        cmul_v(x[l], a[hi]);
       }
     }
     if (active) {

@@ -544,7 +544,7 @@ struct Simulator::Impl {
 
   // A pass of chain shape -- uncontrolled 2x2s on local bits 1, 2, ..., K in
   // order (optionally one more on bit 0 last), each followed by at most one
-  // phase layer that only touches combinations holding that step's bit --
+  // phase layer whose every mask holds that step's bit --
   // is rewritten in place as ChainSteps for the straight-line kernel.
   void to_chain(const PassOp *po, const DiagTerm *dt, const std::vector<PhaseCoef> &coef,
                 size_t nops, size_t nt, size_t off, PassLaunch &pl) {
@@ -563,7 +563,10 @@ struct Simulator::Impl {
       }
       if (st.empty() || st.back().phase >= 0) return;
       const uint32_t b = po[st.back().slot].lbit;
-      if (o.gmask || o.has_phi || o.cmask != (1u << b) || (o.bmask & ~(1u << b))) return;
+      // Every mask must hold the step's bit (then the phase of a combination
+      // with that bit set is exactly beta + its local linear terms, and 0
+      // without it); other common bits need no special handling.
+      if (o.gmask || o.has_phi || !((o.cmask >> b) & 1) || (o.bmask & ~(1u << b))) return;
       st.back().phase = int(j);
     }
     const int S = int(st.size());
