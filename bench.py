@@ -112,7 +112,8 @@ def oracle_threads() -> int:
 
 def suite(qm, C, emit):
     """Every configuration of the measurement plan, one JSON object each:
-    GHZ n=20, QFT scale sweep (fused vs unfused, alternated), brickwork,
+    GHZ n=20, QFT scale sweep (merged passes vs one launch per plan op vs the
+    unfused plan, alternated), brickwork,
     HEA adjoint gradients. Correctness is checked against the complex128
     oracle where it fits (n <= 26) and against invariants above."""
     import numpy as np
@@ -150,19 +151,23 @@ def suite(qm, C, emit):
           "max_abs_z_q": zq, "z0_z19": zz, "shots": 100000,
           "distinct_outcomes": sorted(set(shots)), "plan_ops": g.plan_size(True)})
 
-    # --- QFT scale sweep: fused vs unfused, alternated --------------------
+    # --- QFT scale sweep: fused plan with merged passes, the same plan one
+    # launch per op, and the unfused plan (merged), alternated -------------
     for n in (26, 30, 32, 33, 34):
         qft = C.to_circuit(n, C.qft(n))
         with qm.Simulator(n) as s:
             s.run(C.to_circuit(n, C.product_state_prep(n, seed=2024)))
             s.run(qft)
             s.synchronize()
-            times = {True: [], False: []}
-            for fuse in (True, False, True, False):
-                t0 = time.perf_counter()
-                s.run(qft, fuse=fuse)
-                s.synchronize()
-                times[fuse].append(time.perf_counter() - t0)
+            times = {"merged": [], "op_by_op": [], "unfused": []}
+            for _ in range(2):
+                for arm in times:
+                    s.set_pass_merging(arm != "op_by_op")
+                    t0 = time.perf_counter()
+                    s.run(qft, fuse=arm != "unfused")
+                    s.synchronize()
+                    times[arm].append(time.perf_counter() - t0)
+            s.set_pass_merging(True)
             s.reset()
             s.run(qft)
             norm0 = s.norm()
@@ -178,9 +183,11 @@ def suite(qm, C, emit):
                 ref = O.run(n, prep + C.qft(n))
                 O.set_threads(1)
                 diff = float(np.abs(got - ref).max())
-        tf, tu = min(times[True]), min(times[False])
-        emit({"config": f"qft_n{n}", "fused_s": tf, "unfused_s": tu, "speedup": tu / tf,
+        tf, to, tu = min(times["merged"]), min(times["op_by_op"]), min(times["unfused"])
+        emit({"config": f"qft_n{n}", "fused_s": tf, "fused_op_by_op_s": to, "unfused_s": tu,
+              "speedup_vs_op_by_op": to / tf, "speedup_vs_unfused": tu / tf,
               "plan_ops_fused": qft.plan_size(True), "plan_ops_unfused": qft.plan_size(False),
+              "launches_fused": max(qft.passes()) + 1,
               "hbm_GBps_algorithmic": qft_bytes(n) / tf / 1e9,
               "pct_hbm_peak": 100 * qft_bytes(n) / tf / 1e9 / HBM_PEAK_GBS,
               "qft_zero_norm": norm0, "qft_zero_max_abs_z_q": z0,
@@ -197,6 +204,9 @@ def suite(qm, C, emit):
                 s.reset()
                 s.run(c)
                 s.synchronize()
+            s.set_pass_merging(False)
+            t_op = best_of(run_bw, 1)
+            s.set_pass_merging(True)
             t = best_of(run_bw, 2)
             norm = s.norm()
             diff = None
@@ -212,7 +222,8 @@ def suite(qm, C, emit):
             bytes_ += (16 if k in ("Dense1q", "Dense1qGroup", "Blocked1q", "DiagonalLayer")
                        else 8 if k in ("Controlled1q", "Swap") else 4) * float(1 << n)
         emit({"config": f"brickwork_n{n}_d10", "gates": len(c), "plan_ops": c.plan_size(True),
-              "gpu_s": t, "hbm_GBps_algorithmic": bytes_ / t / 1e9, "norm": norm,
+              "launches": max(c.passes()) + 1, "gpu_s": t, "op_by_op_s": t_op,
+              "hbm_GBps_algorithmic": bytes_ / t / 1e9, "norm": norm,
               "max_abs_diff_vs_oracle": diff})
 
     # --- HEA depth 3 adjoint, TFIM ring -----------------------------------

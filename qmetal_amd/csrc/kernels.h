@@ -222,6 +222,39 @@ struct AdjArgs {
 void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,
                         double *out2);
 
+// Merged backward pass of the adjoint sweep: consecutive inverse gates whose
+// wires fit one thread's 16-amplitude tile, applied to psi and lam together.
+// Each op applies a 2x2 to both states (slot_asm section `sel`, pair mask
+// `pm`; the identity in threads whose base index lacks a bit of `gmask`);
+// type 1 first accumulates the rotation's overlap <lam|P|psi>, P = X, Y, Z
+// (`axis` 0, 1, 2) on tile-local bit `lbit`, in FP64 into slot `rslot`.
+constexpr int kAdjMaxRot = 8;
+struct AdjOp {
+  uint32_t type;
+  uint32_t lbit;
+  uint32_t sel;
+  uint32_t pm;
+  uint32_t axis;
+  uint32_t rslot;
+  uint32_t pad[2];
+  uint64_t gmask;
+  float m[8];
+};
+
+struct AdjPassArgs {
+  float2 *psi;
+  float2 *lam;
+  Layout lay;
+  const AdjOp *ops;
+  uint32_t nops;
+  uint32_t nrot;
+};
+
+// Grid of kRedBlocks workgroups; per-block FP64 partials of each rotation go
+// to `partials` (kAdjMaxRot * kRedBlocks * 2 doubles) and are reduced in a
+// fixed order into out[2 r], out[2 r + 1] (r < nrot).
+void launch_adj_pass(hipStream_t s, const AdjPassArgs &a, double *partials, double *out);
+
 // Sampling: FP64 sums of |a|^2 over 1024-amplitude blocks, an inclusive FP64
 // prefix over them, and per-shot resolution (splitmix64 stream, binary search
 // over blocks, in-block walk clamped to the last non-zero amplitude).

@@ -831,6 +831,122 @@ __device__ __forceinline__ void adj_pair(const AdjArgs &a, float2 &p0, float2 &p
   rot(a.m, l0, l1);
 }
 
+// <lam|P|psi> over the tile's pairs on local bit LB (P on that bit).
+template <int LB>
+__device__ __forceinline__ void tile_overlap(const float2 (&x)[16], const float2 (&y)[16],
+                                             uint32_t axis, double &re, double &im) {
+#pragma unroll
+  for (int l = 0; l < 16; l++) {
+    if (l & (1 << LB)) continue;
+    const int h = l | (1 << LB);
+    if (axis == 0) {         // X: conj(l0) p1 + conj(l1) p0
+      acc_cj(re, im, y[l], x[h], 1.0);
+      acc_cj(re, im, y[h], x[l], 1.0);
+    } else if (axis == 1) {  // Y (before i^ny): -conj(l0) p1 + conj(l1) p0
+      acc_cj(re, im, y[l], x[h], -1.0);
+      acc_cj(re, im, y[h], x[l], 1.0);
+    } else {                 // Z: conj(l0) p0 - conj(l1) p1
+      acc_cj(re, im, y[l], x[l], 1.0);
+      acc_cj(re, im, y[h], x[h], -1.0);
+    }
+  }
+}
+
+// Merged backward pass (AdjOp). Both tiles are written only by slot_asm
+// statements that every op runs (a branch around a write would make the
+// compiler hold second copies); the overlap only reads them. Wave sums of
+// each overlap go to per-wave LDS slots in program order, then to per-block
+// partials in a fixed order: results are run-to-run deterministic.
+template <bool NT>
+__global__ __launch_bounds__(kT) void k_adj_pass(AdjPassArgs a, double *part) {
+  __shared__ double acc[kAdjMaxRot][kT / 64][2];
+  for (int i = threadIdx.x; i < kAdjMaxRot * (kT / 64) * 2; i += kT) (&acc[0][0][0])[i] = 0.0;
+  __syncthreads();
+  uint64_t off[8];
+  offsets<4>(a.lay, off);
+  const const_ptr<AdjOp> ops = as_const(a.ops);
+  const TileAddr ta = tile_addr(a.lay);
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint64_t t0 = uint64_t(blockIdx.x) * kT; t0 < a.lay.nthreads; t0 += stride) {
+    const uint64_t ub = scatter(t0, a.lay) | a.lay.fixed;
+    const bool active = t0 + threadIdx.x < a.lay.nthreads;
+    const uint64_t base = ub | ta.lane_bits;
+    // Lanes past the end read the workgroup's first tile, add nothing to
+    // the overlaps and never store.
+    const uint32_t lb = active ? ta.lane_bytes : 0u;
+    float2 x[16], y[16];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const float4 v = ld4<NT>(pair_ptr(a.psi, ub | off[r], lb));
+      const float4 w = ld4<NT>(pair_ptr(a.lam, ub | off[r], lb));
+      x[2 * r] = make_float2(v.x, v.y);
+      x[2 * r + 1] = make_float2(v.z, v.w);
+      y[2 * r] = make_float2(w.x, w.y);
+      y[2 * r + 1] = make_float2(w.z, w.w);
+    }
+    for (uint32_t o = 0; o < a.nops; o++) {
+      if (ops[o].type == 1) {
+        double re = 0.0, im = 0.0;
+        const uint32_t axis = ops[o].axis;
+        switch (ops[o].lbit) {
+          case 0: tile_overlap<0>(x, y, axis, re, im); break;
+          case 1: tile_overlap<1>(x, y, axis, re, im); break;
+          case 2: tile_overlap<2>(x, y, axis, re, im); break;
+          default: tile_overlap<3>(x, y, axis, re, im); break;
+        }
+        re = wave_allsum(active ? re : 0.0);
+        im = wave_allsum(active ? im : 0.0);
+        if (lane == 0) {
+          const uint32_t r = ops[o].rslot;
+          acc[r][wave][0] += re;
+          acc[r][wave][1] += im;
+        }
+      }
+      const uint64_t gmask = ops[o].gmask;
+      const bool gon = (base & gmask) == gmask;
+      float m[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float mi = ops[o].m[i];
+        m[i] = gon ? mi : (i == 0 || i == 6 ? 1.0f : 0.0f);
+      }
+      const uint32_t sel = __builtin_amdgcn_readfirstlane(ops[o].sel);
+      const uint32_t pm = __builtin_amdgcn_readfirstlane(ops[o].pm);
+      slot_asm<4>(x, sel, pm, m);
+      slot_asm<4>(y, sel, pm, m);
+    }
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        st4<NT>(pair_ptr(a.psi, ub | off[r], ta.lane_bytes),
+                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
+        st4<NT>(pair_ptr(a.lam, ub | off[r], ta.lane_bytes),
+                make_float4(y[2 * r].x, y[2 * r].y, y[2 * r + 1].x, y[2 * r + 1].y));
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < a.nrot * 2) {
+    const uint32_t r = threadIdx.x >> 1, c = threadIdx.x & 1;
+    double t = 0.0;
+    for (int w = 0; w < kT / 64; w++) t += acc[r][w][c];
+    part[(size_t(r) * gridDim.x + blockIdx.x) * 2 + c] = t;
+  }
+}
+
+// Block r sums rotation r's per-block partials in a fixed order.
+__global__ __launch_bounds__(kT) void k_finish_rot(const double *part, int count, double *out) {
+  __shared__ double sh[kT / 64];
+  const size_t r = blockIdx.x;
+  for (int c = 0; c < 2; c++) {
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < count; k += kT) acc += part[(r * count + k) * 2 + c];
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[2 * r + c] = t;
+  }
+}
+
 __global__ __launch_bounds__(kT) void k_adjoint_rot(AdjArgs a, double *part) {
   __shared__ double sh[kT / 64];
   const uint64_t stride = uint64_t(gridDim.x) * kT;
@@ -1130,6 +1246,12 @@ void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,
                         double *out2) {
   k_adjoint_rot<<<kRedBlocks, kT, 0, s>>>(a, partials);
   k_finish<<<1, kT, 0, s>>>(partials, kRedBlocks, 2, out2);
+}
+
+void launch_adj_pass(hipStream_t s, const AdjPassArgs &a, double *partials, double *out) {
+  if (a.lay.streaming) k_adj_pass<true><<<kRedBlocks, kT, 0, s>>>(a, partials);
+  else k_adj_pass<false><<<kRedBlocks, kT, 0, s>>>(a, partials);
+  if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, kRedBlocks, out);
 }
 
 void launch_block_sums(hipStream_t s, const float2 *a, uint64_t dim, double *sums) {

@@ -158,6 +158,10 @@ struct Simulator::Impl {
   hipEvent_t staging_done = nullptr;
   bool staging_pending = false;
 
+  // Per-block FP64 partials of the merged adjoint passes.
+  double *adj_partials = nullptr;
+  size_t adj_partials_cap = 0;
+
   // Sampling scratch.
   double *cdf = nullptr;
   size_t cdf_cap = 0;
@@ -980,6 +984,7 @@ Simulator::~Simulator() {
   hipFree(s->results);
   hipFree(s->arena);
   hipFree(s->cdf);
+  hipFree(s->adj_partials);
   hipFree(s->chunks);
   hipFree(s->shots_dev);
   if (s->staging) hipHostFree(s->staging);
@@ -1270,10 +1275,171 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
     }
   }
   const size_t G = c.gates.size();
-  double *slots = s->result_slots(2 * (G + 1));
+  size_t nrot_total = 0;
+  for (const Gate &g : c.gates) nrot_total += is_rotation(g.kind) ? 1 : 0;
+  // Slots: [0, 2G) per gate (one launch per gate), [2G, 2G + 2) energy,
+  // then per rotation in sweep order (merged passes).
+  double *slots = s->result_slots(2 * (G + 1) + 2 * nrot_total);
   // E = Re<psi|lambda>
   s->pauli_sum(s->state, s->costate, Pauli{}, slots + 2 * G);
   s->dispatches++;
+
+  if (s->merge && s->n >= 6) {
+    // Backward sweep in merged passes: consecutive inverse gates whose
+    // target wires fit one 16-amplitude tile (wire 0 plus three register
+    // wires), at most kAdjMaxRot rotations each, applied to both states in
+    // one read and one write of each; overlaps reduced per pass.
+    struct APass {
+      std::vector<size_t> gates;  // gate indices, sweep order
+      std::vector<uint32_t> reg;
+      uint32_t nrot = 0;
+      size_t rot0 = 0;  // sweep index of its first rotation
+    };
+    auto targets = [](const Gate &g) {
+      std::vector<uint32_t> t;
+      if (g.nq == 1) t.push_back(g.q[0]);
+      else if (g.kind == K_SWAP) t = {g.q[0], g.q[1]};
+      else t.push_back(g.q[g.nq - 1]);
+      return t;
+    };
+    std::vector<APass> passes;
+    APass cur;
+    size_t sweep_rot = 0;
+    for (size_t k = G; k-- > 0;) {
+      const Gate &g = c.gates[k];
+      std::vector<uint32_t> reg = cur.reg;
+      for (uint32_t q : targets(g))
+        if (q != 0 && std::find(reg.begin(), reg.end(), q) == reg.end()) reg.push_back(q);
+      const uint32_t r = is_rotation(g.kind) ? 1u : 0u;
+      if (!cur.gates.empty() && (reg.size() > 3 || cur.nrot + r > uint32_t(kAdjMaxRot))) {
+        passes.push_back(std::move(cur));
+        cur = APass{};
+        reg.clear();
+        for (uint32_t q : targets(g))
+          if (q != 0 && std::find(reg.begin(), reg.end(), q) == reg.end()) reg.push_back(q);
+      }
+      if (cur.gates.empty()) cur.rot0 = sweep_rot;
+      cur.gates.push_back(k);
+      cur.reg = reg;
+      cur.nrot += r;
+      sweep_rot += r;
+    }
+    if (!cur.gates.empty()) passes.push_back(std::move(cur));
+    // Ops of every pass in one staging block.
+    std::vector<size_t> offs(passes.size());
+    size_t total = 0;
+    for (size_t i = 0; i < passes.size(); i++) {
+      offs[i] = total;
+      size_t nops = 0;
+      for (size_t k : passes[i].gates) nops += c.gates[k].kind == K_SWAP ? 3 : 1;
+      total += (nops * sizeof(AdjOp) + 255) & ~size_t(255);
+    }
+    s->stage(total);
+    std::vector<AdjPassArgs> args(passes.size());
+    for (size_t i = 0; i < passes.size(); i++) {
+      const APass &ps = passes[i];
+      LayoutBuilder lb{s->n};
+      for (uint32_t q : ps.reg) lb.add(q);
+      int nl = 0;
+      AdjPassArgs &a = args[i];
+      a.lay = lb.finish(4, &nl);
+      if (nl != 4) fail("internal: adjoint pass width");
+      AdjOp *ops = reinterpret_cast<AdjOp *>(s->staging + offs[i]);
+      uint32_t no = 0, rs = 0;
+      auto slot = [&](uint32_t target, const float *m8, std::vector<uint32_t> ctrls) -> AdjOp & {
+        AdjOp &o = ops[no++];
+        o = AdjOp{};
+        const int lt = lb.local_of(target);
+        if (lt < 0) fail("internal: adjoint target not local");
+        o.lbit = uint32_t(lt);
+        uint32_t cm = 0;
+        for (uint32_t q : ctrls) {
+          const int l = lb.local_of(q);
+          if (l >= 0) cm |= 1u << l;
+          else o.gmask |= 1ull << q;
+        }
+        std::memcpy(o.m, m8, sizeof(o.m));
+        const bool real = m8[1] == 0.0f && m8[3] == 0.0f && m8[5] == 0.0f && m8[7] == 0.0f;
+        o.sel = o.lbit + 4u * (cm ? 2u : (real ? 0u : 1u));
+        uint32_t pidx = 0;
+        for (uint32_t l = 0; l < 16; l++) {
+          if ((l >> o.lbit) & 1) continue;
+          if ((l & cm) == cm) o.pm |= 1u << pidx;
+          pidx++;
+        }
+        return o;
+      };
+      for (size_t k : ps.gates) {
+        const Gate &g = c.gates[k];
+        const Gate inv = inverse(g);
+        float m8[8];
+        if (g.nq == 1) {
+          cd m[4];
+          gate_matrix(inv.kind, inv.p, m);
+          narrow(m, m8);
+          AdjOp &o = slot(g.q[0], m8, {});
+          if (is_rotation(g.kind)) {
+            o.type = 1;
+            o.axis = g.kind == K_RX ? 0 : g.kind == K_RY ? 1 : 2;
+            o.rslot = rs++;
+          }
+          continue;
+        }
+        switch (g.kind) {
+          case K_CX:
+          case K_CY: {
+            cd m[4];
+            gate_matrix(g.kind == K_CX ? K_X : K_Y, nullptr, m);
+            narrow(m, m8);
+            slot(g.q[1], m8, {g.q[0]});
+            break;
+          }
+          case K_CZ:
+          case K_CP: {
+            const cd ph = inv.kind == K_CZ ? cd(-1, 0) : cd(std::cos(inv.p[0]), std::sin(inv.p[0]));
+            const cd m[4] = {1.0, 0.0, 0.0, ph};
+            narrow(m, m8);
+            slot(g.q[1], m8, {g.q[0]});
+            break;
+          }
+          case K_SWAP:
+            slot(g.q[1], kX, {g.q[0]});
+            slot(g.q[0], kX, {g.q[1]});
+            slot(g.q[1], kX, {g.q[0]});
+            break;
+          case K_CCX:
+            slot(g.q[2], kX, {g.q[0], g.q[1]});
+            break;
+          default:
+            fail(std::string("unsupported gate: ") + kind_name(g.kind));
+        }
+      }
+      a.psi = s->state;
+      a.lam = s->costate;
+      a.ops = reinterpret_cast<const AdjOp *>(s->arena + offs[i]);
+      a.nops = no;
+      a.nrot = ps.nrot;
+    }
+    s->upload(total);
+    s->grow(s->adj_partials, s->adj_partials_cap, size_t(kAdjMaxRot) * kRedBlocks * 2,
+            "hipMalloc(adjoint partials)");
+    double *rot_slots = slots + 2 * (G + 1);
+    for (size_t i = 0; i < passes.size(); i++) {
+      launch_adj_pass(s->stream, args[i], s->adj_partials, rot_slots + 2 * passes[i].rot0);
+      s->launched();
+      s->dispatches++;
+    }
+    const double *r = s->fetch_results(2 * (G + 1) + 2 * nrot_total);
+    std::vector<double> grad(G, 0.0);
+    size_t sw = 0;
+    for (size_t k = G; k-- > 0;) {
+      const Gate &g = c.gates[k];
+      if (!is_rotation(g.kind)) continue;
+      const double *o = r + 2 * (G + 1) + 2 * sw++;
+      grad[k] = g.kind == K_RY ? o[0] : o[1];
+    }
+    return {r[2 * G], grad};
+  }
 
   for (size_t k = G; k-- > 0;) {
     const Gate &g = c.gates[k];

@@ -150,3 +150,57 @@ def test_qft_zero_state_is_uniform_merged():
     assert abs(norm - 1) <= TOL1
     assert max(zs) <= TOL1
     assert all(abs(x - 1) <= TOL1 for x in xs)
+
+
+def adjoint_stream(n, count, seed):
+    """Rotations mixed with every other differentiable-circuit gate kind,
+    on wires drawn uniformly (controls land on local and base-index bits)."""
+    r = C.SplitMix64(seed)
+
+    def wires(k):
+        out = []
+        while len(out) < k:
+            q = r.next_u64() % n
+            if q not in out:
+                out.append(q)
+        return tuple(out)
+
+    gates = [(C.H, (q,), ()) for q in range(n)]
+    for _ in range(count):
+        u = r.next_u64() % 10
+        if u < 4:
+            gates.append(((C.RX, C.RY, C.RZ)[r.next_u64() % 3], wires(1), (r.uniform(-3.2, 3.2),)))
+        elif u < 5:
+            gates.append((ONE[r.next_u64() % len(ONE)], wires(1), ()))
+        elif u < 8:
+            gates.append(((C.CX, C.CY, C.CZ)[r.next_u64() % 3], wires(2), ()))
+        elif u < 9:
+            gates.append((C.SWAP, wires(2), ()))
+        else:
+            gates.append((C.CCX, wires(3), ()))
+    return gates
+
+
+def test_merged_adjoint_against_oracle_and_per_gate_sweep():
+    """The backward sweep in merged passes (several inverse gates and rotation
+    overlaps per read of both states) against the complex128 adjoint and the
+    one-launch-per-gate sweep."""
+    for n in (6, 8, 11):
+        terms = C.tfim_ring_hamiltonian(n) + [(0.3, 0, 1 << (n // 2), 0)]
+        h = C.to_hamiltonian(terms)
+        for seed in range(3):
+            gates = adjoint_stream(n, 60, 77 * n + seed)
+            c = C.to_circuit(n, gates)
+            with qm.Simulator(n) as s:
+                e_m, g_m = s.energy_and_gradient(c, h)
+                s.set_pass_merging(False)
+                e_p, g_p = s.energy_and_gradient(c, h)
+            e_ref, g_ref = O.adjoint(n, gates, terms)
+            d_ref = float(np.abs(np.array(g_m) - g_ref).max())
+            d_per = float(np.abs(np.array(g_m) - np.array(g_p)).max())
+            print(f"n={n} seed={seed}: |merged - oracle| {d_ref:.2e}, |merged - per-gate| {d_per:.2e}")
+            assert abs(e_m - e_ref) <= 1e-5 and abs(e_m - e_p) <= 1e-5
+            assert d_ref <= 1e-5 and d_per <= 1e-5
+            for k, (kind, _, _) in enumerate(gates):
+                if kind not in (C.RX, C.RY, C.RZ):
+                    assert g_m[k] == 0.0
