@@ -30,9 +30,9 @@ struct Layout {
   uint64_t nthreads;
   uint64_t fixed;
   uint32_t nins;
-  uint8_t ins[12];
+  uint8_t ins[16];
   uint8_t rpos[8];
-  uint64_t hm[12];
+  uint64_t hm[16];
   // 1 when positions 1..3 are thread-index bits: then eight neighbouring lanes
   // cover one whole 128-byte line and the accesses can bypass the caches
   // (nontemporal); otherwise a line is split across load instructions and
@@ -165,6 +165,28 @@ struct ChainArgs {
   uint32_t nlane;
 };
 
+// Wide chain pass: six chain steps on wires w1..w6 (pos[0..5]) over a
+// workgroup tile of 2^13 amplitudes. Each thread holds 32 amplitudes: bit 0
+// and tile-local bits 1..4 = w1..w4; lane bits 0..5 of the thread index sit
+// on positions lanepos[0..5] and the two wave bits on w5, w6. Steps 0-3 act
+// on local bits 1..4; an LDS exchange then swaps local bits 1, 2 with the
+// wave bits, and steps 4, 5 act on local bits 1, 2 (now w5, w6) while the
+// wave bits stand for w1, w2. `lay` enumerates the tiles (nthreads = tiles,
+// ins = the 13 tile positions and the fixed bits). Steps 0-3 are folded
+// against the first arrangement, steps 4-5 against the second; base-index
+// terms as in ChainArgs (k = step).
+struct WideArgs {
+  float2 *a;
+  Layout lay;
+  uint8_t lanepos[6];
+  uint8_t pos[6];
+  const ChainStep *steps;
+  const DiagTerm *uni;
+  const DiagTerm *lane;
+  uint32_t nuni;
+  uint32_t nlane;
+};
+
 void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim);
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim);
 void launch_gather(hipStream_t s, int nl, const GatherArgs &g);
@@ -175,6 +197,7 @@ void launch_blocked(hipStream_t s, const BlockedArgs &b);
 void launch_pass(hipStream_t s, int nl, const PassArgs &p);
 // nl = 5 with k = 4, or nl = 4 with k in 1..3; `zero`: a last step on bit 0.
 void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const ChainArgs &c);
+void launch_wide(hipStream_t s, bool real, const WideArgs &w);
 
 // FP64 reductions. Each writes per-block partials into `partials` (capacity
 // kRedBlocks doubles or double2) and then reduces them in a fixed order into

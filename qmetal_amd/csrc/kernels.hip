@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace qmamd {
@@ -523,6 +525,47 @@ __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
   }
 }
 
+// One chain step on tile-local bit BIT: the 2x2 (m) on all pairs, then the
+// phase of the combinations holding BIT, e * lo[low two other bits] *
+// hi[the other bits above them] (other local bits ascending).
+template <int NL, int BIT, bool REAL>
+__device__ __forceinline__ void chain_step(float2 (&x)[1 << NL],
+                                           const ChainStep __attribute__((address_space(4))) &st,
+                                           float2 e) {
+  {
+    float m[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) m[j] = st.m[j];
+#pragma unroll
+    for (int l = 0; l < (1 << NL); l++) {
+      if (l & (1 << BIT)) continue;
+      if constexpr (REAL) bfly_real_s(m[0], m[2], m[4], m[6], x[l], x[l | (1 << BIT)]);
+      else bfly_cplx_s(m, x[l], x[l | (1 << BIT)]);
+    }
+  }
+  float2 a[4];
+  a[0] = e;
+#pragma unroll
+  for (int h = 1; h < 4; h++) a[h] = cmul(e, make_float2(st.hi[2 * h - 2], st.hi[2 * h - 1]));
+#pragma unroll
+  for (int l = 0; l < (1 << NL); l++) {
+    if (!(l & (1 << BIT))) continue;
+    const int c = (l & ((1 << BIT) - 1)) | ((l >> (BIT + 1)) << BIT);
+    const int lo = c & 3, hi = c >> 2;
+    if (lo) cmul_s(x[l], st.lo[2 * lo - 2], st.lo[2 * lo - 1]);
+    cmul_v(x[l], a[hi]);
+  }
+}
+
+// Step index i read through an opaque register after the tile's last update,
+// so the compiler cannot hoist every step's scalar constants to the top (it
+// would spill SGPRs).
+__device__ __forceinline__ uint32_t opaque_step(int i, float dep) {
+  uint32_t si = uint32_t(i);
+  asm volatile("" : "+s"(si) : "v"(dep));
+  return si;
+}
+
 // Chain-shaped merged pass (ChainStep): every step's local bit, and which
 // combinations its phase touches, are compile-time, so the step sequence is
 // straight-line code. The loads of the workgroup-uniform terms are issued
@@ -589,44 +632,121 @@ __global__ __launch_bounds__(kT) void k_chain(ChainArgs p) {
     }
 #pragma unroll
     for (int i = 0; i < S; i++) {
-      const int bit = i < K ? i + 1 : 0;
-      // The step's constants are read through an index the compiler cannot
-      // see before the previous step has updated the tile; otherwise it
-      // hoists every step's scalar loads to the top and spills SGPRs.
-      uint32_t si = uint32_t(i);
-      asm volatile("" : "+s"(si) : "v"(x[0].x));
-      const ChainStep __attribute__((address_space(4))) &st = steps[si];
-      {
-        float m[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) m[j] = st.m[j];
-#pragma unroll
-        for (int l = 0; l < (1 << NL); l++) {
-          if (l & (1 << bit)) continue;
-          if constexpr (REAL) bfly_real_s(m[0], m[2], m[4], m[6], x[l], x[l | (1 << bit)]);
-          else bfly_cplx_s(m, x[l], x[l | (1 << bit)]);
-        }
-      }
-      // Phase on the combinations holding the step's bit: the other local
-      // bits (ascending) form c; its low two bits pick lo, the rest hi.
-      float2 a[4];
-      a[0] = e[i];
-#pragma unroll
-      for (int h = 1; h < 4; h++)
-        a[h] = cmul(e[i], make_float2(st.hi[2 * h - 2], st.hi[2 * h - 1]));
-#pragma unroll
-      for (int l = 0; l < (1 << NL); l++) {
-        if (!(l & (1 << bit))) continue;
-        const int c = (l & ((1 << bit) - 1)) | ((l >> (bit + 1)) << bit);
-        const int lo = c & 3, hi = c >> 2;
-        if (lo) cmul_s(x[l], st.lo[2 * lo - 2], st.lo[2 * lo - 1]);
-        cmul_v(x[l], a[hi]);
+      const ChainStep __attribute__((address_space(4))) &st = steps[opaque_step(i, x[0].x)];
+      if (i < K) {
+        if constexpr (K >= 1) if (i == 0) chain_step<NL, 1, REAL>(x, st, e[i]);
+        if constexpr (K >= 2) if (i == 1) chain_step<NL, 2, REAL>(x, st, e[i]);
+        if constexpr (K >= 3) if (i == 2) chain_step<NL, 3, REAL>(x, st, e[i]);
+        if constexpr (K >= 4) if (i == 3) chain_step<NL, 4, REAL>(x, st, e[i]);
+      } else {
+        chain_step<NL, 0, REAL>(x, st, e[i]);
       }
     }
     if (active) {
 #pragma unroll
       for (int r = 0; r < (1 << (NL - 1)); r++)
         st4<NT>(pair_ptr(p.a, ub | off[r], ta.lane_bytes),
+                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
+    }
+  }
+}
+
+// Wide chain pass (WideArgs). One workgroup tile per iteration; the exchange
+// between the two arrangements goes through 16 KB of LDS in four rounds (one
+// per value of local bits 3, 4).
+template <bool REAL>
+__global__ __launch_bounds__(kT) void k_wide(WideArgs p) {
+  __shared__ float4 xch[4][4][64];  // [writer's wave][slot][lane]
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const const_ptr<ChainStep> steps = as_const(p.steps);
+  const const_ptr<DiagTerm> uni = as_const(p.uni);
+  const const_ptr<DiagTerm> lane_terms = as_const(p.lane);
+  uint64_t lanebits = 0;
+#pragma unroll
+  for (int j = 0; j < 6; j++)
+    if ((lane >> j) & 1) lanebits |= 1ull << p.lanepos[j];
+  // Wave bits: w5, w6 before the exchange, w1, w2 after.
+  const uint64_t wave_a = ((wave & 1) ? 1ull << p.pos[4] : 0) | ((wave & 2) ? 1ull << p.pos[5] : 0);
+  const uint64_t wave_b = ((wave & 1) ? 1ull << p.pos[0] : 0) | ((wave & 2) ? 1ull << p.pos[1] : 0);
+  const uint32_t lane_bytes = uint32_t(lanebits) * 8u;
+  for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
+    const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
+    const uint64_t base_a = ub | lanebits | wave_a, base_b = ub | lanebits | wave_b;
+    double bu[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) bu[i] = 0.0;
+    for (uint32_t k0 = 0; k0 < p.nuni; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      const uint32_t kc = k < p.nuni ? k : p.nuni - 1;
+      const uint64_t mask = uni[kc].mask;
+      const double add = (k < p.nuni && (ub & mask) == mask) ? double(uni[kc].ang) : 0.0;
+      const uint32_t st = uni[kc].k;
+#pragma unroll
+      for (int i = 0; i < 6; i++) bu[i] += st == uint32_t(i) ? add : 0.0;
+    }
+    float2 x[32];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      uint64_t o = ub | wave_a;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if ((r >> k) & 1) o |= 1ull << p.pos[k];
+      const float4 v = ld4<true>(pair_ptr(p.a, o, lane_bytes));
+      x[2 * r] = make_float2(v.x, v.y);
+      x[2 * r + 1] = make_float2(v.z, v.w);
+    }
+    double b[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) b[i] = wave_allsum(bu[i]) + steps[i].beta0;
+    for (uint32_t k = 0; k < p.nlane; k++) {
+      const uint64_t mask = lane_terms[k].mask;
+      const uint32_t st = lane_terms[k].k;
+      const uint64_t bs = st < 4 ? base_a : base_b;
+      const double add = (bs & mask) == mask ? double(lane_terms[k].ang) : 0.0;
+#pragma unroll
+      for (int i = 0; i < 6; i++)
+        if (st == uint32_t(i)) b[i] += add;
+    }
+    float2 e[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      float sn, cs;
+      turn_sincos(b[i], sn, cs);
+      e[i] = make_float2(cs, sn);
+    }
+    chain_step<5, 1, REAL>(x, steps[opaque_step(0, x[0].x)], e[0]);
+    chain_step<5, 2, REAL>(x, steps[opaque_step(1, x[0].x)], e[1]);
+    chain_step<5, 3, REAL>(x, steps[opaque_step(2, x[0].x)], e[2]);
+    chain_step<5, 4, REAL>(x, steps[opaque_step(3, x[0].x)], e[3]);
+    // Exchange: local bits 1, 2 (w1, w2) <-> wave bits (w5, w6).
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        const int l = (c << 3) | (a << 1);
+        xch[wave][a][lane] = make_float4(x[l].x, x[l].y, x[l + 1].x, x[l + 1].y);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int d = 0; d < 4; d++) {
+        const int l = (c << 3) | (d << 1);
+        const float4 v = xch[d][wave][lane];
+        x[l] = make_float2(v.x, v.y);
+        x[l + 1] = make_float2(v.z, v.w);
+      }
+      __syncthreads();
+    }
+    chain_step<5, 1, REAL>(x, steps[opaque_step(4, x[0].x)], e[4]);
+    chain_step<5, 2, REAL>(x, steps[opaque_step(5, x[0].x)], e[5]);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      // Local bits 1, 2 now w5, w6; 3, 4 still w3, w4.
+      uint64_t o = ub | wave_b;
+      if (r & 1) o |= 1ull << p.pos[4];
+      if (r & 2) o |= 1ull << p.pos[5];
+      if (r & 4) o |= 1ull << p.pos[2];
+      if (r & 8) o |= 1ull << p.pos[3];
+      st4<true>(pair_ptr(p.a, o, lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
     }
   }
@@ -1214,6 +1334,12 @@ void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const Chai
       default: break;
     }
   }
+}
+
+void launch_wide(hipStream_t s, bool real, const WideArgs &w) {
+  const unsigned grid = unsigned(std::min<uint64_t>(w.lay.nthreads, kMaxGrid));
+  if (real) k_wide<true><<<grid, kT, 0, s>>>(w);
+  else k_wide<false><<<grid, kT, 0, s>>>(w);
 }
 
 void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d) {

@@ -88,6 +88,35 @@ double op_alone_bytes(uint32_t n, const PlanOp &op) {
   return pass_bytes(n, pass_fixed(op));
 }
 
+uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) {
+  wires.clear();
+  if (p.n < kWideMinQubits) return 0;
+  uint32_t j = k;
+  while (j < p.ops.size() && wires.size() < kWideSteps) {
+    const PlanOp &op = p.ops[j];
+    if (op.kind != OpKind::Dense1q) break;
+    const uint32_t w = op.q[0];
+    if (w < kWideMinWire || std::find(wires.begin(), wires.end(), w) != wires.end()) break;
+    wires.push_back(w);
+    j++;
+    if (j < p.ops.size() && p.ops[j].kind == OpKind::Diagonal && !p.ops[j].masks.empty()) {
+      const PlanOp &d = p.ops[j];
+      bool ok = zero_phase(d.global);
+      for (uint64_t m : d.masks) {
+        const int wt = __builtin_popcountll(m);
+        if (wt < 1 || wt > 2 || !((m >> w) & 1)) ok = false;
+      }
+      if (!ok) break;
+      j++;
+    }
+  }
+  if (wires.size() < kWideSteps) {
+    wires.clear();
+    return 0;
+  }
+  return j - k;
+}
+
 std::vector<Pass> schedule_passes(const Plan &p) {
   std::vector<Pass> out;
   const uint32_t n = p.n;
@@ -113,9 +142,20 @@ std::vector<Pass> schedule_passes(const Plan &p) {
     out.push_back(std::move(cur));
     cur = Pass{};
   };
+  std::vector<uint32_t> wide;
   for (uint32_t k = 0; k < p.ops.size(); k++) {
     const PlanOp &op = p.ops[k];
     if (op.kind == OpKind::Diagonal && op.masks.empty()) continue;  // no dispatch
+    if (const uint32_t used = wide_chain_at(p, k, wide)) {
+      flush();
+      Pass s;
+      s.merged = s.wide = true;
+      for (uint32_t j = k; j < k + used; j++) s.ops.push_back(j);
+      s.reg = wide;
+      out.push_back(std::move(s));
+      k += used - 1;
+      continue;
+    }
     if (!pass_mergeable(op)) {
       flush();
       Pass s;

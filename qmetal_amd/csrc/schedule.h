@@ -30,10 +30,25 @@ constexpr uint32_t kPassMaxOps = 16;    // plan ops per merged pass
 
 struct Pass {
   bool merged = false;         // false: ops[0] alone through its own kernel
+  bool wide = false;           // merged: six chain steps over a workgroup tile
   std::vector<uint32_t> ops;   // plan-op indices, in plan order
-  std::vector<uint32_t> reg;   // register wires (merged), first use order
+  std::vector<uint32_t> reg;   // register wires (merged), first use order;
+                               // wide: the six step wires in order
   uint64_t fixed = 0;          // fixed-one bits (merged)
 };
+
+// Wide passes: six chain steps -- each an uncontrolled Dense1q on a distinct
+// wire at or above kWideMinWire, optionally followed by a phase layer whose
+// every mask (weight 1 or 2) holds that wire, with no global phase -- over a
+// workgroup tile of 2^13 amplitudes whose six lowest free positions are the
+// lanes of a 1 KB run. States below kWideMinQubits use ordinary passes.
+constexpr uint32_t kWideSteps = 6;
+constexpr uint32_t kWideMinWire = 7;
+constexpr uint32_t kWideMinQubits = 16;
+
+// Number of plan ops from `k` that form a wide pass (0 if none); `wires`
+// gets the step wires.
+uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires);
 
 // Whether plan op `op` can join a merged pass.
 bool pass_mergeable(const PlanOp &op);

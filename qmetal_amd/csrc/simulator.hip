@@ -371,6 +371,9 @@ struct Simulator::Impl {
     bool chain_zero = false;
     bool chain_real = false;
     ChainArgs c{};
+    // Wide chain pass (six steps over a workgroup tile).
+    bool wide = false;
+    WideArgs w{};
   };
 
   // Local coefficients of a phase op of a merged pass (turns).
@@ -647,13 +650,124 @@ struct Simulator::Impl {
     pl.c.nlane = uint32_t(nlane);
   }
 
+  // Wide chain pass: the six steps' constants and base-index terms, folded
+  // against the tile arrangement each step runs in (WideArgs).
+  void prep_wide(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
+    if (ps.reg.size() != kWideSteps || ps.fixed) fail("internal: wide pass shape");
+    const uint32_t *w = ps.reg.data();
+    WideArgs &wa = pl.w;
+    wa = WideArgs{};
+    uint64_t tile = 1ull;
+    for (int i = 0; i < 6; i++) {
+      wa.pos[i] = uint8_t(w[i]);
+      tile |= 1ull << w[i];
+    }
+    for (uint32_t q = 1, got = 0; got < 6; q++) {
+      if ((tile >> q) & 1) fail("internal: wide pass lanes");
+      wa.lanepos[got++] = uint8_t(q);
+      tile |= 1ull << q;
+    }
+    // Tiles: insert zeros at the 13 tile positions, lowest first.
+    Layout &L = wa.lay;
+    L = Layout{};
+    for (uint32_t q = 0; q < 64; q++)
+      if ((tile >> q) & 1) {
+        L.ins[L.nins] = uint8_t(q);
+        L.hm[L.nins] = ~((uint64_t(1) << q) - 1);
+        L.nins++;
+      }
+    L.nthreads = dim >> L.nins;
+    L.streaming = 1;
+    // Arrangements: local bit -> wire (0 = physical bit 0), and the
+    // thread-varying bits (lanes and the two wave bits).
+    uint64_t lanes = 0;
+    for (int j = 0; j < 6; j++) lanes |= 1ull << wa.lanepos[j];
+    const uint32_t loc_a[5] = {0, w[0], w[1], w[2], w[3]};
+    const uint32_t loc_b[5] = {0, w[4], w[5], w[2], w[3]};
+    const uint64_t var_a = lanes | (1ull << w[4]) | (1ull << w[5]);
+    const uint64_t var_b = lanes | (1ull << w[0]) | (1ull << w[1]);
+    ChainStep *cs = reinterpret_cast<ChainStep *>(staging + off);
+    std::vector<DiagTerm> uni, lan;
+    int step = -1;
+    for (uint32_t k : ps.ops) {
+      const PlanOp &op = p.ops[k];
+      if (op.kind == OpKind::Dense1q) {
+        step++;
+        cs[step] = ChainStep{};
+        float m8[8];
+        narrow(op.m, m8);
+        std::memcpy(cs[step].m, m8, sizeof(m8));
+        for (int cc = 0; cc < 6; cc++) cs[step].lo[cc] = cs[step].hi[cc] = (cc % 2 == 0) ? 1.0f : 0.0f;
+        continue;
+      }
+      // The step's phase layer.
+      const bool second = step >= 4;
+      const uint32_t *loc = second ? loc_b : loc_a;
+      const int b = second ? step - 3 : step + 1;  // the step's local bit
+      auto local_of = [&](uint32_t q) {
+        for (int l = 0; l < 5; l++)
+          if (loc[l] == q) return l;
+        return -1;
+      };
+      double constant = 0.0, beta0[kPassMaxNL] = {}, gamma[kPassMaxNL][kPassMaxNL] = {};
+      std::vector<DiagTerm> t(op.masks.size());
+      uint32_t nuniform = 0;
+      const size_t nt = fold_terms(op, 0, second ? var_b : var_a, local_of, constant, beta0, gamma,
+                                   t.data(), nuniform);
+      if (constant != 0.0) fail("internal: wide pass phase constant");
+      for (size_t u = 0; u < nt; u++) {
+        if (t[u].kind != 2 || t[u].k != uint32_t(b)) fail("internal: wide pass phase term");
+        t[u].k = uint32_t(step);
+        (u < nuniform ? uni : lan).push_back(t[u]);
+      }
+      ChainStep &c = cs[step];
+      c.beta0 = beta0[b];
+      int other[4], no = 0;
+      for (int q = 0; q < 5; q++)
+        if (q != b) other[no++] = q;
+      for (int cc = 1; cc < 4; cc++) {
+        double tl = 0.0, th = 0.0;
+        for (int j = 0; j < 2; j++)
+          if ((cc >> j) & 1) {
+            tl += gamma[std::min(b, other[j])][std::max(b, other[j])];
+            th += gamma[std::min(b, other[2 + j])][std::max(b, other[2 + j])];
+          }
+        tl -= std::rint(tl);
+        th -= std::rint(th);
+        c.lo[2 * cc - 2] = float(std::cos(kTwoPi * tl));
+        c.lo[2 * cc - 1] = float(std::sin(kTwoPi * tl));
+        c.hi[2 * cc - 2] = float(std::cos(kTwoPi * th));
+        c.hi[2 * cc - 1] = float(std::sin(kTwoPi * th));
+      }
+    }
+    if (step != 5) fail("internal: wide pass steps");
+    bool real = true;
+    for (int i = 0; i < 6; i++)
+      if (cs[i].m[1] != 0.0f || cs[i].m[3] != 0.0f || cs[i].m[5] != 0.0f || cs[i].m[7] != 0.0f)
+        real = false;
+    DiagTerm *tu = reinterpret_cast<DiagTerm *>(staging + off + 6 * sizeof(ChainStep));
+    std::copy(uni.begin(), uni.end(), tu);
+    std::copy(lan.begin(), lan.end(), tu + uni.size());
+    wa.a = state;
+    wa.steps = reinterpret_cast<const ChainStep *>(arena + off);
+    wa.uni = reinterpret_cast<const DiagTerm *>(arena + off + 6 * sizeof(ChainStep));
+    wa.lane = wa.uni + uni.size();
+    wa.nuni = uint32_t(uni.size());
+    wa.nlane = uint32_t(lan.size());
+    pl.wide = true;
+    pl.chain_real = real;
+    pl.nl = 5;
+    pl.bytes = pass_bytes(n, 0);
+  }
+
   void pass_launch(const PassLaunch &pl) {
     hipEvent_t a = nullptr;
     if (timing) {
       a = take_event();
       check(hipEventRecord(a, stream), "hipEventRecord");
     }
-    if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);
+    if (pl.wide) launch_wide(stream, pl.chain_real, pl.w);
+    else if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);
     else launch_pass(stream, pl.nl, pl.a);
     launched();
     dispatches++;
@@ -1047,6 +1161,10 @@ void Simulator::run_plan(const Plan &p) {
   if (total) {
     s->stage(total);
     for (size_t k = 0; k < passes.size(); k++) {
+      if (passes[k].wide) {
+        s->prep_wide(p, passes[k], offs[k], merged[k]);
+        continue;
+      }
       if (passes[k].merged) {
         s->prep_pass(p, passes[k], offs[k], merged[k]);
         continue;

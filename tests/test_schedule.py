@@ -49,15 +49,28 @@ def check_grouping(circ, options=None):
     for ops in groups.values():
         if len(ops) > 1:
             wires = set().union(*(moving_wires(o) for o in ops))
-            assert len(wires) <= MAX_REG, wires
             assert all(o["kind"] != "Blocked1q" for o in ops)
+            if len(wires) > MAX_REG:
+                # a wide pass: six Dense1q steps on wires >= 7, each followed
+                # at most by a phase layer whose masks all hold its wire
+                assert len(wires) == 6 and min(wires) >= 7 and circ.num_qubits >= 16, wires
+                steps = [o for o in ops if o["kind"] == "Dense1q"]
+                assert len(steps) == 6
+                last = None
+                for o in ops:
+                    if o["kind"] == "Dense1q":
+                        last = o["qubits"][0]
+                    else:
+                        assert o["kind"] == "DiagonalLayer"
+                        assert all((m >> last) & 1 for m in o["masks"])
     return max(seen) + 1 if seen else 0
 
 
 def test_qft_pass_counts():
-    """QFT: H on wire j then the CP ladder onto j, for j = n-1 .. 0. Four H's
-    and their ladders share a pass (the last pass also takes wire 0)."""
-    for n, want in ((12, 3), (26, 7), (30, 8), (32, 8), (33, 8), (34, 9)):
+    """QFT: H on wire j then the CP ladder onto j, for j = n-1 .. 0. From 16
+    qubits six H's and their ladders on wires >= 7 share a wide pass; the rest
+    go four to a pass (the last pass also takes wire 0)."""
+    for n, want in ((12, 3), (16, 4), (26, 5), (30, 6), (32, 6), (33, 6), (34, 7)):
         c = C.to_circuit(n, C.qft(n))
         assert c.plan_size(True) == 2 * n - 1
         assert check_grouping(c) == want, n
