@@ -190,8 +190,9 @@ _U3 = 18
 
 PLAN_KINDS = ("Dense1q", "Dense1qGroup", "Blocked1q", "Controlled1q", "DiagonalLayer",
               "Swap", "CCX")
-# Timing records: plan-op kinds, then a merged pass of several plan ops.
-TIMING_KINDS = PLAN_KINDS + ("Pass",)
+# Timing records: plan-op kinds, then merged passes of several plan ops
+# (generic pass kernel, chain kernel, wide chain kernel).
+TIMING_KINDS = PLAN_KINDS + ("Pass", "ChainPass", "WidePass")
 
 
 def version() -> tuple[int, int, int]:

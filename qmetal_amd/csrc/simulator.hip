@@ -45,8 +45,11 @@ constexpr int kLocalBits = 4;
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
-// Timing record kind of a merged pass (plan-op kinds are 0..6).
+// Timing record kinds of merged passes (plan-op kinds are 0..6): the
+// generic pass kernel, the chain kernel, the wide chain kernel.
 constexpr uint32_t kTimingPass = 7;
+constexpr uint32_t kTimingChain = 8;
+constexpr uint32_t kTimingWide = 9;
 
 int current_device() {
   int d = 0;
@@ -774,7 +777,8 @@ struct Simulator::Impl {
     if (timing) {
       hipEvent_t b = take_event();
       check(hipEventRecord(b, stream), "hipEventRecord");
-      timings.push_back(OpTiming{kTimingPass, pl.bytes, a, b});
+      const uint32_t kind = pl.wide ? kTimingWide : pl.chain ? kTimingChain : kTimingPass;
+      timings.push_back(OpTiming{kind, pl.bytes, a, b});
     }
   }
 
