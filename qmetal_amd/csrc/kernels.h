@@ -187,6 +187,27 @@ struct WideArgs {
   uint32_t nlane;
 };
 
+// Low chain pass: S chain steps on distinct wires 1..10, then optionally one
+// on wire 0, over per-wave tiles of 2^11 contiguous amplitudes (bits 0..10;
+// workgroup = one wave). The tile is loaded and stored coalesced (lanes on
+// bits 1..6, register bits on 7..10, arrangement 0) and permuted through LDS
+// into arrangement 1 + a for steps 4a .. 4a + 3, whose wires are register bits
+// (tile-local 1..4); the wire-0 step runs in the last arrangement. `lanepos`
+// and `regpos` give each arrangement's bit positions. Steps are folded
+// against their arrangement; base-index terms as in ChainArgs (k = step).
+constexpr int kLowMaxArr = 4;
+struct LowArgs {
+  float2 *a;
+  Layout lay;  // tiles: ins = bits 0..10 and the fixed bits
+  uint8_t lanepos[kLowMaxArr][6];
+  uint8_t regpos[kLowMaxArr][4];
+  const ChainStep *steps;
+  const DiagTerm *uni;
+  const DiagTerm *lane;
+  uint32_t nuni;
+  uint32_t nlane;
+};
+
 void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim);
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim);
 void launch_gather(hipStream_t s, int nl, const GatherArgs &g);
@@ -198,6 +219,8 @@ void launch_pass(hipStream_t s, int nl, const PassArgs &p);
 // nl = 5 with k = 4, or nl = 4 with k in 1..3; `zero`: a last step on bit 0.
 void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const ChainArgs &c);
 void launch_wide(hipStream_t s, bool real, const WideArgs &w);
+// s in 5..10 steps on wires 1..10, `zero`: a last step on wire 0.
+void launch_low(hipStream_t s, int steps, bool zero, bool real, const LowArgs &l);
 
 // FP64 reductions. Each writes per-block partials into `partials` (capacity
 // kRedBlocks doubles or double2) and then reduces them in a fixed order into

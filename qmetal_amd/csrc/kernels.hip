@@ -752,6 +752,140 @@ __global__ __launch_bounds__(kT) void k_wide(WideArgs p) {
   }
 }
 
+// LDS slot of tile index i (float4 units, 10 bits): the low four bits are
+// xored with the higher ones so that lanes stepping through high bits still
+// spread over the banks (a bijection: the high bits are kept).
+__device__ __forceinline__ uint32_t low_swz(uint32_t i) {
+  return i ^ (((i >> 4) ^ (i >> 8)) & 0xFu);
+}
+
+// Tile index (float4 units: tile bit p is index bit p - 1) of this lane's
+// lane bits in arrangement `a`.
+__device__ __forceinline__ uint32_t low_lane_index(const LowArgs &p, int a, uint32_t lane) {
+  uint32_t i = 0;
+#pragma unroll
+  for (int j = 0; j < 6; j++)
+    if ((lane >> j) & 1) i |= 1u << (p.lanepos[a][j] - 1);
+  return i;
+}
+
+__device__ __forceinline__ uint32_t low_reg_index(const LowArgs &p, int a, int r) {
+  uint32_t i = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if ((r >> k) & 1) i |= 1u << (p.regpos[a][k] - 1);
+  return i;
+}
+
+// Moves the tile from arrangement FROM to arrangement TO through LDS. The
+// swizzle is linear over GF(2) and lane and register bits are disjoint, so a
+// slot is the lane's swizzled index xor the register's (workgroup-uniform).
+template <int FROM, int TO>
+__device__ __forceinline__ void low_permute(const LowArgs &p, float4 *lds, float2 (&x)[32],
+                                            uint32_t lane) {
+  const uint32_t lf = low_swz(low_lane_index(p, FROM, lane));
+  const uint32_t lt = low_swz(low_lane_index(p, TO, lane));
+  __syncthreads();  // earlier reads of the buffer are done
+#pragma unroll
+  for (int r = 0; r < 16; r++)
+    lds[lf ^ low_swz(low_reg_index(p, FROM, r))] =
+        make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const float4 v = lds[lt ^ low_swz(low_reg_index(p, TO, r))];
+    x[2 * r] = make_float2(v.x, v.y);
+    x[2 * r + 1] = make_float2(v.z, v.w);
+  }
+}
+
+// Low chain pass (LowArgs); one wave per workgroup, 16 KB of LDS per wave.
+template <int S, bool Z, bool REAL>
+__global__ __launch_bounds__(64) void k_low(LowArgs p) {
+  constexpr int T = S + (Z ? 1 : 0);
+  constexpr int A = (S + 3) / 4;  // step arrangements
+  static_assert(S >= 1 && A + 1 <= kLowMaxArr, "low chain shape");
+  __shared__ float4 lds[1024];
+  const uint32_t lane = threadIdx.x;
+  const const_ptr<ChainStep> steps = as_const(p.steps);
+  const const_ptr<DiagTerm> uni = as_const(p.uni);
+  const const_ptr<DiagTerm> lane_terms = as_const(p.lane);
+  // Lane terms read only lane bits, which are the same for every tile.
+  double bl[T];
+#pragma unroll
+  for (int i = 0; i < T; i++) bl[i] = steps[i].beta0;
+  for (uint32_t k = 0; k < p.nlane; k++) {
+    const uint64_t mask = lane_terms[k].mask;
+    const uint32_t st = lane_terms[k].k;
+    const int a = 1 + (st < uint32_t(S) ? int(st) / 4 : A - 1);
+    uint64_t bs = 0;
+#pragma unroll
+    for (int j = 0; j < 6; j++)
+      if ((lane >> j) & 1) bs |= 1ull << p.lanepos[a][j];
+    const double add = (bs & mask) == mask ? double(lane_terms[k].ang) : 0.0;
+#pragma unroll
+    for (int i = 0; i < T; i++)
+      if (st == uint32_t(i)) bl[i] += add;
+  }
+  float2 e[T];
+#pragma unroll
+  for (int i = 0; i < T; i++) {
+    float sn, cs;
+    turn_sincos(bl[i], sn, cs);
+    e[i] = make_float2(cs, sn);
+  }
+  const uint32_t lane_bytes = lane * 16u;  // arrangement 0: lanes on bits 1..6
+  for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
+    const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
+    if (p.nuni) {
+      // Workgroup-uniform terms, one per lane (branch-free: lanes past the
+      // end read the last term and add zero).
+      double bu[T];
+#pragma unroll
+      for (int i = 0; i < T; i++) bu[i] = 0.0;
+      for (uint32_t k0 = 0; k0 < p.nuni; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        const uint32_t kc = k < p.nuni ? k : p.nuni - 1;
+        const uint64_t mask = uni[kc].mask;
+        const double add = (k < p.nuni && (ub & mask) == mask) ? double(uni[kc].ang) : 0.0;
+        const uint32_t st = uni[kc].k;
+#pragma unroll
+        for (int i = 0; i < T; i++) bu[i] += st == uint32_t(i) ? add : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < T; i++) {
+        float sn, cs;
+        turn_sincos(bl[i] + wave_allsum(bu[i]), sn, cs);
+        e[i] = make_float2(cs, sn);
+      }
+    }
+    float2 x[32];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const float4 v = ld4<true>(pair_ptr(p.a, ub | (uint64_t(r) << 7), lane_bytes));
+      x[2 * r] = make_float2(v.x, v.y);
+      x[2 * r + 1] = make_float2(v.z, v.w);
+    }
+    low_permute<0, 1>(p, lds, x, lane);
+#pragma unroll
+    for (int i = 0; i < T; i++) {
+      if constexpr (A >= 2) if (i == 4) low_permute<1, 2>(p, lds, x, lane);
+      if constexpr (A >= 3) if (i == 8) low_permute<2, 3>(p, lds, x, lane);
+      const ChainStep __attribute__((address_space(4))) &st = steps[opaque_step(i, x[0].x)];
+      if (i >= S) chain_step<5, 0, REAL>(x, st, e[i]);
+      else if (i % 4 == 0) chain_step<5, 1, REAL>(x, st, e[i]);
+      else if (i % 4 == 1) chain_step<5, 2, REAL>(x, st, e[i]);
+      else if (i % 4 == 2) chain_step<5, 3, REAL>(x, st, e[i]);
+      else chain_step<5, 4, REAL>(x, st, e[i]);
+    }
+    low_permute<A, 0>(p, lds, x, lane);
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+      st4<true>(pair_ptr(p.a, ub | (uint64_t(r) << 7), lane_bytes),
+                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
+  }
+}
+
 __global__ __launch_bounds__(kT) void k_blocked(BlockedArgs b) {
   extern __shared__ float2 tile[];
   const uint32_t block = 1u << b.bits, pairs = block >> 1;
@@ -1340,6 +1474,27 @@ void launch_wide(hipStream_t s, bool real, const WideArgs &w) {
   const unsigned grid = unsigned(std::min<uint64_t>(w.lay.nthreads, kMaxGrid));
   if (real) k_wide<true><<<grid, kT, 0, s>>>(w);
   else k_wide<false><<<grid, kT, 0, s>>>(w);
+}
+
+template <int S>
+void launch_low_s(hipStream_t s, bool zero, bool real, const LowArgs &l) {
+  const unsigned grid = unsigned(std::min<uint64_t>(l.lay.nthreads, kMaxGrid));
+  if (zero && real) k_low<S, true, true><<<grid, 64, 0, s>>>(l);
+  else if (zero) k_low<S, true, false><<<grid, 64, 0, s>>>(l);
+  else if (real) k_low<S, false, true><<<grid, 64, 0, s>>>(l);
+  else k_low<S, false, false><<<grid, 64, 0, s>>>(l);
+}
+
+void launch_low(hipStream_t s, int steps, bool zero, bool real, const LowArgs &l) {
+  switch (steps) {
+    case 5: launch_low_s<5>(s, zero, real, l); break;
+    case 6: launch_low_s<6>(s, zero, real, l); break;
+    case 7: launch_low_s<7>(s, zero, real, l); break;
+    case 8: launch_low_s<8>(s, zero, real, l); break;
+    case 9: launch_low_s<9>(s, zero, real, l); break;
+    case 10: launch_low_s<10>(s, zero, real, l); break;
+    default: break;
+  }
 }
 
 void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d) {

@@ -88,33 +88,77 @@ double op_alone_bytes(uint32_t n, const PlanOp &op) {
   return pass_bytes(n, pass_fixed(op));
 }
 
-uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) {
-  wires.clear();
-  if (p.n < kWideMinQubits) return 0;
+namespace {
+
+// Chain steps from plan op `k`: an uncontrolled Dense1q on a distinct wire
+// that `accept` takes, optionally followed by a phase layer (no global
+// phase, masks of weight 1 or 2 that all hold the wire). Stops after
+// `max_steps` steps or at the first op that does not fit; returns the number
+// of plan ops taken and appends the step wires.
+template <typename Accept>
+uint32_t chain_steps(const Plan &p, uint32_t k, size_t max_steps, Accept accept,
+                     std::vector<uint32_t> &wires) {
   uint32_t j = k;
-  while (j < p.ops.size() && wires.size() < kWideSteps) {
+  while (j < p.ops.size() && wires.size() < max_steps) {
     const PlanOp &op = p.ops[j];
     if (op.kind != OpKind::Dense1q) break;
     const uint32_t w = op.q[0];
-    if (w < kWideMinWire || std::find(wires.begin(), wires.end(), w) != wires.end()) break;
-    wires.push_back(w);
-    j++;
-    if (j < p.ops.size() && p.ops[j].kind == OpKind::Diagonal && !p.ops[j].masks.empty()) {
-      const PlanOp &d = p.ops[j];
+    if (!accept(w) || std::find(wires.begin(), wires.end(), w) != wires.end()) break;
+    uint32_t used = 1;
+    if (j + 1 < p.ops.size() && p.ops[j + 1].kind == OpKind::Diagonal &&
+        !p.ops[j + 1].masks.empty()) {
+      const PlanOp &d = p.ops[j + 1];
       bool ok = zero_phase(d.global);
       for (uint64_t m : d.masks) {
         const int wt = __builtin_popcountll(m);
         if (wt < 1 || wt > 2 || !((m >> w) & 1)) ok = false;
       }
-      if (!ok) break;
-      j++;
+      // A layer that does not fit ends the chain after this step.
+      if (ok) used = 2;
+      else max_steps = wires.size() + 1;
     }
+    wires.push_back(w);
+    j += used;
   }
+  return j - k;
+}
+
+}  // namespace
+
+uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) {
+  wires.clear();
+  if (p.n < kWideMinQubits) return 0;
+  const uint32_t used =
+      chain_steps(p, k, kWideSteps, [](uint32_t w) { return w >= kWideMinWire; }, wires);
   if (wires.size() < kWideSteps) {
     wires.clear();
     return 0;
   }
-  return j - k;
+  return used;
+}
+
+uint32_t low_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) {
+  wires.clear();
+  if (p.n < kLowMinQubits) return 0;
+  uint32_t used = chain_steps(
+      p, k, kLowMaxWire, [](uint32_t w) { return w >= 1 && w <= kLowMaxWire; }, wires);
+  if (wires.size() < kLowMinSteps) {
+    wires.clear();
+    return 0;
+  }
+  // A step on wire 0 may close the pass, unless the chain already stopped at
+  // a phase layer it could not take.
+  const uint32_t j = k + used;
+  const bool stopped = j < p.ops.size() && p.ops[j].kind == OpKind::Diagonal;
+  if (!stopped) {
+    std::vector<uint32_t> last;
+    const uint32_t more = chain_steps(p, j, 1, [](uint32_t w) { return w == 0; }, last);
+    if (!last.empty()) {
+      wires.push_back(0);
+      used += more;
+    }
+  }
+  return used;
 }
 
 std::vector<Pass> schedule_passes(const Plan &p) {
@@ -142,16 +186,21 @@ std::vector<Pass> schedule_passes(const Plan &p) {
     out.push_back(std::move(cur));
     cur = Pass{};
   };
-  std::vector<uint32_t> wide;
+  std::vector<uint32_t> steps;
   for (uint32_t k = 0; k < p.ops.size(); k++) {
     const PlanOp &op = p.ops[k];
     if (op.kind == OpKind::Diagonal && op.masks.empty()) continue;  // no dispatch
-    if (const uint32_t used = wide_chain_at(p, k, wide)) {
+    uint32_t used = wide_chain_at(p, k, steps);
+    const bool is_wide = used != 0;
+    if (!used) used = low_chain_at(p, k, steps);
+    if (used) {
       flush();
       Pass s;
-      s.merged = s.wide = true;
+      s.merged = true;
+      s.wide = is_wide;
+      s.low = !is_wide;
       for (uint32_t j = k; j < k + used; j++) s.ops.push_back(j);
-      s.reg = wide;
+      s.reg = steps;
       out.push_back(std::move(s));
       k += used - 1;
       continue;

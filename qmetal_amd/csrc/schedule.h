@@ -31,9 +31,10 @@ constexpr uint32_t kPassMaxOps = 16;    // plan ops per merged pass
 struct Pass {
   bool merged = false;         // false: ops[0] alone through its own kernel
   bool wide = false;           // merged: six chain steps over a workgroup tile
+  bool low = false;            // merged: chain steps over a low 2^11 tile
   std::vector<uint32_t> ops;   // plan-op indices, in plan order
   std::vector<uint32_t> reg;   // register wires (merged), first use order;
-                               // wide: the six step wires in order
+                               // wide and low: the step wires in order
   uint64_t fixed = 0;          // fixed-one bits (merged)
 };
 
@@ -49,6 +50,18 @@ constexpr uint32_t kWideMinQubits = 16;
 // Number of plan ops from `k` that form a wide pass (0 if none); `wires`
 // gets the step wires.
 uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires);
+
+// Low passes: kLowMinSteps to kLowMaxWire chain steps (as for wide passes)
+// on distinct wires 1..kLowMaxWire, optionally followed by one on wire 0,
+// over tiles of 2^(kLowMaxWire+1) contiguous amplitudes that one wave
+// permutes through LDS. States below kLowMinQubits use ordinary passes.
+constexpr uint32_t kLowMaxWire = 10;
+constexpr uint32_t kLowMinSteps = 5;
+constexpr uint32_t kLowMinQubits = 12;
+
+// Number of plan ops from `k` that form a low pass (0 if none); `wires` gets
+// the step wires (a final 0 for a step on wire 0).
+uint32_t low_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires);
 
 // Whether plan op `op` can join a merged pass.
 bool pass_mergeable(const PlanOp &op);

@@ -24,8 +24,8 @@ namespace qmamd {
 constexpr uint32_t kMaxQubits = 34;
 
 // Device time of one launch and its algorithmic bytes: a plan op (kind
-// 0..6) or a merged pass (kinds 7, 8, 9: the generic, chain and wide chain
-// kernels; bytes of one pass over the amplitudes it reads).
+// 0..6) or a merged pass (kinds 7..10: the generic, chain, wide chain and
+// low chain kernels; bytes of one pass over the amplitudes it reads).
 struct OpTime {
   uint32_t kind;
   double ms;

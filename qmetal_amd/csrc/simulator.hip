@@ -46,10 +46,11 @@ constexpr int kLocalBits = 4;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 // Timing record kinds of merged passes (plan-op kinds are 0..6): the
-// generic pass kernel, the chain kernel, the wide chain kernel.
+// generic pass kernel, the chain kernel, the wide and low chain kernels.
 constexpr uint32_t kTimingPass = 7;
 constexpr uint32_t kTimingChain = 8;
 constexpr uint32_t kTimingWide = 9;
+constexpr uint32_t kTimingLow = 10;
 
 int current_device() {
   int d = 0;
@@ -377,6 +378,9 @@ struct Simulator::Impl {
     // Wide chain pass (six steps over a workgroup tile).
     bool wide = false;
     WideArgs w{};
+    // Low chain pass (steps on wires 1..10 and 0 over per-wave tiles).
+    bool low = false;
+    LowArgs l{};
   };
 
   // Local coefficients of a phase op of a merged pass (turns).
@@ -653,6 +657,46 @@ struct Simulator::Impl {
     pl.c.nlane = uint32_t(nlane);
   }
 
+  // Phase tables of a chain step on local bit `b` of a five-bit register
+  // tile: the other four local bits, ascending, form the low pair (lo) and
+  // the high pair (hi); entry c - 1 is the factor for pair value c = 1..3.
+  static void step_tables(ChainStep &c, int b, const double (&gamma)[kPassMaxNL][kPassMaxNL]) {
+    int other[4], no = 0;
+    for (int q = 0; q < 5; q++)
+      if (q != b) other[no++] = q;
+    for (int cc = 1; cc < 4; cc++) {
+      double tl = 0.0, th = 0.0;
+      for (int j = 0; j < 2; j++)
+        if ((cc >> j) & 1) {
+          tl += gamma[std::min(b, other[j])][std::max(b, other[j])];
+          th += gamma[std::min(b, other[2 + j])][std::max(b, other[2 + j])];
+        }
+      tl -= std::rint(tl);
+      th -= std::rint(th);
+      c.lo[2 * cc - 2] = float(std::cos(kTwoPi * tl));
+      c.lo[2 * cc - 1] = float(std::sin(kTwoPi * tl));
+      c.hi[2 * cc - 2] = float(std::cos(kTwoPi * th));
+      c.hi[2 * cc - 1] = float(std::sin(kTwoPi * th));
+    }
+  }
+
+  // A chain step's Dense1q: its matrix, and unit phase tables until a phase
+  // layer sets them.
+  static void step_matrix(ChainStep &c, const PlanOp &op) {
+    c = ChainStep{};
+    float m8[8];
+    narrow(op.m, m8);
+    std::memcpy(c.m, m8, sizeof(m8));
+    for (int cc = 0; cc < 6; cc++) c.lo[cc] = c.hi[cc] = (cc % 2 == 0) ? 1.0f : 0.0f;
+  }
+
+  static bool steps_real(const ChainStep *cs, int count) {
+    for (int i = 0; i < count; i++)
+      if (cs[i].m[1] != 0.0f || cs[i].m[3] != 0.0f || cs[i].m[5] != 0.0f || cs[i].m[7] != 0.0f)
+        return false;
+    return true;
+  }
+
   // Wide chain pass: the six steps' constants and base-index terms, folded
   // against the tile arrangement each step runs in (WideArgs).
   void prep_wide(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
@@ -695,12 +739,7 @@ struct Simulator::Impl {
     for (uint32_t k : ps.ops) {
       const PlanOp &op = p.ops[k];
       if (op.kind == OpKind::Dense1q) {
-        step++;
-        cs[step] = ChainStep{};
-        float m8[8];
-        narrow(op.m, m8);
-        std::memcpy(cs[step].m, m8, sizeof(m8));
-        for (int cc = 0; cc < 6; cc++) cs[step].lo[cc] = cs[step].hi[cc] = (cc % 2 == 0) ? 1.0f : 0.0f;
+        step_matrix(cs[++step], op);
         continue;
       }
       // The step's phase layer.
@@ -723,31 +762,11 @@ struct Simulator::Impl {
         t[u].k = uint32_t(step);
         (u < nuniform ? uni : lan).push_back(t[u]);
       }
-      ChainStep &c = cs[step];
-      c.beta0 = beta0[b];
-      int other[4], no = 0;
-      for (int q = 0; q < 5; q++)
-        if (q != b) other[no++] = q;
-      for (int cc = 1; cc < 4; cc++) {
-        double tl = 0.0, th = 0.0;
-        for (int j = 0; j < 2; j++)
-          if ((cc >> j) & 1) {
-            tl += gamma[std::min(b, other[j])][std::max(b, other[j])];
-            th += gamma[std::min(b, other[2 + j])][std::max(b, other[2 + j])];
-          }
-        tl -= std::rint(tl);
-        th -= std::rint(th);
-        c.lo[2 * cc - 2] = float(std::cos(kTwoPi * tl));
-        c.lo[2 * cc - 1] = float(std::sin(kTwoPi * tl));
-        c.hi[2 * cc - 2] = float(std::cos(kTwoPi * th));
-        c.hi[2 * cc - 1] = float(std::sin(kTwoPi * th));
-      }
+      cs[step].beta0 = beta0[b];
+      step_tables(cs[step], b, gamma);
     }
     if (step != 5) fail("internal: wide pass steps");
-    bool real = true;
-    for (int i = 0; i < 6; i++)
-      if (cs[i].m[1] != 0.0f || cs[i].m[3] != 0.0f || cs[i].m[5] != 0.0f || cs[i].m[7] != 0.0f)
-        real = false;
+    const bool real = steps_real(cs, 6);
     DiagTerm *tu = reinterpret_cast<DiagTerm *>(staging + off + 6 * sizeof(ChainStep));
     std::copy(uni.begin(), uni.end(), tu);
     std::copy(lan.begin(), lan.end(), tu + uni.size());
@@ -763,6 +782,102 @@ struct Simulator::Impl {
     pl.bytes = pass_bytes(n, 0);
   }
 
+  // Low chain pass (LowArgs): arrangement 0 is the coalesced one (lanes on
+  // bits 1..6, registers on 7..10); arrangement 1 + a holds steps 4a..4a+3 on
+  // register bits, filled up with the lowest unused wires, and lanes on the
+  // remaining wires of 1..10, ascending. The wire-0 step runs in the last.
+  void prep_low(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
+    const std::vector<uint32_t> &w = ps.reg;
+    const bool zero = !w.empty() && w.back() == 0;
+    const int S = int(w.size()) - (zero ? 1 : 0);
+    if (S < int(kLowMinSteps) || S > int(kLowMaxWire) || ps.fixed || n < kLowMinQubits)
+      fail("internal: low pass shape");
+    const int A = (S + 3) / 4;
+    LowArgs &la = pl.l;
+    la = LowArgs{};
+    for (int j = 0; j < 6; j++) la.lanepos[0][j] = uint8_t(1 + j);
+    for (int k = 0; k < 4; k++) la.regpos[0][k] = uint8_t(7 + k);
+    uint64_t var[kLowMaxArr] = {};
+    for (int a = 0; a < A; a++) {
+      uint32_t used = 1u;
+      int nr = 0, nl = 0;
+      for (int i = 4 * a; i < std::min(S, 4 * a + 4); i++) {
+        if (w[i] < 1 || w[i] > kLowMaxWire || ((used >> w[i]) & 1)) fail("internal: low pass wires");
+        la.regpos[1 + a][nr++] = uint8_t(w[i]);
+        used |= 1u << w[i];
+      }
+      for (uint32_t q = 1; nr < 4; q++)
+        if (!((used >> q) & 1)) {
+          la.regpos[1 + a][nr++] = uint8_t(q);
+          used |= 1u << q;
+        }
+      for (uint32_t q = 1; q <= kLowMaxWire; q++)
+        if (!((used >> q) & 1)) {
+          la.lanepos[1 + a][nl++] = uint8_t(q);
+          var[1 + a] |= 1ull << q;
+        }
+      if (nl != 6) fail("internal: low pass lanes");
+    }
+    Layout &L = la.lay;
+    L = Layout{};
+    for (uint32_t q = 0; q <= kLowMaxWire; q++) {
+      L.ins[L.nins] = uint8_t(q);
+      L.hm[L.nins] = ~((uint64_t(1) << q) - 1);
+      L.nins++;
+    }
+    L.nthreads = dim >> L.nins;
+    L.streaming = 1;
+    ChainStep *cs = reinterpret_cast<ChainStep *>(staging + off);
+    std::vector<DiagTerm> uni, lan;
+    int step = -1;
+    for (uint32_t k : ps.ops) {
+      const PlanOp &op = p.ops[k];
+      if (op.kind == OpKind::Dense1q) {
+        step_matrix(cs[++step], op);
+        continue;
+      }
+      if (step < 0) fail("internal: low pass order");
+      const int a = 1 + (step < S ? step / 4 : A - 1);
+      const int b = step < S ? 1 + step % 4 : 0;  // the step's local bit
+      auto local_of = [&](uint32_t q) {
+        if (q == 0) return 0;
+        for (int l = 0; l < 4; l++)
+          if (la.regpos[a][l] == q) return 1 + l;
+        return -1;
+      };
+      double constant = 0.0, beta0[kPassMaxNL] = {}, gamma[kPassMaxNL][kPassMaxNL] = {};
+      std::vector<DiagTerm> t(op.masks.size());
+      uint32_t nuniform = 0;
+      const size_t nt =
+          fold_terms(op, 0, var[a], local_of, constant, beta0, gamma, t.data(), nuniform);
+      if (constant != 0.0) fail("internal: low pass phase constant");
+      for (size_t u = 0; u < nt; u++) {
+        if (t[u].kind != 2 || t[u].k != uint32_t(b)) fail("internal: low pass phase term");
+        t[u].k = uint32_t(step);
+        (u < nuniform ? uni : lan).push_back(t[u]);
+      }
+      cs[step].beta0 = beta0[b];
+      step_tables(cs[step], b, gamma);
+    }
+    const int T = S + (zero ? 1 : 0);
+    if (step != T - 1) fail("internal: low pass steps");
+    DiagTerm *tu = reinterpret_cast<DiagTerm *>(staging + off + size_t(T) * sizeof(ChainStep));
+    std::copy(uni.begin(), uni.end(), tu);
+    std::copy(lan.begin(), lan.end(), tu + uni.size());
+    la.a = state;
+    la.steps = reinterpret_cast<const ChainStep *>(arena + off);
+    la.uni = reinterpret_cast<const DiagTerm *>(arena + off + size_t(T) * sizeof(ChainStep));
+    la.lane = la.uni + uni.size();
+    la.nuni = uint32_t(uni.size());
+    la.nlane = uint32_t(lan.size());
+    pl.low = true;
+    pl.chain_k = S;
+    pl.chain_zero = zero;
+    pl.chain_real = steps_real(cs, T);
+    pl.nl = 5;
+    pl.bytes = pass_bytes(n, 0);
+  }
+
   void pass_launch(const PassLaunch &pl) {
     hipEvent_t a = nullptr;
     if (timing) {
@@ -770,6 +885,7 @@ struct Simulator::Impl {
       check(hipEventRecord(a, stream), "hipEventRecord");
     }
     if (pl.wide) launch_wide(stream, pl.chain_real, pl.w);
+    else if (pl.low) launch_low(stream, pl.chain_k, pl.chain_zero, pl.chain_real, pl.l);
     else if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);
     else launch_pass(stream, pl.nl, pl.a);
     launched();
@@ -777,7 +893,10 @@ struct Simulator::Impl {
     if (timing) {
       hipEvent_t b = take_event();
       check(hipEventRecord(b, stream), "hipEventRecord");
-      const uint32_t kind = pl.wide ? kTimingWide : pl.chain ? kTimingChain : kTimingPass;
+      const uint32_t kind = pl.wide  ? kTimingWide
+                            : pl.low ? kTimingLow
+                            : pl.chain ? kTimingChain
+                                       : kTimingPass;
       timings.push_back(OpTiming{kind, pl.bytes, a, b});
     }
   }
@@ -1167,6 +1286,10 @@ void Simulator::run_plan(const Plan &p) {
     for (size_t k = 0; k < passes.size(); k++) {
       if (passes[k].wide) {
         s->prep_wide(p, passes[k], offs[k], merged[k]);
+        continue;
+      }
+      if (passes[k].low) {
+        s->prep_low(p, passes[k], offs[k], merged[k]);
         continue;
       }
       if (passes[k].merged) {

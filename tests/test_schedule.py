@@ -51,26 +51,35 @@ def check_grouping(circ, options=None):
             wires = set().union(*(moving_wires(o) for o in ops))
             assert all(o["kind"] != "Blocked1q" for o in ops)
             if len(wires) > MAX_REG:
-                # a wide pass: six Dense1q steps on wires >= 7, each followed
-                # at most by a phase layer whose masks all hold its wire
-                assert len(wires) == 6 and min(wires) >= 7 and circ.num_qubits >= 16, wires
-                steps = [o for o in ops if o["kind"] == "Dense1q"]
-                assert len(steps) == 6
+                # a chain pass: Dense1q steps on distinct wires, each followed
+                # at most by a phase layer whose masks all hold its wire;
+                # wide: six steps on wires >= 7; low: 5..10 steps on wires
+                # 1..10, then at most one on wire 0
+                steps = [o["qubits"][0] for o in ops if o["kind"] == "Dense1q"]
+                assert len(set(steps)) == len(steps), steps
                 last = None
                 for o in ops:
                     if o["kind"] == "Dense1q":
                         last = o["qubits"][0]
                     else:
-                        assert o["kind"] == "DiagonalLayer"
+                        assert o["kind"] == "DiagonalLayer" and last is not None
                         assert all((m >> last) & 1 for m in o["masks"])
+                        last = None
+                if min(steps) >= 7 and len(steps) == 6:
+                    assert circ.num_qubits >= 16
+                else:
+                    assert circ.num_qubits >= 12, steps
+                    assert 0 not in steps[:-1] and 5 <= len(wires) <= 10, steps
+                    assert max(steps) <= 10, steps
     return max(seen) + 1 if seen else 0
 
 
 def test_qft_pass_counts():
     """QFT: H on wire j then the CP ladder onto j, for j = n-1 .. 0. From 16
-    qubits six H's and their ladders on wires >= 7 share a wide pass; the rest
-    go four to a pass (the last pass also takes wire 0)."""
-    for n, want in ((12, 3), (16, 4), (26, 5), (30, 6), (32, 6), (33, 6), (34, 7)):
+    qubits six H's and their ladders on wires >= 7 share a wide pass; from
+    wire 10 (or lower) down to 0 the rest share one low pass; a wire left
+    between them gets an ordinary pass."""
+    for n, want in ((12, 2), (16, 2), (26, 4), (30, 5), (32, 5), (33, 5), (34, 5)):
         c = C.to_circuit(n, C.qft(n))
         assert c.plan_size(True) == 2 * n - 1
         assert check_grouping(c) == want, n
