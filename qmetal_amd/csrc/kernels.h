@@ -165,21 +165,23 @@ struct ChainArgs {
   uint32_t nlane;
 };
 
-// Wide chain pass: six chain steps on wires w1..w6 (pos[0..5]) over a
-// workgroup tile of 2^13 amplitudes. Each thread holds 32 amplitudes: bit 0
-// and tile-local bits 1..4 = w1..w4; lane bits 0..5 of the thread index sit
-// on positions lanepos[0..5] and the two wave bits on w5, w6. Steps 0-3 act
-// on local bits 1..4; an LDS exchange then swaps local bits 1, 2 with the
-// wave bits, and steps 4, 5 act on local bits 1, 2 (now w5, w6) while the
-// wave bits stand for w1, w2. `lay` enumerates the tiles (nthreads = tiles,
-// ins = the 13 tile positions and the fixed bits). Steps 0-3 are folded
-// against the first arrangement, steps 4-5 against the second; base-index
-// terms as in ChainArgs (k = step).
+// Wide chain pass: 4 + WB chain steps (WB = 2 or 3 wave bits) on wires
+// w1..w(4+WB) (pos[0..]) over a workgroup tile of 2^(11+WB) amplitudes held
+// by 2^WB waves. Each thread holds 32 amplitudes: bit 0 and tile-local bits
+// 1..4 = w1..w4; lane bits 0..5 of the thread index sit on positions
+// lanepos[0..5] and the WB wave bits on w5.. . Steps 0-3 act on local bits
+// 1..4; an LDS exchange then swaps local bits 1..WB with the wave bits, and
+// steps 4.. act on local bits 1..WB (now w5..) while the wave bits stand for
+// w1..wWB. `lay` enumerates the tiles (nthreads = tiles, ins = the tile
+// positions and the fixed bits). Steps 0-3 are folded against the first
+// arrangement, the rest against the second; base-index terms as in ChainArgs
+// (k = step).
+constexpr int kWideArgSteps = 7;
 struct WideArgs {
   float2 *a;
   Layout lay;
   uint8_t lanepos[6];
-  uint8_t pos[6];
+  uint8_t pos[kWideArgSteps];
   const ChainStep *steps;
   const DiagTerm *uni;
   const DiagTerm *lane;
@@ -187,19 +189,20 @@ struct WideArgs {
   uint32_t nlane;
 };
 
-// Low chain pass: S chain steps on distinct wires 1..10, then optionally one
-// on wire 0, over per-wave tiles of 2^11 contiguous amplitudes (bits 0..10;
-// workgroup = one wave). The tile is loaded and stored coalesced (lanes on
-// bits 1..6, register bits on 7..10, arrangement 0) and permuted through LDS
-// into arrangement 1 + a for steps 4a .. 4a + 3, whose wires are register bits
-// (tile-local 1..4); the wire-0 step runs in the last arrangement. `lanepos`
-// and `regpos` give each arrangement's bit positions. Steps are folded
-// against their arrangement; base-index terms as in ChainArgs (k = step).
+// Low chain pass: S chain steps on distinct wires 1..11, then optionally one
+// on wire 0, over workgroup tiles of 2^12 contiguous amplitudes (bits 0..11)
+// held by two waves. The tile is loaded and stored coalesced (arrangement 0:
+// lanes on bits 1..6, the wave bit on 7, register bits on 8..11) and permuted
+// through LDS into arrangement 1 + a for steps 4a .. 4a + 3, whose wires are
+// register bits (tile-local 1..4); the wire-0 step runs in the last
+// arrangement. `lanepos` gives each arrangement's lane bits (0..5) and wave
+// bit (6), `regpos` its register bits. Steps are folded against their
+// arrangement; base-index terms as in ChainArgs (k = step).
 constexpr int kLowMaxArr = 4;
 struct LowArgs {
   float2 *a;
-  Layout lay;  // tiles: ins = bits 0..10 and the fixed bits
-  uint8_t lanepos[kLowMaxArr][6];
+  Layout lay;  // tiles: ins = bits 0..11 and the fixed bits
+  uint8_t lanepos[kLowMaxArr][7];
   uint8_t regpos[kLowMaxArr][4];
   const ChainStep *steps;
   const DiagTerm *uni;
@@ -218,9 +221,13 @@ void launch_blocked(hipStream_t s, const BlockedArgs &b);
 void launch_pass(hipStream_t s, int nl, const PassArgs &p);
 // nl = 5 with k = 4, or nl = 4 with k in 1..3; `zero`: a last step on bit 0.
 void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const ChainArgs &c);
-void launch_wide(hipStream_t s, bool real, const WideArgs &w);
-// s in 5..10 steps on wires 1..10, `zero`: a last step on wire 0.
-void launch_low(hipStream_t s, int steps, bool zero, bool real, const LowArgs &l);
+// Step modes of the wide and low chain kernels: 0 complex 2x2s, 1 real
+// 2x2s, 2 every step a multiple of [[1,1],[1,-1]] with the product of the
+// multiples folded into the last step's matrix.
+// steps = 6 or 7.
+void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w);
+// s in 5..11 steps on wires 1..11, `zero`: a last step on wire 0.
+void launch_low(hipStream_t s, int steps, bool zero, int mode, const LowArgs &l);
 
 // FP64 reductions. Each writes per-block partials into `partials` (capacity
 // kRedBlocks doubles or double2) and then reduces them in a fixed order into

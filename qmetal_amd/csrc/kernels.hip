@@ -30,10 +30,15 @@ namespace {
 constexpr int kT = 256;             // threads per workgroup
 constexpr uint64_t kMaxGrid = 1u << 20;
 
-inline unsigned grid_for(uint64_t work) {
+// Workgroups of a merged pass: enough to fill the chip many times over;
+// more would only repeat the per-workgroup set-up (the lane phase terms, the
+// step constants) for fewer tiles each.
+constexpr uint64_t kPassGrid = 1u << 16;
+
+inline unsigned grid_for(uint64_t work, uint64_t cap = kMaxGrid) {
   uint64_t g = (work + kT - 1) / kT;
   if (g < 1) g = 1;
-  if (g > kMaxGrid) g = kMaxGrid;
+  if (g > cap) g = cap;
   return unsigned(g);
 }
 
@@ -528,18 +533,29 @@ __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
 // One chain step on tile-local bit BIT: the 2x2 (m) on all pairs, then the
 // phase of the combinations holding BIT, e * lo[low two other bits] *
 // hi[the other bits above them] (other local bits ascending).
-template <int NL, int BIT, bool REAL>
+// MODE: 0 complex 2x2, 1 real 2x2, 2 the unscaled butterfly [[1,1],[1,-1]]
+// (the scale of a run of such steps is carried by the run's last step,
+// which uses mode 1).
+template <int NL, int BIT, int MODE>
 __device__ __forceinline__ void chain_step(float2 (&x)[1 << NL],
                                            const ChainStep __attribute__((address_space(4))) &st,
                                            float2 e) {
-  {
+  if constexpr (MODE == 2) {
+#pragma unroll
+    for (int l = 0; l < (1 << NL); l++) {
+      if (l & (1 << BIT)) continue;
+      const float2 u = x[l], v = x[l | (1 << BIT)];
+      x[l] = make_float2(u.x + v.x, u.y + v.y);
+      x[l | (1 << BIT)] = make_float2(u.x - v.x, u.y - v.y);
+    }
+  } else {
     float m[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) m[j] = st.m[j];
 #pragma unroll
     for (int l = 0; l < (1 << NL); l++) {
       if (l & (1 << BIT)) continue;
-      if constexpr (REAL) bfly_real_s(m[0], m[2], m[4], m[6], x[l], x[l | (1 << BIT)]);
+      if constexpr (MODE == 1) bfly_real_s(m[0], m[2], m[4], m[6], x[l], x[l | (1 << BIT)]);
       else bfly_cplx_s(m, x[l], x[l | (1 << BIT)]);
     }
   }
@@ -582,10 +598,21 @@ __global__ __launch_bounds__(kT) void k_chain(ChainArgs p) {
   const TileAddr ta = tile_addr(p.lay);
   const uint64_t stride = uint64_t(gridDim.x) * kT;
   const uint32_t lane = threadIdx.x & 63;
+  // Lane terms read only thread-index bits, the same for every tile.
+  double bl[S];
+#pragma unroll
+  for (int i = 0; i < S; i++) bl[i] = steps[i].beta0;
+  for (uint32_t k = 0; k < p.nlane; k++) {
+    const uint64_t mask = lane_terms[k].mask;
+    const double add = (ta.lane_bits & mask) == mask ? double(lane_terms[k].ang) : 0.0;
+    const uint32_t st = lane_terms[k].k;
+#pragma unroll
+    for (int i = 0; i < S; i++)
+      if (st == uint32_t(i)) bl[i] += add;
+  }
   for (uint64_t t0 = uint64_t(blockIdx.x) * kT; t0 < p.lay.nthreads; t0 += stride) {
     const uint64_t ub = scatter(t0, p.lay) | p.lay.fixed;
     const bool active = t0 + threadIdx.x < p.lay.nthreads;
-    const uint64_t base = ub | ta.lane_bits;
     // Workgroup-uniform terms, one per lane (branch-free: lanes past the end
     // read the last term and add zero).
     double bu[S];
@@ -612,22 +639,11 @@ __global__ __launch_bounds__(kT) void k_chain(ChainArgs p) {
       }
     }
     // Per-thread factor of each step: exp(2 pi i beta).
-    double b[S];
-#pragma unroll
-    for (int i = 0; i < S; i++) b[i] = wave_allsum(bu[i]) + steps[i].beta0;
-    for (uint32_t k = 0; k < p.nlane; k++) {
-      const uint64_t mask = lane_terms[k].mask;
-      const double add = (base & mask) == mask ? double(lane_terms[k].ang) : 0.0;
-      const uint32_t st = lane_terms[k].k;
-#pragma unroll
-      for (int i = 0; i < S; i++)
-        if (st == uint32_t(i)) b[i] += add;
-    }
     float2 e[S];
 #pragma unroll
     for (int i = 0; i < S; i++) {
       float sn, cs;
-      turn_sincos(b[i], sn, cs);
+      turn_sincos(bl[i] + wave_allsum(bu[i]), sn, cs);
       e[i] = make_float2(cs, sn);
     }
 #pragma unroll
@@ -651,13 +667,19 @@ __global__ __launch_bounds__(kT) void k_chain(ChainArgs p) {
   }
 }
 
-// Wide chain pass (WideArgs). One workgroup tile per iteration; the exchange
-// between the two arrangements goes through 16 KB of LDS in four rounds (one
-// per value of local bits 3, 4).
-template <bool REAL>
-__global__ __launch_bounds__(kT) void k_wide(WideArgs p) {
-  __shared__ float4 xch[4][4][64];  // [writer's wave][slot][lane]
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Wide chain pass (WideArgs) with WB wave bits. One workgroup tile per
+// iteration; the exchange between the two arrangements goes through LDS in
+// 2^(4-WB) rounds, one per value of the local bits that stay (16 KB per round
+// for WB = 2, 64 KB for WB = 3).
+template <int WB, int MODE>
+__global__ __launch_bounds__(64 << WB) void k_wide(WideArgs p) {
+  constexpr int ML = MODE == 2 ? 1 : MODE;  // the last step's mode
+  constexpr int W = 1 << WB;         // waves, and registers per exchange round
+  constexpr int ROUNDS = 16 / W;
+  __shared__ float4 xch[W][W][64];  // [writer's wave][slot][lane]
+  // The wave index is wave-uniform: made scalar so that tile addresses are a
+  // scalar base plus the lane's byte offset.
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const const_ptr<ChainStep> steps = as_const(p.steps);
   const const_ptr<DiagTerm> uni = as_const(p.uni);
   const const_ptr<DiagTerm> lane_terms = as_const(p.lane);
@@ -665,16 +687,33 @@ __global__ __launch_bounds__(kT) void k_wide(WideArgs p) {
 #pragma unroll
   for (int j = 0; j < 6; j++)
     if ((lane >> j) & 1) lanebits |= 1ull << p.lanepos[j];
-  // Wave bits: w5, w6 before the exchange, w1, w2 after.
-  const uint64_t wave_a = ((wave & 1) ? 1ull << p.pos[4] : 0) | ((wave & 2) ? 1ull << p.pos[5] : 0);
-  const uint64_t wave_b = ((wave & 1) ? 1ull << p.pos[0] : 0) | ((wave & 2) ? 1ull << p.pos[1] : 0);
+  // Wave bits: w5.. before the exchange, w1..wWB after.
+  uint64_t wave_a = 0, wave_b = 0;
+#pragma unroll
+  for (int j = 0; j < WB; j++)
+    if ((wave >> j) & 1) {
+      wave_a |= 1ull << p.pos[4 + j];
+      wave_b |= 1ull << p.pos[j];
+    }
   const uint32_t lane_bytes = uint32_t(lanebits) * 8u;
+  // Lane terms read only lane and wave bits, the same for every tile.
+  double bl[4 + WB];
+#pragma unroll
+  for (int i = 0; i < 4 + WB; i++) bl[i] = steps[i].beta0;
+  for (uint32_t k = 0; k < p.nlane; k++) {
+    const uint64_t mask = lane_terms[k].mask;
+    const uint32_t st = lane_terms[k].k;
+    const uint64_t bs = lanebits | (st < 4 ? wave_a : wave_b);
+    const double add = (bs & mask) == mask ? double(lane_terms[k].ang) : 0.0;
+#pragma unroll
+    for (int i = 0; i < 4 + WB; i++)
+      if (st == uint32_t(i)) bl[i] += add;
+  }
   for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
     const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
-    const uint64_t base_a = ub | lanebits | wave_a, base_b = ub | lanebits | wave_b;
-    double bu[6];
+    double bu[4 + WB];
 #pragma unroll
-    for (int i = 0; i < 6; i++) bu[i] = 0.0;
+    for (int i = 0; i < 4 + WB; i++) bu[i] = 0.0;
     for (uint32_t k0 = 0; k0 < p.nuni; k0 += 64) {
       const uint32_t k = k0 + lane;
       const uint32_t kc = k < p.nuni ? k : p.nuni - 1;
@@ -682,7 +721,7 @@ __global__ __launch_bounds__(kT) void k_wide(WideArgs p) {
       const double add = (k < p.nuni && (ub & mask) == mask) ? double(uni[kc].ang) : 0.0;
       const uint32_t st = uni[kc].k;
 #pragma unroll
-      for (int i = 0; i < 6; i++) bu[i] += st == uint32_t(i) ? add : 0.0;
+      for (int i = 0; i < 4 + WB; i++) bu[i] += st == uint32_t(i) ? add : 0.0;
     }
     float2 x[32];
 #pragma unroll
@@ -695,148 +734,201 @@ __global__ __launch_bounds__(kT) void k_wide(WideArgs p) {
       x[2 * r] = make_float2(v.x, v.y);
       x[2 * r + 1] = make_float2(v.z, v.w);
     }
-    double b[6];
+    float2 e[4 + WB];
 #pragma unroll
-    for (int i = 0; i < 6; i++) b[i] = wave_allsum(bu[i]) + steps[i].beta0;
-    for (uint32_t k = 0; k < p.nlane; k++) {
-      const uint64_t mask = lane_terms[k].mask;
-      const uint32_t st = lane_terms[k].k;
-      const uint64_t bs = st < 4 ? base_a : base_b;
-      const double add = (bs & mask) == mask ? double(lane_terms[k].ang) : 0.0;
-#pragma unroll
-      for (int i = 0; i < 6; i++)
-        if (st == uint32_t(i)) b[i] += add;
-    }
-    float2 e[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
+    for (int i = 0; i < 4 + WB; i++) {
       float sn, cs;
-      turn_sincos(b[i], sn, cs);
+      turn_sincos(bl[i] + wave_allsum(bu[i]), sn, cs);
       e[i] = make_float2(cs, sn);
     }
-    chain_step<5, 1, REAL>(x, steps[opaque_step(0, x[0].x)], e[0]);
-    chain_step<5, 2, REAL>(x, steps[opaque_step(1, x[0].x)], e[1]);
-    chain_step<5, 3, REAL>(x, steps[opaque_step(2, x[0].x)], e[2]);
-    chain_step<5, 4, REAL>(x, steps[opaque_step(3, x[0].x)], e[3]);
-    // Exchange: local bits 1, 2 (w1, w2) <-> wave bits (w5, w6).
+    chain_step<5, 1, MODE>(x, steps[opaque_step(0, x[0].x)], e[0]);
+    chain_step<5, 2, MODE>(x, steps[opaque_step(1, x[0].x)], e[1]);
+    chain_step<5, 3, MODE>(x, steps[opaque_step(2, x[0].x)], e[2]);
+    chain_step<5, 4, MODE>(x, steps[opaque_step(3, x[0].x)], e[3]);
+    // Exchange: local bits 1..WB (w1..) <-> wave bits (w5..). Register r
+    // (local bits r + 1) holds x[2r], x[2r + 1]; its low WB bits are swapped
+    // with the wave index, its high bits pick the round.
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < ROUNDS; c++) {
 #pragma unroll
-      for (int a = 0; a < 4; a++) {
-        const int l = (c << 3) | (a << 1);
+      for (int a = 0; a < W; a++) {
+        const int l = 2 * ((c << WB) | a);
         xch[wave][a][lane] = make_float4(x[l].x, x[l].y, x[l + 1].x, x[l + 1].y);
       }
       __syncthreads();
 #pragma unroll
-      for (int d = 0; d < 4; d++) {
-        const int l = (c << 3) | (d << 1);
+      for (int d = 0; d < W; d++) {
+        const int l = 2 * ((c << WB) | d);
         const float4 v = xch[d][wave][lane];
         x[l] = make_float2(v.x, v.y);
         x[l + 1] = make_float2(v.z, v.w);
       }
       __syncthreads();
     }
-    chain_step<5, 1, REAL>(x, steps[opaque_step(4, x[0].x)], e[4]);
-    chain_step<5, 2, REAL>(x, steps[opaque_step(5, x[0].x)], e[5]);
+    chain_step<5, 1, MODE>(x, steps[opaque_step(4, x[0].x)], e[4]);
+    if constexpr (WB >= 3) {
+      chain_step<5, 2, MODE>(x, steps[opaque_step(5, x[0].x)], e[5]);
+      chain_step<5, 3, ML>(x, steps[opaque_step(6, x[0].x)], e[6]);
+    } else {
+      chain_step<5, 2, ML>(x, steps[opaque_step(5, x[0].x)], e[5]);
+    }
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-      // Local bits 1, 2 now w5, w6; 3, 4 still w3, w4.
+      // Local bits 1..WB now w5..; the rest still w(WB+1)..w4.
       uint64_t o = ub | wave_b;
-      if (r & 1) o |= 1ull << p.pos[4];
-      if (r & 2) o |= 1ull << p.pos[5];
-      if (r & 4) o |= 1ull << p.pos[2];
-      if (r & 8) o |= 1ull << p.pos[3];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if ((r >> k) & 1) o |= 1ull << (k < WB ? p.pos[4 + k] : p.pos[k]);
       st4<true>(pair_ptr(p.a, o, lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
     }
   }
 }
 
-// LDS slot of tile index i (float4 units, 10 bits): the low four bits are
+// LDS slot of tile index i (float4 units, 11 bits): the low four bits are
 // xored with the higher ones so that lanes stepping through high bits still
-// spread over the banks (a bijection: the high bits are kept).
+// spread over the banks (a bijection: the high bits are kept; linear over
+// GF(2)).
 __device__ __forceinline__ uint32_t low_swz(uint32_t i) {
   return i ^ (((i >> 4) ^ (i >> 8)) & 0xFu);
 }
 
-// Tile index (float4 units: tile bit p is index bit p - 1) of this lane's
-// lane bits in arrangement `a`.
-__device__ __forceinline__ uint32_t low_lane_index(const LowArgs &p, int a, uint32_t lane) {
+// Tile index (float4 units: tile bit p is index bit p - 1) of this thread's
+// lane and wave bits in arrangement `a`.
+__device__ __forceinline__ uint32_t low_thread_index(const LowArgs &p, int a, uint32_t lane,
+                                                     uint32_t wave) {
   uint32_t i = 0;
 #pragma unroll
   for (int j = 0; j < 6; j++)
     if ((lane >> j) & 1) i |= 1u << (p.lanepos[a][j] - 1);
-  return i;
-}
-
-__device__ __forceinline__ uint32_t low_reg_index(const LowArgs &p, int a, int r) {
-  uint32_t i = 0;
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-    if ((r >> k) & 1) i |= 1u << (p.regpos[a][k] - 1);
+  if (wave) i |= 1u << (p.lanepos[a][6] - 1);
   return i;
 }
 
 // Moves the tile from arrangement FROM to arrangement TO through LDS. The
-// swizzle is linear over GF(2) and lane and register bits are disjoint, so a
-// slot is the lane's swizzled index xor the register's (workgroup-uniform).
+// swizzle is linear and thread and register bits are disjoint, so a slot is
+// the thread's swizzled index xor the xor of the swizzled register bits
+// (workgroup-uniform). Byte offsets are kept opaque so that the 32 slot
+// addresses are formed here (one xor each) instead of being hoisted out of
+// the tile loop into registers.
 template <int FROM, int TO>
 __device__ __forceinline__ void low_permute(const LowArgs &p, float4 *lds, float2 (&x)[32],
-                                            uint32_t lane) {
-  const uint32_t lf = low_swz(low_lane_index(p, FROM, lane));
-  const uint32_t lt = low_swz(low_lane_index(p, TO, lane));
+                                            uint32_t lane, uint32_t wave) {
+  uint32_t lf = low_swz(low_thread_index(p, FROM, lane, wave)) << 4;
+  uint32_t lt = low_swz(low_thread_index(p, TO, lane, wave)) << 4;
+  uint32_t cf[4], ct[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    cf[k] = low_swz(1u << (p.regpos[FROM][k] - 1)) << 4;
+    ct[k] = low_swz(1u << (p.regpos[TO][k] - 1)) << 4;
+  }
+  asm volatile("" : "+v"(lf), "+v"(lt) : "v"(x[0].x));
+  asm volatile("" : "+s"(cf[0]), "+s"(cf[1]), "+s"(cf[2]), "+s"(cf[3]), "+s"(ct[0]), "+s"(ct[1]),
+               "+s"(ct[2]), "+s"(ct[3]));
+  char *base = reinterpret_cast<char *>(lds);
   __syncthreads();  // earlier reads of the buffer are done
 #pragma unroll
-  for (int r = 0; r < 16; r++)
-    lds[lf ^ low_swz(low_reg_index(p, FROM, r))] =
+  for (int r = 0; r < 16; r++) {
+    const uint32_t o = ((r & 1) ? cf[0] : 0u) ^ ((r & 2) ? cf[1] : 0u) ^ ((r & 4) ? cf[2] : 0u) ^
+                       ((r & 8) ? cf[3] : 0u);
+    *reinterpret_cast<float4 *>(base + (lf ^ o)) =
         make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
+  }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < 16; r++) {
-    const float4 v = lds[lt ^ low_swz(low_reg_index(p, TO, r))];
+    const uint32_t o = ((r & 1) ? ct[0] : 0u) ^ ((r & 2) ? ct[1] : 0u) ^ ((r & 4) ? ct[2] : 0u) ^
+                       ((r & 8) ? ct[3] : 0u);
+    const float4 v = *reinterpret_cast<const float4 *>(base + (lt ^ o));
     x[2 * r] = make_float2(v.x, v.y);
     x[2 * r + 1] = make_float2(v.z, v.w);
   }
 }
 
-// Low chain pass (LowArgs); one wave per workgroup, 16 KB of LDS per wave.
-template <int S, bool Z, bool REAL>
-__global__ __launch_bounds__(64) void k_low(LowArgs p) {
+// Steps I.. of a low chain pass as straight-line code (template recursion,
+// so that every step index is a compile-time constant however long the
+// chain): step i runs in arrangement 1 + i / 4 on register bit 1 + i % 4,
+// the wire-0 step (i == S) on bit 0 of the last arrangement.
+template <int I, int S, int T, int MODE>
+__device__ __forceinline__ void low_steps(const LowArgs &p, float4 *lds, float2 (&x)[32],
+                                          const float2 (&e)[T], const_ptr<ChainStep> steps,
+                                          uint32_t lane, uint32_t wave) {
+  if constexpr (I < T) {
+    if constexpr (I == 4 && S > 4) low_permute<1, 2>(p, lds, x, lane, wave);
+    if constexpr (I == 8 && S > 8) low_permute<2, 3>(p, lds, x, lane, wave);
+    const ChainStep __attribute__((address_space(4))) &st = steps[opaque_step(I, x[0].x)];
+    constexpr int M = (MODE == 2 && I == T - 1) ? 1 : MODE;
+    if constexpr (I >= S) chain_step<5, 0, M>(x, st, e[I]);
+    else chain_step<5, 1 + I % 4, M>(x, st, e[I]);
+    low_steps<I + 1, S, T, MODE>(p, lds, x, e, steps, lane, wave);
+  }
+}
+
+// Low chain pass (LowArgs): two waves per workgroup tile, 32 KB of LDS. The
+// next tile's loads are issued before the current tile is worked on.
+template <int S, bool Z, int MODE>
+__global__ __launch_bounds__(128) void k_low(LowArgs p) {
   constexpr int T = S + (Z ? 1 : 0);
   constexpr int A = (S + 3) / 4;  // step arrangements
   static_assert(S >= 1 && A + 1 <= kLowMaxArr, "low chain shape");
-  __shared__ float4 lds[1024];
-  const uint32_t lane = threadIdx.x;
+  __shared__ float4 lds[2048];
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const const_ptr<ChainStep> steps = as_const(p.steps);
   const const_ptr<DiagTerm> uni = as_const(p.uni);
   const const_ptr<DiagTerm> lane_terms = as_const(p.lane);
-  // Lane terms read only lane bits, which are the same for every tile.
-  double bl[T];
+  // Each step's factor exp(2 pi i beta): lane terms read only thread bits,
+  // which are the same for every tile, so without workgroup-uniform terms
+  // the factors are computed once.
+  auto lane_beta = [&](double (&b)[T]) {
 #pragma unroll
-  for (int i = 0; i < T; i++) bl[i] = steps[i].beta0;
-  for (uint32_t k = 0; k < p.nlane; k++) {
-    const uint64_t mask = lane_terms[k].mask;
-    const uint32_t st = lane_terms[k].k;
-    const int a = 1 + (st < uint32_t(S) ? int(st) / 4 : A - 1);
-    uint64_t bs = 0;
+    for (int i = 0; i < T; i++) b[i] = steps[i].beta0;
+    for (uint32_t k = 0; k < p.nlane; k++) {
+      const uint64_t mask = lane_terms[k].mask;
+      const uint32_t st = lane_terms[k].k;
+      const int a = 1 + (st < uint32_t(S) ? int(st) / 4 : A - 1);
+      uint64_t bs = wave ? 1ull << p.lanepos[a][6] : 0ull;
 #pragma unroll
-    for (int j = 0; j < 6; j++)
-      if ((lane >> j) & 1) bs |= 1ull << p.lanepos[a][j];
-    const double add = (bs & mask) == mask ? double(lane_terms[k].ang) : 0.0;
+      for (int j = 0; j < 6; j++)
+        if ((lane >> j) & 1) bs |= 1ull << p.lanepos[a][j];
+      const double add = (bs & mask) == mask ? double(lane_terms[k].ang) : 0.0;
 #pragma unroll
-    for (int i = 0; i < T; i++)
-      if (st == uint32_t(i)) bl[i] += add;
-  }
+      for (int i = 0; i < T; i++)
+        if (st == uint32_t(i)) b[i] += add;
+    }
+  };
   float2 e[T];
+  {
+    double b[T];
+    lane_beta(b);
 #pragma unroll
-  for (int i = 0; i < T; i++) {
-    float sn, cs;
-    turn_sincos(bl[i], sn, cs);
-    e[i] = make_float2(cs, sn);
+    for (int i = 0; i < T; i++) {
+      float sn, cs;
+      turn_sincos(b[i], sn, cs);
+      e[i] = make_float2(cs, sn);
+    }
   }
-  const uint32_t lane_bytes = lane * 16u;  // arrangement 0: lanes on bits 1..6
+  // Arrangement 0: lanes on bits 1..6, the wave bit on 7, registers 8..11.
+  const uint32_t lane_bytes = lane * 16u;
+  const uint64_t wave_off = uint64_t(wave) << 7;
+  float4 y[16];
+  if (blockIdx.x < p.lay.nthreads) {
+    const uint64_t ub0 = scatter(blockIdx.x, p.lay) | p.lay.fixed | wave_off;
+#pragma unroll
+    for (int r = 0; r < 16; r++) y[r] = ld4<true>(pair_ptr(p.a, ub0 | (uint64_t(r) << 8), lane_bytes));
+  }
   for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
     const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
+    float2 x[32];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      x[2 * r] = make_float2(y[r].x, y[r].y);
+      x[2 * r + 1] = make_float2(y[r].z, y[r].w);
+    }
+    if (tile + gridDim.x < p.lay.nthreads) {
+      const uint64_t ubn = scatter(tile + gridDim.x, p.lay) | p.lay.fixed | wave_off;
+#pragma unroll
+      for (int r = 0; r < 16; r++) y[r] = ld4<true>(pair_ptr(p.a, ubn | (uint64_t(r) << 8), lane_bytes));
+    }
     if (p.nuni) {
       // Workgroup-uniform terms, one per lane (branch-free: lanes past the
       // end read the last term and add zero).
@@ -852,36 +944,21 @@ __global__ __launch_bounds__(64) void k_low(LowArgs p) {
 #pragma unroll
         for (int i = 0; i < T; i++) bu[i] += st == uint32_t(i) ? add : 0.0;
       }
+      double b[T];
+      lane_beta(b);
 #pragma unroll
       for (int i = 0; i < T; i++) {
         float sn, cs;
-        turn_sincos(bl[i] + wave_allsum(bu[i]), sn, cs);
+        turn_sincos(b[i] + wave_allsum(bu[i]), sn, cs);
         e[i] = make_float2(cs, sn);
       }
     }
-    float2 x[32];
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const float4 v = ld4<true>(pair_ptr(p.a, ub | (uint64_t(r) << 7), lane_bytes));
-      x[2 * r] = make_float2(v.x, v.y);
-      x[2 * r + 1] = make_float2(v.z, v.w);
-    }
-    low_permute<0, 1>(p, lds, x, lane);
-#pragma unroll
-    for (int i = 0; i < T; i++) {
-      if constexpr (A >= 2) if (i == 4) low_permute<1, 2>(p, lds, x, lane);
-      if constexpr (A >= 3) if (i == 8) low_permute<2, 3>(p, lds, x, lane);
-      const ChainStep __attribute__((address_space(4))) &st = steps[opaque_step(i, x[0].x)];
-      if (i >= S) chain_step<5, 0, REAL>(x, st, e[i]);
-      else if (i % 4 == 0) chain_step<5, 1, REAL>(x, st, e[i]);
-      else if (i % 4 == 1) chain_step<5, 2, REAL>(x, st, e[i]);
-      else if (i % 4 == 2) chain_step<5, 3, REAL>(x, st, e[i]);
-      else chain_step<5, 4, REAL>(x, st, e[i]);
-    }
-    low_permute<A, 0>(p, lds, x, lane);
+    low_permute<0, 1>(p, lds, x, lane, wave);
+    low_steps<0, S, T, MODE>(p, lds, x, e, steps, lane, wave);
+    low_permute<A, 0>(p, lds, x, lane, wave);
 #pragma unroll
     for (int r = 0; r < 16; r++)
-      st4<true>(pair_ptr(p.a, ub | (uint64_t(r) << 7), lane_bytes),
+      st4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << 8), lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
   }
 }
@@ -1430,7 +1507,7 @@ void launch_diag(hipStream_t s, int nl, const DiagArgs &d) {
 
 template <bool NT>
 void launch_pass_t(hipStream_t s, int nl, const PassArgs &p) {
-  const unsigned grid = grid_for(p.lay.nthreads);
+  const unsigned grid = grid_for(p.lay.nthreads, kPassGrid);
   switch (nl) {
     case 4: k_pass<4, NT><<<grid, kT, 0, s>>>(p); break;
     case 5: k_pass<5, NT><<<grid, kT, 0, s>>>(p); break;
@@ -1445,7 +1522,7 @@ void launch_pass(hipStream_t s, int nl, const PassArgs &p) {
 
 template <int NL, int K, bool Z>
 void launch_chain_k(hipStream_t s, bool nt, bool real, const ChainArgs &c) {
-  const unsigned grid = grid_for(c.lay.nthreads);
+  const unsigned grid = grid_for(c.lay.nthreads, kPassGrid);
   if (nt && real) k_chain<NL, true, K, Z, true><<<grid, kT, 0, s>>>(c);
   else if (nt) k_chain<NL, true, K, Z, false><<<grid, kT, 0, s>>>(c);
   else if (real) k_chain<NL, false, K, Z, true><<<grid, kT, 0, s>>>(c);
@@ -1470,29 +1547,46 @@ void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const Chai
   }
 }
 
-void launch_wide(hipStream_t s, bool real, const WideArgs &w) {
-  const unsigned grid = unsigned(std::min<uint64_t>(w.lay.nthreads, kMaxGrid));
-  if (real) k_wide<true><<<grid, kT, 0, s>>>(w);
-  else k_wide<false><<<grid, kT, 0, s>>>(w);
+template <int WB>
+void launch_wide_wb(hipStream_t s, int mode, const WideArgs &w) {
+  const unsigned grid = unsigned(std::min<uint64_t>(w.lay.nthreads, kPassGrid));
+  if (mode == 2) k_wide<WB, 2><<<grid, 64 << WB, 0, s>>>(w);
+  else if (mode == 1) k_wide<WB, 1><<<grid, 64 << WB, 0, s>>>(w);
+  else k_wide<WB, 0><<<grid, 64 << WB, 0, s>>>(w);
 }
+
+void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w) {
+  if (steps == 7) launch_wide_wb<3>(s, mode, w);
+  else launch_wide_wb<2>(s, mode, w);
+}
+
+// Workgroups of a low pass: enough to fill the chip many times over, few
+// enough that the one spare prefetch per workgroup stays negligible.
+constexpr uint64_t kLowGrid = 40960;
 
 template <int S>
-void launch_low_s(hipStream_t s, bool zero, bool real, const LowArgs &l) {
-  const unsigned grid = unsigned(std::min<uint64_t>(l.lay.nthreads, kMaxGrid));
-  if (zero && real) k_low<S, true, true><<<grid, 64, 0, s>>>(l);
-  else if (zero) k_low<S, true, false><<<grid, 64, 0, s>>>(l);
-  else if (real) k_low<S, false, true><<<grid, 64, 0, s>>>(l);
-  else k_low<S, false, false><<<grid, 64, 0, s>>>(l);
+void launch_low_s(hipStream_t s, bool zero, int mode, const LowArgs &l) {
+  const unsigned grid = unsigned(std::min<uint64_t>(l.lay.nthreads, kLowGrid));
+  if (zero) {
+    if (mode == 2) k_low<S, true, 2><<<grid, 128, 0, s>>>(l);
+    else if (mode == 1) k_low<S, true, 1><<<grid, 128, 0, s>>>(l);
+    else k_low<S, true, 0><<<grid, 128, 0, s>>>(l);
+  } else {
+    if (mode == 2) k_low<S, false, 2><<<grid, 128, 0, s>>>(l);
+    else if (mode == 1) k_low<S, false, 1><<<grid, 128, 0, s>>>(l);
+    else k_low<S, false, 0><<<grid, 128, 0, s>>>(l);
+  }
 }
 
-void launch_low(hipStream_t s, int steps, bool zero, bool real, const LowArgs &l) {
+void launch_low(hipStream_t s, int steps, bool zero, int mode, const LowArgs &l) {
   switch (steps) {
-    case 5: launch_low_s<5>(s, zero, real, l); break;
-    case 6: launch_low_s<6>(s, zero, real, l); break;
-    case 7: launch_low_s<7>(s, zero, real, l); break;
-    case 8: launch_low_s<8>(s, zero, real, l); break;
-    case 9: launch_low_s<9>(s, zero, real, l); break;
-    case 10: launch_low_s<10>(s, zero, real, l); break;
+    case 5: launch_low_s<5>(s, zero, mode, l); break;
+    case 6: launch_low_s<6>(s, zero, mode, l); break;
+    case 7: launch_low_s<7>(s, zero, mode, l); break;
+    case 8: launch_low_s<8>(s, zero, mode, l); break;
+    case 9: launch_low_s<9>(s, zero, mode, l); break;
+    case 10: launch_low_s<10>(s, zero, mode, l); break;
+    case 11: launch_low_s<11>(s, zero, mode, l); break;
     default: break;
   }
 }

@@ -129,8 +129,8 @@ uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) 
   wires.clear();
   if (p.n < kWideMinQubits) return 0;
   const uint32_t used =
-      chain_steps(p, k, kWideSteps, [](uint32_t w) { return w >= kWideMinWire; }, wires);
-  if (wires.size() < kWideSteps) {
+      chain_steps(p, k, kWideMaxSteps, [](uint32_t w) { return w >= kWideMinWire; }, wires);
+  if (wires.size() < kWideMinSteps) {
     wires.clear();
     return 0;
   }

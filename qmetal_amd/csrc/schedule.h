@@ -38,12 +38,14 @@ struct Pass {
   uint64_t fixed = 0;          // fixed-one bits (merged)
 };
 
-// Wide passes: six chain steps -- each an uncontrolled Dense1q on a distinct
-// wire at or above kWideMinWire, optionally followed by a phase layer whose
-// every mask (weight 1 or 2) holds that wire, with no global phase -- over a
-// workgroup tile of 2^13 amplitudes whose six lowest free positions are the
-// lanes of a 1 KB run. States below kWideMinQubits use ordinary passes.
-constexpr uint32_t kWideSteps = 6;
+// Wide passes: six or seven chain steps -- each an uncontrolled Dense1q on a
+// distinct wire at or above kWideMinWire, optionally followed by a phase
+// layer whose every mask (weight 1 or 2) holds that wire, with no global
+// phase -- over a workgroup tile of 2^(7+steps) amplitudes whose six lowest
+// free positions are the lanes of a 1 KB run. States below kWideMinQubits
+// use ordinary passes.
+constexpr uint32_t kWideMinSteps = 6;
+constexpr uint32_t kWideMaxSteps = 7;
 constexpr uint32_t kWideMinWire = 7;
 constexpr uint32_t kWideMinQubits = 16;
 
@@ -53,9 +55,9 @@ uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires);
 
 // Low passes: kLowMinSteps to kLowMaxWire chain steps (as for wide passes)
 // on distinct wires 1..kLowMaxWire, optionally followed by one on wire 0,
-// over tiles of 2^(kLowMaxWire+1) contiguous amplitudes that one wave
-// permutes through LDS. States below kLowMinQubits use ordinary passes.
-constexpr uint32_t kLowMaxWire = 10;
+// over tiles of 2^(kLowMaxWire+1) contiguous amplitudes that two waves
+// permute through LDS. States below kLowMinQubits use ordinary passes.
+constexpr uint32_t kLowMaxWire = 11;
 constexpr uint32_t kLowMinSteps = 5;
 constexpr uint32_t kLowMinQubits = 12;
 

@@ -378,9 +378,10 @@ struct Simulator::Impl {
     // Wide chain pass (six steps over a workgroup tile).
     bool wide = false;
     WideArgs w{};
-    // Low chain pass (steps on wires 1..10 and 0 over per-wave tiles).
+    // Low chain pass (steps on wires 1..11 and 0 over two-wave tiles).
     bool low = false;
     LowArgs l{};
+    int mode = 0;  // wide / low step mode (launch_wide)
   };
 
   // Local coefficients of a phase op of a merged pass (turns).
@@ -697,15 +698,34 @@ struct Simulator::Impl {
     return true;
   }
 
+  // Step mode of a wide or low pass (launch_wide): 2 when every step is
+  // s_i [[1,1],[1,-1]] -- then the last step's matrix takes the product of
+  // the earlier s_i and the others run unscaled -- else 1 when all are real,
+  // else 0.
+  static int steps_mode(ChainStep *cs, int count) {
+    if (!steps_real(cs, count)) return 0;
+    double scale = 1.0;
+    for (int i = 0; i < count; i++) {
+      const float *m = cs[i].m;
+      if (m[0] == 0.0f || m[2] != m[0] || m[4] != m[0] || m[6] != -m[0]) return 1;
+      if (i + 1 < count) scale *= double(m[0]);
+    }
+    for (int j = 0; j < 8; j += 2) cs[count - 1].m[j] = float(double(cs[count - 1].m[j]) * scale);
+    return 2;
+  }
+
   // Wide chain pass: the six steps' constants and base-index terms, folded
   // against the tile arrangement each step runs in (WideArgs).
   void prep_wide(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
-    if (ps.reg.size() != kWideSteps || ps.fixed) fail("internal: wide pass shape");
+    const int steps = int(ps.reg.size());
+    if (steps < int(kWideMinSteps) || steps > int(kWideMaxSteps) || ps.fixed)
+      fail("internal: wide pass shape");
+    const int wb = steps - 4;  // wave bits
     const uint32_t *w = ps.reg.data();
     WideArgs &wa = pl.w;
     wa = WideArgs{};
     uint64_t tile = 1ull;
-    for (int i = 0; i < 6; i++) {
+    for (int i = 0; i < steps; i++) {
       wa.pos[i] = uint8_t(w[i]);
       tile |= 1ull << w[i];
     }
@@ -714,7 +734,7 @@ struct Simulator::Impl {
       wa.lanepos[got++] = uint8_t(q);
       tile |= 1ull << q;
     }
-    // Tiles: insert zeros at the 13 tile positions, lowest first.
+    // Tiles: insert zeros at the tile positions, lowest first.
     Layout &L = wa.lay;
     L = Layout{};
     for (uint32_t q = 0; q < 64; q++)
@@ -726,13 +746,18 @@ struct Simulator::Impl {
     L.nthreads = dim >> L.nins;
     L.streaming = 1;
     // Arrangements: local bit -> wire (0 = physical bit 0), and the
-    // thread-varying bits (lanes and the two wave bits).
+    // thread-varying bits (lanes and the wave bits). The exchange swaps local
+    // bits 1..wb with the wave bits.
     uint64_t lanes = 0;
     for (int j = 0; j < 6; j++) lanes |= 1ull << wa.lanepos[j];
     const uint32_t loc_a[5] = {0, w[0], w[1], w[2], w[3]};
-    const uint32_t loc_b[5] = {0, w[4], w[5], w[2], w[3]};
-    const uint64_t var_a = lanes | (1ull << w[4]) | (1ull << w[5]);
-    const uint64_t var_b = lanes | (1ull << w[0]) | (1ull << w[1]);
+    uint32_t loc_b[5] = {0, w[0], w[1], w[2], w[3]};
+    uint64_t var_a = lanes, var_b = lanes;
+    for (int j = 0; j < wb; j++) {
+      loc_b[1 + j] = w[4 + j];
+      var_a |= 1ull << w[4 + j];
+      var_b |= 1ull << w[j];
+    }
     ChainStep *cs = reinterpret_cast<ChainStep *>(staging + off);
     std::vector<DiagTerm> uni, lan;
     int step = -1;
@@ -765,27 +790,28 @@ struct Simulator::Impl {
       cs[step].beta0 = beta0[b];
       step_tables(cs[step], b, gamma);
     }
-    if (step != 5) fail("internal: wide pass steps");
-    const bool real = steps_real(cs, 6);
-    DiagTerm *tu = reinterpret_cast<DiagTerm *>(staging + off + 6 * sizeof(ChainStep));
+    if (step != steps - 1) fail("internal: wide pass steps");
+    pl.mode = steps_mode(cs, steps);
+    DiagTerm *tu = reinterpret_cast<DiagTerm *>(staging + off + size_t(steps) * sizeof(ChainStep));
     std::copy(uni.begin(), uni.end(), tu);
     std::copy(lan.begin(), lan.end(), tu + uni.size());
     wa.a = state;
     wa.steps = reinterpret_cast<const ChainStep *>(arena + off);
-    wa.uni = reinterpret_cast<const DiagTerm *>(arena + off + 6 * sizeof(ChainStep));
+    wa.uni = reinterpret_cast<const DiagTerm *>(arena + off + size_t(steps) * sizeof(ChainStep));
     wa.lane = wa.uni + uni.size();
     wa.nuni = uint32_t(uni.size());
     wa.nlane = uint32_t(lan.size());
     pl.wide = true;
-    pl.chain_real = real;
+    pl.chain_k = steps;
     pl.nl = 5;
     pl.bytes = pass_bytes(n, 0);
   }
 
   // Low chain pass (LowArgs): arrangement 0 is the coalesced one (lanes on
-  // bits 1..6, registers on 7..10); arrangement 1 + a holds steps 4a..4a+3 on
-  // register bits, filled up with the lowest unused wires, and lanes on the
-  // remaining wires of 1..10, ascending. The wire-0 step runs in the last.
+  // bits 1..6, the wave bit on 7, registers on 8..11); arrangement 1 + a
+  // holds steps 4a..4a+3 on register bits, filled up with the lowest unused
+  // wires, and lanes then the wave bit on the remaining wires of 1..11,
+  // ascending. The wire-0 step runs in the last.
   void prep_low(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
     const std::vector<uint32_t> &w = ps.reg;
     const bool zero = !w.empty() && w.back() == 0;
@@ -795,8 +821,8 @@ struct Simulator::Impl {
     const int A = (S + 3) / 4;
     LowArgs &la = pl.l;
     la = LowArgs{};
-    for (int j = 0; j < 6; j++) la.lanepos[0][j] = uint8_t(1 + j);
-    for (int k = 0; k < 4; k++) la.regpos[0][k] = uint8_t(7 + k);
+    for (int j = 0; j < 7; j++) la.lanepos[0][j] = uint8_t(1 + j);
+    for (int k = 0; k < 4; k++) la.regpos[0][k] = uint8_t(8 + k);
     uint64_t var[kLowMaxArr] = {};
     for (int a = 0; a < A; a++) {
       uint32_t used = 1u;
@@ -816,7 +842,7 @@ struct Simulator::Impl {
           la.lanepos[1 + a][nl++] = uint8_t(q);
           var[1 + a] |= 1ull << q;
         }
-      if (nl != 6) fail("internal: low pass lanes");
+      if (nl != 7) fail("internal: low pass lanes");
     }
     Layout &L = la.lay;
     L = Layout{};
@@ -873,7 +899,7 @@ struct Simulator::Impl {
     pl.low = true;
     pl.chain_k = S;
     pl.chain_zero = zero;
-    pl.chain_real = steps_real(cs, T);
+    pl.mode = steps_mode(cs, T);
     pl.nl = 5;
     pl.bytes = pass_bytes(n, 0);
   }
@@ -884,8 +910,8 @@ struct Simulator::Impl {
       a = take_event();
       check(hipEventRecord(a, stream), "hipEventRecord");
     }
-    if (pl.wide) launch_wide(stream, pl.chain_real, pl.w);
-    else if (pl.low) launch_low(stream, pl.chain_k, pl.chain_zero, pl.chain_real, pl.l);
+    if (pl.wide) launch_wide(stream, pl.chain_k, pl.mode, pl.w);
+    else if (pl.low) launch_low(stream, pl.chain_k, pl.chain_zero, pl.mode, pl.l);
     else if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);
     else launch_pass(stream, pl.nl, pl.a);
     launched();

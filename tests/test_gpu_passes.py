@@ -136,7 +136,7 @@ def test_qft_24_merged_vs_op_by_op():
     ref, _, ref_launches = run(n, C.qft(n), merge=False, prep=prep)
     dm = np.abs(got - ref).max()
     print(f"QFT24: {launches} launches vs {ref_launches}; merged vs op-by-op {dm:.2e}")
-    assert launches == 4 and ref_launches == 2 * n - 1
+    assert launches == 3 and ref_launches == 2 * n - 1
     assert dm <= TOL_SAME and abs(norm - 1) <= TOL1
 
 
@@ -206,23 +206,34 @@ def test_merged_adjoint_against_oracle_and_per_gate_sweep():
                     assert g_m[k] == 0.0
 
 
-def wide_stream(n, rounds, seed, real):
-    """Rounds of six chain steps on distinct wires >= 7: a 2x2 (H, or a U3
-    when complex) followed by phase gates that all hold that wire (CP, CZ, P
-    onto any other wire: pass wires, lane wires and workgroup-uniform wires)."""
+def step_gate(r, w, kind):
+    """A chain step's 2x2: H ("h": the butterfly mode), RY ("real": a real
+    matrix), U3 ("cplx")."""
+    if kind == "h":
+        return (C.H, (w,), ())
+    if kind == "real":
+        return (C.RY, (w,), (r.uniform(-3.2, 3.2),))
+    return (C.U3, (w,), tuple(r.uniform(-3.2, 3.2) for _ in range(3)))
+
+
+STEP_KINDS = ((1, "h"), (2, "cplx"), (3, "real"))
+
+
+def wide_stream(n, rounds, seed, kind, steps=6):
+    """Rounds of `steps` chain steps on distinct wires >= 7: a 2x2 (step_gate)
+    followed by phase gates that all hold that wire (CP, CZ, P onto any other
+    wire: pass wires, lane wires and workgroup-uniform wires); a CX separates
+    the rounds."""
     r = C.SplitMix64(seed)
     gates = []
     for _ in range(rounds):
         ws = []
-        while len(ws) < 6:
+        while len(ws) < steps:
             q = 7 + r.next_u64() % (n - 7)
             if q not in ws:
                 ws.append(q)
         for w in ws:
-            if real:
-                gates.append((C.H, (w,), ()))
-            else:
-                gates.append((C.U3, (w,), tuple(r.uniform(-3.2, 3.2) for _ in range(3))))
+            gates.append(step_gate(r, w, kind))
             for j in range(1 + r.next_u64() % 4):
                 k = r.next_u64() % n
                 if k == w:
@@ -235,52 +246,52 @@ def wide_stream(n, rounds, seed, real):
                     gates.append((C.CZ, (w, k), ()))
                 else:
                     gates.append((C.P, (w,), (r.uniform(-3.2, 3.2),)))
+        gates.append((C.CX, (0, n - 1), ()))
     return gates
 
 
-def test_wide_passes_against_oracle_and_op_by_op():
-    """Six-step chains over a workgroup tile (two steps after the LDS
-    exchange), real and complex, with phase terms on every kind of bit."""
+@pytest.mark.parametrize("steps", [6, 7])
+def test_wide_passes_against_oracle_and_op_by_op(steps):
+    """Six- and seven-step chains over a workgroup tile (two or three steps
+    after the LDS exchange), real and complex, with phase terms on every kind
+    of bit."""
     for n in (16, 18):
-        for seed, real in ((1, True), (2, False), (3, False)):
+        for seed, kind in STEP_KINDS:
             prep = C.product_state_prep(n, seed=seed)
-            gates = wide_stream(n, 2, 100 * n + seed, real)
+            gates = wide_stream(n, 2, 100 * n + seed + 7 * steps, kind, steps)
             c = C.to_circuit(n, gates)
             plan, passes = c.plan(), c.passes()
             wide = [p for p in set(passes) if p >= 0 and sum(
-                1 for k, q in enumerate(passes) if q == p and plan[k]["kind"] == "Dense1q") == 6]
-            assert wide, "no wide pass formed"
+                1 for k, q in enumerate(passes) if q == p and plan[k]["kind"] == "Dense1q") == steps]
+            assert len(wide) == 2, "wide passes not formed"
             want = O.run(n, prep + gates)
             got, norm, launches = run(n, gates, prep=prep)
             ref, _, _ = run(n, gates, merge=False, prep=prep)
             d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
-            print(f"wide n={n} seed={seed} real={real}: {len(wide)} wide passes, {launches} launches, "
+            print(f"wide{steps} n={n} seed={seed} {kind}: {len(wide)} wide passes, {launches} launches, "
                   f"diff {d:.2e}, vs op-by-op {dm:.2e}")
             assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
             assert dm <= TOL_SAME
 
 
-def low_stream(n, rounds, seed, real):
-    """Rounds of 5..10 chain steps on distinct wires 1..10, some closed by a
-    step on wire 0, each step a 2x2 (H, or a U3 when complex) followed by
-    phase gates that all hold its wire (onto register, lane and
+def low_stream(n, rounds, seed, kind):
+    """Rounds of 5..11 chain steps on distinct wires 1..11, some closed by a
+    step on wire 0, each step a 2x2 (step_gate) followed by
+    phase gates that all hold its wire (onto register, lane, wave and
     workgroup-uniform wires); a CX separates the rounds."""
     r = C.SplitMix64(seed)
     gates = []
     for i in range(rounds):
-        count = 5 + r.next_u64() % 6
+        count = 5 + r.next_u64() % 7
         ws = []
         while len(ws) < count:
-            q = 1 + r.next_u64() % 10
+            q = 1 + r.next_u64() % 11
             if q not in ws:
                 ws.append(q)
         if i % 2 == 0:
             ws.append(0)
         for w in ws:
-            if real:
-                gates.append((C.H, (w,), ()))
-            else:
-                gates.append((C.U3, (w,), tuple(r.uniform(-3.2, 3.2) for _ in range(3))))
+            gates.append(step_gate(r, w, kind))
             for j in range(1 + r.next_u64() % 4):
                 k = r.next_u64() % n
                 if k == w:
@@ -297,13 +308,13 @@ def low_stream(n, rounds, seed, real):
 
 
 def test_low_passes_against_oracle_and_op_by_op():
-    """Chains of 5..10 steps on wires 1..10 (and 0) over per-wave tiles
+    """Chains of 5..11 steps on wires 1..11 (and 0) over two-wave tiles
     permuted through LDS, real and complex, with phase terms on every kind
     of bit."""
     for n in (12, 14, 17):
-        for seed, real in ((1, True), (2, False), (3, False)):
+        for seed, kind in STEP_KINDS:
             prep = C.product_state_prep(n, seed=seed)
-            gates = low_stream(n, 4, 300 * n + seed, real)
+            gates = low_stream(n, 4, 300 * n + seed, kind)
             c = C.to_circuit(n, gates)
             plan, passes = c.plan(), c.passes()
             low = [p for p in set(passes) if p >= 0 and sum(
@@ -313,7 +324,7 @@ def test_low_passes_against_oracle_and_op_by_op():
             got, norm, launches = run(n, gates, prep=prep)
             ref, _, _ = run(n, gates, merge=False, prep=prep)
             d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
-            print(f"low n={n} seed={seed} real={real}: {len(low)} low passes, {launches} launches, "
+            print(f"low n={n} seed={seed} {kind}: {len(low)} low passes, {launches} launches, "
                   f"diff {d:.2e}, vs op-by-op {dm:.2e}")
             assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
             assert dm <= TOL_SAME

@@ -53,8 +53,8 @@ def check_grouping(circ, options=None):
             if len(wires) > MAX_REG:
                 # a chain pass: Dense1q steps on distinct wires, each followed
                 # at most by a phase layer whose masks all hold its wire;
-                # wide: six steps on wires >= 7; low: 5..10 steps on wires
-                # 1..10, then at most one on wire 0
+                # wide: six or seven steps on wires >= 7; low: 5..11 steps on
+                # wires 1..11, then at most one on wire 0
                 steps = [o["qubits"][0] for o in ops if o["kind"] == "Dense1q"]
                 assert len(set(steps)) == len(steps), steps
                 last = None
@@ -65,21 +65,22 @@ def check_grouping(circ, options=None):
                         assert o["kind"] == "DiagonalLayer" and last is not None
                         assert all((m >> last) & 1 for m in o["masks"])
                         last = None
-                if min(steps) >= 7 and len(steps) == 6:
+                if min(steps) >= 7 and len(steps) in (6, 7):
                     assert circ.num_qubits >= 16
                 else:
                     assert circ.num_qubits >= 12, steps
-                    assert 0 not in steps[:-1] and 5 <= len(wires) <= 10, steps
-                    assert max(steps) <= 10, steps
+                    assert 0 not in steps[:-1] and 5 <= len(wires) <= 11, steps
+                    assert max(steps) <= 11, steps
     return max(seen) + 1 if seen else 0
 
 
 def test_qft_pass_counts():
     """QFT: H on wire j then the CP ladder onto j, for j = n-1 .. 0. From 16
-    qubits six H's and their ladders on wires >= 7 share a wide pass; from
-    wire 10 (or lower) down to 0 the rest share one low pass; a wire left
-    between them gets an ordinary pass."""
-    for n, want in ((12, 2), (16, 2), (26, 4), (30, 5), (32, 5), (33, 5), (34, 5)):
+    qubits seven (else six) H's and their ladders on wires >= 7 share a wide
+    pass; from wire 11 (or lower) down to 0 the rest share one low pass; a
+    wire left between them gets an ordinary pass."""
+    for n, want in ((12, 1), (13, 2), (16, 2), (20, 3), (26, 3), (30, 4), (32, 4), (33, 4),
+                    (34, 5)):
         c = C.to_circuit(n, C.qft(n))
         assert c.plan_size(True) == 2 * n - 1
         assert check_grouping(c) == want, n
