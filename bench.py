@@ -269,13 +269,17 @@ def suite(qm, C, emit):
         emit(rec)
 
 
-def load_traffic(workload: str):
+def load_traffic(workload: str, kind: str):
+    """HBM bytes per launch of the workload's `kind` kernel, measured with
+    rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE) and kept in
+    profiles/traffic.json; None when not measured."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(workload)
+            entry = json.load(f).get(workload)
     except (OSError, ValueError):
         return None
+    return entry.get(kind) if isinstance(entry, dict) else None
 
 
 def main() -> int:
@@ -306,10 +310,18 @@ def main() -> int:
         import torch
         src = torch.ones(1 << args.copy_probe, dtype=torch.complex64, device="cuda")
         dst = torch.empty_like(src)
+        best = None
         for _ in range(3):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
             dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"copy probe: 3 x {8 << args.copy_probe} bytes read and written", flush=True)
+            b.record()
+            b.synchronize()
+            ms = a.elapsed_time(b)
+            best = ms if best is None else min(best, ms)
+        nbytes = 8 << args.copy_probe
+        print(f"copy probe: 3 x {nbytes} bytes read and written; best {best:.3f} ms = "
+              f"{2 * nbytes / (best * 1e-3) / 1e9:.1f} GB/s (read + write)", flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported",
@@ -447,7 +459,11 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(workload)},
+                     "traffic": load_traffic(workload, dom),
+                     "kernels": {k: {"launches": c // args.steps,
+                                     "avg_ms": round(ms / c, 3),
+                                     "GBps": round(by / (ms * 1e-3) / 1e9, 1)}
+                                 for k, (c, ms, by) in sorted(kinds.items())}},
         "cpu_baseline": cpu,
     }
     if op_by_op is not None:
