@@ -6,8 +6,8 @@ events on the simulator's stream and the complex128 CPU oracle timed beside it.
     python bench.py [--steps K] [--warmup W] [--n 33] [--cpu-n 24] [--no-cpu]
 
 One step is one qm_run of QFT(n) (n(n+1)/2 gates, a plan of 2n-1 fused ops,
-which the executor merges into about n/4 read-modify-write passes over HBM;
---no-merge launches one kernel per plan op) on the resident state. Inputs are synthetic: RY(theta_q), RZ(phi_q) on every wire with
+which the executor merges into a few read-modify-write passes over HBM, 4 at
+n=33; --no-merge launches one kernel per plan op) on the resident state. Inputs are synthetic: RY(theta_q), RZ(phi_q) on every wire with
 theta, phi ~ U[0, 2 pi) from splitmix64(2024), applied before timing.
 """
 from __future__ import annotations
@@ -269,6 +269,89 @@ def suite(qm, C, emit):
         emit(rec)
 
 
+# Published ratios of the reference on its own device (Apple M4 Max, n=26):
+# fusion speedup per circuit (results/fusion_bench_n26.txt:5-13) and the
+# fusion x blocking grid at b=10 (results/ablation_n26_b10_extended.txt:6-15).
+REF_FUSION_N26 = {"ghz": 1.01, "qft": 8.42, "qaoa_ring_p6": 6.06, "tfim_trotter_20": 5.16,
+                  "grover_proxy_4": 5.84, "random_d10": 1.88, "hea_d3": 3.09}
+REF_ABLATION_N26 = {"fusion_alone": 4.01, "blocking_alone": 1.37, "blocking_on_fusion": 1.02}
+
+FROZEN = ("ghz", "qft", "qaoa_ring_p6", "tfim_trotter_20", "grover_proxy_4", "random_d10",
+          "hea_d3", "local_patch_w8")
+
+
+def frozen_streams(n: int):
+    """The reference's own gate streams of its extended frozen benchmark set
+    (src/circuits.cpp:158-190) at n, as recorded in the committed fixtures
+    (tests/golden/frozen_large.json.gz: n = 24, 26, 33)."""
+    import gzip
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "frozen_large.json.gz"), "rt") as f:
+        d = json.load(f)
+    out = {}
+    for name in FROZEN:
+        rec = d.get(f"{name}_{n}")
+        if rec is None:
+            raise SystemExit(f"bench.py: no recorded frozen stream {name} at n={n} (24, 26, 33)")
+        out[name] = [(k, tuple(q[:nq]), tuple(float.fromhex(x) for x in p[:np_]))
+                     for k, nq, np_, q, p in rec["gates"]]
+    return out
+
+
+def frozen(qm, C, n: int, reps: int, block_bits: int, emit):
+    """The reference's fusion_bench and ablation protocols (bench/fusion_bench.mm,
+    bench/ablation.mm) on this GPU: per frozen circuit, plans without fusion
+    (base), fused (F), blocking only (B), fused + blocking (FB), each run one
+    launch per plan op as the reference's executor does, plus the fused plan
+    under this build's pass merging (M, the default qm_run). Arms alternate,
+    each timed best of `reps` after a warm-up, twice; one JSON line per
+    circuit and one with the geometric means beside the published ones."""
+    streams = frozen_streams(n)
+    arms = {"base": qm.FusionOptions(0, 0, 0, 1, 0), "F": qm.FusionOptions(1, 1, 1, 4, 0),
+            "B": qm.FusionOptions(0, 0, 0, 1, block_bits),
+            "FB": qm.FusionOptions(1, 1, 1, 4, block_bits)}
+    geo = {"F/base": 0.0, "B/base": 0.0, "FB/F": 0.0, "M/F": 0.0, "M/base": 0.0}
+    with qm.Simulator(n) as s:
+        def time_arm(c, opts, merge):
+            s.set_pass_merging(merge)
+            best = math.inf
+            for r in range(reps + 1):
+                s.reset()
+                s.synchronize()
+                t0 = time.perf_counter()
+                s.run(c, options=opts)
+                s.synchronize()
+                dt = time.perf_counter() - t0
+                if r > 0:
+                    best = min(best, dt)
+            return best
+
+        for name in FROZEN:
+            c = C.to_circuit(n, streams[name])
+            t = {}
+            for rnd in range(2):
+                for arm, opts in arms.items():
+                    t[arm] = min(t.get(arm, math.inf), time_arm(c, opts, False))
+                    print(f"# {name} round {rnd} {arm}: {t[arm]:.4f} s", file=sys.stderr, flush=True)
+                t["M"] = min(t.get("M", math.inf), time_arm(c, arms["F"], True))
+            s.set_pass_merging(True)
+            rec = {"config": f"frozen_{name}_n{n}", "gates": len(c),
+                   "plan_ops": {a: len(c.plan(o)) for a, o in arms.items()},
+                   "launches_M": max(c.passes(arms["F"])) + 1,
+                   "s": {a: round(v, 6) for a, v in t.items()},
+                   "F/base": round(t["base"] / t["F"], 3), "B/base": round(t["base"] / t["B"], 3),
+                   "FB/F": round(t["F"] / t["FB"], 3), "M/F": round(t["F"] / t["M"], 3),
+                   "M/base": round(t["base"] / t["M"], 3)}
+            if n == 26 and name in REF_FUSION_N26:
+                rec["reference_F/base_m4max"] = REF_FUSION_N26[name]
+            for k in geo:
+                geo[k] += math.log(rec[k])
+            emit(rec)
+    k = len(FROZEN)
+    emit({"config": f"frozen_geomean_n{n}", "block_bits": block_bits,
+          **{k2: round(math.exp(v / k), 3) for k2, v in geo.items()},
+          "reference_m4max_n26": REF_ABLATION_N26})
+
+
 def load_traffic(workload: str, kind: str):
     """HBM bytes per launch of the workload's `kind` kernel, measured with
     rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE) and kept in
@@ -301,6 +384,11 @@ def main() -> int:
                     help="instances solved one circuit per launch for the comparison")
     ap.add_argument("--suite", action="store_true",
                     help="measure every configuration (one JSON line each) instead of the headline")
+    ap.add_argument("--frozen", type=int, default=0, metavar="N",
+                    help="run the reference's fusion_bench / ablation protocols on its frozen "
+                         "benchmark set at n=N (24, 26 or 33) instead of the headline")
+    ap.add_argument("--frozen-reps", type=int, default=2)
+    ap.add_argument("--block-bits", type=int, default=10)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only run a torch device copy of 2^N complex64 (8 * 2^N bytes read "
                          "and written) three times: the known-size kernel that calibrates the "
@@ -336,6 +424,11 @@ def main() -> int:
     if args.suite:
         print(f"# device {qm.device_name()}  suite", file=sys.stderr, flush=True)
         suite(qm, C, lambda rec: print(json.dumps(rec), flush=True))
+        return 0
+    if args.frozen:
+        print(f"# device {qm.device_name()}  frozen set n={args.frozen}", file=sys.stderr, flush=True)
+        frozen(qm, C, args.frozen, args.frozen_reps, args.block_bits,
+               lambda rec: print(json.dumps(rec), flush=True))
         return 0
 
     n = args.n
