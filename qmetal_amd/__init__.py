@@ -191,8 +191,9 @@ _U3 = 18
 PLAN_KINDS = ("Dense1q", "Dense1qGroup", "Blocked1q", "Controlled1q", "DiagonalLayer",
               "Swap", "CCX")
 # Timing records: plan-op kinds, then merged passes of several plan ops
-# (generic pass kernel, chain kernel, wide and low chain kernels).
-TIMING_KINDS = PLAN_KINDS + ("Pass", "ChainPass", "WidePass", "LowPass")
+# (generic pass kernel, chain kernel, wide and low chain kernels, generic
+# low tile kernel).
+TIMING_KINDS = PLAN_KINDS + ("Pass", "ChainPass", "WidePass", "LowPass", "TilePass")
 
 
 def version() -> tuple[int, int, int]:

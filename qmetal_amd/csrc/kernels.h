@@ -211,6 +211,34 @@ struct LowArgs {
   uint32_t nlane;
 };
 
+// Generic low tile pass: merged-pass ops on wires 0..11 over workgroup tiles
+// of 2^12 contiguous amplitudes held by two waves, in segments. Segment s
+// runs ops [op0, op1) (PassOps with NL = 5) in its own arrangement of the
+// tile: local bit 0 = wire 0, local bits 1..4 = the wires at rpos[0..3], the
+// thread bits (lanes 0..5, then the wave bit) on the wires at tpos[0..6].
+// `tslot` / `rslot` are the LDS slot bases (float4 units) of those bits. The
+// tile is loaded and stored coalesced (lanes on 1..6, wave bit 7, registers
+// 8..11) and permuted through LDS before each segment and after the last.
+struct TileSeg {
+  uint32_t op0, op1;
+  uint32_t tslot[7];
+  uint32_t rslot[4];
+  uint8_t tpos[7];
+  uint8_t pad;
+};
+struct TileArgs {
+  float2 *a;
+  Layout lay;  // tiles: ins = bits 0..11
+  const TileSeg *segs;
+  uint32_t nsegs;
+  const PassOp *ops;
+  const DiagTerm *terms;
+};
+
+// LDS slot (float4 index) of tile index i, i = (tile bits 1..11) >> 1: the
+// low four bits are xored with the higher ones (linear over GF(2)).
+constexpr uint32_t tile_swizzle(uint32_t i) { return i ^ (((i >> 4) ^ (i >> 8)) & 0xFu); }
+
 void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim);
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim);
 void launch_gather(hipStream_t s, int nl, const GatherArgs &g);
@@ -228,6 +256,7 @@ void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const Chai
 void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w);
 // s in 5..11 steps on wires 1..11, `zero`: a last step on wire 0.
 void launch_low(hipStream_t s, int steps, bool zero, int mode, const LowArgs &l);
+void launch_tile(hipStream_t s, const TileArgs &t);
 
 // FP64 reductions. Each writes per-block partials into `partials` (capacity
 // kRedBlocks doubles or double2) and then reduces them in a fixed order into

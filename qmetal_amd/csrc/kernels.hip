@@ -393,6 +393,109 @@ __device__ __forceinline__ const_ptr<T> as_const(const T *p) {
   return (const_ptr<T>)p;
 }
 
+// Ops [o0, o1) of a merged pass applied to the thread's 2^NL amplitudes:
+// `ub` is the workgroup's base index (the bits every thread shares), `base`
+// the thread's (ub plus its thread-index bits).
+template <int NL>
+__device__ __forceinline__ void pass_ops(float2 (&x)[1 << NL], const_ptr<PassOp> ops,
+                                         const_ptr<DiagTerm> terms, uint32_t o0, uint32_t o1,
+                                         uint64_t ub, uint64_t base, uint32_t lane) {
+  // Every op runs both statements that write the tile, the 2x2 slot and
+  // the per-combination phase multiply, each of which skips itself inside
+  // (a branch around either would be a second definition of the tile).
+  for (uint32_t o = o0; o < o1; o++) {
+    const bool is_phase = ops[o].type == kPassPhase;
+    const uint64_t gmask = ops[o].gmask;
+    const uint32_t cm = ops[o].cmask;
+    const bool gon = (base & gmask) == gmask;
+    {
+      // A control on a base-index bit: the identity where this lane's
+      // base lacks it. A phase op's section code matches nothing.
+      float m[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float mi = ops[o].m[i];
+        m[i] = gon ? mi : (i == 0 || i == 6 ? 1.0f : 0.0f);
+      }
+      slot_asm<NL>(x, __builtin_amdgcn_readfirstlane(ops[o].sel),
+                   __builtin_amdgcn_readfirstlane(ops[o].pm), m);
+    }
+    // The phase multiply, for a phase op only (a uniform branch around
+    // in-place statements: the tile keeps its registers on both paths).
+    if (is_phase) {
+      const uint32_t bmask = ops[o].bmask;
+      const bool has_phi = ops[o].has_phi != 0;
+      float2 e[NL];
+#pragma unroll
+      for (int j = 0; j < NL; j++) e[j] = make_float2(1.0f, 0.0f);
+      float2 ephi = make_float2(1.0f, 0.0f);
+      // Workgroup-uniform terms: one term per lane, summed over the wave
+      // (every lane takes part, active or not).
+      const uint32_t nuniform = ops[o].nuniform, nterms = ops[o].nterms;
+      const uint32_t umask = ops[o].umask;
+      const const_ptr<DiagTerm> tl = terms + ops[o].term0;
+      double phi = 0.0;
+      double beta[NL];
+#pragma unroll
+      for (int j = 0; j < NL; j++) beta[j] = 0.0;
+      for (uint32_t k0 = 0; k0 < nuniform; k0 += 64) {
+        // Branch-free: lanes past the end read the last term and add zero.
+        const uint32_t k = k0 + lane;
+        const uint32_t kc = k < nuniform ? k : nuniform - 1;
+        const uint64_t mask = tl[kc].mask;
+        const double add = (k < nuniform && (ub & mask) == mask) ? double(tl[kc].ang) : 0.0;
+        const uint32_t kind = tl[kc].kind, kk = tl[kc].k;
+        phi += kind != 2 ? add : 0.0;
+#pragma unroll
+        for (int j = 0; j < NL; j++) beta[j] += (kind == 2 && kk == uint32_t(j)) ? add : 0.0;
+      }
+      if (umask & 1u) phi = wave_allsum(phi);
+#pragma unroll
+      for (int j = 0; j < NL; j++)
+        if ((umask >> (j + 1)) & 1u) beta[j] = wave_allsum(beta[j]);
+      for (uint32_t k = nuniform; k < nterms; k++) {
+        const uint64_t mask = tl[k].mask;
+        const double add = (base & mask) == mask ? double(tl[k].ang) : 0.0;
+        const uint32_t kind = tl[k].kind, kk = tl[k].k;
+        if (kind == 2) {
+#pragma unroll
+          for (int j = 0; j < NL; j++) beta[j] += kk == uint32_t(j) ? add : 0.0;
+        } else {
+          phi += add;
+        }
+      }
+      // One FP64-reduced sin/cos per thread-dependent sum; each amplitude
+      // then takes a table entry times the factors of its local bits.
+#pragma unroll
+      for (int j = 0; j < NL; j++) {
+        if ((bmask >> j) & 1u) {
+          float sn, cs;
+          turn_sincos(beta[j] + ops[o].beta0[j], sn, cs);
+          e[j] = make_float2(cs, sn);
+        }
+      }
+      if (has_phi) {
+        float sn, cs;
+        turn_sincos(phi, sn, cs);
+        ephi = make_float2(cs, sn);
+      }
+#pragma unroll
+      for (int l = 0; l < (1 << NL); l++) {
+        float2 f = make_float2(ops[o].table[2 * l], ops[o].table[2 * l + 1]);
+        if (has_phi) f = cmul(f, ephi);
+#pragma unroll
+        for (int h = 0; h < NL; h++)
+          if (((l >> h) & 1) && ((bmask >> h) & 1u)) f = cmul(f, e[h]);
+        // A thread whose base lacks a bit every term needs keeps phase 0.
+        if (gmask) f = gon ? f : make_float2(1.0f, 0.0f);
+        // Combinations without the local bits every term needs are left
+        // alone.
+        cmul_inplace(x[l], f, __builtin_amdgcn_readfirstlane((uint32_t(l) & cm) != cm));
+      }
+    }
+  }
+}
+
 // Merged pass: the thread's 2^NL amplitudes are loaded once, every op of the
 // pass is applied to them in plan order, and they are stored once. Ops are
 // workgroup-uniform; their descriptors are read with scalar loads.
@@ -427,100 +530,7 @@ __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
         x[2 * r + 1] = make_float2(v[r].z, v[r].w);
       }
     }
-    // Every op runs both statements that write the tile, the 2x2 slot and
-    // the per-combination phase multiply, each of which skips itself inside
-    // (a branch around either would be a second definition of the tile).
-    for (uint32_t o = 0; o < p.nops; o++) {
-      const bool is_phase = ops[o].type == kPassPhase;
-      const uint64_t gmask = ops[o].gmask;
-      const uint32_t cm = ops[o].cmask;
-      const bool gon = (base & gmask) == gmask;
-      {
-        // A control on a base-index bit: the identity where this lane's
-        // base lacks it. A phase op's section code matches nothing.
-        float m[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-          const float mi = ops[o].m[i];
-          m[i] = gon ? mi : (i == 0 || i == 6 ? 1.0f : 0.0f);
-        }
-        slot_asm<NL>(x, __builtin_amdgcn_readfirstlane(ops[o].sel),
-                     __builtin_amdgcn_readfirstlane(ops[o].pm), m);
-      }
-      // Phase factors (temporaries; computed for a phase op only).
-      const uint32_t bmask = ops[o].bmask;
-      const bool has_phi = ops[o].has_phi != 0;
-      float2 e[NL];
-#pragma unroll
-      for (int j = 0; j < NL; j++) e[j] = make_float2(1.0f, 0.0f);
-      float2 ephi = make_float2(1.0f, 0.0f);
-      if (is_phase) {
-        // Workgroup-uniform terms: one term per lane, summed over the wave
-        // (every lane takes part, active or not).
-        const uint32_t nuniform = ops[o].nuniform, nterms = ops[o].nterms;
-        const uint32_t umask = ops[o].umask;
-        const const_ptr<DiagTerm> tl = terms + ops[o].term0;
-        double phi = 0.0;
-        double beta[NL];
-#pragma unroll
-        for (int j = 0; j < NL; j++) beta[j] = 0.0;
-        for (uint32_t k0 = 0; k0 < nuniform; k0 += 64) {
-          // Branch-free: lanes past the end read the last term and add zero.
-          const uint32_t k = k0 + lane;
-          const uint32_t kc = k < nuniform ? k : nuniform - 1;
-          const uint64_t mask = tl[kc].mask;
-          const double add = (k < nuniform && (ub & mask) == mask) ? double(tl[kc].ang) : 0.0;
-          const uint32_t kind = tl[kc].kind, kk = tl[kc].k;
-          phi += kind != 2 ? add : 0.0;
-#pragma unroll
-          for (int j = 0; j < NL; j++) beta[j] += (kind == 2 && kk == uint32_t(j)) ? add : 0.0;
-        }
-        if (umask & 1u) phi = wave_allsum(phi);
-#pragma unroll
-        for (int j = 0; j < NL; j++)
-          if ((umask >> (j + 1)) & 1u) beta[j] = wave_allsum(beta[j]);
-        for (uint32_t k = nuniform; k < nterms; k++) {
-          const uint64_t mask = tl[k].mask;
-          const double add = (base & mask) == mask ? double(tl[k].ang) : 0.0;
-          const uint32_t kind = tl[k].kind, kk = tl[k].k;
-          if (kind == 2) {
-#pragma unroll
-            for (int j = 0; j < NL; j++) beta[j] += kk == uint32_t(j) ? add : 0.0;
-          } else {
-            phi += add;
-          }
-        }
-        // One FP64-reduced sin/cos per thread-dependent sum; each amplitude
-        // then takes a table entry times the factors of its local bits.
-#pragma unroll
-        for (int j = 0; j < NL; j++) {
-          if ((bmask >> j) & 1u) {
-            float sn, cs;
-            turn_sincos(beta[j] + ops[o].beta0[j], sn, cs);
-            e[j] = make_float2(cs, sn);
-          }
-        }
-        if (has_phi) {
-          float sn, cs;
-          turn_sincos(phi, sn, cs);
-          ephi = make_float2(cs, sn);
-        }
-      }
-#pragma unroll
-      for (int l = 0; l < (1 << NL); l++) {
-        float2 f = make_float2(ops[o].table[2 * l], ops[o].table[2 * l + 1]);
-        if (has_phi) f = cmul(f, ephi);
-#pragma unroll
-        for (int h = 0; h < NL; h++)
-          if (((l >> h) & 1) && ((bmask >> h) & 1u)) f = cmul(f, e[h]);
-        // A thread whose base lacks a bit every term needs keeps phase 0.
-        if (gmask) f = gon ? f : make_float2(1.0f, 0.0f);
-        // Slot ops, and combinations without the local bits every term
-        // needs, are left alone.
-        cmul_inplace(x[l], f,
-                     __builtin_amdgcn_readfirstlane(!is_phase || (uint32_t(l) & cm) != cm));
-      }
-    }
+    pass_ops<NL>(x, ops, terms, 0, p.nops, ub, base, lane);
     if (active) {
 #pragma unroll
       for (int r = 0; r < (1 << (NL - 1)); r++)
@@ -789,9 +799,7 @@ __global__ __launch_bounds__(64 << WB) void k_wide(WideArgs p) {
 // xored with the higher ones so that lanes stepping through high bits still
 // spread over the banks (a bijection: the high bits are kept; linear over
 // GF(2)).
-__device__ __forceinline__ uint32_t low_swz(uint32_t i) {
-  return i ^ (((i >> 4) ^ (i >> 8)) & 0xFu);
-}
+__device__ __forceinline__ uint32_t low_swz(uint32_t i) { return tile_swizzle(i); }
 
 // Tile index (float4 units: tile bit p is index bit p - 1) of this thread's
 // lane and wave bits in arrangement `a`.
@@ -956,6 +964,90 @@ __global__ __launch_bounds__(128) void k_low(LowArgs p) {
     low_permute<0, 1>(p, lds, x, lane, wave);
     low_steps<0, S, T, MODE>(p, lds, x, e, steps, lane, wave);
     low_permute<A, 0>(p, lds, x, lane, wave);
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+      st4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << 8), lane_bytes),
+                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
+  }
+}
+
+// Moves the tile between two arrangements given by their LDS slot bases:
+// `tf` / `tt` the thread's own slot part (from / to), `rf` / `rt` the
+// register bits' bases (workgroup-uniform).
+__device__ __forceinline__ void tile_permute(float4 *lds, float2 (&x)[32], uint32_t tf,
+                                             const uint32_t (&rf)[4], uint32_t tt,
+                                             const uint32_t (&rt)[4]) {
+  char *base = reinterpret_cast<char *>(lds);
+  __syncthreads();  // earlier reads of the buffer are done
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const uint32_t o = ((r & 1) ? rf[0] : 0u) ^ ((r & 2) ? rf[1] : 0u) ^ ((r & 4) ? rf[2] : 0u) ^
+                       ((r & 8) ? rf[3] : 0u);
+    *reinterpret_cast<float4 *>(base + ((tf ^ o) << 4)) =
+        make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const uint32_t o = ((r & 1) ? rt[0] : 0u) ^ ((r & 2) ? rt[1] : 0u) ^ ((r & 4) ? rt[2] : 0u) ^
+                       ((r & 8) ? rt[3] : 0u);
+    const float4 v = *reinterpret_cast<const float4 *>(base + ((tt ^ o) << 4));
+    x[2 * r] = make_float2(v.x, v.y);
+    x[2 * r + 1] = make_float2(v.z, v.w);
+  }
+}
+
+// Generic low tile pass (TileArgs): two waves per 2^12-amplitude tile.
+__global__ __launch_bounds__(128) void k_tile(TileArgs p) {
+  __shared__ float4 lds[2048];
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const const_ptr<TileSeg> segs = as_const(p.segs);
+  const const_ptr<PassOp> ops = as_const(p.ops);
+  const const_ptr<DiagTerm> terms = as_const(p.terms);
+  const uint32_t lane_bytes = lane * 16u;
+  const uint64_t wave_off = uint64_t(wave) << 7;
+  // Arrangement 0 (loads and stores): thread bit j on tile bit j + 1,
+  // register bit k on tile bit 8 + k.
+  uint32_t t0 = 0;
+#pragma unroll
+  for (int j = 0; j < 6; j++)
+    if ((lane >> j) & 1) t0 ^= tile_swizzle(1u << j);
+  if (wave) t0 ^= tile_swizzle(1u << 6);
+  uint32_t r0[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) r0[k] = tile_swizzle(1u << (7 + k));
+  for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
+    const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
+    float2 x[32];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const float4 v = ld4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << 8), lane_bytes));
+      x[2 * r] = make_float2(v.x, v.y);
+      x[2 * r + 1] = make_float2(v.z, v.w);
+    }
+    uint32_t tf = t0, rf[4] = {r0[0], r0[1], r0[2], r0[3]};
+    for (uint32_t s = 0; s < p.nsegs; s++) {
+      const TileSeg __attribute__((address_space(4))) &sg = segs[s];
+      uint32_t tt = 0;
+      uint64_t base = ub;
+#pragma unroll
+      for (int j = 0; j < 7; j++) {
+        const bool on = j < 6 ? ((lane >> j) & 1) != 0 : wave != 0;
+        if (on) {
+          tt ^= sg.tslot[j];
+          base |= 1ull << sg.tpos[j];
+        }
+      }
+      uint32_t rt[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) rt[k] = sg.rslot[k];
+      tile_permute(lds, x, tf, rf, tt, rt);
+      pass_ops<5>(x, ops, terms, sg.op0, sg.op1, ub, base, lane);
+      tf = tt;
+#pragma unroll
+      for (int k = 0; k < 4; k++) rf[k] = rt[k];
+    }
+    tile_permute(lds, x, tf, rf, t0, r0);
 #pragma unroll
     for (int r = 0; r < 16; r++)
       st4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << 8), lane_bytes),
@@ -1589,6 +1681,11 @@ void launch_low(hipStream_t s, int steps, bool zero, int mode, const LowArgs &l)
     case 11: launch_low_s<11>(s, zero, mode, l); break;
     default: break;
   }
+}
+
+void launch_tile(hipStream_t s, const TileArgs &t) {
+  const unsigned grid = unsigned(std::min<uint64_t>(t.lay.nthreads, kLowGrid));
+  k_tile<<<grid, 128, 0, s>>>(t);
 }
 
 void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d) {

@@ -161,18 +161,13 @@ uint32_t low_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) {
   return used;
 }
 
-std::vector<Pass> schedule_passes(const Plan &p) {
-  std::vector<Pass> out;
+namespace {
+
+// Register-tile grouping of plan ops [k0, k1), greedy in plan order: an op
+// joins the open pass when its wires still fit and the merged pass moves no
+// more bytes than the open pass and the op would separately.
+void register_passes(const Plan &p, uint32_t k0, uint32_t k1, std::vector<Pass> &out) {
   const uint32_t n = p.n;
-  if (n < kPassMinQubits) {
-    for (uint32_t k = 0; k < p.ops.size(); k++) {
-      if (p.ops[k].kind == OpKind::Diagonal && p.ops[k].masks.empty()) continue;
-      Pass s;
-      s.ops.push_back(k);
-      out.push_back(std::move(s));
-    }
-    return out;
-  }
   Pass cur;
   auto flush = [&]() {
     if (cur.ops.empty()) return;
@@ -186,25 +181,9 @@ std::vector<Pass> schedule_passes(const Plan &p) {
     out.push_back(std::move(cur));
     cur = Pass{};
   };
-  std::vector<uint32_t> steps;
-  for (uint32_t k = 0; k < p.ops.size(); k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     const PlanOp &op = p.ops[k];
     if (op.kind == OpKind::Diagonal && op.masks.empty()) continue;  // no dispatch
-    uint32_t used = wide_chain_at(p, k, steps);
-    const bool is_wide = used != 0;
-    if (!used) used = low_chain_at(p, k, steps);
-    if (used) {
-      flush();
-      Pass s;
-      s.merged = true;
-      s.wide = is_wide;
-      s.low = !is_wide;
-      for (uint32_t j = k; j < k + used; j++) s.ops.push_back(j);
-      s.reg = steps;
-      out.push_back(std::move(s));
-      k += used - 1;
-      continue;
-    }
     if (!pass_mergeable(op)) {
       flush();
       Pass s;
@@ -245,6 +224,115 @@ std::vector<Pass> schedule_passes(const Plan &p) {
     cur.fixed = f;
   }
   flush();
+}
+
+double passes_bytes(const Plan &p, const std::vector<Pass> &ps) {
+  double b = 0.0;
+  for (const Pass &s : ps)
+    b += s.merged ? pass_bytes(p.n, s.fixed) : op_alone_bytes(p.n, p.ops[s.ops[0]]);
+  return b;
+}
+
+}  // namespace
+
+uint32_t tile_pass_at(const Plan &p, uint32_t k, Pass &pass) {
+  pass = Pass{};
+  if (p.n < kTileMinQubits) return 0;
+  std::vector<uint32_t> cur, chain;
+  uint32_t j = k, len = 0;
+  while (j < p.ops.size() && j - k < kTileMaxOps) {
+    const PlanOp &op = p.ops[j];
+    if (op.kind == OpKind::Diagonal && op.masks.empty()) break;
+    if (!pass_mergeable(op)) break;
+    // A chain pass starting here runs on its own, faster kernel.
+    if (j > k && (wide_chain_at(p, j, chain) || low_chain_at(p, j, chain))) break;
+    const std::vector<uint32_t> w = pass_wires(op);
+    bool inside = true;
+    for (uint32_t q : w)
+      if (q > kTileMaxWire) inside = false;
+    if (!inside) break;
+    std::vector<uint32_t> reg = cur;
+    for (uint32_t q : w)
+      if (std::find(reg.begin(), reg.end(), q) == reg.end()) reg.push_back(q);
+    if (pass.seg_len.empty() || reg.size() > kPassMaxReg) {
+      if (pass.seg_len.size() == kTileMaxSegs) break;
+      if (!pass.seg_len.empty()) {
+        pass.seg_len.back() = len;
+        pass.seg_reg.back() = cur;
+      }
+      pass.seg_len.push_back(0);
+      pass.seg_reg.emplace_back();
+      cur = w;
+      len = 0;
+    } else {
+      cur = reg;
+    }
+    pass.ops.push_back(j);
+    len++;
+    j++;
+  }
+  if (!pass.seg_len.empty()) {
+    pass.seg_len.back() = len;
+    pass.seg_reg.back() = cur;
+  }
+  if (pass.seg_len.size() < 2) {
+    pass = Pass{};
+    return 0;
+  }
+  std::vector<Pass> regular;
+  register_passes(p, k, j, regular);
+  if (pass_bytes(p.n, 0) >= passes_bytes(p, regular)) {
+    pass = Pass{};
+    return 0;
+  }
+  pass.merged = pass.tile = true;
+  return j - k;
+}
+
+std::vector<Pass> schedule_passes(const Plan &p) {
+  std::vector<Pass> out;
+  const uint32_t n = p.n;
+  if (n < kPassMinQubits) {
+    for (uint32_t k = 0; k < p.ops.size(); k++) {
+      if (p.ops[k].kind == OpKind::Diagonal && p.ops[k].masks.empty()) continue;
+      Pass s;
+      s.ops.push_back(k);
+      out.push_back(std::move(s));
+    }
+    return out;
+  }
+  // Runs of ops that no chain or tile pass starts in go to the register-tile
+  // grouping as a whole.
+  uint32_t run0 = 0;
+  std::vector<uint32_t> steps;
+  for (uint32_t k = 0; k < p.ops.size(); k++) {
+    const PlanOp &op = p.ops[k];
+    if (op.kind == OpKind::Diagonal && op.masks.empty()) continue;  // no dispatch
+    uint32_t used = wide_chain_at(p, k, steps);
+    const bool is_wide = used != 0;
+    if (!used) used = low_chain_at(p, k, steps);
+    if (used) {
+      register_passes(p, run0, k, out);
+      Pass s;
+      s.merged = true;
+      s.wide = is_wide;
+      s.low = !is_wide;
+      for (uint32_t j = k; j < k + used; j++) s.ops.push_back(j);
+      s.reg = steps;
+      out.push_back(std::move(s));
+      k += used - 1;
+      run0 = k + 1;
+      continue;
+    }
+    Pass t;
+    if (const uint32_t tu = tile_pass_at(p, k, t)) {
+      register_passes(p, run0, k, out);
+      out.push_back(std::move(t));
+      k += tu - 1;
+      run0 = k + 1;
+    }
+  }
+  register_passes(p, run0, uint32_t(p.ops.size()), out);
   return out;
 }
 

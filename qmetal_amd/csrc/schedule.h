@@ -31,7 +31,11 @@ constexpr uint32_t kPassMaxOps = 16;    // plan ops per merged pass
 struct Pass {
   bool merged = false;         // false: ops[0] alone through its own kernel
   bool wide = false;           // merged: six chain steps over a workgroup tile
-  bool low = false;            // merged: chain steps over a low 2^11 tile
+  bool low = false;            // merged: chain steps over a low 2^12 tile
+  bool tile = false;           // merged: any ops on wires 0..11 over a low
+                               // 2^12 tile, in segments
+  std::vector<uint32_t> seg_len;               // tile: plan ops per segment
+  std::vector<std::vector<uint32_t>> seg_reg;  // tile: register wires per segment
   std::vector<uint32_t> ops;   // plan-op indices, in plan order
   std::vector<uint32_t> reg;   // register wires (merged), first use order;
                                // wide and low: the step wires in order
@@ -64,6 +68,21 @@ constexpr uint32_t kLowMinQubits = 12;
 // Number of plan ops from `k` that form a low pass (0 if none); `wires` gets
 // the step wires (a final 0 for a step on wire 0).
 uint32_t low_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires);
+
+// Generic low tile passes: consecutive mergeable plan ops whose moving wires
+// all lie in 1..kTileMaxWire (controls and phase masks anywhere), in
+// segments of at most kPassMaxReg register wires (wire 0 is always local),
+// over tiles of 2^(kTileMaxWire+1) contiguous amplitudes permuted through LDS
+// between segments. Taken when it needs at least two segments and moves
+// fewer bytes than the register-tile passes over the same ops would.
+constexpr uint32_t kTileMaxWire = 11;
+constexpr uint32_t kTileMinQubits = 12;
+constexpr uint32_t kTileMaxSegs = 8;
+constexpr uint32_t kTileMaxOps = 64;
+
+// Number of plan ops from `k` that form a generic low tile pass (0 if
+// none); `pass` gets its ops and segments.
+uint32_t tile_pass_at(const Plan &p, uint32_t k, Pass &pass);
 
 // Whether plan op `op` can join a merged pass.
 bool pass_mergeable(const PlanOp &op);

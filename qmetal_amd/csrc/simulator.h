@@ -24,8 +24,9 @@ namespace qmamd {
 constexpr uint32_t kMaxQubits = 34;
 
 // Device time of one launch and its algorithmic bytes: a plan op (kind
-// 0..6) or a merged pass (kinds 7..10: the generic, chain, wide chain and
-// low chain kernels; bytes of one pass over the amplitudes it reads).
+// 0..6) or a merged pass (kinds 7..11: the generic, chain, wide chain, low
+// chain and generic low tile kernels; bytes of one pass over the amplitudes
+// it reads).
 struct OpTime {
   uint32_t kind;
   double ms;

@@ -46,11 +46,13 @@ constexpr int kLocalBits = 4;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 // Timing record kinds of merged passes (plan-op kinds are 0..6): the
-// generic pass kernel, the chain kernel, the wide and low chain kernels.
+// generic pass kernel, the chain kernel, the wide and low chain kernels, the
+// generic low tile kernel.
 constexpr uint32_t kTimingPass = 7;
 constexpr uint32_t kTimingChain = 8;
 constexpr uint32_t kTimingWide = 9;
 constexpr uint32_t kTimingLow = 10;
+constexpr uint32_t kTimingTile = 11;
 
 int current_device() {
   int d = 0;
@@ -382,6 +384,9 @@ struct Simulator::Impl {
     bool low = false;
     LowArgs l{};
     int mode = 0;  // wide / low step mode (launch_wide)
+    // Generic low tile pass (segments of merged-pass ops).
+    bool tile = false;
+    TileArgs t{};
   };
 
   // Local coefficients of a phase op of a merged pass (turns).
@@ -404,35 +409,34 @@ struct Simulator::Impl {
       b += pass_slots(p.ops[k]) * sizeof(PassOp);
       if (p.ops[k].kind == OpKind::Diagonal) b += p.ops[k].masks.size() * sizeof(DiagTerm);
     }
+    b += ps.seg_len.size() * sizeof(TileSeg);
     return (b + 255) & ~size_t(255);
   }
 
   // Lays out a merged pass: register wires first, fixed bits, fill bits at
   // the top; then one PassOp per 2x2 slot or phase layer, in plan order.
-  void prep_pass(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
-    LayoutBuilder lb{n};
-    lb.fixed = ps.fixed;
-    for (uint32_t q : ps.reg) lb.add(q);
-    const int want = lb.reg.size() + 1 <= 4 ? 4 : 5;
-    pl.a.lay = lb.finish(want, &pl.nl);
-    if (pl.nl != want) fail("internal: pass width");
-    const uint64_t lane = lane_bits(pl.a.lay);
-    size_t nops = 0;
-    for (uint32_t k : ps.ops) nops += pass_slots(p.ops[k]);
-    PassOp *po = reinterpret_cast<PassOp *>(staging + off);
-    DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off + nops * sizeof(PassOp));
-    size_t i = 0, nt = 0;
-    std::vector<PhaseCoef> coef(nops);
+  // PassOps of plan ops `ks` (in order) against a tile frame: `local_of`
+  // maps a wire to its tile-local bit (0..nl-1) or -1, `fixed` bits are 1 for
+  // every amplitude the pass touches, `var` are the base-index bits that
+  // differ between the threads of a workgroup (their terms are per thread,
+  // the rest workgroup-uniform). Writes ops to `po` and phase terms to
+  // dt[nt..], advancing nt; `coef` gets each phase op's local coefficients.
+  // Returns the number of ops written.
+  template <typename LocalOf>
+  size_t emit_ops(const Plan &p, const uint32_t *ks, size_t count, LocalOf local_of,
+                  uint64_t fixed, uint64_t var, int nl, PassOp *po, DiagTerm *dt, size_t &nt,
+                  PhaseCoef *coef) const {
+    size_t i = 0;
     auto local = [&](uint32_t q) {
-      const int l = lb.local_of(q);
+      const int l = local_of(q);
       if (l < 0) fail("internal: pass wire not local");
       return uint32_t(l);
     };
     // A control: nothing if fixed (always 1 here), a tile-local bit, or a
     // bit of the thread's base index.
     auto control = [&](uint32_t c, PassOp &o) {
-      if ((lb.fixed >> c) & 1) return;
-      const int l = lb.local_of(c);
+      if ((fixed >> c) & 1) return;
+      const int l = local_of(c);
       if (l >= 0) o.cmask |= 1u << l;
       else o.gmask |= 1ull << c;
     };
@@ -445,8 +449,8 @@ struct Simulator::Impl {
       std::memcpy(o.m, m8, sizeof(o.m));
       return o;
     };
-    for (uint32_t k : ps.ops) {
-      const PlanOp &op = p.ops[k];
+    for (size_t kk = 0; kk < count; kk++) {
+      const PlanOp &op = p.ops[ks[kk]];
       float m8[8];
       switch (op.kind) {
         case OpKind::Dense1q:
@@ -480,6 +484,7 @@ struct Simulator::Impl {
         }
         case OpKind::Diagonal: {
           PhaseCoef &pc = coef[i];
+          pc = PhaseCoef{};
           PassOp &o = po[i++];
           o = PassOp{};
           o.type = kPassPhase;
@@ -489,15 +494,14 @@ struct Simulator::Impl {
           for (uint64_t m : op.masks) common &= m;
           if (turns(op.global) == 0.0) {
             for (uint32_t q = 0; q < 64; q++)
-              if (((common & ~lb.fixed) >> q) & 1) control(q, o);
+              if (((common & ~fixed) >> q) & 1) control(q, o);
           }
           o.term0 = uint32_t(nt);
           double constant = 0.0;
           double(&beta0)[kPassMaxNL] = pc.beta0;
           double(&gamma)[kPassMaxNL][kPassMaxNL] = pc.gamma;
-          o.nterms = uint32_t(fold_terms(
-              op, lb.fixed, lane, [&](uint32_t q) { return lb.local_of(q); }, constant, beta0,
-              gamma, dt + nt, o.nuniform));
+          o.nterms = uint32_t(fold_terms(op, fixed, var, local_of, constant, beta0, gamma,
+                                         dt + nt, o.nuniform));
           // Which sums the base-index terms feed; the workgroup-uniform ones
           // are reduced over the wave only where they feed something.
           for (uint32_t u = 0; u < o.nterms; u++) {
@@ -511,9 +515,9 @@ struct Simulator::Impl {
           // combination: exp(2 pi i (constant + sum beta0 + sum gamma)).
           for (uint32_t h = 0; h < uint32_t(kPassMaxNL); h++)
             if ((o.bmask >> h) & 1) o.beta0[h] = beta0[h];
-          for (uint32_t l = 0; l < (1u << pl.nl); l++) {
+          for (uint32_t l = 0; l < (1u << nl); l++) {
             double th = constant;
-            for (int h = 0; h < pl.nl; h++) {
+            for (int h = 0; h < nl; h++) {
               if (!((l >> h) & 1)) continue;
               if (!((o.bmask >> h) & 1)) th += beta0[h];
               for (int k = 0; k < h; k++)
@@ -533,22 +537,44 @@ struct Simulator::Impl {
     // Code section of each slot: its local bit and kind (real over all
     // pairs, complex over all pairs, complex over the pairs whose local
     // controls are set); a phase op selects no section.
-    for (size_t j = 0; j < nops; j++) {
+    for (size_t j = 0; j < i; j++) {
       PassOp &o = po[j];
       if (o.type == kPassPhase) {
         o.sel = ~0u;
         continue;
       }
       const uint32_t kind = o.cmask ? 2u : (o.type == kPassRealSlot ? 0u : 1u);
-      o.sel = o.lbit + uint32_t(pl.nl) * kind;
+      o.sel = o.lbit + uint32_t(nl) * kind;
       o.pm = 0;
       uint32_t pidx = 0;
-      for (uint32_t l = 0; l < (1u << pl.nl); l++) {
+      for (uint32_t l = 0; l < (1u << nl); l++) {
         if ((l >> o.lbit) & 1) continue;
         if ((l & o.cmask) == o.cmask) o.pm |= 1u << pidx;
         pidx++;
       }
     }
+    return i;
+  }
+
+  // Lays out a merged pass: register wires first, fixed bits, fill bits at
+  // the top; then one PassOp per 2x2 slot or phase layer, in plan order.
+  void prep_pass(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
+    LayoutBuilder lb{n};
+    lb.fixed = ps.fixed;
+    for (uint32_t q : ps.reg) lb.add(q);
+    const int want = lb.reg.size() + 1 <= 4 ? 4 : 5;
+    pl.a.lay = lb.finish(want, &pl.nl);
+    if (pl.nl != want) fail("internal: pass width");
+    const uint64_t lane = lane_bits(pl.a.lay);
+    size_t nops = 0;
+    for (uint32_t k : ps.ops) nops += pass_slots(p.ops[k]);
+    PassOp *po = reinterpret_cast<PassOp *>(staging + off);
+    DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off + nops * sizeof(PassOp));
+    size_t nt = 0;
+    std::vector<PhaseCoef> coef(nops);
+    if (emit_ops(p, ps.ops.data(), ps.ops.size(), [&](uint32_t q) { return lb.local_of(q); },
+                 lb.fixed, lane, pl.nl, po, dt, nt, coef.data()) != nops)
+      fail("internal: pass op count");
     pl.a.a = state;
     pl.a.ops = reinterpret_cast<const PassOp *>(arena + off);
     pl.a.terms = reinterpret_cast<const DiagTerm *>(arena + off + nops * sizeof(PassOp));
@@ -904,6 +930,81 @@ struct Simulator::Impl {
     pl.bytes = pass_bytes(n, 0);
   }
 
+  // Generic low tile pass (TileArgs): per segment, its register wires
+  // (filled up to four with the lowest unused wires of 1..11) and the thread
+  // bits on the remaining seven, ascending; the ops emitted against that
+  // arrangement. Region layout: segments, then ops, then phase terms.
+  void prep_tile(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
+    const size_t nseg = ps.seg_len.size();
+    if (nseg < 2 || nseg != ps.seg_reg.size() || ps.fixed || n < kTileMinQubits)
+      fail("internal: tile pass shape");
+    size_t nops = 0;
+    for (uint32_t k : ps.ops) nops += pass_slots(p.ops[k]);
+    TileSeg *sg = reinterpret_cast<TileSeg *>(staging + off);
+    const size_t ops_off = off + nseg * sizeof(TileSeg);
+    PassOp *po = reinterpret_cast<PassOp *>(staging + ops_off);
+    DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + ops_off + nops * sizeof(PassOp));
+    size_t nt = 0, o = 0, k0 = 0;
+    std::vector<PhaseCoef> coef(nops);
+    for (size_t s = 0; s < nseg; s++) {
+      TileSeg &g = sg[s];
+      g = TileSeg{};
+      uint32_t used = 1u, rpos[4];
+      int nr = 0, nth = 0;
+      for (uint32_t q : ps.seg_reg[s]) {
+        if (q < 1 || q > kTileMaxWire || ((used >> q) & 1) || nr == 4)
+          fail("internal: tile pass wires");
+        rpos[nr++] = q;
+        used |= 1u << q;
+      }
+      for (uint32_t q = 1; nr < 4; q++)
+        if (!((used >> q) & 1)) {
+          rpos[nr++] = q;
+          used |= 1u << q;
+        }
+      uint64_t var = 0;
+      for (uint32_t q = 1; q <= kTileMaxWire; q++)
+        if (!((used >> q) & 1)) {
+          g.tpos[nth] = uint8_t(q);
+          g.tslot[nth] = tile_swizzle(1u << (q - 1));
+          var |= 1ull << q;
+          nth++;
+        }
+      if (nth != 7) fail("internal: tile pass thread bits");
+      for (int k = 0; k < 4; k++) g.rslot[k] = tile_swizzle(1u << (rpos[k] - 1));
+      auto local_of = [&](uint32_t q) {
+        if (q == 0) return 0;
+        for (int k = 0; k < 4; k++)
+          if (rpos[k] == q) return k + 1;
+        return -1;
+      };
+      g.op0 = uint32_t(o);
+      o += emit_ops(p, ps.ops.data() + k0, ps.seg_len[s], local_of, 0, var, 5, po + o, dt, nt,
+                    coef.data() + o);
+      g.op1 = uint32_t(o);
+      k0 += ps.seg_len[s];
+    }
+    if (o != nops || k0 != ps.ops.size()) fail("internal: tile pass op count");
+    TileArgs &ta = pl.t;
+    ta = TileArgs{};
+    Layout &L = ta.lay;
+    for (uint32_t q = 0; q <= kTileMaxWire; q++) {
+      L.ins[L.nins] = uint8_t(q);
+      L.hm[L.nins] = ~((uint64_t(1) << q) - 1);
+      L.nins++;
+    }
+    L.nthreads = dim >> L.nins;
+    L.streaming = 1;
+    ta.a = state;
+    ta.segs = reinterpret_cast<const TileSeg *>(arena + off);
+    ta.nsegs = uint32_t(nseg);
+    ta.ops = reinterpret_cast<const PassOp *>(arena + ops_off);
+    ta.terms = reinterpret_cast<const DiagTerm *>(arena + ops_off + nops * sizeof(PassOp));
+    pl.tile = true;
+    pl.nl = 5;
+    pl.bytes = pass_bytes(n, 0);
+  }
+
   void pass_launch(const PassLaunch &pl) {
     hipEvent_t a = nullptr;
     if (timing) {
@@ -912,6 +1013,7 @@ struct Simulator::Impl {
     }
     if (pl.wide) launch_wide(stream, pl.chain_k, pl.mode, pl.w);
     else if (pl.low) launch_low(stream, pl.chain_k, pl.chain_zero, pl.mode, pl.l);
+    else if (pl.tile) launch_tile(stream, pl.t);
     else if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);
     else launch_pass(stream, pl.nl, pl.a);
     launched();
@@ -919,8 +1021,9 @@ struct Simulator::Impl {
     if (timing) {
       hipEvent_t b = take_event();
       check(hipEventRecord(b, stream), "hipEventRecord");
-      const uint32_t kind = pl.wide  ? kTimingWide
-                            : pl.low ? kTimingLow
+      const uint32_t kind = pl.wide    ? kTimingWide
+                            : pl.low   ? kTimingLow
+                            : pl.tile  ? kTimingTile
                             : pl.chain ? kTimingChain
                                        : kTimingPass;
       timings.push_back(OpTiming{kind, pl.bytes, a, b});
@@ -1316,6 +1419,10 @@ void Simulator::run_plan(const Plan &p) {
       }
       if (passes[k].low) {
         s->prep_low(p, passes[k], offs[k], merged[k]);
+        continue;
+      }
+      if (passes[k].tile) {
+        s->prep_tile(p, passes[k], offs[k], merged[k]);
         continue;
       }
       if (passes[k].merged) {

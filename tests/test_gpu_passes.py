@@ -328,3 +328,72 @@ def test_low_passes_against_oracle_and_op_by_op():
                   f"diff {d:.2e}, vs op-by-op {dm:.2e}")
             assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
             assert dm <= TOL_SAME
+
+
+def tile_stream(n, count, seed):
+    """Every mergeable gate kind with targets on wires 0..11 and controls,
+    swap partners and phase wires anywhere, in runs long enough to need
+    several register arrangements (generic low tile passes)."""
+    r = C.SplitMix64(seed)
+    gates = [(C.H, (q,), ()) for q in range(n)]
+
+    def low():
+        return r.next_u64() % 12
+
+    def other(avoid):
+        while True:
+            q = r.next_u64() % n
+            if q not in avoid:
+                return q
+
+    for _ in range(count):
+        u = r.next_u64() % 12
+        t = low()
+        if u < 3:
+            gates.append((C.U3, (t,), tuple(r.uniform(-3.2, 3.2) for _ in range(3))))
+        elif u < 4:
+            gates.append((ONE[r.next_u64() % len(ONE)], (t,), ()))
+        elif u < 6:
+            gates.append(((C.CX, C.CY)[r.next_u64() % 2], (other((t,)), t), ()))
+        elif u < 8:
+            gates.append((C.CP, (other((t,)), t), (r.uniform(-3.2, 3.2),)))
+        elif u < 9:
+            a = low()
+            b = low() if a else 1 + low() % 11
+            if a != b:
+                gates.append((C.SWAP, (a, b), ()))
+        elif u < 10:
+            c0 = other((t,))
+            gates.append((C.CCX, (c0, other((t, c0)), t), ()))
+        else:
+            a, b = t, other((t,))
+            gates += C.zz(a, b, r.uniform(-3.2, 3.2))
+    return gates
+
+
+def test_tile_passes_against_oracle_and_op_by_op():
+    """Generic low tile passes (merged-pass ops in several register
+    arrangements of a 2^12 tile, permuted through LDS) with controls and
+    phase terms on register, thread and workgroup-uniform bits."""
+    from test_schedule import tile_pass, chain_pass, moving_wires
+    for n in (12, 13, 15, 17):
+        for seed in range(3):
+            gates = tile_stream(n, 60, 500 * n + seed)
+            c = C.to_circuit(n, gates)
+            plan, passes = c.plan(), c.passes()
+            groups = {}
+            for k, q in enumerate(passes):
+                if q >= 0:
+                    groups.setdefault(q, []).append(plan[k])
+            tiles = [g for g in groups.values()
+                     if len(set().union(*(moving_wires(o) for o in g))) > 4
+                     and not chain_pass(g, n)]
+            assert tiles and all(tile_pass(g, n) for g in tiles), "no tile pass formed"
+            want = O.run(n, gates)
+            got, norm, launches = run(n, gates)
+            ref, _, _ = run(n, gates, merge=False)
+            d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
+            print(f"tile n={n} seed={seed}: {len(tiles)} tile passes, {launches} launches "
+                  f"(plan {len(plan)}), diff {d:.2e}, vs op-by-op {dm:.2e}")
+            assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
+            assert dm <= TOL_SAME

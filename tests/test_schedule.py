@@ -28,6 +28,50 @@ def moving_wires(op):
     return w - {0}
 
 
+def chain_pass(ops, n):
+    """A chain pass: Dense1q steps on distinct wires, each followed at most by
+    a phase layer whose masks all hold its wire; wide: six or seven steps on
+    wires >= 7 (n >= 16); low: 5..11 steps on wires 1..11, then at most one
+    on wire 0 (n >= 12)."""
+    steps, last = [], None
+    for o in ops:
+        if o["kind"] == "Dense1q":
+            last = o["qubits"][0]
+            steps.append(last)
+        elif o["kind"] == "DiagonalLayer" and last is not None:
+            if not all((m >> last) & 1 for m in o["masks"]):
+                return False
+            last = None
+        else:
+            return False
+    if len(set(steps)) != len(steps):
+        return False
+    if min(steps) >= 7 and len(steps) in (6, 7):
+        return n >= 16
+    body = steps[:-1] if steps[-1] == 0 else steps
+    return n >= 12 and 0 not in body and 5 <= len(body) <= 11 and max(body) <= 11
+
+
+def tile_pass(ops, n):
+    """A generic low tile pass: mergeable ops whose moving wires lie in
+    0..11 (n >= 12), at most 64 of them, that split in plan order into at
+    most 8 segments of at most 4 register wires."""
+    if n < 12 or len(ops) > 64:
+        return False
+    segs, cur = 0, None
+    for o in ops:
+        if o["kind"] == "Blocked1q":
+            return False
+        w = moving_wires(o)
+        if any(q > 11 for q in w):
+            return False
+        if cur is None or len(cur | w) > MAX_REG:
+            segs, cur = segs + 1, set(w)
+        else:
+            cur |= w
+    return 2 <= segs <= 8
+
+
 def check_grouping(circ, options=None):
     plan = circ.plan(options)
     passes = circ.passes(options)
@@ -51,26 +95,7 @@ def check_grouping(circ, options=None):
             wires = set().union(*(moving_wires(o) for o in ops))
             assert all(o["kind"] != "Blocked1q" for o in ops)
             if len(wires) > MAX_REG:
-                # a chain pass: Dense1q steps on distinct wires, each followed
-                # at most by a phase layer whose masks all hold its wire;
-                # wide: six or seven steps on wires >= 7; low: 5..11 steps on
-                # wires 1..11, then at most one on wire 0
-                steps = [o["qubits"][0] for o in ops if o["kind"] == "Dense1q"]
-                assert len(set(steps)) == len(steps), steps
-                last = None
-                for o in ops:
-                    if o["kind"] == "Dense1q":
-                        last = o["qubits"][0]
-                    else:
-                        assert o["kind"] == "DiagonalLayer" and last is not None
-                        assert all((m >> last) & 1 for m in o["masks"])
-                        last = None
-                if min(steps) >= 7 and len(steps) in (6, 7):
-                    assert circ.num_qubits >= 16
-                else:
-                    assert circ.num_qubits >= 12, steps
-                    assert 0 not in steps[:-1] and 5 <= len(wires) <= 11, steps
-                    assert max(steps) <= 11, steps
+                assert chain_pass(ops, circ.num_qubits) or tile_pass(ops, circ.num_qubits), ops
     return max(seen) + 1 if seen else 0
 
 
@@ -87,8 +112,11 @@ def test_qft_pass_counts():
 
 
 def test_ghz_pass_count():
-    # H(0), CX(0,1), ..., CX(18,19): four new targets per pass
-    assert check_grouping(C.to_circuit(20, C.ghz(20))) == 5
+    # H(0), CX(0,1), ..., CX(18,19): targets 1..11 in one low tile pass (three
+    # segments), then four new targets per register-tile pass
+    assert check_grouping(C.to_circuit(20, C.ghz(20))) == 3
+    # below 12 qubits there are no tile passes
+    assert check_grouping(C.to_circuit(11, C.ghz(11))) == 3
 
 
 def test_narrow_states_run_op_by_op():
