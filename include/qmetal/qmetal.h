@@ -198,9 +198,9 @@ int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
  * one read-modify-write pass over the state; phase layers join any pass.
  * The plan and qm_circuit_plan_size are unchanged; qm_dispatch_count counts
  * the launches actually made. On by default; qm_set_pass_merging(s, 0)
- * restores one launch per plan op. Timing records of kind 7 to 11 are
+ * restores one launch per plan op. Timing records of kind 7 to 12 are
  * merged passes (generic, chain-shaped, wide and low chain-shaped, generic
- * low tile kernels).
+ * low tile and generic wide kernels).
  * qm_plan_passes reports the grouping qm_run would use: pass_of_op
  * (qm_plan_size entries, may be NULL) gets each op's pass index (UINT32_MAX
  * for an op that launches nothing), *num_passes the number of launches. */

@@ -192,8 +192,8 @@ PLAN_KINDS = ("Dense1q", "Dense1qGroup", "Blocked1q", "Controlled1q", "DiagonalL
               "Swap", "CCX")
 # Timing records: plan-op kinds, then merged passes of several plan ops
 # (generic pass kernel, chain kernel, wide and low chain kernels, generic
-# low tile kernel).
-TIMING_KINDS = PLAN_KINDS + ("Pass", "ChainPass", "WidePass", "LowPass", "TilePass")
+# low tile and generic wide kernels).
+TIMING_KINDS = PLAN_KINDS + ("Pass", "ChainPass", "WidePass", "LowPass", "TilePass", "GWidePass")
 
 
 def version() -> tuple[int, int, int]:

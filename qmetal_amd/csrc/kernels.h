@@ -235,6 +235,22 @@ struct TileArgs {
   const DiagTerm *terms;
 };
 
+// Generic wide pass: merged-pass ops (NL = 5) over the workgroup tile of a
+// wide chain pass (WideArgs: wires pos[0..3+WB], lanes on lanepos): ops
+// [0, na) run in the first arrangement (local bits 1..4 = pos[0..3]), ops
+// [na, nops) after the LDS exchange (local bits 1..WB = pos[4..3+WB], the
+// rest still pos[WB..3]).
+struct GWideArgs {
+  float2 *a;
+  Layout lay;
+  uint8_t lanepos[6];
+  uint8_t pos[kWideArgSteps];
+  const PassOp *ops;
+  const DiagTerm *terms;
+  uint32_t na;
+  uint32_t nops;
+};
+
 // LDS slot (float4 index) of tile index i, i = (tile bits 1..11) >> 1: the
 // low four bits are xored with the higher ones (linear over GF(2)).
 constexpr uint32_t tile_swizzle(uint32_t i) { return i ^ (((i >> 4) ^ (i >> 8)) & 0xFu); }
@@ -257,6 +273,8 @@ void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w);
 // s in 5..11 steps on wires 1..11, `zero`: a last step on wire 0.
 void launch_low(hipStream_t s, int steps, bool zero, int mode, const LowArgs &l);
 void launch_tile(hipStream_t s, const TileArgs &t);
+// wb = 2 or 3 wave bits.
+void launch_gwide(hipStream_t s, int wb, const GWideArgs &g);
 
 // FP64 reductions. Each writes per-block partials into `partials` (capacity
 // kRedBlocks doubles or double2) and then reduces them in a fixed order into

@@ -971,6 +971,72 @@ __global__ __launch_bounds__(128) void k_low(LowArgs p) {
   }
 }
 
+// Generic wide pass (GWideArgs): the wide chain kernel's tile, load, LDS
+// exchange and store around two runs of merged-pass ops.
+template <int WB>
+__global__ __launch_bounds__(64 << WB) void k_gwide(GWideArgs p) {
+  constexpr int W = 1 << WB;
+  constexpr int ROUNDS = 16 / W;
+  __shared__ float4 xch[W][W][64];  // [writer's wave][slot][lane]
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const const_ptr<PassOp> ops = as_const(p.ops);
+  const const_ptr<DiagTerm> terms = as_const(p.terms);
+  uint64_t lanebits = 0;
+#pragma unroll
+  for (int j = 0; j < 6; j++)
+    if ((lane >> j) & 1) lanebits |= 1ull << p.lanepos[j];
+  uint64_t wave_a = 0, wave_b = 0;
+#pragma unroll
+  for (int j = 0; j < WB; j++)
+    if ((wave >> j) & 1) {
+      wave_a |= 1ull << p.pos[4 + j];
+      wave_b |= 1ull << p.pos[j];
+    }
+  const uint32_t lane_bytes = uint32_t(lanebits) * 8u;
+  for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
+    const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
+    float2 x[32];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      uint64_t o = ub | wave_a;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if ((r >> k) & 1) o |= 1ull << p.pos[k];
+      const float4 v = ld4<true>(pair_ptr(p.a, o, lane_bytes));
+      x[2 * r] = make_float2(v.x, v.y);
+      x[2 * r + 1] = make_float2(v.z, v.w);
+    }
+    pass_ops<5>(x, ops, terms, 0, p.na, ub, ub | lanebits | wave_a, lane);
+#pragma unroll
+    for (int c = 0; c < ROUNDS; c++) {
+#pragma unroll
+      for (int a = 0; a < W; a++) {
+        const int l = 2 * ((c << WB) | a);
+        xch[wave][a][lane] = make_float4(x[l].x, x[l].y, x[l + 1].x, x[l + 1].y);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int d = 0; d < W; d++) {
+        const int l = 2 * ((c << WB) | d);
+        const float4 v = xch[d][wave][lane];
+        x[l] = make_float2(v.x, v.y);
+        x[l + 1] = make_float2(v.z, v.w);
+      }
+      __syncthreads();
+    }
+    pass_ops<5>(x, ops, terms, p.na, p.nops, ub, ub | lanebits | wave_b, lane);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      uint64_t o = ub | wave_b;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if ((r >> k) & 1) o |= 1ull << (k < WB ? p.pos[4 + k] : p.pos[k]);
+      st4<true>(pair_ptr(p.a, o, lane_bytes),
+                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
+    }
+  }
+}
+
 // Moves the tile between two arrangements given by their LDS slot bases:
 // `tf` / `tt` the thread's own slot part (from / to), `rf` / `rt` the
 // register bits' bases (workgroup-uniform).
@@ -1681,6 +1747,12 @@ void launch_low(hipStream_t s, int steps, bool zero, int mode, const LowArgs &l)
     case 11: launch_low_s<11>(s, zero, mode, l); break;
     default: break;
   }
+}
+
+void launch_gwide(hipStream_t s, int wb, const GWideArgs &g) {
+  const unsigned grid = unsigned(std::min<uint64_t>(g.lay.nthreads, kPassGrid));
+  if (wb == 3) k_gwide<3><<<grid, 512, 0, s>>>(g);
+  else k_gwide<2><<<grid, 256, 0, s>>>(g);
 }
 
 void launch_tile(hipStream_t s, const TileArgs &t) {

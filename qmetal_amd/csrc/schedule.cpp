@@ -289,6 +289,84 @@ uint32_t tile_pass_at(const Plan &p, uint32_t k, Pass &pass) {
   return j - k;
 }
 
+uint32_t gwide_pass_at(const Plan &p, uint32_t k, Pass &pass) {
+  pass = Pass{};
+  if (p.n < kWideMinQubits) return 0;
+  std::vector<uint32_t> A, fresh, keep, chain;
+  auto has = [](const std::vector<uint32_t> &v, uint32_t q) {
+    return std::find(v.begin(), v.end(), q) != v.end();
+  };
+  uint32_t j = k, na = 0, nb = 0;
+  bool second = false;
+  while (j < p.ops.size() && j - k < kTileMaxOps) {
+    const PlanOp &op = p.ops[j];
+    if (op.kind == OpKind::Diagonal && op.masks.empty()) break;
+    if (!pass_mergeable(op)) break;
+    if (j > k && (wide_chain_at(p, j, chain) || low_chain_at(p, j, chain))) break;
+    const std::vector<uint32_t> w = pass_wires(op);
+    bool high = true;
+    for (uint32_t q : w)
+      if (q < kWideMinWire) high = false;
+    if (!high) break;
+    if (!second) {
+      std::vector<uint32_t> a = A;
+      for (uint32_t q : w)
+        if (!has(a, q)) a.push_back(q);
+      if (a.size() <= 4) {
+        A = a;
+        na++;
+        j++;
+        continue;
+      }
+      second = true;
+    }
+    std::vector<uint32_t> f = fresh, kp = keep;
+    for (uint32_t q : w) {
+      if (has(A, q)) {
+        if (!has(kp, q)) kp.push_back(q);
+      } else if (!has(f, q)) {
+        f.push_back(q);
+      }
+    }
+    const size_t wb = std::max<size_t>(2, f.size());
+    if (f.size() > 3 || kp.size() > 4 - wb) break;
+    fresh = f;
+    keep = kp;
+    nb++;
+    j++;
+  }
+  if (na == 0 || nb == 0) return 0;
+  const size_t wb = std::max<size_t>(2, fresh.size());
+  // Register positions: the wires that swap out first (local bits 1..wb),
+  // then the ones that stay; every other wire of the tile is a spare at or
+  // above kWideMinWire.
+  std::vector<uint32_t> stay = keep, out;
+  for (uint32_t q : A)
+    if (!has(stay, q)) (stay.size() < 4 - wb ? stay : out).push_back(q);
+  std::vector<uint32_t> wave = fresh;
+  uint32_t spare = kWideMinWire;
+  auto next_spare = [&]() {
+    while (has(A, spare) || has(fresh, spare) || has(out, spare) || has(stay, spare) ||
+           has(wave, spare))
+      spare++;
+    return spare;
+  };
+  while (out.size() < wb) out.push_back(next_spare());
+  while (stay.size() < 4 - wb) stay.push_back(next_spare());
+  while (wave.size() < wb) wave.push_back(next_spare());
+  if (spare >= p.n) return 0;
+  std::vector<Pass> regular;
+  register_passes(p, k, j, regular);
+  if (pass_bytes(p.n, 0) >= passes_bytes(p, regular)) return 0;
+  pass.merged = pass.gwide = true;
+  pass.reg = out;
+  pass.reg.insert(pass.reg.end(), stay.begin(), stay.end());
+  pass.reg.insert(pass.reg.end(), wave.begin(), wave.end());
+  for (uint32_t i = k; i < j; i++) pass.ops.push_back(i);
+  pass.seg_len = {na, nb};
+  return j - k;
+}
+
 std::vector<Pass> schedule_passes(const Plan &p) {
   std::vector<Pass> out;
   const uint32_t n = p.n;
@@ -324,11 +402,14 @@ std::vector<Pass> schedule_passes(const Plan &p) {
       run0 = k + 1;
       continue;
     }
-    Pass t;
-    if (const uint32_t tu = tile_pass_at(p, k, t)) {
+    // A generic low tile or wide pass from here: the one taking more ops.
+    Pass t, g;
+    const uint32_t tu = tile_pass_at(p, k, t);
+    const uint32_t gu = gwide_pass_at(p, k, g);
+    if (tu || gu) {
       register_passes(p, run0, k, out);
-      out.push_back(std::move(t));
-      k += tu - 1;
+      out.push_back(std::move(gu > tu ? g : t));
+      k += std::max(tu, gu) - 1;
       run0 = k + 1;
     }
   }

@@ -34,6 +34,9 @@ struct Pass {
   bool low = false;            // merged: chain steps over a low 2^12 tile
   bool tile = false;           // merged: any ops on wires 0..11 over a low
                                // 2^12 tile, in segments
+  bool gwide = false;          // merged: any ops on wires >= 7 over a wide
+                               // chain pass's tile (reg: its wires), in two
+                               // segments around the exchange
   std::vector<uint32_t> seg_len;               // tile: plan ops per segment
   std::vector<std::vector<uint32_t>> seg_reg;  // tile: register wires per segment
   std::vector<uint32_t> ops;   // plan-op indices, in plan order
@@ -83,6 +86,13 @@ constexpr uint32_t kTileMaxOps = 64;
 // Number of plan ops from `k` that form a generic low tile pass (0 if
 // none); `pass` gets its ops and segments.
 uint32_t tile_pass_at(const Plan &p, uint32_t k, Pass &pass);
+
+// Generic wide passes: consecutive mergeable plan ops whose moving wires are
+// all at or above kWideMinWire, over the tile of a wide chain pass: ops on at
+// most four register wires, then, after the exchange, ops on at most three
+// wires that were wave bits plus the register wires that stay. Taken under
+// the same conditions as tile passes.
+uint32_t gwide_pass_at(const Plan &p, uint32_t k, Pass &pass);
 
 // Whether plan op `op` can join a merged pass.
 bool pass_mergeable(const PlanOp &op);

@@ -24,9 +24,9 @@ namespace qmamd {
 constexpr uint32_t kMaxQubits = 34;
 
 // Device time of one launch and its algorithmic bytes: a plan op (kind
-// 0..6) or a merged pass (kinds 7..11: the generic, chain, wide chain, low
-// chain and generic low tile kernels; bytes of one pass over the amplitudes
-// it reads).
+// 0..6) or a merged pass (kinds 7..12: the generic, chain, wide chain, low
+// chain, generic low tile and generic wide kernels; bytes of one pass over
+// the amplitudes it reads).
 struct OpTime {
   uint32_t kind;
   double ms;

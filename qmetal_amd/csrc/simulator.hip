@@ -47,12 +47,13 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 // Timing record kinds of merged passes (plan-op kinds are 0..6): the
 // generic pass kernel, the chain kernel, the wide and low chain kernels, the
-// generic low tile kernel.
+// generic low tile and generic wide kernels.
 constexpr uint32_t kTimingPass = 7;
 constexpr uint32_t kTimingChain = 8;
 constexpr uint32_t kTimingWide = 9;
 constexpr uint32_t kTimingLow = 10;
 constexpr uint32_t kTimingTile = 11;
+constexpr uint32_t kTimingGWide = 12;
 
 int current_device() {
   int d = 0;
@@ -387,6 +388,9 @@ struct Simulator::Impl {
     // Generic low tile pass (segments of merged-pass ops).
     bool tile = false;
     TileArgs t{};
+    // Generic wide pass (merged-pass ops around the wide exchange).
+    bool gwide = false;
+    GWideArgs g{};
   };
 
   // Local coefficients of a phase op of a merged pass (turns).
@@ -1005,6 +1009,75 @@ struct Simulator::Impl {
     pl.bytes = pass_bytes(n, 0);
   }
 
+  // Generic wide pass (GWideArgs): the tile of a wide chain pass on wires
+  // ps.reg (four register wires, the first wb of which swap with the wb wave
+  // wires after them); ops of the first segment emitted against the first
+  // arrangement, the rest against the second.
+  void prep_gwide(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
+    const int steps = int(ps.reg.size()), wb = steps - 4;
+    if (wb < 2 || wb > 3 || ps.fixed || ps.seg_len.size() != 2) fail("internal: wide pass shape");
+    GWideArgs &ga = pl.g;
+    ga = GWideArgs{};
+    const uint32_t *w = ps.reg.data();
+    uint64_t tile = 1ull;
+    for (int i = 0; i < steps; i++) {
+      if (w[i] < kWideMinWire || w[i] >= n || ((tile >> w[i]) & 1)) fail("internal: wide pass wires");
+      ga.pos[i] = uint8_t(w[i]);
+      tile |= 1ull << w[i];
+    }
+    uint64_t lanes = 0;
+    for (uint32_t q = 1, got = 0; got < 6; q++) {
+      if ((tile >> q) & 1) fail("internal: wide pass lanes");
+      ga.lanepos[got++] = uint8_t(q);
+      lanes |= 1ull << q;
+    }
+    tile |= lanes;
+    Layout &L = ga.lay;
+    for (uint32_t q = 0; q < 64; q++)
+      if ((tile >> q) & 1) {
+        L.ins[L.nins] = uint8_t(q);
+        L.hm[L.nins] = ~((uint64_t(1) << q) - 1);
+        L.nins++;
+      }
+    L.nthreads = dim >> L.nins;
+    L.streaming = 1;
+    uint32_t loc_a[5] = {0, w[0], w[1], w[2], w[3]}, loc_b[5] = {0, w[0], w[1], w[2], w[3]};
+    uint64_t var_a = lanes, var_b = lanes;
+    for (int j = 0; j < wb; j++) {
+      loc_b[1 + j] = w[4 + j];
+      var_a |= 1ull << w[4 + j];
+      var_b |= 1ull << w[j];
+    }
+    size_t nops = 0;
+    for (uint32_t k : ps.ops) nops += pass_slots(p.ops[k]);
+    PassOp *po = reinterpret_cast<PassOp *>(staging + off);
+    DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off + nops * sizeof(PassOp));
+    size_t nt = 0;
+    std::vector<PhaseCoef> coef(nops);
+    auto frame = [](const uint32_t(&loc)[5]) {
+      return [&loc](uint32_t q) {
+        for (int l = 0; l < 5; l++)
+          if (loc[l] == q) return l;
+        return -1;
+      };
+    };
+    const size_t na = emit_ops(p, ps.ops.data(), ps.seg_len[0], frame(loc_a), 0, var_a, 5, po, dt,
+                               nt, coef.data());
+    const size_t nb = emit_ops(p, ps.ops.data() + ps.seg_len[0], ps.seg_len[1], frame(loc_b), 0,
+                               var_b, 5, po + na, dt, nt, coef.data() + na);
+    if (na + nb != nops || ps.seg_len[0] + ps.seg_len[1] != ps.ops.size())
+      fail("internal: wide pass op count");
+    ga.a = state;
+    ga.ops = reinterpret_cast<const PassOp *>(arena + off);
+    ga.terms = reinterpret_cast<const DiagTerm *>(arena + off + nops * sizeof(PassOp));
+    ga.na = uint32_t(na);
+    ga.nops = uint32_t(nops);
+    pl.gwide = true;
+    pl.chain_k = steps;
+    pl.nl = 5;
+    pl.bytes = pass_bytes(n, 0);
+  }
+
   void pass_launch(const PassLaunch &pl) {
     hipEvent_t a = nullptr;
     if (timing) {
@@ -1014,6 +1087,7 @@ struct Simulator::Impl {
     if (pl.wide) launch_wide(stream, pl.chain_k, pl.mode, pl.w);
     else if (pl.low) launch_low(stream, pl.chain_k, pl.chain_zero, pl.mode, pl.l);
     else if (pl.tile) launch_tile(stream, pl.t);
+    else if (pl.gwide) launch_gwide(stream, pl.chain_k - 4, pl.g);
     else if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);
     else launch_pass(stream, pl.nl, pl.a);
     launched();
@@ -1024,6 +1098,7 @@ struct Simulator::Impl {
       const uint32_t kind = pl.wide    ? kTimingWide
                             : pl.low   ? kTimingLow
                             : pl.tile  ? kTimingTile
+                            : pl.gwide ? kTimingGWide
                             : pl.chain ? kTimingChain
                                        : kTimingPass;
       timings.push_back(OpTiming{kind, pl.bytes, a, b});
@@ -1423,6 +1498,10 @@ void Simulator::run_plan(const Plan &p) {
       }
       if (passes[k].tile) {
         s->prep_tile(p, passes[k], offs[k], merged[k]);
+        continue;
+      }
+      if (passes[k].gwide) {
+        s->prep_gwide(p, passes[k], offs[k], merged[k]);
         continue;
       }
       if (passes[k].merged) {

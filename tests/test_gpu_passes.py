@@ -375,7 +375,7 @@ def test_tile_passes_against_oracle_and_op_by_op():
     """Generic low tile passes (merged-pass ops in several register
     arrangements of a 2^12 tile, permuted through LDS) with controls and
     phase terms on register, thread and workgroup-uniform bits."""
-    from test_schedule import tile_pass, chain_pass, moving_wires
+    from test_schedule import tile_pass, chain_pass, gwide_pass, moving_wires
     for n in (12, 13, 15, 17):
         for seed in range(3):
             gates = tile_stream(n, 60, 500 * n + seed)
@@ -385,15 +385,93 @@ def test_tile_passes_against_oracle_and_op_by_op():
             for k, q in enumerate(passes):
                 if q >= 0:
                     groups.setdefault(q, []).append(plan[k])
-            tiles = [g for g in groups.values()
-                     if len(set().union(*(moving_wires(o) for o in g))) > 4
-                     and not chain_pass(g, n)]
-            assert tiles and all(tile_pass(g, n) for g in tiles), "no tile pass formed"
+            big = [g for g in groups.values()
+                   if len(set().union(*(moving_wires(o) for o in g))) > 4]
+            assert all(chain_pass(g, n) or tile_pass(g, n) or gwide_pass(g, n) for g in big)
+            tiles = [g for g in big if tile_pass(g, n)]
+            assert tiles, "no tile pass formed"
             want = O.run(n, gates)
             got, norm, launches = run(n, gates)
             ref, _, _ = run(n, gates, merge=False)
             d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
             print(f"tile n={n} seed={seed}: {len(tiles)} tile passes, {launches} launches "
+                  f"(plan {len(plan)}), diff {d:.2e}, vs op-by-op {dm:.2e}")
+            assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
+            assert dm <= TOL_SAME
+
+
+def gwide_stream(n, rounds, seed):
+    """Rounds of mergeable gates on seven wires >= 7: a run with targets on
+    four of them, then a run with targets on the other three and one of the
+    four, with controls, swap partners and phase wires anywhere; an H on wire
+    3 separates the rounds."""
+    r = C.SplitMix64(seed)
+    gates = [(C.H, (q,), ()) for q in range(n)]
+
+    def pick(ws):
+        return ws[r.next_u64() % len(ws)]
+
+    def other(avoid):
+        while True:
+            q = r.next_u64() % n
+            if q not in avoid:
+                return q
+
+    def run(ws, count):
+        for _ in range(count):
+            u = r.next_u64() % 8
+            t = pick(ws)
+            if u < 2:
+                gates.append((C.U3, (t,), tuple(r.uniform(-3.2, 3.2) for _ in range(3))))
+            elif u < 3:
+                gates.append((C.RY, (t,), (r.uniform(-3.2, 3.2),)))
+            elif u < 5:
+                gates.append((C.CX, (other((t,)), t), ()))
+            elif u < 6:
+                gates.append((C.CP, (other((t,)), t), (r.uniform(-3.2, 3.2),)))
+            elif u < 7:
+                c0 = other((t,))
+                gates.append((C.CCX, (c0, other((t, c0)), t), ()))
+            else:
+                b = pick(ws)
+                if b != t:
+                    gates.append((C.SWAP, (t, b), ()))
+
+    for _ in range(rounds):
+        ws = []
+        while len(ws) < 7:
+            q = 7 + r.next_u64() % (n - 7)
+            if q not in ws:
+                ws.append(q)
+        run(ws[:4], 10)
+        run(ws[4:] + [ws[3]], 10)
+        gates.append((C.H, (3,), ()))
+    return gates
+
+
+def test_gwide_passes_against_oracle_and_op_by_op():
+    """Generic wide passes (merged-pass ops on a wide chain tile before and
+    after its LDS exchange) with controls and phase terms on register, lane,
+    wave and workgroup-uniform bits."""
+    from test_schedule import gwide_pass, chain_pass, tile_pass, moving_wires
+    for n in (16, 18):
+        for seed in range(3):
+            gates = gwide_stream(n, 3, 700 * n + seed)
+            c = C.to_circuit(n, gates)
+            plan, passes = c.plan(), c.passes()
+            groups = {}
+            for k, q in enumerate(passes):
+                if q >= 0:
+                    groups.setdefault(q, []).append(plan[k])
+            wide = [g for g in groups.values()
+                    if len(set().union(*(moving_wires(o) for o in g))) > 4
+                    and not chain_pass(g, n) and not tile_pass(g, n)]
+            assert len(wide) >= 2 and all(gwide_pass(g, n) for g in wide), "no generic wide pass"
+            want = O.run(n, gates)
+            got, norm, launches = run(n, gates)
+            ref, _, _ = run(n, gates, merge=False)
+            d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
+            print(f"gwide n={n} seed={seed}: {len(wide)} generic wide passes, {launches} launches "
                   f"(plan {len(plan)}), diff {d:.2e}, vs op-by-op {dm:.2e}")
             assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
             assert dm <= TOL_SAME

@@ -72,6 +72,26 @@ def tile_pass(ops, n):
     return 2 <= segs <= 8
 
 
+def gwide_pass(ops, n):
+    """A generic wide pass: mergeable ops on wires >= 7 (n >= 16), at most
+    64, that split in plan order into a run on at most four wires A and a run
+    on at most three new wires plus at most 4 - max(2, new) wires of A."""
+    if n < 16 or len(ops) > 64:
+        return False
+    ws = [moving_wires(o) for o in ops]
+    if any(o["kind"] == "Blocked1q" for o in ops) or any(q < 7 for w in ws for q in w):
+        return False
+    for cut in range(1, len(ops)):
+        a = set().union(*ws[:cut])
+        if len(a) > 4:
+            break
+        rest = set().union(*ws[cut:])
+        new, keep = rest - a, rest & a
+        if len(new) <= 3 and len(keep) <= 4 - max(2, len(new)):
+            return True
+    return False
+
+
 def check_grouping(circ, options=None):
     plan = circ.plan(options)
     passes = circ.passes(options)
@@ -95,7 +115,8 @@ def check_grouping(circ, options=None):
             wires = set().union(*(moving_wires(o) for o in ops))
             assert all(o["kind"] != "Blocked1q" for o in ops)
             if len(wires) > MAX_REG:
-                assert chain_pass(ops, circ.num_qubits) or tile_pass(ops, circ.num_qubits), ops
+                n = circ.num_qubits
+                assert chain_pass(ops, n) or tile_pass(ops, n) or gwide_pass(ops, n), ops
     return max(seen) + 1 if seen else 0
 
 
