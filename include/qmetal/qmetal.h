@@ -202,10 +202,17 @@ int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
  * merged passes (generic, chain-shaped, wide and low chain-shaped, generic
  * low tile and generic wide kernels).
  * qm_plan_passes reports the grouping qm_run would use: pass_of_op
- * (qm_plan_size entries, may be NULL) gets each op's pass index (UINT32_MAX
- * for an op that launches nothing), *num_passes the number of launches. */
+ * (qm_plan_size entries, may be NULL) gets the index of the first pass that
+ * runs each op (UINT32_MAX for an op that launches nothing), *num_passes the
+ * number of launches. Ops whose wires do not overlap may run out of plan
+ * order, and a Dense1qGroup's 2x2s may run in different passes:
+ * qm_plan_schedule lists every pass entry in execution order -- plan op,
+ * group wires it applies (0: the whole op), pass index -- writing at most
+ * `cap` entries (arrays may be NULL) and setting *count to their number. */
 int qm_set_pass_merging(qm_simulator s, int enable);
 int qm_plan_passes(qm_plan p, uint32_t *pass_of_op, size_t *num_passes);
+int qm_plan_schedule(qm_plan p, uint32_t *entry_op, uint64_t *entry_wires,
+                     uint32_t *entry_pass, size_t cap, size_t *count);
 
 #ifdef __cplusplus
 }

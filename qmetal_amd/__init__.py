@@ -170,6 +170,8 @@ _plan_nops = _decl("qm_plan_size", c_size_t, c_void_p)
 _plan_get = _decl("qm_plan_get", c_int, c_void_p, c_size_t, POINTER(_PlanOp))
 _plan_arrays = _decl("qm_plan_get_arrays", c_int, c_void_p, c_size_t, _u32p, _f64p, _u64p, _f64p)
 _plan_passes = _decl("qm_plan_passes", c_int, c_void_p, _u32p, POINTER(c_size_t))
+_plan_schedule = _decl("qm_plan_schedule", c_int, c_void_p, _u32p, POINTER(c_uint64), _u32p,
+                       c_size_t, POINTER(c_size_t))
 _set_pass_merging = _decl("qm_set_pass_merging", c_int, c_void_p, c_int)
 
 
@@ -307,10 +309,32 @@ class Circuit:
         """GPU passes this circuit lowers to -- what execution actually costs."""
         return int(_plan_size(self._h, 1 if fuse else 0))
 
+    def schedule(self, options: FusionOptions | None = None) -> list[list[tuple[int, int]]]:
+        """The passes qm_run makes for this circuit, in execution order: each
+        a list of (plan op index, group wires mask -- 0 for the whole op)."""
+        opts = options if options is not None else FusionOptions.default()
+        p = _plan_build(self._h, byref(opts))
+        if not p:
+            raise QMetalError(_err())
+        try:
+            count = c_size_t()
+            _check(_plan_schedule(p, None, None, None, 0, byref(count)))
+            m = max(int(count.value), 1)
+            ops, wires, pas = (c_uint32 * m)(), (c_uint64 * m)(), (c_uint32 * m)()
+            _check(_plan_schedule(p, ops, wires, pas, m, byref(count)))
+            out: list[list[tuple[int, int]]] = []
+            for e in range(int(count.value)):
+                if pas[e] == len(out):
+                    out.append([])
+                out[pas[e]].append((int(ops[e]), int(wires[e])))
+            return out
+        finally:
+            _plan_destroy(p)
+
     def passes(self, options: FusionOptions | None = None) -> list[int]:
-        """Pass index of every plan op under executor-level pass merging (-1
-        for an op that launches nothing); max + 1 is the number of launches
-        qm_run makes for this circuit."""
+        """Index of the first pass that runs each plan op under executor-level
+        pass merging (-1 for an op that launches nothing); max + 1 is the
+        number of launches qm_run makes for this circuit."""
         opts = options if options is not None else FusionOptions.default()
         p = _plan_build(self._h, byref(opts))
         if not p:

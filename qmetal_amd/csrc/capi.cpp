@@ -504,8 +504,28 @@ int qm_plan_passes(qm_plan p, uint32_t *pass_of_op, size_t *num_passes) {
       for (size_t k = 0; k < p->p.ops.size(); k++) pass_of_op[k] = UINT32_MAX;
     for (size_t i = 0; i < passes.size(); i++)
       for (uint32_t k : passes[i].ops)
-        if (pass_of_op) pass_of_op[k] = uint32_t(i);
+        if (pass_of_op && pass_of_op[k] == UINT32_MAX) pass_of_op[k] = uint32_t(i);
     *num_passes = passes.size();
+  });
+}
+
+int qm_plan_schedule(qm_plan p, uint32_t *entry_op, uint64_t *entry_wires,
+                     uint32_t *entry_pass, size_t cap, size_t *count) {
+  if (!p || !count) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] {
+    const std::vector<Pass> passes = schedule_passes(p->p);
+    size_t e = 0;
+    for (size_t i = 0; i < passes.size(); i++)
+      for (size_t j = 0; j < passes[i].ops.size(); j++, e++) {
+        if (e >= cap) continue;
+        if (entry_op) entry_op[e] = passes[i].ops[j];
+        if (entry_wires) entry_wires[e] = passes[i].wmask[j];
+        if (entry_pass) entry_pass[e] = uint32_t(i);
+      }
+    *count = e;
   });
 }
 

@@ -163,209 +163,316 @@ uint32_t low_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) {
 
 namespace {
 
-// Register-tile grouping of plan ops [k0, k1), greedy in plan order: an op
-// joins the open pass when its wires still fit and the merged pass moves no
-// more bytes than the open pass and the op would separately.
-void register_passes(const Plan &p, uint32_t k0, uint32_t k1, std::vector<Pass> &out) {
-  const uint32_t n = p.n;
-  Pass cur;
-  auto flush = [&]() {
-    if (cur.ops.empty()) return;
-    // A pass of one op runs through that op's own kernel, which can skip
-    // more (e.g. a diagonal's common low bits) and keeps its launch shape.
-    if (cur.ops.size() == 1) {
-      cur.merged = false;
-      cur.reg.clear();
-      cur.fixed = 0;
-    }
-    out.push_back(std::move(cur));
-    cur = Pass{};
-  };
+uint64_t bit(uint32_t q) { return uint64_t(1) << q; }
+
+// One schedulable unit: a plan op, or one wire's 2x2 of a Dense1qGroup (the
+// group's 2x2s act on distinct wires and commute, so they may run in
+// different passes).
+struct Atom {
+  uint32_t op;
+  uint64_t wires;  // group wire of this atom (0: the whole op)
+  uint64_t mov;    // wires it needs in registers (bit 0 excluded)
+  uint64_t sup;    // wires it reads or writes (controls, phase mask bits)
+  bool diag;       // a phase layer: commutes with every other phase layer
+  bool ok;         // may join a merged pass
+  std::vector<uint32_t> succ;
+  uint32_t ndeps = 0;
+};
+
+// Atoms of plan ops [k0, k1) and their order constraints: two atoms whose
+// wires overlap keep their plan order unless both are phase layers.
+std::vector<Atom> make_atoms(const Plan &p, uint32_t k0, uint32_t k1) {
+  std::vector<Atom> at;
   for (uint32_t k = k0; k < k1; k++) {
     const PlanOp &op = p.ops[k];
-    if (op.kind == OpKind::Diagonal && op.masks.empty()) continue;  // no dispatch
-    if (!pass_mergeable(op)) {
-      flush();
-      Pass s;
-      s.ops.push_back(k);
-      out.push_back(std::move(s));
-      continue;
-    }
-    const std::vector<uint32_t> w = pass_wires(op);
-    const uint64_t f = pass_fixed(op);
-    bool join = !cur.ops.empty() && cur.ops.size() < kPassMaxOps;
-    std::vector<uint32_t> reg = cur.reg;
-    if (join) {
-      for (uint32_t q : w)
-        if (std::find(reg.begin(), reg.end(), q) == reg.end()) reg.push_back(q);
-      const uint64_t fixed = cur.fixed & f;
-      // Registers plus fixed bits plus bit 0 must leave thread bits.
-      if (reg.size() > kPassMaxReg ||
-          reg.size() + 1 + uint32_t(__builtin_popcountll(fixed)) + 2 > n)
-        join = false;
-      else if (pass_bytes(n, fixed) > pass_bytes(n, cur.fixed) + op_alone_bytes(n, op))
-        join = false;
-      if (join) {
-        // No register wire may be a fixed bit.
-        for (uint32_t q : reg)
-          if ((fixed >> q) & 1) join = false;
+    auto add = [&](uint64_t wires, uint64_t mov, uint64_t sup, bool diag, bool ok) {
+      Atom a;
+      a.op = k;
+      a.wires = wires;
+      a.mov = mov & ~uint64_t(1);
+      a.sup = sup;
+      a.diag = diag;
+      a.ok = ok;
+      at.push_back(std::move(a));
+    };
+    switch (op.kind) {
+      case OpKind::Dense1q: add(0, bit(op.q[0]), bit(op.q[0]), false, true); break;
+      case OpKind::Group:
+        for (uint32_t w : op.wires) add(bit(w), bit(w), bit(w), false, true);
+        break;
+      case OpKind::Controlled: add(0, bit(op.q[1]), bit(op.q[0]) | bit(op.q[1]), false, true); break;
+      case OpKind::Swap:
+        add(0, bit(op.q[0]) | bit(op.q[1]), bit(op.q[0]) | bit(op.q[1]), false, true);
+        break;
+      case OpKind::CCX:
+        add(0, bit(op.q[2]), bit(op.q[0]) | bit(op.q[1]) | bit(op.q[2]), false, true);
+        break;
+      case OpKind::Diagonal: {
+        if (op.masks.empty()) break;  // launches nothing
+        uint64_t sup = 0;
+        for (uint64_t m : op.masks) sup |= m;
+        add(0, 0, sup, true, pass_mergeable(op));
+        break;
       }
-      if (join) {
-        cur.reg = reg;
-        cur.fixed = fixed;
-        cur.ops.push_back(k);
-        continue;
+      case OpKind::Blocked: {
+        uint64_t sup = 0;
+        for (uint32_t w : op.wires) sup |= bit(w);
+        add(0, sup, ~uint64_t(0), false, false);
+        break;
       }
     }
-    flush();
-    cur.merged = true;
-    cur.ops.push_back(k);
-    cur.reg = w;
-    cur.fixed = f;
   }
-  flush();
+  // Per wire: the last atom that is not a phase layer, and the phase layers
+  // after it.
+  std::vector<int64_t> last(64, -1);
+  std::vector<std::vector<uint32_t>> diags(64);
+  std::vector<uint32_t> deps;
+  for (uint32_t i = 0; i < at.size(); i++) {
+    deps.clear();
+    for (uint32_t q = 0; q < 64; q++) {
+      if (!((at[i].sup >> q) & 1)) continue;
+      if (last[q] >= 0) deps.push_back(uint32_t(last[q]));
+      if (!at[i].diag) deps.insert(deps.end(), diags[q].begin(), diags[q].end());
+    }
+    std::sort(deps.begin(), deps.end());
+    deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
+    for (uint32_t d : deps) at[d].succ.push_back(i);
+    at[i].ndeps = uint32_t(deps.size());
+    for (uint32_t q = 0; q < 64; q++) {
+      if (!((at[i].sup >> q) & 1)) continue;
+      if (at[i].diag) {
+        diags[q].push_back(i);
+      } else {
+        last[q] = i;
+        diags[q].clear();
+      }
+    }
+  }
+  return at;
 }
 
-double passes_bytes(const Plan &p, const std::vector<Pass> &ps) {
-  double b = 0.0;
-  for (const Pass &s : ps)
-    b += s.merged ? pass_bytes(p.n, s.fixed) : op_alone_bytes(p.n, p.ops[s.ops[0]]);
-  return b;
+enum class Kind { Reg, Tile, GWide };
+
+// A pass being built: its atoms in execution order and the register state of
+// its kind.
+struct Draft {
+  Kind kind;
+  std::vector<uint32_t> atoms;
+  // Reg / Tile: register wires of the current (tile: last) segment.
+  uint64_t reg = 0;
+  std::vector<uint32_t> seg_len;
+  std::vector<uint64_t> seg_reg;
+  // GWide: wires of the first arrangement, then the new wires and the kept
+  // first-arrangement wires of the second.
+  bool second = false;
+  uint64_t a = 0, fresh = 0, keep = 0;
+  uint32_t na = 0;
+};
+
+int popc(uint64_t m) { return __builtin_popcountll(m); }
+
+// Cost of adding an atom moving `mov` to the draft (lower is better), or -1
+// when it does not fit; `apply` commits it.
+int fit(const Plan &p, Draft &d, uint64_t mov, bool apply) {
+  switch (d.kind) {
+    case Kind::Reg: {
+      const uint64_t r = d.reg | mov;
+      if (popc(r) > int(kPassMaxReg) || popc(r) + 3 > int(p.n)) return -1;
+      const int c = popc(r) - popc(d.reg);
+      if (apply) d.reg = r;
+      return c;
+    }
+    case Kind::Tile: {
+      if (mov >> (kTileMaxWire + 1)) return -1;
+      const uint64_t r = d.reg | mov;
+      if (!d.seg_len.empty() && popc(r) <= int(kPassMaxReg)) {
+        const int c = popc(r) - popc(d.reg);
+        if (apply) {
+          d.reg = r;
+          d.seg_len.back()++;
+          d.seg_reg.back() = r;
+        }
+        return c;
+      }
+      if (d.seg_len.size() >= kTileMaxSegs || popc(mov) > int(kPassMaxReg)) return -1;
+      if (apply) {
+        d.reg = mov;
+        d.seg_len.push_back(1);
+        d.seg_reg.push_back(mov);
+      }
+      return 8;
+    }
+    case Kind::GWide: {
+      if (mov & (bit(kWideMinWire) - 1)) return -1;
+      if (!d.second) {
+        const uint64_t a = d.a | mov;
+        if (popc(a) <= 4) {
+          const int c = popc(a) - popc(d.a);
+          if (apply) {
+            d.a = a;
+            d.na++;
+          }
+          return c;
+        }
+      }
+      const uint64_t f = d.fresh | (mov & ~d.a), k = d.keep | (mov & d.a);
+      const int wb = std::max(2, popc(f));
+      if (popc(f) > 3 || popc(k) > 4 - wb || popc(d.a | f) > 7) return -1;
+      const int c = 5 + popc(f) - popc(d.fresh);
+      if (apply) {
+        d.second = true;
+        d.fresh = f;
+        d.keep = k;
+      }
+      return c;
+    }
+  }
+  return -1;
+}
+
+// Greedy draft of one kind from the ready atoms: repeatedly take the
+// fitting atom with the lowest cost (then the earliest), letting atoms whose
+// order constraints it releases join later.
+Draft draft(const Plan &p, const std::vector<Atom> &at, const std::vector<uint32_t> &ready,
+            std::vector<uint32_t> &ndeps, Kind kind) {
+  Draft d;
+  d.kind = kind;
+  std::vector<uint32_t> cand = ready, touched;
+  while (d.atoms.size() < kTileMaxOps) {
+    int best = -1, best_c = 0;
+    size_t best_j = 0;
+    for (size_t j = 0; j < cand.size(); j++) {
+      const Atom &a = at[cand[j]];
+      if (!a.ok) continue;
+      const int c = fit(p, d, a.mov, false);
+      if (c < 0) continue;
+      if (best < 0 || c < best_c || (c == best_c && cand[j] < uint32_t(best))) {
+        best = int(cand[j]);
+        best_c = c;
+        best_j = j;
+      }
+    }
+    if (best < 0) break;
+    fit(p, d, at[best].mov, true);
+    d.atoms.push_back(uint32_t(best));
+    cand.erase(cand.begin() + long(best_j));
+    for (uint32_t s : at[best].succ) {
+      touched.push_back(s);
+      if (--ndeps[s] == 0) cand.push_back(s);
+    }
+  }
+  for (uint32_t s : touched) ndeps[s]++;  // undo: only the chosen draft commits
+  return d;
+}
+
+uint64_t fixed_of(const PlanOp &op, uint64_t wires) { return wires ? 0 : pass_fixed(op); }
+
+// Schedules plan ops [k0, k1) (no chain passes among them) into passes.
+void dag_passes(const Plan &p, uint32_t k0, uint32_t k1, std::vector<Pass> &out) {
+  std::vector<Atom> at = make_atoms(p, k0, k1);
+  std::vector<uint32_t> ndeps(at.size()), ready;
+  for (uint32_t i = 0; i < at.size(); i++) {
+    ndeps[i] = at[i].ndeps;
+    if (!ndeps[i]) ready.push_back(i);
+  }
+  size_t done = 0;
+  while (done < at.size()) {
+    Draft best = draft(p, at, ready, ndeps, Kind::Reg);
+    if (p.n >= kTileMinQubits) {
+      Draft t = draft(p, at, ready, ndeps, Kind::Tile);
+      if (t.seg_len.size() >= 2 && t.atoms.size() > best.atoms.size()) best = std::move(t);
+    }
+    if (p.n >= kWideMinQubits) {
+      Draft g = draft(p, at, ready, ndeps, Kind::GWide);
+      if (g.second && g.atoms.size() > best.atoms.size()) best = std::move(g);
+    }
+    Pass s;
+    if (best.atoms.empty()) {
+      // Only ops that cannot join a merged pass are ready: the earliest
+      // runs alone.
+      best.atoms = {*std::min_element(ready.begin(), ready.end())};
+      best.kind = Kind::Reg;
+    }
+    for (uint32_t i : best.atoms) {
+      s.ops.push_back(at[i].op);
+      s.wmask.push_back(at[i].wires);
+    }
+    if (best.kind == Kind::Tile) {
+      s.merged = s.tile = true;
+      s.seg_len = best.seg_len;
+      for (uint64_t r : best.seg_reg) {
+        std::vector<uint32_t> w;
+        for (uint32_t q = 1; q < 64; q++)
+          if ((r >> q) & 1) w.push_back(q);
+        s.seg_reg.push_back(w);
+      }
+    } else if (best.kind == Kind::GWide) {
+      s.merged = s.gwide = true;
+      s.seg_len = {best.na, uint32_t(best.atoms.size()) - best.na};
+      const int wb = std::max(2, popc(best.fresh));
+      // Register positions: wires that swap out first (local bits 1..wb),
+      // then the ones that stay, then the wave wires; spares at or above
+      // kWideMinWire fill up.
+      std::vector<uint32_t> out_w, stay, wave;
+      for (uint32_t q = 0; q < 64; q++) {
+        if ((best.keep >> q) & 1) stay.push_back(q);
+        if ((best.fresh >> q) & 1) wave.push_back(q);
+      }
+      for (uint32_t q = 0; q < 64; q++)
+        if (((best.a & ~best.keep) >> q) & 1) (int(stay.size()) < 4 - wb ? stay : out_w).push_back(q);
+      uint64_t used = best.a | best.fresh;
+      uint32_t spare = kWideMinWire;
+      auto next = [&]() {
+        while ((used >> spare) & 1) spare++;
+        used |= bit(spare);
+        return spare;
+      };
+      while (int(out_w.size()) < wb) out_w.push_back(next());
+      while (int(stay.size()) < 4 - wb) stay.push_back(next());
+      while (int(wave.size()) < wb) wave.push_back(next());
+      if (spare >= p.n) {
+        // No spare wire: run the atoms as register passes instead.
+        s = Pass{};
+        best.kind = Kind::Reg;
+        best.atoms.resize(1);
+        s.ops.push_back(at[best.atoms[0]].op);
+        s.wmask.push_back(at[best.atoms[0]].wires);
+      } else {
+        s.reg = out_w;
+        s.reg.insert(s.reg.end(), stay.begin(), stay.end());
+        s.reg.insert(s.reg.end(), wave.begin(), wave.end());
+      }
+    }
+    if (best.kind == Kind::Reg) {
+      const bool whole = s.ops.size() == 1 && s.wmask[0] == 0;
+      if (whole) {
+        s.merged = false;
+      } else {
+        s.merged = true;
+        uint64_t fixed = ~uint64_t(0), reg = 0;
+        for (uint32_t i : best.atoms) {
+          fixed &= fixed_of(p.ops[at[i].op], at[i].wires);
+          reg |= at[i].mov;
+        }
+        s.fixed = fixed & ~reg & ~uint64_t(1);
+        // Registers, fixed bits and bit 0 must leave thread bits.
+        while (popc(reg) + 1 + popc(s.fixed) + 2 > int(p.n)) s.fixed &= s.fixed - 1;
+        for (uint32_t q = 1; q < 64; q++)
+          if ((reg >> q) & 1) s.reg.push_back(q);
+      }
+    }
+    // Commit.
+    for (uint32_t i : best.atoms) {
+      ready.erase(std::find(ready.begin(), ready.end(), i));
+      for (uint32_t t : at[i].succ)
+        if (--ndeps[t] == 0) ready.push_back(t);
+      done++;
+    }
+    out.push_back(std::move(s));
+  }
 }
 
 }  // namespace
-
-uint32_t tile_pass_at(const Plan &p, uint32_t k, Pass &pass) {
-  pass = Pass{};
-  if (p.n < kTileMinQubits) return 0;
-  std::vector<uint32_t> cur, chain;
-  uint32_t j = k, len = 0;
-  while (j < p.ops.size() && j - k < kTileMaxOps) {
-    const PlanOp &op = p.ops[j];
-    if (op.kind == OpKind::Diagonal && op.masks.empty()) break;
-    if (!pass_mergeable(op)) break;
-    // A chain pass starting here runs on its own, faster kernel.
-    if (j > k && (wide_chain_at(p, j, chain) || low_chain_at(p, j, chain))) break;
-    const std::vector<uint32_t> w = pass_wires(op);
-    bool inside = true;
-    for (uint32_t q : w)
-      if (q > kTileMaxWire) inside = false;
-    if (!inside) break;
-    std::vector<uint32_t> reg = cur;
-    for (uint32_t q : w)
-      if (std::find(reg.begin(), reg.end(), q) == reg.end()) reg.push_back(q);
-    if (pass.seg_len.empty() || reg.size() > kPassMaxReg) {
-      if (pass.seg_len.size() == kTileMaxSegs) break;
-      if (!pass.seg_len.empty()) {
-        pass.seg_len.back() = len;
-        pass.seg_reg.back() = cur;
-      }
-      pass.seg_len.push_back(0);
-      pass.seg_reg.emplace_back();
-      cur = w;
-      len = 0;
-    } else {
-      cur = reg;
-    }
-    pass.ops.push_back(j);
-    len++;
-    j++;
-  }
-  if (!pass.seg_len.empty()) {
-    pass.seg_len.back() = len;
-    pass.seg_reg.back() = cur;
-  }
-  if (pass.seg_len.size() < 2) {
-    pass = Pass{};
-    return 0;
-  }
-  std::vector<Pass> regular;
-  register_passes(p, k, j, regular);
-  if (pass_bytes(p.n, 0) >= passes_bytes(p, regular)) {
-    pass = Pass{};
-    return 0;
-  }
-  pass.merged = pass.tile = true;
-  return j - k;
-}
-
-uint32_t gwide_pass_at(const Plan &p, uint32_t k, Pass &pass) {
-  pass = Pass{};
-  if (p.n < kWideMinQubits) return 0;
-  std::vector<uint32_t> A, fresh, keep, chain;
-  auto has = [](const std::vector<uint32_t> &v, uint32_t q) {
-    return std::find(v.begin(), v.end(), q) != v.end();
-  };
-  uint32_t j = k, na = 0, nb = 0;
-  bool second = false;
-  while (j < p.ops.size() && j - k < kTileMaxOps) {
-    const PlanOp &op = p.ops[j];
-    if (op.kind == OpKind::Diagonal && op.masks.empty()) break;
-    if (!pass_mergeable(op)) break;
-    if (j > k && (wide_chain_at(p, j, chain) || low_chain_at(p, j, chain))) break;
-    const std::vector<uint32_t> w = pass_wires(op);
-    bool high = true;
-    for (uint32_t q : w)
-      if (q < kWideMinWire) high = false;
-    if (!high) break;
-    if (!second) {
-      std::vector<uint32_t> a = A;
-      for (uint32_t q : w)
-        if (!has(a, q)) a.push_back(q);
-      if (a.size() <= 4) {
-        A = a;
-        na++;
-        j++;
-        continue;
-      }
-      second = true;
-    }
-    std::vector<uint32_t> f = fresh, kp = keep;
-    for (uint32_t q : w) {
-      if (has(A, q)) {
-        if (!has(kp, q)) kp.push_back(q);
-      } else if (!has(f, q)) {
-        f.push_back(q);
-      }
-    }
-    const size_t wb = std::max<size_t>(2, f.size());
-    if (f.size() > 3 || kp.size() > 4 - wb) break;
-    fresh = f;
-    keep = kp;
-    nb++;
-    j++;
-  }
-  if (na == 0 || nb == 0) return 0;
-  const size_t wb = std::max<size_t>(2, fresh.size());
-  // Register positions: the wires that swap out first (local bits 1..wb),
-  // then the ones that stay; every other wire of the tile is a spare at or
-  // above kWideMinWire.
-  std::vector<uint32_t> stay = keep, out;
-  for (uint32_t q : A)
-    if (!has(stay, q)) (stay.size() < 4 - wb ? stay : out).push_back(q);
-  std::vector<uint32_t> wave = fresh;
-  uint32_t spare = kWideMinWire;
-  auto next_spare = [&]() {
-    while (has(A, spare) || has(fresh, spare) || has(out, spare) || has(stay, spare) ||
-           has(wave, spare))
-      spare++;
-    return spare;
-  };
-  while (out.size() < wb) out.push_back(next_spare());
-  while (stay.size() < 4 - wb) stay.push_back(next_spare());
-  while (wave.size() < wb) wave.push_back(next_spare());
-  if (spare >= p.n) return 0;
-  std::vector<Pass> regular;
-  register_passes(p, k, j, regular);
-  if (pass_bytes(p.n, 0) >= passes_bytes(p, regular)) return 0;
-  pass.merged = pass.gwide = true;
-  pass.reg = out;
-  pass.reg.insert(pass.reg.end(), stay.begin(), stay.end());
-  pass.reg.insert(pass.reg.end(), wave.begin(), wave.end());
-  for (uint32_t i = k; i < j; i++) pass.ops.push_back(i);
-  pass.seg_len = {na, nb};
-  return j - k;
-}
 
 std::vector<Pass> schedule_passes(const Plan &p) {
   std::vector<Pass> out;
@@ -375,12 +482,13 @@ std::vector<Pass> schedule_passes(const Plan &p) {
       if (p.ops[k].kind == OpKind::Diagonal && p.ops[k].masks.empty()) continue;
       Pass s;
       s.ops.push_back(k);
+      s.wmask.push_back(0);
       out.push_back(std::move(s));
     }
     return out;
   }
-  // Runs of ops that no chain or tile pass starts in go to the register-tile
-  // grouping as a whole.
+  // Chain passes are cut out in plan order; the ops between them are
+  // scheduled as a dependency graph.
   uint32_t run0 = 0;
   std::vector<uint32_t> steps;
   for (uint32_t k = 0; k < p.ops.size(); k++) {
@@ -389,31 +497,22 @@ std::vector<Pass> schedule_passes(const Plan &p) {
     uint32_t used = wide_chain_at(p, k, steps);
     const bool is_wide = used != 0;
     if (!used) used = low_chain_at(p, k, steps);
-    if (used) {
-      register_passes(p, run0, k, out);
-      Pass s;
-      s.merged = true;
-      s.wide = is_wide;
-      s.low = !is_wide;
-      for (uint32_t j = k; j < k + used; j++) s.ops.push_back(j);
-      s.reg = steps;
-      out.push_back(std::move(s));
-      k += used - 1;
-      run0 = k + 1;
-      continue;
+    if (!used) continue;
+    dag_passes(p, run0, k, out);
+    Pass s;
+    s.merged = true;
+    s.wide = is_wide;
+    s.low = !is_wide;
+    for (uint32_t j = k; j < k + used; j++) {
+      s.ops.push_back(j);
+      s.wmask.push_back(0);
     }
-    // A generic low tile or wide pass from here: the one taking more ops.
-    Pass t, g;
-    const uint32_t tu = tile_pass_at(p, k, t);
-    const uint32_t gu = gwide_pass_at(p, k, g);
-    if (tu || gu) {
-      register_passes(p, run0, k, out);
-      out.push_back(std::move(gu > tu ? g : t));
-      k += std::max(tu, gu) - 1;
-      run0 = k + 1;
-    }
+    s.reg = steps;
+    out.push_back(std::move(s));
+    k += used - 1;
+    run0 = k + 1;
   }
-  register_passes(p, run0, uint32_t(p.ops.size()), out);
+  dag_passes(p, run0, uint32_t(p.ops.size()), out);
   return out;
 }
 

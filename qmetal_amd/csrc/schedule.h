@@ -39,7 +39,9 @@ struct Pass {
                                // segments around the exchange
   std::vector<uint32_t> seg_len;               // tile: plan ops per segment
   std::vector<std::vector<uint32_t>> seg_reg;  // tile: register wires per segment
-  std::vector<uint32_t> ops;   // plan-op indices, in plan order
+  std::vector<uint32_t> ops;   // plan-op indices, in execution order
+  std::vector<uint64_t> wmask; // per entry of ops: for a Dense1qGroup the
+                               // wires of it this pass applies (0: all)
   std::vector<uint32_t> reg;   // register wires (merged), first use order;
                                // wide and low: the step wires in order
   uint64_t fixed = 0;          // fixed-one bits (merged)
@@ -72,27 +74,27 @@ constexpr uint32_t kLowMinQubits = 12;
 // the step wires (a final 0 for a step on wire 0).
 uint32_t low_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires);
 
-// Generic low tile passes: consecutive mergeable plan ops whose moving wires
-// all lie in 1..kTileMaxWire (controls and phase masks anywhere), in
-// segments of at most kPassMaxReg register wires (wire 0 is always local),
-// over tiles of 2^(kTileMaxWire+1) contiguous amplitudes permuted through LDS
-// between segments. Taken when it needs at least two segments and moves
-// fewer bytes than the register-tile passes over the same ops would.
+// Between chain passes the plan's ops are scheduled as a dependency graph:
+// two ops keep their plan order when their wires overlap, unless both are
+// phase layers (those commute), and a Dense1qGroup's 2x2s are separate
+// units. Each pass is drafted greedily from the ops whose predecessors have
+// run, in three shapes, and the one taking the most ops wins (ties: the
+// register tile):
+//   register tile: at most kPassMaxReg wires moved (bit 0 aside);
+//   generic low tile: moved wires in 1..kTileMaxWire, in at most
+//     kTileMaxSegs segments of kPassMaxReg register wires, over tiles of
+//     2^(kTileMaxWire+1) contiguous amplitudes permuted through LDS between
+//     segments (taken with two segments or more);
+//   generic wide: moved wires at or above kWideMinWire, over the tile of a
+//     wide chain pass: ops on at most four register wires, then, after the
+//     exchange, on at most three new wires plus the register wires that stay
+//     (taken when the second part is used).
+// At most kTileMaxOps units per pass; an op that cannot join a merged pass,
+// and a pass of one whole op, runs through the op's own kernel.
 constexpr uint32_t kTileMaxWire = 11;
 constexpr uint32_t kTileMinQubits = 12;
 constexpr uint32_t kTileMaxSegs = 8;
 constexpr uint32_t kTileMaxOps = 64;
-
-// Number of plan ops from `k` that form a generic low tile pass (0 if
-// none); `pass` gets its ops and segments.
-uint32_t tile_pass_at(const Plan &p, uint32_t k, Pass &pass);
-
-// Generic wide passes: consecutive mergeable plan ops whose moving wires are
-// all at or above kWideMinWire, over the tile of a wide chain pass: ops on at
-// most four register wires, then, after the exchange, ops on at most three
-// wires that were wave bits plus the register wires that stay. Taken under
-// the same conditions as tile passes.
-uint32_t gwide_pass_at(const Plan &p, uint32_t k, Pass &pass);
 
 // Whether plan op `op` can join a merged pass.
 bool pass_mergeable(const PlanOp &op);
@@ -106,9 +108,7 @@ double pass_bytes(uint32_t n, uint64_t fixed);
 // HBM bytes `op` moves when it runs alone through its own kernel.
 double op_alone_bytes(uint32_t n, const PlanOp &op);
 
-// Greedy grouping in plan order: an op joins the open pass when its wires
-// still fit and the merged pass moves no more bytes than the open pass and
-// the op would separately.
+// The passes of a plan, in execution order.
 std::vector<Pass> schedule_passes(const Plan &p);
 
 }  // namespace qmamd

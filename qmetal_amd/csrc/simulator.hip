@@ -399,9 +399,11 @@ struct Simulator::Impl {
     double gamma[kPassMaxNL][kPassMaxNL] = {};
   };
 
-  static size_t pass_slots(const PlanOp &op) {
+  // 2x2 slots of a pass entry (`wires`: the Dense1qGroup wires it applies,
+  // 0 for all).
+  static size_t pass_slots(const PlanOp &op, uint64_t wires = 0) {
     switch (op.kind) {
-      case OpKind::Group: return op.wires.size();
+      case OpKind::Group: return wires ? size_t(__builtin_popcountll(wires)) : op.wires.size();
       case OpKind::Swap: return 3;
       default: return 1;
     }
@@ -409,17 +411,24 @@ struct Simulator::Impl {
 
   size_t pass_param_bytes(const Plan &p, const Pass &ps) const {
     size_t b = 0;
-    for (uint32_t k : ps.ops) {
-      b += pass_slots(p.ops[k]) * sizeof(PassOp);
-      if (p.ops[k].kind == OpKind::Diagonal) b += p.ops[k].masks.size() * sizeof(DiagTerm);
+    for (size_t e = 0; e < ps.ops.size(); e++) {
+      const PlanOp &op = p.ops[ps.ops[e]];
+      b += pass_slots(op, ps.wmask[e]) * sizeof(PassOp);
+      if (op.kind == OpKind::Diagonal) b += op.masks.size() * sizeof(DiagTerm);
     }
     b += ps.seg_len.size() * sizeof(TileSeg);
     return (b + 255) & ~size_t(255);
   }
 
-  // Lays out a merged pass: register wires first, fixed bits, fill bits at
-  // the top; then one PassOp per 2x2 slot or phase layer, in plan order.
-  // PassOps of plan ops `ks` (in order) against a tile frame: `local_of`
+  // 2x2 slots and phase layers of pass entries [e0, e1).
+  static size_t entry_slots(const Plan &p, const Pass &ps, size_t e0, size_t e1) {
+    size_t n = 0;
+    for (size_t e = e0; e < e1; e++) n += pass_slots(p.ops[ps.ops[e]], ps.wmask[e]);
+    return n;
+  }
+
+  // PassOps of plan ops `ks` (in order; `wmask`: the Dense1qGroup wires each
+  // entry applies, 0 for all) against a tile frame: `local_of`
   // maps a wire to its tile-local bit (0..nl-1) or -1, `fixed` bits are 1 for
   // every amplitude the pass touches, `var` are the base-index bits that
   // differ between the threads of a workgroup (their terms are per thread,
@@ -427,9 +436,9 @@ struct Simulator::Impl {
   // dt[nt..], advancing nt; `coef` gets each phase op's local coefficients.
   // Returns the number of ops written.
   template <typename LocalOf>
-  size_t emit_ops(const Plan &p, const uint32_t *ks, size_t count, LocalOf local_of,
-                  uint64_t fixed, uint64_t var, int nl, PassOp *po, DiagTerm *dt, size_t &nt,
-                  PhaseCoef *coef) const {
+  size_t emit_ops(const Plan &p, const uint32_t *ks, const uint64_t *wmask, size_t count,
+                  LocalOf local_of, uint64_t fixed, uint64_t var, int nl, PassOp *po,
+                  DiagTerm *dt, size_t &nt, PhaseCoef *coef) const {
     size_t i = 0;
     auto local = [&](uint32_t q) {
       const int l = local_of(q);
@@ -463,6 +472,7 @@ struct Simulator::Impl {
           break;
         case OpKind::Group:
           for (size_t w = 0; w < op.wires.size(); w++) {
+            if (wmask[kk] && !((wmask[kk] >> op.wires[w]) & 1)) continue;
             narrow(&op.mats[4 * w], m8);
             slot(op.wires[w], m8);
           }
@@ -570,14 +580,14 @@ struct Simulator::Impl {
     pl.a.lay = lb.finish(want, &pl.nl);
     if (pl.nl != want) fail("internal: pass width");
     const uint64_t lane = lane_bits(pl.a.lay);
-    size_t nops = 0;
-    for (uint32_t k : ps.ops) nops += pass_slots(p.ops[k]);
+    const size_t nops = entry_slots(p, ps, 0, ps.ops.size());
     PassOp *po = reinterpret_cast<PassOp *>(staging + off);
     DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off + nops * sizeof(PassOp));
     size_t nt = 0;
     std::vector<PhaseCoef> coef(nops);
-    if (emit_ops(p, ps.ops.data(), ps.ops.size(), [&](uint32_t q) { return lb.local_of(q); },
-                 lb.fixed, lane, pl.nl, po, dt, nt, coef.data()) != nops)
+    if (emit_ops(p, ps.ops.data(), ps.wmask.data(), ps.ops.size(),
+                 [&](uint32_t q) { return lb.local_of(q); }, lb.fixed, lane, pl.nl, po, dt, nt,
+                 coef.data()) != nops)
       fail("internal: pass op count");
     pl.a.a = state;
     pl.a.ops = reinterpret_cast<const PassOp *>(arena + off);
@@ -942,8 +952,7 @@ struct Simulator::Impl {
     const size_t nseg = ps.seg_len.size();
     if (nseg < 2 || nseg != ps.seg_reg.size() || ps.fixed || n < kTileMinQubits)
       fail("internal: tile pass shape");
-    size_t nops = 0;
-    for (uint32_t k : ps.ops) nops += pass_slots(p.ops[k]);
+    const size_t nops = entry_slots(p, ps, 0, ps.ops.size());
     TileSeg *sg = reinterpret_cast<TileSeg *>(staging + off);
     const size_t ops_off = off + nseg * sizeof(TileSeg);
     PassOp *po = reinterpret_cast<PassOp *>(staging + ops_off);
@@ -983,8 +992,8 @@ struct Simulator::Impl {
         return -1;
       };
       g.op0 = uint32_t(o);
-      o += emit_ops(p, ps.ops.data() + k0, ps.seg_len[s], local_of, 0, var, 5, po + o, dt, nt,
-                    coef.data() + o);
+      o += emit_ops(p, ps.ops.data() + k0, ps.wmask.data() + k0, ps.seg_len[s], local_of, 0, var,
+                    5, po + o, dt, nt, coef.data() + o);
       g.op1 = uint32_t(o);
       k0 += ps.seg_len[s];
     }
@@ -1048,8 +1057,7 @@ struct Simulator::Impl {
       var_a |= 1ull << w[4 + j];
       var_b |= 1ull << w[j];
     }
-    size_t nops = 0;
-    for (uint32_t k : ps.ops) nops += pass_slots(p.ops[k]);
+    const size_t nops = entry_slots(p, ps, 0, ps.ops.size());
     PassOp *po = reinterpret_cast<PassOp *>(staging + off);
     DiagTerm *dt = reinterpret_cast<DiagTerm *>(staging + off + nops * sizeof(PassOp));
     size_t nt = 0;
@@ -1061,10 +1069,11 @@ struct Simulator::Impl {
         return -1;
       };
     };
-    const size_t na = emit_ops(p, ps.ops.data(), ps.seg_len[0], frame(loc_a), 0, var_a, 5, po, dt,
-                               nt, coef.data());
-    const size_t nb = emit_ops(p, ps.ops.data() + ps.seg_len[0], ps.seg_len[1], frame(loc_b), 0,
-                               var_b, 5, po + na, dt, nt, coef.data() + na);
+    const size_t na = emit_ops(p, ps.ops.data(), ps.wmask.data(), ps.seg_len[0], frame(loc_a), 0,
+                               var_a, 5, po, dt, nt, coef.data());
+    const size_t nb = emit_ops(p, ps.ops.data() + ps.seg_len[0], ps.wmask.data() + ps.seg_len[0],
+                               ps.seg_len[1], frame(loc_b), 0, var_b, 5, po + na, dt, nt,
+                               coef.data() + na);
     if (na + nb != nops || ps.seg_len[0] + ps.seg_len[1] != ps.ops.size())
       fail("internal: wide pass op count");
     ga.a = state;
@@ -1471,6 +1480,7 @@ void Simulator::run_plan(const Plan &p) {
     for (uint32_t k = 0; k < p.ops.size(); k++) {
       Pass ps;
       ps.ops.push_back(k);
+      ps.wmask.push_back(0);
       passes.push_back(std::move(ps));
     }
   }

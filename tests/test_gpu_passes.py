@@ -58,6 +58,15 @@ def mixed_stream(n, count, seed):
     return gates
 
 
+def shapes(c, options=None):
+    """Shape of every pass qm_run makes for `c` (after validating the
+    schedule: every op once, overlapping ops in plan order)."""
+    from test_schedule import check_schedule, pass_shape
+    check_schedule(c, options)
+    plan = c.plan(options)
+    return [pass_shape(plan, e, c.num_qubits) for e in c.schedule(options)]
+
+
 def run(n, gates, merge=True, options=None, prep=None):
     with qm.Simulator(n) as s:
         s.set_pass_merging(merge)
@@ -102,12 +111,12 @@ def test_configurations_merged_vs_op_by_op(name, n):
         got, norm, launches = run(n, gates, options=options, prep=prep)
         ref, _, ref_launches = run(n, gates, merge=False, options=options, prep=prep)
         d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
-        passes = c.passes(options)
+        passes = c.schedule(options)
         print(f"{name:9s} n={n:2d} {opt:9s} plan={len(passes):4d} launches={launches:4d} "
               f"diff={d:.2e} vs op-by-op {dm:.2e}")
         assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP, (opt, d)
         assert dm <= TOL_SAME, (opt, dm)
-        assert launches == max(passes) + 1
+        assert launches == len(passes)
         assert ref_launches == sum(1 for k, op in enumerate(c.plan(options))
                                    if not (op["kind"] == "DiagonalLayer" and not op["masks"]))
 
@@ -126,7 +135,7 @@ def test_dispatch_count_follows_grouping():
         before = s.dispatch_count
         s.run(c)
         s.synchronize()
-        assert s.dispatch_count - before == max(c.passes()) + 1 < len(c)
+        assert s.dispatch_count - before == len(c.schedule()) < len(c)
 
 
 def test_qft_24_merged_vs_op_by_op():
@@ -260,9 +269,9 @@ def test_wide_passes_against_oracle_and_op_by_op(steps):
             prep = C.product_state_prep(n, seed=seed)
             gates = wide_stream(n, 2, 100 * n + seed + 7 * steps, kind, steps)
             c = C.to_circuit(n, gates)
-            plan, passes = c.plan(), c.passes()
-            wide = [p for p in set(passes) if p >= 0 and sum(
-                1 for k, q in enumerate(passes) if q == p and plan[k]["kind"] == "Dense1q") == steps]
+            plan = c.plan()
+            wide = [e for e, sh in zip(c.schedule(), shapes(c)) if sh == "chain"
+                    and sum(plan[k]["kind"] == "Dense1q" for k, _ in e) == steps]
             assert len(wide) == 2, "wide passes not formed"
             want = O.run(n, prep + gates)
             got, norm, launches = run(n, gates, prep=prep)
@@ -316,9 +325,8 @@ def test_low_passes_against_oracle_and_op_by_op():
             prep = C.product_state_prep(n, seed=seed)
             gates = low_stream(n, 4, 300 * n + seed, kind)
             c = C.to_circuit(n, gates)
-            plan, passes = c.plan(), c.passes()
-            low = [p for p in set(passes) if p >= 0 and sum(
-                1 for k, q in enumerate(passes) if q == p and plan[k]["kind"] == "Dense1q") >= 5]
+            plan = c.plan()
+            low = [sh for sh in shapes(c) if sh == "chain"]
             assert len(low) >= 3, "low passes not formed"
             want = O.run(n, prep + gates)
             got, norm, launches = run(n, gates, prep=prep)
@@ -375,20 +383,12 @@ def test_tile_passes_against_oracle_and_op_by_op():
     """Generic low tile passes (merged-pass ops in several register
     arrangements of a 2^12 tile, permuted through LDS) with controls and
     phase terms on register, thread and workgroup-uniform bits."""
-    from test_schedule import tile_pass, chain_pass, gwide_pass, moving_wires
     for n in (12, 13, 15, 17):
         for seed in range(3):
             gates = tile_stream(n, 60, 500 * n + seed)
             c = C.to_circuit(n, gates)
-            plan, passes = c.plan(), c.passes()
-            groups = {}
-            for k, q in enumerate(passes):
-                if q >= 0:
-                    groups.setdefault(q, []).append(plan[k])
-            big = [g for g in groups.values()
-                   if len(set().union(*(moving_wires(o) for o in g))) > 4]
-            assert all(chain_pass(g, n) or tile_pass(g, n) or gwide_pass(g, n) for g in big)
-            tiles = [g for g in big if tile_pass(g, n)]
+            plan = c.plan()
+            tiles = [sh for sh in shapes(c) if sh == "tile"]
             assert tiles, "no tile pass formed"
             want = O.run(n, gates)
             got, norm, launches = run(n, gates)
@@ -453,20 +453,13 @@ def test_gwide_passes_against_oracle_and_op_by_op():
     """Generic wide passes (merged-pass ops on a wide chain tile before and
     after its LDS exchange) with controls and phase terms on register, lane,
     wave and workgroup-uniform bits."""
-    from test_schedule import gwide_pass, chain_pass, tile_pass, moving_wires
     for n in (16, 18):
         for seed in range(3):
             gates = gwide_stream(n, 3, 700 * n + seed)
             c = C.to_circuit(n, gates)
-            plan, passes = c.plan(), c.passes()
-            groups = {}
-            for k, q in enumerate(passes):
-                if q >= 0:
-                    groups.setdefault(q, []).append(plan[k])
-            wide = [g for g in groups.values()
-                    if len(set().union(*(moving_wires(o) for o in g))) > 4
-                    and not chain_pass(g, n) and not tile_pass(g, n)]
-            assert len(wide) >= 2 and all(gwide_pass(g, n) for g in wide), "no generic wide pass"
+            plan = c.plan()
+            wide = [sh for sh in shapes(c) if sh == "gwide"]
+            assert len(wide) >= 2, "no generic wide pass"
             want = O.run(n, gates)
             got, norm, launches = run(n, gates)
             ref, _, _ = run(n, gates, merge=False)
