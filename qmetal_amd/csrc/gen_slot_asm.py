@@ -23,47 +23,47 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def bfly_real(a, b):
+    """(a, b) <- (m00 a + m01 b, m10 a + m11 b), real m: each output's first
+    product goes to a temporary, the second is a fused multiply-add written
+    in place as the input's last use (no moves)."""
     ra, ia, rb, ib = f"%[r{a}]", f"%[i{a}]", f"%[r{b}]", f"%[i{b}]"
-    return [
-        f"v_mul_f32 %[t0], %[m1r], {rb}",
-        f"v_fmac_f32 %[t0], %[m0r], {ra}",
-        f"v_mul_f32 %[t1], %[m1r], {ib}",
-        f"v_fmac_f32 %[t1], %[m0r], {ia}",
-        f"v_mul_f32 {rb}, %[m3r], {rb}",
-        f"v_fmac_f32 {rb}, %[m2r], {ra}",
-        f"v_mul_f32 {ib}, %[m3r], {ib}",
-        f"v_fmac_f32 {ib}, %[m2r], {ia}",
-        f"v_mov_b32 {ra}, %[t0]",
-        f"v_mov_b32 {ia}, %[t1]",
-    ]
+    out = []
+    for xa, xb in ((ra, rb), (ia, ib)):
+        out += [
+            f"v_mul_f32 %[t0], %[m2r], {xa}",
+            f"v_mul_f32 %[t1], %[m1r], {xb}",
+            f"v_fma_f32 {xb}, %[m3r], {xb}, %[t0]",
+            f"v_fma_f32 {xa}, %[m0r], {xa}, %[t1]",
+        ]
+    return out
 
 
 def bfly_cplx(a, b):
+    """Complex m: three terms of each output into four temporaries, then the
+    fourth term of each, fused and written in place (no moves)."""
     ra, ia, rb, ib = f"%[r{a}]", f"%[i{a}]", f"%[r{b}]", f"%[i{b}]"
     return [
         # a'.x = m00r ax - m00i ay + m01r bx - m01i by
-        f"v_mul_f32 %[t0], %[m0r], {ra}",
-        f"v_fma_f32 %[t0], -%[m0i], {ia}, %[t0]",
-        f"v_fmac_f32 %[t0], %[m1r], {rb}",
+        f"v_mul_f32 %[t0], %[m0i], {ia}",
+        f"v_fma_f32 %[t0], %[m1r], {rb}, -%[t0]",
         f"v_fma_f32 %[t0], -%[m1i], {ib}, %[t0]",
         # a'.y = m00r ay + m00i ax + m01r by + m01i bx
-        f"v_mul_f32 %[t1], %[m0r], {ia}",
-        f"v_fmac_f32 %[t1], %[m0i], {ra}",
+        f"v_mul_f32 %[t1], %[m0i], {ra}",
         f"v_fmac_f32 %[t1], %[m1r], {ib}",
         f"v_fmac_f32 %[t1], %[m1i], {rb}",
         # b'.x = m10r ax - m10i ay + m11r bx - m11i by
         f"v_mul_f32 %[t2], %[m2r], {ra}",
         f"v_fma_f32 %[t2], -%[m2i], {ia}, %[t2]",
-        f"v_fmac_f32 %[t2], %[m3r], {rb}",
         f"v_fma_f32 %[t2], -%[m3i], {ib}, %[t2]",
-        # b'.y = m10r ay + m10i ax + m11r by + m11i bx, in place (last reader)
-        f"v_mul_f32 {ib}, %[m3r], {ib}",
-        f"v_fmac_f32 {ib}, %[m2r], {ia}",
-        f"v_fmac_f32 {ib}, %[m2i], {ra}",
-        f"v_fmac_f32 {ib}, %[m3i], {rb}",
-        f"v_mov_b32 {ra}, %[t0]",
-        f"v_mov_b32 {ia}, %[t1]",
-        f"v_mov_b32 {rb}, %[t2]",
+        # b'.y = m10r ay + m10i ax + m11r by + m11i bx
+        f"v_mul_f32 %[t3], %[m2r], {ia}",
+        f"v_fmac_f32 %[t3], %[m2i], {ra}",
+        f"v_fmac_f32 %[t3], %[m3i], {rb}",
+        # last terms, in place
+        f"v_fma_f32 {ra}, %[m0r], {ra}, %[t0]",
+        f"v_fma_f32 {ia}, %[m0r], {ia}, %[t1]",
+        f"v_fma_f32 {rb}, %[m3r], {rb}, %[t2]",
+        f"v_fma_f32 {ib}, %[m3r], {ib}, %[t3]",
     ]
 
 
@@ -98,11 +98,11 @@ def statement(nl):
     return f"""template <>
 __device__ __forceinline__ void slot_asm<{nl}>(float2 (&x)[{n}], uint32_t sel, uint32_t pm,
                                              const float (&m)[8]) {{
-  float t0, t1, t2;
+  float t0, t1, t2, t3;
   asm(
 {body}
       : {outs.strip()},
-        [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2)
+        [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
       : [m0r] "v"(m[0]), [m0i] "v"(m[1]), [m1r] "v"(m[2]), [m1i] "v"(m[3]),
         [m2r] "v"(m[4]), [m2i] "v"(m[5]), [m3r] "v"(m[6]), [m3i] "v"(m[7]),
         [sel] "s"(sel), [pm] "s"(pm)

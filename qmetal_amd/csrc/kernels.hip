@@ -103,17 +103,19 @@ __device__ __forceinline__ void rot(const float *m, float2 &x, float2 &y) {
 
 // a *= f in place (f per lane) unless `skip` (workgroup-uniform, in an SGPR)
 // is set; the branch is inside the statement, so a is defined once either way.
+// Complex products in place: (ax, ay) <- (fx ax - fy ay, fx ay + fy ax), the
+// cross products into temporaries first, then one fused multiply-add per
+// component written in place (no moves).
 __device__ __forceinline__ void cmul_inplace(float2 &a, float2 f, uint32_t skip) {
-  float t;
+  float t, u;
   asm("s_cmp_lg_u32 %[skip], 0\n\t"
       "s_cbranch_scc1 .Lqm%=\n\t"
-      "v_mul_f32 %[t], %[ax], %[fy]\n\t"
-      "v_fmac_f32 %[t], %[ay], %[fx]\n\t"
-      "v_mul_f32 %[ax], %[ax], %[fx]\n\t"
-      "v_fma_f32 %[ax], -%[ay], %[fy], %[ax]\n\t"
-      "v_mov_b32 %[ay], %[t]\n"
+      "v_mul_f32 %[t], %[fy], %[ax]\n\t"
+      "v_mul_f32 %[u], %[fy], %[ay]\n\t"
+      "v_fma_f32 %[ax], %[fx], %[ax], -%[u]\n\t"
+      "v_fma_f32 %[ay], %[fx], %[ay], %[t]\n"
       ".Lqm%=:"
-      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t)
+      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t), [u] "=&v"(u)
       : [fx] "v"(f.x), [fy] "v"(f.y), [skip] "s"(skip)
       : "scc");
 }
@@ -126,69 +128,62 @@ __device__ __forceinline__ void cmul_inplace(float2 &a, float2 f, uint32_t skip)
 // operands are workgroup-uniform (SGPRs) unless the name ends in _v.
 __device__ __forceinline__ void bfly_real_s(float m00, float m01, float m10, float m11,
                                             float2 &a, float2 &b) {
-  float t0, t1;
-  asm("v_mul_f32 %[t0], %[m01], %[bx]\n\t"
-      "v_fmac_f32 %[t0], %[m00], %[ax]\n\t"
-      "v_mul_f32 %[t1], %[m01], %[by]\n\t"
-      "v_fmac_f32 %[t1], %[m00], %[ay]\n\t"
-      "v_mul_f32 %[bx], %[m11], %[bx]\n\t"
-      "v_fmac_f32 %[bx], %[m10], %[ax]\n\t"
-      "v_mul_f32 %[by], %[m11], %[by]\n\t"
-      "v_fmac_f32 %[by], %[m10], %[ay]\n\t"
-      "v_mov_b32 %[ax], %[t0]\n\t"
-      "v_mov_b32 %[ay], %[t1]"
+  float t0, t1, t2, t3;
+  asm("v_mul_f32 %[t0], %[m10], %[ax]\n\t"
+      "v_mul_f32 %[t1], %[m01], %[bx]\n\t"
+      "v_mul_f32 %[t2], %[m10], %[ay]\n\t"
+      "v_mul_f32 %[t3], %[m01], %[by]\n\t"
+      "v_fma_f32 %[bx], %[m11], %[bx], %[t0]\n\t"
+      "v_fma_f32 %[ax], %[m00], %[ax], %[t1]\n\t"
+      "v_fma_f32 %[by], %[m11], %[by], %[t2]\n\t"
+      "v_fma_f32 %[ay], %[m00], %[ay], %[t3]"
       : [ax] "+v"(a.x), [ay] "+v"(a.y), [bx] "+v"(b.x), [by] "+v"(b.y), [t0] "=&v"(t0),
-        [t1] "=&v"(t1)
+        [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
       : [m00] "s"(m00), [m01] "s"(m01), [m10] "s"(m10), [m11] "s"(m11));
 }
 
 __device__ __forceinline__ void bfly_cplx_s(const float *m, float2 &a, float2 &b) {
-  float t0, t1, t2;
-  asm("v_mul_f32 %[t0], %[m0r], %[ax]\n\t"
-      "v_fma_f32 %[t0], -%[m0i], %[ay], %[t0]\n\t"
-      "v_fmac_f32 %[t0], %[m1r], %[bx]\n\t"
+  float t0, t1, t2, t3;
+  asm("v_mul_f32 %[t0], %[m0i], %[ay]\n\t"
+      "v_fma_f32 %[t0], %[m1r], %[bx], -%[t0]\n\t"
       "v_fma_f32 %[t0], -%[m1i], %[by], %[t0]\n\t"
-      "v_mul_f32 %[t1], %[m0r], %[ay]\n\t"
-      "v_fmac_f32 %[t1], %[m0i], %[ax]\n\t"
+      "v_mul_f32 %[t1], %[m0i], %[ax]\n\t"
       "v_fmac_f32 %[t1], %[m1r], %[by]\n\t"
       "v_fmac_f32 %[t1], %[m1i], %[bx]\n\t"
       "v_mul_f32 %[t2], %[m2r], %[ax]\n\t"
       "v_fma_f32 %[t2], -%[m2i], %[ay], %[t2]\n\t"
-      "v_fmac_f32 %[t2], %[m3r], %[bx]\n\t"
       "v_fma_f32 %[t2], -%[m3i], %[by], %[t2]\n\t"
-      "v_mul_f32 %[by], %[m3r], %[by]\n\t"
-      "v_fmac_f32 %[by], %[m2r], %[ay]\n\t"
-      "v_fmac_f32 %[by], %[m2i], %[ax]\n\t"
-      "v_fmac_f32 %[by], %[m3i], %[bx]\n\t"
-      "v_mov_b32 %[ax], %[t0]\n\t"
-      "v_mov_b32 %[ay], %[t1]\n\t"
-      "v_mov_b32 %[bx], %[t2]"
+      "v_mul_f32 %[t3], %[m2r], %[ay]\n\t"
+      "v_fmac_f32 %[t3], %[m2i], %[ax]\n\t"
+      "v_fmac_f32 %[t3], %[m3i], %[bx]\n\t"
+      "v_fma_f32 %[ax], %[m0r], %[ax], %[t0]\n\t"
+      "v_fma_f32 %[ay], %[m0r], %[ay], %[t1]\n\t"
+      "v_fma_f32 %[bx], %[m3r], %[bx], %[t2]\n\t"
+      "v_fma_f32 %[by], %[m3r], %[by], %[t3]"
       : [ax] "+v"(a.x), [ay] "+v"(a.y), [bx] "+v"(b.x), [by] "+v"(b.y), [t0] "=&v"(t0),
-        [t1] "=&v"(t1), [t2] "=&v"(t2)
+        [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
       : [m0r] "s"(m[0]), [m0i] "s"(m[1]), [m1r] "s"(m[2]), [m1i] "s"(m[3]), [m2r] "s"(m[4]),
         [m2i] "s"(m[5]), [m3r] "s"(m[6]), [m3i] "s"(m[7]));
 }
 
 // a *= f in place.
 __device__ __forceinline__ void cmul_s(float2 &a, float fx, float fy) {
-  float t;
+  float t, u;
   asm("v_mul_f32 %[t], %[fy], %[ax]\n\t"
-      "v_fmac_f32 %[t], %[fx], %[ay]\n\t"
-      "v_mul_f32 %[ax], %[fx], %[ax]\n\t"
-      "v_fma_f32 %[ax], -%[fy], %[ay], %[ax]\n\t"
-      "v_mov_b32 %[ay], %[t]"
-      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t)
+      "v_mul_f32 %[u], %[fy], %[ay]\n\t"
+      "v_fma_f32 %[ax], %[fx], %[ax], -%[u]\n\t"
+      "v_fma_f32 %[ay], %[fx], %[ay], %[t]"
+      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t), [u] "=&v"(u)
       : [fx] "s"(fx), [fy] "s"(fy));
 }
 
 __device__ __forceinline__ void cmul_v(float2 &a, float2 f) {
-  float t;
+  float t, u;
   asm("v_mul_f32 %[t], %[fy], %[ax]\n\t"
-      "v_fmac_f32 %[t], %[fx], %[ay]\n\t"
-      "v_mul_f32 %[ax], %[fx], %[ax]\n\t"
-      "v_fma_f32 %[ax], -%[fy], %[ay], %[ax]\n\t"
-      "v_mov_b32 %[ay], %[t]"
-      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t)
+      "v_mul_f32 %[u], %[fy], %[ay]\n\t"
+      "v_fma_f32 %[ax], %[fx], %[ax], -%[u]\n\t"
+      "v_fma_f32 %[ay], %[fx], %[ay], %[t]"
+      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t), [u] "=&v"(u)
       : [fx] "v"(f.x), [fy] "v"(f.y));
 }
 
@@ -403,22 +398,49 @@ __device__ __forceinline__ void pass_ops(float2 (&x)[1 << NL], const_ptr<PassOp>
   // Every op runs both statements that write the tile, the 2x2 slot and
   // the per-combination phase multiply, each of which skips itself inside
   // (a branch around either would be a second definition of the tile).
+  // Each op's header is loaded as one block into scalar registers (otherwise
+  // each field is loaded, and waited for, in its own branch), one op ahead of
+  // its use so that the load overlaps the current op's arithmetic.
+  struct Hdr {
+    uint32_t type, cm, sel, pm;
+    uint64_t gmask;
+    float m[8];
+  };
+  auto load_hdr = [&](uint32_t o, Hdr &h) {
+    h.type = ops[o].type;
+    h.cm = ops[o].cmask;
+    h.sel = ops[o].sel;
+    h.pm = ops[o].pm;
+    h.gmask = ops[o].gmask;
+#pragma unroll
+    for (int i = 0; i < 8; i++) h.m[i] = ops[o].m[i];
+  };
+  auto pin = [](Hdr &h) {
+    asm volatile("" : "+s"(h.type), "+s"(h.cm), "+s"(h.sel), "+s"(h.pm), "+s"(h.gmask),
+                 "+s"(h.m[0]), "+s"(h.m[1]), "+s"(h.m[2]), "+s"(h.m[3]), "+s"(h.m[4]),
+                 "+s"(h.m[5]), "+s"(h.m[6]), "+s"(h.m[7]));
+  };
+  Hdr next{};
+  if (o0 < o1) {
+    load_hdr(o0, next);
+    pin(next);
+  }
   for (uint32_t o = o0; o < o1; o++) {
-    const bool is_phase = ops[o].type == kPassPhase;
-    const uint64_t gmask = ops[o].gmask;
-    const uint32_t cm = ops[o].cmask;
+    Hdr h = next;
+    load_hdr(o + 1 < o1 ? o + 1 : o, next);
+    const uint32_t type = h.type, cm = h.cm, sel = h.sel, pm = h.pm;
+    const uint64_t gmask = h.gmask;
+    float m[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) m[i] = h.m[i];
+    const bool is_phase = type == kPassPhase;
     const bool gon = (base & gmask) == gmask;
     {
       // A control on a base-index bit: the identity where this lane's
       // base lacks it. A phase op's section code matches nothing.
-      float m[8];
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        const float mi = ops[o].m[i];
-        m[i] = gon ? mi : (i == 0 || i == 6 ? 1.0f : 0.0f);
-      }
-      slot_asm<NL>(x, __builtin_amdgcn_readfirstlane(ops[o].sel),
-                   __builtin_amdgcn_readfirstlane(ops[o].pm), m);
+      for (int i = 0; i < 8; i++) m[i] = gon ? m[i] : (i == 0 || i == 6 ? 1.0f : 0.0f);
+      slot_asm<NL>(x, sel, pm, m);
     }
     // The phase multiply, for a phase op only (a uniform branch around
     // in-place statements: the tile keeps its registers on both paths).
@@ -493,6 +515,7 @@ __device__ __forceinline__ void pass_ops(float2 (&x)[1 << NL], const_ptr<PassOp>
         cmul_inplace(x[l], f, __builtin_amdgcn_readfirstlane((uint32_t(l) & cm) != cm));
       }
     }
+    pin(next);
   }
 }
 
