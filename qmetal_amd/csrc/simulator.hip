@@ -1752,12 +1752,16 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   s->dispatches++;
 
   if (s->merge && s->n >= 6) {
-    // Backward sweep in merged passes: consecutive inverse gates whose
-    // target wires fit one 16-amplitude tile (wire 0 plus three register
-    // wires), at most kAdjMaxRot rotations each, applied to both states in
-    // one read and one write of each; overlaps reduced per pass.
+    // Backward sweep in merged passes: inverse gates whose target wires fit
+    // one 16-amplitude tile (wire 0 plus three register wires), at most
+    // kAdjMaxRot rotations each, applied to both states in one read and one
+    // write of each; overlaps reduced per pass. Gates are taken from a
+    // dependency graph of the reversed circuit: two gates keep their reverse
+    // order when their wires overlap, unless both are diagonal. A gate moved
+    // past others commutes with them, so each rotation's overlap <l|P|psi> is
+    // the one the gate-by-gate sweep would see (U'P U'^dagger = P).
     struct APass {
-      std::vector<size_t> gates;  // gate indices, sweep order
+      std::vector<size_t> gates;  // gate indices, execution order
       std::vector<uint32_t> reg;
       uint32_t nrot = 0;
       size_t rot0 = 0;  // sweep index of its first rotation
@@ -1769,29 +1773,82 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       else t.push_back(g.q[g.nq - 1]);
       return t;
     };
-    std::vector<APass> passes;
-    APass cur;
-    size_t sweep_rot = 0;
-    for (size_t k = G; k-- > 0;) {
-      const Gate &g = c.gates[k];
-      std::vector<uint32_t> reg = cur.reg;
-      for (uint32_t q : targets(g))
-        if (q != 0 && std::find(reg.begin(), reg.end(), q) == reg.end()) reg.push_back(q);
-      const uint32_t r = is_rotation(g.kind) ? 1u : 0u;
-      if (!cur.gates.empty() && (reg.size() > 3 || cur.nrot + r > uint32_t(kAdjMaxRot))) {
-        passes.push_back(std::move(cur));
-        cur = APass{};
-        reg.clear();
-        for (uint32_t q : targets(g))
-          if (q != 0 && std::find(reg.begin(), reg.end(), q) == reg.end()) reg.push_back(q);
+    // Units in sweep order (gate G-1 first) and their order constraints.
+    std::vector<std::vector<uint32_t>> succ(G);
+    std::vector<uint32_t> ndeps(G, 0);
+    {
+      std::vector<int64_t> last(64, -1);
+      std::vector<std::vector<uint32_t>> diags(64);
+      std::vector<uint32_t> deps;
+      for (size_t u = 0; u < G; u++) {
+        const Gate &g = c.gates[G - 1 - u];
+        const bool dg = is_diagonal(g);
+        deps.clear();
+        for (uint32_t j = 0; j < g.nq; j++) {
+          const uint32_t q = g.q[j];
+          if (last[q] >= 0) deps.push_back(uint32_t(last[q]));
+          if (!dg) deps.insert(deps.end(), diags[q].begin(), diags[q].end());
+        }
+        std::sort(deps.begin(), deps.end());
+        deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
+        for (uint32_t d : deps) succ[d].push_back(uint32_t(u));
+        ndeps[u] = uint32_t(deps.size());
+        for (uint32_t j = 0; j < g.nq; j++) {
+          const uint32_t q = g.q[j];
+          if (dg) {
+            diags[q].push_back(uint32_t(u));
+          } else {
+            last[q] = int64_t(u);
+            diags[q].clear();
+          }
+        }
       }
-      if (cur.gates.empty()) cur.rot0 = sweep_rot;
-      cur.gates.push_back(k);
-      cur.reg = reg;
-      cur.nrot += r;
-      sweep_rot += r;
     }
-    if (!cur.gates.empty()) passes.push_back(std::move(cur));
+    std::vector<uint32_t> ready;
+    for (uint32_t u = 0; u < G; u++)
+      if (!ndeps[u]) ready.push_back(u);
+    std::vector<APass> passes;
+    std::vector<size_t> rot_index(G, 0);  // sweep index of each rotation
+    size_t sweep_rot = 0, done = 0;
+    while (done < G) {
+      APass cur;
+      cur.rot0 = sweep_rot;
+      for (;;) {
+        // The ready gate adding the fewest register wires, then the earliest.
+        int best = -1;
+        size_t best_j = 0, best_add = 0;
+        for (size_t j = 0; j < ready.size(); j++) {
+          const Gate &g = c.gates[G - 1 - ready[j]];
+          size_t add = 0;
+          for (uint32_t q : targets(g))
+            if (q != 0 && std::find(cur.reg.begin(), cur.reg.end(), q) == cur.reg.end()) add++;
+          const uint32_t r = is_rotation(g.kind) ? 1u : 0u;
+          if (cur.reg.size() + add > 3 || cur.nrot + r > uint32_t(kAdjMaxRot)) continue;
+          if (best < 0 || add < best_add || (add == best_add && ready[j] < uint32_t(best))) {
+            best = int(ready[j]);
+            best_j = j;
+            best_add = add;
+          }
+        }
+        if (best < 0) break;
+        const size_t k = G - 1 - size_t(best);
+        const Gate &g = c.gates[k];
+        for (uint32_t q : targets(g))
+          if (q != 0 && std::find(cur.reg.begin(), cur.reg.end(), q) == cur.reg.end())
+            cur.reg.push_back(q);
+        if (is_rotation(g.kind)) {
+          rot_index[k] = sweep_rot++;
+          cur.nrot++;
+        }
+        cur.gates.push_back(k);
+        ready.erase(ready.begin() + long(best_j));
+        for (uint32_t t : succ[size_t(best)])
+          if (--ndeps[t] == 0) ready.push_back(t);
+        done++;
+      }
+      if (cur.gates.empty()) fail("internal: adjoint schedule");
+      passes.push_back(std::move(cur));
+    }
     // Ops of every pass in one staging block.
     std::vector<size_t> offs(passes.size());
     size_t total = 0;
@@ -1898,11 +1955,10 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
     }
     const double *r = s->fetch_results(2 * (G + 1) + 2 * nrot_total);
     std::vector<double> grad(G, 0.0);
-    size_t sw = 0;
-    for (size_t k = G; k-- > 0;) {
+    for (size_t k = 0; k < G; k++) {
       const Gate &g = c.gates[k];
       if (!is_rotation(g.kind)) continue;
-      const double *o = r + 2 * (G + 1) + 2 * sw++;
+      const double *o = r + 2 * (G + 1) + 2 * rot_index[k];
       grad[k] = g.kind == K_RY ? o[0] : o[1];
     }
     return {r[2 * G], grad};
