@@ -132,33 +132,47 @@ __device__ __forceinline__ void acc_cj(double &re, double &im, float2 x, float2 
   im += s * (double(x.x) * y.y - double(x.y) * y.x);
 }
 
+// Every pass below handles several units per thread per iteration, loading
+// all of them before any arithmetic, so that a workgroup keeps enough loads
+// in flight to stream a state that lives in HBM (one workgroup per instance).
+
 template <int K>
 __device__ void group(float2 *s, uint32_t dim, const BOp &op, const float *mats) {
+  constexpr int U = K >= 4 ? 1 : 16 >> K;  // units per thread per iteration
   const uint32_t units = dim >> K;
-  for (uint32_t u = threadIdx.x; u < units; u += blockDim.x) {
-    uint32_t base = u;
+  for (uint32_t u0 = threadIdx.x; u0 < units; u0 += U * blockDim.x) {
+    float2 x[U][1 << K];
+    uint32_t idx[U][1 << K];
 #pragma unroll
-    for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
-    float2 x[1 << K];
-    uint32_t idx[1 << K];
+    for (int r = 0; r < U; r++) {
+      const uint32_t u = u0 + r * blockDim.x;
+      if (u >= units) continue;
+      uint32_t base = u;
 #pragma unroll
-    for (int l = 0; l < (1 << K); l++) {
-      uint32_t o = base;
+      for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
 #pragma unroll
-      for (int j = 0; j < K; j++)
-        if ((l >> j) & 1) o |= 1u << op.w[j];
-      idx[l] = o;
-      x[l] = s[o];
+      for (int l = 0; l < (1 << K); l++) {
+        uint32_t o = base;
+#pragma unroll
+        for (int j = 0; j < K; j++)
+          if ((l >> j) & 1) o |= 1u << op.w[j];
+        idx[r][l] = o;
+        x[r][l] = s[o];
+      }
     }
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-      const float *m = mats + op.a + 8 * j;
+    for (int r = 0; r < U; r++) {
+      if (u0 + r * blockDim.x >= units) continue;
 #pragma unroll
-      for (int l = 0; l < (1 << K); l++)
-        if (!((l >> j) & 1)) rot2(m, x[l], x[l | (1 << j)]);
+      for (int j = 0; j < K; j++) {
+        const float *m = mats + op.a + 8 * j;
+#pragma unroll
+        for (int l = 0; l < (1 << K); l++)
+          if (!((l >> j) & 1)) rot2(m, x[r][l], x[r][l | (1 << j)]);
+      }
+#pragma unroll
+      for (int l = 0; l < (1 << K); l++) s[idx[r][l]] = x[r][l];
     }
-#pragma unroll
-    for (int l = 0; l < (1 << K); l++) s[idx[l]] = x[l];
   }
 }
 
@@ -179,52 +193,62 @@ template <int K>
 __device__ void rotgroup(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
                          const float *mats, const uint32_t *aux, double *out,
                          double (*red)[kWaves]) {
+  constexpr int U = 8 >> K;  // units per thread per iteration
   double acc[2 * K];
 #pragma unroll
   for (int i = 0; i < 2 * K; i++) acc[i] = 0.0;
   const uint32_t units = dim >> K;
-  for (uint32_t u = threadIdx.x; u < units; u += blockDim.x) {
-    uint32_t base = u;
+  for (uint32_t u0 = threadIdx.x; u0 < units; u0 += U * blockDim.x) {
+    float2 p[U][1 << K], l[U][1 << K];
+    uint32_t idx[U][1 << K];
 #pragma unroll
-    for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
-    float2 p[1 << K], l[1 << K];
-    uint32_t idx[1 << K];
+    for (int r = 0; r < U; r++) {
+      const uint32_t u = u0 + r * blockDim.x;
+      if (u >= units) continue;
+      uint32_t base = u;
 #pragma unroll
-    for (int t = 0; t < (1 << K); t++) {
-      uint32_t o = base;
-#pragma unroll
-      for (int j = 0; j < K; j++)
-        if ((t >> j) & 1) o |= 1u << op.w[j];
-      idx[t] = o;
-      p[t] = psi[o];
-      l[t] = lam[o];
-    }
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-      const uint32_t axis = aux[op.c + 2 * j];
+      for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
 #pragma unroll
       for (int t = 0; t < (1 << K); t++) {
-        if ((t >> j) & 1) continue;
-        const int t1 = t | (1 << j);
-        if (axis == 0) {         // X
-          acc_cj(acc[2 * j], acc[2 * j + 1], l[t], p[t1], 1.0);
-          acc_cj(acc[2 * j], acc[2 * j + 1], l[t1], p[t], 1.0);
-        } else if (axis == 1) {  // Y, before its factor i
-          acc_cj(acc[2 * j], acc[2 * j + 1], l[t], p[t1], -1.0);
-          acc_cj(acc[2 * j], acc[2 * j + 1], l[t1], p[t], 1.0);
-        } else if (axis == 2) {  // Z
-          acc_cj(acc[2 * j], acc[2 * j + 1], l[t], p[t], 1.0);
-          acc_cj(acc[2 * j], acc[2 * j + 1], l[t1], p[t1], -1.0);
-        }
-        const float *m = mats + op.a + 8 * j;
-        rot2(m, p[t], p[t1]);
-        rot2(m, l[t], l[t1]);
+        uint32_t o = base;
+#pragma unroll
+        for (int j = 0; j < K; j++)
+          if ((t >> j) & 1) o |= 1u << op.w[j];
+        idx[r][t] = o;
+        p[r][t] = psi[o];
+        l[r][t] = lam[o];
       }
     }
 #pragma unroll
-    for (int t = 0; t < (1 << K); t++) {
-      psi[idx[t]] = p[t];
-      lam[idx[t]] = l[t];
+    for (int r = 0; r < U; r++) {
+      if (u0 + r * blockDim.x >= units) continue;
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        const uint32_t axis = aux[op.c + 2 * j];
+#pragma unroll
+        for (int t = 0; t < (1 << K); t++) {
+          if ((t >> j) & 1) continue;
+          const int t1 = t | (1 << j);
+          if (axis == 0) {         // X
+            acc_cj(acc[2 * j], acc[2 * j + 1], l[r][t], p[r][t1], 1.0);
+            acc_cj(acc[2 * j], acc[2 * j + 1], l[r][t1], p[r][t], 1.0);
+          } else if (axis == 1) {  // Y, before its factor i
+            acc_cj(acc[2 * j], acc[2 * j + 1], l[r][t], p[r][t1], -1.0);
+            acc_cj(acc[2 * j], acc[2 * j + 1], l[r][t1], p[r][t], 1.0);
+          } else if (axis == 2) {  // Z
+            acc_cj(acc[2 * j], acc[2 * j + 1], l[r][t], p[r][t], 1.0);
+            acc_cj(acc[2 * j], acc[2 * j + 1], l[r][t1], p[r][t1], -1.0);
+          }
+          const float *m = mats + op.a + 8 * j;
+          rot2(m, p[r][t], p[r][t1]);
+          rot2(m, l[r][t], l[r][t1]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < (1 << K); t++) {
+        psi[idx[r][t]] = p[r][t];
+        lam[idx[r][t]] = l[r][t];
+      }
     }
   }
   block_sum<2 * K>(acc, red);
@@ -248,26 +272,57 @@ __device__ void rotgroup_any(float2 *psi, float2 *lam, uint32_t dim, const BOp &
   }
 }
 
+constexpr int kElems = 8;  // elements per thread per iteration of the flat passes
+
 __device__ void ctrl(float2 *s, uint32_t dim, const BOp &op) {
-  const uint32_t t = op.w[0], bit = 1u << t, cm = uint32_t(op.mask);
-  for (uint32_t u = threadIdx.x; u < dim / 2; u += blockDim.x) {
-    const uint32_t i = ins0(u, t);
-    if ((i & cm) != cm) continue;
-    float2 x = s[i], y = s[i | bit];
-    rot2(op.m, x, y);
-    s[i] = x;
-    s[i | bit] = y;
+  const uint32_t t = op.w[0], bit = 1u << t, cm = uint32_t(op.mask), half = dim / 2;
+  for (uint32_t u0 = threadIdx.x; u0 < half; u0 += kElems * blockDim.x) {
+    float2 x[kElems], y[kElems];
+    uint32_t idx[kElems];
+    bool on[kElems];
+#pragma unroll
+    for (int r = 0; r < kElems; r++) {
+      const uint32_t u = u0 + r * blockDim.x;
+      const uint32_t i = ins0(u, t);
+      idx[r] = i;
+      on[r] = u < half && (i & cm) == cm;
+      if (on[r]) {
+        x[r] = s[i];
+        y[r] = s[i | bit];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kElems; r++) {
+      if (!on[r]) continue;
+      rot2(op.m, x[r], y[r]);
+      s[idx[r]] = x[r];
+      s[idx[r] | bit] = y[r];
+    }
   }
 }
 
 __device__ void swap2(float2 *s, uint32_t dim, const BOp &op) {
   const uint32_t a = op.w[0], b = op.w[1], lo = a < b ? a : b, hi = a < b ? b : a;
-  for (uint32_t u = threadIdx.x; u < dim / 4; u += blockDim.x) {
-    const uint32_t base = ins0(ins0(u, lo), hi);
-    const uint32_t i = base | (1u << a), j = base | (1u << b);
-    const float2 x = s[i];
-    s[i] = s[j];
-    s[j] = x;
+  const uint32_t quarter = dim / 4;
+  for (uint32_t u0 = threadIdx.x; u0 < quarter; u0 += kElems * blockDim.x) {
+    float2 x[kElems], y[kElems];
+    uint32_t ia[kElems], ib[kElems];
+#pragma unroll
+    for (int r = 0; r < kElems; r++) {
+      const uint32_t u = u0 + r * blockDim.x;
+      if (u >= quarter) continue;
+      const uint32_t base = ins0(ins0(u, lo), hi);
+      ia[r] = base | (1u << a);
+      ib[r] = base | (1u << b);
+      x[r] = s[ia[r]];
+      y[r] = s[ib[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < kElems; r++) {
+      if (u0 + r * blockDim.x >= quarter) continue;
+      s[ia[r]] = y[r];
+      s[ib[r]] = x[r];
+    }
   }
 }
 
@@ -287,8 +342,19 @@ __device__ __forceinline__ float2 cmulf(float2 v, float2 e) {
 }
 
 __device__ void diag(float2 *s, uint32_t dim, const BOp &op, const BTerm *terms) {
-  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x)
-    s[i] = cmulf(s[i], phase_of(i, terms, op.a, op.b));
+  for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
+    float2 x[kElems];
+#pragma unroll
+    for (int r = 0; r < kElems; r++) {
+      const uint32_t i = i0 + r * blockDim.x;
+      if (i < dim) x[r] = s[i];
+    }
+#pragma unroll
+    for (int r = 0; r < kElems; r++) {
+      const uint32_t i = i0 + r * blockDim.x;
+      if (i < dim) s[i] = cmulf(x[r], phase_of(i, terms, op.a, op.b));
+    }
+  }
 }
 
 // Overlaps Im<lam|Z_m|psi> for each Z-string mask of the run (in chunks of
@@ -303,50 +369,101 @@ __device__ void diagrun(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
       acc[r] = 0.0;
       zm[r] = c0 + r < op.d ? terms[op.c + c0 + r].mask : 0;
     }
-    for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) {
-      const float2 l = lam[i], p = psi[i];
-      const double w = double(l.x) * p.y - double(l.y) * p.x;  // Im(conj(l) p)
+    for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
+      float2 l[kElems], p[kElems];
 #pragma unroll
-      for (int r = 0; r < kOverlapChunk; r++) acc[r] += sgn(i, zm[r]) * w;
+      for (int e = 0; e < kElems; e++) {
+        const uint32_t i = i0 + e * blockDim.x;
+        if (i < dim) {
+          l[e] = lam[i];
+          p[e] = psi[i];
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < kElems; e++) {
+        const uint32_t i = i0 + e * blockDim.x;
+        if (i >= dim) continue;
+        const double w = double(l[e].x) * p[e].y - double(l[e].y) * p[e].x;  // Im(conj(l) p)
+#pragma unroll
+        for (int r = 0; r < kOverlapChunk; r++) acc[r] += sgn(i, zm[r]) * w;
+      }
     }
     block_sum<kOverlapChunk>(acc, red);
     if (threadIdx.x == 0)
       for (uint32_t r = 0; r < kOverlapChunk && c0 + r < op.d; r++)
         out[uint32_t(terms[op.c + c0 + r].x)] = acc[r];
   }
-  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) {
-    const float2 e = phase_of(i, terms, op.a, op.b);
-    psi[i] = cmulf(psi[i], e);
-    lam[i] = cmulf(lam[i], e);
+  for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
+    float2 l[kElems], p[kElems];
+#pragma unroll
+    for (int e = 0; e < kElems; e++) {
+      const uint32_t i = i0 + e * blockDim.x;
+      if (i < dim) {
+        l[e] = lam[i];
+        p[e] = psi[i];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < kElems; e++) {
+      const uint32_t i = i0 + e * blockDim.x;
+      if (i >= dim) continue;
+      const float2 ph = phase_of(i, terms, op.a, op.b);
+      psi[i] = cmulf(p[e], ph);
+      lam[i] = cmulf(l[e], ph);
+    }
   }
 }
 
 __device__ void accum(const float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
                       const BTerm *terms) {
   const uint32_t flip = uint32_t(op.mask);
-  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) {
-    const uint32_t j = i ^ flip;
-    double cr = 0.0, ci = 0.0;
-    for (uint32_t t = op.a; t < op.a + op.b; t++) {
-      const double sg = sgn(j, terms[t].mask);
-      cr += sg * terms[t].x;
-      ci += sg * terms[t].y;
+  for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
+    float2 y[kElems], l[kElems];
+#pragma unroll
+    for (int e = 0; e < kElems; e++) {
+      const uint32_t i = i0 + e * blockDim.x;
+      if (i >= dim) continue;
+      y[e] = psi[i ^ flip];
+      if (!(op.flags & F_ASSIGN)) l[e] = lam[i];
     }
-    const float2 y = psi[j];
-    float2 v = make_float2(float(cr * y.x - ci * y.y), float(cr * y.y + ci * y.x));
-    if (!(op.flags & F_ASSIGN)) {
-      const float2 l = lam[i];
-      v.x += l.x;
-      v.y += l.y;
+#pragma unroll
+    for (int e = 0; e < kElems; e++) {
+      const uint32_t i = i0 + e * blockDim.x;
+      if (i >= dim) continue;
+      const uint32_t j = i ^ flip;
+      double cr = 0.0, ci = 0.0;
+      for (uint32_t t = op.a; t < op.a + op.b; t++) {
+        const double sg = sgn(j, terms[t].mask);
+        cr += sg * terms[t].x;
+        ci += sg * terms[t].y;
+      }
+      float2 v = make_float2(float(cr * y[e].x - ci * y[e].y), float(cr * y[e].y + ci * y[e].x));
+      if (!(op.flags & F_ASSIGN)) {
+        v.x += l[e].x;
+        v.y += l[e].y;
+      }
+      lam[i] = v;
     }
-    lam[i] = v;
   }
 }
 
 __device__ void energy(const float2 *psi, const float2 *lam, uint32_t dim, const BOp &op,
                        double *out, double (*red)[kWaves]) {
   double v[2] = {0.0, 0.0};
-  for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) acc_cj(v[0], v[1], psi[i], lam[i], 1.0);
+  for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
+    float2 p[kElems], l[kElems];
+#pragma unroll
+    for (int e = 0; e < kElems; e++) {
+      const uint32_t i = i0 + e * blockDim.x;
+      if (i < dim) {
+        p[e] = psi[i];
+        l[e] = lam[i];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < kElems; e++)
+      if (i0 + e * blockDim.x < dim) acc_cj(v[0], v[1], p[e], l[e], 1.0);
+  }
   block_sum<2>(v, red);
   if (threadIdx.x == 0) out[op.a] = v[0];
 }
