@@ -91,6 +91,31 @@ def test_mixed_streams_against_oracle():
             assert dm <= TOL_SAME, (n, seed, dm)
 
 
+def test_mixed_streams_wide_states_every_fusion_setting():
+    """Every gate kind on any wires at 16 and 18 qubits, where the scheduler
+    reorders across disjoint wires, splits groups and uses every pass shape,
+    under every fusion setting: the schedule is validated on the host, the
+    amplitudes against the oracle and against one launch per plan op."""
+    fusion = {"fused": (1, 1, 1, 4, 0), "none": (0, 0, 0, 1, 0), "diag_only": (0, 1, 1, 1, 0),
+              "win_only": (1, 0, 0, 4, 0), "blocked8": (1, 1, 1, 4, 8)}
+    for n in (16, 18):
+        for seed in range(2):
+            gates = mixed_stream(n, 120, 4000 * n + seed)
+            want = O.run(n, gates)
+            c = C.to_circuit(n, gates)
+            for opt, f in fusion.items():
+                options = FusionOptions(*f)
+                kinds = shapes(c, options)
+                got, norm, launches = run(n, gates, options=options)
+                ref, _, _ = run(n, gates, merge=False, options=options)
+                d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
+                print(f"mixed n={n} seed={seed} {opt}: {launches} launches "
+                      f"{ {k: kinds.count(k) for k in set(kinds)} } diff {d:.2e} vs op-by-op {dm:.2e}")
+                assert launches == len(kinds)
+                assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP, (n, seed, opt, d)
+                assert dm <= TOL_SAME, (n, seed, opt, dm)
+
+
 @pytest.mark.parametrize("name,n", [
     ("qft", 14), ("qft", 20), ("ghz", 20), ("qaoa", 16), ("tfim", 14), ("grover", 12),
     ("hea", 14), ("brickwork", 16), ("random", 12)])
