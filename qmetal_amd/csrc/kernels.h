@@ -189,20 +189,21 @@ struct WideArgs {
   uint32_t nlane;
 };
 
-// Low chain pass: S chain steps on distinct wires 1..11, then optionally one
-// on wire 0, over workgroup tiles of 2^12 contiguous amplitudes (bits 0..11)
-// held by two waves. The tile is loaded and stored coalesced (arrangement 0:
-// lanes on bits 1..6, the wave bit on 7, register bits on 8..11) and permuted
-// through LDS into arrangement 1 + a for steps 4a .. 4a + 3, whose wires are
-// register bits (tile-local 1..4); the wire-0 step runs in the last
-// arrangement. `lanepos` gives each arrangement's lane bits (0..5) and wave
-// bit (6), `regpos` its register bits. Steps are folded against their
-// arrangement; base-index terms as in ChainArgs (k = step).
+// Low chain pass: S chain steps on distinct wires 1..10+WB, then optionally
+// one on wire 0, over workgroup tiles of 2^(11+WB) contiguous amplitudes
+// (bits 0..10+WB) held by 2^WB waves (WB = 1 or 2). The tile is loaded and
+// stored coalesced (arrangement 0: lanes on bits 1..6, the wave bits on 7..,
+// the register bits on the four above them) and permuted through LDS into
+// arrangement 1 + a for steps 4a .. 4a + 3, whose wires are register bits
+// (tile-local 1..4); the wire-0 step runs in the last arrangement. `lanepos`
+// gives each arrangement's lane bits (0..5) and wave bits (6..), `regpos` its
+// register bits. Steps are folded against their arrangement; base-index
+// terms as in ChainArgs (k = step).
 constexpr int kLowMaxArr = 4;
 struct LowArgs {
   float2 *a;
-  Layout lay;  // tiles: ins = bits 0..11 and the fixed bits
-  uint8_t lanepos[kLowMaxArr][7];
+  Layout lay;  // tiles: ins = the tile bits
+  uint8_t lanepos[kLowMaxArr][8];
   uint8_t regpos[kLowMaxArr][4];
   const ChainStep *steps;
   const DiagTerm *uni;
@@ -270,8 +271,8 @@ void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const Chai
 // multiples folded into the last step's matrix.
 // steps = 6 or 7.
 void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w);
-// s in 5..11 steps on wires 1..11, `zero`: a last step on wire 0.
-void launch_low(hipStream_t s, int steps, bool zero, int mode, const LowArgs &l);
+// s in 5..12 steps on wires 1..10+wb, `zero`: a last step on wire 0.
+void launch_low(hipStream_t s, int steps, bool zero, int mode, int wb, const LowArgs &l);
 void launch_tile(hipStream_t s, const TileArgs &t);
 // wb = 2 or 3 wave bits.
 void launch_gwide(hipStream_t s, int wb, const GWideArgs &g);

@@ -818,7 +818,7 @@ __global__ __launch_bounds__(64 << WB) void k_wide(WideArgs p) {
   }
 }
 
-// LDS slot of tile index i (float4 units, 11 bits): the low four bits are
+// LDS slot of tile index i (float4 units, up to 12 bits): the low four bits are
 // xored with the higher ones so that lanes stepping through high bits still
 // spread over the banks (a bijection: the high bits are kept; linear over
 // GF(2)).
@@ -826,13 +826,16 @@ __device__ __forceinline__ uint32_t low_swz(uint32_t i) { return tile_swizzle(i)
 
 // Tile index (float4 units: tile bit p is index bit p - 1) of this thread's
 // lane and wave bits in arrangement `a`.
+template <int WB>
 __device__ __forceinline__ uint32_t low_thread_index(const LowArgs &p, int a, uint32_t lane,
                                                      uint32_t wave) {
   uint32_t i = 0;
 #pragma unroll
   for (int j = 0; j < 6; j++)
     if ((lane >> j) & 1) i |= 1u << (p.lanepos[a][j] - 1);
-  if (wave) i |= 1u << (p.lanepos[a][6] - 1);
+#pragma unroll
+  for (int k = 0; k < WB; k++)
+    if ((wave >> k) & 1) i |= 1u << (p.lanepos[a][6 + k] - 1);
   return i;
 }
 
@@ -842,11 +845,11 @@ __device__ __forceinline__ uint32_t low_thread_index(const LowArgs &p, int a, ui
 // (workgroup-uniform). Byte offsets are kept opaque so that the 32 slot
 // addresses are formed here (one xor each) instead of being hoisted out of
 // the tile loop into registers.
-template <int FROM, int TO>
+template <int FROM, int TO, int WB>
 __device__ __forceinline__ void low_permute(const LowArgs &p, float4 *lds, float2 (&x)[32],
                                             uint32_t lane, uint32_t wave) {
-  uint32_t lf = low_swz(low_thread_index(p, FROM, lane, wave)) << 4;
-  uint32_t lt = low_swz(low_thread_index(p, TO, lane, wave)) << 4;
+  uint32_t lf = low_swz(low_thread_index<WB>(p, FROM, lane, wave)) << 4;
+  uint32_t lt = low_swz(low_thread_index<WB>(p, TO, lane, wave)) << 4;
   uint32_t cf[4], ct[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -880,29 +883,30 @@ __device__ __forceinline__ void low_permute(const LowArgs &p, float4 *lds, float
 // so that every step index is a compile-time constant however long the
 // chain): step i runs in arrangement 1 + i / 4 on register bit 1 + i % 4,
 // the wire-0 step (i == S) on bit 0 of the last arrangement.
-template <int I, int S, int T, int MODE>
+template <int I, int S, int T, int MODE, int WB>
 __device__ __forceinline__ void low_steps(const LowArgs &p, float4 *lds, float2 (&x)[32],
                                           const float2 (&e)[T], const_ptr<ChainStep> steps,
                                           uint32_t lane, uint32_t wave) {
   if constexpr (I < T) {
-    if constexpr (I == 4 && S > 4) low_permute<1, 2>(p, lds, x, lane, wave);
-    if constexpr (I == 8 && S > 8) low_permute<2, 3>(p, lds, x, lane, wave);
+    if constexpr (I == 4 && S > 4) low_permute<1, 2, WB>(p, lds, x, lane, wave);
+    if constexpr (I == 8 && S > 8) low_permute<2, 3, WB>(p, lds, x, lane, wave);
     const ChainStep __attribute__((address_space(4))) &st = steps[opaque_step(I, x[0].x)];
     constexpr int M = (MODE == 2 && I == T - 1) ? 1 : MODE;
     if constexpr (I >= S) chain_step<5, 0, M>(x, st, e[I]);
     else chain_step<5, 1 + I % 4, M>(x, st, e[I]);
-    low_steps<I + 1, S, T, MODE>(p, lds, x, e, steps, lane, wave);
+    low_steps<I + 1, S, T, MODE, WB>(p, lds, x, e, steps, lane, wave);
   }
 }
 
-// Low chain pass (LowArgs): two waves per workgroup tile, 32 KB of LDS. The
-// next tile's loads are issued before the current tile is worked on.
-template <int S, bool Z, int MODE>
-__global__ __launch_bounds__(128) void k_low(LowArgs p) {
+// Low chain pass (LowArgs): 2^WB waves per workgroup tile of 2^(11+WB)
+// amplitudes, 2^(15+WB) bytes of LDS. The next tile's loads are issued
+// before the current tile is worked on.
+template <int S, bool Z, int MODE, int WB>
+__global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
   constexpr int T = S + (Z ? 1 : 0);
   constexpr int A = (S + 3) / 4;  // step arrangements
   static_assert(S >= 1 && A + 1 <= kLowMaxArr, "low chain shape");
-  __shared__ float4 lds[2048];
+  __shared__ float4 lds[1024 << WB];
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const const_ptr<ChainStep> steps = as_const(p.steps);
   const const_ptr<DiagTerm> uni = as_const(p.uni);
@@ -917,7 +921,10 @@ __global__ __launch_bounds__(128) void k_low(LowArgs p) {
       const uint64_t mask = lane_terms[k].mask;
       const uint32_t st = lane_terms[k].k;
       const int a = 1 + (st < uint32_t(S) ? int(st) / 4 : A - 1);
-      uint64_t bs = wave ? 1ull << p.lanepos[a][6] : 0ull;
+      uint64_t bs = 0;
+#pragma unroll
+      for (int v = 0; v < WB; v++)
+        if ((wave >> v) & 1) bs |= 1ull << p.lanepos[a][6 + v];
 #pragma unroll
       for (int j = 0; j < 6; j++)
         if ((lane >> j) & 1) bs |= 1ull << p.lanepos[a][j];
@@ -938,14 +945,16 @@ __global__ __launch_bounds__(128) void k_low(LowArgs p) {
       e[i] = make_float2(cs, sn);
     }
   }
-  // Arrangement 0: lanes on bits 1..6, the wave bit on 7, registers 8..11.
+  // Arrangement 0: lanes on bits 1..6, the wave bits on 7.., the registers
+  // on the four bits above them.
   const uint32_t lane_bytes = lane * 16u;
   const uint64_t wave_off = uint64_t(wave) << 7;
   float4 y[16];
   if (blockIdx.x < p.lay.nthreads) {
     const uint64_t ub0 = scatter(blockIdx.x, p.lay) | p.lay.fixed | wave_off;
 #pragma unroll
-    for (int r = 0; r < 16; r++) y[r] = ld4<true>(pair_ptr(p.a, ub0 | (uint64_t(r) << 8), lane_bytes));
+    for (int r = 0; r < 16; r++)
+      y[r] = ld4<true>(pair_ptr(p.a, ub0 | (uint64_t(r) << (7 + WB)), lane_bytes));
   }
   for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
     const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
@@ -958,7 +967,8 @@ __global__ __launch_bounds__(128) void k_low(LowArgs p) {
     if (tile + gridDim.x < p.lay.nthreads) {
       const uint64_t ubn = scatter(tile + gridDim.x, p.lay) | p.lay.fixed | wave_off;
 #pragma unroll
-      for (int r = 0; r < 16; r++) y[r] = ld4<true>(pair_ptr(p.a, ubn | (uint64_t(r) << 8), lane_bytes));
+      for (int r = 0; r < 16; r++)
+        y[r] = ld4<true>(pair_ptr(p.a, ubn | (uint64_t(r) << (7 + WB)), lane_bytes));
     }
     if (p.nuni) {
       // Workgroup-uniform terms, one per lane (branch-free: lanes past the
@@ -984,12 +994,12 @@ __global__ __launch_bounds__(128) void k_low(LowArgs p) {
         e[i] = make_float2(cs, sn);
       }
     }
-    low_permute<0, 1>(p, lds, x, lane, wave);
-    low_steps<0, S, T, MODE>(p, lds, x, e, steps, lane, wave);
-    low_permute<A, 0>(p, lds, x, lane, wave);
+    low_permute<0, 1, WB>(p, lds, x, lane, wave);
+    low_steps<0, S, T, MODE, WB>(p, lds, x, e, steps, lane, wave);
+    low_permute<A, 0, WB>(p, lds, x, lane, wave);
 #pragma unroll
     for (int r = 0; r < 16; r++)
-      st4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << 8), lane_bytes),
+      st4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << (7 + WB)), lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
   }
 }
@@ -1745,29 +1755,37 @@ void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w) {
 // enough that the one spare prefetch per workgroup stays negligible.
 constexpr uint64_t kLowGrid = 40960;
 
-template <int S>
+template <int S, int WB>
 void launch_low_s(hipStream_t s, bool zero, int mode, const LowArgs &l) {
-  const unsigned grid = unsigned(std::min<uint64_t>(l.lay.nthreads, kLowGrid));
+  const unsigned grid = unsigned(std::min<uint64_t>(l.lay.nthreads, kLowGrid >> (WB - 1)));
+  constexpr int threads = 64 << WB;
   if (zero) {
-    if (mode == 2) k_low<S, true, 2><<<grid, 128, 0, s>>>(l);
-    else if (mode == 1) k_low<S, true, 1><<<grid, 128, 0, s>>>(l);
-    else k_low<S, true, 0><<<grid, 128, 0, s>>>(l);
+    if (mode == 2) k_low<S, true, 2, WB><<<grid, threads, 0, s>>>(l);
+    else if (mode == 1) k_low<S, true, 1, WB><<<grid, threads, 0, s>>>(l);
+    else k_low<S, true, 0, WB><<<grid, threads, 0, s>>>(l);
   } else {
-    if (mode == 2) k_low<S, false, 2><<<grid, 128, 0, s>>>(l);
-    else if (mode == 1) k_low<S, false, 1><<<grid, 128, 0, s>>>(l);
-    else k_low<S, false, 0><<<grid, 128, 0, s>>>(l);
+    if (mode == 2) k_low<S, false, 2, WB><<<grid, threads, 0, s>>>(l);
+    else if (mode == 1) k_low<S, false, 1, WB><<<grid, threads, 0, s>>>(l);
+    else k_low<S, false, 0, WB><<<grid, threads, 0, s>>>(l);
   }
 }
 
-void launch_low(hipStream_t s, int steps, bool zero, int mode, const LowArgs &l) {
+template <int S>
+void launch_low_w(hipStream_t s, bool zero, int mode, int wb, const LowArgs &l) {
+  if (wb == 2) launch_low_s<S, 2>(s, zero, mode, l);
+  else launch_low_s<S, 1>(s, zero, mode, l);
+}
+
+void launch_low(hipStream_t s, int steps, bool zero, int mode, int wb, const LowArgs &l) {
   switch (steps) {
-    case 5: launch_low_s<5>(s, zero, mode, l); break;
-    case 6: launch_low_s<6>(s, zero, mode, l); break;
-    case 7: launch_low_s<7>(s, zero, mode, l); break;
-    case 8: launch_low_s<8>(s, zero, mode, l); break;
-    case 9: launch_low_s<9>(s, zero, mode, l); break;
-    case 10: launch_low_s<10>(s, zero, mode, l); break;
-    case 11: launch_low_s<11>(s, zero, mode, l); break;
+    case 5: launch_low_w<5>(s, zero, mode, wb, l); break;
+    case 6: launch_low_w<6>(s, zero, mode, wb, l); break;
+    case 7: launch_low_w<7>(s, zero, mode, wb, l); break;
+    case 8: launch_low_w<8>(s, zero, mode, wb, l); break;
+    case 9: launch_low_w<9>(s, zero, mode, wb, l); break;
+    case 10: launch_low_w<10>(s, zero, mode, wb, l); break;
+    case 11: launch_low_w<11>(s, zero, mode, wb, l); break;
+    case 12: launch_low_w<12>(s, zero, mode, wb, l); break;
     default: break;
   }
 }

@@ -140,8 +140,8 @@ uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) 
 uint32_t low_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires) {
   wires.clear();
   if (p.n < kLowMinQubits) return 0;
-  uint32_t used = chain_steps(
-      p, k, kLowMaxWire, [](uint32_t w) { return w >= 1 && w <= kLowMaxWire; }, wires);
+  const uint32_t top = p.n > kLowMaxWire ? kLowMaxWire : kLowTwoWaveWire;
+  uint32_t used = chain_steps(p, k, top, [top](uint32_t w) { return w >= 1 && w <= top; }, wires);
   if (wires.size() < kLowMinSteps) {
     wires.clear();
     return 0;
@@ -494,9 +494,15 @@ std::vector<Pass> schedule_passes(const Plan &p) {
   for (uint32_t k = 0; k < p.ops.size(); k++) {
     const PlanOp &op = p.ops[k];
     if (op.kind == OpKind::Diagonal && op.masks.empty()) continue;  // no dispatch
+    // Where both shapes start, the one that takes more plan ops.
+    std::vector<uint32_t> low_steps;
     uint32_t used = wide_chain_at(p, k, steps);
-    const bool is_wide = used != 0;
-    if (!used) used = low_chain_at(p, k, steps);
+    const uint32_t low_used = low_chain_at(p, k, low_steps);
+    const bool is_wide = used != 0 && used >= low_used;
+    if (!is_wide) {
+      used = low_used;
+      steps.swap(low_steps);
+    }
     if (!used) continue;
     dag_passes(p, run0, k, out);
     Pass s;

@@ -64,9 +64,12 @@ uint32_t wide_chain_at(const Plan &p, uint32_t k, std::vector<uint32_t> &wires);
 
 // Low passes: kLowMinSteps to kLowMaxWire chain steps (as for wide passes)
 // on distinct wires 1..kLowMaxWire, optionally followed by one on wire 0,
-// over tiles of 2^(kLowMaxWire+1) contiguous amplitudes that two waves
-// permute through LDS. States below kLowMinQubits use ordinary passes.
-constexpr uint32_t kLowMaxWire = 11;
+// over tiles of contiguous amplitudes that the waves of a workgroup permute
+// through LDS: 2^12 amplitudes (two waves) when the wires stay within
+// 1..kLowTwoWaveWire, else 2^13 (four waves; states of 13 qubits or more).
+// States below kLowMinQubits use ordinary passes.
+constexpr uint32_t kLowMaxWire = 12;
+constexpr uint32_t kLowTwoWaveWire = 11;
 constexpr uint32_t kLowMinSteps = 5;
 constexpr uint32_t kLowMinQubits = 12;
 

@@ -383,6 +383,7 @@ struct Simulator::Impl {
     WideArgs w{};
     // Low chain pass (steps on wires 1..11 and 0 over two-wave tiles).
     bool low = false;
+    int low_wb = 1;  // low pass wave bits
     LowArgs l{};
     int mode = 0;  // wide / low step mode (launch_wide)
     // Generic low tile pass (segments of merged-pass ops).
@@ -847,28 +848,33 @@ struct Simulator::Impl {
     pl.bytes = pass_bytes(n, 0);
   }
 
-  // Low chain pass (LowArgs): arrangement 0 is the coalesced one (lanes on
-  // bits 1..6, the wave bit on 7, registers on 8..11); arrangement 1 + a
-  // holds steps 4a..4a+3 on register bits, filled up with the lowest unused
-  // wires, and lanes then the wave bit on the remaining wires of 1..11,
-  // ascending. The wire-0 step runs in the last.
+  // Low chain pass (LowArgs): WB wave bits (1 when the wires stay within
+  // 1..11, else 2), tile bits 0..10+WB. Arrangement 0 is the coalesced one
+  // (lanes on bits 1..6, the wave bits on 7.., registers on the four above);
+  // arrangement 1 + a holds steps 4a..4a+3 on register bits, filled up with
+  // the lowest unused wires, and lanes then the wave bits on the remaining
+  // tile wires, ascending. The wire-0 step runs in the last.
   void prep_low(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
     const std::vector<uint32_t> &w = ps.reg;
     const bool zero = !w.empty() && w.back() == 0;
     const int S = int(w.size()) - (zero ? 1 : 0);
     if (S < int(kLowMinSteps) || S > int(kLowMaxWire) || ps.fixed || n < kLowMinQubits)
       fail("internal: low pass shape");
+    const uint32_t wmax = *std::max_element(w.begin(), w.end());
+    const int WB = wmax <= kLowTwoWaveWire ? 1 : 2;
+    const uint32_t top = uint32_t(10 + WB);  // highest tile wire
+    if (wmax > kLowMaxWire || top >= n) fail("internal: low pass tile");
     const int A = (S + 3) / 4;
     LowArgs &la = pl.l;
     la = LowArgs{};
-    for (int j = 0; j < 7; j++) la.lanepos[0][j] = uint8_t(1 + j);
-    for (int k = 0; k < 4; k++) la.regpos[0][k] = uint8_t(8 + k);
+    for (int j = 0; j < 6 + WB; j++) la.lanepos[0][j] = uint8_t(1 + j);
+    for (int k = 0; k < 4; k++) la.regpos[0][k] = uint8_t(7 + WB + k);
     uint64_t var[kLowMaxArr] = {};
     for (int a = 0; a < A; a++) {
       uint32_t used = 1u;
       int nr = 0, nl = 0;
       for (int i = 4 * a; i < std::min(S, 4 * a + 4); i++) {
-        if (w[i] < 1 || w[i] > kLowMaxWire || ((used >> w[i]) & 1)) fail("internal: low pass wires");
+        if (w[i] < 1 || w[i] > top || ((used >> w[i]) & 1)) fail("internal: low pass wires");
         la.regpos[1 + a][nr++] = uint8_t(w[i]);
         used |= 1u << w[i];
       }
@@ -877,16 +883,16 @@ struct Simulator::Impl {
           la.regpos[1 + a][nr++] = uint8_t(q);
           used |= 1u << q;
         }
-      for (uint32_t q = 1; q <= kLowMaxWire; q++)
+      for (uint32_t q = 1; q <= top; q++)
         if (!((used >> q) & 1)) {
           la.lanepos[1 + a][nl++] = uint8_t(q);
           var[1 + a] |= 1ull << q;
         }
-      if (nl != 7) fail("internal: low pass lanes");
+      if (nl != 6 + WB) fail("internal: low pass lanes");
     }
     Layout &L = la.lay;
     L = Layout{};
-    for (uint32_t q = 0; q <= kLowMaxWire; q++) {
+    for (uint32_t q = 0; q <= top; q++) {
       L.ins[L.nins] = uint8_t(q);
       L.hm[L.nins] = ~((uint64_t(1) << q) - 1);
       L.nins++;
@@ -937,6 +943,7 @@ struct Simulator::Impl {
     la.nuni = uint32_t(uni.size());
     la.nlane = uint32_t(lan.size());
     pl.low = true;
+    pl.low_wb = WB;
     pl.chain_k = S;
     pl.chain_zero = zero;
     pl.mode = steps_mode(cs, T);
@@ -1094,7 +1101,7 @@ struct Simulator::Impl {
       check(hipEventRecord(a, stream), "hipEventRecord");
     }
     if (pl.wide) launch_wide(stream, pl.chain_k, pl.mode, pl.w);
-    else if (pl.low) launch_low(stream, pl.chain_k, pl.chain_zero, pl.mode, pl.l);
+    else if (pl.low) launch_low(stream, pl.chain_k, pl.chain_zero, pl.mode, pl.low_wb, pl.l);
     else if (pl.tile) launch_tile(stream, pl.t);
     else if (pl.gwide) launch_gwide(stream, pl.chain_k - 4, pl.g);
     else if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);

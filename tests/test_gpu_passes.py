@@ -308,18 +308,19 @@ def test_wide_passes_against_oracle_and_op_by_op(steps):
             assert dm <= TOL_SAME
 
 
-def low_stream(n, rounds, seed, kind):
-    """Rounds of 5..11 chain steps on distinct wires 1..11, some closed by a
-    step on wire 0, each step a 2x2 (step_gate) followed by
+def low_stream(n, rounds, seed, kind, top=11):
+    """Rounds of 5..top chain steps on distinct wires 1..top, some closed by
+    a step on wire 0, each step a 2x2 (step_gate) followed by
     phase gates that all hold its wire (onto register, lane, wave and
-    workgroup-uniform wires); a CX separates the rounds."""
+    workgroup-uniform wires); a CX separates the rounds. With top = 12 every
+    other round holds a step on wire 12 (four-wave tiles)."""
     r = C.SplitMix64(seed)
     gates = []
     for i in range(rounds):
-        count = 5 + r.next_u64() % 7
-        ws = []
+        count = 5 + r.next_u64() % (top - 4)
+        ws = [12] if top == 12 and i % 2 == 1 else []
         while len(ws) < count:
-            q = 1 + r.next_u64() % 11
+            q = 1 + r.next_u64() % top
             if q not in ws:
                 ws.append(q)
         if i % 2 == 0:
@@ -342,22 +343,29 @@ def low_stream(n, rounds, seed, kind):
 
 
 def test_low_passes_against_oracle_and_op_by_op():
-    """Chains of 5..11 steps on wires 1..11 (and 0) over two-wave tiles
-    permuted through LDS, real and complex, with phase terms on every kind
-    of bit."""
-    for n in (12, 14, 17):
+    """Chains of 5..11 steps on wires 1..11 (and 0) over two-wave tiles, and
+    of 5..12 steps on wires 1..12 over four-wave tiles (states of 13 qubits
+    or more), permuted through LDS, real and complex, with phase terms on
+    every kind of bit."""
+    for n, top in ((12, 11), (14, 11), (17, 11), (13, 12), (14, 12), (17, 12)):
         for seed, kind in STEP_KINDS:
             prep = C.product_state_prep(n, seed=seed)
-            gates = low_stream(n, 4, 300 * n + seed, kind)
+            gates = low_stream(n, 4, 300 * n + 7 * top + seed, kind, top)
             c = C.to_circuit(n, gates)
             plan = c.plan()
-            low = [sh for sh in shapes(c) if sh == "chain"]
+            sched = c.schedule()
+            low = [e for e, sh in zip(sched, shapes(c)) if sh == "chain"]
             assert len(low) >= 3, "low passes not formed"
+            if top == 12:
+                four = [e for e in low if any(plan[k]["kind"] == "Dense1q"
+                                              and plan[k]["qubits"][0] == 12 for k, _ in e)]
+                assert len(four) >= 2, "four-wave low passes not formed"
             want = O.run(n, prep + gates)
             got, norm, launches = run(n, gates, prep=prep)
             ref, _, _ = run(n, gates, merge=False, prep=prep)
             d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
-            print(f"low n={n} seed={seed} {kind}: {len(low)} low passes, {launches} launches, "
+            print(f"low n={n} top={top} seed={seed} {kind}: {len(low)} low passes, "
+                  f"{launches} launches, "
                   f"diff {d:.2e}, vs op-by-op {dm:.2e}")
             assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
             assert dm <= TOL_SAME

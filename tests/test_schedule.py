@@ -66,8 +66,8 @@ def mergeable(op):
 def chain_pass(ops, n):
     """A chain pass: Dense1q steps on distinct wires, each followed at most by
     a phase layer whose masks all hold its wire; wide: six or seven steps on
-    wires >= 7 (n >= 16); low: 5..11 steps on wires 1..11, then at most one
-    on wire 0 (n >= 12)."""
+    wires >= 7 (n >= 16); low: 5..12 steps on wires 1..11 (n >= 12) or 1..12
+    (n >= 13), then at most one on wire 0."""
     steps, last = [], None
     for o in ops:
         if o["kind"] == "Dense1q":
@@ -84,7 +84,8 @@ def chain_pass(ops, n):
     if min(steps) >= 7 and len(steps) in (6, 7):
         return n >= 16
     body = steps[:-1] if steps[-1] == 0 else steps
-    return n >= 12 and 0 not in body and 5 <= len(body) <= 11 and max(body) <= 11
+    top = 12 if n >= 13 else 11
+    return n >= 12 and 0 not in body and 5 <= len(body) <= top and max(body) <= top
 
 
 def tile_pass(ws, n):
@@ -180,10 +181,11 @@ def check_schedule(circ, options=None):
 def test_qft_pass_counts():
     """QFT: H on wire j then the CP ladder onto j, for j = n-1 .. 0. From 16
     qubits seven (else six) H's and their ladders on wires >= 7 share a wide
-    pass; from wire 11 (or lower) down to 0 the rest share one low pass; a
-    wire left between them gets an ordinary pass."""
-    for n, want in ((12, 1), (13, 2), (16, 2), (20, 3), (26, 3), (30, 4), (32, 4), (33, 4),
-                    (34, 5)):
+    pass; from wire 12 (11 at 12 qubits, or lower) down to 0 the rest share
+    one low pass (the longer of the two shapes where both start); a wire
+    left between them gets an ordinary pass."""
+    for n, want in ((12, 1), (13, 1), (14, 2), (16, 2), (19, 2), (20, 2), (26, 3), (30, 4),
+                    (32, 4), (33, 4), (34, 4)):
         c = C.to_circuit(n, C.qft(n))
         assert c.plan_size(True) == 2 * n - 1
         assert check_schedule(c) == want, n
