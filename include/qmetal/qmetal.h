@@ -181,6 +181,15 @@ int qm_batch_set_config(qm_batch b, uint32_t threads, uint32_t per_cu,
                         uint32_t lds_bits);
 int qm_batch_run(qm_batch b, double *out_energy, double *out_grad,
                  double *out_ms);
+/* Instances of min_qubits or more (0 = none, the default; autotune picks it
+ * by timing the whole batch) run spread: their states live in HBM side by
+ * side and every program step is one launch over all of them, each instance
+ * split over many workgroups; the narrower ones stay in the persistent
+ * kernel. The same results to rounding either way (reductions in a fixed
+ * order, so replays are identical). */
+int qm_batch_set_spread(qm_batch b, uint32_t min_qubits);
+/* The current spread width (0 = none; 0 for a NULL handle). */
+uint32_t qm_batch_spread(qm_batch b);
 
 /* Device timing of plan-op kernels, measured with events on the simulator's
  * own stream around each launch. qm_read_timing synchronises, reports every

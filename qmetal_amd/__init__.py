@@ -163,6 +163,8 @@ _batch_prepare = _decl("qm_batch_prepare", c_int, c_void_p, c_size_t, POINTER(c_
 _batch_autotune = _decl("qm_batch_autotune", c_int, c_void_p, _u32p, _u32p, _u32p)
 _batch_set_config = _decl("qm_batch_set_config", c_int, c_void_p, c_uint32, c_uint32, c_uint32)
 _batch_run = _decl("qm_batch_run", c_int, c_void_p, _f64p, _f64p, _f64p)
+_batch_set_spread = _decl("qm_batch_set_spread", c_int, c_void_p, c_uint32)
+_batch_spread = _decl("qm_batch_spread", c_uint32, c_void_p)
 
 _plan_build = _decl("qm_plan_build", c_void_p, c_void_p, POINTER(FusionOptions))
 _plan_destroy = _decl("qm_plan_destroy", None, c_void_p)
@@ -527,10 +529,18 @@ class Batch:
     def autotune(self):
         t, p, b = c_uint32(), c_uint32(), c_uint32()
         _check(_batch_autotune(self._h, byref(t), byref(p), byref(b)))
-        return {"threads": t.value, "per_cu": p.value, "lds_bits": b.value}
+        return {"threads": t.value, "per_cu": p.value, "lds_bits": b.value,
+                "spread_from": int(_batch_spread(self._h))}
 
-    def set_config(self, threads: int, per_cu: int, lds_bits: int):
+    def set_config(self, threads: int, per_cu: int, lds_bits: int, spread_from=None):
         _check(_batch_set_config(self._h, threads, per_cu, lds_bits))
+        if spread_from is not None:
+            self.set_spread(spread_from)
+        return self
+
+    def set_spread(self, min_qubits: int):
+        """Instances of min_qubits or more run spread (0: none)."""
+        _check(_batch_set_spread(self._h, min_qubits))
         return self
 
     def run(self):

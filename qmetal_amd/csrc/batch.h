@@ -1,5 +1,5 @@
 // batch.h -- many small circuits' energies and adjoint gradients in one
-// persistent launch.
+// persistent launch (and, for wide instances, lockstep spread launches).
 //
 // Each instance is compiled on the host into a short program (forward plan,
 // co-state build, energy, backward sweep). A persistent kernel of a few
@@ -24,7 +24,15 @@ struct BatchConfig {
   uint32_t threads = 512;   // workgroup size (64..512)
   uint32_t per_cu = 1;      // persistent workgroups per CU
   uint32_t lds_bits = 12;   // instances with n <= lds_bits run in LDS
+  // Instances with n >= spread_bits (0: none) run spread: several
+  // workgroups per instance, one launch per program step for all of them,
+  // before the persistent kernel takes the rest. autotune picks it by timing
+  // the whole batch (none, or from the widest instance down by up to 2).
+  uint32_t spread_bits = 0;
 };
+
+// Narrowest spread width autotune tries (states wider than any LDS cut-off).
+constexpr uint32_t kSpreadMinQubits = 14;
 
 class Batch {
  public:
@@ -39,7 +47,10 @@ class Batch {
   // Pick the launch configuration by timing each candidate (twice) on the
   // prepared batch; returns the chosen one.
   BatchConfig autotune();
+  // threads, per_cu and lds_bits (spread_bits is kept).
   void set_config(const BatchConfig &c);
+  // Instances with n >= bits run spread (0: none).
+  void set_spread(uint32_t bits);
   BatchConfig config() const;
   // One replay of the captured launch; results copied back.
   void run(double *energies, double *grads);

@@ -134,19 +134,27 @@ __device__ __forceinline__ void acc_cj(double &re, double &im, float2 x, float2 
 
 // Every pass below handles several units per thread per iteration, loading
 // all of them before any arithmetic, so that a workgroup keeps enough loads
-// in flight to stream a state that lives in HBM (one workgroup per instance).
+// in flight to stream a state that lives in HBM. A pass covers chunk c of nc
+// equal chunks of its units: the whole state (c = 0, nc = 1) when one
+// workgroup runs the instance, a slice when several workgroups share it.
+
+struct Range {
+  uint32_t lo, hi;
+};
+__device__ __forceinline__ Range chunk_of(uint32_t units, uint32_t c, uint32_t nc) {
+  return Range{uint32_t(uint64_t(units) * c / nc), uint32_t(uint64_t(units) * (c + 1) / nc)};
+}
 
 template <int K>
-__device__ void group(float2 *s, uint32_t dim, const BOp &op, const float *mats) {
+__device__ void group(float2 *s, Range rg, const BOp &op, const float *mats) {
   constexpr int U = K >= 4 ? 1 : 16 >> K;  // units per thread per iteration
-  const uint32_t units = dim >> K;
-  for (uint32_t u0 = threadIdx.x; u0 < units; u0 += U * blockDim.x) {
+  for (uint32_t u0 = rg.lo + threadIdx.x; u0 < rg.hi; u0 += U * blockDim.x) {
     float2 x[U][1 << K];
     uint32_t idx[U][1 << K];
 #pragma unroll
     for (int r = 0; r < U; r++) {
       const uint32_t u = u0 + r * blockDim.x;
-      if (u >= units) continue;
+      if (u >= rg.hi) continue;
       uint32_t base = u;
 #pragma unroll
       for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
@@ -162,7 +170,7 @@ __device__ void group(float2 *s, uint32_t dim, const BOp &op, const float *mats)
     }
 #pragma unroll
     for (int r = 0; r < U; r++) {
-      if (u0 + r * blockDim.x >= units) continue;
+      if (u0 + r * blockDim.x >= rg.hi) continue;
 #pragma unroll
       for (int j = 0; j < K; j++) {
         const float *m = mats + op.a + 8 * j;
@@ -176,12 +184,14 @@ __device__ void group(float2 *s, uint32_t dim, const BOp &op, const float *mats)
   }
 }
 
-__device__ void group_any(float2 *s, uint32_t dim, const BOp &op, const float *mats) {
+__device__ void group_any(float2 *s, uint32_t dim, uint32_t c, uint32_t nc, const BOp &op,
+                          const float *mats) {
+  const Range rg = chunk_of(dim >> op.k, c, nc);
   switch (op.k) {
-    case 1: group<1>(s, dim, op, mats); break;
-    case 2: group<2>(s, dim, op, mats); break;
-    case 3: group<3>(s, dim, op, mats); break;
-    case 4: group<4>(s, dim, op, mats); break;
+    case 1: group<1>(s, rg, op, mats); break;
+    case 2: group<2>(s, rg, op, mats); break;
+    case 3: group<3>(s, rg, op, mats); break;
+    case 4: group<4>(s, rg, op, mats); break;
     default: break;
   }
 }
@@ -189,22 +199,23 @@ __device__ void group_any(float2 *s, uint32_t dim, const BOp &op, const float *m
 // Up to three one-qubit gates on distinct wires, on both states: for each
 // wire, the overlap <lam|P|psi> of its rotation (if any), then its inverse.
 // aux holds (axis, grad slot) per wire; axis 0/1/2 = X/Y/Z, kNoAxis = none.
+// With `part` the workgroup's sums go to part[j] (one chunk's share) instead
+// of the gradient.
 template <int K>
-__device__ void rotgroup(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
-                         const float *mats, const uint32_t *aux, double *out,
+__device__ void rotgroup(float2 *psi, float2 *lam, Range rg, const BOp &op, const float *mats,
+                         const uint32_t *aux, double *out, double *part,
                          double (*red)[kWaves]) {
   constexpr int U = 8 >> K;  // units per thread per iteration
   double acc[2 * K];
 #pragma unroll
   for (int i = 0; i < 2 * K; i++) acc[i] = 0.0;
-  const uint32_t units = dim >> K;
-  for (uint32_t u0 = threadIdx.x; u0 < units; u0 += U * blockDim.x) {
+  for (uint32_t u0 = rg.lo + threadIdx.x; u0 < rg.hi; u0 += U * blockDim.x) {
     float2 p[U][1 << K], l[U][1 << K];
     uint32_t idx[U][1 << K];
 #pragma unroll
     for (int r = 0; r < U; r++) {
       const uint32_t u = u0 + r * blockDim.x;
-      if (u >= units) continue;
+      if (u >= rg.hi) continue;
       uint32_t base = u;
 #pragma unroll
       for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
@@ -221,7 +232,7 @@ __device__ void rotgroup(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
     }
 #pragma unroll
     for (int r = 0; r < U; r++) {
-      if (u0 + r * blockDim.x >= units) continue;
+      if (u0 + r * blockDim.x >= rg.hi) continue;
 #pragma unroll
       for (int j = 0; j < K; j++) {
         const uint32_t axis = aux[op.c + 2 * j];
@@ -257,26 +268,29 @@ __device__ void rotgroup(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
     for (int j = 0; j < K; j++) {
       const uint32_t axis = aux[op.c + 2 * j];
       // dE/dt = Im(i^ny S): ny = 1 for Y.
-      if (axis != kNoAxis) out[aux[op.c + 2 * j + 1]] = axis == 1 ? acc[2 * j] : acc[2 * j + 1];
+      const double v = axis == 1 ? acc[2 * j] : acc[2 * j + 1];
+      if (part) part[j] = v;
+      else if (axis != kNoAxis) out[aux[op.c + 2 * j + 1]] = v;
     }
 }
 
-__device__ void rotgroup_any(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
-                             const float *mats, const uint32_t *aux, double *out,
-                             double (*red)[kWaves]) {
+__device__ void rotgroup_any(float2 *psi, float2 *lam, uint32_t dim, uint32_t c, uint32_t nc,
+                             const BOp &op, const float *mats, const uint32_t *aux, double *out,
+                             double *part, double (*red)[kWaves]) {
+  const Range rg = chunk_of(dim >> op.k, c, nc);
   switch (op.k) {
-    case 1: rotgroup<1>(psi, lam, dim, op, mats, aux, out, red); break;
-    case 2: rotgroup<2>(psi, lam, dim, op, mats, aux, out, red); break;
-    case 3: rotgroup<3>(psi, lam, dim, op, mats, aux, out, red); break;
+    case 1: rotgroup<1>(psi, lam, rg, op, mats, aux, out, part, red); break;
+    case 2: rotgroup<2>(psi, lam, rg, op, mats, aux, out, part, red); break;
+    case 3: rotgroup<3>(psi, lam, rg, op, mats, aux, out, part, red); break;
     default: break;
   }
 }
 
 constexpr int kElems = 8;  // elements per thread per iteration of the flat passes
 
-__device__ void ctrl(float2 *s, uint32_t dim, const BOp &op) {
-  const uint32_t t = op.w[0], bit = 1u << t, cm = uint32_t(op.mask), half = dim / 2;
-  for (uint32_t u0 = threadIdx.x; u0 < half; u0 += kElems * blockDim.x) {
+__device__ void ctrl(float2 *s, Range rg, const BOp &op) {
+  const uint32_t t = op.w[0], bit = 1u << t, cm = uint32_t(op.mask), half = rg.hi;
+  for (uint32_t u0 = rg.lo + threadIdx.x; u0 < half; u0 += kElems * blockDim.x) {
     float2 x[kElems], y[kElems];
     uint32_t idx[kElems];
     bool on[kElems];
@@ -301,10 +315,10 @@ __device__ void ctrl(float2 *s, uint32_t dim, const BOp &op) {
   }
 }
 
-__device__ void swap2(float2 *s, uint32_t dim, const BOp &op) {
+__device__ void swap2(float2 *s, Range rg, const BOp &op) {
   const uint32_t a = op.w[0], b = op.w[1], lo = a < b ? a : b, hi = a < b ? b : a;
-  const uint32_t quarter = dim / 4;
-  for (uint32_t u0 = threadIdx.x; u0 < quarter; u0 += kElems * blockDim.x) {
+  const uint32_t quarter = rg.hi;
+  for (uint32_t u0 = rg.lo + threadIdx.x; u0 < quarter; u0 += kElems * blockDim.x) {
     float2 x[kElems], y[kElems];
     uint32_t ia[kElems], ib[kElems];
 #pragma unroll
@@ -326,41 +340,148 @@ __device__ void swap2(float2 *s, uint32_t dim, const BOp &op) {
   }
 }
 
-__device__ __forceinline__ float2 phase_of(uint32_t i, const BTerm *terms, uint32_t a,
-                                           uint32_t b) {
-  double ph = 0.0;
-  for (uint32_t t = a; t < a + b; t++) {
-    const uint64_t m = terms[t].mask;
-    if ((uint64_t(i) & m) == m) ph += terms[t].x;
-  }
-  const float r = float(ph - rint(ph));
-  return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
-}
-
 __device__ __forceinline__ float2 cmulf(float2 v, float2 e) {
   return make_float2(v.x * e.x - v.y * e.y, v.x * e.y + v.y * e.x);
 }
 
-__device__ void diag(float2 *s, uint32_t dim, const BOp &op, const BTerm *terms) {
-  for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
-    float2 x[kElems];
-#pragma unroll
-    for (int r = 0; r < kElems; r++) {
-      const uint32_t i = i0 + r * blockDim.x;
-      if (i < dim) x[r] = s[i];
+// Phase of a diagonal op, sum_t angle_t [mask_t within i] (turns), over a
+// block of 2^13 aligned indices: the bits above the block are fixed, so every
+// term folds into one of four tables over the block's low bits (A = bits
+// 0..6, B = bits 7..12). Masks hold at most two bits (one- and two-qubit
+// diagonal gates), so a term with bits in both A and B has exactly one in B:
+//   phase(i) = c + a[i_A] + b[i_B] + sum over set bits r of i_B of x[r][i_A].
+// The tables come from the block's coefficients: the summed angle of every
+// low-bit mask of weight <= 2 over the terms whose high bits hold.
+constexpr int kTabA = 7, kTabB = 6;
+constexpr uint32_t kPhaseBlock = 1u << (kTabA + kTabB);
+constexpr int kBits = kTabA + kTabB;
+constexpr int kCoefs = 1 + kBits + kBits * (kBits - 1) / 2;  // masks of weight 0, 1, 2
+struct PhaseTab {
+  double a[1 << kTabA];
+  double b[1 << kTabB];
+  double x[kTabB][1 << kTabA];
+  double c;
+  double coef[kCoefs];  // [0] weight 0, [1 + q] bit q, then pairs q < r
+};
+
+// Coefficient slot of a low mask of weight <= 2 (pairs in row-major q < r).
+__host__ __device__ constexpr int pair_slot(int q, int r) {
+  return 1 + kBits + q * (2 * kBits - q - 1) / 2 + (r - q - 1);
+}
+
+// All threads: the tables of terms [ta, ta + nt) for the block holding index
+// `base`; ends with a barrier.
+__device__ void build_tab(PhaseTab &T, uint32_t base, const BTerm *terms, uint32_t ta,
+                          uint32_t nt) {
+  const uint64_t hi = base & ~(kPhaseBlock - 1u);
+  // One thread per coefficient, terms in order (deterministic sums).
+  for (int k = int(threadIdx.x); k < kCoefs; k += int(blockDim.x)) {
+    uint32_t want = 0;
+    if (k >= 1 && k <= kBits) {
+      want = 1u << (k - 1);
+    } else if (k > kBits) {
+      int q = 0, rem = k - 1 - kBits;  // inverse of pair_slot
+      while (rem >= kBits - 1 - q) {
+        rem -= kBits - 1 - q;
+        q++;
+      }
+      want = (1u << q) | (1u << (q + 1 + rem));
     }
-#pragma unroll
-    for (int r = 0; r < kElems; r++) {
-      const uint32_t i = i0 + r * blockDim.x;
-      if (i < dim) s[i] = cmulf(x[r], phase_of(i, terms, op.a, op.b));
+    double sum = 0.0;
+    for (uint32_t t = ta; t < ta + nt; t++) {
+      const uint64_t m = terms[t].mask;
+      const uint64_t mh = m & ~uint64_t(kPhaseBlock - 1u);
+      if ((hi & mh) == mh && uint32_t(m & (kPhaseBlock - 1u)) == want) sum += terms[t].x;
     }
+    T.coef[k] = sum;
+  }
+  __syncthreads();
+  // Entry e: 0..127 a, 128..191 b, 192..959 x, 960 c.
+  constexpr uint32_t kEntries = (1u << kTabA) + (1u << kTabB) + kTabB * (1u << kTabA) + 1;
+  for (uint32_t e = threadIdx.x; e < kEntries; e += blockDim.x) {
+    double sum = 0.0;
+    if (e < 128) {  // bits and pairs within A
+      for (int q = 0; q < kTabA; q++) {
+        if (!((e >> q) & 1)) continue;
+        sum += T.coef[1 + q];
+        for (int r = q + 1; r < kTabA; r++)
+          if ((e >> r) & 1) sum += T.coef[pair_slot(q, r)];
+      }
+      T.a[e] = sum;
+    } else if (e < 192) {  // bits and pairs within B
+      const uint32_t bv = e - 128;
+      for (int q = 0; q < kTabB; q++) {
+        if (!((bv >> q) & 1)) continue;
+        sum += T.coef[1 + kTabA + q];
+        for (int r = q + 1; r < kTabB; r++)
+          if ((bv >> r) & 1) sum += T.coef[pair_slot(kTabA + q, kTabA + r)];
+      }
+      T.b[bv] = sum;
+    } else if (e < 960) {  // pairs (q in A, bit r of B)
+      const uint32_t r = (e - 192) >> kTabA, av = (e - 192) & 127u;
+      for (int q = 0; q < kTabA; q++)
+        if ((av >> q) & 1) sum += T.coef[pair_slot(q, kTabA + int(r))];
+      T.x[r][av] = sum;
+    } else {
+      T.c = T.coef[0];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float2 phase_tab(const PhaseTab &T, uint32_t i) {
+  const uint32_t a = i & 127u, b = (i >> kTabA) & 63u;
+  double ph = T.c + T.a[a] + T.b[b];
+#pragma unroll
+  for (int r = 0; r < kTabB; r++)
+    if ((b >> r) & 1) ph += T.x[r][a];
+  const float f = float(ph - rint(ph));
+  return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
+}
+
+// Block by block over the range: the block's tables, then its elements.
+__device__ void diag(float2 *s, Range rg, const BOp &op, const BTerm *terms, PhaseTab &T) {
+  for (uint32_t blk = rg.lo & ~(kPhaseBlock - 1u); blk < rg.hi; blk += kPhaseBlock) {
+    build_tab(T, blk, terms, op.a, op.b);
+    const uint32_t lo = blk > rg.lo ? blk : rg.lo;
+    const uint32_t end = blk + kPhaseBlock < rg.hi ? blk + kPhaseBlock : rg.hi;
+    for (uint32_t i0 = lo + threadIdx.x; i0 < end; i0 += kElems * blockDim.x) {
+      float2 x[kElems];
+#pragma unroll
+      for (int r = 0; r < kElems; r++) {
+        const uint32_t i = i0 + r * blockDim.x;
+        if (i < end) x[r] = s[i];
+      }
+#pragma unroll
+      for (int r = 0; r < kElems; r++) {
+        const uint32_t i = i0 + r * blockDim.x;
+        if (i < end) s[i] = cmulf(x[r], phase_tab(T, i));
+      }
+    }
+    __syncthreads();  // the tables are rebuilt for the next block
   }
 }
 
-// Overlaps Im<lam|Z_m|psi> for each Z-string mask of the run (in chunks of
-// kOverlapChunk), then the run's inverse phase on both states.
-__device__ void diagrun(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
-                        const BTerm *terms, double *out, double (*red)[kWaves]) {
+// Overlap sums of the masks [c0, c0 + kOverlapChunk) of the run from one
+// thread's elements; block-summed into part[] or the gradient.
+__device__ __forceinline__ void overlap_out(double (&acc)[kOverlapChunk], const BOp &op,
+                                            const BTerm *terms, uint32_t c0, double *out,
+                                            double *part, double (*red)[kWaves]) {
+  block_sum<kOverlapChunk>(acc, red);
+  if (threadIdx.x == 0)
+    for (uint32_t r = 0; r < kOverlapChunk && c0 + r < op.d; r++) {
+      if (part) part[c0 + r] = acc[r];
+      else out[uint32_t(terms[op.c + c0 + r].x)] = acc[r];
+    }
+}
+
+// Overlaps Im<lam|Z_m|psi> for each Z-string mask of the run (one read pass
+// per kOverlapChunk masks), then the run's inverse phase on both states,
+// block by block. With `part` the workgroup's sums go to part[r] for overlap
+// r.
+__device__ void diagrun(float2 *psi, float2 *lam, Range rg, const BOp &op, const BTerm *terms,
+                        double *out, double *part, double (*red)[kWaves], PhaseTab &T) {
+  const uint32_t dim = rg.hi;
   for (uint32_t c0 = 0; c0 < op.d; c0 += kOverlapChunk) {
     double acc[kOverlapChunk];
     uint64_t zm[kOverlapChunk];
@@ -369,7 +490,7 @@ __device__ void diagrun(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
       acc[r] = 0.0;
       zm[r] = c0 + r < op.d ? terms[op.c + c0 + r].mask : 0;
     }
-    for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
+    for (uint32_t i0 = rg.lo + threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
       float2 l[kElems], p[kElems];
 #pragma unroll
       for (int e = 0; e < kElems; e++) {
@@ -388,36 +509,39 @@ __device__ void diagrun(float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
         for (int r = 0; r < kOverlapChunk; r++) acc[r] += sgn(i, zm[r]) * w;
       }
     }
-    block_sum<kOverlapChunk>(acc, red);
-    if (threadIdx.x == 0)
-      for (uint32_t r = 0; r < kOverlapChunk && c0 + r < op.d; r++)
-        out[uint32_t(terms[op.c + c0 + r].x)] = acc[r];
+    overlap_out(acc, op, terms, c0, out, part, red);
   }
-  for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
-    float2 l[kElems], p[kElems];
+  for (uint32_t blk = rg.lo & ~(kPhaseBlock - 1u); blk < rg.hi; blk += kPhaseBlock) {
+    build_tab(T, blk, terms, op.a, op.b);
+    const uint32_t lo = blk > rg.lo ? blk : rg.lo;
+    const uint32_t end = blk + kPhaseBlock < rg.hi ? blk + kPhaseBlock : rg.hi;
+    for (uint32_t i0 = lo + threadIdx.x; i0 < end; i0 += kElems * blockDim.x) {
+      float2 l[kElems], p[kElems];
 #pragma unroll
-    for (int e = 0; e < kElems; e++) {
-      const uint32_t i = i0 + e * blockDim.x;
-      if (i < dim) {
-        l[e] = lam[i];
-        p[e] = psi[i];
+      for (int e = 0; e < kElems; e++) {
+        const uint32_t i = i0 + e * blockDim.x;
+        if (i < end) {
+          l[e] = lam[i];
+          p[e] = psi[i];
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < kElems; e++) {
+        const uint32_t i = i0 + e * blockDim.x;
+        if (i >= end) continue;
+        const float2 ph = phase_tab(T, i);
+        psi[i] = cmulf(p[e], ph);
+        lam[i] = cmulf(l[e], ph);
       }
     }
-#pragma unroll
-    for (int e = 0; e < kElems; e++) {
-      const uint32_t i = i0 + e * blockDim.x;
-      if (i >= dim) continue;
-      const float2 ph = phase_of(i, terms, op.a, op.b);
-      psi[i] = cmulf(p[e], ph);
-      lam[i] = cmulf(l[e], ph);
-    }
+    __syncthreads();
   }
 }
 
-__device__ void accum(const float2 *psi, float2 *lam, uint32_t dim, const BOp &op,
+__device__ void accum(const float2 *psi, float2 *lam, Range rg, const BOp &op,
                       const BTerm *terms) {
-  const uint32_t flip = uint32_t(op.mask);
-  for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
+  const uint32_t flip = uint32_t(op.mask), dim = rg.hi;
+  for (uint32_t i0 = rg.lo + threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
     float2 y[kElems], l[kElems];
 #pragma unroll
     for (int e = 0; e < kElems; e++) {
@@ -447,10 +571,11 @@ __device__ void accum(const float2 *psi, float2 *lam, uint32_t dim, const BOp &o
   }
 }
 
-__device__ void energy(const float2 *psi, const float2 *lam, uint32_t dim, const BOp &op,
-                       double *out, double (*red)[kWaves]) {
+__device__ void energy(const float2 *psi, const float2 *lam, Range rg, const BOp &op,
+                       double *out, double *part, double (*red)[kWaves]) {
+  const uint32_t dim = rg.hi;
   double v[2] = {0.0, 0.0};
-  for (uint32_t i0 = threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
+  for (uint32_t i0 = rg.lo + threadIdx.x; i0 < dim; i0 += kElems * blockDim.x) {
     float2 p[kElems], l[kElems];
 #pragma unroll
     for (int e = 0; e < kElems; e++) {
@@ -465,13 +590,61 @@ __device__ void energy(const float2 *psi, const float2 *lam, uint32_t dim, const
       if (i0 + e * blockDim.x < dim) acc_cj(v[0], v[1], p[e], l[e], 1.0);
   }
   block_sum<2>(v, red);
-  if (threadIdx.x == 0) out[op.a] = v[0];
+  if (threadIdx.x == 0) {
+    if (part) part[0] = v[0];
+    else out[op.a] = v[0];
+  }
+}
+
+// One program op on chunk c of nc of the instance's states; `part` (spread
+// launches) takes the workgroup's reduction sums instead of the outputs.
+__device__ void run_op(const BatchArgs &A, const BOp &op, float2 *psi, float2 *lam, uint32_t dim,
+                       uint32_t c, uint32_t nc, double *part, double (*red)[kWaves],
+                       PhaseTab &T) {
+  switch (op.code) {
+    case B_INIT: {
+      const Range rg = chunk_of(dim, c, nc);
+      for (uint32_t i = rg.lo + threadIdx.x; i < rg.hi; i += blockDim.x)
+        psi[i] = make_float2(i == 0 ? 1.0f : 0.0f, 0.0f);
+      break;
+    }
+    case B_GROUP:
+      group_any(psi, dim, c, nc, op, A.mats);
+      if (op.flags & F_BOTH) group_any(lam, dim, c, nc, op, A.mats);
+      break;
+    case B_CTRL:
+      ctrl(psi, chunk_of(dim / 2, c, nc), op);
+      if (op.flags & F_BOTH) ctrl(lam, chunk_of(dim / 2, c, nc), op);
+      break;
+    case B_SWAP:
+      swap2(psi, chunk_of(dim / 4, c, nc), op);
+      if (op.flags & F_BOTH) swap2(lam, chunk_of(dim / 4, c, nc), op);
+      break;
+    case B_DIAG:
+      diag(psi, chunk_of(dim, c, nc), op, A.terms, T);
+      break;
+    case B_ACCUM:
+      accum(psi, lam, chunk_of(dim, c, nc), op, A.terms);
+      break;
+    case B_ENERGY:
+      energy(psi, lam, chunk_of(dim, c, nc), op, A.energy, part, red);
+      break;
+    case B_ROTGROUP:
+      rotgroup_any(psi, lam, dim, c, nc, op, A.mats, A.aux, A.grad, part, red);
+      break;
+    case B_DIAGRUN:
+      diagrun(psi, lam, chunk_of(dim, c, nc), op, A.terms, A.grad, part, red, T);
+      break;
+    default:
+      break;
+  }
 }
 
 __global__ __launch_bounds__(kMaxThreads) void k_batch(BatchArgs A) {
   extern __shared__ float2 lds[];
   __shared__ uint32_t s_idx;
   __shared__ double red[kOverlapChunk][kWaves];
+  __shared__ PhaseTab tab;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_idx = atomicAdd(A.counter, 1u);
@@ -489,43 +662,59 @@ __global__ __launch_bounds__(kMaxThreads) void k_batch(BatchArgs A) {
       lam = psi + A.slot;
     }
     for (uint32_t o = in.op_begin; o < in.op_end; o++) {
-      const BOp op = A.ops[o];
-      switch (op.code) {
-        case B_INIT:
-          for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x)
-            psi[i] = make_float2(i == 0 ? 1.0f : 0.0f, 0.0f);
-          break;
-        case B_GROUP:
-          group_any(psi, dim, op, A.mats);
-          if (op.flags & F_BOTH) group_any(lam, dim, op, A.mats);
-          break;
-        case B_CTRL:
-          ctrl(psi, dim, op);
-          if (op.flags & F_BOTH) ctrl(lam, dim, op);
-          break;
-        case B_SWAP:
-          swap2(psi, dim, op);
-          if (op.flags & F_BOTH) swap2(lam, dim, op);
-          break;
-        case B_DIAG:
-          diag(psi, dim, op, A.terms);
-          break;
-        case B_ACCUM:
-          accum(psi, lam, dim, op, A.terms);
-          break;
-        case B_ENERGY:
-          energy(psi, lam, dim, op, A.energy, red);
-          break;
-        case B_ROTGROUP:
-          rotgroup_any(psi, lam, dim, op, A.mats, A.aux, A.grad, red);
-          break;
-        case B_DIAGRUN:
-          diagrun(psi, lam, dim, op, A.terms, A.grad, red);
-          break;
-        default:
-          break;
-      }
+      run_op(A, A.ops[o], psi, lam, dim, 0, 1, nullptr, red, tab);
       __syncthreads();
+    }
+  }
+}
+
+// Spread instances (states in HBM, several workgroups each) run in lockstep:
+// one launch per program step, workgroup w taking chunk w - wg_base[l] of
+// local instance l = wg_inst[w]. Reductions leave one row of `slots` partial
+// sums per workgroup; k_spread_reduce adds each instance's rows in chunk
+// order (deterministic) into the energies and gradients.
+struct SpreadArgs {
+  const uint32_t *inst;     // local instance -> instance
+  const uint32_t *wg_base;  // local instance -> first workgroup (count + 1 entries)
+  const uint32_t *wg_inst;  // workgroup -> local instance
+  const uint64_t *state;    // local instance -> offset of psi in scratch (lam follows)
+  float2 *scratch;
+  double *part;
+  uint32_t slots;
+};
+
+__global__ __launch_bounds__(kMaxThreads) void k_spread(BatchArgs A, SpreadArgs S, uint32_t step) {
+  __shared__ double red[kOverlapChunk][kWaves];
+  __shared__ PhaseTab tab;
+  const uint32_t l = S.wg_inst[blockIdx.x];
+  const BInst in = A.insts[S.inst[l]];
+  if (in.op_begin + step >= in.op_end) return;
+  const uint32_t c = blockIdx.x - S.wg_base[l], nc = S.wg_base[l + 1] - S.wg_base[l];
+  const uint32_t dim = 1u << in.n;
+  float2 *psi = S.scratch + S.state[l];
+  run_op(A, A.ops[in.op_begin + step], psi, psi + dim, dim, c, nc,
+         S.part + uint64_t(blockIdx.x) * S.slots, red, tab);
+}
+
+__global__ __launch_bounds__(64) void k_spread_reduce(BatchArgs A, SpreadArgs S, uint32_t step) {
+  const uint32_t l = blockIdx.x;
+  const BInst in = A.insts[S.inst[l]];
+  if (in.op_begin + step >= in.op_end) return;
+  const BOp op = A.ops[in.op_begin + step];
+  const uint32_t w0 = S.wg_base[l], w1 = S.wg_base[l + 1];
+  uint32_t nslot = 0;
+  if (op.code == B_ROTGROUP) nslot = op.k;
+  else if (op.code == B_DIAGRUN) nslot = op.d;
+  else if (op.code == B_ENERGY) nslot = 1;
+  for (uint32_t r = threadIdx.x; r < nslot; r += blockDim.x) {
+    double v = 0.0;
+    for (uint32_t w = w0; w < w1; w++) v += S.part[uint64_t(w) * S.slots + r];
+    if (op.code == B_ENERGY) {
+      A.energy[op.a] = v;
+    } else if (op.code == B_DIAGRUN) {
+      A.grad[uint32_t(A.terms[op.c + r].x)] = v;
+    } else if (A.aux[op.c + 2 * r] != kNoAxis) {
+      A.grad[A.aux[op.c + 2 * r + 1]] = v;
     }
   }
 }
@@ -758,6 +947,7 @@ void compile(const Circuit &c, const Hamiltonian &h, uint32_t energy_slot, uint3
   }
   for (const Term &t : h)
     if (!t.p.valid(c.n)) fail("gradient: invalid Pauli term");
+  const size_t op0 = pg.ops.size();
   pg.add(B_INIT);
   for (const PlanOp &p : build_plan(c, FuseOpts{}).ops) pg.plan_op(p);
   std::map<uint64_t, std::vector<const Term *>> by_flip;
@@ -822,6 +1012,14 @@ void compile(const Circuit &c, const Hamiltonian &h, uint32_t energy_slot, uint3
     else if (sg.kind == 1) pg.diagrun_backward(c, sg.blocks, grad_base);
     else pg.gate(inverse_gate(c.gates[sg.i]), F_BOTH);
   }
+  // The phase tables take masks of at most two bits (one- and two-qubit
+  // diagonal gates are all that reach a phase op).
+  for (size_t o = op0; o < pg.ops.size(); o++) {
+    const BOp &op = pg.ops[o];
+    if (op.code != B_DIAG && op.code != B_DIAGRUN) continue;
+    for (uint32_t t = op.a; t < op.a + op.b; t++)
+      if (__builtin_popcountll(pg.terms[t].mask) > 2) fail("batch: internal: phase mask weight");
+  }
 }
 
 template <typename T>
@@ -834,6 +1032,22 @@ void upload(T *&dst, const std::vector<T> &src) {
 }
 
 }  // namespace
+
+// Spread launches: each workgroup takes 2^kSpreadChunkBits amplitudes of one
+// instance; the states of one group of spread instances live in HBM together
+// (at most kSpreadBudget bytes, groups run one after another).
+constexpr uint32_t kSpreadChunkBits = 13;
+constexpr uint64_t kSpreadBudget = uint64_t(16) << 30;
+
+struct SpreadGroup {
+  uint32_t inst_off = 0;  // into the local-instance arrays (inst, state)
+  uint32_t base_off = 0;  // into wg_base (count + 1 entries per group)
+  uint32_t wg_off = 0;    // into wg_inst
+  uint32_t count = 0;
+  std::vector<uint32_t> step_insts;  // per step: instances still running (a prefix)
+  std::vector<uint32_t> step_wgs;    // per step: their workgroups (a prefix)
+  std::vector<uint8_t> step_reduce;  // per step: some running op reduces
+};
 
 struct Batch::Impl {
   int device = 0;
@@ -859,11 +1073,118 @@ struct Batch::Impl {
   uint32_t max_n = 0;
   size_t ngrad = 0;
   BatchConfig cfg;
+  // Host copies the spread plan is built from.
+  std::vector<BInst> h_insts;
+  std::vector<uint32_t> h_order;  // every instance, largest cost first
+  std::vector<uint32_t> h_nslot;  // per op: reduction sums it leaves
+  // Spread plan for cfg.spread_bits (built by build_spread).
+  uint32_t built_spread = ~0u;
+  std::vector<SpreadGroup> groups;
+  uint32_t *sp_inst = nullptr, *sp_wg_base = nullptr, *sp_wg_inst = nullptr;
+  uint64_t *sp_state = nullptr;
+  float2 *sp_scratch = nullptr;
+  double *sp_part = nullptr;
+  uint32_t sp_slots = 1;
+  uint32_t p_count = 0;  // instances run by the persistent kernel
   hipGraphExec_t exec = nullptr;
   bool captured = false;
   double ms = 0.0;
 
+  void free_spread() {
+    hipFree(sp_inst);
+    hipFree(sp_wg_base);
+    hipFree(sp_wg_inst);
+    hipFree(sp_state);
+    hipFree(sp_scratch);
+    hipFree(sp_part);
+    sp_inst = sp_wg_base = sp_wg_inst = nullptr;
+    sp_state = nullptr;
+    sp_scratch = nullptr;
+    sp_part = nullptr;
+    groups.clear();
+    built_spread = ~0u;
+  }
+
+  // Instances with n >= cfg.spread_bits (none when 0) leave the persistent
+  // queue: sorted by program length, cut into groups whose states fit the
+  // budget, each instance given 2^(n - kSpreadChunkBits) workgroups (at
+  // least one). The persistent kernel keeps the rest, in cost order, and its
+  // autotuning sample is every 16th of them.
+  void build_spread() {
+    if (built_spread == cfg.spread_bits) return;
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    free_spread();
+    std::vector<uint32_t> sp, rest;
+    for (uint32_t i : h_order)
+      (cfg.spread_bits && h_insts[i].n >= cfg.spread_bits ? sp : rest).push_back(i);
+    std::vector<uint32_t> sample;
+    for (size_t k = 0; k < rest.size(); k += 16) sample.push_back(rest[k]);
+    upload(order, rest);
+    upload(sample_order, sample);
+    p_count = uint32_t(rest.size());
+    sample_count = uint32_t(sample.size());
+    auto nops = [&](uint32_t i) { return h_insts[i].op_end - h_insts[i].op_begin; };
+    std::stable_sort(sp.begin(), sp.end(), [&](uint32_t x, uint32_t y) { return nops(x) > nops(y); });
+    std::vector<uint32_t> inst, base, wg_inst;
+    std::vector<uint64_t> state;
+    uint64_t scratch_max = 0, wgs_max = 0;
+    sp_slots = 1;
+    for (size_t k = 0; k < sp.size();) {
+      SpreadGroup g;
+      g.inst_off = uint32_t(inst.size());
+      g.base_off = uint32_t(base.size());
+      g.wg_off = uint32_t(wg_inst.size());
+      uint64_t elems = 0;
+      uint32_t wgs = 0;
+      while (k < sp.size()) {
+        const BInst &in = h_insts[sp[k]];
+        const uint64_t need = uint64_t(2) << in.n;
+        if (g.count && (elems + need) * sizeof(float2) > kSpreadBudget) break;
+        inst.push_back(sp[k]);
+        state.push_back(elems);
+        base.push_back(wgs);
+        const uint32_t chunks = in.n > kSpreadChunkBits ? 1u << (in.n - kSpreadChunkBits) : 1u;
+        for (uint32_t c = 0; c < chunks; c++) wg_inst.push_back(g.count);
+        wgs += chunks;
+        elems += need;
+        for (uint32_t o = in.op_begin; o < in.op_end; o++) sp_slots = std::max(sp_slots, h_nslot[o]);
+        g.count++;
+        k++;
+      }
+      base.push_back(wgs);
+      const uint32_t steps = nops(inst[g.inst_off]);
+      for (uint32_t j = 0; j < steps; j++) {
+        uint32_t running = 0;
+        bool reduce = false;
+        for (uint32_t l = 0; l < g.count; l++) {
+          const BInst &in = h_insts[inst[g.inst_off + l]];
+          if (in.op_begin + j >= in.op_end) break;  // sorted: the rest are shorter
+          running++;
+          reduce = reduce || h_nslot[in.op_begin + j] > 0;
+        }
+        g.step_insts.push_back(running);
+        g.step_wgs.push_back(base[g.base_off + running]);
+        g.step_reduce.push_back(reduce);
+      }
+      scratch_max = std::max(scratch_max, elems);
+      wgs_max = std::max<uint64_t>(wgs_max, wgs);
+      groups.push_back(std::move(g));
+    }
+    if (!groups.empty()) {
+      upload(sp_inst, inst);
+      upload(sp_wg_base, base);
+      upload(sp_wg_inst, wg_inst);
+      upload(sp_state, state);
+      check(hipMalloc(reinterpret_cast<void **>(&sp_scratch), scratch_max * sizeof(float2)),
+            "hipMalloc(spread states)");
+      check(hipMalloc(reinterpret_cast<void **>(&sp_part), wgs_max * sp_slots * sizeof(double)),
+            "hipMalloc(spread partials)");
+    }
+    built_spread = cfg.spread_bits;
+  }
+
   void free_all() {
+    free_spread();
     hipFree(ops);
     hipFree(insts);
     hipFree(order);
@@ -885,9 +1206,14 @@ struct Batch::Impl {
   }
 
   size_t lds_bytes(const BatchConfig &c) const { return size_t(2) << c.lds_bits << 3; }
+  // LDS of the persistent kernel beyond the states: phase tables, reductions.
+  static constexpr size_t kStaticLds = sizeof(PhaseTab) + sizeof(double) * kOverlapChunk * kWaves + 64;
 
   void ensure_scratch(uint32_t wgs) {
-    const uint64_t need_slot = max_n > cfg.lds_bits ? (uint64_t(1) << max_n) : 0;
+    uint32_t widest = 0;  // widest instance the persistent kernel runs
+    for (const BInst &in : h_insts)
+      if (!cfg.spread_bits || in.n < cfg.spread_bits) widest = std::max(widest, in.n);
+    const uint64_t need_slot = widest > cfg.lds_bits ? (uint64_t(1) << widest) : 0;
     if (need_slot == 0) return;
     if (scratch && slot >= need_slot && scratch_wgs >= wgs) return;
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -907,7 +1233,7 @@ struct Batch::Impl {
     A.terms = terms;
     A.mats = mats;
     A.aux = aux;
-    A.count = sample ? sample_count : count;
+    A.count = sample ? sample_count : p_count;
     A.lds_bits = cfg.lds_bits;
     A.slot = slot;
     A.scratch = scratch;
@@ -917,15 +1243,36 @@ struct Batch::Impl {
     return A;
   }
 
+  // The spread groups step by step (the full batch only), then the
+  // persistent kernel over the rest. The spread plan and scratch are set up
+  // beforehand (build_spread, ensure_scratch): nothing here synchronises, so
+  // it can be captured.
   void enqueue(bool sample) {
     const uint32_t wgs = uint32_t(cus) * cfg.per_cu;
     ensure_scratch(wgs);
+    const BatchArgs A = args(sample);
+    if (!sample)
+      for (const SpreadGroup &g : groups) {
+        SpreadArgs S{};
+        S.inst = sp_inst + g.inst_off;
+        S.wg_base = sp_wg_base + g.base_off;
+        S.wg_inst = sp_wg_inst + g.wg_off;
+        S.state = sp_state + g.inst_off;
+        S.scratch = sp_scratch;
+        S.part = sp_part;
+        S.slots = sp_slots;
+        for (uint32_t j = 0; j < g.step_wgs.size(); j++) {
+          k_spread<<<g.step_wgs[j], kMaxThreads, 0, stream>>>(A, S, j);
+          if (g.step_reduce[j]) k_spread_reduce<<<g.step_insts[j], 64, 0, stream>>>(A, S, j);
+        }
+      }
+    if (!A.count) return;
     const size_t lds = lds_bytes(cfg);
     check(hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)),
           "hipFuncSetAttribute");
     k_reset<<<1, 64, 0, stream>>>(counter);
-    k_batch<<<wgs, cfg.threads, lds, stream>>>(args(sample));
+    k_batch<<<wgs, cfg.threads, lds, stream>>>(A);
   }
 
   void capture() {
@@ -933,6 +1280,7 @@ struct Batch::Impl {
       check(hipGraphExecDestroy(exec), "hipGraphExecDestroy");
       exec = nullptr;
     }
+    build_spread();
     ensure_scratch(uint32_t(cus) * cfg.per_cu);
     hipGraph_t graph;
     check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -944,13 +1292,15 @@ struct Batch::Impl {
   }
 
   double timed(bool sample) {
-    check(hipEventRecord(ev0, stream), "hipEventRecord");
     if (sample) {
-      enqueue(true);
-    } else {
-      if (!captured) capture();
-      check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+      build_spread();
+      ensure_scratch(uint32_t(cus) * cfg.per_cu);
+    } else if (!captured) {
+      capture();
     }
+    check(hipEventRecord(ev0, stream), "hipEventRecord");
+    if (sample) enqueue(true);
+    else check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
     check(hipEventRecord(ev1, stream), "hipEventRecord");
     check(hipStreamSynchronize(stream), "GPU error");
     float t = 0.0f;
@@ -972,7 +1322,7 @@ Batch::Batch() : impl_(new Impl) {
       size_t(optin) > b->max_lds)
     b->max_lds = size_t(optin);
   (void)hipGetLastError();
-  b->cfg.lds_bits = b->max_lds >= (size_t(2) << 13 << 3) ? 13 : 12;
+  b->cfg.lds_bits = b->max_lds >= (size_t(2) << 13 << 3) + Impl::kStaticLds ? 13 : 12;
   b->cfg.threads = 512;
   check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
   check(hipEventCreate(&b->ev0), "hipEventCreate");
@@ -1020,21 +1370,25 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(),
                    [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
-  // Every 16th instance of that order: the autotuner's sample.
-  std::vector<uint32_t> sample;
-  for (size_t k = 0; k < order.size(); k += 16) sample.push_back(order[k]);
   check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
   b->free_all();
   b->max_n = max_n;
+  b->h_insts = insts;
+  b->h_order = order;
+  b->h_nslot.resize(pg.ops.size());
+  for (size_t o = 0; o < pg.ops.size(); o++) {
+    const BOp &op = pg.ops[o];
+    b->h_nslot[o] = op.code == B_ROTGROUP ? op.k
+                    : op.code == B_DIAGRUN ? op.d
+                    : op.code == B_ENERGY  ? 1u
+                                           : 0u;
+  }
   upload(b->ops, pg.ops);
   upload(b->insts, insts);
-  upload(b->order, order);
-  upload(b->sample_order, sample);
   upload(b->terms, pg.terms);
   upload(b->mats, pg.mats);
   upload(b->aux, pg.aux);
   b->count = uint32_t(insts.size());
-  b->sample_count = uint32_t(sample.size());
   b->ngrad = grad_base;
   check(hipMalloc(reinterpret_cast<void **>(&b->energy), std::max<size_t>(1, b->count) * 8),
         "hipMalloc(energy)");
@@ -1043,6 +1397,7 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
   // Gates without a parameter keep gradient 0: the kernel never writes them.
   check(hipMemset(b->grad, 0, std::max<size_t>(1, b->ngrad) * 8), "hipMemset(grad)");
   b->captured = false;
+  b->build_spread();
 }
 
 void Batch::set_config(const BatchConfig &c) {
@@ -1050,8 +1405,18 @@ void Batch::set_config(const BatchConfig &c) {
   if (c.threads < 64 || c.threads > uint32_t(kMaxThreads) || c.threads % 64)
     fail("batch: threads must be a multiple of 64 up to 512");
   if (c.per_cu < 1 || c.per_cu > 4) fail("batch: per_cu must be 1..4");
-  if (c.lds_bits > 14 || b->lds_bytes(c) > b->max_lds) fail("batch: LDS cut-off too large");
+  if (c.lds_bits > 14 || b->lds_bytes(c) + Impl::kStaticLds > b->max_lds)
+    fail("batch: LDS cut-off too large");
+  const uint32_t spread = b->cfg.spread_bits;
   b->cfg = c;
+  b->cfg.spread_bits = spread;
+  b->captured = false;
+}
+
+void Batch::set_spread(uint32_t bits) {
+  Impl *b = impl_.get();
+  if (bits > kBatchMaxQubits) fail("batch: spread width must be 0..20");
+  b->cfg.spread_bits = bits;
   b->captured = false;
 }
 
@@ -1063,15 +1428,18 @@ BatchConfig Batch::autotune() {
   Impl *b = impl_.get();
   if (!b->count) return b->cfg;
   check(hipSetDevice(b->device), "hipSetDevice");
+  // The persistent kernel is tuned on a sample of every instance (nothing
+  // spread); the spread width is then picked by timing the whole batch.
+  set_spread(0);
   BatchConfig best = b->cfg;
   double best_ms = 1e300;
   for (uint32_t bits : {11u, 12u, 13u}) {
     BatchConfig c;
     c.lds_bits = bits;
-    if (b->lds_bytes(c) > b->max_lds) continue;
+    if (b->lds_bytes(c) + Impl::kStaticLds > b->max_lds) continue;
     for (uint32_t per : {1u, 2u}) {
       // All persistent workgroups of a CU must fit its 160 KiB of LDS.
-      if (per * b->lds_bytes(c) > (size_t(160) << 10)) continue;
+      if (per * (b->lds_bytes(c) + Impl::kStaticLds) > (size_t(160) << 10)) continue;
       for (uint32_t th : {256u, 512u}) {
         if (per * th > 2048) continue;
         c.per_cu = per;
@@ -1087,7 +1455,22 @@ BatchConfig Batch::autotune() {
     }
   }
   set_config(best);
-  return best;
+  uint32_t best_spread = 0;
+  if (b->max_n >= kSpreadMinQubits) {
+    double best_full = 1e300;
+    for (uint32_t sp : {0u, b->max_n, b->max_n - 1, b->max_n - 2}) {
+      if (sp && sp < kSpreadMinQubits) continue;
+      set_spread(sp);
+      b->timed(false);
+      const double ms = b->timed(false);
+      if (ms < best_full) {
+        best_full = ms;
+        best_spread = sp;
+      }
+    }
+  }
+  set_spread(best_spread);
+  return b->cfg;
 }
 
 void Batch::run(double *energies, double *grads) {

@@ -387,6 +387,16 @@ int qm_batch_set_config(qm_batch b, uint32_t threads, uint32_t per_cu, uint32_t 
   });
 }
 
+int qm_batch_set_spread(qm_batch b, uint32_t min_qubits) {
+  if (!b) {
+    set_error("null batch");
+    return -1;
+  }
+  return guard([&] { b->b->set_spread(min_qubits); });
+}
+
+uint32_t qm_batch_spread(qm_batch b) { return b ? b->b->config().spread_bits : 0; }
+
 int qm_batch_run(qm_batch b, double *out_energy, double *out_grad, double *out_ms) {
   if (!b) {
     set_error("null batch");

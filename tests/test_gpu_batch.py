@@ -58,6 +58,31 @@ def test_batch_matches_plain_path_and_oracle():
         assert abs(e[i] - O.expect_h(O.run(n, g), h)) <= 1e-5
 
 
+def test_spread_instances_match_plain_path():
+    """Every width split over many workgroups (one launch per program step
+    for all spread instances), none, and the default (14 qubits and up):
+    the same energies and gradients as the one-circuit path."""
+    inst = mixed_instances()
+    inst += [(n, C.qaoa_ring(n, 2, [0.7, 0.1], [0.4, 0.3]), C.ring_zz_hamiltonian(n))
+             for n in (17, 18)]
+    circs = [C.to_circuit(n, g) for n, g, _ in inst]
+    hams = [C.to_hamiltonian(h) for _, _, h in inst]
+    want = [plain(n, g, h) for n, g, h in inst]
+    b = qm.Batch().prepare(circs, hams)
+    results = {}
+    for spread in (3, 0, 14, 9):
+        b.set_spread(spread)
+        e, grads, ms = b.run()
+        results[spread] = (e, grads)
+        worst = max(max(abs(e[i] - e0), np.abs(grads[i] - g0).max())
+                    for i, (e0, g0) in enumerate(want))
+        print(f"spread from {spread} qubits: {ms:.2f} ms, worst diff vs plain {worst:.2e}")
+        assert worst <= 1e-5, spread
+    e, grads, _ = b.run()  # a replay with the last setting: identical
+    assert np.array_equal(e, results[9][0])
+    assert all(np.array_equal(x, y) for x, y in zip(grads, results[9][1]))
+
+
 def test_batch_graph_replays_are_identical():
     inst = C.qaoa_instances(64, lo=6, hi=14, seed=5)
     circs = [C.to_circuit(n, C.qaoa_ring(n, 6, ga, be)) for n, ga, be in inst]

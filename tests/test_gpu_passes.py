@@ -174,6 +174,20 @@ def test_qft_24_merged_vs_op_by_op():
     assert dm <= TOL_SAME and abs(norm - 1) <= TOL1
 
 
+@pytest.mark.parametrize("n,want", [(13, 1), (14, 2), (20, 2)])
+def test_qft_four_wave_low_pass_against_oracle(n, want):
+    """QFT wires 12..0 in one low pass over four-wave 2^13 tiles (after a
+    wide pass on 19..13 at n=20, a register pass on 13 at n=14)."""
+    prep = C.product_state_prep(n, seed=77 + n)
+    got, norm, launches = run(n, C.qft(n), prep=prep)
+    ref, _, _ = run(n, C.qft(n), merge=False, prep=prep)
+    want_amps = O.run(n, prep + C.qft(n))
+    d, dm = np.abs(got - want_amps).max(), np.abs(got - ref).max()
+    print(f"QFT{n}: {launches} launches; diff vs oracle {d:.2e}, vs op-by-op {dm:.2e}")
+    assert launches == want
+    assert d <= TOL1 and dm <= TOL_SAME and abs(norm - 1) <= TOL1
+
+
 def test_qft_zero_state_is_uniform_merged():
     n = 28
     with qm.Simulator(n) as s:
