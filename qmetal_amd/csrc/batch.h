@@ -27,7 +27,7 @@ struct BatchConfig {
   // Instances with n >= spread_bits (0: none) run spread: several
   // workgroups per instance, one launch per program step for all of them,
   // before the persistent kernel takes the rest. autotune picks it by timing
-  // the whole batch (none, or from the widest instance down by up to 2).
+  // the whole batch (none, or from one or three below the widest instance).
   uint32_t spread_bits = 0;
 };
 

@@ -1037,6 +1037,9 @@ void upload(T *&dst, const std::vector<T> &src) {
 // instance; the states of one group of spread instances live in HBM together
 // (at most kSpreadBudget bytes, groups run one after another).
 constexpr uint32_t kSpreadChunkBits = 13;
+// 256 threads: a spread workgroup (one wave per SIMD) fits beside a
+// persistent one on the same CU.
+constexpr int kSpreadThreads = 256;
 constexpr uint64_t kSpreadBudget = uint64_t(16) << 30;
 
 struct SpreadGroup {
@@ -1054,6 +1057,8 @@ struct Batch::Impl {
   int cus = 0;
   size_t max_lds = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // spread launches, beside the persistent kernel
+  hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   BOp *ops = nullptr;
   BInst *insts = nullptr;
@@ -1247,10 +1252,21 @@ struct Batch::Impl {
   // persistent kernel over the rest. The spread plan and scratch are set up
   // beforehand (build_spread, ensure_scratch): nothing here synchronises, so
   // it can be captured.
+  //
+  // With both parts present the spread launches go to a second stream that
+  // forks from and joins back into the first, so that they run beside the
+  // persistent kernel (which keeps its LDS-resident instances compute-bound
+  // while the spread steps stream from HBM).
   void enqueue(bool sample) {
     const uint32_t wgs = uint32_t(cus) * cfg.per_cu;
     ensure_scratch(wgs);
     const BatchArgs A = args(sample);
+    const bool both = !sample && !groups.empty() && A.count;
+    hipStream_t ss = both ? side : stream;
+    if (both) {
+      check(hipEventRecord(fork, stream), "hipEventRecord");
+      check(hipStreamWaitEvent(side, fork, 0), "hipStreamWaitEvent");
+    }
     if (!sample)
       for (const SpreadGroup &g : groups) {
         SpreadArgs S{};
@@ -1262,8 +1278,8 @@ struct Batch::Impl {
         S.part = sp_part;
         S.slots = sp_slots;
         for (uint32_t j = 0; j < g.step_wgs.size(); j++) {
-          k_spread<<<g.step_wgs[j], kMaxThreads, 0, stream>>>(A, S, j);
-          if (g.step_reduce[j]) k_spread_reduce<<<g.step_insts[j], 64, 0, stream>>>(A, S, j);
+          k_spread<<<g.step_wgs[j], kSpreadThreads, 0, ss>>>(A, S, j);
+          if (g.step_reduce[j]) k_spread_reduce<<<g.step_insts[j], 64, 0, ss>>>(A, S, j);
         }
       }
     if (!A.count) return;
@@ -1273,6 +1289,10 @@ struct Batch::Impl {
           "hipFuncSetAttribute");
     k_reset<<<1, 64, 0, stream>>>(counter);
     k_batch<<<wgs, cfg.threads, lds, stream>>>(A);
+    if (both) {
+      check(hipEventRecord(join, side), "hipEventRecord");
+      check(hipStreamWaitEvent(stream, join, 0), "hipStreamWaitEvent");
+    }
   }
 
   void capture() {
@@ -1325,6 +1345,9 @@ Batch::Batch() : impl_(new Impl) {
   b->cfg.lds_bits = b->max_lds >= (size_t(2) << 13 << 3) + Impl::kStaticLds ? 13 : 12;
   b->cfg.threads = 512;
   check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+  check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
+  check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+  check(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
   check(hipEventCreate(&b->ev0), "hipEventCreate");
   check(hipEventCreate(&b->ev1), "hipEventCreate");
   check(hipMalloc(reinterpret_cast<void **>(&b->counter), 64), "hipMalloc(counter)");
@@ -1340,6 +1363,9 @@ Batch::~Batch() {
   hipFree(b->counter);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->join) hipEventDestroy(b->join);
+  if (b->side) hipStreamDestroy(b->side);
   if (b->stream) hipStreamDestroy(b->stream);
 }
 
@@ -1458,7 +1484,7 @@ BatchConfig Batch::autotune() {
   uint32_t best_spread = 0;
   if (b->max_n >= kSpreadMinQubits) {
     double best_full = 1e300;
-    for (uint32_t sp : {0u, b->max_n, b->max_n - 1, b->max_n - 2}) {
+    for (uint32_t sp : {0u, b->max_n - 1, b->max_n - 3}) {
       if (sp && sp < kSpreadMinQubits) continue;
       set_spread(sp);
       b->timed(false);
