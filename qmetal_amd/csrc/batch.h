@@ -27,7 +27,7 @@ struct BatchConfig {
   // Instances with n >= spread_bits (0: none) run spread: several
   // workgroups per instance, one launch per program step for all of them,
   // before the persistent kernel takes the rest. autotune picks it by timing
-  // the whole batch (none, or from one or three below the widest instance).
+  // the whole batch (none, or from 1, 3 or 5 below the widest instance).
   uint32_t spread_bits = 0;
 };
 

@@ -668,16 +668,31 @@ __global__ __launch_bounds__(kMaxThreads) void k_batch(BatchArgs A) {
   }
 }
 
+// Reduction sums an op leaves (one chunk's share each).
+__host__ __device__ inline uint32_t op_slots(const BOp &op) {
+  return op.code == B_ROTGROUP ? op.k
+         : op.code == B_DIAGRUN ? op.d
+         : op.code == B_ENERGY  ? 1u
+                                : 0u;
+}
+
 // Spread instances (states in HBM, several workgroups each) run in lockstep:
-// one launch per program step, workgroup w taking chunk w - wg_base[l] of
-// local instance l = wg_inst[w]. Reductions leave one row of `slots` partial
-// sums per workgroup; k_spread_reduce adds each instance's rows in chunk
-// order (deterministic) into the energies and gradients.
+// one launch per step, workgroup w taking chunk w - wg_base[l] of local
+// instance l = wg_inst[w]. A step is one op, or a run of consecutive ops
+// whose every unit stays inside one chunk (ops [steps[step_off[l] + j],
+// steps[step_off[l] + j + 1]) for step j), applied one after another on the
+// workgroup's chunk. Reductions leave one row of `slots` partial sums per
+// workgroup (the step's ops' sums side by side); k_spread_reduce adds each
+// instance's rows in chunk order (deterministic) into the energies and
+// gradients.
 struct SpreadArgs {
-  const uint32_t *inst;     // local instance -> instance
-  const uint32_t *wg_base;  // local instance -> first workgroup (count + 1 entries)
-  const uint32_t *wg_inst;  // workgroup -> local instance
-  const uint64_t *state;    // local instance -> offset of psi in scratch (lam follows)
+  const uint32_t *inst;      // local instance -> instance
+  const uint32_t *wg_base;   // local instance -> first workgroup (count + 1 entries)
+  const uint32_t *wg_inst;   // workgroup -> local instance
+  const uint64_t *state;     // local instance -> offset of psi in scratch (lam follows)
+  const uint32_t *step_off;  // local instance -> its first entry in steps
+  const uint32_t *nsteps;    // local instance -> its number of steps
+  const uint32_t *steps;     // op boundaries (nsteps + 1 entries per instance)
   float2 *scratch;
   double *part;
   uint32_t slots;
@@ -687,35 +702,42 @@ __global__ __launch_bounds__(kMaxThreads) void k_spread(BatchArgs A, SpreadArgs 
   __shared__ double red[kOverlapChunk][kWaves];
   __shared__ PhaseTab tab;
   const uint32_t l = S.wg_inst[blockIdx.x];
+  if (step >= S.nsteps[l]) return;
   const BInst in = A.insts[S.inst[l]];
-  if (in.op_begin + step >= in.op_end) return;
   const uint32_t c = blockIdx.x - S.wg_base[l], nc = S.wg_base[l + 1] - S.wg_base[l];
   const uint32_t dim = 1u << in.n;
   float2 *psi = S.scratch + S.state[l];
-  run_op(A, A.ops[in.op_begin + step], psi, psi + dim, dim, c, nc,
-         S.part + uint64_t(blockIdx.x) * S.slots, red, tab);
+  double *row = S.part + uint64_t(blockIdx.x) * S.slots;
+  const uint32_t o0 = S.steps[S.step_off[l] + step], o1 = S.steps[S.step_off[l] + step + 1];
+  for (uint32_t o = o0; o < o1; o++) {
+    const BOp op = A.ops[o];
+    if (o > o0) __syncthreads();  // the previous op's writes to the chunk
+    run_op(A, op, psi, psi + dim, dim, c, nc, row, red, tab);
+    row += op_slots(op);
+  }
 }
 
 __global__ __launch_bounds__(64) void k_spread_reduce(BatchArgs A, SpreadArgs S, uint32_t step) {
   const uint32_t l = blockIdx.x;
-  const BInst in = A.insts[S.inst[l]];
-  if (in.op_begin + step >= in.op_end) return;
-  const BOp op = A.ops[in.op_begin + step];
+  if (step >= S.nsteps[l]) return;
+  const uint32_t o0 = S.steps[S.step_off[l] + step], o1 = S.steps[S.step_off[l] + step + 1];
   const uint32_t w0 = S.wg_base[l], w1 = S.wg_base[l + 1];
-  uint32_t nslot = 0;
-  if (op.code == B_ROTGROUP) nslot = op.k;
-  else if (op.code == B_DIAGRUN) nslot = op.d;
-  else if (op.code == B_ENERGY) nslot = 1;
-  for (uint32_t r = threadIdx.x; r < nslot; r += blockDim.x) {
-    double v = 0.0;
-    for (uint32_t w = w0; w < w1; w++) v += S.part[uint64_t(w) * S.slots + r];
-    if (op.code == B_ENERGY) {
-      A.energy[op.a] = v;
-    } else if (op.code == B_DIAGRUN) {
-      A.grad[uint32_t(A.terms[op.c + r].x)] = v;
-    } else if (A.aux[op.c + 2 * r] != kNoAxis) {
-      A.grad[A.aux[op.c + 2 * r + 1]] = v;
+  uint32_t off = 0;
+  for (uint32_t o = o0; o < o1; o++) {
+    const BOp op = A.ops[o];
+    const uint32_t nslot = op_slots(op);
+    for (uint32_t r = threadIdx.x; r < nslot; r += blockDim.x) {
+      double v = 0.0;
+      for (uint32_t w = w0; w < w1; w++) v += S.part[uint64_t(w) * S.slots + off + r];
+      if (op.code == B_ENERGY) {
+        A.energy[op.a] = v;
+      } else if (op.code == B_DIAGRUN) {
+        A.grad[uint32_t(A.terms[op.c + r].x)] = v;
+      } else if (A.aux[op.c + 2 * r] != kNoAxis) {
+        A.grad[A.aux[op.c + 2 * r + 1]] = v;
+      }
     }
+    off += nslot;
   }
 }
 
@@ -1041,6 +1063,7 @@ constexpr uint32_t kSpreadChunkBits = 13;
 // persistent one on the same CU.
 constexpr int kSpreadThreads = 256;
 constexpr uint64_t kSpreadBudget = uint64_t(16) << 30;
+constexpr uint32_t kFuseOps = 24, kFuseSlots = 64;
 
 struct SpreadGroup {
   uint32_t inst_off = 0;  // into the local-instance arrays (inst, state)
@@ -1081,11 +1104,12 @@ struct Batch::Impl {
   // Host copies the spread plan is built from.
   std::vector<BInst> h_insts;
   std::vector<uint32_t> h_order;  // every instance, largest cost first
-  std::vector<uint32_t> h_nslot;  // per op: reduction sums it leaves
+  std::vector<BOp> h_ops;
   // Spread plan for cfg.spread_bits (built by build_spread).
   uint32_t built_spread = ~0u;
   std::vector<SpreadGroup> groups;
   uint32_t *sp_inst = nullptr, *sp_wg_base = nullptr, *sp_wg_inst = nullptr;
+  uint32_t *sp_step_off = nullptr, *sp_nsteps = nullptr, *sp_steps = nullptr;
   uint64_t *sp_state = nullptr;
   float2 *sp_scratch = nullptr;
   double *sp_part = nullptr;
@@ -1096,6 +1120,10 @@ struct Batch::Impl {
   double ms = 0.0;
 
   void free_spread() {
+    hipFree(sp_step_off);
+    hipFree(sp_nsteps);
+    hipFree(sp_steps);
+    sp_step_off = sp_nsteps = sp_steps = nullptr;
     hipFree(sp_inst);
     hipFree(sp_wg_base);
     hipFree(sp_wg_inst);
@@ -1128,9 +1156,13 @@ struct Batch::Impl {
     upload(sample_order, sample);
     p_count = uint32_t(rest.size());
     sample_count = uint32_t(sample.size());
-    auto nops = [&](uint32_t i) { return h_insts[i].op_end - h_insts[i].op_begin; };
-    std::stable_sort(sp.begin(), sp.end(), [&](uint32_t x, uint32_t y) { return nops(x) > nops(y); });
-    std::vector<uint32_t> inst, base, wg_inst;
+    // Steps of each spread instance: chunk-local ops merge into runs.
+    std::map<uint32_t, std::vector<uint32_t>> bounds;
+    for (uint32_t i : sp) bounds[i] = spread_steps(h_insts[i]);
+    auto nsteps = [&](uint32_t i) { return uint32_t(bounds[i].size() - 1); };
+    std::stable_sort(sp.begin(), sp.end(),
+                     [&](uint32_t x, uint32_t y) { return nsteps(x) > nsteps(y); });
+    std::vector<uint32_t> inst, base, wg_inst, step_off, nst, steps;
     std::vector<uint64_t> state;
     uint64_t scratch_max = 0, wgs_max = 0;
     sp_slots = 1;
@@ -1152,20 +1184,29 @@ struct Batch::Impl {
         for (uint32_t c = 0; c < chunks; c++) wg_inst.push_back(g.count);
         wgs += chunks;
         elems += need;
-        for (uint32_t o = in.op_begin; o < in.op_end; o++) sp_slots = std::max(sp_slots, h_nslot[o]);
+        const std::vector<uint32_t> &bd = bounds[sp[k]];
+        step_off.push_back(uint32_t(steps.size()));
+        nst.push_back(uint32_t(bd.size() - 1));
+        steps.insert(steps.end(), bd.begin(), bd.end());
+        for (size_t j = 0; j + 1 < bd.size(); j++) {
+          uint32_t sl = 0;
+          for (uint32_t o = bd[j]; o < bd[j + 1]; o++) sl += op_slots(h_ops[o]);
+          sp_slots = std::max(sp_slots, sl);
+        }
         g.count++;
         k++;
       }
       base.push_back(wgs);
-      const uint32_t steps = nops(inst[g.inst_off]);
-      for (uint32_t j = 0; j < steps; j++) {
+      const uint32_t gsteps = nst[g.inst_off];
+      for (uint32_t j = 0; j < gsteps; j++) {
         uint32_t running = 0;
         bool reduce = false;
         for (uint32_t l = 0; l < g.count; l++) {
-          const BInst &in = h_insts[inst[g.inst_off + l]];
-          if (in.op_begin + j >= in.op_end) break;  // sorted: the rest are shorter
+          const uint32_t li = g.inst_off + l;
+          if (j >= nst[li]) break;  // sorted: the rest have fewer steps
           running++;
-          reduce = reduce || h_nslot[in.op_begin + j] > 0;
+          for (uint32_t o = steps[step_off[li] + j]; o < steps[step_off[li] + j + 1]; o++)
+            reduce = reduce || op_slots(h_ops[o]) > 0;
         }
         g.step_insts.push_back(running);
         g.step_wgs.push_back(base[g.base_off + running]);
@@ -1180,12 +1221,65 @@ struct Batch::Impl {
       upload(sp_wg_base, base);
       upload(sp_wg_inst, wg_inst);
       upload(sp_state, state);
+      upload(sp_step_off, step_off);
+      upload(sp_nsteps, nst);
+      upload(sp_steps, steps);
       check(hipMalloc(reinterpret_cast<void **>(&sp_scratch), scratch_max * sizeof(float2)),
             "hipMalloc(spread states)");
       check(hipMalloc(reinterpret_cast<void **>(&sp_part), wgs_max * sp_slots * sizeof(double)),
             "hipMalloc(spread partials)");
     }
     built_spread = cfg.spread_bits;
+  }
+
+  // Whether every unit of `op` on an n-qubit state stays inside one spread
+  // chunk (2^kSpreadChunkBits aligned amplitudes): its moved wires lie below
+  // the chunk bits (controls and phase masks may be anywhere).
+  static bool chunk_local(const BOp &op, uint32_t n) {
+    if (n <= kSpreadChunkBits) return true;
+    switch (op.code) {
+      case B_INIT:
+      case B_DIAG:
+      case B_DIAGRUN:
+      case B_ENERGY:
+        return true;
+      case B_ACCUM:
+        return (op.mask >> kSpreadChunkBits) == 0;
+      case B_GROUP:
+      case B_ROTGROUP:
+        for (uint32_t j = 0; j < op.k; j++)
+          if (op.w[j] >= kSpreadChunkBits) return false;
+        return true;
+      case B_CTRL:
+        return op.w[0] < kSpreadChunkBits;
+      case B_SWAP:
+        return op.w[0] < kSpreadChunkBits && op.w[1] < kSpreadChunkBits;
+      default:
+        return false;
+    }
+  }
+
+  // Op boundaries of an instance's spread steps: runs of chunk-local ops (at
+  // most kFuseOps ops and kFuseSlots reduction sums each), other ops alone.
+  std::vector<uint32_t> spread_steps(const BInst &in) const {
+    std::vector<uint32_t> bd{in.op_begin};
+    uint32_t run = 0, slots = 0;
+    bool run_local = false;
+    for (uint32_t o = in.op_begin; o < in.op_end; o++) {
+      const bool loc = chunk_local(h_ops[o], in.n);
+      const uint32_t sl = op_slots(h_ops[o]);
+      const bool join = run > 0 && loc && run_local && run < kFuseOps && slots + sl <= kFuseSlots;
+      if (run > 0 && !join) {
+        bd.push_back(o);
+        run = 0;
+        slots = 0;
+      }
+      run++;
+      slots += sl;
+      run_local = loc;
+    }
+    bd.push_back(in.op_end);
+    return bd;
   }
 
   void free_all() {
@@ -1274,6 +1368,9 @@ struct Batch::Impl {
         S.wg_base = sp_wg_base + g.base_off;
         S.wg_inst = sp_wg_inst + g.wg_off;
         S.state = sp_state + g.inst_off;
+        S.step_off = sp_step_off + g.inst_off;
+        S.nsteps = sp_nsteps + g.inst_off;
+        S.steps = sp_steps;
         S.scratch = sp_scratch;
         S.part = sp_part;
         S.slots = sp_slots;
@@ -1401,14 +1498,7 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
   b->max_n = max_n;
   b->h_insts = insts;
   b->h_order = order;
-  b->h_nslot.resize(pg.ops.size());
-  for (size_t o = 0; o < pg.ops.size(); o++) {
-    const BOp &op = pg.ops[o];
-    b->h_nslot[o] = op.code == B_ROTGROUP ? op.k
-                    : op.code == B_DIAGRUN ? op.d
-                    : op.code == B_ENERGY  ? 1u
-                                           : 0u;
-  }
+  b->h_ops = pg.ops;
   upload(b->ops, pg.ops);
   upload(b->insts, insts);
   upload(b->terms, pg.terms);
@@ -1484,7 +1574,7 @@ BatchConfig Batch::autotune() {
   uint32_t best_spread = 0;
   if (b->max_n >= kSpreadMinQubits) {
     double best_full = 1e300;
-    for (uint32_t sp : {0u, b->max_n - 1, b->max_n - 3}) {
+    for (uint32_t sp : {0u, b->max_n - 1, b->max_n - 3, b->max_n - 5}) {
       if (sp && sp < kSpreadMinQubits) continue;
       set_spread(sp);
       b->timed(false);
