@@ -145,17 +145,55 @@ __device__ __forceinline__ Range chunk_of(uint32_t units, uint32_t c, uint32_t n
   return Range{uint32_t(uint64_t(units) * c / nc), uint32_t(uint64_t(units) * (c + 1) / nc)};
 }
 
-template <int K>
-__device__ void group(float2 *s, Range rg, const BOp &op, const float *mats) {
+// The units of an op a workgroup takes, by local index v < count: the
+// contiguous range [base, base + count) (CMap), or, in a high chunk, every
+// unit whose unit-space bits [lo_pos, lo_pos + mbits) equal `base` (HMap: the
+// bits of v below lo_pos as they are, the rest moved above the fixed ones).
+struct CMap {
+  uint32_t base, count;
+  __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return base + v; }
+};
+struct HMap {
+  uint32_t base, count, lo_pos, mbits;
+  __device__ __forceinline__ uint32_t operator()(uint32_t v) const {
+    const uint32_t low = v & ((1u << lo_pos) - 1u);
+    return low | (base << lo_pos) | ((v >> lo_pos) << (lo_pos + mbits));
+  }
+};
+
+// Where a workgroup works: chunk c of nc contiguous ones, or (high) chunk c
+// of the 2^mbits chunks that fix index bits [kHighLow, kHighLow + mbits).
+constexpr uint32_t kHighLow = 6;
+struct Geo {
+  uint32_t c, nc, mbits;
+  bool high;
+};
+
+// Unit map of an op with `units` units whose moved wires are w[0..k): the
+// contiguous chunk (HIGH false) or the high chunk of the geometry.
+template <bool HIGH>
+__device__ __forceinline__ auto unit_map(const Geo &g, uint32_t units, const uint8_t *w, int k) {
+  if constexpr (!HIGH) {
+    const Range rg = chunk_of(units, g.c, g.nc);
+    return CMap{rg.lo, rg.hi - rg.lo};
+  } else {
+    uint32_t kl = 0;  // moved wires below the fixed bits
+    for (int j = 0; j < k; j++) kl += w[j] < kHighLow ? 1u : 0u;
+    return HMap{g.c, units >> g.mbits, kHighLow - kl, g.mbits};
+  }
+}
+
+template <int K, typename M>
+__device__ void group(float2 *s, M m, const BOp &op, const float *mats) {
   constexpr int U = K >= 4 ? 1 : 16 >> K;  // units per thread per iteration
-  for (uint32_t u0 = rg.lo + threadIdx.x; u0 < rg.hi; u0 += U * blockDim.x) {
+  for (uint32_t v0 = threadIdx.x; v0 < m.count; v0 += U * blockDim.x) {
     float2 x[U][1 << K];
     uint32_t idx[U][1 << K];
 #pragma unroll
     for (int r = 0; r < U; r++) {
-      const uint32_t u = u0 + r * blockDim.x;
-      if (u >= rg.hi) continue;
-      uint32_t base = u;
+      const uint32_t v = v0 + r * blockDim.x;
+      if (v >= m.count) continue;
+      uint32_t base = m(v);
 #pragma unroll
       for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
 #pragma unroll
@@ -170,13 +208,13 @@ __device__ void group(float2 *s, Range rg, const BOp &op, const float *mats) {
     }
 #pragma unroll
     for (int r = 0; r < U; r++) {
-      if (u0 + r * blockDim.x >= rg.hi) continue;
+      if (v0 + r * blockDim.x >= m.count) continue;
 #pragma unroll
       for (int j = 0; j < K; j++) {
-        const float *m = mats + op.a + 8 * j;
+        const float *mt = mats + op.a + 8 * j;
 #pragma unroll
         for (int l = 0; l < (1 << K); l++)
-          if (!((l >> j) & 1)) rot2(m, x[r][l], x[r][l | (1 << j)]);
+          if (!((l >> j) & 1)) rot2(mt, x[r][l], x[r][l | (1 << j)]);
       }
 #pragma unroll
       for (int l = 0; l < (1 << K); l++) s[idx[r][l]] = x[r][l];
@@ -184,14 +222,15 @@ __device__ void group(float2 *s, Range rg, const BOp &op, const float *mats) {
   }
 }
 
-__device__ void group_any(float2 *s, uint32_t dim, uint32_t c, uint32_t nc, const BOp &op,
+template <bool HIGH>
+__device__ void group_any(float2 *s, uint32_t dim, const Geo &g, const BOp &op,
                           const float *mats) {
-  const Range rg = chunk_of(dim >> op.k, c, nc);
+  const auto m = unit_map<HIGH>(g, dim >> op.k, op.w, op.k);
   switch (op.k) {
-    case 1: group<1>(s, rg, op, mats); break;
-    case 2: group<2>(s, rg, op, mats); break;
-    case 3: group<3>(s, rg, op, mats); break;
-    case 4: group<4>(s, rg, op, mats); break;
+    case 1: group<1>(s, m, op, mats); break;
+    case 2: group<2>(s, m, op, mats); break;
+    case 3: group<3>(s, m, op, mats); break;
+    case 4: group<4>(s, m, op, mats); break;
     default: break;
   }
 }
@@ -201,22 +240,22 @@ __device__ void group_any(float2 *s, uint32_t dim, uint32_t c, uint32_t nc, cons
 // aux holds (axis, grad slot) per wire; axis 0/1/2 = X/Y/Z, kNoAxis = none.
 // With `part` the workgroup's sums go to part[j] (one chunk's share) instead
 // of the gradient.
-template <int K>
-__device__ void rotgroup(float2 *psi, float2 *lam, Range rg, const BOp &op, const float *mats,
+template <int K, typename M>
+__device__ void rotgroup(float2 *psi, float2 *lam, M m, const BOp &op, const float *mats,
                          const uint32_t *aux, double *out, double *part,
                          double (*red)[kWaves]) {
   constexpr int U = 8 >> K;  // units per thread per iteration
   double acc[2 * K];
 #pragma unroll
   for (int i = 0; i < 2 * K; i++) acc[i] = 0.0;
-  for (uint32_t u0 = rg.lo + threadIdx.x; u0 < rg.hi; u0 += U * blockDim.x) {
+  for (uint32_t v0 = threadIdx.x; v0 < m.count; v0 += U * blockDim.x) {
     float2 p[U][1 << K], l[U][1 << K];
     uint32_t idx[U][1 << K];
 #pragma unroll
     for (int r = 0; r < U; r++) {
-      const uint32_t u = u0 + r * blockDim.x;
-      if (u >= rg.hi) continue;
-      uint32_t base = u;
+      const uint32_t v = v0 + r * blockDim.x;
+      if (v >= m.count) continue;
+      uint32_t base = m(v);
 #pragma unroll
       for (int j = 0; j < K; j++) base = ins0(base, op.w[j]);
 #pragma unroll
@@ -232,7 +271,7 @@ __device__ void rotgroup(float2 *psi, float2 *lam, Range rg, const BOp &op, cons
     }
 #pragma unroll
     for (int r = 0; r < U; r++) {
-      if (u0 + r * blockDim.x >= rg.hi) continue;
+      if (v0 + r * blockDim.x >= m.count) continue;
 #pragma unroll
       for (int j = 0; j < K; j++) {
         const uint32_t axis = aux[op.c + 2 * j];
@@ -250,9 +289,9 @@ __device__ void rotgroup(float2 *psi, float2 *lam, Range rg, const BOp &op, cons
             acc_cj(acc[2 * j], acc[2 * j + 1], l[r][t], p[r][t], 1.0);
             acc_cj(acc[2 * j], acc[2 * j + 1], l[r][t1], p[r][t1], -1.0);
           }
-          const float *m = mats + op.a + 8 * j;
-          rot2(m, p[r][t], p[r][t1]);
-          rot2(m, l[r][t], l[r][t1]);
+          const float *mt = mats + op.a + 8 * j;
+          rot2(mt, p[r][t], p[r][t1]);
+          rot2(mt, l[r][t], l[r][t1]);
         }
       }
 #pragma unroll
@@ -274,32 +313,34 @@ __device__ void rotgroup(float2 *psi, float2 *lam, Range rg, const BOp &op, cons
     }
 }
 
-__device__ void rotgroup_any(float2 *psi, float2 *lam, uint32_t dim, uint32_t c, uint32_t nc,
-                             const BOp &op, const float *mats, const uint32_t *aux, double *out,
-                             double *part, double (*red)[kWaves]) {
-  const Range rg = chunk_of(dim >> op.k, c, nc);
+template <bool HIGH>
+__device__ void rotgroup_any(float2 *psi, float2 *lam, uint32_t dim, const Geo &g, const BOp &op,
+                             const float *mats, const uint32_t *aux, double *out, double *part,
+                             double (*red)[kWaves]) {
+  const auto m = unit_map<HIGH>(g, dim >> op.k, op.w, op.k);
   switch (op.k) {
-    case 1: rotgroup<1>(psi, lam, rg, op, mats, aux, out, part, red); break;
-    case 2: rotgroup<2>(psi, lam, rg, op, mats, aux, out, part, red); break;
-    case 3: rotgroup<3>(psi, lam, rg, op, mats, aux, out, part, red); break;
+    case 1: rotgroup<1>(psi, lam, m, op, mats, aux, out, part, red); break;
+    case 2: rotgroup<2>(psi, lam, m, op, mats, aux, out, part, red); break;
+    case 3: rotgroup<3>(psi, lam, m, op, mats, aux, out, part, red); break;
     default: break;
   }
 }
 
 constexpr int kElems = 8;  // elements per thread per iteration of the flat passes
 
-__device__ void ctrl(float2 *s, Range rg, const BOp &op) {
-  const uint32_t t = op.w[0], bit = 1u << t, cm = uint32_t(op.mask), half = rg.hi;
-  for (uint32_t u0 = rg.lo + threadIdx.x; u0 < half; u0 += kElems * blockDim.x) {
+template <typename M>
+__device__ void ctrl(float2 *s, M m, const BOp &op) {
+  const uint32_t t = op.w[0], bit = 1u << t, cm = uint32_t(op.mask);
+  for (uint32_t v0 = threadIdx.x; v0 < m.count; v0 += kElems * blockDim.x) {
     float2 x[kElems], y[kElems];
     uint32_t idx[kElems];
     bool on[kElems];
 #pragma unroll
     for (int r = 0; r < kElems; r++) {
-      const uint32_t u = u0 + r * blockDim.x;
-      const uint32_t i = ins0(u, t);
+      const uint32_t v = v0 + r * blockDim.x;
+      const uint32_t i = ins0(m(v), t);
       idx[r] = i;
-      on[r] = u < half && (i & cm) == cm;
+      on[r] = v < m.count && (i & cm) == cm;
       if (on[r]) {
         x[r] = s[i];
         y[r] = s[i | bit];
@@ -315,17 +356,17 @@ __device__ void ctrl(float2 *s, Range rg, const BOp &op) {
   }
 }
 
-__device__ void swap2(float2 *s, Range rg, const BOp &op) {
+template <typename M>
+__device__ void swap2(float2 *s, M m, const BOp &op) {
   const uint32_t a = op.w[0], b = op.w[1], lo = a < b ? a : b, hi = a < b ? b : a;
-  const uint32_t quarter = rg.hi;
-  for (uint32_t u0 = rg.lo + threadIdx.x; u0 < quarter; u0 += kElems * blockDim.x) {
+  for (uint32_t v0 = threadIdx.x; v0 < m.count; v0 += kElems * blockDim.x) {
     float2 x[kElems], y[kElems];
     uint32_t ia[kElems], ib[kElems];
 #pragma unroll
     for (int r = 0; r < kElems; r++) {
-      const uint32_t u = u0 + r * blockDim.x;
-      if (u >= quarter) continue;
-      const uint32_t base = ins0(ins0(u, lo), hi);
+      const uint32_t v = v0 + r * blockDim.x;
+      if (v >= m.count) continue;
+      const uint32_t base = ins0(ins0(m(v), lo), hi);
       ia[r] = base | (1u << a);
       ib[r] = base | (1u << b);
       x[r] = s[ia[r]];
@@ -333,7 +374,7 @@ __device__ void swap2(float2 *s, Range rg, const BOp &op) {
     }
 #pragma unroll
     for (int r = 0; r < kElems; r++) {
-      if (u0 + r * blockDim.x >= quarter) continue;
+      if (v0 + r * blockDim.x >= m.count) continue;
       s[ia[r]] = y[r];
       s[ib[r]] = x[r];
     }
@@ -598,9 +639,34 @@ __device__ void energy(const float2 *psi, const float2 *lam, Range rg, const BOp
 
 // One program op on chunk c of nc of the instance's states; `part` (spread
 // launches) takes the workgroup's reduction sums instead of the outputs.
+// HIGH: a high-chunk step, which holds only groups, rotation groups,
+// controlled 2x2s and swaps.
+template <bool HIGH>
 __device__ void run_op(const BatchArgs &A, const BOp &op, float2 *psi, float2 *lam, uint32_t dim,
-                       uint32_t c, uint32_t nc, double *part, double (*red)[kWaves],
-                       PhaseTab &T) {
+                       const Geo &g, double *part, double (*red)[kWaves], PhaseTab &T) {
+  const uint32_t c = g.c, nc = g.nc;  // flat ops: contiguous chunks only
+  if constexpr (HIGH) {
+    switch (op.code) {
+      case B_GROUP:
+        group_any<true>(psi, dim, g, op, A.mats);
+        if (op.flags & F_BOTH) group_any<true>(lam, dim, g, op, A.mats);
+        break;
+      case B_CTRL:
+        ctrl(psi, unit_map<true>(g, dim / 2, op.w, 1), op);
+        if (op.flags & F_BOTH) ctrl(lam, unit_map<true>(g, dim / 2, op.w, 1), op);
+        break;
+      case B_SWAP:
+        swap2(psi, unit_map<true>(g, dim / 4, op.w, 2), op);
+        if (op.flags & F_BOTH) swap2(lam, unit_map<true>(g, dim / 4, op.w, 2), op);
+        break;
+      case B_ROTGROUP:
+        rotgroup_any<true>(psi, lam, dim, g, op, A.mats, A.aux, A.grad, part, red);
+        break;
+      default:
+        break;
+    }
+    return;
+  }
   switch (op.code) {
     case B_INIT: {
       const Range rg = chunk_of(dim, c, nc);
@@ -609,16 +675,16 @@ __device__ void run_op(const BatchArgs &A, const BOp &op, float2 *psi, float2 *l
       break;
     }
     case B_GROUP:
-      group_any(psi, dim, c, nc, op, A.mats);
-      if (op.flags & F_BOTH) group_any(lam, dim, c, nc, op, A.mats);
+      group_any<false>(psi, dim, g, op, A.mats);
+      if (op.flags & F_BOTH) group_any<false>(lam, dim, g, op, A.mats);
       break;
     case B_CTRL:
-      ctrl(psi, chunk_of(dim / 2, c, nc), op);
-      if (op.flags & F_BOTH) ctrl(lam, chunk_of(dim / 2, c, nc), op);
+      ctrl(psi, unit_map<false>(g, dim / 2, op.w, 1), op);
+      if (op.flags & F_BOTH) ctrl(lam, unit_map<false>(g, dim / 2, op.w, 1), op);
       break;
     case B_SWAP:
-      swap2(psi, chunk_of(dim / 4, c, nc), op);
-      if (op.flags & F_BOTH) swap2(lam, chunk_of(dim / 4, c, nc), op);
+      swap2(psi, unit_map<false>(g, dim / 4, op.w, 2), op);
+      if (op.flags & F_BOTH) swap2(lam, unit_map<false>(g, dim / 4, op.w, 2), op);
       break;
     case B_DIAG:
       diag(psi, chunk_of(dim, c, nc), op, A.terms, T);
@@ -630,7 +696,7 @@ __device__ void run_op(const BatchArgs &A, const BOp &op, float2 *psi, float2 *l
       energy(psi, lam, chunk_of(dim, c, nc), op, A.energy, part, red);
       break;
     case B_ROTGROUP:
-      rotgroup_any(psi, lam, dim, c, nc, op, A.mats, A.aux, A.grad, part, red);
+      rotgroup_any<false>(psi, lam, dim, g, op, A.mats, A.aux, A.grad, part, red);
       break;
     case B_DIAGRUN:
       diagrun(psi, lam, chunk_of(dim, c, nc), op, A.terms, A.grad, part, red, T);
@@ -662,7 +728,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_batch(BatchArgs A) {
       lam = psi + A.slot;
     }
     for (uint32_t o = in.op_begin; o < in.op_end; o++) {
-      run_op(A, A.ops[o], psi, lam, dim, 0, 1, nullptr, red, tab);
+      run_op<false>(A, A.ops[o], psi, lam, dim, Geo{0, 1, 0, false}, nullptr, red, tab);
       __syncthreads();
     }
   }
@@ -685,6 +751,9 @@ __host__ __device__ inline uint32_t op_slots(const BOp &op) {
 // workgroup (the step's ops' sums side by side); k_spread_reduce adds each
 // instance's rows in chunk order (deterministic) into the energies and
 // gradients.
+constexpr uint32_t kStepHigh = 0x80000000u;  // step boundary flag: high chunks
+constexpr uint32_t kStepOps = 0x7FFFFFFFu;
+
 struct SpreadArgs {
   const uint32_t *inst;      // local instance -> instance
   const uint32_t *wg_base;   // local instance -> first workgroup (count + 1 entries)
@@ -698,6 +767,10 @@ struct SpreadArgs {
   uint32_t slots;
 };
 
+// One kernel per chunk kind (in one kernel the two loop nests share a
+// register allocation and spill): the workgroups of instances whose step is
+// of the other kind return at once.
+template <bool HIGH>
 __global__ __launch_bounds__(kMaxThreads) void k_spread(BatchArgs A, SpreadArgs S, uint32_t step) {
   __shared__ double red[kOverlapChunk][kWaves];
   __shared__ PhaseTab tab;
@@ -708,11 +781,26 @@ __global__ __launch_bounds__(kMaxThreads) void k_spread(BatchArgs A, SpreadArgs 
   const uint32_t dim = 1u << in.n;
   float2 *psi = S.scratch + S.state[l];
   double *row = S.part + uint64_t(blockIdx.x) * S.slots;
-  const uint32_t o0 = S.steps[S.step_off[l] + step], o1 = S.steps[S.step_off[l] + step + 1];
+  const uint32_t s0 = S.steps[S.step_off[l] + step], s1 = S.steps[S.step_off[l] + step + 1];
+  const uint32_t o0 = s0 & kStepOps, o1 = s1 & kStepOps;
+  if (((s0 & kStepHigh) != 0) != HIGH) return;
+  Geo g{c, nc, 0, false};
+  if constexpr (HIGH) {
+    // High chunks: 2^(n-14) of them, each 2^14 amplitudes; the other
+    // workgroups of the instance only zero their partial sums.
+    const uint32_t mbits = in.n > kHighLow + 8 ? in.n - kHighLow - 8 : 0;
+    if (c >= (1u << mbits)) {
+      uint32_t ns = 0;
+      for (uint32_t o = o0; o < o1; o++) ns += op_slots(A.ops[o]);
+      for (uint32_t r = threadIdx.x; r < ns; r += blockDim.x) row[r] = 0.0;
+      return;
+    }
+    g = Geo{c, 1u << mbits, mbits, true};
+  }
   for (uint32_t o = o0; o < o1; o++) {
     const BOp op = A.ops[o];
     if (o > o0) __syncthreads();  // the previous op's writes to the chunk
-    run_op(A, op, psi, psi + dim, dim, c, nc, row, red, tab);
+    run_op<HIGH>(A, op, psi, psi + dim, dim, g, row, red, tab);
     row += op_slots(op);
   }
 }
@@ -720,7 +808,8 @@ __global__ __launch_bounds__(kMaxThreads) void k_spread(BatchArgs A, SpreadArgs 
 __global__ __launch_bounds__(64) void k_spread_reduce(BatchArgs A, SpreadArgs S, uint32_t step) {
   const uint32_t l = blockIdx.x;
   if (step >= S.nsteps[l]) return;
-  const uint32_t o0 = S.steps[S.step_off[l] + step], o1 = S.steps[S.step_off[l] + step + 1];
+  const uint32_t o0 = S.steps[S.step_off[l] + step] & kStepOps;
+  const uint32_t o1 = S.steps[S.step_off[l] + step + 1] & kStepOps;
   const uint32_t w0 = S.wg_base[l], w1 = S.wg_base[l + 1];
   uint32_t off = 0;
   for (uint32_t o = o0; o < o1; o++) {
@@ -1073,6 +1162,7 @@ struct SpreadGroup {
   std::vector<uint32_t> step_insts;  // per step: instances still running (a prefix)
   std::vector<uint32_t> step_wgs;    // per step: their workgroups (a prefix)
   std::vector<uint8_t> step_reduce;  // per step: some running op reduces
+  std::vector<uint8_t> step_kinds;   // per step: bit 0 contiguous, bit 1 high chunks
 };
 
 struct Batch::Impl {
@@ -1190,7 +1280,8 @@ struct Batch::Impl {
         steps.insert(steps.end(), bd.begin(), bd.end());
         for (size_t j = 0; j + 1 < bd.size(); j++) {
           uint32_t sl = 0;
-          for (uint32_t o = bd[j]; o < bd[j + 1]; o++) sl += op_slots(h_ops[o]);
+          for (uint32_t o = bd[j] & kStepOps; o < (bd[j + 1] & kStepOps); o++)
+            sl += op_slots(h_ops[o]);
           sp_slots = std::max(sp_slots, sl);
         }
         g.count++;
@@ -1201,16 +1292,20 @@ struct Batch::Impl {
       for (uint32_t j = 0; j < gsteps; j++) {
         uint32_t running = 0;
         bool reduce = false;
+        uint8_t kinds = 0;
         for (uint32_t l = 0; l < g.count; l++) {
           const uint32_t li = g.inst_off + l;
           if (j >= nst[li]) break;  // sorted: the rest have fewer steps
           running++;
-          for (uint32_t o = steps[step_off[li] + j]; o < steps[step_off[li] + j + 1]; o++)
+          kinds |= (steps[step_off[li] + j] & kStepHigh) ? 2 : 1;
+          for (uint32_t o = steps[step_off[li] + j] & kStepOps;
+               o < (steps[step_off[li] + j + 1] & kStepOps); o++)
             reduce = reduce || op_slots(h_ops[o]) > 0;
         }
         g.step_insts.push_back(running);
         g.step_wgs.push_back(base[g.base_off + running]);
         g.step_reduce.push_back(reduce);
+        g.step_kinds.push_back(kinds);
       }
       scratch_max = std::max(scratch_max, elems);
       wgs_max = std::max<uint64_t>(wgs_max, wgs);
@@ -1259,24 +1354,54 @@ struct Batch::Impl {
     }
   }
 
-  // Op boundaries of an instance's spread steps: runs of chunk-local ops (at
-  // most kFuseOps ops and kFuseSlots reduction sums each), other ops alone.
+  // Whether `op` stays inside one high chunk of an n-qubit state (bits
+  // [kHighLow, n - 8) fixed, 2^14 amplitudes): a group, rotation group,
+  // controlled 2x2 or swap whose moved wires lie below kHighLow or among the
+  // top n - 8 .. n - 1.
+  static bool high_local(const BOp &op, uint32_t n) {
+    if (n <= kSpreadChunkBits) return false;
+    auto ok = [&](uint32_t w) { return w < kHighLow || w + 8 >= n; };
+    switch (op.code) {
+      case B_GROUP:
+      case B_ROTGROUP:
+        for (uint32_t j = 0; j < op.k; j++)
+          if (!ok(op.w[j])) return false;
+        return true;
+      case B_CTRL:
+        return ok(op.w[0]);
+      case B_SWAP:
+        return ok(op.w[0]) && ok(op.w[1]);
+      default:
+        return false;
+    }
+  }
+
+  // Step boundaries of an instance's spread program: runs of ops that all
+  // stay inside one contiguous chunk, or all inside one high chunk (flagged
+  // kStepHigh on the run's first boundary), at most kFuseOps ops and
+  // kFuseSlots reduction sums each; other ops alone. An op that fits both
+  // kinds starts a high run only when the next op fits high chunks alone.
   std::vector<uint32_t> spread_steps(const BInst &in) const {
-    std::vector<uint32_t> bd{in.op_begin};
+    std::vector<uint32_t> bd;
     uint32_t run = 0, slots = 0;
-    bool run_local = false;
+    int mode = 0;  // of the open run: 1 contiguous, 2 high, 0 a single op
     for (uint32_t o = in.op_begin; o < in.op_end; o++) {
-      const bool loc = chunk_local(h_ops[o], in.n);
+      const bool lo = chunk_local(h_ops[o], in.n), hi = high_local(h_ops[o], in.n);
       const uint32_t sl = op_slots(h_ops[o]);
-      const bool join = run > 0 && loc && run_local && run < kFuseOps && slots + sl <= kFuseSlots;
-      if (run > 0 && !join) {
-        bd.push_back(o);
-        run = 0;
-        slots = 0;
+      const bool room = run < kFuseOps && slots + sl <= kFuseSlots;
+      if (run > 0 && room && ((mode == 1 && lo) || (mode == 2 && hi))) {
+        run++;
+        slots += sl;
+        continue;
       }
-      run++;
-      slots += sl;
-      run_local = loc;
+      int m = lo ? 1 : hi ? 2 : 0;
+      if (lo && hi && o + 1 < in.op_end && !chunk_local(h_ops[o + 1], in.n) &&
+          high_local(h_ops[o + 1], in.n))
+        m = 2;
+      bd.push_back(o | (m == 2 ? kStepHigh : 0u));
+      mode = m;
+      run = 1;
+      slots = sl;
     }
     bd.push_back(in.op_end);
     return bd;
@@ -1375,7 +1500,8 @@ struct Batch::Impl {
         S.part = sp_part;
         S.slots = sp_slots;
         for (uint32_t j = 0; j < g.step_wgs.size(); j++) {
-          k_spread<<<g.step_wgs[j], kSpreadThreads, 0, ss>>>(A, S, j);
+          if (g.step_kinds[j] & 1) k_spread<false><<<g.step_wgs[j], kSpreadThreads, 0, ss>>>(A, S, j);
+          if (g.step_kinds[j] & 2) k_spread<true><<<g.step_wgs[j], kSpreadThreads, 0, ss>>>(A, S, j);
           if (g.step_reduce[j]) k_spread_reduce<<<g.step_insts[j], 64, 0, ss>>>(A, S, j);
         }
       }
