@@ -35,15 +35,20 @@ constexpr uint64_t kMaxGrid = 1u << 20;
 // step constants) for fewer tiles each.
 constexpr uint64_t kPassGrid = 1u << 16;
 
-// Wide-tile passes (2^14-amplitude tiles): at least two tiles per
-// workgroup, so that each workgroup's set-up is shared (QFT n=30: 3.57 ->
-// 3.09 ms per wide pass), and at most kPassGrid workgroups.
-inline unsigned wide_grid(uint64_t tiles) {
-  uint64_t g = tiles / 2;
+// Workgroups of a pass over `tiles` tiles: `per` tiles per workgroup, so
+// that each workgroup's set-up is shared (QFT n=30: 3.57 -> 3.09 ms per wide
+// pass at two), but one per workgroup while that leaves fewer than
+// kFillGrid, and at most `cap`.
+constexpr uint64_t kFillGrid = 1024;
+inline unsigned tile_grid(uint64_t tiles, uint64_t per, uint64_t cap) {
+  uint64_t g = tiles / per;
+  const uint64_t fill = tiles < kFillGrid ? tiles : kFillGrid;
+  if (g < fill) g = fill;
   if (g < 1) g = 1;
-  if (g > kPassGrid) g = kPassGrid;
+  if (g > cap) g = cap;
   return unsigned(g);
 }
+inline unsigned wide_grid(uint64_t tiles) { return tile_grid(tiles, 2, kPassGrid); }
 
 inline unsigned grid_for(uint64_t work, uint64_t cap = kMaxGrid) {
   uint64_t g = (work + kT - 1) / kT;
@@ -1767,10 +1772,9 @@ constexpr uint64_t kLowGrid = 40960;
 
 template <int S, int WB>
 void launch_low_s(hipStream_t s, bool zero, int mode, const LowArgs &l) {
-  // At least four tiles per workgroup (the set-up evaluates every step's lane
-  // phase terms), at most kLowGrid workgroups' worth of waves.
-  const uint64_t want = l.lay.nthreads / 4 ? l.lay.nthreads / 4 : 1;
-  const unsigned grid = unsigned(std::min<uint64_t>(want, kLowGrid >> (WB - 1)));
+  // Four tiles per workgroup (the set-up evaluates every step's lane phase
+  // terms), at most kLowGrid workgroups' worth of waves.
+  const unsigned grid = tile_grid(l.lay.nthreads, 4, kLowGrid >> (WB - 1));
   constexpr int threads = 64 << WB;
   if (zero) {
     if (mode == 2) k_low<S, true, 2, WB><<<grid, threads, 0, s>>>(l);
