@@ -22,6 +22,9 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("QMETAL_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "simulator.hip", "batch.hip"]
+# low_kernels.hip is built once per pair of step counts (its kernel instances
+# take most of the build; the objects compile in parallel).
+LOW_PAIRS = [5, 7, 9, 11]
 CXX_SOURCES = ["gates.cpp", "plan.cpp", "schedule.cpp", "capi.cpp"]
 
 HIPCC_FLAGS = [
@@ -51,9 +54,16 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(OUT, src + ".o")
         jobs.append((obj, [os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, *inc, "-c",
                            os.path.join(CSRC, src), "-o", obj]))
+    for s0 in LOW_PAIRS:
+        obj = os.path.join(OUT, f"low_kernels.{s0}.o")
+        jobs.append((obj, [os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, *inc,
+                           f"-DQM_LOW_S0={s0}", "-c", os.path.join(CSRC, "low_kernels.hip"),
+                           "-o", obj]))
     for src in CXX_SOURCES:
         obj = os.path.join(OUT, src + ".o")
         jobs.append((obj, ["g++", *CXX_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj]))
+    # Longest first: the low kernel objects, then the rest.
+    jobs.sort(key=lambda j: "low_kernels" not in j[0])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for (obj, cmd), _ in zip(jobs, ex.map(lambda j: _run(j[1]), jobs)):
             if verbose:

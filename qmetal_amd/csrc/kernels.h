@@ -273,6 +273,9 @@ void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const Chai
 void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w);
 // s in 5..12 steps on wires 1..10+wb, `zero`: a last step on wire 0.
 void launch_low(hipStream_t s, int steps, bool zero, int mode, int wb, const LowArgs &l);
+// The low chain kernels for steps S0 and S0 + 1 (S0 = 5, 7, 9, 11).
+template <int S0>
+void launch_low_pair(hipStream_t s, int steps, bool zero, int mode, int wb, const LowArgs &l);
 void launch_tile(hipStream_t s, const TileArgs &t);
 // wb = 2 or 3 wave bits.
 void launch_gwide(hipStream_t s, int wb, const GWideArgs &g);

@@ -23,234 +23,10 @@
 #include <algorithm>
 
 #include "kernels.h"
+#include "kernel_util.h"
 
 namespace qmamd {
 namespace {
-
-constexpr int kT = 256;             // threads per workgroup
-constexpr uint64_t kMaxGrid = 1u << 20;
-
-// Workgroups of a merged pass: enough to fill the chip many times over;
-// more would only repeat the per-workgroup set-up (the lane phase terms, the
-// step constants) for fewer tiles each.
-constexpr uint64_t kPassGrid = 1u << 16;
-
-// Workgroups of a pass over `tiles` tiles: `per` tiles per workgroup, so
-// that each workgroup's set-up is shared (QFT n=30: 3.57 -> 3.09 ms per wide
-// pass at two), but one per workgroup while that leaves fewer than
-// kFillGrid, and at most `cap`.
-constexpr uint64_t kFillGrid = 1024;
-inline unsigned tile_grid(uint64_t tiles, uint64_t per, uint64_t cap) {
-  uint64_t g = tiles / per;
-  const uint64_t fill = tiles < kFillGrid ? tiles : kFillGrid;
-  if (g < fill) g = fill;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return unsigned(g);
-}
-inline unsigned wide_grid(uint64_t tiles) { return tile_grid(tiles, 2, kPassGrid); }
-
-inline unsigned grid_for(uint64_t work, uint64_t cap = kMaxGrid) {
-  uint64_t g = (work + kT - 1) / kT;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return unsigned(g);
-}
-
-// Thread index -> base index without the fixed bits: zero bits inserted at
-// the layout's positions, lowest first.
-__device__ __forceinline__ uint64_t scatter(uint64_t t, const Layout &L) {
-  for (uint32_t k = 0; k < L.nins; k++) t += t & L.hm[k];
-  return t;
-}
-
-// Base index of thread-index value t0 + tid with t0 a multiple of the
-// workgroup size: scatter is a bit placement, so the workgroup-uniform part
-// scatter(t0) | fixed and the per-thread part scatter(tid) are disjoint. The
-// per-thread part stays below 2^24 (the workgroup's index bits land on the
-// lowest free positions), so a thread addresses its amplitudes as a uniform
-// 64-bit pointer plus one 32-bit byte offset.
-struct TileAddr {
-  uint32_t lane_bytes;  // 8 * scatter(tid)
-  uint64_t lane_bits;   // scatter(tid)
-};
-
-__device__ __forceinline__ TileAddr tile_addr(const Layout &L) {
-  TileAddr a;
-  a.lane_bits = scatter(threadIdx.x, L);
-  a.lane_bytes = uint32_t(a.lane_bits) * 8u;
-  return a;
-}
-
-// Amplitude pair (ub | o) + the thread's part, as a 16-byte address.
-__device__ __forceinline__ const float2 *pair_ptr(const float2 *a, uint64_t ubo,
-                                                  uint32_t lane_bytes) {
-  return reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(a + ubo) + lane_bytes);
-}
-__device__ __forceinline__ float2 *pair_ptr(float2 *a, uint64_t ubo, uint32_t lane_bytes) {
-  return reinterpret_cast<float2 *>(reinterpret_cast<char *>(a + ubo) + lane_bytes);
-}
-
-template <int NL>
-__device__ __forceinline__ void offsets(const Layout &L, uint64_t (&off)[1 << (NL - 1)]) {
-#pragma unroll
-  for (int r = 0; r < (1 << (NL - 1)); r++) {
-    uint64_t o = 0;
-#pragma unroll
-    for (int k = 0; k < NL - 1; k++)
-      if ((r >> k) & 1) o |= 1ull << L.rpos[k];
-    off[r] = o;
-  }
-}
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// x' = m00 x + m01 y, y' = m10 x + m11 y
-__device__ __forceinline__ void rot(const float *m, float2 &x, float2 &y) {
-  const float2 u = x, v = y;
-  x.x = m[0] * u.x - m[1] * u.y + m[2] * v.x - m[3] * v.y;
-  x.y = m[0] * u.y + m[1] * u.x + m[2] * v.y + m[3] * v.x;
-  y.x = m[4] * u.x - m[5] * u.y + m[6] * v.x - m[7] * v.y;
-  y.y = m[4] * u.y + m[5] * u.x + m[6] * v.y + m[7] * v.x;
-}
-
-// a *= f in place (f per lane) unless `skip` (workgroup-uniform, in an SGPR)
-// is set; the branch is inside the statement, so a is defined once either way.
-// Complex products in place: (ax, ay) <- (fx ax - fy ay, fx ay + fy ax), the
-// cross products into temporaries first, then one fused multiply-add per
-// component written in place (no moves).
-__device__ __forceinline__ void cmul_inplace(float2 &a, float2 f, uint32_t skip) {
-  float t, u;
-  asm("s_cmp_lg_u32 %[skip], 0\n\t"
-      "s_cbranch_scc1 .Lqm%=\n\t"
-      "v_mul_f32 %[t], %[fy], %[ax]\n\t"
-      "v_mul_f32 %[u], %[fy], %[ay]\n\t"
-      "v_fma_f32 %[ax], %[fx], %[ax], -%[u]\n\t"
-      "v_fma_f32 %[ay], %[fx], %[ay], %[t]\n"
-      ".Lqm%=:"
-      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t), [u] "=&v"(u)
-      : [fx] "v"(f.x), [fy] "v"(f.y), [skip] "s"(skip)
-      : "scc");
-}
-
-#include "slot_asm.inc"
-
-// In-place updates with tied operands for straight-line code (the chain
-// kernel): every amplitude keeps its registers, so the compiler cannot hold
-// a step's results and its inputs at the same time. The matrix / factor
-// operands are workgroup-uniform (SGPRs) unless the name ends in _v.
-__device__ __forceinline__ void bfly_real_s(float m00, float m01, float m10, float m11,
-                                            float2 &a, float2 &b) {
-  float t0, t1, t2, t3;
-  asm("v_mul_f32 %[t0], %[m10], %[ax]\n\t"
-      "v_mul_f32 %[t1], %[m01], %[bx]\n\t"
-      "v_mul_f32 %[t2], %[m10], %[ay]\n\t"
-      "v_mul_f32 %[t3], %[m01], %[by]\n\t"
-      "v_fma_f32 %[bx], %[m11], %[bx], %[t0]\n\t"
-      "v_fma_f32 %[ax], %[m00], %[ax], %[t1]\n\t"
-      "v_fma_f32 %[by], %[m11], %[by], %[t2]\n\t"
-      "v_fma_f32 %[ay], %[m00], %[ay], %[t3]"
-      : [ax] "+v"(a.x), [ay] "+v"(a.y), [bx] "+v"(b.x), [by] "+v"(b.y), [t0] "=&v"(t0),
-        [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
-      : [m00] "s"(m00), [m01] "s"(m01), [m10] "s"(m10), [m11] "s"(m11));
-}
-
-__device__ __forceinline__ void bfly_cplx_s(const float *m, float2 &a, float2 &b) {
-  float t0, t1, t2, t3;
-  asm("v_mul_f32 %[t0], %[m0i], %[ay]\n\t"
-      "v_fma_f32 %[t0], %[m1r], %[bx], -%[t0]\n\t"
-      "v_fma_f32 %[t0], -%[m1i], %[by], %[t0]\n\t"
-      "v_mul_f32 %[t1], %[m0i], %[ax]\n\t"
-      "v_fmac_f32 %[t1], %[m1r], %[by]\n\t"
-      "v_fmac_f32 %[t1], %[m1i], %[bx]\n\t"
-      "v_mul_f32 %[t2], %[m2r], %[ax]\n\t"
-      "v_fma_f32 %[t2], -%[m2i], %[ay], %[t2]\n\t"
-      "v_fma_f32 %[t2], -%[m3i], %[by], %[t2]\n\t"
-      "v_mul_f32 %[t3], %[m2r], %[ay]\n\t"
-      "v_fmac_f32 %[t3], %[m2i], %[ax]\n\t"
-      "v_fmac_f32 %[t3], %[m3i], %[bx]\n\t"
-      "v_fma_f32 %[ax], %[m0r], %[ax], %[t0]\n\t"
-      "v_fma_f32 %[ay], %[m0r], %[ay], %[t1]\n\t"
-      "v_fma_f32 %[bx], %[m3r], %[bx], %[t2]\n\t"
-      "v_fma_f32 %[by], %[m3r], %[by], %[t3]"
-      : [ax] "+v"(a.x), [ay] "+v"(a.y), [bx] "+v"(b.x), [by] "+v"(b.y), [t0] "=&v"(t0),
-        [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
-      : [m0r] "s"(m[0]), [m0i] "s"(m[1]), [m1r] "s"(m[2]), [m1i] "s"(m[3]), [m2r] "s"(m[4]),
-        [m2i] "s"(m[5]), [m3r] "s"(m[6]), [m3i] "s"(m[7]));
-}
-
-// a *= f in place.
-__device__ __forceinline__ void cmul_s(float2 &a, float fx, float fy) {
-  float t, u;
-  asm("v_mul_f32 %[t], %[fy], %[ax]\n\t"
-      "v_mul_f32 %[u], %[fy], %[ay]\n\t"
-      "v_fma_f32 %[ax], %[fx], %[ax], -%[u]\n\t"
-      "v_fma_f32 %[ay], %[fx], %[ay], %[t]"
-      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t), [u] "=&v"(u)
-      : [fx] "s"(fx), [fy] "s"(fy));
-}
-
-__device__ __forceinline__ void cmul_v(float2 &a, float2 f) {
-  float t, u;
-  asm("v_mul_f32 %[t], %[fy], %[ax]\n\t"
-      "v_mul_f32 %[u], %[fy], %[ay]\n\t"
-      "v_fma_f32 %[ax], %[fx], %[ax], -%[u]\n\t"
-      "v_fma_f32 %[ay], %[fx], %[ay], %[t]"
-      : [ax] "+v"(a.x), [ay] "+v"(a.y), [t] "=&v"(t), [u] "=&v"(u)
-      : [fx] "v"(f.x), [fy] "v"(f.y));
-}
-
-template <int NL, int LB>
-__device__ __forceinline__ void apply_slot(float2 (&x)[1 << NL], uint32_t cm, const float *m) {
-#pragma unroll
-  for (int l = 0; l < (1 << NL); l++) {
-    if (l & (1 << LB)) continue;
-    if ((uint32_t(l) & cm) != cm) continue;
-    rot(m, x[l], x[l | (1 << LB)]);
-  }
-}
-
-// 2x2 `m` on tile-local bit `lbit` wherever the tile-local bits `cm` are 1.
-template <int NL>
-__device__ __forceinline__ void apply_any(float2 (&x)[1 << NL], uint32_t lbit, uint32_t cm,
-                                          const float *m) {
-  switch (lbit) {
-    case 0: apply_slot<NL, 0>(x, cm, m); break;
-    case 1: if constexpr (NL > 1) apply_slot<NL, 1>(x, cm, m); break;
-    case 2: if constexpr (NL > 2) apply_slot<NL, 2>(x, cm, m); break;
-    case 3: if constexpr (NL > 3) apply_slot<NL, 3>(x, cm, m); break;
-    case 4: if constexpr (NL > 4) apply_slot<NL, 4>(x, cm, m); break;
-    default: break;
-  }
-}
-
-template <int NL>
-__device__ __forceinline__ void apply_any(float2 (&x)[1 << NL], const Slot &s) {
-  apply_any<NL>(x, s.lbit, s.cmask, s.m);
-}
-
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <bool NT>
-__device__ __forceinline__ float4 ld4(const float2 *p) {
-  if constexpr (NT) {
-    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
-    return make_float4(v.x, v.y, v.z, v.w);
-  }
-  return *reinterpret_cast<const float4 *>(p);
-}
-
-template <bool NT>
-__device__ __forceinline__ void st4(float2 *p, float4 v) {
-  if constexpr (NT) {
-    const v4f w = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(w, reinterpret_cast<v4f *>(p));
-  } else {
-    *reinterpret_cast<float4 *>(p) = v;
-  }
-}
 
 template <int NL, bool NT>
 __device__ __forceinline__ void gather_one(float2 *__restrict__ a, uint64_t ub,
@@ -283,22 +59,6 @@ __global__ __launch_bounds__(kT) void k_gather(GatherArgs g) {
     gather_one<NL, NT>(g.a, ub, ta.lane_bytes, off, g);
     if (g.b) gather_one<NL, NT>(g.b, ub, ta.lane_bytes, off, g);
   }
-}
-
-// sin and cos of 2 pi t for a phase t in turns: t is reduced to [-1/2, 1/2]
-// in FP64, then the hardware transcendental unit, which takes its argument in
-// turns, evaluates both (v_sin_f32 / v_cos_f32, no range reduction needed).
-__device__ __forceinline__ void turn_sincos(double t, float &s, float &c) {
-  const float r = float(t - rint(t));
-  s = __builtin_amdgcn_sinf(r);
-  c = __builtin_amdgcn_cosf(r);
-}
-
-// Sum over the wave; every lane gets the total (butterfly).
-__device__ __forceinline__ double wave_allsum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
 }
 
 template <int NL, bool NT>
@@ -390,17 +150,6 @@ __global__ __launch_bounds__(kT) void k_diag_generic(DiagGenericArgs d) {
     turn_sincos(ph, s, c);
     d.a[i] = cmul(d.a[i], make_float2(c, s));
   }
-}
-
-// Read-only kernel inputs seen through the constant address space: uniform
-// loads through such a pointer become scalar loads into SGPRs (the compiler
-// need not prove that the state stores of the kernel do not alias them).
-template <typename T>
-using const_ptr = const __attribute__((address_space(4))) T *;
-
-template <typename T>
-__device__ __forceinline__ const_ptr<T> as_const(const T *p) {
-  return (const_ptr<T>)p;
 }
 
 // Ops [o0, o1) of a merged pass applied to the thread's 2^NL amplitudes:
@@ -576,58 +325,6 @@ __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
     }
   }
-}
-
-// One chain step on tile-local bit BIT: the 2x2 (m) on all pairs, then the
-// phase of the combinations holding BIT, e * lo[low two other bits] *
-// hi[the other bits above them] (other local bits ascending).
-// MODE: 0 complex 2x2, 1 real 2x2, 2 the unscaled butterfly [[1,1],[1,-1]]
-// (the scale of a run of such steps is carried by the run's last step,
-// which uses mode 1).
-template <int NL, int BIT, int MODE>
-__device__ __forceinline__ void chain_step(float2 (&x)[1 << NL],
-                                           const ChainStep __attribute__((address_space(4))) &st,
-                                           float2 e) {
-  if constexpr (MODE == 2) {
-#pragma unroll
-    for (int l = 0; l < (1 << NL); l++) {
-      if (l & (1 << BIT)) continue;
-      const float2 u = x[l], v = x[l | (1 << BIT)];
-      x[l] = make_float2(u.x + v.x, u.y + v.y);
-      x[l | (1 << BIT)] = make_float2(u.x - v.x, u.y - v.y);
-    }
-  } else {
-    float m[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) m[j] = st.m[j];
-#pragma unroll
-    for (int l = 0; l < (1 << NL); l++) {
-      if (l & (1 << BIT)) continue;
-      if constexpr (MODE == 1) bfly_real_s(m[0], m[2], m[4], m[6], x[l], x[l | (1 << BIT)]);
-      else bfly_cplx_s(m, x[l], x[l | (1 << BIT)]);
-    }
-  }
-  float2 a[4];
-  a[0] = e;
-#pragma unroll
-  for (int h = 1; h < 4; h++) a[h] = cmul(e, make_float2(st.hi[2 * h - 2], st.hi[2 * h - 1]));
-#pragma unroll
-  for (int l = 0; l < (1 << NL); l++) {
-    if (!(l & (1 << BIT))) continue;
-    const int c = (l & ((1 << BIT) - 1)) | ((l >> (BIT + 1)) << BIT);
-    const int lo = c & 3, hi = c >> 2;
-    if (lo) cmul_s(x[l], st.lo[2 * lo - 2], st.lo[2 * lo - 1]);
-    cmul_v(x[l], a[hi]);
-  }
-}
-
-// Step index i read through an opaque register after the tile's last update,
-// so the compiler cannot hoist every step's scalar constants to the top (it
-// would spill SGPRs).
-__device__ __forceinline__ uint32_t opaque_step(int i, float dep) {
-  uint32_t si = uint32_t(i);
-  asm volatile("" : "+s"(si) : "v"(dep));
-  return si;
 }
 
 // Chain-shaped merged pass (ChainStep): every step's local bit, and which
@@ -830,192 +527,6 @@ __global__ __launch_bounds__(64 << WB) void k_wide(WideArgs p) {
       st4<true>(pair_ptr(p.a, o, lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
     }
-  }
-}
-
-// LDS slot of tile index i (float4 units, up to 12 bits): the low four bits are
-// xored with the higher ones so that lanes stepping through high bits still
-// spread over the banks (a bijection: the high bits are kept; linear over
-// GF(2)).
-__device__ __forceinline__ uint32_t low_swz(uint32_t i) { return tile_swizzle(i); }
-
-// Tile index (float4 units: tile bit p is index bit p - 1) of this thread's
-// lane and wave bits in arrangement `a`.
-template <int WB>
-__device__ __forceinline__ uint32_t low_thread_index(const LowArgs &p, int a, uint32_t lane,
-                                                     uint32_t wave) {
-  uint32_t i = 0;
-#pragma unroll
-  for (int j = 0; j < 6; j++)
-    if ((lane >> j) & 1) i |= 1u << (p.lanepos[a][j] - 1);
-#pragma unroll
-  for (int k = 0; k < WB; k++)
-    if ((wave >> k) & 1) i |= 1u << (p.lanepos[a][6 + k] - 1);
-  return i;
-}
-
-// Moves the tile from arrangement FROM to arrangement TO through LDS. The
-// swizzle is linear and thread and register bits are disjoint, so a slot is
-// the thread's swizzled index xor the xor of the swizzled register bits
-// (workgroup-uniform). Byte offsets are kept opaque so that the 32 slot
-// addresses are formed here (one xor each) instead of being hoisted out of
-// the tile loop into registers.
-template <int FROM, int TO, int WB>
-__device__ __forceinline__ void low_permute(const LowArgs &p, float4 *lds, float2 (&x)[32],
-                                            uint32_t lane, uint32_t wave) {
-  uint32_t lf = low_swz(low_thread_index<WB>(p, FROM, lane, wave)) << 4;
-  uint32_t lt = low_swz(low_thread_index<WB>(p, TO, lane, wave)) << 4;
-  uint32_t cf[4], ct[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    cf[k] = low_swz(1u << (p.regpos[FROM][k] - 1)) << 4;
-    ct[k] = low_swz(1u << (p.regpos[TO][k] - 1)) << 4;
-  }
-  asm volatile("" : "+v"(lf), "+v"(lt) : "v"(x[0].x));
-  asm volatile("" : "+s"(cf[0]), "+s"(cf[1]), "+s"(cf[2]), "+s"(cf[3]), "+s"(ct[0]), "+s"(ct[1]),
-               "+s"(ct[2]), "+s"(ct[3]));
-  char *base = reinterpret_cast<char *>(lds);
-  __syncthreads();  // earlier reads of the buffer are done
-#pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const uint32_t o = ((r & 1) ? cf[0] : 0u) ^ ((r & 2) ? cf[1] : 0u) ^ ((r & 4) ? cf[2] : 0u) ^
-                       ((r & 8) ? cf[3] : 0u);
-    *reinterpret_cast<float4 *>(base + (lf ^ o)) =
-        make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const uint32_t o = ((r & 1) ? ct[0] : 0u) ^ ((r & 2) ? ct[1] : 0u) ^ ((r & 4) ? ct[2] : 0u) ^
-                       ((r & 8) ? ct[3] : 0u);
-    const float4 v = *reinterpret_cast<const float4 *>(base + (lt ^ o));
-    x[2 * r] = make_float2(v.x, v.y);
-    x[2 * r + 1] = make_float2(v.z, v.w);
-  }
-}
-
-// Steps I.. of a low chain pass as straight-line code (template recursion,
-// so that every step index is a compile-time constant however long the
-// chain): step i runs in arrangement 1 + i / 4 on register bit 1 + i % 4,
-// the wire-0 step (i == S) on bit 0 of the last arrangement.
-template <int I, int S, int T, int MODE, int WB>
-__device__ __forceinline__ void low_steps(const LowArgs &p, float4 *lds, float2 (&x)[32],
-                                          const float2 (&e)[T], const_ptr<ChainStep> steps,
-                                          uint32_t lane, uint32_t wave) {
-  if constexpr (I < T) {
-    if constexpr (I == 4 && S > 4) low_permute<1, 2, WB>(p, lds, x, lane, wave);
-    if constexpr (I == 8 && S > 8) low_permute<2, 3, WB>(p, lds, x, lane, wave);
-    const ChainStep __attribute__((address_space(4))) &st = steps[opaque_step(I, x[0].x)];
-    constexpr int M = (MODE == 2 && I == T - 1) ? 1 : MODE;
-    if constexpr (I >= S) chain_step<5, 0, M>(x, st, e[I]);
-    else chain_step<5, 1 + I % 4, M>(x, st, e[I]);
-    low_steps<I + 1, S, T, MODE, WB>(p, lds, x, e, steps, lane, wave);
-  }
-}
-
-// Low chain pass (LowArgs): 2^WB waves per workgroup tile of 2^(11+WB)
-// amplitudes, 2^(15+WB) bytes of LDS. The next tile's loads are issued
-// before the current tile is worked on.
-template <int S, bool Z, int MODE, int WB>
-__global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
-  constexpr int T = S + (Z ? 1 : 0);
-  constexpr int A = (S + 3) / 4;  // step arrangements
-  static_assert(S >= 1 && A + 1 <= kLowMaxArr, "low chain shape");
-  __shared__ float4 lds[1024 << WB];
-  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const const_ptr<ChainStep> steps = as_const(p.steps);
-  const const_ptr<DiagTerm> uni = as_const(p.uni);
-  const const_ptr<DiagTerm> lane_terms = as_const(p.lane);
-  // Each step's factor exp(2 pi i beta): lane terms read only thread bits,
-  // which are the same for every tile, so without workgroup-uniform terms
-  // the factors are computed once.
-  auto lane_beta = [&](double (&b)[T]) {
-#pragma unroll
-    for (int i = 0; i < T; i++) b[i] = steps[i].beta0;
-    for (uint32_t k = 0; k < p.nlane; k++) {
-      const uint64_t mask = lane_terms[k].mask;
-      const uint32_t st = lane_terms[k].k;
-      const int a = 1 + (st < uint32_t(S) ? int(st) / 4 : A - 1);
-      uint64_t bs = 0;
-#pragma unroll
-      for (int v = 0; v < WB; v++)
-        if ((wave >> v) & 1) bs |= 1ull << p.lanepos[a][6 + v];
-#pragma unroll
-      for (int j = 0; j < 6; j++)
-        if ((lane >> j) & 1) bs |= 1ull << p.lanepos[a][j];
-      const double add = (bs & mask) == mask ? double(lane_terms[k].ang) : 0.0;
-#pragma unroll
-      for (int i = 0; i < T; i++)
-        if (st == uint32_t(i)) b[i] += add;
-    }
-  };
-  float2 e[T];
-  {
-    double b[T];
-    lane_beta(b);
-#pragma unroll
-    for (int i = 0; i < T; i++) {
-      float sn, cs;
-      turn_sincos(b[i], sn, cs);
-      e[i] = make_float2(cs, sn);
-    }
-  }
-  // Arrangement 0: lanes on bits 1..6, the wave bits on 7.., the registers
-  // on the four bits above them.
-  const uint32_t lane_bytes = lane * 16u;
-  const uint64_t wave_off = uint64_t(wave) << 7;
-  float4 y[16];
-  if (blockIdx.x < p.lay.nthreads) {
-    const uint64_t ub0 = scatter(blockIdx.x, p.lay) | p.lay.fixed | wave_off;
-#pragma unroll
-    for (int r = 0; r < 16; r++)
-      y[r] = ld4<true>(pair_ptr(p.a, ub0 | (uint64_t(r) << (7 + WB)), lane_bytes));
-  }
-  for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
-    const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
-    float2 x[32];
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      x[2 * r] = make_float2(y[r].x, y[r].y);
-      x[2 * r + 1] = make_float2(y[r].z, y[r].w);
-    }
-    if (tile + gridDim.x < p.lay.nthreads) {
-      const uint64_t ubn = scatter(tile + gridDim.x, p.lay) | p.lay.fixed | wave_off;
-#pragma unroll
-      for (int r = 0; r < 16; r++)
-        y[r] = ld4<true>(pair_ptr(p.a, ubn | (uint64_t(r) << (7 + WB)), lane_bytes));
-    }
-    if (p.nuni) {
-      // Workgroup-uniform terms, one per lane (branch-free: lanes past the
-      // end read the last term and add zero).
-      double bu[T];
-#pragma unroll
-      for (int i = 0; i < T; i++) bu[i] = 0.0;
-      for (uint32_t k0 = 0; k0 < p.nuni; k0 += 64) {
-        const uint32_t k = k0 + lane;
-        const uint32_t kc = k < p.nuni ? k : p.nuni - 1;
-        const uint64_t mask = uni[kc].mask;
-        const double add = (k < p.nuni && (ub & mask) == mask) ? double(uni[kc].ang) : 0.0;
-        const uint32_t st = uni[kc].k;
-#pragma unroll
-        for (int i = 0; i < T; i++) bu[i] += st == uint32_t(i) ? add : 0.0;
-      }
-      double b[T];
-      lane_beta(b);
-#pragma unroll
-      for (int i = 0; i < T; i++) {
-        float sn, cs;
-        turn_sincos(b[i] + wave_allsum(bu[i]), sn, cs);
-        e[i] = make_float2(cs, sn);
-      }
-    }
-    low_permute<0, 1, WB>(p, lds, x, lane, wave);
-    low_steps<0, S, T, MODE, WB>(p, lds, x, e, steps, lane, wave);
-    low_permute<A, 0, WB>(p, lds, x, lane, wave);
-#pragma unroll
-    for (int r = 0; r < 16; r++)
-      st4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << (7 + WB)), lane_bytes),
-                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
   }
 }
 
@@ -1768,41 +1279,15 @@ void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w) {
 
 // Workgroups of a low pass: enough to fill the chip many times over, few
 // enough that the one spare prefetch per workgroup stays negligible.
-constexpr uint64_t kLowGrid = 40960;
-
-template <int S, int WB>
-void launch_low_s(hipStream_t s, bool zero, int mode, const LowArgs &l) {
-  // Four tiles per workgroup (the set-up evaluates every step's lane phase
-  // terms), at most kLowGrid workgroups' worth of waves.
-  const unsigned grid = tile_grid(l.lay.nthreads, 4, kLowGrid >> (WB - 1));
-  constexpr int threads = 64 << WB;
-  if (zero) {
-    if (mode == 2) k_low<S, true, 2, WB><<<grid, threads, 0, s>>>(l);
-    else if (mode == 1) k_low<S, true, 1, WB><<<grid, threads, 0, s>>>(l);
-    else k_low<S, true, 0, WB><<<grid, threads, 0, s>>>(l);
-  } else {
-    if (mode == 2) k_low<S, false, 2, WB><<<grid, threads, 0, s>>>(l);
-    else if (mode == 1) k_low<S, false, 1, WB><<<grid, threads, 0, s>>>(l);
-    else k_low<S, false, 0, WB><<<grid, threads, 0, s>>>(l);
-  }
-}
-
-template <int S>
-void launch_low_w(hipStream_t s, bool zero, int mode, int wb, const LowArgs &l) {
-  if (wb == 2) launch_low_s<S, 2>(s, zero, mode, l);
-  else launch_low_s<S, 1>(s, zero, mode, l);
-}
 
 void launch_low(hipStream_t s, int steps, bool zero, int mode, int wb, const LowArgs &l) {
+  // The low chain kernels are built in low_kernels.hip, a pair of step
+  // counts per object (they take most of the build).
   switch (steps) {
-    case 5: launch_low_w<5>(s, zero, mode, wb, l); break;
-    case 6: launch_low_w<6>(s, zero, mode, wb, l); break;
-    case 7: launch_low_w<7>(s, zero, mode, wb, l); break;
-    case 8: launch_low_w<8>(s, zero, mode, wb, l); break;
-    case 9: launch_low_w<9>(s, zero, mode, wb, l); break;
-    case 10: launch_low_w<10>(s, zero, mode, wb, l); break;
-    case 11: launch_low_w<11>(s, zero, mode, wb, l); break;
-    case 12: launch_low_w<12>(s, zero, mode, wb, l); break;
+    case 5: case 6: launch_low_pair<5>(s, steps, zero, mode, wb, l); break;
+    case 7: case 8: launch_low_pair<7>(s, steps, zero, mode, wb, l); break;
+    case 9: case 10: launch_low_pair<9>(s, steps, zero, mode, wb, l); break;
+    case 11: case 12: launch_low_pair<11>(s, steps, zero, mode, wb, l); break;
     default: break;
   }
 }
