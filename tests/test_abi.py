@@ -88,6 +88,12 @@ def test_null_handles():
     lib.qm_destroy(None)
     lib.qm_circuit_destroy.argtypes = [ctypes.c_void_p]
     lib.qm_circuit_destroy(None)
+    lib.qm_batch_spread.restype = ctypes.c_uint32
+    lib.qm_batch_spread.argtypes = [ctypes.c_void_p]
+    assert lib.qm_batch_spread(None) == 0
+    lib.qm_batch_set_spread.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    assert lib.qm_batch_set_spread(None, 14) == -1
+    assert qm._err() == "null batch"
 
 
 def test_oversized_simulator_raises():
