@@ -210,6 +210,9 @@ struct LowArgs {
   const DiagTerm *lane;
   uint32_t nuni;
   uint32_t nlane;
+  // Arrangement 1 is arrangement 0 (the first four steps' wires are its
+  // register bits, in step order): no permute before the first steps.
+  uint32_t first_in_place;
 };
 
 // Generic low tile pass: merged-pass ops on wires 0..11 over workgroup tiles

@@ -141,7 +141,14 @@ __global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
     }
   }
   // Arrangement 0: lanes on bits 1..6, the wave bits on 7.., the registers
-  // on the four bits above them.
+  // on the four bits above them (register bit k on bit regpos[0][k]).
+  uint64_t rbit[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) rbit[k] = 1ull << p.regpos[0][k];
+  auto roff = [&](int r) {
+    return ((r & 1) ? rbit[0] : 0ull) | ((r & 2) ? rbit[1] : 0ull) | ((r & 4) ? rbit[2] : 0ull) |
+           ((r & 8) ? rbit[3] : 0ull);
+  };
   const uint32_t lane_bytes = lane * 16u;
   const uint64_t wave_off = uint64_t(wave) << 7;
   float4 y[16];
@@ -149,7 +156,7 @@ __global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
     const uint64_t ub0 = scatter(blockIdx.x, p.lay) | p.lay.fixed | wave_off;
 #pragma unroll
     for (int r = 0; r < 16; r++)
-      y[r] = ld4<true>(pair_ptr(p.a, ub0 | (uint64_t(r) << (7 + WB)), lane_bytes));
+      y[r] = ld4<true>(pair_ptr(p.a, ub0 | roff(r), lane_bytes));
   }
   for (uint64_t tile = blockIdx.x; tile < p.lay.nthreads; tile += gridDim.x) {
     const uint64_t ub = scatter(tile, p.lay) | p.lay.fixed;
@@ -163,7 +170,7 @@ __global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
       const uint64_t ubn = scatter(tile + gridDim.x, p.lay) | p.lay.fixed | wave_off;
 #pragma unroll
       for (int r = 0; r < 16; r++)
-        y[r] = ld4<true>(pair_ptr(p.a, ubn | (uint64_t(r) << (7 + WB)), lane_bytes));
+        y[r] = ld4<true>(pair_ptr(p.a, ubn | roff(r), lane_bytes));
     }
     if (p.nuni) {
       // Workgroup-uniform terms, one per lane (branch-free: lanes past the
@@ -189,12 +196,12 @@ __global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
         e[i] = make_float2(cs, sn);
       }
     }
-    low_permute<0, 1, WB>(p, lds, x, lane, wave);
+    if (!p.first_in_place) low_permute<0, 1, WB>(p, lds, x, lane, wave);
     low_steps<0, S, T, MODE, WB>(p, lds, x, e, steps, lane, wave);
     low_permute<A, 0, WB>(p, lds, x, lane, wave);
 #pragma unroll
     for (int r = 0; r < 16; r++)
-      st4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << (7 + WB)), lane_bytes),
+      st4<true>(pair_ptr(p.a, ub | wave_off | roff(r), lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
   }
 }

@@ -890,6 +890,21 @@ struct Simulator::Impl {
         }
       if (nl != 6 + WB) fail("internal: low pass lanes");
     }
+    // When the first four steps' wires are arrangement 0's register bits,
+    // arrangement 0 takes their order and serves the first steps in place.
+    if (S >= 4) {
+      uint32_t m0 = 0, m1 = 0;
+      for (int k = 0; k < 4; k++) {
+        m0 |= 1u << la.regpos[0][k];
+        m1 |= 1u << la.regpos[1][k];
+      }
+      bool same_lanes = true;
+      for (int j = 0; j < 6 + WB; j++) same_lanes = same_lanes && la.lanepos[0][j] == la.lanepos[1][j];
+      if (m0 == m1 && same_lanes) {
+        for (int k = 0; k < 4; k++) la.regpos[0][k] = la.regpos[1][k];
+        la.first_in_place = 1;
+      }
+    }
     Layout &L = la.lay;
     L = Layout{};
     for (uint32_t q = 0; q <= top; q++) {

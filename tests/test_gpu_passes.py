@@ -327,12 +327,16 @@ def low_stream(n, rounds, seed, kind, top=11):
     a step on wire 0, each step a 2x2 (step_gate) followed by
     phase gates that all hold its wire (onto register, lane, wave and
     workgroup-uniform wires); a CX separates the rounds. With top = 12 every
-    other round holds a step on wire 12 (four-wave tiles)."""
+    other round holds a step on wire 12 (four-wave tiles). Every third round
+    starts with the four top wires in a scrambled order (the coalesced
+    arrangement then serves the first four steps in place)."""
     r = C.SplitMix64(seed)
     gates = []
     for i in range(rounds):
         count = 5 + r.next_u64() % (top - 4)
         ws = [12] if top == 12 and i % 2 == 1 else []
+        if i % 3 == 2:
+            ws = [top, top - 2, top - 1, top - 3]
         while len(ws) < count:
             q = 1 + r.next_u64() % top
             if q not in ws:
