@@ -1146,7 +1146,8 @@ void upload(T *&dst, const std::vector<T> &src) {
 
 // Spread launches: each workgroup takes 2^kSpreadChunkBits amplitudes of one
 // instance; the states of one group of spread instances live in HBM together
-// (at most kSpreadBudget bytes, groups run one after another).
+// (at most kSpreadBudget bytes or half the free device memory, groups run one
+// after another).
 constexpr uint32_t kSpreadChunkBits = 13;
 // 256 threads: a spread workgroup (one wave per SIMD) fits beside a
 // persistent one on the same CU.
@@ -1252,6 +1253,11 @@ struct Batch::Impl {
     auto nsteps = [&](uint32_t i) { return uint32_t(bounds[i].size() - 1); };
     std::stable_sort(sp.begin(), sp.end(),
                      [&](uint32_t x, uint32_t y) { return nsteps(x) > nsteps(y); });
+    // States of one group: kSpreadBudget, or half the free device memory.
+    size_t free_b = 0, total_b = 0;
+    uint64_t budget = kSpreadBudget;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) budget = std::min<uint64_t>(budget, free_b / 2);
+    (void)hipGetLastError();
     std::vector<uint32_t> inst, base, wg_inst, step_off, nst, steps;
     std::vector<uint64_t> state;
     uint64_t scratch_max = 0, wgs_max = 0;
@@ -1266,7 +1272,7 @@ struct Batch::Impl {
       while (k < sp.size()) {
         const BInst &in = h_insts[sp[k]];
         const uint64_t need = uint64_t(2) << in.n;
-        if (g.count && (elems + need) * sizeof(float2) > kSpreadBudget) break;
+        if (g.count && (elems + need) * sizeof(float2) > budget) break;
         inst.push_back(sp[k]);
         state.push_back(elems);
         base.push_back(wgs);
