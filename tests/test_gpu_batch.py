@@ -100,6 +100,27 @@ def test_batch_graph_replays_are_identical():
         assert abs(e1[i] - e0) <= 1e-5 and np.abs(g1[i] - g0).max() <= 1e-5
 
 
+def test_autotuned_spread_batch_matches_plain_path():
+    """Wide instances (14..18 qubits): autotune times spreading from 1, 3 or
+    5 below the widest instance, or not at all, on the whole batch; whatever
+    it picks, replays are identical and every instance matches the
+    one-circuit path."""
+    inst = C.qaoa_instances(24, p=3, lo=14, hi=18, seed=11)
+    top = max(n for n, _, _ in inst)
+    circs = [C.to_circuit(n, C.qaoa_ring(n, 3, ga, be)) for n, ga, be in inst]
+    hams = [C.to_hamiltonian(C.ring_zz_hamiltonian(n)) for n, _, _ in inst]
+    b = qm.Batch().prepare(circs, hams)
+    cfg = b.autotune()
+    print("autotuned:", cfg)
+    assert cfg["spread_from"] == 0 or 14 <= cfg["spread_from"] <= top
+    e1, g1, _ = b.run()
+    e2, g2, _ = b.run()
+    assert np.array_equal(e1, e2) and all(np.array_equal(x, y) for x, y in zip(g1, g2))
+    for i, (n, ga, be) in enumerate(inst):
+        e0, g0 = plain(n, C.qaoa_ring(n, 3, ga, be), C.ring_zz_hamiltonian(n))
+        assert abs(e1[i] - e0) <= 1e-5 and np.abs(g1[i] - g0).max() <= 1e-5, i
+
+
 def test_batch_fails_closed():
     b = qm.Batch()
     bad = C.to_circuit(3, [(C.H, (0,), ()), (C.CP, (0, 1), (0.4,))])
