@@ -314,6 +314,25 @@ struct AccumArgs {
 };
 void launch_pauli_accum(hipStream_t s, const AccumArgs &a);
 
+// lambda += sum_t c_t s_t(i ^ f_t) psi[i ^ f_t] for terms whose flip f_t is 0
+// or one bit of the register tile {bit 0, w[0..nw)} (nw <= 3): a thread holds
+// the 2^(nw+1) amplitudes of a tile of both states, so one pass serves every
+// such flip. Terms are sorted by slot: slot 0 = flip bit 0, slot 1 + k =
+// flip bit w[k], slot nw + 1 = no flip; slot s owns terms [off[s], off[s+1]).
+constexpr int kAccumTileWires = 3;
+struct AccumTileArgs {
+  float2 *lam;
+  const float2 *psi;
+  uint64_t dim;
+  uint32_t nw;
+  uint8_t w[kAccumTileWires];
+  uint32_t off[kAccumTileWires + 3];
+  uint64_t sign[kAccumTerms];
+  double cre[kAccumTerms];
+  double cim[kAccumTerms];
+};
+void launch_pauli_accum_tile(hipStream_t s, const AccumTileArgs &a);
+
 // Adjoint step for a rotation exp(-i t P / 2), P in {X, Y, Z} on wire q:
 // accumulates <lam|P|psi> (FP64, per-block partials reduced into
 // out[2*slot], out[2*slot+1]) and then applies the inverse rotation to both

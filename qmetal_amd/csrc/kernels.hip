@@ -860,6 +860,70 @@ __global__ __launch_bounds__(kT) void k_pauli_accum(AccumArgs p) {
   }
 }
 
+// Co-state terms of several flips in one pass (AccumTileArgs). Thread tile:
+// bit 0 inside a float4, register bit k on wire w[k]; the source of element e
+// under flip slot s is element e ^ (1 << s), a compile-time index. Each
+// element's coefficient per slot is summed in FP64 over that slot's terms
+// (signs from the source index), then applied in FP32 as in k_pauli_accum.
+template <int NW>
+__global__ __launch_bounds__(kT) void k_pauli_accum_tile(AccumTileArgs p) {
+  constexpr int E = 2 << NW;  // elements per thread
+  const uint64_t units = p.dim >> (NW + 1);
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  uint64_t wbit[NW > 0 ? NW : 1];
+#pragma unroll
+  for (int k = 0; k < NW; k++) wbit[k] = 1ull << p.w[k];
+  for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < units; u += stride) {
+    uint64_t base = u << 1;
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+      const uint64_t lo = base & (wbit[k] - 1);
+      base = ((base - lo) << 1) | lo;
+    }
+    uint64_t idx[E / 2];
+    float2 y[E], l[E];
+#pragma unroll
+    for (int r = 0; r < E / 2; r++) {
+      uint64_t o = base;
+#pragma unroll
+      for (int k = 0; k < NW; k++)
+        if ((r >> k) & 1) o |= wbit[k];
+      idx[r] = o;
+      const float4 v = *reinterpret_cast<const float4 *>(p.psi + o);
+      const float4 w = *reinterpret_cast<const float4 *>(p.lam + o);
+      y[2 * r] = make_float2(v.x, v.y);
+      y[2 * r + 1] = make_float2(v.z, v.w);
+      l[2 * r] = make_float2(w.x, w.y);
+      l[2 * r + 1] = make_float2(w.z, w.w);
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const uint64_t i = idx[e >> 1] | uint64_t(e & 1);
+#pragma unroll
+      for (int sl = 0; sl < NW + 2; sl++) {
+        // slot sl: flip bit 0 (sl = 0), wire w[sl - 1], or none (sl = NW + 1)
+        const uint64_t f = sl == 0 ? 1ull : sl <= NW ? wbit[sl - 1] : 0ull;
+        const int src = sl == 0 ? (e ^ 1) : sl <= NW ? (e ^ (1 << sl)) : e;
+        const uint64_t j = i ^ f;
+        double cr = 0.0, ci = 0.0;
+        for (uint32_t t = p.off[sl]; t < p.off[sl + 1]; t++) {
+          const double sg = sgn(j, p.sign[t]);
+          cr += sg * p.cre[t];
+          ci += sg * p.cim[t];
+        }
+        if (p.off[sl] == p.off[sl + 1]) continue;
+        const float2 ys = y[src];
+        l[e].x += float(cr * ys.x - ci * ys.y);
+        l[e].y += float(cr * ys.y + ci * ys.x);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < E / 2; r++)
+      *reinterpret_cast<float4 *>(p.lam + idx[r]) =
+          make_float4(l[2 * r].x, l[2 * r].y, l[2 * r + 1].x, l[2 * r + 1].y);
+  }
+}
+
 // Fused adjoint step for a rotation on wire q: overlap <lam|P|psi> in FP64
 // (P = X, Y or Z on q, taken on the pair (i0, i1 = i0 | 2^q)), then the
 // inverse rotation on both states.
@@ -1327,6 +1391,17 @@ void launch_pauli(hipStream_t s, const float2 *bra, const float2 *ket,
 
 void launch_pauli_accum(hipStream_t s, const AccumArgs &a) {
   k_pauli_accum<<<grid_for(a.dim / 2), kT, 0, s>>>(a);
+}
+
+void launch_pauli_accum_tile(hipStream_t s, const AccumTileArgs &a) {
+  const unsigned grid = grid_for(a.dim >> (a.nw + 1));
+  switch (a.nw) {
+    case 0: k_pauli_accum_tile<0><<<grid, kT, 0, s>>>(a); break;
+    case 1: k_pauli_accum_tile<1><<<grid, kT, 0, s>>>(a); break;
+    case 2: k_pauli_accum_tile<2><<<grid, kT, 0, s>>>(a); break;
+    case 3: k_pauli_accum_tile<3><<<grid, kT, 0, s>>>(a); break;
+    default: break;
+  }
 }
 
 void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,

@@ -1735,7 +1735,67 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   s->launched();
   std::map<uint64_t, std::vector<size_t>> by_flip;
   for (size_t k = 0; k < h.size(); k++) by_flip[h[k].p.x | h[k].p.y].push_back(k);
+  auto coef = [&](const Term &t, double &re, double &im) {  // c * i^ny
+    const uint32_t ny = uint32_t(__builtin_popcountll(t.p.y));
+    re = t.c;
+    im = 0.0;
+    switch (ny & 3u) {
+      case 1: re = 0.0; im = t.c; break;
+      case 2: re = -t.c; break;
+      case 3: re = 0.0; im = -t.c; break;
+      default: break;
+    }
+  };
+  // Flips of at most one bit share register-tile passes: bit 0 and the
+  // no-flip terms ride with the first tile, the other flipped wires go three
+  // to a tile. Flips of two or more bits take one pass each.
+  std::vector<uint32_t> wires1;
+  for (const auto &kv : by_flip)
+    if (__builtin_popcountll(kv.first) == 1 && kv.first != 1)
+      wires1.push_back(uint32_t(__builtin_ctzll(kv.first)));
+  const bool has_tile = by_flip.count(0) || by_flip.count(1) || !wires1.empty();
+  const size_t ntiles = has_tile ? std::max<size_t>(1, (wires1.size() + kAccumTileWires - 1) / kAccumTileWires) : 0;
+  for (size_t g = 0; g < ntiles; g++) {
+    const size_t w0 = g * kAccumTileWires;
+    const uint32_t nw = uint32_t(std::min<size_t>(kAccumTileWires, wires1.size() - std::min(w0, wires1.size())));
+    // Terms of this tile by slot: 0 flip bit 0, 1 + k wire k, nw + 1 none.
+    std::vector<std::vector<size_t>> slot(nw + 2);
+    if (g == 0) {
+      if (by_flip.count(1)) slot[0] = by_flip[1];
+      if (by_flip.count(0)) slot[nw + 1] = by_flip[0];
+    }
+    for (uint32_t k = 0; k < nw; k++) slot[1 + k] = by_flip[1ull << wires1[w0 + k]];
+    // Launches of at most kAccumTerms terms, each slot's terms contiguous.
+    std::vector<std::pair<uint32_t, size_t>> order;  // (slot, term)
+    for (uint32_t sl = 0; sl < nw + 2; sl++)
+      for (size_t t : slot[sl]) order.push_back({sl, t});
+    for (size_t b = 0; b < order.size(); b += kAccumTerms) {
+      AccumTileArgs a{};
+      a.lam = s->costate;
+      a.psi = s->state;
+      a.dim = s->dim;
+      a.nw = nw;
+      for (uint32_t k = 0; k < nw; k++) a.w[k] = uint8_t(wires1[w0 + k]);
+      const size_t e = std::min(order.size(), b + kAccumTerms);
+      uint32_t n = 0;
+      for (uint32_t sl = 0; sl < nw + 2; sl++) {
+        a.off[sl] = n;
+        for (size_t j = b; j < e; j++)
+          if (order[j].first == sl) {
+            const Term &t = h[order[j].second];
+            coef(t, a.cre[n], a.cim[n]);
+            a.sign[n] = t.p.y | t.p.z;
+            n++;
+          }
+      }
+      a.off[nw + 2] = n;
+      launch_pauli_accum_tile(s->stream, a);
+      s->launched();
+      s->dispatches++;
+    }
+  }
   for (const auto &kv : by_flip) {
+    if (__builtin_popcountll(kv.first) <= 1) continue;  // done on a tile above
     const std::vector<size_t> &ts = kv.second;
     for (size_t b = 0; b < ts.size(); b += kAccumTerms) {
       AccumArgs a{};
@@ -1745,17 +1805,8 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       a.flip = kv.first;
       for (size_t j = b; j < ts.size() && j < b + kAccumTerms; j++) {
         const Term &t = h[ts[j]];
-        const uint32_t ny = uint32_t(__builtin_popcountll(t.p.y));
-        double re = t.c, im = 0.0;  // c * i^ny
-        switch (ny & 3u) {
-          case 1: re = 0.0; im = t.c; break;
-          case 2: re = -t.c; break;
-          case 3: re = 0.0; im = -t.c; break;
-          default: break;
-        }
+        coef(t, a.cre[a.nterms], a.cim[a.nterms]);
         a.sign[a.nterms] = t.p.y | t.p.z;
-        a.cre[a.nterms] = re;
-        a.cim[a.nterms] = im;
         a.nterms++;
       }
       launch_pauli_accum(s->stream, a);
