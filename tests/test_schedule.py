@@ -232,6 +232,29 @@ def test_schedule_invariants_on_streams(opt):
             check_schedule(C.to_circuit(n, gates), opts)
 
 
+def test_low_chains_reaching_wire_12():
+    """Chains of H / phase steps whose wires reach 12 (four-wave low tiles
+    from 13 qubits; two-wave ones, or none, below): every pass within its
+    shape, every op once, overlapping ops in plan order."""
+    for n in (12, 13, 16):
+        for seed in range(3):
+            r = C.SplitMix64(77 * n + seed)
+            gates = []
+            for i in range(4):
+                ws = [12 - k for k in range(4)] if i % 2 else []
+                while len(ws) < 5 + r.next_u64() % 8:
+                    q = 1 + r.next_u64() % 12
+                    if q not in ws:
+                        ws.append(q)
+                ws = [w for w in ws if w < n] + [0]
+                for w in ws:
+                    gates.append((C.H, (w,), ()))
+                    gates.append((C.CP, ((w + 1 + r.next_u64() % (n - 1)) % n, w),
+                                  (r.uniform(-3, 3),)))
+                gates.append((C.CX, (n - 1, 0), ()))
+            check_schedule(C.to_circuit(n, gates))
+
+
 def test_scheduling_reduces_launches():
     for n, gates in ((20, C.brickwork(20, 10)), (16, C.qaoa_ring(16, 6)),
                      (18, C.hardware_efficient(18, 3))):
