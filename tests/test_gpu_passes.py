@@ -188,6 +188,45 @@ def test_qft_four_wave_low_pass_against_oracle(n, want):
     assert d <= TOL1 and dm <= TOL_SAME and abs(norm - 1) <= TOL1
 
 
+@pytest.mark.parametrize("n", [33, 34])
+def test_qft_basis_state_full_size_closed_form(n):
+    """QFT of a basis state |k> at full size (64 / 128 GiB; two- and
+    four-wave low passes): the reference's QFT (no swap layer) leaves qubit q
+    in (|0> + e^(i phi_q)|1>)/sqrt(2) with phi_q = 2 pi k / 2^(q+1), so
+    <X_q> = cos phi_q, <Y_q> = sin phi_q, <Z_q> = 0 (the closed form is
+    pinned against the oracle by the next test)."""
+    k = 0x2F5A3C1B7 & ((1 << n) - 1)
+    prep = [(C.X, (q,), ()) for q in range(n) if (k >> q) & 1]
+    with qm.Simulator(n) as s:
+        s.run(C.to_circuit(n, prep))
+        s.run(C.to_circuit(n, C.qft(n)))
+        norm = s.norm()
+        worst = 0.0
+        for q in (0, 1, 5, 11, 12, 13, 20, n - 1):
+            phi = 2 * np.pi * ((k % (1 << (q + 1))) / (1 << (q + 1)))
+            ex, ey, ez = (s.expectation_pauli(x=1 << q), s.expectation_pauli(y=1 << q),
+                          s.expectation_pauli(z=1 << q))
+            worst = max(worst, abs(ex - np.cos(phi)), abs(ey - np.sin(phi)), abs(ez))
+    print(f"QFT{n}|k>: norm {norm:.8f}, worst Bloch-component error {worst:.2e}")
+    assert abs(norm - 1) <= TOL1 and worst <= TOL1
+
+
+def test_qft_basis_state_closed_form_matches_oracle():
+    """The closed form used at full size, pinned against the complex128 oracle
+    at n = 6 for three basis states (Bloch components of every qubit)."""
+    n = 6
+    idx = np.arange(1 << n)
+    for k in (5, 37, 50):
+        prep = [(C.X, (q,), ()) for q in range(n) if (k >> q) & 1]
+        st = O.run(n, prep + C.qft(n))
+        for q in range(n):
+            a0 = st[((idx >> q) & 1) == 0]
+            a1 = st[(idx ^ (1 << q))[((idx >> q) & 1) == 0]]
+            ov = np.sum(np.conj(a0) * a1)
+            phi = 2 * np.pi * k / (1 << (q + 1))
+            assert abs(2 * ov.real - np.cos(phi)) < 1e-12 and abs(2 * ov.imag - np.sin(phi)) < 1e-12
+
+
 def test_qft_zero_state_is_uniform_merged():
     n = 28
     with qm.Simulator(n) as s:
