@@ -124,7 +124,7 @@ def oracle_threads() -> int:
 
 def suite(qm, C, emit):
     """Every configuration of the measurement plan, one JSON object each:
-    GHZ n=20, QFT scale sweep (merged passes vs one launch per plan op vs the
+    GHZ n=20, GHZ n=30..33, QFT scale sweep (merged passes vs one launch per plan op vs the
     unfused plan, alternated), brickwork,
     HEA adjoint gradients. Correctness is checked against the complex128
     oracle where it fits (n <= 26) and against invariants above."""
@@ -162,6 +162,38 @@ def suite(qm, C, emit):
           "max_abs_diff_vs_oracle": float(np.abs(amps - ref).max()), "norm": norm,
           "max_abs_z_q": zq, "z0_z19": zz, "shots": 100000,
           "distinct_outcomes": sorted(set(shots)), "plan_ops": g.plan_size(True)})
+
+    # --- GHZ scale sweep (the reference's published GHZ row, README.md:12-19:
+    # 0.81 / 1.65 / 3.41 / 7.67 s at n = 30..33 on M4 Max): wall time per
+    # circuit from |0...0>, merged passes vs one launch per plan op, with the
+    # invariants <Z_q> = 0, <Z_0 Z_(n-1)> = 1, norm 1 and two-pole shots.
+    for n in (30, 31, 32, 33):
+        g = C.to_circuit(n, C.ghz(n))
+        with qm.Simulator(n) as s:
+            times = {"merged": [], "op_by_op": []}
+            for _ in range(2):
+                for arm in times:
+                    s.set_pass_merging(arm == "merged")
+                    s.reset()
+                    s.synchronize()
+                    t0 = time.perf_counter()
+                    s.run(g)
+                    s.synchronize()
+                    times[arm].append(time.perf_counter() - t0)
+            s.set_pass_merging(True)
+            s.reset()
+            before = s.dispatch_count
+            s.run(g)
+            launches = s.dispatch_count - before
+            zq = max(abs(s.expectation_pauli(z=1 << q)) for q in (0, 1, n // 2, n - 1))
+            zz = s.expectation_pauli(z=1 | (1 << (n - 1)))
+            norm = s.norm()
+            shots = s.sample(1000, 7)
+        emit({"config": f"ghz_n{n}", "merged_s": min(times["merged"]),
+              "op_by_op_s": min(times["op_by_op"]), "launches": launches,
+              "plan_ops": g.plan_size(True), "norm": norm, "max_abs_z_q": zq, "z0_zlast": zz,
+              "shots_on_the_two_poles": set(shots) <= {0, (1 << n) - 1},
+              "reference_m4max_s": {30: 0.81, 31: 1.65, 32: 3.41, 33: 7.67}[n]})
 
     # --- QFT scale sweep: fused plan with merged passes, the same plan one
     # launch per op, and the unfused plan (merged), alternated -------------
