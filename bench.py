@@ -574,6 +574,15 @@ def main() -> int:
                "sample_s": round(dt, 3)}
         print(f"# cpu oracle n={args.cpu_n}: {dt:.3f} s (scaled to n={n}: {extrap:.1f} s)",
               file=sys.stderr, flush=True)
+        # The second arm of the CPU baseline: the same oracle threaded over the
+        # host cores (the reference itself has no threading).
+        th = oracle_threads()
+        if th > 1 and args.cpu_threads == 1:
+            dt_t, extrap_t = cpu_baseline(args.cpu_n, n, th)
+            cpu["threaded"] = {"value": round(extrap_t, 1), "unit": "s/circuit", "cores": th,
+                               "sample_s": round(dt_t, 3)}
+            print(f"# cpu oracle n={args.cpu_n}, {th} threads: {dt_t:.3f} s "
+                  f"(scaled to n={n}: {extrap_t:.1f} s)", file=sys.stderr, flush=True)
 
     out = {
         "metric": "s/circuit + HBM GB/s (% of peak) at n=30-34",
