@@ -182,9 +182,9 @@ def suite(qm, C, emit):
                     times[arm].append(time.perf_counter() - t0)
             s.set_pass_merging(True)
             s.reset()
-            before = s.dispatch_count
+            before = s.launch_count
             s.run(g)
-            launches = s.dispatch_count - before
+            launches = s.launch_count - before
             zq = max(abs(s.expectation_pauli(z=1 << q)) for q in (0, 1, n // 2, n - 1))
             zz = s.expectation_pauli(z=1 | (1 << (n - 1)))
             norm = s.norm()

@@ -205,9 +205,12 @@ int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
  * src/metal/simulator.mm:436-590). Consecutive plan ops whose moving wires fit
  * one thread's register tile (bit 0 plus up to 4 wires) share one kernel and
  * one read-modify-write pass over the state; phase layers join any pass.
- * The plan and qm_circuit_plan_size are unchanged; qm_dispatch_count counts
- * the launches actually made. On by default; qm_set_pass_merging(s, 0)
- * restores one launch per plan op. Timing records of kind 7 to 12 are
+ * The plan, qm_circuit_plan_size and qm_dispatch_count are unchanged
+ * (qm_dispatch_count still adds one per plan op, none for a diagonal layer
+ * without terms, as src/metal/simulator.mm:450-595 and
+ * tests/test_gpu.mm:247-256 count); qm_launch_count counts the kernels
+ * actually launched, merged passes once. On by default;
+ * qm_set_pass_merging(s, 0) restores one launch per plan op. Timing records of kind 7 to 12 are
  * merged passes (generic, chain-shaped, wide and low chain-shaped, generic
  * low tile and generic wide kernels).
  * qm_plan_passes reports the grouping qm_run would use: pass_of_op
@@ -219,6 +222,7 @@ int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
  * group wires it applies (0: the whole op), pass index -- writing at most
  * `cap` entries (arrays may be NULL) and setting *count to their number. */
 int qm_set_pass_merging(qm_simulator s, int enable);
+uint64_t qm_launch_count(qm_simulator s);
 int qm_plan_passes(qm_plan p, uint32_t *pass_of_op, size_t *num_passes);
 int qm_plan_schedule(qm_plan p, uint32_t *entry_op, uint64_t *entry_wires,
                      uint32_t *entry_pass, size_t cap, size_t *count);

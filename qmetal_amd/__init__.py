@@ -144,6 +144,7 @@ _sync = _decl("qm_synchronize", c_int, c_void_p)
 _num_qubits = _decl("qm_num_qubits", c_uint32, c_void_p)
 _dim = _decl("qm_dim", c_uint64, c_void_p)
 _dispatch_count = _decl("qm_dispatch_count", c_uint64, c_void_p)
+_launch_count = _decl("qm_launch_count", c_uint64, c_void_p)
 _amplitudes = _decl("qm_amplitudes", POINTER(c_float), c_void_p)
 _norm = _decl("qm_norm", c_int, c_void_p, _f64p)
 _expect_pauli = _decl("qm_expectation_pauli", c_int, c_void_p, c_uint64, c_uint64, c_uint64, _f64p)
@@ -409,6 +410,11 @@ class Simulator:
     def dim(self) -> int: return int(_dim(self._h))
     @property
     def dispatch_count(self) -> int: return int(_dispatch_count(self._h))
+    @property
+    def launch_count(self) -> int:
+        """Kernels actually launched: a merged pass counts once, while
+        dispatch_count adds one per plan op as the reference does."""
+        return int(_launch_count(self._h))
 
     def reset(self): _check(_reset(self._h)); return self
 

@@ -233,6 +233,7 @@ int qm_synchronize(qm_simulator s) {
 uint32_t qm_num_qubits(qm_simulator s) { return s ? s->s->num_qubits() : 0; }
 uint64_t qm_dim(qm_simulator s) { return s ? s->s->dim() : 0; }
 uint64_t qm_dispatch_count(qm_simulator s) { return s ? s->s->dispatch_count() : 0; }
+uint64_t qm_launch_count(qm_simulator s) { return s ? s->s->launch_count() : 0; }
 
 const float *qm_amplitudes(qm_simulator s) {
   if (!s) {

@@ -52,7 +52,11 @@ class Simulator {
 
   uint32_t num_qubits() const;
   uint64_t dim() const;
+  // One per plan op run (merged or not; none for a diagonal layer without
+  // terms), reset, reduction and sample, as the reference counts.
   uint64_t dispatch_count() const;
+  // Kernels actually launched (merged passes count once).
+  uint64_t launch_count() const;
 
   // Host copy of the interleaved complex64 state, refreshed on every call and
   // valid until the next call or mutation.

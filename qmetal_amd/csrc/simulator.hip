@@ -144,7 +144,10 @@ struct Simulator::Impl {
   uint32_t n = 0;
   uint64_t dim = 0;
   int device = 0;
+  // dispatches: one per plan op run (merged or not), reset, reduction and
+  // sample; launches: kernels actually launched.
   uint64_t dispatches = 0;
+  uint64_t launches = 0;
   hipStream_t stream = nullptr;
   float2 *state = nullptr;
   float2 *costate = nullptr;
@@ -1122,7 +1125,7 @@ struct Simulator::Impl {
     else if (pl.chain) launch_chain(stream, pl.nl, pl.chain_k, pl.chain_zero, pl.chain_real, pl.c);
     else launch_pass(stream, pl.nl, pl.a);
     launched();
-    dispatches++;
+    launches++;
     if (timing) {
       hipEvent_t b = take_event();
       check(hipEventRecord(b, stream), "hipEventRecord");
@@ -1320,7 +1323,7 @@ struct Simulator::Impl {
         controlled({p.q[0], p.q[1]}, p.q[2], kX, nullptr);
         break;
     }
-    dispatches++;
+    launches++;
   }
 
   // Gate path: one gate on the state (and, for the adjoint, the same gate on
@@ -1473,6 +1476,7 @@ Simulator::~Simulator() {
 uint32_t Simulator::num_qubits() const { return impl_->n; }
 uint64_t Simulator::dim() const { return impl_->dim; }
 uint64_t Simulator::dispatch_count() const { return impl_->dispatches; }
+uint64_t Simulator::launch_count() const { return impl_->launches; }
 void *Simulator::device_state() { return impl_->state; }
 
 void Simulator::reset() {
@@ -1481,6 +1485,7 @@ void Simulator::reset() {
   launch_fill_basis(s->stream, s->state, s->dim);
   s->launched();
   s->dispatches++;
+  s->launches++;
 }
 
 void Simulator::run(const Circuit &c, const FuseOpts &o) {
@@ -1552,6 +1557,10 @@ void Simulator::run_plan(const Plan &p) {
     if (passes[k].merged) s->pass_launch(merged[k]);
     else s->op(p.ops[passes[k].ops[0]], &pend[k]);
   }
+  // The reference's accounting (simulator.mm:450-595): one dispatch per plan
+  // op, none for a diagonal layer without terms, however the ops were merged.
+  for (const PlanOp &op : p.ops)
+    if (!(op.kind == OpKind::Diagonal && op.masks.empty())) s->dispatches++;
 }
 
 void Simulator::set_pass_merging(bool on) { impl_->merge = on; }
@@ -1580,6 +1589,7 @@ void Simulator::apply_gate(const Gate &g) {
   s->use();
   s->gate(g, nullptr);
   s->dispatches++;
+  s->launches++;
 }
 
 void Simulator::synchronize() {
@@ -1606,6 +1616,7 @@ double Simulator::abs2sum() {
   launch_abs2(s->stream, s->state, s->dim, s->partials, slot);
   s->launched();
   s->dispatches++;
+  s->launches++;
   return s->fetch_results(1)[0];
 }
 
@@ -1641,6 +1652,7 @@ double Simulator::expectation(const Hamiltonian &h) {
   for (size_t k = 0; k < h.size(); k++) {
     s->pauli_sum(s->state, s->state, h[k].p, slots + 2 * k);
     s->dispatches++;
+    s->launches++;
   }
   const double *r = s->fetch_results(2 * h.size());
   double total = 0.0;
@@ -1668,6 +1680,7 @@ std::vector<uint64_t> Simulator::sample(size_t shots, uint64_t seed) {
   launch_shots(s->stream, s->state, s->dim, s->cdf, nb, shots, seed, s->shots_dev);
   s->launched();
   s->dispatches++;
+  s->launches++;
   check(hipMemcpyAsync(out.data(), s->shots_dev, shots * sizeof(uint64_t), hipMemcpyDeviceToHost,
                        s->stream),
         "hipMemcpyAsync(shots)");
@@ -1792,6 +1805,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       launch_pauli_accum_tile(s->stream, a);
       s->launched();
       s->dispatches++;
+      s->launches++;
     }
   }
   for (const auto &kv : by_flip) {
@@ -1812,6 +1826,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       launch_pauli_accum(s->stream, a);
       s->launched();
       s->dispatches++;
+      s->launches++;
     }
   }
   const size_t G = c.gates.size();
@@ -1823,6 +1838,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   // E = Re<psi|lambda>
   s->pauli_sum(s->state, s->costate, Pauli{}, slots + 2 * G);
   s->dispatches++;
+  s->launches++;
 
   if (s->merge && s->n >= 6) {
     // Backward sweep in merged passes: inverse gates whose target wires fit
@@ -2025,6 +2041,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       launch_adj_pass(s->stream, args[i], s->adj_partials, rot_slots + 2 * passes[i].rot0);
       s->launched();
       s->dispatches++;
+      s->launches++;
     }
     const double *r = s->fetch_results(2 * (G + 1) + 2 * nrot_total);
     std::vector<double> grad(G, 0.0);
@@ -2056,6 +2073,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       s->gate(inv, s->costate);
     }
     s->dispatches++;
+    s->launches++;
   }
   const double *r = s->fetch_results(2 * (G + 1));
   std::vector<double> grad(G, 0.0);

@@ -231,8 +231,9 @@ def test_blocked_plans_at_width():
 
 
 def test_dispatch_count():
+    """tests/test_gpu.mm:247-256: the constructor's reset counts one, a run
+    one per plan op; the default (merged) executor keeps that count."""
     with qm.Simulator(8) as s:
-        s.set_pass_merging(False)  # one launch per plan op
         before = s.dispatch_count
         assert before == 1  # the constructor's reset
         c = C.to_circuit(8, C.ghz(8))
@@ -240,6 +241,10 @@ def test_dispatch_count():
         s.synchronize()
         assert s.dispatch_count - before == len(c) == c.plan_size(True)
         q = C.to_circuit(8, C.qft(8))
+        b = s.dispatch_count
+        s.run(q)
+        assert s.dispatch_count - b == q.plan_size(True)
+        s.set_pass_merging(False)  # one launch per plan op: same count
         b = s.dispatch_count
         s.run(q)
         assert s.dispatch_count - b == q.plan_size(True)

@@ -72,10 +72,10 @@ def run(n, gates, merge=True, options=None, prep=None):
         s.set_pass_merging(merge)
         if prep:
             s.run(C.to_circuit(n, prep))
-        before = s.dispatch_count
+        before = s.launch_count
         s.run(C.to_circuit(n, gates), options=options)
         s.synchronize()
-        launches = s.dispatch_count - before
+        launches = s.launch_count - before
         return s.amplitudes().astype(np.complex128), s.norm(), launches
 
 
@@ -146,21 +146,24 @@ def test_configurations_merged_vs_op_by_op(name, n):
                                    if not (op["kind"] == "DiagonalLayer" and not op["masks"]))
 
 
-def test_dispatch_count_follows_grouping():
-    """One launch per plan op with merging off (the reference's accounting,
-    tests/test_gpu.mm:247-256); one per merged pass with it on."""
+def test_dispatch_count_follows_plan_launch_count_follows_grouping():
+    """dispatch_count adds one per plan op whether or not ops are merged (the
+    reference's accounting, tests/test_gpu.mm:247-256); launch_count adds one
+    per plan op with merging off and one per merged pass with it on."""
     c = C.to_circuit(8, C.ghz(8))
     with qm.Simulator(8) as s:
         s.set_pass_merging(False)
-        before = s.dispatch_count
+        before, lb = s.dispatch_count, s.launch_count
         s.run(c)
         s.synchronize()
         assert s.dispatch_count - before == len(c) == c.plan_size(True)
+        assert s.launch_count - lb == len(c)
         s.set_pass_merging(True)
-        before = s.dispatch_count
+        before, lb = s.dispatch_count, s.launch_count
         s.run(c)
         s.synchronize()
-        assert s.dispatch_count - before == len(c.schedule()) < len(c)
+        assert s.dispatch_count - before == len(c)
+        assert s.launch_count - lb == len(c.schedule()) < len(c)
 
 
 def test_qft_24_merged_vs_op_by_op():
