@@ -144,6 +144,27 @@ def test_ghz20_invariants_and_1e5_shots():
         assert shots == {0, (1 << n) - 1}
 
 
+@pytest.mark.parametrize("n", [33, 34])
+def test_ghz_full_size_invariants(n):
+    """GHZ at 64 and 128 GiB (one allocation), merged passes: norm 1,
+    <Z_q> = 0, <Z_0 Z_(n-1)> = 1, <X...X> = 1 (the two poles' coherence),
+    and shots only on the two poles, both of them, near half each (4000
+    shots: 2000 +- 32 sd, the bound is 6 sd)."""
+    top = (1 << n) - 1
+    with sim_with(n, C.ghz(n)) as s:
+        norm = s.norm()
+        zq = max(abs(s.expectation_pauli(z=1 << q)) for q in (0, 1, 12, 13, n // 2, n - 1))
+        zz = s.expectation_pauli(z=1 | (1 << (n - 1)))
+        xx = s.expectation_pauli(x=top)
+        shots = s.sample(4000, 2024)
+    ones = sum(1 for v in shots if v == top)
+    print(f"GHZ{n}: norm {norm:.8f}, max|<Z_q>| {zq:.2e}, <Z0 Z{n - 1}> {zz:.8f}, "
+          f"<X..X> {xx:.8f}, {ones} of 4000 shots on |1..1>")
+    assert abs(norm - 1) <= 1e-5 and zq <= 1e-5
+    assert abs(zz - 1) <= 1e-5 and abs(xx - 1) <= 1e-5
+    assert set(shots) == {0, top} and abs(ones - 2000) <= 190
+
+
 def tfim6():
     n = 6
     h = [(1.0, 0, 0, 3 << q) for q in range(n - 1)] + [(0.4, 1 << q, 0, 0) for q in range(n)]
