@@ -1,9 +1,11 @@
 """Headline benchmark: fused QFT on one MI355X, in place on a seeded product
 state, reported as seconds per circuit and as algorithmic HBM bandwidth
 against the 8 TB/s peak, with the dominant kernel's roofline measured by HIP
-events on the simulator's stream and the complex128 CPU oracle timed beside it.
+events on the simulator's stream and the reference's complex128 CPU oracle
+(built from its sources into oracle/_ref; the build's own restatement where
+that is absent) timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--n 33] [--cpu-n 24] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n 33] [--cpu-n 26] [--no-cpu]
 
 One step is one qm_run of QFT(n) (n(n+1)/2 gates, a plan of 2n-1 fused ops,
 which the executor merges into a few read-modify-write passes over HBM, 4 at
@@ -43,6 +45,35 @@ def cpu_baseline(n_cpu: int, n_gpu: int, threads: int):
     O.run(n_cpu, gates, state=prep)
     dt = time.perf_counter() - t0
     O.set_threads(1)
+    scale = float(1 << (n_gpu - n_cpu)) * (n_gpu * (n_gpu + 1) / 2) / (n_cpu * (n_cpu + 1) / 2)
+    return dt, dt * scale
+
+
+REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+
+
+def ref_cpu_baseline(n_cpu: int, n_gpu: int):
+    """The reference's own complex128 oracle (Reference::run, one thread),
+    compiled from its sources into oracle/_ref/ref_dump, on the same
+    workload at n_cpu: the product-state preparation untimed, then QFT gate
+    by gate, timed inside the binary. None where the binary is absent."""
+    import subprocess
+    from qmetal_amd import circuits as C
+    if not os.access(REF_DUMP, os.X_OK):
+        return None
+    prep = C.product_state_prep(n_cpu, seed=2024)
+    lines = [str(n_cpu)]
+    for kind, q, p in prep + C.qft(n_cpu):
+        qq = list(q) + [0] * (3 - len(q))
+        pp = [float(v) for v in p] + [0.0] * (3 - len(p))
+        lines.append(f"{kind} {len(q)} {qq[0]} {qq[1]} {qq[2]} "
+                     f"{pp[0].hex()} {pp[1].hex()} {pp[2].hex()}")
+    out = subprocess.run([REF_DUMP, "time", str(len(prep))], input="\n".join(lines) + "\n",
+                         capture_output=True, text=True, check=True, timeout=600).stdout
+    r = json.loads(out)
+    if abs(r["norm"] - 1) > 1e-9 or r["gates"] != n_cpu * (n_cpu + 1) // 2:
+        raise RuntimeError(f"reference oracle run looks wrong: {r}")
+    dt = float(r["seconds"])
     scale = float(1 << (n_gpu - n_cpu)) * (n_gpu * (n_gpu + 1) / 2) / (n_cpu * (n_cpu + 1) / 2)
     return dt, dt * scale
 
@@ -565,13 +596,24 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
+        sample = (f"QFT n={args.cpu_n} gate by gate ({args.cpu_n * (args.cpu_n + 1) // 2} "
+                  f"gates) on the same seeded product state, complex128")
+        ref = ref_cpu_baseline(args.cpu_n, n) if args.cpu_threads == 1 else None
         dt, extrap = cpu_baseline(args.cpu_n, n, args.cpu_threads)
-        cpu = {"value": round(extrap, 1), "unit": "s/circuit", "cores": args.cpu_threads,
-               "kind": "port",
-               "sample": f"QFT n={args.cpu_n} gate by gate ({args.cpu_n * (args.cpu_n + 1) // 2} "
-                         f"gates) on the same seeded product state, complex128 oracle: "
-                         f"{dt:.2f} s measured, scaled to n={n} by amplitude-gates",
-               "sample_s": round(dt, 3)}
+        port = {"value": round(extrap, 1), "unit": "s/circuit", "cores": args.cpu_threads,
+                "kind": "port", "sample": f"{sample} oracle restatement (oracle/qm_oracle.c): "
+                                          f"{dt:.2f} s measured, scaled to n={n} by amplitude-gates",
+                "sample_s": round(dt, 3)}
+        if ref is not None:
+            cpu = {"value": round(ref[1], 1), "unit": "s/circuit", "cores": 1, "kind": "reference",
+                   "sample": f"{sample} reference oracle (src/reference.cpp, built from its "
+                             f"sources into oracle/_ref): {ref[0]:.2f} s measured, scaled to "
+                             f"n={n} by amplitude-gates",
+                   "sample_s": round(ref[0], 3), "port": port}
+            print(f"# cpu reference oracle n={args.cpu_n}: {ref[0]:.3f} s (scaled to n={n}: "
+                  f"{ref[1]:.1f} s)", file=sys.stderr, flush=True)
+        else:
+            cpu = port
         print(f"# cpu oracle n={args.cpu_n}: {dt:.3f} s (scaled to n={n}: {extrap:.1f} s)",
               file=sys.stderr, flush=True)
         # The second arm of the CPU baseline: the same oracle threaded over the

@@ -3,7 +3,8 @@
 // reference sources by oracle/Makefile into oracle/_ref/ref_dump.
 //
 // TEST INFRASTRUCTURE ONLY: it writes golden fixtures (gate streams, plans,
-// amplitudes, expectations) that tests/golden/make_golden.py commits. Nothing
+// amplitudes, expectations) that tests/golden/make_golden.py commits, and
+// times the reference oracle for the CPU baseline leg of bench.py. Nothing
 // in the product links or runs it.
 //
 // Usage:
@@ -11,9 +12,15 @@
 //   ref_dump stream [amps]              gate stream read from stdin:
 //                                       line 1: n; then per gate
 //                                       "kind nq q0 q1 q2 p0 p1 p2"
-// Prints one JSON object: the gates, the plans for six FusionOptions and,
+//   ref_dump time <k>                   same stdin stream; the reference
+//                                       oracle runs its first k gates
+//                                       untimed, then the rest gate by gate
+//                                       (Reference::run, one thread), and
+//                                       prints {"n":..,"gates":..,"seconds":..}
+// Otherwise prints one JSON object: the gates, the plans for six FusionOptions and,
 // with "amps", the oracle's amplitudes (n <= 14) and a few expectations.
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -142,9 +149,37 @@ static void dump_all(const Circuit &c, bool amps) {
   printf("}\n");
 }
 
+// One gate stream from stdin: line 1 n, then "kind nq q0 q1 q2 p0 p1 p2".
+bool read_stream(Circuit &c) {
+  char buf[512];
+  if (!fgets(buf, sizeof buf, stdin)) return false;
+  c.num_qubits = uint32_t(atoi(buf));
+  while (fgets(buf, sizeof buf, stdin)) {
+    if (buf[0] == '\n' || buf[0] == 0) continue;
+    Gate g{};
+    int kind, nq;
+    unsigned q0, q1, q2;
+    char p0[64], p1[64], p2[64];
+    if (sscanf(buf, "%d %d %u %u %u %63s %63s %63s", &kind, &nq, &q0, &q1,
+               &q2, p0, p1, p2) != 8)
+      return false;
+    g.kind = GateKind(kind);
+    g.num_qubits = uint8_t(nq);
+    g.qubits[0] = q0;
+    g.qubits[1] = q1;
+    g.qubits[2] = q2;
+    g.params[0] = strtod(p0, nullptr);
+    g.params[1] = strtod(p1, nullptr);
+    g.params[2] = strtod(p2, nullptr);
+    g.num_params = (kind >= 8 && kind <= 11) || kind == 16 ? 1 : 0;
+    c.add(g);
+  }
+  return true;
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: ref_dump frozen <name> <n> [amps] | stream [amps]\n");
+    fprintf(stderr, "usage: ref_dump frozen <name> <n> [amps] | stream [amps] | time <k>\n");
     return 2;
   }
   std::string mode = argv[1];
@@ -162,33 +197,26 @@ int main(int argc, char **argv) {
     fprintf(stderr, "unknown circuit %s\n", name.c_str());
     return 2;
   }
-  if (mode == "stream") {
-    bool amps = argc > 2 && std::string(argv[2]) == "amps";
+  if (mode == "stream" || mode == "time") {
     Circuit c;
-    char buf[512];
-    if (!fgets(buf, sizeof buf, stdin)) return 2;
-    c.num_qubits = uint32_t(atoi(buf));
-    while (fgets(buf, sizeof buf, stdin)) {
-      if (buf[0] == '\n' || buf[0] == 0) continue;
-      Gate g{};
-      int kind, nq;
-      unsigned q0, q1, q2;
-      char p0[64], p1[64], p2[64];
-      if (sscanf(buf, "%d %d %u %u %u %63s %63s %63s", &kind, &nq, &q0, &q1,
-                 &q2, p0, p1, p2) != 8)
-        return 3;
-      g.kind = GateKind(kind);
-      g.num_qubits = uint8_t(nq);
-      g.qubits[0] = q0;
-      g.qubits[1] = q1;
-      g.qubits[2] = q2;
-      g.params[0] = strtod(p0, nullptr);
-      g.params[1] = strtod(p1, nullptr);
-      g.params[2] = strtod(p2, nullptr);
-      g.num_params = (kind >= 8 && kind <= 11) || kind == 16 ? 1 : 0;
-      c.add(g);
+    if (!read_stream(c)) return 3;
+    if (mode == "stream") {
+      dump_all(c, argc > 2 && std::string(argv[2]) == "amps");
+      return 0;
     }
-    dump_all(c, amps);
+    const size_t k = argc > 2 ? size_t(strtoull(argv[2], nullptr, 10)) : 0;
+    if (k > c.ops.size()) return 2;
+    Circuit prep, body;
+    prep.num_qubits = body.num_qubits = c.num_qubits;
+    for (size_t i = 0; i < c.ops.size(); i++) (i < k ? prep : body).add(c.ops[i]);
+    Reference r(c.num_qubits);
+    r.run(prep);
+    const auto t0 = std::chrono::steady_clock::now();
+    r.run(body);
+    const auto t1 = std::chrono::steady_clock::now();
+    printf("{\"n\":%u,\"gates\":%zu,\"seconds\":%.6f,\"norm\":%.17g}\n",
+           c.num_qubits, body.ops.size(), std::chrono::duration<double>(t1 - t0).count(),
+           r.norm());
     return 0;
   }
   return 2;
