@@ -75,7 +75,7 @@ def ref_cpu_baseline(n_cpu: int, n_gpu: int):
         raise RuntimeError(f"reference oracle run looks wrong: {r}")
     dt = float(r["seconds"])
     scale = float(1 << (n_gpu - n_cpu)) * (n_gpu * (n_gpu + 1) / 2) / (n_cpu * (n_cpu + 1) / 2)
-    return dt, dt * scale
+    return dt, dt * scale, r
 
 
 def batched_qaoa(qm, C, count: int, repeats: int, plain_count: int):

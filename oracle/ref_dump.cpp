@@ -16,7 +16,9 @@
 //                                       oracle runs its first k gates
 //                                       untimed, then the rest gate by gate
 //                                       (Reference::run, one thread), and
-//                                       prints {"n":..,"gates":..,"seconds":..}
+//                                       prints {"n", "gates", "seconds",
+//                                       "norm", "first", "last"} (the
+//                                       first and last amplitudes)
 // Otherwise prints one JSON object: the gates, the plans for six FusionOptions and,
 // with "amps", the oracle's amplitudes (n <= 14) and a few expectations.
 
@@ -214,9 +216,13 @@ int main(int argc, char **argv) {
     const auto t0 = std::chrono::steady_clock::now();
     r.run(body);
     const auto t1 = std::chrono::steady_clock::now();
-    printf("{\"n\":%u,\"gates\":%zu,\"seconds\":%.6f,\"norm\":%.17g}\n",
+    // The first and last amplitudes, so a caller can check the timed run
+    // against another implementation of the same workload.
+    const auto &a0 = r.state()[0], &a1 = r.state()[r.dim() - 1];
+    printf("{\"n\":%u,\"gates\":%zu,\"seconds\":%.6f,\"norm\":%.17g,"
+           "\"first\":[%.17g,%.17g],\"last\":[%.17g,%.17g]}\n",
            c.num_qubits, body.ops.size(), std::chrono::duration<double>(t1 - t0).count(),
-           r.norm());
+           r.norm(), a0.real(), a0.imag(), a1.real(), a1.imag());
     return 0;
   }
   return 2;

@@ -204,3 +204,24 @@ def test_adjoint_matches_parameter_shift():
         assert abs(e - math.cos(t)) <= 1e-14 and abs(g[0] + math.sin(t)) <= 1e-14
     with pytest.raises(ValueError):
         O.adjoint(2, [(C.H, (0,), ()), (C.CP, (0, 1), (0.4,))], [(1.0, 0, 0, 1)])
+
+
+def test_cpu_baseline_legs_run_the_same_workload():
+    """The bench's CPU baseline: the reference's own oracle (timed inside
+    oracle/_ref/ref_dump) and the restatement run the same preparation and
+    QFT; their first and last amplitudes agree to 1e-12."""
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    if not os.access(bench.REF_DUMP, os.X_OK):
+        pytest.skip("oracle/_ref not built (the reference tree is absent)")
+    n = 10
+    dt, scaled, rec = bench.ref_cpu_baseline(n, 12)
+    assert rec["gates"] == n * (n + 1) // 2 and dt >= 0
+    assert scaled == pytest.approx(dt * 4 * (12 * 13) / (n * (n + 1)))
+    st = O.run(n, C.qft(n), state=O.run(n, C.product_state_prep(n, seed=2024)))
+    assert abs(complex(*rec["first"]) - st[0]) < 1e-12
+    assert abs(complex(*rec["last"]) - st[-1]) < 1e-12
+    assert abs(rec["norm"] - 1) < 1e-12
