@@ -4,7 +4,10 @@ ordered pair, CCX on every triple, the frozen set, drift with depth, launch
 grouping invariance, fail-closed behaviour and dispatch accounting.
 
 Tolerances: 1e-5 max amplitude difference for single gates (complex64 floor,
-test_gpu.mm:50), 1e-4 for deep circuits and the norm (test_gpu.mm:155-156).
+test_gpu.mm:50), 1e-4 for deep circuits and the norm (test_gpu.mm:155-156);
+with depth, a bound that grows with the gate count G: amplitudes within
+1e-6 + 1e-8 G and the norm within 1e-6 + 2e-8 G (TFIM n=12 measured
+2-4e-9 G and 6e-9 G up to G = 4500).
 """
 import math
 
@@ -21,6 +24,16 @@ pytestmark = pytest.mark.gpu
 
 TOL1 = 1e-5
 TOL_DEEP = 1e-4
+
+
+def tol_amp(gates: int) -> float:
+    """Max amplitude difference allowed after `gates` complex64 gates."""
+    return 1e-6 + 1e-8 * gates
+
+
+def tol_norm(gates: int) -> float:
+    """Norm drift allowed after `gates` complex64 gates."""
+    return 1e-6 + 2e-8 * gates
 
 
 def gpu_state(n, gates, options=None, fuse=True):
@@ -149,6 +162,7 @@ def test_drift_with_depth():
             nerr = abs(s.norm() - 1)
         print(f"depth {depth:4d} gates {len(gates):5d} diff {d:.3e} norm_err {nerr:.3e}")
         assert d <= TOL_DEEP and nerr <= TOL_DEEP
+        assert d <= tol_amp(len(gates)) and nerr <= tol_norm(len(gates)), depth
 
 
 def test_launch_grouping_invariance():
