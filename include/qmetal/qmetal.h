@@ -227,6 +227,13 @@ int qm_plan_passes(qm_plan p, uint32_t *pass_of_op, size_t *num_passes);
 int qm_plan_schedule(qm_plan p, uint32_t *entry_op, uint64_t *entry_wires,
                      uint32_t *entry_pass, size_t cap, size_t *count);
 
+/* Amplitudes at chosen basis indices, for states too large to copy whole
+ * (qm_amplitudes copies all 2^n): out[2k], out[2k+1] get the (re, im) of
+ * amplitude indices[k], k < count. Fails, writing nothing, if an index is
+ * not below qm_dim(s). No counterpart in the reference, whose amplitudes are
+ * a zero-copy view of unified memory (src/metal/simulator.mm:599-602). */
+int qm_amplitudes_at(qm_simulator s, const uint64_t *indices, size_t count, float *out);
+
 #ifdef __cplusplus
 }
 #endif

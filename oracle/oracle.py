@@ -53,8 +53,9 @@ _norm = _d("qo_norm", c_double, _f64p, c_uint32)
 _expect = _d("qo_expect_pauli", c_double, _f64p, c_uint32, c_uint64, c_uint64, c_uint64)
 _splitmix = _d("qo_splitmix64", c_uint64, _u64p)
 _sample = _d("qo_sample", None, _f64p, c_uint32, c_size_t, c_uint64, _u64p)
-_adjoint = _d("qo_adjoint", c_int, c_uint32, c_size_t, POINTER(c_int32), POINTER(c_uint8), _u32p,
-              _f64p, c_uint32, _f64p, _u64p, _u64p, _u64p, _f64p, _f64p, _f64p)
+_adjoint = _d("qo_adjoint", c_int, c_uint32, c_size_t, POINTER(c_int32), POINTER(c_uint8),
+              POINTER(c_uint8), _u32p, _f64p, c_uint32, _f64p, _u64p, _u64p, _u64p, _f64p,
+              _f64p, _f64p)
 
 
 def _p(a, t):
@@ -165,6 +166,7 @@ def sample(state: np.ndarray, shots: int, seed: int) -> np.ndarray:
 def adjoint(n: int, gates, terms):
     """(energy, grad) from the complex128 adjoint sweep."""
     kinds, nq, qs, ps = _pack(gates)
+    npar = np.array([len(p) for _, _, p in gates] or [0], dtype=np.uint8)
     t = len(terms)
     co = np.array([x[0] for x in terms], dtype=np.float64)
     xm = np.array([x[1] for x in terms], dtype=np.uint64)
@@ -173,7 +175,8 @@ def adjoint(n: int, gates, terms):
     work = np.zeros(4 << n)
     e = np.zeros(1)
     g = np.zeros(max(len(gates), 1))
-    rc = _adjoint(n, len(gates), _p(kinds, c_int32), _p(nq, c_uint8), _p(qs, c_uint32),
+    rc = _adjoint(n, len(gates), _p(kinds, c_int32), _p(nq, c_uint8), _p(npar, c_uint8),
+                  _p(qs, c_uint32),
                   _p(ps, c_double), t, _p(co, c_double), _p(xm, c_uint64), _p(ym, c_uint64),
                   _p(zm, c_uint64), _p(work, c_double), _p(e, c_double), _p(g, c_double))
     if rc != 0:
@@ -182,10 +185,11 @@ def adjoint(n: int, gates, terms):
 
 
 def param_shift(n: int, gates, terms):
-    """Exact parameter-shift gradient for RX/RY/RZ: [E(t+pi/2) - E(t-pi/2)] / 2."""
+    """Exact parameter-shift gradient for RX/RY/RZ carrying a parameter:
+    [E(t+pi/2) - E(t-pi/2)] / 2."""
     grad = np.zeros(len(gates))
     for k, (kind, q, p) in enumerate(gates):
-        if kind not in (8, 9, 10):
+        if kind not in (8, 9, 10) or not p:
             continue
         vals = []
         for d in (np.pi / 2, -np.pi / 2):

@@ -334,15 +334,15 @@ static void invert(int kind, const double *p, int *ik, double *ip) {
  * lambda = H psi, E = Re<psi|lambda>, then for k = G-1..0 take
  * grad[k] = Im<lambda|P_k|psi> before undoing gate k on both states. */
 int qo_adjoint(uint32_t n, size_t ng, const int32_t *kinds, const uint8_t *nq,
-               const uint32_t *qubits, const double *params, uint32_t nterms,
-               const double *coeffs, const uint64_t *xm, const uint64_t *ym,
-               const uint64_t *zm, double *work, double *energy, double *grad) {
+               const uint8_t *npar, const uint32_t *qubits, const double *params,
+               uint32_t nterms, const double *coeffs, const uint64_t *xm,
+               const uint64_t *ym, const uint64_t *zm, double *work, double *energy,
+               double *grad) {
   const size_t dim = (size_t)1 << n;
+  /* simulator.mm:822-828: a gate carrying a parameter needs a generator. */
   for (size_t g = 0; g < ng; g++) {
     int k = kinds[g];
-    int has_param = (k == QO_RX || k == QO_RY || k == QO_RZ || k == QO_P ||
-                     k == QO_CP || k == QO_U3);
-    if (has_param && !(k == QO_RX || k == QO_RY || k == QO_RZ)) return -1;
+    if (npar[g] > 0 && !(k == QO_RX || k == QO_RY || k == QO_RZ)) return -1;
   }
   double *psi = work, *lam = work + 2 * dim;
   qo_reset(psi, n);
@@ -357,7 +357,8 @@ int qo_adjoint(uint32_t n, size_t ng, const int32_t *kinds, const uint8_t *nq,
   for (size_t g = ng; g-- > 0;) {
     int k = kinds[g];
     grad[g] = 0.0;
-    if (k == QO_RX || k == QO_RY || k == QO_RZ) {
+    /* simulator.mm:917: only gates with num_params > 0 get a gradient. */
+    if (npar[g] > 0 && (k == QO_RX || k == QO_RY || k == QO_RZ)) {
       uint64_t bit = 1ull << qubits[3 * g];
       uint64_t x = k == QO_RX ? bit : 0, y = k == QO_RY ? bit : 0,
                z = k == QO_RZ ? bit : 0;

@@ -98,7 +98,8 @@ void qo_sample(const double *state, uint32_t n, size_t shots, uint64_t seed,
  * grad has ngates entries (0 where the gate carries no parameter). Returns 0,
  * or -1 if a parameterised gate is not RX/RY/RZ. `work` must hold 2 states. */
 int qo_adjoint(uint32_t n, size_t ngates, const int32_t *kinds,
-               const uint8_t *nq, const uint32_t *qubits, const double *params,
+               const uint8_t *nq, const uint8_t *npar, const uint32_t *qubits,
+               const double *params,
                uint32_t nterms, const double *coeffs, const uint64_t *xm,
                const uint64_t *ym, const uint64_t *zm, double *work,
                double *energy, double *grad);

@@ -146,6 +146,7 @@ _dim = _decl("qm_dim", c_uint64, c_void_p)
 _dispatch_count = _decl("qm_dispatch_count", c_uint64, c_void_p)
 _launch_count = _decl("qm_launch_count", c_uint64, c_void_p)
 _amplitudes = _decl("qm_amplitudes", POINTER(c_float), c_void_p)
+_amplitudes_at = _decl("qm_amplitudes_at", c_int, c_void_p, _u64p, c_size_t, POINTER(c_float))
 _norm = _decl("qm_norm", c_int, c_void_p, _f64p)
 _expect_pauli = _decl("qm_expectation_pauli", c_int, c_void_p, c_uint64, c_uint64, c_uint64, _f64p)
 _expect = _decl("qm_expectation", c_int, c_void_p, POINTER(_Hamiltonian), _f64p)
@@ -439,6 +440,16 @@ class Simulator:
             return [complex(p[2 * i], p[2 * i + 1]) for i in range(n)]
         buf = np.ctypeslib.as_array(p, shape=(2 * n,))
         return buf.view(np.complex64).copy()
+
+    def amplitudes_at(self, indices):
+        """numpy complex64 amplitudes at the given basis indices, read on the
+        device (for states too large to copy whole)."""
+        import numpy as np
+        idx = np.ascontiguousarray(indices, dtype=np.uint64).ravel()
+        out = np.zeros(2 * max(idx.size, 1), dtype=np.float32)
+        _check(_amplitudes_at(self._h, idx.ctypes.data_as(_u64p), idx.size,
+                              out.ctypes.data_as(POINTER(c_float))))
+        return out[:2 * idx.size].view(np.complex64).copy()
 
     def norm(self) -> float:
         out = c_double()

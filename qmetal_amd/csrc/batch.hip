@@ -1049,9 +1049,9 @@ void compile(const Circuit &c, const Hamiltonian &h, uint32_t energy_slot, uint3
     fail("batch: circuit width must be 1.." + std::to_string(kBatchMaxQubits));
   check_circuit(c);
   for (const Gate &g : c.gates) {
-    const bool param = g.np > 0 || g.kind == K_RX || g.kind == K_RY || g.kind == K_RZ ||
-                       g.kind == K_P || g.kind == K_CP || g.kind == K_U3;
-    if (param && !is_rot(g.kind))
+    // Reference simulator.mm:822-828: only a gate that carries a parameter
+    // must have a native generator.
+    if (g.np > 0 && !is_rot(g.kind))
       fail(std::string("gradient: ") + kind_name(g.kind) +
            " carries a parameter but has no native generator; only rx, ry and rz are "
            "differentiable");
@@ -1196,6 +1196,9 @@ struct Batch::Impl {
   std::vector<BInst> h_insts;
   std::vector<uint32_t> h_order;  // every instance, largest cost first
   std::vector<BOp> h_ops;
+  // Gradient slots of rotations added without a parameter: reported as 0,
+  // as the reference does for gates with num_params == 0.
+  std::vector<uint32_t> h_nograd;
   // Spread plan for cfg.spread_bits (built by build_spread).
   uint32_t built_spread = ~0u;
   std::vector<SpreadGroup> groups;
@@ -1608,6 +1611,7 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
   std::vector<double> cost;
   uint32_t grad_base = 0;
   uint32_t max_n = 0;
+  std::vector<uint32_t> nograd;
   for (size_t i = 0; i < circuits.size(); i++) {
     BInst in{};
     in.n = circuits[i]->n;
@@ -1616,6 +1620,9 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
     in.op_end = uint32_t(pg.ops.size());
     insts.push_back(in);
     cost.push_back(double(in.op_end - in.op_begin) * double(uint64_t(1) << in.n));
+    for (size_t k = 0; k < circuits[i]->gates.size(); k++)
+      if (circuits[i]->gates[k].np == 0 && is_rot(circuits[i]->gates[k].kind))
+        nograd.push_back(grad_base + uint32_t(k));
     grad_base += uint32_t(circuits[i]->gates.size());
     max_n = std::max(max_n, in.n);
   }
@@ -1631,6 +1638,7 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
   b->h_insts = insts;
   b->h_order = order;
   b->h_ops = pg.ops;
+  b->h_nograd = nograd;
   upload(b->ops, pg.ops);
   upload(b->insts, insts);
   upload(b->terms, pg.terms);
@@ -1729,9 +1737,11 @@ void Batch::run(double *energies, double *grads) {
   if (energies)
     check(hipMemcpy(energies, b->energy, b->count * sizeof(double), hipMemcpyDeviceToHost),
           "hipMemcpy(energy)");
-  if (grads && b->ngrad)
+  if (grads && b->ngrad) {
     check(hipMemcpy(grads, b->grad, b->ngrad * sizeof(double), hipMemcpyDeviceToHost),
           "hipMemcpy(grad)");
+    for (uint32_t k : b->h_nograd) grads[k] = 0.0;
+  }
 }
 
 double Batch::last_ms() const { return impl_->ms; }

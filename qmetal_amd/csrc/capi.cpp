@@ -540,6 +540,14 @@ int qm_plan_schedule(qm_plan p, uint32_t *entry_op, uint64_t *entry_wires,
   });
 }
 
+int qm_amplitudes_at(qm_simulator s, const uint64_t *indices, size_t count, float *out) {
+  if (!s || (count && (!indices || !out))) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] { s->s->amplitudes_at(indices, count, out); });
+}
+
 int qm_set_pass_merging(qm_simulator s, int enable) {
   if (!s) {
     set_error("null simulator");

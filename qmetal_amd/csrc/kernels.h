@@ -261,6 +261,9 @@ constexpr uint32_t tile_swizzle(uint32_t i) { return i ^ (((i >> 4) ^ (i >> 8)) 
 
 void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim);
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim);
+// out[k] = a[idx[k]] for k < count (indices below the state size).
+void launch_pick(hipStream_t s, const float2 *a, const uint64_t *idx, uint64_t count,
+                 float2 *out);
 void launch_gather(hipStream_t s, int nl, const GatherArgs &g);
 void launch_diag(hipStream_t s, int nl, const DiagArgs &d);
 void launch_diag_generic(hipStream_t s, const DiagGenericArgs &d);

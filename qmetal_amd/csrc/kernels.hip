@@ -715,6 +715,14 @@ __global__ __launch_bounds__(kT) void k_zero(float2 *a, uint64_t dim) {
     v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// out[k] = a[idx[k]]; every index was checked against the state size.
+__global__ __launch_bounds__(kT) void k_pick(const float2 *a, const uint64_t *idx, uint64_t count,
+                                             float2 *out) {
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  for (uint64_t k = uint64_t(blockIdx.x) * kT + threadIdx.x; k < count; k += stride)
+    out[k] = a[idx[k]];
+}
+
 // --- FP64 reductions --------------------------------------------------------
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1247,6 +1255,11 @@ void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim) {
 
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim) {
   k_zero<<<grid_for(dim / 2), kT, 0, s>>>(a, dim);
+}
+
+void launch_pick(hipStream_t s, const float2 *a, const uint64_t *idx, uint64_t count,
+                 float2 *out) {
+  k_pick<<<grid_for(count), kT, 0, s>>>(a, idx, count, out);
 }
 
 template <bool NT>

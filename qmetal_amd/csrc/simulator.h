@@ -61,6 +61,9 @@ class Simulator {
   // Host copy of the interleaved complex64 state, refreshed on every call and
   // valid until the next call or mutation.
   const float *amplitudes();
+  // Interleaved complex64 values of the amplitudes at idx[0..count) into
+  // out[0..2 count); fails on an index past the state.
+  void amplitudes_at(const uint64_t *idx, size_t count, float *out);
 
   double abs2sum();
   double norm();

@@ -182,6 +182,12 @@ struct Simulator::Impl {
 
   std::vector<float> mirror;
 
+  // Selected-amplitude reads: indices in, amplitudes out.
+  uint64_t *pick_idx = nullptr;
+  size_t pick_idx_cap = 0;
+  float2 *pick_out = nullptr;
+  size_t pick_out_cap = 0;
+
   void use() { check(hipSetDevice(device), "hipSetDevice"); }
 
   void launched() { check(hipGetLastError(), "kernel launch"); }
@@ -1462,6 +1468,8 @@ Simulator::~Simulator() {
   hipFree(s->adj_partials);
   hipFree(s->chunks);
   hipFree(s->shots_dev);
+  hipFree(s->pick_idx);
+  hipFree(s->pick_out);
   if (s->staging) hipHostFree(s->staging);
   if (s->host_results) hipHostFree(s->host_results);
   if (s->staging_done) hipEventDestroy(s->staging_done);
@@ -1608,6 +1616,29 @@ const float *Simulator::amplitudes() {
   return s->mirror.data();
 }
 
+// Amplitudes at chosen indices, for states too large to copy whole: the
+// indices go to the device, one gather launch, the values come back.
+void Simulator::amplitudes_at(const uint64_t *idx, size_t count, float *out) {
+  Impl *s = impl_.get();
+  for (size_t k = 0; k < count; k++)
+    if (idx[k] >= s->dim)
+      fail("amplitude index " + std::to_string(idx[k]) + " out of range for " +
+           std::to_string(s->n) + " qubits");
+  if (!count) return;
+  s->use();
+  s->grow(s->pick_idx, s->pick_idx_cap, count, "hipMalloc(pick indices)");
+  s->grow(s->pick_out, s->pick_out_cap, count, "hipMalloc(pick values)");
+  check(hipMemcpyAsync(s->pick_idx, idx, count * sizeof(uint64_t), hipMemcpyHostToDevice,
+                       s->stream),
+        "hipMemcpyAsync(pick indices)");
+  launch_pick(s->stream, s->state, s->pick_idx, count, s->pick_out);
+  s->launched();
+  check(hipMemcpyAsync(out, s->pick_out, count * sizeof(float2), hipMemcpyDeviceToHost,
+                       s->stream),
+        "hipMemcpyAsync(pick values)");
+  check(hipStreamSynchronize(s->stream), "GPU error");
+}
+
 // Reference simulator.mm:615-627, with FP64 accumulation on the GPU.
 double Simulator::abs2sum() {
   Impl *s = impl_.get();
@@ -1705,10 +1736,10 @@ Gate inverse(const Gate &g) {
 }
 
 bool is_rotation(uint8_t k) { return k == K_RX || k == K_RY || k == K_RZ; }
-bool has_parameter(const Gate &g) {
-  return g.np > 0 || g.kind == K_RX || g.kind == K_RY || g.kind == K_RZ ||
-         g.kind == K_P || g.kind == K_CP || g.kind == K_U3;
-}
+// A gate gets a gradient entry when it carries a parameter and has a native
+// generator (reference simulator.mm:822-828, :917): RX/RY/RZ added with no
+// parameter report 0, and a P/CP/U3 without one is not differentiated.
+bool differentiable(const Gate &g) { return g.np > 0 && is_rotation(g.kind); }
 }  // namespace
 
 // Adjoint gradient (reference simulator.mm:814-963). The forward pass runs
@@ -1723,7 +1754,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   Impl *s = impl_.get();
   if (c.n != s->n) fail("circuit/simulator size mismatch");
   for (const Gate &g : c.gates)
-    if (has_parameter(g) && !is_rotation(g.kind))
+    if (g.np > 0 && !is_rotation(g.kind))
       fail(std::string("gradient: ") + kind_name(g.kind) +
            " carries a parameter but has no native generator; only rx, ry and rz are "
            "differentiable");
@@ -2047,7 +2078,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
     std::vector<double> grad(G, 0.0);
     for (size_t k = 0; k < G; k++) {
       const Gate &g = c.gates[k];
-      if (!is_rotation(g.kind)) continue;
+      if (!differentiable(g)) continue;
       const double *o = r + 2 * (G + 1) + 2 * rot_index[k];
       grad[k] = g.kind == K_RY ? o[0] : o[1];
     }
@@ -2079,7 +2110,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   std::vector<double> grad(G, 0.0);
   for (size_t k = 0; k < G; k++) {
     const Gate &g = c.gates[k];
-    if (!is_rotation(g.kind)) continue;
+    if (!differentiable(g)) continue;
     // grad = Im(i^ny S): ny = 1 for RY, 0 for RX/RZ.
     grad[k] = g.kind == K_RY ? r[2 * k] : r[2 * k + 1];
   }
@@ -2119,7 +2150,10 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
     check(hipStreamSynchronize(s->stream), "GPU error");
   }
   Circuit cc = c;
-  for (size_t j = 0; j < nparams; j++) cc.gates[where[j]].p[0] = host[j];
+  for (size_t j = 0; j < nparams; j++) {
+    cc.gates[where[j]].p[0] = host[j];
+    cc.gates[where[j]].np = std::max<uint8_t>(cc.gates[where[j]].np, 1);
+  }
   auto eg = energy_and_gradient(cc, h);
   // Results leave through pinned staging so the copies stay asynchronous.
   uint8_t *st = s->stage((nparams + 1) * sizeof(double));

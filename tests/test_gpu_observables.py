@@ -122,15 +122,31 @@ def test_sampling_total_variation():
 
 def test_sampling_matches_oracle_procedure_on_gpu_amplitudes():
     """Given the GPU's own amplitudes, the oracle's two-level procedure picks
-    the same index for every shot except ones on a CDF boundary."""
+    the same index for every shot except ones whose uniform lies on a CDF
+    boundary: each differing shot's u = (r >> 11) 2^-53 total must sit within
+    round-off of the cumulative probability at one of the two indices
+    (the GPU sums blocks and prefixes in another order than the oracle)."""
+    shots_n = 20000
+    us = np.array([(r >> 11) * 2.0 ** -53 for r in O.splitmix64_stream(99, shots_n)])
     for n, gates in ((11, C.qaoa_ring(11, 2)), (14, C.brickwork(14, 4)), (20, C.qft(20))):
         with sim_with(n, gates) as s:
-            shots = np.array(s.sample(20000, 99), dtype=np.uint64)
+            shots = np.array(s.sample(shots_n, 99), dtype=np.uint64)
             amps = s.amplitudes().astype(np.complex128)
-        want = O.sample(amps, 20000, 99)
-        mism = int(np.sum(shots != want))
-        print(f"n={n}: {mism} of 20000 shots differ from the oracle procedure")
-        assert mism <= 2
+        want = O.sample(amps, shots_n, 99)
+        mism = np.nonzero(shots != want)[0]
+        cum = np.cumsum(np.abs(amps) ** 2)
+        total = cum[-1]
+        for k in mism:
+            u = us[k] * total
+            edges = []
+            for i in (int(shots[k]), int(want[k])):
+                edges += [cum[i], cum[i - 1] if i else 0.0]
+            gap = min(abs(u - e) for e in edges)
+            print(f"n={n}: shot {k} GPU {int(shots[k])} oracle {int(want[k])}, "
+                  f"|u - CDF edge| = {gap:.2e}")
+            assert gap <= 1e-12 * total, (n, k)
+        print(f"n={n}: {len(mism)} of {shots_n} shots differ from the oracle procedure")
+        assert len(mism) <= 2
 
 
 def test_ghz20_invariants_and_1e5_shots():
@@ -196,6 +212,33 @@ def test_gradient_ry_closed_form():
             e, g = s.energy_and_gradient(C.to_circuit(2, [(C.RY, (0,), (t,))]),
                                          qm.Hamiltonian().add_string(1.0, "ZI"))
         assert abs(e - math.cos(t)) <= 1e-6 and abs(g[0] + math.sin(t)) <= 1e-6
+
+
+def test_gradient_follows_num_params():
+    """Through the C ABI, as simulator.mm:822-828, :917: RX added with no
+    parameter gets gradient 0, a P without one is accepted, a P with one
+    fails; the rest equals the complex128 adjoint."""
+    n = 3
+    gates = [(C.H, (0,), ()), (C.RX, (0,), ()), (C.RY, (1,), (0.7,)), (C.P, (1,), ()),
+             (C.CX, (1, 2), ()), (C.RZ, (2,), (0.3,)), (C.RX, (1,), (1.1,))]
+    terms = [(1.0, 0, 0, 1), (0.5, 2, 0, 0), (0.7, 0, 0, 6)]
+    for merge in (True, False):
+        with qm.Simulator(n) as s:
+            s.set_pass_merging(merge)
+            e, g = s.energy_and_gradient(C.to_circuit(n, gates), C.to_hamiltonian(terms))
+        e_ref, g_ref = O.adjoint(n, gates, terms)
+        assert g[1] == 0.0 and g[3] == 0.0
+        assert abs(e - e_ref) <= 1e-6 and np.abs(np.array(g) - g_ref).max() <= 1e-6
+    b = qm.Batch()
+    b.prepare([C.to_circuit(n, gates)], [C.to_hamiltonian(terms)])
+    eb, gb, _ = b.run()
+    b.close()
+    assert gb[0][1] == 0.0 and gb[0][3] == 0.0
+    assert abs(eb[0] - e_ref) <= 1e-6 and np.abs(gb[0] - g_ref).max() <= 1e-6
+    bad = C.to_circuit(n, [(C.P, (1,), (0.2,))])
+    with qm.Simulator(n) as s:
+        with pytest.raises(qm.QMetalError, match="differentiable"):
+            s.energy_and_gradient(bad, C.to_hamiltonian(terms))
 
 
 def test_gradient_fails_closed():

@@ -225,3 +225,51 @@ def test_cpu_baseline_legs_run_the_same_workload():
     assert abs(complex(*rec["first"]) - st[0]) < 1e-12
     assert abs(complex(*rec["last"]) - st[-1]) < 1e-12
     assert abs(rec["norm"] - 1) < 1e-12
+
+
+def test_qft_product_state_closed_form_matches_oracle():
+    """The full-size checks' closed forms, pinned on the complex128 oracle:
+    QFT (no swap layer) of a seeded product state at every output index, the
+    product state's Bloch vectors, and the inverse stream undoing QFT and
+    brickwork circuits."""
+    import closed_forms as CF
+    for n in (1, 2, 5, 8):
+        prep = C.product_state_prep(n, seed=300 + n)
+        a, b = CF.product_factors(prep)
+        st = O.run(n, prep)
+        xs, ys, zs = CF.bloch(a, b)
+        for q in range(n):
+            assert abs(O.expect(st, x=1 << q) - xs[q]) < 1e-12
+            assert abs(O.expect(st, y=1 << q) - ys[q]) < 1e-12
+            assert abs(O.expect(st, z=1 << q) - zs[q]) < 1e-12
+        out = O.run(n, C.qft(n), state=st)
+        want = CF.qft_product_amplitudes(a, b, np.arange(1 << n))
+        assert np.abs(out - want).max() < 1e-12, n
+        back = O.run(n, CF.inverse_stream(C.qft(n)), state=out)
+        assert np.abs(back - st).max() < 1e-12
+    # A dropped bit reversal or a flipped phase sign fails the form.
+    n = 5
+    prep = C.product_state_prep(n, seed=305)
+    a, b = CF.product_factors(prep)
+    out = O.run(n, prep + C.qft(n))
+    idx = np.arange(1 << n)
+    assert np.abs(out - CF.qft_product_amplitudes(a, b, CF.bit_reverse(idx, n))).max() > 1e-3
+    assert np.abs(out - CF.qft_product_amplitudes(np.conj(a), np.conj(b), idx)).max() > 1e-3
+    g = C.brickwork(6, 4, seed=9)
+    assert np.abs(O.run(6, CF.inverse_stream(g), state=O.run(6, g)) - O.zero_state(6)).max() < 1e-12
+
+
+def test_adjoint_follows_num_params():
+    """simulator.mm:822-828, :917: a gate carrying a parameter must be a
+    rotation; a rotation added without one gets gradient 0, and a P added
+    without one is accepted."""
+    h = [(1.0, 0, 0, 1), (0.5, 2, 0, 0)]
+    gates = [(C.H, (0,), ()), (C.RX, (0,), ()), (C.RY, (1,), (0.7,)), (C.P, (1,), ()),
+             (C.CX, (1, 0), ()), (C.RZ, (0,), (0.3,))]
+    e, g = O.adjoint(2, gates, h)
+    assert g[1] == 0.0 and g[3] == 0.0
+    assert abs(g[2]) > 1e-3
+    want = O.param_shift(2, gates, h)
+    assert np.abs(g - want).max() < 1e-12
+    with pytest.raises(ValueError):
+        O.adjoint(2, [(C.P, (1,), (0.2,))], h)
