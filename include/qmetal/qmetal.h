@@ -234,6 +234,20 @@ int qm_plan_schedule(qm_plan p, uint32_t *entry_op, uint64_t *entry_wires,
  * a zero-copy view of unified memory (src/metal/simulator.mm:599-602). */
 int qm_amplitudes_at(qm_simulator s, const uint64_t *indices, size_t count, float *out);
 
+/* qm_sample without allocating: the same procedure, kernels and draws as
+ * qm_sample (identical seeds give identical indices), with every piece of
+ * device scratch -- block CDF, scan totals, drawn indices -- in a fixed
+ * 16 MiB array per GPU that lives until the process ends. It makes no
+ * hipMalloc, hipFree or pinned allocation on any call. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * around HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Limits: states of up to 30 qubits and up to 2^20 shots per
+ * call; past either it fails with a message naming the limit. All calls on
+ * one GPU share its array: calls that overlap on one GPU (from several host
+ * threads) race, calls made one after another do not (each waits for its
+ * own work). `out` may be pageable or pinned host memory. */
+int qm_sample_noalloc(qm_simulator s, size_t shots, uint64_t seed, uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -548,6 +548,14 @@ int qm_amplitudes_at(qm_simulator s, const uint64_t *indices, size_t count, floa
   return guard([&] { s->s->amplitudes_at(indices, count, out); });
 }
 
+int qm_sample_noalloc(qm_simulator s, size_t shots, uint64_t seed, uint64_t *out) {
+  if (!s || (shots && !out)) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] { s->s->sample_noalloc(shots, seed, out); });
+}
+
 int qm_set_pass_merging(qm_simulator s, int enable) {
   if (!s) {
     set_error("null simulator");

@@ -70,6 +70,9 @@ class Simulator {
   double expectation(const Pauli &p);
   double expectation(const Hamiltonian &h);
   std::vector<uint64_t> sample(size_t shots, uint64_t seed);
+  // The same draws into out[0..shots) with no device allocation: scratch in
+  // a fixed per-GPU array (states of up to 30 qubits, up to 2^20 shots).
+  void sample_noalloc(size_t shots, uint64_t seed, uint64_t *out);
 
   std::pair<double, std::vector<double>> energy_and_gradient(
       const Circuit &c, const Hamiltonian &h);

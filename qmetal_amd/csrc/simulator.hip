@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "kernels.h"
@@ -1717,6 +1718,78 @@ std::vector<uint64_t> Simulator::sample(size_t shots, uint64_t seed) {
         "hipMemcpyAsync(shots)");
   check(hipStreamSynchronize(s->stream), "GPU error");
   return out;
+}
+
+// --- allocation-free sampling ----------------------------------------------
+//
+// qm_sample_noalloc keeps its scratch in `scratch`, one copy per GPU for the
+// life of the process: the 1024-amplitude block sums that become the CDF
+// (states of up to kNoAllocMaxQubits qubits), the prefix scan's chunk totals,
+// and the drawn indices (up to kNoAllocMaxShots). Nothing in it needs
+// clearing: each kernel writes every element the next one reads. Every call
+// on a GPU shares that GPU's copy, so calls that overlap on one GPU (from
+// several threads) race; calls made one after another do not, because each
+// waits for its own work before returning.
+namespace {
+constexpr uint32_t kNoAllocMaxQubits = 30;
+constexpr uint64_t kNoAllocMaxShots = uint64_t(1) << 20;
+constexpr uint64_t kNoAllocBlocks = (uint64_t(1) << kNoAllocMaxQubits) / kSampleBlock;
+constexpr uint64_t kNoAllocChunks = (kNoAllocBlocks + 1023) / 1024;
+constexpr uint64_t kScratchDoubles = kNoAllocBlocks + kNoAllocChunks + kNoAllocMaxShots;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// Device address of the current GPU's `scratch`. In host code the array's
+// name is only a stand-in for the symbol, so the address is looked up once
+// per device, with that device current, and used wherever the host hands the
+// memory to a kernel or a copy.
+double *scratch_here() {
+  const int d = current_device();
+  if (d < 0 || d >= kMaxDevices) fail("sample_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[d]) {
+    void *p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) fail("sample_noalloc: no device address for the scratch array");
+    g_scratch[d] = static_cast<double *>(p);
+  }
+  return g_scratch[d];
+}
+}  // namespace
+
+// qm_sample's procedure and kernels with the scratch above instead of the
+// simulator's grown buffers: no device or pinned allocation on any call.
+void Simulator::sample_noalloc(size_t shots, uint64_t seed, uint64_t *out) {
+  Impl *s = impl_.get();
+  if (s->n > kNoAllocMaxQubits)
+    fail("sample_noalloc: " + std::to_string(s->n) + " qubits is past the limit of " +
+         std::to_string(kNoAllocMaxQubits) + " (its fixed scratch holds the CDF of 2^" +
+         std::to_string(kNoAllocMaxQubits) + " amplitudes); use qm_sample");
+  if (shots > kNoAllocMaxShots)
+    fail("sample_noalloc: " + std::to_string(shots) + " shots is past the limit of " +
+         std::to_string(kNoAllocMaxShots) + " per call; use qm_sample");
+  if (shots == 0) return;
+  s->use();
+  double *base = scratch_here();
+  double *cdf = base;
+  double *chunks = base + kNoAllocBlocks;
+  uint64_t *drawn = reinterpret_cast<uint64_t *>(base + kNoAllocBlocks + kNoAllocChunks);
+  const uint64_t nb = (s->dim + kSampleBlock - 1) / kSampleBlock;
+  launch_block_sums(s->stream, s->state, s->dim, cdf);
+  s->launched();
+  launch_prefix(s->stream, cdf, nb, chunks);
+  s->launched();
+  launch_shots(s->stream, s->state, s->dim, cdf, nb, shots, seed, drawn);
+  s->launched();
+  s->dispatches++;
+  s->launches++;
+  check(hipMemcpyAsync(out, drawn, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream),
+        "hipMemcpyAsync(shots)");
+  check(hipStreamSynchronize(s->stream), "GPU error");
 }
 
 namespace {
