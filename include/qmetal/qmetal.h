@@ -212,7 +212,9 @@ int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
  * actually launched, merged passes once. On by default;
  * qm_set_pass_merging(s, 0) restores one launch per plan op. Timing records of kind 7 to 12 are
  * merged passes (generic, chain-shaped, wide and low chain-shaped, generic
- * low tile and generic wide kernels).
+ * low tile and generic wide kernels); 13 to 16 are the adjoint gradient's
+ * kernels (merged backward passes over both states, co-state build, energy
+ * overlap, one-launch-per-rotation steps).
  * qm_plan_passes reports the grouping qm_run would use: pass_of_op
  * (qm_plan_size entries, may be NULL) gets the index of the first pass that
  * runs each op (UINT32_MAX for an op that launches nothing), *num_passes the

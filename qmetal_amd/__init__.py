@@ -198,8 +198,11 @@ PLAN_KINDS = ("Dense1q", "Dense1qGroup", "Blocked1q", "Controlled1q", "DiagonalL
               "Swap", "CCX")
 # Timing records: plan-op kinds, then merged passes of several plan ops
 # (generic pass kernel, chain kernel, wide and low chain kernels, generic
-# low tile and generic wide kernels).
-TIMING_KINDS = PLAN_KINDS + ("Pass", "ChainPass", "WidePass", "LowPass", "TilePass", "GWidePass")
+# low tile and generic wide kernels), then the adjoint gradient's kernels
+# (merged backward passes over both states, co-state build, energy overlap,
+# one-launch-per-rotation steps).
+TIMING_KINDS = PLAN_KINDS + ("Pass", "ChainPass", "WidePass", "LowPass", "TilePass", "GWidePass",
+                             "AdjPass", "CoState", "Overlap", "AdjRot")
 
 
 def version() -> tuple[int, int, int]:

@@ -55,6 +55,13 @@ constexpr uint32_t kTimingWide = 9;
 constexpr uint32_t kTimingLow = 10;
 constexpr uint32_t kTimingTile = 11;
 constexpr uint32_t kTimingGWide = 12;
+// The adjoint's kernels: merged backward passes over both states, the
+// co-state build (zeroing and Pauli accumulation), the energy overlap, and
+// the one-launch-per-rotation step.
+constexpr uint32_t kTimingAdjPass = 13;
+constexpr uint32_t kTimingCoState = 14;
+constexpr uint32_t kTimingOverlap = 15;
+constexpr uint32_t kTimingAdjRot = 16;
 
 int current_device() {
   int d = 0;
@@ -1283,6 +1290,24 @@ struct Simulator::Impl {
     return e;
   }
 
+  // Runs `launch` between two timing events when timing is on; `bytes` is
+  // one read and one write of every amplitude the kernel changes plus one
+  // read of every amplitude it only reads.
+  template <typename F>
+  void timed(uint32_t kind, double bytes, F &&launch) {
+    hipEvent_t a = nullptr;
+    if (timing) {
+      a = take_event();
+      check(hipEventRecord(a, stream), "hipEventRecord");
+    }
+    launch();
+    if (timing) {
+      hipEvent_t b = take_event();
+      check(hipEventRecord(b, stream), "hipEventRecord");
+      timings.push_back(OpTiming{kind, bytes, a, b});
+    }
+  }
+
   void op(const PlanOp &p, const Pending *pl) {
     if (p.kind == OpKind::Diagonal && p.masks.empty()) return;  // no dispatch
     hipEvent_t a = nullptr;
@@ -1848,7 +1873,8 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   run_plan(build_plan(c, FuseOpts{}));
 
   // lambda = H psi, terms grouped by flip mask.
-  launch_zero(s->stream, s->costate, s->dim);
+  const double amp_bytes = 8.0 * double(s->dim);
+  s->timed(kTimingCoState, amp_bytes, [&] { launch_zero(s->stream, s->costate, s->dim); });
   s->launched();
   std::map<uint64_t, std::vector<size_t>> by_flip;
   for (size_t k = 0; k < h.size(); k++) by_flip[h[k].p.x | h[k].p.y].push_back(k);
@@ -1906,7 +1932,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
           }
       }
       a.off[nw + 2] = n;
-      launch_pauli_accum_tile(s->stream, a);
+      s->timed(kTimingCoState, 3 * amp_bytes, [&] { launch_pauli_accum_tile(s->stream, a); });
       s->launched();
       s->dispatches++;
       s->launches++;
@@ -1927,7 +1953,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
         a.sign[a.nterms] = t.p.y | t.p.z;
         a.nterms++;
       }
-      launch_pauli_accum(s->stream, a);
+      s->timed(kTimingCoState, 3 * amp_bytes, [&] { launch_pauli_accum(s->stream, a); });
       s->launched();
       s->dispatches++;
       s->launches++;
@@ -1940,7 +1966,8 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   // then per rotation in sweep order (merged passes).
   double *slots = s->result_slots(2 * (G + 1) + 2 * nrot_total);
   // E = Re<psi|lambda>
-  s->pauli_sum(s->state, s->costate, Pauli{}, slots + 2 * G);
+  s->timed(kTimingOverlap, 2 * amp_bytes,
+           [&] { s->pauli_sum(s->state, s->costate, Pauli{}, slots + 2 * G); });
   s->dispatches++;
   s->launches++;
 
@@ -2142,7 +2169,9 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
             "hipMalloc(adjoint partials)");
     double *rot_slots = slots + 2 * (G + 1);
     for (size_t i = 0; i < passes.size(); i++) {
-      launch_adj_pass(s->stream, args[i], s->adj_partials, rot_slots + 2 * passes[i].rot0);
+      s->timed(kTimingAdjPass, 4 * amp_bytes, [&] {
+        launch_adj_pass(s->stream, args[i], s->adj_partials, rot_slots + 2 * passes[i].rot0);
+      });
       s->launched();
       s->dispatches++;
       s->launches++;
@@ -2171,7 +2200,8 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       a.q = g.q[0];
       a.axis = g.kind == K_RX ? 0 : g.kind == K_RY ? 1 : 2;
       narrow(m, a.m);
-      launch_adjoint_rot(s->stream, a, s->partials, slots + 2 * k);
+      s->timed(kTimingAdjRot, 4 * amp_bytes,
+               [&] { launch_adjoint_rot(s->stream, a, s->partials, slots + 2 * k); });
       s->launched();
     } else {
       s->gate(inv, s->costate);
