@@ -549,7 +549,8 @@ def hea_adjoint(qm, C, n: int, reps: int):
         "plan_equiv_GBps": round(ref_bytes / best / 1e9, 1),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(f"hea_d3_tfim_ring_n{n}", dom),
                      "kernels": {k: {"launches": c_, "avg_ms": round(ms / c_, 3),
                                      "GBps": round(by / (ms * 1e-3) / 1e9, 1)}
                                  for k, (c_, ms, by) in sorted(kinds.items())}},
@@ -597,6 +598,7 @@ def brickwork_bench(qm, C, n: int, reps: int):
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": load_traffic(f"brickwork_d10_n{n}", dom),
                          "kernels": {k: {"launches": c_, "avg_ms": round(ms / c_, 3),
                                          "GBps": round(by / (ms * 1e-3) / 1e9, 1)}
                                      for k, (c_, ms, by) in sorted(kinds.items())}}}
@@ -642,6 +644,10 @@ def main() -> int:
                     help="run the reference's fusion_bench / ablation protocols on its frozen "
                          "benchmark set at n=N (24, 26 or 33) instead of the headline")
     ap.add_argument("--frozen-reps", type=int, default=2)
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time the HEA depth-3 adjoint at n=--adjoint-n, per kernel")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the batched QAOA measurement")
     ap.add_argument("--brickwork", type=int, default=0, metavar="N",
                     help="only time brickwork depth 10 at n=N, per kernel, instead of the headline")
     ap.add_argument("--block-bits", type=int, default=10)
@@ -680,6 +686,17 @@ def main() -> int:
     if args.suite:
         print(f"# device {qm.device_name()}  suite", file=sys.stderr, flush=True)
         suite(qm, C, lambda rec: print(json.dumps(rec), flush=True))
+        return 0
+    if args.adjoint_only:
+        print(f"# device {qm.device_name()}  HEA d3 adjoint n={args.adjoint_n}", file=sys.stderr,
+              flush=True)
+        print(json.dumps(hea_adjoint(qm, C, args.adjoint_n, 2)), flush=True)
+        return 0
+    if args.batch_only:
+        print(f"# device {qm.device_name()}  batched QAOA x{args.batch_count}", file=sys.stderr,
+              flush=True)
+        print(json.dumps(batched_qaoa(qm, C, args.batch_count, args.batch_repeats,
+                                      args.plain_count)), flush=True)
         return 0
     if args.brickwork:
         print(f"# device {qm.device_name()}  brickwork d10 n={args.brickwork}", file=sys.stderr,

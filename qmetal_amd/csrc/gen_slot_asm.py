@@ -10,12 +10,16 @@ whole tile across the op loop (double the VGPRs, half the occupancy). One
 statement whose outputs are tied to its inputs defines every amplitude exactly
 once per op, whatever section runs inside it.
 
-Section `sel` = lbit + NL * kind:
+Section `sel` = 8 * kind + lbit:
   kind 0  real matrix (m00, m01, m10, m11 = m[0], m[2], m[4], m[6]), all pairs
-  kind 1  complex matrix, all pairs
+          (8 instructions per pair)
+  kind 1  complex matrix, all pairs (16 per pair)
   kind 2  complex matrix, only the pairs whose bit is set in `pm` (pair p is
           the p-th index l with bit lbit clear, ascending)
+  kind 5  X: swap the two amplitudes of the pairs in `pm` (2 per pair)
 Any other `sel` leaves the tile unchanged. The matrix is per lane (VGPRs).
+(Kinds 3 and 4, complex matrices on the matrix core, were measured and
+dropped: slower than the VALU in these passes.)
 """
 import os
 
@@ -71,25 +75,43 @@ def pairs(nl, h):
     return [(l, l | (1 << h)) for l in range(1 << nl) if not (l >> h) & 1]
 
 
+def swap_section(nl, h):
+    out = []
+    for p, (a, b) in enumerate(pairs(nl, h)):
+        out += [f"s_bitcmp1_b32 %[pm], {p}", f"s_cbranch_scc0 .Lqx%=_{h}_{p}",
+                f"v_swap_b32 %[r{a}], %[r{b}]", f"v_swap_b32 %[i{a}], %[i{b}]",
+                f".Lqx%=_{h}_{p}:"]
+    return out
+
+
+KIND_ORDER = (1, 5, 0, 2)  # dispatch order: the common kinds first
+
+
 def statement(nl):
-    lines = []
-    nsec = 3 * nl
-    for sec in range(nsec):
-        lines += [f"s_cmp_eq_u32 %[sel], {sec}", f"s_cbranch_scc1 .Lqs%=_{sec}"]
+    lines = ["s_lshr_b32 %[kk], %[sel], 3", "s_and_b32 %[hh], %[sel], 7"]
+    for kind in KIND_ORDER:
+        lines += [f"s_cmp_eq_u32 %[kk], {kind}", f"s_cbranch_scc1 .Lqk%=_{kind}"]
     lines.append("s_branch .Lqs%=_end")
-    for sec in range(nsec):
-        h, kind = sec % nl, sec // nl
-        lines.append(f".Lqs%=_{sec}:")
-        for p, (a, b) in enumerate(pairs(nl, h)):
-            if kind == 0:
-                lines += bfly_real(a, b)
-            elif kind == 1:
-                lines += bfly_cplx(a, b)
-            else:
-                lines += [f"s_bitcmp1_b32 %[pm], {p}", f"s_cbranch_scc0 .Lqs%=_{sec}_{p}"]
-                lines += bfly_cplx(a, b)
-                lines.append(f".Lqs%=_{sec}_{p}:")
+    for kind in KIND_ORDER:
+        lines.append(f".Lqk%=_{kind}:")
+        for h in range(nl):
+            lines += [f"s_cmp_eq_u32 %[hh], {h}", f"s_cbranch_scc1 .Lqs%=_{kind}_{h}"]
         lines.append("s_branch .Lqs%=_end")
+        for h in range(nl):
+            lines.append(f".Lqs%=_{kind}_{h}:")
+            if kind == 5:
+                lines += swap_section(nl, h)
+            else:
+                for p, (a, b) in enumerate(pairs(nl, h)):
+                    if kind == 0:
+                        lines += bfly_real(a, b)
+                    elif kind == 1:
+                        lines += bfly_cplx(a, b)
+                    else:
+                        lines += [f"s_bitcmp1_b32 %[pm], {p}", f"s_cbranch_scc0 .Lqs%=_{kind}_{h}_{p}"]
+                        lines += bfly_cplx(a, b)
+                        lines.append(f".Lqs%=_{kind}_{h}_{p}:")
+            lines.append("s_branch .Lqs%=_end")
     lines.append(".Lqs%=_end:")
     body = "\n".join(f'      "{ln}\\n"' for ln in lines)
     n = 1 << nl
@@ -99,10 +121,12 @@ def statement(nl):
 __device__ __forceinline__ void slot_asm<{nl}>(float2 (&x)[{n}], uint32_t sel, uint32_t pm,
                                              const float (&m)[8]) {{
   float t0, t1, t2, t3;
+  uint32_t kk, hh;
   asm(
 {body}
       : {outs.strip()},
-        [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
+        [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3),
+        [kk] "=&s"(kk), [hh] "=&s"(hh)
       : [m0r] "v"(m[0]), [m0i] "v"(m[1]), [m1r] "v"(m[2]), [m1i] "v"(m[3]),
         [m2r] "v"(m[4]), [m2i] "v"(m[5]), [m3r] "v"(m[6]), [m3i] "v"(m[7]),
         [sel] "s"(sel), [pm] "s"(pm)

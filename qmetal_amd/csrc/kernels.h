@@ -112,7 +112,7 @@ struct PassOp {
   uint32_t type;
   uint32_t lbit;
   uint32_t cmask;
-  uint32_t sel;   // slot: code section, lbit + NL * (0 real, 1 complex, 2 masked)
+  uint32_t sel;   // slot: code section, 8 * kind + lbit (kinds: slot_asm.inc's generator)
   uint32_t pm;    // slot, masked: pair p (p-th local index with lbit clear) applies
   uint32_t term0;
   uint32_t nterms;
@@ -358,6 +358,7 @@ void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,
 // type 1 first accumulates the rotation's overlap <lam|P|psi>, P = X, Y, Z
 // (`axis` 0, 1, 2) on tile-local bit `lbit`, in FP64 into slot `rslot`.
 constexpr int kAdjMaxRot = 8;
+constexpr int kAdjBlocks = 4096;
 struct AdjOp {
   uint32_t type;
   uint32_t lbit;
@@ -379,8 +380,10 @@ struct AdjPassArgs {
   uint32_t nrot;
 };
 
-// Grid of kRedBlocks workgroups; per-block FP64 partials of each rotation go
-// to `partials` (kAdjMaxRot * kRedBlocks * 2 doubles) and are reduced in a
+// Grid of kAdjBlocks workgroups (four waves per SIMD at the kernel's
+// register use, which a streaming pass over two states needs to keep enough
+// loads in flight); per-block FP64 partials of each rotation go to
+// `partials` (kAdjMaxRot * kAdjBlocks * 2 doubles) and are reduced in a
 // fixed order into out[2 r], out[2 r + 1] (r < nrot).
 void launch_adj_pass(hipStream_t s, const AdjPassArgs &a, double *partials, double *out);
 

@@ -1424,9 +1424,9 @@ void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,
 }
 
 void launch_adj_pass(hipStream_t s, const AdjPassArgs &a, double *partials, double *out) {
-  if (a.lay.streaming) k_adj_pass<true><<<kRedBlocks, kT, 0, s>>>(a, partials);
-  else k_adj_pass<false><<<kRedBlocks, kT, 0, s>>>(a, partials);
-  if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, kRedBlocks, out);
+  if (a.lay.streaming) k_adj_pass<true><<<kAdjBlocks, kT, 0, s>>>(a, partials);
+  else k_adj_pass<false><<<kAdjBlocks, kT, 0, s>>>(a, partials);
+  if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, kAdjBlocks, out);
 }
 
 void launch_block_sums(hipStream_t s, const float2 *a, uint64_t dim, double *sums) {

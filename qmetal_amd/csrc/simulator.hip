@@ -146,6 +146,16 @@ Slot make_slot(int lbit, uint32_t cmask, const float *m8) {
 
 const float kX[8] = {0, 0, 1, 0, 1, 0, 0, 0};
 
+// Section kind of a 2x2 slot (slot_asm, sel = 8 * kind + local bit): an X
+// whose controls are all tile-local (or absent) swaps amplitudes (5, two
+// instructions per pair instead of 16); a control on a base-index bit makes
+// the matrix per lane; otherwise real (0), complex (1), or complex over the
+// locally controlled pairs (2).
+uint32_t slot_kind(const float *m8, bool real, bool local_ctrl, bool lane_ctrl) {
+  if (!lane_ctrl && std::memcmp(m8, kX, sizeof(kX)) == 0) return 5u;
+  return local_ctrl ? 2u : (real ? 0u : 1u);
+}
+
 }  // namespace
 
 struct Simulator::Impl {
@@ -575,8 +585,7 @@ struct Simulator::Impl {
         o.sel = ~0u;
         continue;
       }
-      const uint32_t kind = o.cmask ? 2u : (o.type == kPassRealSlot ? 0u : 1u);
-      o.sel = o.lbit + uint32_t(nl) * kind;
+      o.sel = o.lbit + 8u * slot_kind(o.m, o.type == kPassRealSlot, o.cmask != 0, o.gmask != 0);
       o.pm = 0;
       uint32_t pidx = 0;
       for (uint32_t l = 0; l < (1u << nl); l++) {
@@ -2104,7 +2113,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
         }
         std::memcpy(o.m, m8, sizeof(o.m));
         const bool real = m8[1] == 0.0f && m8[3] == 0.0f && m8[5] == 0.0f && m8[7] == 0.0f;
-        o.sel = o.lbit + 4u * (cm ? 2u : (real ? 0u : 1u));
+        o.sel = o.lbit + 8u * slot_kind(m8, real, cm != 0, o.gmask != 0);
         uint32_t pidx = 0;
         for (uint32_t l = 0; l < 16; l++) {
           if ((l >> o.lbit) & 1) continue;
@@ -2165,7 +2174,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       a.nrot = ps.nrot;
     }
     s->upload(total);
-    s->grow(s->adj_partials, s->adj_partials_cap, size_t(kAdjMaxRot) * kRedBlocks * 2,
+    s->grow(s->adj_partials, s->adj_partials_cap, size_t(kAdjMaxRot) * kAdjBlocks * 2,
             "hipMalloc(adjoint partials)");
     double *rot_slots = slots + 2 * (G + 1);
     for (size_t i = 0; i < passes.size(); i++) {
