@@ -190,6 +190,12 @@ int qm_batch_run(qm_batch b, double *out_energy, double *out_grad,
 int qm_batch_set_spread(qm_batch b, uint32_t min_qubits);
 /* The current spread width (0 = none; 0 for a NULL handle). */
 uint32_t qm_batch_spread(qm_batch b);
+/* Spread instances of 14 qubits or more that share a width, a gate
+ * structure (kinds and wires in order) and a Hamiltonian run as one batched
+ * program of the single-circuit merged passes (every pass one launch over all
+ * of them) instead of the lockstep spread steps; *out gets how many instances
+ * run that way under the current spread width. */
+int qm_batch_batched_instances(qm_batch b, size_t *out);
 
 /* Device timing of plan-op kernels, measured with events on the simulator's
  * own stream around each launch. qm_read_timing synchronises, reports every

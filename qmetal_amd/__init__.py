@@ -167,6 +167,7 @@ _batch_set_config = _decl("qm_batch_set_config", c_int, c_void_p, c_uint32, c_ui
 _batch_run = _decl("qm_batch_run", c_int, c_void_p, _f64p, _f64p, _f64p)
 _batch_set_spread = _decl("qm_batch_set_spread", c_int, c_void_p, c_uint32)
 _batch_spread = _decl("qm_batch_spread", c_uint32, c_void_p)
+_batch_batched = _decl("qm_batch_batched_instances", c_int, c_void_p, POINTER(c_size_t))
 
 _plan_build = _decl("qm_plan_build", c_void_p, c_void_p, POINTER(FusionOptions))
 _plan_destroy = _decl("qm_plan_destroy", None, c_void_p)
@@ -562,6 +563,14 @@ class Batch:
         """Instances of min_qubits or more run spread (0: none)."""
         _check(_batch_set_spread(self._h, min_qubits))
         return self
+
+    def batched_instances(self) -> int:
+        """Spread instances that run as batched merged-pass programs (classes
+        sharing width, gate structure and Hamiltonian) under the current
+        spread width."""
+        out = c_size_t()
+        _check(_batch_batched(self._h, byref(out)))
+        return int(out.value)
 
     def run(self):
         """(energies, per-instance gradient lists, device ms)."""

@@ -52,6 +52,11 @@ class Batch {
   // Instances with n >= bits run spread (0: none).
   void set_spread(uint32_t bits);
   BatchConfig config() const;
+  // Instances that run as batched merged-pass programs (one per class of
+  // spread instances sharing width, gate structure and Hamiltonian) under
+  // the current spread width; the other spread instances take the lockstep
+  // spread steps.
+  size_t batched_instances();
   // One replay of the captured launch; results copied back.
   void run(double *energies, double *grads);
   // Device milliseconds of the last run (events around the graph).

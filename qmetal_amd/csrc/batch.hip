@@ -32,8 +32,10 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <string>
 
 #include "batch.h"
+#include "simulator.h"
 
 namespace qmamd {
 namespace {
@@ -1149,6 +1151,8 @@ void upload(T *&dst, const std::vector<T> &src) {
 // (at most kSpreadBudget bytes or half the free device memory, groups run one
 // after another).
 constexpr uint32_t kSpreadChunkBits = 13;
+// Widest instances first form batched classes (see Batch::Impl::Class).
+constexpr uint32_t kClassMinQubits = 14;
 // 256 threads: a spread workgroup (one wave per SIMD) fits beside a
 // persistent one on the same CU.
 constexpr int kSpreadThreads = 256;
@@ -1201,6 +1205,23 @@ struct Batch::Impl {
   std::vector<uint32_t> h_nograd;
   // Spread plan for cfg.spread_bits (built by build_spread).
   uint32_t built_spread = ~0u;
+  // Classes of wide instances (at least kClassMinQubits) that share a width,
+  // a gate structure and a Hamiltonian: when spread, a class runs as one
+  // batched program of the single-circuit merged passes (every pass one
+  // launch over the class, BatchedAdjoint) instead of the lockstep spread
+  // steps; a class that does not compile keeps the spread steps.
+  struct Class {
+    uint32_t n = 0;
+    std::vector<uint32_t> members;
+    bool tried = false;
+    std::unique_ptr<BatchedAdjoint> prog;
+  };
+  std::vector<Class> classes;
+  std::vector<int32_t> class_of;  // per instance; -1: none
+  std::vector<Circuit> h_circ;    // per instance in a class, else empty
+  std::vector<Hamiltonian> h_ham;
+  std::vector<uint32_t> h_gbase;
+  std::vector<uint32_t> active;  // classes run batched under cfg.spread_bits
   std::vector<SpreadGroup> groups;
   uint32_t *sp_inst = nullptr, *sp_wg_base = nullptr, *sp_wg_inst = nullptr;
   uint32_t *sp_step_off = nullptr, *sp_nsteps = nullptr, *sp_steps = nullptr;
@@ -1237,13 +1258,38 @@ struct Batch::Impl {
   // budget, each instance given 2^(n - kSpreadChunkBits) workgroups (at
   // least one). The persistent kernel keeps the rest, in cost order, and its
   // autotuning sample is every 16th of them.
+  // Compiles class k once (a simulator of its width prepares the passes).
+  void compile_class(Class &c) {
+    if (c.tried) return;
+    c.tried = true;
+    std::vector<const Circuit *> cs;
+    std::vector<uint32_t> gb;
+    for (uint32_t i : c.members) {
+      cs.push_back(&h_circ[i]);
+      gb.push_back(h_gbase[i]);
+    }
+    check(hipSetDevice(device), "hipSetDevice");
+    Simulator sim(c.n);
+    c.prog = sim.compile_batched(cs, h_ham[c.members[0]], c.members, gb);
+  }
+
   void build_spread() {
     if (built_spread == cfg.spread_bits) return;
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     free_spread();
+    active.clear();
+    for (size_t k = 0; k < classes.size(); k++) {
+      Class &c = classes[k];
+      if (!cfg.spread_bits || c.n < cfg.spread_bits) continue;
+      compile_class(c);
+      if (c.prog) active.push_back(uint32_t(k));
+    }
     std::vector<uint32_t> sp, rest;
-    for (uint32_t i : h_order)
-      (cfg.spread_bits && h_insts[i].n >= cfg.spread_bits ? sp : rest).push_back(i);
+    for (uint32_t i : h_order) {
+      const bool spread = cfg.spread_bits && h_insts[i].n >= cfg.spread_bits;
+      if (spread && class_of[i] >= 0 && classes[size_t(class_of[i])].prog) continue;
+      (spread ? sp : rest).push_back(i);
+    }
     std::vector<uint32_t> sample;
     for (size_t k = 0; k < rest.size(); k += 16) sample.push_back(rest[k]);
     upload(order, rest);
@@ -1418,6 +1464,12 @@ struct Batch::Impl {
 
   void free_all() {
     free_spread();
+    classes.clear();
+    class_of.clear();
+    h_circ.clear();
+    h_ham.clear();
+    h_gbase.clear();
+    active.clear();
     hipFree(ops);
     hipFree(insts);
     hipFree(order);
@@ -1489,12 +1541,14 @@ struct Batch::Impl {
     const uint32_t wgs = uint32_t(cus) * cfg.per_cu;
     ensure_scratch(wgs);
     const BatchArgs A = args(sample);
-    const bool both = !sample && !groups.empty() && A.count;
+    const bool both = !sample && (!groups.empty() || !active.empty()) && A.count;
     hipStream_t ss = both ? side : stream;
     if (both) {
       check(hipEventRecord(fork, stream), "hipEventRecord");
       check(hipStreamWaitEvent(side, fork, 0), "hipStreamWaitEvent");
     }
+    if (!sample)
+      for (uint32_t k : active) classes[k].prog->enqueue(ss, energy, grad);
     if (!sample)
       for (const SpreadGroup &g : groups) {
         SpreadArgs S{};
@@ -1639,6 +1693,43 @@ void Batch::prepare(const std::vector<const Circuit *> &circuits,
   b->h_order = order;
   b->h_ops = pg.ops;
   b->h_nograd = nograd;
+  // Classes of wide instances: same width, gate kinds and wires, and
+  // Hamiltonian (keyed by their bytes).
+  b->class_of.assign(circuits.size(), -1);
+  b->h_circ.resize(circuits.size());
+  b->h_ham.resize(circuits.size());
+  b->h_gbase.assign(circuits.size(), 0);
+  {
+    std::map<std::string, int32_t> keys;
+    uint32_t gb = 0;
+    for (size_t i = 0; i < circuits.size(); i++) {
+      const Circuit &c = *circuits[i];
+      b->h_gbase[i] = gb;
+      gb += uint32_t(c.gates.size());
+      if (c.n < kClassMinQubits) continue;
+      std::string key(reinterpret_cast<const char *>(&c.n), sizeof(c.n));
+      for (const Gate &g : c.gates) {
+        const uint8_t hdr[3] = {g.kind, g.nq, uint8_t(g.np > 0)};
+        key.append(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+        key.append(reinterpret_cast<const char *>(g.q), sizeof(g.q));
+      }
+      key.push_back('|');
+      for (const Term &t : *hams[i]) {
+        key.append(reinterpret_cast<const char *>(&t.c), sizeof(t.c));
+        key.append(reinterpret_cast<const char *>(&t.p), sizeof(t.p));
+      }
+      auto it = keys.find(key);
+      if (it == keys.end()) {
+        it = keys.emplace(key, int32_t(b->classes.size())).first;
+        b->classes.emplace_back();
+        b->classes.back().n = c.n;
+      }
+      b->class_of[i] = it->second;
+      b->classes[size_t(it->second)].members.push_back(uint32_t(i));
+      b->h_circ[i] = c;
+      b->h_ham[i] = *hams[i];
+    }
+  }
   upload(b->ops, pg.ops);
   upload(b->insts, insts);
   upload(b->terms, pg.terms);
@@ -1745,6 +1836,14 @@ void Batch::run(double *energies, double *grads) {
 }
 
 double Batch::last_ms() const { return impl_->ms; }
+
+size_t Batch::batched_instances() {
+  Impl *b = impl_.get();
+  b->build_spread();
+  size_t k = 0;
+  for (uint32_t c : b->active) k += b->classes[c].members.size();
+  return k;
+}
 size_t Batch::total_gates() const { return impl_->ngrad; }
 
 }  // namespace qmamd

@@ -398,6 +398,14 @@ int qm_batch_set_spread(qm_batch b, uint32_t min_qubits) {
 
 uint32_t qm_batch_spread(qm_batch b) { return b ? b->b->config().spread_bits : 0; }
 
+int qm_batch_batched_instances(qm_batch b, size_t *out) {
+  if (!b || !out) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] { *out = b->b->batched_instances(); });
+}
+
 int qm_batch_run(qm_batch b, double *out_energy, double *out_grad, double *out_ms) {
   if (!b) {
     set_error("null batch");

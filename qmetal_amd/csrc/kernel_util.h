@@ -261,6 +261,14 @@ __device__ __forceinline__ double wave_allsum(double v) {
 // Read-only kernel inputs seen through the constant address space: uniform
 // loads through such a pointer become scalar loads into SGPRs (the compiler
 // need not prove that the state stores of the kernel do not alias them).
+// Instance blockIdx.y of a batched launch reads its own state and its own
+// region of the op arena: the state pointer moves by `bstate` amplitudes and
+// arena pointers by `barena` bytes per instance (both 0 for one instance).
+template <typename T>
+__device__ __forceinline__ T *bshift(T *p, uint64_t bytes) {
+  return reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) + bytes);
+}
+
 template <typename T>
 using const_ptr = const __attribute__((address_space(4))) T *;
 

@@ -133,6 +133,8 @@ struct PassArgs {
   const PassOp *ops;
   const DiagTerm *terms;
   uint32_t nops;
+  uint64_t bstate;  // batched launches: per-instance state stride (amplitudes)
+  uint64_t barena;  // and op-arena stride (bytes); 0 for one instance
 };
 
 // A merged pass of chain shape: step i applies an uncontrolled 2x2 to
@@ -237,6 +239,8 @@ struct TileArgs {
   uint32_t nsegs;
   const PassOp *ops;
   const DiagTerm *terms;
+  uint64_t bstate;  // batched launches, as in PassArgs
+  uint64_t barena;
 };
 
 // Generic wide pass: merged-pass ops (NL = 5) over the workgroup tile of a
@@ -253,6 +257,8 @@ struct GWideArgs {
   const DiagTerm *terms;
   uint32_t na;
   uint32_t nops;
+  uint64_t bstate;  // batched launches, as in PassArgs
+  uint64_t barena;
 };
 
 // LDS slot (float4 index) of tile index i, i = (tile bits 1..11) >> 1: the
@@ -308,6 +314,7 @@ constexpr int kAccumTerms = 32;
 struct AccumArgs {
   float2 *lam;
   const float2 *psi;
+  uint64_t bstate;  // batched launches: per-instance state stride
   uint64_t dim;
   uint64_t flip;
   uint32_t nterms;
@@ -326,6 +333,7 @@ constexpr int kAccumTileWires = 3;
 struct AccumTileArgs {
   float2 *lam;
   const float2 *psi;
+  uint64_t bstate;  // batched launches: per-instance state stride
   uint64_t dim;
   uint32_t nw;
   uint8_t w[kAccumTileWires];
@@ -352,12 +360,17 @@ void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,
                         double *out2);
 
 // Merged backward pass of the adjoint sweep: consecutive inverse gates whose
-// wires fit one thread's 16-amplitude tile, applied to psi and lam together.
+// wires fit one thread's tile of 2^nl amplitudes (bit 0 and up to
+// kAdjMaxReg register wires), applied to psi and lam together.
 // Each op applies a 2x2 to both states (slot_asm section `sel`, pair mask
 // `pm`; the identity in threads whose base index lacks a bit of `gmask`);
 // type 1 first accumulates the rotation's overlap <lam|P|psi>, P = X, Y, Z
 // (`axis` 0, 1, 2) on tile-local bit `lbit`, in FP64 into slot `rslot`.
 constexpr int kAdjMaxRot = 8;
+// Register wires per backward pass. Four (a 32-amplitude tile of each state,
+// 175 VGPRs, two waves per SIMD) cut HEA n=32 from 31 to 24 passes but made
+// each pass 29% slower: measured no faster, so three.
+constexpr int kAdjMaxReg = 3;
 constexpr int kAdjBlocks = 4096;
 struct AdjOp {
   uint32_t type;
@@ -378,6 +391,9 @@ struct AdjPassArgs {
   const AdjOp *ops;
   uint32_t nops;
   uint32_t nrot;
+  uint32_t nl;  // tile: bit 0 plus nl - 1 register wires (4)
+  uint64_t bstate;  // batched launches, as in PassArgs
+  uint64_t barena;
 };
 
 // Grid of kAdjBlocks workgroups (four waves per SIMD at the kernel's
@@ -397,5 +413,36 @@ void launch_prefix(hipStream_t s, double *v, uint64_t count, double *chunk);
 void launch_shots(hipStream_t s, const float2 *a, uint64_t dim,
                   const double *cdf, uint64_t nblocks, uint64_t shots,
                   uint64_t seed, uint64_t *out);
+
+// Batched launches: `count` instances side by side (grid row y = instance
+// y), each reading its state at the args' state stride and its ops at the
+// arena stride; reductions and their outputs per instance back to back.
+void launch_pass_batched(hipStream_t s, int nl, const PassArgs &p, uint32_t count);
+void launch_tile_batched(hipStream_t s, const TileArgs &t, uint32_t count);
+void launch_gwide_batched(hipStream_t s, int wb, const GWideArgs &g, uint32_t count);
+void launch_fill_basis_batched(hipStream_t s, float2 *a, uint64_t dim, uint32_t count);
+void launch_zero_batched(hipStream_t s, float2 *a, uint64_t dim, uint32_t count);
+void launch_pauli_accum_tile_batched(hipStream_t s, const AccumTileArgs &a, uint32_t count);
+void launch_pauli_accum_batched(hipStream_t s, const AccumArgs &a, uint32_t count);
+// Workgroups per instance of the batched reductions below (partials hold
+// count * blocks * 2 doubles for launch_pauli_batched, count * kAdjMaxRot *
+// blocks * 2 for launch_adj_pass_batched).
+unsigned pauli_batched_blocks(uint64_t dim);
+unsigned adj_batched_blocks(const Layout &lay);
+// out[2 i], out[2 i + 1] = instance i's sum (as launch_pauli).
+void launch_pauli_batched(hipStream_t s, const float2 *bra, const float2 *ket, uint64_t dim,
+                          uint64_t flip, uint64_t sign, double *partials, double *out,
+                          uint32_t count);
+// Instance i's rotation overlaps to out + i * ostride + 2 r.
+void launch_adj_pass_batched(hipStream_t s, const AdjPassArgs &a, double *partials, double *out,
+                             uint64_t ostride, uint32_t count);
+// Per instance i: energy_out[inst[i]] = eslots[2 i]; for every rotation r of
+// the sweep, grad_out[gbase[i] + (rgate[r] & 0x7fffffff)] = the real part of
+// its overlap (rslots[i * 2 nrot + 2 r]) when bit 31 of rgate[r] is set (RY),
+// else the imaginary part.
+void launch_batched_results(hipStream_t s, const double *eslots, const double *rslots,
+                            uint32_t count, uint32_t nrot, const uint32_t *rgate,
+                            const uint32_t *inst, const uint32_t *gbase, double *energy_out,
+                            double *grad_out);
 
 }  // namespace qmamd

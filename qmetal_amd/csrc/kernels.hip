@@ -290,8 +290,10 @@ template <int NL, bool NT>
 __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
   uint64_t off[1 << (NL - 1)];
   offsets<NL>(p.lay, off);
-  const const_ptr<PassOp> ops = as_const(p.ops);
-  const const_ptr<DiagTerm> terms = as_const(p.terms);
+  const uint64_t inst = blockIdx.y;
+  float2 *const A = p.a + inst * p.bstate;
+  const const_ptr<PassOp> ops = as_const(bshift(p.ops, inst * p.barena));
+  const const_ptr<DiagTerm> terms = as_const(bshift(p.terms, inst * p.barena));
   const TileAddr ta = tile_addr(p.lay);
   const uint64_t stride = uint64_t(gridDim.x) * kT;
   const uint32_t lane = threadIdx.x & 63;
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
       const uint32_t lb = active ? ta.lane_bytes : 0u;
       float4 v[1 << (NL - 1)];
 #pragma unroll
-      for (int r = 0; r < (1 << (NL - 1)); r++) v[r] = ld4<NT>(pair_ptr(p.a, ub | off[r], lb));
+      for (int r = 0; r < (1 << (NL - 1)); r++) v[r] = ld4<NT>(pair_ptr(A, ub | off[r], lb));
 #pragma unroll
       for (int r = 0; r < (1 << (NL - 1)); r++) {
         x[2 * r] = make_float2(v[r].x, v[r].y);
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(kT) void k_pass(PassArgs p) {
     if (active) {
 #pragma unroll
       for (int r = 0; r < (1 << (NL - 1)); r++)
-        st4<NT>(pair_ptr(p.a, ub | off[r], ta.lane_bytes),
+        st4<NT>(pair_ptr(A, ub | off[r], ta.lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
     }
   }
@@ -538,8 +540,10 @@ __global__ __launch_bounds__(64 << WB) void k_gwide(GWideArgs p) {
   constexpr int ROUNDS = 16 / W;
   __shared__ float4 xch[W][W][64];  // [writer's wave][slot][lane]
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const const_ptr<PassOp> ops = as_const(p.ops);
-  const const_ptr<DiagTerm> terms = as_const(p.terms);
+  const uint64_t inst = blockIdx.y;
+  float2 *const A = p.a + inst * p.bstate;
+  const const_ptr<PassOp> ops = as_const(bshift(p.ops, inst * p.barena));
+  const const_ptr<DiagTerm> terms = as_const(bshift(p.terms, inst * p.barena));
   uint64_t lanebits = 0;
 #pragma unroll
   for (int j = 0; j < 6; j++)
@@ -561,7 +565,7 @@ __global__ __launch_bounds__(64 << WB) void k_gwide(GWideArgs p) {
 #pragma unroll
       for (int k = 0; k < 4; k++)
         if ((r >> k) & 1) o |= 1ull << p.pos[k];
-      const float4 v = ld4<true>(pair_ptr(p.a, o, lane_bytes));
+      const float4 v = ld4<true>(pair_ptr(A, o, lane_bytes));
       x[2 * r] = make_float2(v.x, v.y);
       x[2 * r + 1] = make_float2(v.z, v.w);
     }
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(64 << WB) void k_gwide(GWideArgs p) {
 #pragma unroll
       for (int k = 0; k < 4; k++)
         if ((r >> k) & 1) o |= 1ull << (k < WB ? p.pos[4 + k] : p.pos[k]);
-      st4<true>(pair_ptr(p.a, o, lane_bytes),
+      st4<true>(pair_ptr(A, o, lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
     }
   }
@@ -626,9 +630,11 @@ __device__ __forceinline__ void tile_permute(float4 *lds, float2 (&x)[32], uint3
 __global__ __launch_bounds__(128) void k_tile(TileArgs p) {
   __shared__ float4 lds[2048];
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const const_ptr<TileSeg> segs = as_const(p.segs);
-  const const_ptr<PassOp> ops = as_const(p.ops);
-  const const_ptr<DiagTerm> terms = as_const(p.terms);
+  const uint64_t inst = blockIdx.y;
+  float2 *const A = p.a + inst * p.bstate;
+  const const_ptr<TileSeg> segs = as_const(bshift(p.segs, inst * p.barena));
+  const const_ptr<PassOp> ops = as_const(bshift(p.ops, inst * p.barena));
+  const const_ptr<DiagTerm> terms = as_const(bshift(p.terms, inst * p.barena));
   const uint32_t lane_bytes = lane * 16u;
   const uint64_t wave_off = uint64_t(wave) << 7;
   // Arrangement 0 (loads and stores): thread bit j on tile bit j + 1,
@@ -646,7 +652,7 @@ __global__ __launch_bounds__(128) void k_tile(TileArgs p) {
     float2 x[32];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-      const float4 v = ld4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << 8), lane_bytes));
+      const float4 v = ld4<true>(pair_ptr(A, ub | wave_off | (uint64_t(r) << 8), lane_bytes));
       x[2 * r] = make_float2(v.x, v.y);
       x[2 * r + 1] = make_float2(v.z, v.w);
     }
@@ -675,7 +681,7 @@ __global__ __launch_bounds__(128) void k_tile(TileArgs p) {
     tile_permute(lds, x, tf, rf, t0, r0);
 #pragma unroll
     for (int r = 0; r < 16; r++)
-      st4<true>(pair_ptr(p.a, ub | wave_off | (uint64_t(r) << 8), lane_bytes),
+      st4<true>(pair_ptr(A, ub | wave_off | (uint64_t(r) << 8), lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
   }
 }
@@ -703,14 +709,14 @@ __global__ __launch_bounds__(kT) void k_blocked(BlockedArgs b) {
 
 __global__ __launch_bounds__(kT) void k_fill_basis(float2 *a, uint64_t dim) {
   const uint64_t stride = uint64_t(gridDim.x) * kT;
-  float4 *v = reinterpret_cast<float4 *>(a);
+  float4 *v = reinterpret_cast<float4 *>(a + uint64_t(blockIdx.y) * dim);  // instance blockIdx.y
   for (uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x; i < dim / 2; i += stride)
     v[i] = make_float4(i == 0 ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(kT) void k_zero(float2 *a, uint64_t dim) {
   const uint64_t stride = uint64_t(gridDim.x) * kT;
-  float4 *v = reinterpret_cast<float4 *>(a);
+  float4 *v = reinterpret_cast<float4 *>(a + uint64_t(blockIdx.y) * dim);  // instance blockIdx.y
   for (uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x; i < dim / 2; i += stride)
     v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -773,6 +779,11 @@ __device__ __forceinline__ void acc_cj(double &re, double &im, float2 x, float2 
 __global__ __launch_bounds__(kT) void k_pauli(const float2 *bra, const float2 *ket, uint64_t dim,
                                               uint64_t flip, uint64_t sign, double *part) {
   __shared__ double sh[kT / 64];
+  // Batched launches: instance blockIdx.y, states of `dim` amplitudes back to
+  // back, partials per instance.
+  bra += uint64_t(blockIdx.y) * dim;
+  ket += uint64_t(blockIdx.y) * dim;
+  part += size_t(blockIdx.y) * gridDim.x * 2;
   const uint64_t stride = uint64_t(gridDim.x) * kT;
   double re = 0.0, im = 0.0;
   if (flip == 0) {
@@ -828,10 +839,14 @@ __global__ __launch_bounds__(kT) void k_pauli(const float2 *bra, const float2 *k
   }
 }
 
-// Reduce `count` partials with component stride `comps` into out[0..comps).
+// Reduce `count` partials with component stride `comps` into out[0..comps)
+// (block b: instance b of a batched reduction, partials and outputs back to
+// back).
 __global__ __launch_bounds__(kT) void k_finish(const double *part, int count, int comps,
                                                double *out) {
   __shared__ double sh[kT / 64];
+  part += size_t(blockIdx.x) * count * comps;
+  out += size_t(blockIdx.x) * comps;
   for (int c = 0; c < comps; c++) {
     double acc = 0.0;
     for (int k = threadIdx.x; k < count; k += kT) acc += part[comps * k + c];
@@ -844,10 +859,12 @@ __global__ __launch_bounds__(kT) void k_pauli_accum(AccumArgs p) {
   const uint64_t stride = uint64_t(gridDim.x) * kT;
   const uint64_t fe = p.flip & ~1ull;
   const bool odd = p.flip & 1ull;
+  const float2 *psi = p.psi + uint64_t(blockIdx.y) * p.bstate;
+  float2 *lam = p.lam + uint64_t(blockIdx.y) * p.bstate;
   for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < p.dim / 2; u += stride) {
     const uint64_t i = 2 * u, src = i ^ fe;
-    const float4 s = *reinterpret_cast<const float4 *>(p.psi + src);
-    float4 *l = reinterpret_cast<float4 *>(p.lam + i);
+    const float4 s = *reinterpret_cast<const float4 *>(psi + src);
+    float4 *l = reinterpret_cast<float4 *>(lam + i);
     float4 lv = *l;
     const float2 y0 = odd ? make_float2(s.z, s.w) : make_float2(s.x, s.y);
     const float2 y1 = odd ? make_float2(s.x, s.y) : make_float2(s.z, s.w);
@@ -878,6 +895,8 @@ __global__ __launch_bounds__(kT) void k_pauli_accum_tile(AccumTileArgs p) {
   constexpr int E = 2 << NW;  // elements per thread
   const uint64_t units = p.dim >> (NW + 1);
   const uint64_t stride = uint64_t(gridDim.x) * kT;
+  const float2 *psi = p.psi + uint64_t(blockIdx.y) * p.bstate;
+  float2 *lam = p.lam + uint64_t(blockIdx.y) * p.bstate;
   uint64_t wbit[NW > 0 ? NW : 1];
 #pragma unroll
   for (int k = 0; k < NW; k++) wbit[k] = 1ull << p.w[k];
@@ -897,8 +916,8 @@ __global__ __launch_bounds__(kT) void k_pauli_accum_tile(AccumTileArgs p) {
       for (int k = 0; k < NW; k++)
         if ((r >> k) & 1) o |= wbit[k];
       idx[r] = o;
-      const float4 v = *reinterpret_cast<const float4 *>(p.psi + o);
-      const float4 w = *reinterpret_cast<const float4 *>(p.lam + o);
+      const float4 v = *reinterpret_cast<const float4 *>(psi + o);
+      const float4 w = *reinterpret_cast<const float4 *>(lam + o);
       y[2 * r] = make_float2(v.x, v.y);
       y[2 * r + 1] = make_float2(v.z, v.w);
       l[2 * r] = make_float2(w.x, w.y);
@@ -927,7 +946,7 @@ __global__ __launch_bounds__(kT) void k_pauli_accum_tile(AccumTileArgs p) {
     }
 #pragma unroll
     for (int r = 0; r < E / 2; r++)
-      *reinterpret_cast<float4 *>(p.lam + idx[r]) =
+      *reinterpret_cast<float4 *>(lam + idx[r]) =
           make_float4(l[2 * r].x, l[2 * r].y, l[2 * r + 1].x, l[2 * r + 1].y);
   }
 }
@@ -952,11 +971,11 @@ __device__ __forceinline__ void adj_pair(const AdjArgs &a, float2 &p0, float2 &p
 }
 
 // <lam|P|psi> over the tile's pairs on local bit LB (P on that bit).
-template <int LB>
-__device__ __forceinline__ void tile_overlap(const float2 (&x)[16], const float2 (&y)[16],
+template <int LB, int N>
+__device__ __forceinline__ void tile_overlap(const float2 (&x)[N], const float2 (&y)[N],
                                              uint32_t axis, double &re, double &im) {
 #pragma unroll
-  for (int l = 0; l < 16; l++) {
+  for (int l = 0; l < N; l++) {
     if (l & (1 << LB)) continue;
     const int h = l | (1 << LB);
     if (axis == 0) {         // X: conj(l0) p1 + conj(l1) p0
@@ -977,14 +996,19 @@ __device__ __forceinline__ void tile_overlap(const float2 (&x)[16], const float2
 // compiler hold second copies); the overlap only reads them. Wave sums of
 // each overlap go to per-wave LDS slots in program order, then to per-block
 // partials in a fixed order: results are run-to-run deterministic.
-template <bool NT>
+template <int NL, bool NT>
 __global__ __launch_bounds__(kT) void k_adj_pass(AdjPassArgs a, double *part) {
+  constexpr int N = 1 << NL;
   __shared__ double acc[kAdjMaxRot][kT / 64][2];
   for (int i = threadIdx.x; i < kAdjMaxRot * (kT / 64) * 2; i += kT) (&acc[0][0][0])[i] = 0.0;
   __syncthreads();
-  uint64_t off[8];
-  offsets<4>(a.lay, off);
-  const const_ptr<AdjOp> ops = as_const(a.ops);
+  uint64_t off[N / 2];
+  offsets<NL>(a.lay, off);
+  const uint64_t inst = blockIdx.y;
+  float2 *const psi = a.psi + inst * a.bstate;
+  float2 *const lam = a.lam + inst * a.bstate;
+  const const_ptr<AdjOp> ops = as_const(bshift(a.ops, inst * a.barena));
+  part += size_t(inst) * kAdjMaxRot * gridDim.x * 2;
   const TileAddr ta = tile_addr(a.lay);
   const uint64_t stride = uint64_t(gridDim.x) * kT;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -995,11 +1019,11 @@ __global__ __launch_bounds__(kT) void k_adj_pass(AdjPassArgs a, double *part) {
     // Lanes past the end read the workgroup's first tile, add nothing to
     // the overlaps and never store.
     const uint32_t lb = active ? ta.lane_bytes : 0u;
-    float2 x[16], y[16];
+    float2 x[N], y[N];
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-      const float4 v = ld4<NT>(pair_ptr(a.psi, ub | off[r], lb));
-      const float4 w = ld4<NT>(pair_ptr(a.lam, ub | off[r], lb));
+    for (int r = 0; r < N / 2; r++) {
+      const float4 v = ld4<NT>(pair_ptr(psi, ub | off[r], lb));
+      const float4 w = ld4<NT>(pair_ptr(lam, ub | off[r], lb));
       x[2 * r] = make_float2(v.x, v.y);
       x[2 * r + 1] = make_float2(v.z, v.w);
       y[2 * r] = make_float2(w.x, w.y);
@@ -1013,7 +1037,8 @@ __global__ __launch_bounds__(kT) void k_adj_pass(AdjPassArgs a, double *part) {
           case 0: tile_overlap<0>(x, y, axis, re, im); break;
           case 1: tile_overlap<1>(x, y, axis, re, im); break;
           case 2: tile_overlap<2>(x, y, axis, re, im); break;
-          default: tile_overlap<3>(x, y, axis, re, im); break;
+          case 3: tile_overlap<3>(x, y, axis, re, im); break;
+          default: tile_overlap<NL - 1>(x, y, axis, re, im); break;
         }
         re = wave_allsum(active ? re : 0.0);
         im = wave_allsum(active ? im : 0.0);
@@ -1033,15 +1058,15 @@ __global__ __launch_bounds__(kT) void k_adj_pass(AdjPassArgs a, double *part) {
       }
       const uint32_t sel = __builtin_amdgcn_readfirstlane(ops[o].sel);
       const uint32_t pm = __builtin_amdgcn_readfirstlane(ops[o].pm);
-      slot_asm<4>(x, sel, pm, m);
-      slot_asm<4>(y, sel, pm, m);
+      slot_asm<NL>(x, sel, pm, m);
+      slot_asm<NL>(y, sel, pm, m);
     }
     if (active) {
 #pragma unroll
-      for (int r = 0; r < 8; r++) {
-        st4<NT>(pair_ptr(a.psi, ub | off[r], ta.lane_bytes),
+      for (int r = 0; r < N / 2; r++) {
+        st4<NT>(pair_ptr(psi, ub | off[r], ta.lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
-        st4<NT>(pair_ptr(a.lam, ub | off[r], ta.lane_bytes),
+        st4<NT>(pair_ptr(lam, ub | off[r], ta.lane_bytes),
                 make_float4(y[2 * r].x, y[2 * r].y, y[2 * r + 1].x, y[2 * r + 1].y));
       }
     }
@@ -1055,10 +1080,15 @@ __global__ __launch_bounds__(kT) void k_adj_pass(AdjPassArgs a, double *part) {
   }
 }
 
-// Block r sums rotation r's per-block partials in a fixed order.
-__global__ __launch_bounds__(kT) void k_finish_rot(const double *part, int count, double *out) {
+// Block r sums rotation r's per-block partials in a fixed order (block row y:
+// instance y of a batched pass, its partials kAdjMaxRot * count * 2 doubles
+// apart and its outputs `ostride` doubles apart).
+__global__ __launch_bounds__(kT) void k_finish_rot(const double *part, int count, double *out,
+                                                   uint64_t ostride) {
   __shared__ double sh[kT / 64];
   const size_t r = blockIdx.x;
+  part += size_t(blockIdx.y) * kAdjMaxRot * count * 2;
+  out += size_t(blockIdx.y) * ostride;
   for (int c = 0; c < 2; c++) {
     double acc = 0.0;
     for (int k = threadIdx.x; k < count; k += kT) acc += part[(r * count + k) * 2 + c];
@@ -1245,6 +1275,27 @@ __global__ __launch_bounds__(kT) void k_shots(const float2 *a, uint64_t dim, con
   }
 }
 
+// Batched adjoint results: one thread per (instance, rotation), plus the
+// energy.
+__global__ __launch_bounds__(kT) void k_batched_results(const double *eslots, const double *rslots,
+                                                        uint32_t count, uint32_t nrot,
+                                                        const uint32_t *rgate, const uint32_t *inst,
+                                                        const uint32_t *gbase, double *energy_out,
+                                                        double *grad_out) {
+  const uint64_t per = uint64_t(nrot) + 1;
+  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  for (uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x; t < per * count; t += stride) {
+    const uint32_t i = uint32_t(t / per), r = uint32_t(t % per);
+    if (r == nrot) {
+      energy_out[inst[i]] = eslots[2 * size_t(i)];
+      continue;
+    }
+    const uint32_t g = rgate[r];
+    const double *o = rslots + (size_t(i) * nrot + r) * 2;
+    grad_out[size_t(gbase[i]) + (g & 0x7fffffffu)] = (g >> 31) ? o[0] : o[1];
+  }
+}
+
 }  // namespace
 
 // --- launchers --------------------------------------------------------------
@@ -1423,10 +1474,15 @@ void launch_adjoint_rot(hipStream_t s, const AdjArgs &a, double *partials,
   k_finish<<<1, kT, 0, s>>>(partials, kRedBlocks, 2, out2);
 }
 
+template <int NL>
+void adj_pass_nl(dim3 grid, hipStream_t s, const AdjPassArgs &a, double *partials) {
+  if (a.lay.streaming) k_adj_pass<NL, true><<<grid, kT, 0, s>>>(a, partials);
+  else k_adj_pass<NL, false><<<grid, kT, 0, s>>>(a, partials);
+}
+
 void launch_adj_pass(hipStream_t s, const AdjPassArgs &a, double *partials, double *out) {
-  if (a.lay.streaming) k_adj_pass<true><<<kAdjBlocks, kT, 0, s>>>(a, partials);
-  else k_adj_pass<false><<<kAdjBlocks, kT, 0, s>>>(a, partials);
-  if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, kAdjBlocks, out);
+  adj_pass_nl<4>(dim3(kAdjBlocks), s, a, partials);
+  if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, kAdjBlocks, out, 0);
 }
 
 void launch_block_sums(hipStream_t s, const float2 *a, uint64_t dim, double *sums) {
@@ -1447,6 +1503,82 @@ void launch_shots(hipStream_t s, const float2 *a, uint64_t dim,
                   const double *cdf, uint64_t nblocks, uint64_t shots,
                   uint64_t seed, uint64_t *out) {
   k_shots<<<grid_for(shots), kT, 0, s>>>(a, dim, cdf, nblocks, shots, seed, out);
+}
+
+// --- batched launches ------------------------------------------------------
+
+void launch_pass_batched(hipStream_t s, int nl, const PassArgs &p, uint32_t count) {
+  const dim3 grid(grid_for(p.lay.nthreads, kPassGrid), count);
+  const bool nt = p.lay.streaming != 0;
+  if (nl == 4) {
+    if (nt) k_pass<4, true><<<grid, kT, 0, s>>>(p);
+    else k_pass<4, false><<<grid, kT, 0, s>>>(p);
+  } else if (nl == 5) {
+    if (nt) k_pass<5, true><<<grid, kT, 0, s>>>(p);
+    else k_pass<5, false><<<grid, kT, 0, s>>>(p);
+  }
+}
+
+void launch_tile_batched(hipStream_t s, const TileArgs &t, uint32_t count) {
+  const dim3 grid(unsigned(std::min<uint64_t>(t.lay.nthreads, kLowGrid)), count);
+  k_tile<<<grid, 128, 0, s>>>(t);
+}
+
+void launch_gwide_batched(hipStream_t s, int wb, const GWideArgs &g, uint32_t count) {
+  const dim3 grid(wide_grid(g.lay.nthreads), count);
+  if (wb == 3) k_gwide<3><<<grid, 512, 0, s>>>(g);
+  else k_gwide<2><<<grid, 256, 0, s>>>(g);
+}
+
+void launch_fill_basis_batched(hipStream_t s, float2 *a, uint64_t dim, uint32_t count) {
+  k_fill_basis<<<dim3(grid_for(dim / 2), count), kT, 0, s>>>(a, dim);
+}
+
+void launch_zero_batched(hipStream_t s, float2 *a, uint64_t dim, uint32_t count) {
+  k_zero<<<dim3(grid_for(dim / 2), count), kT, 0, s>>>(a, dim);
+}
+
+void launch_pauli_accum_tile_batched(hipStream_t s, const AccumTileArgs &a, uint32_t count) {
+  const dim3 grid(grid_for(a.dim >> (a.nw + 1)), count);
+  switch (a.nw) {
+    case 0: k_pauli_accum_tile<0><<<grid, kT, 0, s>>>(a); break;
+    case 1: k_pauli_accum_tile<1><<<grid, kT, 0, s>>>(a); break;
+    case 2: k_pauli_accum_tile<2><<<grid, kT, 0, s>>>(a); break;
+    case 3: k_pauli_accum_tile<3><<<grid, kT, 0, s>>>(a); break;
+    default: break;
+  }
+}
+
+void launch_pauli_accum_batched(hipStream_t s, const AccumArgs &a, uint32_t count) {
+  k_pauli_accum<<<dim3(grid_for(a.dim / 2), count), kT, 0, s>>>(a);
+}
+
+unsigned pauli_batched_blocks(uint64_t dim) { return grid_for(dim / 2, kRedBlocks); }
+
+unsigned adj_batched_blocks(const Layout &lay) { return grid_for(lay.nthreads, kAdjBlocks); }
+
+void launch_pauli_batched(hipStream_t s, const float2 *bra, const float2 *ket, uint64_t dim,
+                          uint64_t flip, uint64_t sign, double *partials, double *out,
+                          uint32_t count) {
+  const unsigned blocks = pauli_batched_blocks(dim);
+  k_pauli<<<dim3(blocks, count), kT, 0, s>>>(bra, ket, dim, flip, sign, partials);
+  k_finish<<<count, kT, 0, s>>>(partials, int(blocks), 2, out);
+}
+
+void launch_adj_pass_batched(hipStream_t s, const AdjPassArgs &a, double *partials, double *out,
+                             uint64_t ostride, uint32_t count) {
+  const unsigned blocks = adj_batched_blocks(a.lay);
+  const dim3 grid(blocks, count);
+  adj_pass_nl<4>(grid, s, a, partials);
+  if (a.nrot) k_finish_rot<<<dim3(a.nrot, count), kT, 0, s>>>(partials, int(blocks), out, ostride);
+}
+
+void launch_batched_results(hipStream_t s, const double *eslots, const double *rslots,
+                            uint32_t count, uint32_t nrot, const uint32_t *rgate,
+                            const uint32_t *inst, const uint32_t *gbase, double *energy_out,
+                            double *grad_out) {
+  k_batched_results<<<grid_for(uint64_t(count) * (nrot + 1)), kT, 0, s>>>(
+      eslots, rslots, count, nrot, rgate, inst, gbase, energy_out, grad_out);
 }
 
 }  // namespace qmamd

@@ -33,6 +33,32 @@ struct OpTime {
   double bytes;
 };
 
+// Adjoint energy and gradient of many circuits that share one width, one
+// gate structure (kinds and wires in order; angles free) and one
+// Hamiltonian, compiled into batched launches: every kernel of the merged
+// single-circuit path is one launch over all instances (one grid row each),
+// each instance reading its own state and its own region of a shared op
+// arena. Built by Simulator::compile_batched; owns its device buffers.
+class BatchedAdjoint {
+ public:
+  BatchedAdjoint();
+  ~BatchedAdjoint();
+  BatchedAdjoint(const BatchedAdjoint &) = delete;
+  BatchedAdjoint &operator=(const BatchedAdjoint &) = delete;
+  uint32_t count() const;
+  uint32_t num_qubits() const;
+  // Enqueues the whole program on `stream` (a hipStream_t): instance i's
+  // energy to energy_out[inst[i]] and its rotations' gradients to
+  // grad_out[gbase[i] + gate index] (inst, gbase as compiled). Nothing
+  // synchronises or allocates, so a graph can capture it.
+  void enqueue(void *stream, double *energy_out, double *grad_out) const;
+
+ private:
+  friend class Simulator;
+  struct Data;
+  std::unique_ptr<Data> d_;
+};
+
 class Simulator {
  public:
   explicit Simulator(uint32_t n);
@@ -84,6 +110,18 @@ class Simulator {
                                   const double *params, size_t nparams,
                                   double *out_energy, double *out_grad,
                                   void *caller_stream);
+
+  // Compiles `circuits` (this simulator's width, identical gate kinds and
+  // wires, each differentiable as for energy_and_gradient) under `h` into a
+  // batched program (BatchedAdjoint); instance i reports to inst[i] and
+  // gbase[i]. Returns null when the circuits' plans differ in structure or
+  // need a pass kind the batched kernels do not run (wide or low chain
+  // passes, an op that cannot join a merged pass). Allocates the instances'
+  // two states each.
+  std::unique_ptr<BatchedAdjoint> compile_batched(const std::vector<const Circuit *> &circuits,
+                                                  const Hamiltonian &h,
+                                                  const std::vector<uint32_t> &inst,
+                                                  const std::vector<uint32_t> &gbase);
 
   // Per-op timing with events on the simulator's stream (off by default).
   void set_timing(bool on);

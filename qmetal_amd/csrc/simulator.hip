@@ -156,6 +156,26 @@ uint32_t slot_kind(const float *m8, bool real, bool local_ctrl, bool lane_ctrl) 
   return local_ctrl ? 2u : (real ? 0u : 1u);
 }
 
+// U^dagger (reference simulator.mm:786-799); U3(t,p,l)^dagger = U3(-t,-l,-p).
+Gate inverse(const Gate &g) {
+  Gate v = g;
+  switch (g.kind) {
+    case K_RX: case K_RY: case K_RZ: case K_P: case K_CP: v.p[0] = -g.p[0]; break;
+    case K_S: v.kind = K_SDG; break;
+    case K_SDG: v.kind = K_S; break;
+    case K_T: v.kind = K_TDG; break;
+    case K_TDG: v.kind = K_T; break;
+    case K_U3: v.p[0] = -g.p[0]; v.p[1] = -g.p[2]; v.p[2] = -g.p[1]; break;
+    default: break;
+  }
+  return v;
+}
+
+bool is_rotation(uint8_t k) { return k == K_RX || k == K_RY || k == K_RZ; }
+// A gate gets a gradient entry when it carries a parameter and has a native
+// generator (reference simulator.mm:822-828, :917): RX/RY/RZ added with no
+// parameter report 0, and a P/CP/U3 without one is not differentiated.
+bool differentiable(const Gate &g) { return g.np > 0 && is_rotation(g.kind); }
 }  // namespace
 
 struct Simulator::Impl {
@@ -630,6 +650,9 @@ struct Simulator::Impl {
   // is rewritten in place as ChainSteps for the straight-line kernel.
   void to_chain(const PassOp *po, const DiagTerm *dt, const std::vector<PhaseCoef> &coef,
                 size_t nops, size_t nt, size_t off, PassLaunch &pl) {
+    // Batched programs keep the generic kernel: the chain kernel's real /
+    // complex choice depends on each instance's matrices.
+    if (batch_compile) return;
     struct Step {
       int slot, phase;
     };
@@ -1163,6 +1186,294 @@ struct Simulator::Impl {
   }
 
   bool merge = true;
+  // Set while compiling a batched program (compile_batched).
+  bool batch_compile = false;
+
+  // --- adjoint sweep building blocks (energy_and_gradient, batched programs)
+
+  // One merged backward pass: inverse gates (indices into the circuit, in
+  // execution order) whose targets fit wire 0 plus `reg` (<= kAdjMaxReg wires), with
+  // `nrot` rotations, the first at sweep index rot0.
+  struct APass {
+    std::vector<size_t> gates;
+    std::vector<uint32_t> reg;
+    uint32_t nrot = 0;
+    size_t rot0 = 0;
+  };
+
+  // Backward sweep in merged passes: inverse gates whose target wires fit one
+  // thread's tile (wire 0 plus kAdjMaxReg register wires), at most kAdjMaxRot
+  // rotations each, applied to both states in one read and one write of
+  // each; overlaps reduced per pass. Gates are taken from a dependency graph
+  // of the reversed circuit: two gates keep their reverse order when their
+  // wires overlap, unless both are diagonal. A gate moved past others
+  // commutes with them, so each rotation's overlap <l|P|psi> is the one the
+  // gate-by-gate sweep would see (U'P U'^dagger = P). rot_index gets each
+  // rotation's sweep index.
+  static std::vector<APass> adjoint_schedule(const Circuit &c, std::vector<size_t> &rot_index) {
+    const size_t G = c.gates.size();
+    auto targets = [](const Gate &g) {
+      std::vector<uint32_t> t;
+      if (g.nq == 1) t.push_back(g.q[0]);
+      else if (g.kind == K_SWAP) t = {g.q[0], g.q[1]};
+      else t.push_back(g.q[g.nq - 1]);
+      return t;
+    };
+    // Units in sweep order (gate G-1 first) and their order constraints.
+    std::vector<std::vector<uint32_t>> succ(G);
+    std::vector<uint32_t> ndeps(G, 0);
+    {
+      std::vector<int64_t> last(64, -1);
+      std::vector<std::vector<uint32_t>> diags(64);
+      std::vector<uint32_t> deps;
+      for (size_t u = 0; u < G; u++) {
+        const Gate &g = c.gates[G - 1 - u];
+        const bool dg = is_diagonal(g);
+        deps.clear();
+        for (uint32_t j = 0; j < g.nq; j++) {
+          const uint32_t q = g.q[j];
+          if (last[q] >= 0) deps.push_back(uint32_t(last[q]));
+          if (!dg) deps.insert(deps.end(), diags[q].begin(), diags[q].end());
+        }
+        std::sort(deps.begin(), deps.end());
+        deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
+        for (uint32_t d : deps) succ[d].push_back(uint32_t(u));
+        ndeps[u] = uint32_t(deps.size());
+        for (uint32_t j = 0; j < g.nq; j++) {
+          const uint32_t q = g.q[j];
+          if (dg) {
+            diags[q].push_back(uint32_t(u));
+          } else {
+            last[q] = int64_t(u);
+            diags[q].clear();
+          }
+        }
+      }
+    }
+    std::vector<uint32_t> ready;
+    for (uint32_t u = 0; u < G; u++)
+      if (!ndeps[u]) ready.push_back(u);
+    std::vector<APass> passes;
+    rot_index.assign(G, 0);
+    size_t sweep_rot = 0, done = 0;
+    while (done < G) {
+      APass cur;
+      cur.rot0 = sweep_rot;
+      for (;;) {
+        // The ready gate adding the fewest register wires, then the earliest.
+        int best = -1;
+        size_t best_j = 0, best_add = 0;
+        for (size_t j = 0; j < ready.size(); j++) {
+          const Gate &g = c.gates[G - 1 - ready[j]];
+          size_t add = 0;
+          for (uint32_t q : targets(g))
+            if (q != 0 && std::find(cur.reg.begin(), cur.reg.end(), q) == cur.reg.end()) add++;
+          const uint32_t r = is_rotation(g.kind) ? 1u : 0u;
+          if (cur.reg.size() + add > size_t(kAdjMaxReg) || cur.nrot + r > uint32_t(kAdjMaxRot))
+            continue;
+          if (best < 0 || add < best_add || (add == best_add && ready[j] < uint32_t(best))) {
+            best = int(ready[j]);
+            best_j = j;
+            best_add = add;
+          }
+        }
+        if (best < 0) break;
+        const size_t k = G - 1 - size_t(best);
+        const Gate &g = c.gates[k];
+        for (uint32_t q : targets(g))
+          if (q != 0 && std::find(cur.reg.begin(), cur.reg.end(), q) == cur.reg.end())
+            cur.reg.push_back(q);
+        if (is_rotation(g.kind)) {
+          rot_index[k] = sweep_rot++;
+          cur.nrot++;
+        }
+        cur.gates.push_back(k);
+        ready.erase(ready.begin() + long(best_j));
+        for (uint32_t t : succ[size_t(best)])
+          if (--ndeps[t] == 0) ready.push_back(t);
+        done++;
+      }
+      if (cur.gates.empty()) fail("internal: adjoint schedule");
+      passes.push_back(std::move(cur));
+    }
+    return passes;
+  }
+
+  // Staging bytes of a backward pass's AdjOps (256-aligned).
+  static size_t adj_pass_bytes(const Circuit &c, const APass &ps) {
+    size_t nops = 0;
+    for (size_t k : ps.gates) nops += c.gates[k].kind == K_SWAP ? 3 : 1;
+    return (nops * sizeof(AdjOp) + 255) & ~size_t(255);
+  }
+
+  // AdjOps of backward pass `ps` into `ops`; sets a.lay, a.nops, a.nrot.
+  void emit_adj_pass(const Circuit &c, const APass &ps, AdjOp *ops, AdjPassArgs &a) const {
+    LayoutBuilder lb{n};
+    for (uint32_t q : ps.reg) lb.add(q);
+    int nl = 0;
+    a.lay = lb.finish(kAdjMaxReg + 1, &nl);
+    if (nl != kAdjMaxReg + 1) fail("internal: adjoint pass width");
+    a.nl = uint32_t(nl);
+    uint32_t no = 0, rs = 0;
+    auto slot = [&](uint32_t target, const float *m8, std::vector<uint32_t> ctrls) -> AdjOp & {
+      AdjOp &o = ops[no++];
+      o = AdjOp{};
+      const int lt = lb.local_of(target);
+      if (lt < 0) fail("internal: adjoint target not local");
+      o.lbit = uint32_t(lt);
+      uint32_t cm = 0;
+      for (uint32_t q : ctrls) {
+        const int l = lb.local_of(q);
+        if (l >= 0) cm |= 1u << l;
+        else o.gmask |= 1ull << q;
+      }
+      std::memcpy(o.m, m8, sizeof(o.m));
+      const bool real = m8[1] == 0.0f && m8[3] == 0.0f && m8[5] == 0.0f && m8[7] == 0.0f;
+      o.sel = o.lbit + 8u * slot_kind(m8, real, cm != 0, o.gmask != 0);
+      uint32_t pidx = 0;
+      for (uint32_t l = 0; l < (1u << nl); l++) {
+        if ((l >> o.lbit) & 1) continue;
+        if ((l & cm) == cm) o.pm |= 1u << pidx;
+        pidx++;
+      }
+      return o;
+    };
+    for (size_t k : ps.gates) {
+      const Gate &g = c.gates[k];
+      const Gate inv = inverse(g);
+      float m8[8];
+      if (g.nq == 1) {
+        cd m[4];
+        gate_matrix(inv.kind, inv.p, m);
+        narrow(m, m8);
+        AdjOp &o = slot(g.q[0], m8, {});
+        if (is_rotation(g.kind)) {
+          o.type = 1;
+          o.axis = g.kind == K_RX ? 0 : g.kind == K_RY ? 1 : 2;
+          o.rslot = rs++;
+        }
+        continue;
+      }
+      switch (g.kind) {
+        case K_CX:
+        case K_CY: {
+          cd m[4];
+          gate_matrix(g.kind == K_CX ? K_X : K_Y, nullptr, m);
+          narrow(m, m8);
+          slot(g.q[1], m8, {g.q[0]});
+          break;
+        }
+        case K_CZ:
+        case K_CP: {
+          const cd ph = inv.kind == K_CZ ? cd(-1, 0) : cd(std::cos(inv.p[0]), std::sin(inv.p[0]));
+          const cd m[4] = {1.0, 0.0, 0.0, ph};
+          narrow(m, m8);
+          slot(g.q[1], m8, {g.q[0]});
+          break;
+        }
+        case K_SWAP:
+          slot(g.q[1], kX, {g.q[0]});
+          slot(g.q[0], kX, {g.q[1]});
+          slot(g.q[1], kX, {g.q[0]});
+          break;
+        case K_CCX:
+          slot(g.q[2], kX, {g.q[0], g.q[1]});
+          break;
+        default:
+          fail(std::string("unsupported gate: ") + kind_name(g.kind));
+      }
+    }
+    a.nops = no;
+    a.nrot = ps.nrot;
+  }
+
+  // Launches that build lambda = H psi (after zeroing it): terms grouped by
+  // flip mask; flips of at most one bit share register-tile passes (bit 0
+  // and the no-flip terms ride with the first tile, the other flipped wires
+  // three to a tile), flips of two or more bits take one pass each. State
+  // pointers and sizes are left for the caller.
+  struct AccumLaunch {
+    bool tile = false;
+    AccumTileArgs t{};
+    AccumArgs g{};
+  };
+  static std::vector<AccumLaunch> costate_launches(const Hamiltonian &h) {
+    std::vector<AccumLaunch> out;
+    std::map<uint64_t, std::vector<size_t>> by_flip;
+    for (size_t k = 0; k < h.size(); k++) by_flip[h[k].p.x | h[k].p.y].push_back(k);
+    auto coef = [&](const Term &t, double &re, double &im) {  // c * i^ny
+      const uint32_t ny = uint32_t(__builtin_popcountll(t.p.y));
+      re = t.c;
+      im = 0.0;
+      switch (ny & 3u) {
+        case 1: re = 0.0; im = t.c; break;
+        case 2: re = -t.c; break;
+        case 3: re = 0.0; im = -t.c; break;
+        default: break;
+      }
+    };
+    std::vector<uint32_t> wires1;
+    for (const auto &kv : by_flip)
+      if (__builtin_popcountll(kv.first) == 1 && kv.first != 1)
+        wires1.push_back(uint32_t(__builtin_ctzll(kv.first)));
+    const bool has_tile = by_flip.count(0) || by_flip.count(1) || !wires1.empty();
+    const size_t ntiles =
+        has_tile ? std::max<size_t>(1, (wires1.size() + kAccumTileWires - 1) / kAccumTileWires) : 0;
+    for (size_t g = 0; g < ntiles; g++) {
+      const size_t w0 = g * kAccumTileWires;
+      const uint32_t nw =
+          uint32_t(std::min<size_t>(kAccumTileWires, wires1.size() - std::min(w0, wires1.size())));
+      // Terms of this tile by slot: 0 flip bit 0, 1 + k wire k, nw + 1 none.
+      std::vector<std::vector<size_t>> slot(nw + 2);
+      if (g == 0) {
+        if (by_flip.count(1)) slot[0] = by_flip[1];
+        if (by_flip.count(0)) slot[nw + 1] = by_flip[0];
+      }
+      for (uint32_t k = 0; k < nw; k++) slot[1 + k] = by_flip[1ull << wires1[w0 + k]];
+      // Launches of at most kAccumTerms terms, each slot's terms contiguous.
+      std::vector<std::pair<uint32_t, size_t>> order;  // (slot, term)
+      for (uint32_t sl = 0; sl < nw + 2; sl++)
+        for (size_t t : slot[sl]) order.push_back({sl, t});
+      for (size_t b = 0; b < order.size(); b += kAccumTerms) {
+        AccumLaunch L;
+        L.tile = true;
+        AccumTileArgs &a = L.t;
+        a.nw = nw;
+        for (uint32_t k = 0; k < nw; k++) a.w[k] = uint8_t(wires1[w0 + k]);
+        const size_t e = std::min(order.size(), b + kAccumTerms);
+        uint32_t cnt = 0;
+        for (uint32_t sl = 0; sl < nw + 2; sl++) {
+          a.off[sl] = cnt;
+          for (size_t j = b; j < e; j++)
+            if (order[j].first == sl) {
+              const Term &t = h[order[j].second];
+              coef(t, a.cre[cnt], a.cim[cnt]);
+              a.sign[cnt] = t.p.y | t.p.z;
+              cnt++;
+            }
+        }
+        a.off[nw + 2] = cnt;
+        out.push_back(L);
+      }
+    }
+    for (const auto &kv : by_flip) {
+      if (__builtin_popcountll(kv.first) <= 1) continue;  // done on a tile above
+      const std::vector<size_t> &ts = kv.second;
+      for (size_t b = 0; b < ts.size(); b += kAccumTerms) {
+        AccumLaunch L;
+        AccumArgs &a = L.g;
+        a.flip = kv.first;
+        for (size_t j = b; j < ts.size() && j < b + kAccumTerms; j++) {
+          const Term &t = h[ts[j]];
+          coef(t, a.cre[a.nterms], a.cim[a.nterms]);
+          a.sign[a.nterms] = t.p.y | t.p.z;
+          a.nterms++;
+        }
+        out.push_back(L);
+      }
+    }
+    return out;
+  }
 
   // Prepared launch of a parameterised op whose parameters sit in the arena.
   struct Pending {
@@ -1826,28 +2137,7 @@ void Simulator::sample_noalloc(size_t shots, uint64_t seed, uint64_t *out) {
   check(hipStreamSynchronize(s->stream), "GPU error");
 }
 
-namespace {
-// U^dagger (reference simulator.mm:786-799); U3(t,p,l)^dagger = U3(-t,-l,-p).
-Gate inverse(const Gate &g) {
-  Gate v = g;
-  switch (g.kind) {
-    case K_RX: case K_RY: case K_RZ: case K_P: case K_CP: v.p[0] = -g.p[0]; break;
-    case K_S: v.kind = K_SDG; break;
-    case K_SDG: v.kind = K_S; break;
-    case K_T: v.kind = K_TDG; break;
-    case K_TDG: v.kind = K_T; break;
-    case K_U3: v.p[0] = -g.p[0]; v.p[1] = -g.p[2]; v.p[2] = -g.p[1]; break;
-    default: break;
-  }
-  return v;
-}
 
-bool is_rotation(uint8_t k) { return k == K_RX || k == K_RY || k == K_RZ; }
-// A gate gets a gradient entry when it carries a parameter and has a native
-// generator (reference simulator.mm:822-828, :917): RX/RY/RZ added with no
-// parameter report 0, and a P/CP/U3 without one is not differentiated.
-bool differentiable(const Gate &g) { return g.np > 0 && is_rotation(g.kind); }
-}  // namespace
 
 // Adjoint gradient (reference simulator.mm:814-963). The forward pass runs
 // the fused plan (only the final state is needed); the co-state is built
@@ -1885,88 +2175,21 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   const double amp_bytes = 8.0 * double(s->dim);
   s->timed(kTimingCoState, amp_bytes, [&] { launch_zero(s->stream, s->costate, s->dim); });
   s->launched();
-  std::map<uint64_t, std::vector<size_t>> by_flip;
-  for (size_t k = 0; k < h.size(); k++) by_flip[h[k].p.x | h[k].p.y].push_back(k);
-  auto coef = [&](const Term &t, double &re, double &im) {  // c * i^ny
-    const uint32_t ny = uint32_t(__builtin_popcountll(t.p.y));
-    re = t.c;
-    im = 0.0;
-    switch (ny & 3u) {
-      case 1: re = 0.0; im = t.c; break;
-      case 2: re = -t.c; break;
-      case 3: re = 0.0; im = -t.c; break;
-      default: break;
+  for (Impl::AccumLaunch &L : Impl::costate_launches(h)) {
+    if (L.tile) {
+      L.t.lam = s->costate;
+      L.t.psi = s->state;
+      L.t.dim = s->dim;
+      s->timed(kTimingCoState, 3 * amp_bytes, [&] { launch_pauli_accum_tile(s->stream, L.t); });
+    } else {
+      L.g.lam = s->costate;
+      L.g.psi = s->state;
+      L.g.dim = s->dim;
+      s->timed(kTimingCoState, 3 * amp_bytes, [&] { launch_pauli_accum(s->stream, L.g); });
     }
-  };
-  // Flips of at most one bit share register-tile passes: bit 0 and the
-  // no-flip terms ride with the first tile, the other flipped wires go three
-  // to a tile. Flips of two or more bits take one pass each.
-  std::vector<uint32_t> wires1;
-  for (const auto &kv : by_flip)
-    if (__builtin_popcountll(kv.first) == 1 && kv.first != 1)
-      wires1.push_back(uint32_t(__builtin_ctzll(kv.first)));
-  const bool has_tile = by_flip.count(0) || by_flip.count(1) || !wires1.empty();
-  const size_t ntiles = has_tile ? std::max<size_t>(1, (wires1.size() + kAccumTileWires - 1) / kAccumTileWires) : 0;
-  for (size_t g = 0; g < ntiles; g++) {
-    const size_t w0 = g * kAccumTileWires;
-    const uint32_t nw = uint32_t(std::min<size_t>(kAccumTileWires, wires1.size() - std::min(w0, wires1.size())));
-    // Terms of this tile by slot: 0 flip bit 0, 1 + k wire k, nw + 1 none.
-    std::vector<std::vector<size_t>> slot(nw + 2);
-    if (g == 0) {
-      if (by_flip.count(1)) slot[0] = by_flip[1];
-      if (by_flip.count(0)) slot[nw + 1] = by_flip[0];
-    }
-    for (uint32_t k = 0; k < nw; k++) slot[1 + k] = by_flip[1ull << wires1[w0 + k]];
-    // Launches of at most kAccumTerms terms, each slot's terms contiguous.
-    std::vector<std::pair<uint32_t, size_t>> order;  // (slot, term)
-    for (uint32_t sl = 0; sl < nw + 2; sl++)
-      for (size_t t : slot[sl]) order.push_back({sl, t});
-    for (size_t b = 0; b < order.size(); b += kAccumTerms) {
-      AccumTileArgs a{};
-      a.lam = s->costate;
-      a.psi = s->state;
-      a.dim = s->dim;
-      a.nw = nw;
-      for (uint32_t k = 0; k < nw; k++) a.w[k] = uint8_t(wires1[w0 + k]);
-      const size_t e = std::min(order.size(), b + kAccumTerms);
-      uint32_t n = 0;
-      for (uint32_t sl = 0; sl < nw + 2; sl++) {
-        a.off[sl] = n;
-        for (size_t j = b; j < e; j++)
-          if (order[j].first == sl) {
-            const Term &t = h[order[j].second];
-            coef(t, a.cre[n], a.cim[n]);
-            a.sign[n] = t.p.y | t.p.z;
-            n++;
-          }
-      }
-      a.off[nw + 2] = n;
-      s->timed(kTimingCoState, 3 * amp_bytes, [&] { launch_pauli_accum_tile(s->stream, a); });
-      s->launched();
-      s->dispatches++;
-      s->launches++;
-    }
-  }
-  for (const auto &kv : by_flip) {
-    if (__builtin_popcountll(kv.first) <= 1) continue;  // done on a tile above
-    const std::vector<size_t> &ts = kv.second;
-    for (size_t b = 0; b < ts.size(); b += kAccumTerms) {
-      AccumArgs a{};
-      a.lam = s->costate;
-      a.psi = s->state;
-      a.dim = s->dim;
-      a.flip = kv.first;
-      for (size_t j = b; j < ts.size() && j < b + kAccumTerms; j++) {
-        const Term &t = h[ts[j]];
-        coef(t, a.cre[a.nterms], a.cim[a.nterms]);
-        a.sign[a.nterms] = t.p.y | t.p.z;
-        a.nterms++;
-      }
-      s->timed(kTimingCoState, 3 * amp_bytes, [&] { launch_pauli_accum(s->stream, a); });
-      s->launched();
-      s->dispatches++;
-      s->launches++;
-    }
+    s->launched();
+    s->dispatches++;
+    s->launches++;
   }
   const size_t G = c.gates.size();
   size_t nrot_total = 0;
@@ -1981,197 +2204,24 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   s->launches++;
 
   if (s->merge && s->n >= 6) {
-    // Backward sweep in merged passes: inverse gates whose target wires fit
-    // one 16-amplitude tile (wire 0 plus three register wires), at most
-    // kAdjMaxRot rotations each, applied to both states in one read and one
-    // write of each; overlaps reduced per pass. Gates are taken from a
-    // dependency graph of the reversed circuit: two gates keep their reverse
-    // order when their wires overlap, unless both are diagonal. A gate moved
-    // past others commutes with them, so each rotation's overlap <l|P|psi> is
-    // the one the gate-by-gate sweep would see (U'P U'^dagger = P).
-    struct APass {
-      std::vector<size_t> gates;  // gate indices, execution order
-      std::vector<uint32_t> reg;
-      uint32_t nrot = 0;
-      size_t rot0 = 0;  // sweep index of its first rotation
-    };
-    auto targets = [](const Gate &g) {
-      std::vector<uint32_t> t;
-      if (g.nq == 1) t.push_back(g.q[0]);
-      else if (g.kind == K_SWAP) t = {g.q[0], g.q[1]};
-      else t.push_back(g.q[g.nq - 1]);
-      return t;
-    };
-    // Units in sweep order (gate G-1 first) and their order constraints.
-    std::vector<std::vector<uint32_t>> succ(G);
-    std::vector<uint32_t> ndeps(G, 0);
-    {
-      std::vector<int64_t> last(64, -1);
-      std::vector<std::vector<uint32_t>> diags(64);
-      std::vector<uint32_t> deps;
-      for (size_t u = 0; u < G; u++) {
-        const Gate &g = c.gates[G - 1 - u];
-        const bool dg = is_diagonal(g);
-        deps.clear();
-        for (uint32_t j = 0; j < g.nq; j++) {
-          const uint32_t q = g.q[j];
-          if (last[q] >= 0) deps.push_back(uint32_t(last[q]));
-          if (!dg) deps.insert(deps.end(), diags[q].begin(), diags[q].end());
-        }
-        std::sort(deps.begin(), deps.end());
-        deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
-        for (uint32_t d : deps) succ[d].push_back(uint32_t(u));
-        ndeps[u] = uint32_t(deps.size());
-        for (uint32_t j = 0; j < g.nq; j++) {
-          const uint32_t q = g.q[j];
-          if (dg) {
-            diags[q].push_back(uint32_t(u));
-          } else {
-            last[q] = int64_t(u);
-            diags[q].clear();
-          }
-        }
-      }
-    }
-    std::vector<uint32_t> ready;
-    for (uint32_t u = 0; u < G; u++)
-      if (!ndeps[u]) ready.push_back(u);
-    std::vector<APass> passes;
-    std::vector<size_t> rot_index(G, 0);  // sweep index of each rotation
-    size_t sweep_rot = 0, done = 0;
-    while (done < G) {
-      APass cur;
-      cur.rot0 = sweep_rot;
-      for (;;) {
-        // The ready gate adding the fewest register wires, then the earliest.
-        int best = -1;
-        size_t best_j = 0, best_add = 0;
-        for (size_t j = 0; j < ready.size(); j++) {
-          const Gate &g = c.gates[G - 1 - ready[j]];
-          size_t add = 0;
-          for (uint32_t q : targets(g))
-            if (q != 0 && std::find(cur.reg.begin(), cur.reg.end(), q) == cur.reg.end()) add++;
-          const uint32_t r = is_rotation(g.kind) ? 1u : 0u;
-          if (cur.reg.size() + add > 3 || cur.nrot + r > uint32_t(kAdjMaxRot)) continue;
-          if (best < 0 || add < best_add || (add == best_add && ready[j] < uint32_t(best))) {
-            best = int(ready[j]);
-            best_j = j;
-            best_add = add;
-          }
-        }
-        if (best < 0) break;
-        const size_t k = G - 1 - size_t(best);
-        const Gate &g = c.gates[k];
-        for (uint32_t q : targets(g))
-          if (q != 0 && std::find(cur.reg.begin(), cur.reg.end(), q) == cur.reg.end())
-            cur.reg.push_back(q);
-        if (is_rotation(g.kind)) {
-          rot_index[k] = sweep_rot++;
-          cur.nrot++;
-        }
-        cur.gates.push_back(k);
-        ready.erase(ready.begin() + long(best_j));
-        for (uint32_t t : succ[size_t(best)])
-          if (--ndeps[t] == 0) ready.push_back(t);
-        done++;
-      }
-      if (cur.gates.empty()) fail("internal: adjoint schedule");
-      passes.push_back(std::move(cur));
-    }
-    // Ops of every pass in one staging block.
+    // Backward sweep in merged passes (adjoint_schedule); the ops of every
+    // pass in one staging block.
+    std::vector<size_t> rot_index;
+    const std::vector<Impl::APass> passes = Impl::adjoint_schedule(c, rot_index);
     std::vector<size_t> offs(passes.size());
     size_t total = 0;
     for (size_t i = 0; i < passes.size(); i++) {
       offs[i] = total;
-      size_t nops = 0;
-      for (size_t k : passes[i].gates) nops += c.gates[k].kind == K_SWAP ? 3 : 1;
-      total += (nops * sizeof(AdjOp) + 255) & ~size_t(255);
+      total += Impl::adj_pass_bytes(c, passes[i]);
     }
     s->stage(total);
     std::vector<AdjPassArgs> args(passes.size());
     for (size_t i = 0; i < passes.size(); i++) {
-      const APass &ps = passes[i];
-      LayoutBuilder lb{s->n};
-      for (uint32_t q : ps.reg) lb.add(q);
-      int nl = 0;
       AdjPassArgs &a = args[i];
-      a.lay = lb.finish(4, &nl);
-      if (nl != 4) fail("internal: adjoint pass width");
-      AdjOp *ops = reinterpret_cast<AdjOp *>(s->staging + offs[i]);
-      uint32_t no = 0, rs = 0;
-      auto slot = [&](uint32_t target, const float *m8, std::vector<uint32_t> ctrls) -> AdjOp & {
-        AdjOp &o = ops[no++];
-        o = AdjOp{};
-        const int lt = lb.local_of(target);
-        if (lt < 0) fail("internal: adjoint target not local");
-        o.lbit = uint32_t(lt);
-        uint32_t cm = 0;
-        for (uint32_t q : ctrls) {
-          const int l = lb.local_of(q);
-          if (l >= 0) cm |= 1u << l;
-          else o.gmask |= 1ull << q;
-        }
-        std::memcpy(o.m, m8, sizeof(o.m));
-        const bool real = m8[1] == 0.0f && m8[3] == 0.0f && m8[5] == 0.0f && m8[7] == 0.0f;
-        o.sel = o.lbit + 8u * slot_kind(m8, real, cm != 0, o.gmask != 0);
-        uint32_t pidx = 0;
-        for (uint32_t l = 0; l < 16; l++) {
-          if ((l >> o.lbit) & 1) continue;
-          if ((l & cm) == cm) o.pm |= 1u << pidx;
-          pidx++;
-        }
-        return o;
-      };
-      for (size_t k : ps.gates) {
-        const Gate &g = c.gates[k];
-        const Gate inv = inverse(g);
-        float m8[8];
-        if (g.nq == 1) {
-          cd m[4];
-          gate_matrix(inv.kind, inv.p, m);
-          narrow(m, m8);
-          AdjOp &o = slot(g.q[0], m8, {});
-          if (is_rotation(g.kind)) {
-            o.type = 1;
-            o.axis = g.kind == K_RX ? 0 : g.kind == K_RY ? 1 : 2;
-            o.rslot = rs++;
-          }
-          continue;
-        }
-        switch (g.kind) {
-          case K_CX:
-          case K_CY: {
-            cd m[4];
-            gate_matrix(g.kind == K_CX ? K_X : K_Y, nullptr, m);
-            narrow(m, m8);
-            slot(g.q[1], m8, {g.q[0]});
-            break;
-          }
-          case K_CZ:
-          case K_CP: {
-            const cd ph = inv.kind == K_CZ ? cd(-1, 0) : cd(std::cos(inv.p[0]), std::sin(inv.p[0]));
-            const cd m[4] = {1.0, 0.0, 0.0, ph};
-            narrow(m, m8);
-            slot(g.q[1], m8, {g.q[0]});
-            break;
-          }
-          case K_SWAP:
-            slot(g.q[1], kX, {g.q[0]});
-            slot(g.q[0], kX, {g.q[1]});
-            slot(g.q[1], kX, {g.q[0]});
-            break;
-          case K_CCX:
-            slot(g.q[2], kX, {g.q[0], g.q[1]});
-            break;
-          default:
-            fail(std::string("unsupported gate: ") + kind_name(g.kind));
-        }
-      }
+      s->emit_adj_pass(c, passes[i], reinterpret_cast<AdjOp *>(s->staging + offs[i]), a);
       a.psi = s->state;
       a.lam = s->costate;
       a.ops = reinterpret_cast<const AdjOp *>(s->arena + offs[i]);
-      a.nops = no;
-      a.nrot = ps.nrot;
     }
     s->upload(total);
     s->grow(s->adj_partials, s->adj_partials_cap, size_t(kAdjMaxRot) * kAdjBlocks * 2,
@@ -2282,6 +2332,306 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
   s->staging_pending = true;
   check(hipEventRecord(ev, s->stream), "hipEventRecord(done)");
   check(hipStreamWaitEvent(cs, ev, 0), "hipStreamWaitEvent(caller)");
+}
+
+// --- batched adjoint programs ------------------------------------------------
+
+namespace {
+// One forward pass of a batched program: the generic register-tile, low tile
+// or wide tile kernel with its (instance 0) arguments.
+struct BatchedPass {
+  int kind = 0;  // 0 register tile, 1 low tile, 2 wide tile
+  int nl = 0, wb = 0;
+  PassArgs a{};
+  TileArgs t{};
+  GWideArgs g{};
+};
+struct BatchedAccum {
+  bool tile = false;
+  AccumTileArgs t{};
+  AccumArgs g{};
+};
+
+bool same_layout(const Layout &x, const Layout &y) {
+  if (x.nthreads != y.nthreads || x.fixed != y.fixed || x.nins != y.nins || x.streaming != y.streaming)
+    return false;
+  for (uint32_t k = 0; k < x.nins; k++)
+    if (x.ins[k] != y.ins[k] || x.hm[k] != y.hm[k]) return false;
+  return std::memcmp(x.rpos, y.rpos, sizeof(x.rpos)) == 0;
+}
+
+// Plans equal in everything the executor's pass structure reads: kinds,
+// wires, masks, block bits and which phase layers have a zero global phase.
+bool same_structure(const Plan &x, const Plan &y) {
+  if (x.n != y.n || x.ops.size() != y.ops.size()) return false;
+  for (size_t k = 0; k < x.ops.size(); k++) {
+    const PlanOp &a = x.ops[k], &b = y.ops[k];
+    if (a.kind != b.kind || std::memcmp(a.q, b.q, sizeof(a.q)) != 0 || a.wires != b.wires ||
+        a.masks != b.masks || a.block_bits != b.block_bits ||
+        (turns(a.global) == 0.0) != (turns(b.global) == 0.0))
+      return false;
+  }
+  return true;
+}
+
+template <typename T>
+T *device_copy(const std::vector<T> &v, const char *what) {
+  T *p = nullptr;
+  check(hipMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(v.size(), 1) * sizeof(T)), what);
+  if (!v.empty())
+    check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+  return p;
+}
+}  // namespace
+
+struct BatchedAdjoint::Data {
+  uint32_t n = 0, count = 0, nrot = 0;
+  uint64_t dim = 0;
+  float2 *psi = nullptr, *lam = nullptr;
+  uint8_t *arena = nullptr;
+  double *red = nullptr, *adj = nullptr, *eslots = nullptr, *rslots = nullptr;
+  uint32_t *rgate = nullptr, *inst = nullptr, *gbase = nullptr;
+  std::vector<BatchedPass> fwd;
+  std::vector<BatchedAccum> costate;
+  std::vector<AdjPassArgs> bwd;
+  std::vector<size_t> bwd_rot0;
+
+  ~Data() {
+    hipFree(psi);
+    hipFree(lam);
+    hipFree(arena);
+    hipFree(red);
+    hipFree(adj);
+    hipFree(eslots);
+    hipFree(rslots);
+    hipFree(rgate);
+    hipFree(inst);
+    hipFree(gbase);
+  }
+};
+
+BatchedAdjoint::BatchedAdjoint() : d_(new Data) {}
+BatchedAdjoint::~BatchedAdjoint() = default;
+uint32_t BatchedAdjoint::count() const { return d_->count; }
+uint32_t BatchedAdjoint::num_qubits() const { return d_->n; }
+
+// The single-circuit adjoint (energy_and_gradient with merged passes) for
+// every instance at once: |0...0>, the forward passes, lambda = H psi, E =
+// Re<psi|lambda>, the merged backward passes with their overlaps, then the
+// results scattered to the caller's arrays.
+void BatchedAdjoint::enqueue(void *stream, double *energy_out, double *grad_out) const {
+  const Data &d = *d_;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  launch_fill_basis_batched(st, d.psi, d.dim, d.count);
+  for (const BatchedPass &b : d.fwd) {
+    if (b.kind == 1) launch_tile_batched(st, b.t, d.count);
+    else if (b.kind == 2) launch_gwide_batched(st, b.wb, b.g, d.count);
+    else launch_pass_batched(st, b.nl, b.a, d.count);
+  }
+  launch_zero_batched(st, d.lam, d.dim, d.count);
+  for (const BatchedAccum &b : d.costate) {
+    if (b.tile) launch_pauli_accum_tile_batched(st, b.t, d.count);
+    else launch_pauli_accum_batched(st, b.g, d.count);
+  }
+  launch_pauli_batched(st, d.psi, d.lam, d.dim, 0, 0, d.red, d.eslots, d.count);
+  for (size_t j = 0; j < d.bwd.size(); j++)
+    launch_adj_pass_batched(st, d.bwd[j], d.adj, d.rslots + 2 * d.bwd_rot0[j], 2ull * d.nrot,
+                            d.count);
+  launch_batched_results(st, d.eslots, d.rslots, d.count, d.nrot, d.rgate, d.inst, d.gbase,
+                         energy_out, grad_out);
+  check(hipGetLastError(), "batched adjoint launch");
+}
+
+std::unique_ptr<BatchedAdjoint> Simulator::compile_batched(
+    const std::vector<const Circuit *> &circuits, const Hamiltonian &h,
+    const std::vector<uint32_t> &inst, const std::vector<uint32_t> &gbase) {
+  Impl *s = impl_.get();
+  const size_t m = circuits.size();
+  if (!m || inst.size() != m || gbase.size() != m) fail("compile_batched: argument sizes");
+  if (s->n < kPassMinQubits) return nullptr;
+  const Circuit &c0 = *circuits[0];
+  for (const Circuit *c : circuits) {
+    if (c->n != s->n || c->gates.size() != c0.gates.size()) return nullptr;
+    for (size_t k = 0; k < c0.gates.size(); k++) {
+      const Gate &a = c->gates[k], &b = c0.gates[k];
+      if (a.kind != b.kind || a.nq != b.nq || (a.np > 0) != (b.np > 0) ||
+          std::memcmp(a.q, b.q, sizeof(a.q)) != 0)
+        return nullptr;
+      if (a.np > 0 && !is_rotation(a.kind)) fail("compile_batched: non-differentiable parameter");
+    }
+    check_circuit(*c);
+  }
+  for (const Term &t : h)
+    if (!t.p.valid(s->n)) fail("compile_batched: invalid Pauli term");
+  std::vector<Plan> plans(m);
+  for (size_t i = 0; i < m; i++) {
+    plans[i] = build_plan(*circuits[i], FuseOpts{});
+    if (!same_structure(plans[i], plans[0])) return nullptr;
+  }
+  // Forward passes: the schedule of the first plan; a single op becomes a
+  // one-op register-tile pass, chain-shaped passes are not batched.
+  std::vector<Pass> passes;
+  for (Pass &ps : schedule_passes(plans[0])) {
+    if (ps.wide || ps.low) return nullptr;
+    if (!ps.merged) {
+      const PlanOp &op = plans[0].ops[ps.ops[0]];
+      if (op.kind == OpKind::Diagonal && op.masks.empty()) continue;  // no work
+      if (!pass_mergeable(op)) return nullptr;
+      for (size_t i = 1; i < m; i++)
+        if (pass_fixed(plans[i].ops[ps.ops[0]]) != pass_fixed(op)) return nullptr;
+      ps.merged = true;
+      ps.reg = pass_wires(op);
+      ps.fixed = pass_fixed(op);
+      if (ps.reg.size() > kPassMaxReg) return nullptr;
+    }
+    passes.push_back(std::move(ps));
+  }
+  std::vector<size_t> rot_index;
+  const std::vector<Impl::APass> apasses = Impl::adjoint_schedule(c0, rot_index);
+  // Per-instance arena region: forward passes, then backward passes.
+  std::vector<size_t> foff(passes.size()), boff(apasses.size());
+  size_t stride = 0;
+  for (size_t k = 0; k < passes.size(); k++) {
+    foff[k] = stride;
+    stride += s->pass_param_bytes(plans[0], passes[k]);
+  }
+  for (size_t j = 0; j < apasses.size(); j++) {
+    boff[j] = stride;
+    stride += Impl::adj_pass_bytes(c0, apasses[j]);
+  }
+  stride = (stride + 255) & ~size_t(255);
+  const uint64_t state_bytes = uint64_t(m) * s->dim * sizeof(float2);
+  if (2 * state_bytes + m * stride > free_memory() / 2) return nullptr;
+
+  std::unique_ptr<BatchedAdjoint> out(new BatchedAdjoint);
+  BatchedAdjoint::Data &d = *out->d_;
+  d.n = s->n;
+  d.dim = s->dim;
+  d.count = uint32_t(m);
+  check(hipMalloc(reinterpret_cast<void **>(&d.arena), m * stride), "hipMalloc(batched arena)");
+  std::vector<uint8_t> host(m * stride);
+  {
+    // Compile through the executor's own pass preparation, pointed at this
+    // program's host block and arena.
+    struct Swap {
+      Impl *s;
+      uint8_t *st, *ar;
+      ~Swap() {
+        s->staging = st;
+        s->arena = ar;
+        s->batch_compile = false;
+      }
+    } swap{s, s->staging, s->arena};
+    s->staging = host.data();
+    s->arena = d.arena;
+    s->batch_compile = true;
+    std::vector<Impl::PassLaunch> tmpl(passes.size());
+    for (size_t i = 0; i < m; i++) {
+      for (size_t k = 0; k < passes.size(); k++) {
+        Impl::PassLaunch pl{};
+        const size_t off = i * stride + foff[k];
+        if (passes[k].tile) s->prep_tile(plans[i], passes[k], off, pl);
+        else if (passes[k].gwide) s->prep_gwide(plans[i], passes[k], off, pl);
+        else s->prep_pass(plans[i], passes[k], off, pl);
+        if (pl.chain || pl.wide || pl.low) return nullptr;
+        if (i == 0) {
+          tmpl[k] = pl;
+          continue;
+        }
+        const Impl::PassLaunch &t = tmpl[k];
+        bool same = pl.tile == t.tile && pl.gwide == t.gwide && pl.nl == t.nl &&
+                    pl.chain_k == t.chain_k;
+        if (same && pl.tile) same = same_layout(pl.t.lay, t.t.lay) && pl.t.nsegs == t.t.nsegs;
+        else if (same && pl.gwide)
+          same = same_layout(pl.g.lay, t.g.lay) && pl.g.na == t.g.na && pl.g.nops == t.g.nops &&
+                 std::memcmp(pl.g.pos, t.g.pos, sizeof(pl.g.pos)) == 0 &&
+                 std::memcmp(pl.g.lanepos, t.g.lanepos, sizeof(pl.g.lanepos)) == 0;
+        else if (same) same = same_layout(pl.a.lay, t.a.lay) && pl.a.nops == t.a.nops;
+        if (!same) return nullptr;
+      }
+      std::vector<AdjPassArgs> bw(apasses.size());
+      for (size_t j = 0; j < apasses.size(); j++) {
+        AdjPassArgs a{};
+        s->emit_adj_pass(*circuits[i], apasses[j],
+                         reinterpret_cast<AdjOp *>(host.data() + i * stride + boff[j]), a);
+        if (i == 0) {
+          a.ops = reinterpret_cast<const AdjOp *>(d.arena + boff[j]);
+          d.bwd.push_back(a);
+          d.bwd_rot0.push_back(apasses[j].rot0);
+        } else if (!same_layout(a.lay, d.bwd[j].lay) || a.nops != d.bwd[j].nops ||
+                   a.nrot != d.bwd[j].nrot) {
+          return nullptr;
+        }
+      }
+    }
+    for (size_t k = 0; k < passes.size(); k++) {
+      BatchedPass b;
+      const Impl::PassLaunch &t = tmpl[k];
+      b.nl = t.nl;
+      if (t.tile) {
+        b.kind = 1;
+        b.t = t.t;
+      } else if (t.gwide) {
+        b.kind = 2;
+        b.wb = t.chain_k - 4;
+        b.g = t.g;
+      } else {
+        b.a = t.a;
+      }
+      d.fwd.push_back(b);
+    }
+  }
+  check(hipMemcpy(d.arena, host.data(), host.size(), hipMemcpyHostToDevice), "batched arena");
+  check(hipMalloc(reinterpret_cast<void **>(&d.psi), state_bytes), "hipMalloc(batched states)");
+  check(hipMalloc(reinterpret_cast<void **>(&d.lam), state_bytes), "hipMalloc(batched co-states)");
+  for (BatchedPass &b : d.fwd) {
+    b.a.a = b.t.a = b.g.a = d.psi;
+    b.a.bstate = b.t.bstate = b.g.bstate = d.dim;
+    b.a.barena = b.t.barena = b.g.barena = stride;
+  }
+  for (AdjPassArgs &a : d.bwd) {
+    a.psi = d.psi;
+    a.lam = d.lam;
+    a.bstate = d.dim;
+    a.barena = stride;
+  }
+  for (Impl::AccumLaunch &L : Impl::costate_launches(h)) {
+    BatchedAccum b;
+    b.tile = L.tile;
+    b.t = L.t;
+    b.g = L.g;
+    b.t.lam = b.g.lam = d.lam;
+    b.t.psi = b.g.psi = d.psi;
+    b.t.dim = b.g.dim = d.dim;
+    b.t.bstate = b.g.bstate = d.dim;
+    d.costate.push_back(b);
+  }
+  // Rotations in sweep order: their gate and which part of the overlap is
+  // the gradient (Im(i^ny S): the real part for RY, the imaginary part else).
+  std::vector<uint32_t> rgate;
+  for (size_t k = 0; k < c0.gates.size(); k++)
+    if (is_rotation(c0.gates[k].kind)) d.nrot++;
+  rgate.assign(d.nrot, 0);
+  for (size_t k = 0; k < c0.gates.size(); k++)
+    if (is_rotation(c0.gates[k].kind))
+      rgate[rot_index[k]] = uint32_t(k) | (c0.gates[k].kind == K_RY ? 0x80000000u : 0u);
+  d.rgate = device_copy(rgate, "batched rotation map");
+  d.inst = device_copy(inst, "batched instances");
+  d.gbase = device_copy(gbase, "batched gradient bases");
+  check(hipMalloc(reinterpret_cast<void **>(&d.red),
+                  size_t(m) * pauli_batched_blocks(d.dim) * 2 * sizeof(double)),
+        "hipMalloc(batched partials)");
+  unsigned ablocks = 1;
+  for (const AdjPassArgs &a : d.bwd) ablocks = std::max(ablocks, adj_batched_blocks(a.lay));
+  check(hipMalloc(reinterpret_cast<void **>(&d.adj),
+                  size_t(m) * kAdjMaxRot * ablocks * 2 * sizeof(double)),
+        "hipMalloc(batched adjoint partials)");
+  check(hipMalloc(reinterpret_cast<void **>(&d.eslots), size_t(m) * 2 * sizeof(double)),
+        "hipMalloc(batched energies)");
+  check(hipMalloc(reinterpret_cast<void **>(&d.rslots),
+                  size_t(m) * std::max<uint32_t>(d.nrot, 1) * 2 * sizeof(double)),
+        "hipMalloc(batched overlaps)");
+  return out;
 }
 
 }  // namespace qmamd

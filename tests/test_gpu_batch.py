@@ -163,3 +163,37 @@ def test_bench_workload_against_fp64_adjoint():
     print(f"autotuned {cfg}")
     for n, i, e0, de, dg, gm in rows:
         assert dg <= 1e-5 and de <= 3e-6 * max(1.0, abs(e0)), (n, i, de, dg)
+
+
+def test_batched_classes_run_the_merged_passes():
+    """Spread instances of 14 qubits or more that share width, gate
+    structure and Hamiltonian run as one batched program of the
+    single-circuit merged passes per class (every pass one launch over the
+    class). Three classes (QAOA p=3 at 14, 15, 16 qubits, several instances
+    each with their own angles) plus one instance whose Hamiltonian differs
+    (a class of its own): all against the one-circuit path, replays
+    bit-identical, and the batched count as expected under each spread
+    width."""
+    rng = C.SplitMix64(77)
+    inst = []
+    for n in (14, 15, 16, 14, 15, 16, 16):
+        ga = [rng.uniform(0, 3.14) for _ in range(3)]
+        be = [rng.uniform(0, 1.57) for _ in range(3)]
+        inst.append((n, C.qaoa_ring(n, 3, ga, be), C.ring_zz_hamiltonian(n)))
+    n = 15
+    odd_h = C.ring_zz_hamiltonian(n) + [(0.3, 1 << 4, 0, 0)]
+    inst.append((n, C.qaoa_ring(n, 3, [0.2, 0.4, 0.6], [0.1, 0.3, 0.5]), odd_h))
+    circs = [C.to_circuit(n, g) for n, g, _ in inst]
+    hams = [C.to_hamiltonian(h) for _, _, h in inst]
+    want = [plain(n, g, h) for n, g, h in inst]
+    b = qm.Batch().prepare(circs, hams)
+    for spread, batched in ((14, 8), (16, 3), (0, 0)):
+        b.set_spread(spread)
+        assert b.batched_instances() == batched, (spread, b.batched_instances())
+        e1, g1, _ = b.run()
+        e2, g2, _ = b.run()
+        assert np.array_equal(e1, e2) and all(np.array_equal(x, y) for x, y in zip(g1, g2))
+        worst = max(max(abs(e1[i] - e0), np.abs(g1[i] - g0).max())
+                    for i, (e0, g0) in enumerate(want))
+        print(f"spread from {spread}: {batched} batched instances, worst diff vs plain {worst:.2e}")
+        assert worst <= 1e-5, spread
