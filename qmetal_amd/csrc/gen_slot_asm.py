@@ -17,6 +17,7 @@ Section `sel` = 8 * kind + lbit:
   kind 2  complex matrix, only the pairs whose bit is set in `pm` (pair p is
           the p-th index l with bit lbit clear, ascending)
   kind 5  X: swap the two amplitudes of the pairs in `pm` (2 per pair)
+  kind 6  diagonal matrix diag(m00, m11) on the pairs in `pm` (8 per pair)
 Any other `sel` leaves the tile unchanged. The matrix is per lane (VGPRs).
 (Kinds 3 and 4, complex matrices on the matrix core, were measured and
 dropped: slower than the VALU in these passes.)
@@ -84,7 +85,23 @@ def swap_section(nl, h):
     return out
 
 
-KIND_ORDER = (1, 5, 0, 2)  # dispatch order: the common kinds first
+def diag_section(nl, h):
+    """a <- m00 a, b <- m11 b (complex), on the pairs in pm: two products
+    into temporaries, then the fused in-place updates (no moves)."""
+    out = []
+    for p, (a, b) in enumerate(pairs(nl, h)):
+        out += [f"s_bitcmp1_b32 %[pm], {p}", f"s_cbranch_scc0 .Lqd%=_{h}_{p}"]
+        for x, mr, mi in ((a, "%[m0r]", "%[m0i]"), (b, "%[m3r]", "%[m3i]")):
+            rx, ix = f"%[r{x}]", f"%[i{x}]"
+            out += [f"v_mul_f32 %[t0], {mi}, {ix}",
+                    f"v_mul_f32 %[t1], {mi}, {rx}",
+                    f"v_fma_f32 {rx}, {mr}, {rx}, -%[t0]",
+                    f"v_fma_f32 {ix}, {mr}, {ix}, %[t1]"]
+        out.append(f".Lqd%=_{h}_{p}:")
+    return out
+
+
+KIND_ORDER = (1, 6, 5, 0, 2)  # dispatch order: the common kinds first
 
 
 def statement(nl):
@@ -101,6 +118,8 @@ def statement(nl):
             lines.append(f".Lqs%=_{kind}_{h}:")
             if kind == 5:
                 lines += swap_section(nl, h)
+            elif kind == 6:
+                lines += diag_section(nl, h)
             else:
                 for p, (a, b) in enumerate(pairs(nl, h)):
                     if kind == 0:

@@ -148,11 +148,14 @@ const float kX[8] = {0, 0, 1, 0, 1, 0, 0, 0};
 
 // Section kind of a 2x2 slot (slot_asm, sel = 8 * kind + local bit): an X
 // whose controls are all tile-local (or absent) swaps amplitudes (5, two
-// instructions per pair instead of 16); a control on a base-index bit makes
-// the matrix per lane; otherwise real (0), complex (1), or complex over the
-// locally controlled pairs (2).
+// instructions per pair instead of 16); a diagonal matrix multiplies each
+// amplitude (6, eight per pair: RZ, and the phases of CZ and CP, on the
+// locally controlled pairs; a control on a base-index bit turns into the
+// identity matrix per lane, which is diagonal too); otherwise real (0),
+// complex (1), or complex over the locally controlled pairs (2).
 uint32_t slot_kind(const float *m8, bool real, bool local_ctrl, bool lane_ctrl) {
   if (!lane_ctrl && std::memcmp(m8, kX, sizeof(kX)) == 0) return 5u;
+  if (m8[2] == 0.0f && m8[3] == 0.0f && m8[4] == 0.0f && m8[5] == 0.0f) return 6u;
   return local_ctrl ? 2u : (real ? 0u : 1u);
 }
 
