@@ -149,13 +149,19 @@ int qm_plan_get_arrays(qm_plan p, size_t k, uint32_t *group_qubits,
                        double *group_matrices, uint64_t *masks,
                        double *angles);
 
-/* Adjoint energy and gradient with device-resident arguments, for framework
+/* Adjoint energy and gradient with device-memory arguments, for framework
  * tensors (ROCm PyTorch): `params` holds one double per RX/RY/RZ gate in
  * circuit order and replaces those gates' angles; `out_energy` (1 double) and
- * `out_grad` (num_params doubles, one per rotation) are device pointers.
+ * `out_grad` (num_params doubles, one per rotation) are device pointers;
+ * `out_grad` may be NULL, and then only the energy is computed (forward
+ * passes and the Hamiltonian's reductions: no co-state, no backward sweep).
  * `stream` is the caller's hipStream_t (or NULL for the null stream): work
  * already queued there completes before the parameters are read, and work
- * queued there afterwards sees the results. */
+ * queued there afterwards sees the results. Host-synchronous: the parameters
+ * are copied to the host (the plan's matrices and phase tables are built
+ * there), so the call waits for the caller's stream up to that point and
+ * cannot be recorded into a graph; the results are written back
+ * asynchronously. */
 int qm_energy_and_gradient_device(qm_simulator s, qm_circuit c,
                                   const qm_hamiltonian *h, const double *params,
                                   size_t num_params, double *out_energy,

@@ -307,7 +307,7 @@ int qm_energy_and_gradient(qm_simulator s, qm_circuit c, const qm_hamiltonian *h
 int qm_energy_and_gradient_device(qm_simulator s, qm_circuit c, const qm_hamiltonian *h,
                                   const double *params, size_t num_params,
                                   double *out_energy, double *out_grad, void *stream) {
-  if (!s || !c || !h || !out_energy || (num_params && (!params || !out_grad))) {
+  if (!s || !c || !h || !out_energy || (num_params && !params)) {
     set_error("null argument");
     return -1;
   }

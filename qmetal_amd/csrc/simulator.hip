@@ -2282,10 +2282,15 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   return {r[2 * G], grad};
 }
 
-// Device-resident variant for framework tensors: parameters are read from
-// device memory (one per RX/RY/RZ, circuit order), energy and gradient are
-// written to device memory, and the caller's stream is ordered before and
-// after this work through events.
+// Variant for framework tensors: parameters are read from device memory (one
+// per RX/RY/RZ, circuit order), energy and gradient are written to device
+// memory, and the caller's stream is ordered before and after this work
+// through events. The parameters cross to the host (the plan's window
+// products and phase tables are built there), so the call waits for the
+// caller's stream up to that point and cannot be captured in a graph; the
+// results go back asynchronously. Without out_grad only the energy is
+// computed: the forward passes and one FP64 reduction per term, no co-state
+// and no backward sweep.
 void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &h,
                                            const double *params, size_t nparams,
                                            double *out_energy, double *out_grad,
@@ -2319,15 +2324,23 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
     cc.gates[where[j]].p[0] = host[j];
     cc.gates[where[j]].np = std::max<uint8_t>(cc.gates[where[j]].np, 1);
   }
-  auto eg = energy_and_gradient(cc, h);
+  std::pair<double, std::vector<double>> eg;
+  if (out_grad) {
+    eg = energy_and_gradient(cc, h);
+  } else {
+    reset();
+    run(cc, FuseOpts{});
+    eg.first = expectation(h);
+  }
   // Results leave through pinned staging so the copies stay asynchronous.
   uint8_t *st = s->stage((nparams + 1) * sizeof(double));
   double *hv = reinterpret_cast<double *>(st);
   hv[0] = eg.first;
-  for (size_t j = 0; j < nparams; j++) hv[1 + j] = eg.second[where[j]];
+  if (out_grad)
+    for (size_t j = 0; j < nparams; j++) hv[1 + j] = eg.second[where[j]];
   check(hipMemcpyAsync(out_energy, hv, sizeof(double), hipMemcpyHostToDevice, s->stream),
         "hipMemcpyAsync(energy)");
-  if (nparams)
+  if (nparams && out_grad)
     check(hipMemcpyAsync(out_grad, hv + 1, nparams * sizeof(double), hipMemcpyHostToDevice,
                          s->stream),
           "hipMemcpyAsync(grad)");

@@ -9,9 +9,14 @@
 
 `circuit` is a template: its RX/RY/RZ gates, in order, take their angles from
 `theta` (one entry each). The tensors cross the C ABI as raw device pointers
-(qm_energy_and_gradient_device); the forward call runs the fused forward pass,
-the co-state build and the backward kernels, and stores dE/dtheta for the
-backward call. Tensors must be contiguous float64 on the simulator's device.
+(qm_energy_and_gradient_device); when theta requires a gradient, the forward
+call runs the fused forward pass, the co-state build and the backward
+kernels, and stores dE/dtheta for the backward call; otherwise (or through
+`adjoint_energy` with gradients disabled) it runs the forward pass and the
+energy reductions only. The call is host-synchronous: the angles are read
+back to build the plan, so it waits for the caller's stream and cannot be
+captured in a CUDA/HIP graph. Tensors must be contiguous float64 on the
+simulator's device.
 """
 from __future__ import annotations
 
@@ -34,16 +39,18 @@ def _stream_handle(device) -> int:
 
 
 def energy_and_grad(sim: Simulator, circuit: Circuit, hamiltonian: Hamiltonian,
-                    params: torch.Tensor):
-    """(energy, grad) as device tensors, one gradient entry per rotation."""
+                    params: torch.Tensor, need_grad: bool = True):
+    """(energy, grad) as device tensors, one gradient entry per rotation;
+    grad is None when need_grad is False (no co-state, no backward sweep)."""
     if not params.is_cuda or params.dtype != torch.float64:
         raise QMetalError("params must be a float64 tensor on the GPU")
     p = params.detach().contiguous()
     energy = torch.empty((), dtype=torch.float64, device=p.device)
-    grad = torch.empty_like(p)
+    grad = torch.empty_like(p) if need_grad else None
     h = hamiltonian._c()
     _check(_eg_device(sim._h, circuit._h, byref(h), c_void_p(p.data_ptr()), p.numel(),
-                      c_void_p(energy.data_ptr()), c_void_p(grad.data_ptr()),
+                      c_void_p(energy.data_ptr()),
+                      c_void_p(grad.data_ptr() if grad is not None else None),
                       c_void_p(_stream_handle(p.device))))
     return energy, grad
 
@@ -53,11 +60,23 @@ class AdjointEnergy(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, params, sim, circuit, hamiltonian):
-        energy, grad = energy_and_grad(sim, circuit, hamiltonian, params)
-        ctx.save_for_backward(grad)
+        need = bool(ctx.needs_input_grad[0])
+        energy, grad = energy_and_grad(sim, circuit, hamiltonian, params, need_grad=need)
+        if need:
+            ctx.save_for_backward(grad)
         return energy
 
     @staticmethod
     def backward(ctx, grad_out):
         (grad,) = ctx.saved_tensors
         return grad_out * grad, None, None, None
+
+
+def adjoint_energy(params: torch.Tensor, sim: Simulator, circuit: Circuit,
+                   hamiltonian: Hamiltonian) -> torch.Tensor:
+    """AdjointEnergy.apply, or the energy alone (no backward sweep) when
+    gradients are disabled or params does not require one."""
+    if torch.is_grad_enabled() and params.requires_grad:
+        return AdjointEnergy.apply(params, sim, circuit, hamiltonian)
+    energy, _ = energy_and_grad(sim, circuit, hamiltonian, params, need_grad=False)
+    return energy

@@ -72,3 +72,43 @@ def test_torch_parameter_count_checked():
     bad = torch.zeros(5, dtype=torch.float64, device="cuda", requires_grad=True)
     with pytest.raises(qm.QMetalError, match="parameters"):
         AdjointEnergy.apply(bad, sim, circ, C.to_hamiltonian(C.ring_zz_hamiltonian(n)))
+
+
+def test_energy_only_when_no_gradient_is_needed():
+    """Without a gradient to compute (params not requiring one, or gradients
+    disabled) the call runs the forward passes and the energy reductions only:
+    the same energy as the full adjoint call, and measurably less work."""
+    import time
+    from qmetal_amd.torch_adjoint import AdjointEnergy, adjoint_energy, energy_and_grad
+    n = 18
+    rng = C.SplitMix64(5)
+    p0 = [rng.uniform(0, 2 * np.pi) for _ in range(6 * n)]
+    circ = C.to_circuit(n, hea_template(n, 3, p0))
+    ham = C.to_hamiltonian(C.tfim_ring_hamiltonian(n))
+    with qm.Simulator(n) as sim:
+        theta = torch.tensor(p0, dtype=torch.float64, device="cuda", requires_grad=True)
+        e_full, g_full = energy_and_grad(sim, circ, ham, theta)
+        e_only, g_none = energy_and_grad(sim, circ, ham, theta, need_grad=False)
+        assert g_none is None
+        assert abs(e_full.item() - e_only.item()) <= 1e-6
+        with torch.no_grad():
+            e_ng = adjoint_energy(theta, sim, circ, ham)
+        e_plain = AdjointEnergy.apply(theta.detach(), sim, circ, ham)
+        assert abs(e_ng.item() - e_full.item()) <= 1e-6
+        assert abs(e_plain.item() - e_full.item()) <= 1e-6
+        e_g = adjoint_energy(theta, sim, circ, ham)
+        (g,) = torch.autograd.grad(e_g, theta)
+        assert torch.allclose(g, g_full, atol=1e-9)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            energy_and_grad(sim, circ, ham, theta)
+        torch.cuda.synchronize()
+        t_full = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(5):
+            energy_and_grad(sim, circ, ham, theta, need_grad=False)
+        torch.cuda.synchronize()
+        t_energy = time.perf_counter() - t0
+    print(f"energy and gradient {t_full / 5 * 1e3:.2f} ms, energy only {t_energy / 5 * 1e3:.2f} ms")
+    assert t_energy < t_full
