@@ -372,6 +372,12 @@ constexpr int kAdjMaxRot = 8;
 // each pass 29% slower: measured no faster, so three.
 constexpr int kAdjMaxReg = 3;
 constexpr int kAdjBlocks = 4096;
+// Tile passes (k_adj_tile) also take: axis 3 = Z on wires outside the
+// thread's tile-local bits (the overlap of every amplitude, signed per
+// thread), axis 4 = Z Z on tile-local bits lbit and lbit2; a thread whose base
+// index has odd parity over `pmask` applies `malt` instead of `m` and negates
+// its overlap (a diagonal phase or Z generator on a wire that is not
+// tile-local).
 struct AdjOp {
   uint32_t type;
   uint32_t lbit;
@@ -379,9 +385,12 @@ struct AdjOp {
   uint32_t pm;
   uint32_t axis;
   uint32_t rslot;
-  uint32_t pad[2];
+  uint32_t lbit2;
+  uint32_t pad;
   uint64_t gmask;
+  uint64_t pmask;
   float m[8];
+  float malt[8];
 };
 
 struct AdjPassArgs {
@@ -402,6 +411,41 @@ struct AdjPassArgs {
 // `partials` (kAdjMaxRot * kAdjBlocks * 2 doubles) and are reduced in a
 // fixed order into out[2 r], out[2 r + 1] (r < nrot).
 void launch_adj_pass(hipStream_t s, const AdjPassArgs &a, double *partials, double *out);
+
+// Backward tile pass: each workgroup (kAdjTileThreads threads) holds a tile of
+// 2^12 amplitudes of both states, over wires 0..6 and the five wires hpos
+// (loaded with lanes on wires 1..6, the two wave bits on hpos[0..1] and the
+// three register bits on hpos[2..4]); per segment the tile is permuted through
+// LDS so that the segment's three register wires are any three tile wires
+// other than 0, and the segment's AdjOps run on the thread's 16 amplitudes of
+// each state as in the register-tile pass. The slot of tile bit b (1..11) in
+// LDS is tile_swizzle(1 << (b - 1)), xor-combined.
+constexpr uint32_t kAdjTileMinQubits = 12;
+constexpr int kAdjTileThreads = 256;
+constexpr int kAdjTileMaxSeg = 8;
+constexpr int kAdjTileMaxRot = 32;
+constexpr int kAdjTileBlocks = 4096;
+struct AdjSeg {
+  uint32_t op0, op1;  // AdjOps [op0, op1) run in this arrangement
+  uint32_t tslot[8];  // slot part of thread bit j (lane bits 0..5, wave bits 6..7)
+  uint32_t rslot[3];  // slot part of register bit k (tile-local bit k + 1)
+  uint8_t tpos[8];    // wire of thread bit j
+  uint32_t pad;
+};
+struct AdjTileArgs {
+  float2 *psi;
+  float2 *lam;
+  Layout lay;  // tiles: ins = wires 0..6 and hpos, ascending
+  const AdjSeg *segs;
+  const AdjOp *ops;
+  uint32_t nsegs;
+  uint32_t nrot;
+  uint8_t hpos[5];
+  uint64_t bstate;  // batched launches, as in PassArgs
+  uint64_t barena;
+};
+// Per-block partials: kAdjTileMaxRot * kAdjTileBlocks * 2 doubles.
+void launch_adj_tile(hipStream_t s, const AdjTileArgs &a, double *partials, double *out);
 
 // Sampling: FP64 sums of |a|^2 over 1024-amplitude blocks, an inclusive FP64
 // prefix over them, and per-shot resolution (splitmix64 stream, binary search
@@ -429,6 +473,11 @@ void launch_pauli_accum_batched(hipStream_t s, const AccumArgs &a, uint32_t coun
 // blocks * 2 for launch_adj_pass_batched).
 unsigned pauli_batched_blocks(uint64_t dim);
 unsigned adj_batched_blocks(const Layout &lay);
+// Workgroups per instance of launch_adj_tile_batched (partials: count *
+// kAdjTileMaxRot * blocks * 2 doubles).
+unsigned adj_tile_batched_blocks(const Layout &lay);
+void launch_adj_tile_batched(hipStream_t s, const AdjTileArgs &a, double *partials, double *out,
+                             uint64_t ostride, uint32_t count);
 // out[2 i], out[2 i + 1] = instance i's sum (as launch_pauli).
 void launch_pauli_batched(hipStream_t s, const float2 *bra, const float2 *ket, uint64_t dim,
                           uint64_t flip, uint64_t sign, double *partials, double *out,

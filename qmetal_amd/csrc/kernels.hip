@@ -1084,16 +1084,233 @@ __global__ __launch_bounds__(kT) void k_adj_pass(AdjPassArgs a, double *part) {
 // instance y of a batched pass, its partials kAdjMaxRot * count * 2 doubles
 // apart and its outputs `ostride` doubles apart).
 __global__ __launch_bounds__(kT) void k_finish_rot(const double *part, int count, double *out,
-                                                   uint64_t ostride) {
+                                                   uint64_t ostride, int maxrot) {
   __shared__ double sh[kT / 64];
   const size_t r = blockIdx.x;
-  part += size_t(blockIdx.y) * kAdjMaxRot * count * 2;
+  part += size_t(blockIdx.y) * maxrot * count * 2;
   out += size_t(blockIdx.y) * ostride;
   for (int c = 0; c < 2; c++) {
     double acc = 0.0;
     for (int k = threadIdx.x; k < count; k += kT) acc += part[(r * count + k) * 2 + c];
     const double t = block_sum(acc, sh);
     if (threadIdx.x == 0) out[2 * r + c] = t;
+  }
+}
+
+// FP64 conj(x) y accumulated with fused multiply-adds (NEG: subtracted).
+template <bool NEG>
+__device__ __forceinline__ void acc_cj_fma(double &re, double &im, float2 x, float2 y) {
+  const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+  if (NEG) {
+    re = fma(-xr, yr, re);
+    re = fma(-xi, yi, re);
+    im = fma(-xr, yi, im);
+    im = fma(xi, yr, im);
+  } else {
+    re = fma(xr, yr, re);
+    re = fma(xi, yi, re);
+    im = fma(xr, yi, im);
+    im = fma(-xi, yr, im);
+  }
+}
+
+// <lam|P|psi> over a 16-amplitude tile for P = X / Y / Z on local bit LB
+// (axis 0..2), as tile_overlap, with fused accumulation.
+template <int LB>
+__device__ __forceinline__ void tile_overlap16(const float2 (&x)[16], const float2 (&y)[16],
+                                               uint32_t axis, double &re, double &im) {
+#pragma unroll
+  for (int l = 0; l < 16; l++) {
+    if (l & (1 << LB)) continue;
+    const int h = l | (1 << LB);
+    if (axis == 0) {
+      acc_cj_fma<false>(re, im, y[l], x[h]);
+      acc_cj_fma<false>(re, im, y[h], x[l]);
+    } else if (axis == 1) {
+      acc_cj_fma<true>(re, im, y[l], x[h]);
+      acc_cj_fma<false>(re, im, y[h], x[l]);
+    } else {
+      acc_cj_fma<false>(re, im, y[l], x[l]);
+      acc_cj_fma<true>(re, im, y[h], x[h]);
+    }
+  }
+}
+
+// sum_l s(l) conj(y_l) x_l with s(l) = (-1)^(bit LA of l + bit LB of l); LA =
+// LB = -1: s = 1.
+template <int LA, int LB>
+__device__ __forceinline__ void tile_overlap_parity(const float2 (&x)[16], const float2 (&y)[16],
+                                                    double &re, double &im) {
+#pragma unroll
+  for (int l = 0; l < 16; l++) {
+    const int par = (LA >= 0 ? (l >> LA) & 1 : 0) ^ (LB >= 0 ? (l >> LB) & 1 : 0);
+    if (par) acc_cj_fma<true>(re, im, y[l], x[l]);
+    else acc_cj_fma<false>(re, im, y[l], x[l]);
+  }
+}
+
+__device__ __forceinline__ void adj_op_overlap(const float2 (&x)[16], const float2 (&y)[16],
+                                               uint32_t axis, uint32_t lb, uint32_t lb2,
+                                               double &re, double &im) {
+  if (axis <= 2) {
+    switch (lb) {
+      case 0: tile_overlap16<0>(x, y, axis, re, im); break;
+      case 1: tile_overlap16<1>(x, y, axis, re, im); break;
+      case 2: tile_overlap16<2>(x, y, axis, re, im); break;
+      default: tile_overlap16<3>(x, y, axis, re, im); break;
+    }
+  } else if (axis == 3) {
+    tile_overlap_parity<-1, -1>(x, y, re, im);
+  } else {
+    switch (lb * 4 + lb2) {  // lb < lb2
+      case 1: tile_overlap_parity<0, 1>(x, y, re, im); break;
+      case 2: tile_overlap_parity<0, 2>(x, y, re, im); break;
+      case 3: tile_overlap_parity<0, 3>(x, y, re, im); break;
+      case 6: tile_overlap_parity<1, 2>(x, y, re, im); break;
+      case 7: tile_overlap_parity<1, 3>(x, y, re, im); break;
+      default: tile_overlap_parity<2, 3>(x, y, re, im); break;
+    }
+  }
+}
+
+// Moves 16 amplitudes per thread (8 slots) between two arrangements, as
+// tile_permute does for 32.
+__device__ __forceinline__ void tile_permute16(float4 *lds, float2 (&x)[16], uint32_t tf,
+                                               const uint32_t (&rf)[3], uint32_t tt,
+                                               const uint32_t (&rt)[3]) {
+  char *base = reinterpret_cast<char *>(lds);
+  __syncthreads();  // earlier reads of the buffer are done
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const uint32_t o = ((r & 1) ? rf[0] : 0u) ^ ((r & 2) ? rf[1] : 0u) ^ ((r & 4) ? rf[2] : 0u);
+    *reinterpret_cast<float4 *>(base + ((tf ^ o) << 4)) =
+        make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const uint32_t o = ((r & 1) ? rt[0] : 0u) ^ ((r & 2) ? rt[1] : 0u) ^ ((r & 4) ? rt[2] : 0u);
+    const float4 v = *reinterpret_cast<const float4 *>(base + ((tt ^ o) << 4));
+    x[2 * r] = make_float2(v.x, v.y);
+    x[2 * r + 1] = make_float2(v.z, v.w);
+  }
+}
+
+// Backward tile pass (AdjTileArgs). Every workgroup works on whole tiles, so
+// all its threads take part in every op; overlaps are reduced as in
+// k_adj_pass (wave sums into per-wave LDS slots in program order, per-block
+// partials, a fixed-order finish).
+__global__ __launch_bounds__(kAdjTileThreads) void k_adj_tile(AdjTileArgs a, double *part) {
+  constexpr int W = kAdjTileThreads / 64;
+  __shared__ float4 lds[2048];
+  __shared__ double acc[kAdjTileMaxRot][W][2];
+  for (int i = threadIdx.x; i < kAdjTileMaxRot * W * 2; i += kAdjTileThreads)
+    (&acc[0][0][0])[i] = 0.0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t inst = blockIdx.y;
+  float2 *const psi = a.psi + inst * a.bstate;
+  float2 *const lam = a.lam + inst * a.bstate;
+  const const_ptr<AdjSeg> segs = as_const(bshift(a.segs, inst * a.barena));
+  const const_ptr<AdjOp> ops = as_const(bshift(a.ops, inst * a.barena));
+  part += size_t(inst) * kAdjTileMaxRot * gridDim.x * 2;
+  const uint32_t lane_bytes = lane * 16u;
+  // Load arrangement: lane bit j on tile bit j + 1 (wire j + 1), wave bit w
+  // on tile bit 7 + w (hpos[w]), register bit k on tile bit 9 + k
+  // (hpos[2 + k]).
+  const uint64_t wave_off = ((wave & 1u) ? 1ull << a.hpos[0] : 0ull) |
+                            ((wave & 2u) ? 1ull << a.hpos[1] : 0ull);
+  uint64_t reg_off[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++)
+    reg_off[r] = ((r & 1) ? 1ull << a.hpos[2] : 0ull) | ((r & 2) ? 1ull << a.hpos[3] : 0ull) |
+                 ((r & 4) ? 1ull << a.hpos[4] : 0ull);
+  uint32_t t0 = 0;
+#pragma unroll
+  for (int j = 0; j < 6; j++)
+    if ((lane >> j) & 1) t0 ^= tile_swizzle(1u << j);
+#pragma unroll
+  for (int w = 0; w < 2; w++)
+    if ((wave >> w) & 1) t0 ^= tile_swizzle(1u << (6 + w));
+  uint32_t r0[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) r0[k] = tile_swizzle(1u << (8 + k));
+  for (uint64_t tile = blockIdx.x; tile < a.lay.nthreads; tile += gridDim.x) {
+    const uint64_t ub = scatter(tile, a.lay) | a.lay.fixed;
+    float2 x[16], y[16];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const float4 v = ld4<true>(pair_ptr(psi, ub | wave_off | reg_off[r], lane_bytes));
+      const float4 w = ld4<true>(pair_ptr(lam, ub | wave_off | reg_off[r], lane_bytes));
+      x[2 * r] = make_float2(v.x, v.y);
+      x[2 * r + 1] = make_float2(v.z, v.w);
+      y[2 * r] = make_float2(w.x, w.y);
+      y[2 * r + 1] = make_float2(w.z, w.w);
+    }
+    uint32_t tf = t0, rf[3] = {r0[0], r0[1], r0[2]};
+    for (uint32_t s = 0; s < a.nsegs; s++) {
+      const AdjSeg __attribute__((address_space(4))) &sg = segs[s];
+      uint32_t tt = 0;
+      uint64_t base = ub;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const bool on = j < 6 ? ((lane >> j) & 1) != 0 : ((wave >> (j - 6)) & 1) != 0;
+        if (on) {
+          tt ^= sg.tslot[j];
+          base |= 1ull << sg.tpos[j];
+        }
+      }
+      uint32_t rt[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) rt[k] = sg.rslot[k];
+      tile_permute16(lds, x, tf, rf, tt, rt);
+      tile_permute16(lds, y, tf, rf, tt, rt);
+      for (uint32_t o = sg.op0; o < sg.op1; o++) {
+        const bool odd = (__builtin_popcountll(base & ops[o].pmask) & 1) != 0;
+        if (ops[o].type == 1) {
+          double re = 0.0, im = 0.0;
+          adj_op_overlap(x, y, ops[o].axis, ops[o].lbit, ops[o].lbit2, re, im);
+          re = wave_allsum(odd ? -re : re);
+          im = wave_allsum(odd ? -im : im);
+          if (lane == 0) {
+            const uint32_t r = ops[o].rslot;
+            acc[r][wave][0] += re;
+            acc[r][wave][1] += im;
+          }
+        }
+        const uint64_t gmask = ops[o].gmask;
+        const bool gon = (base & gmask) == gmask;
+        float m[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const float mi = odd ? ops[o].malt[i] : ops[o].m[i];
+          m[i] = gon ? mi : (i == 0 || i == 6 ? 1.0f : 0.0f);
+        }
+        const uint32_t sel = __builtin_amdgcn_readfirstlane(ops[o].sel);
+        const uint32_t pm = __builtin_amdgcn_readfirstlane(ops[o].pm);
+        slot_asm<4>(x, sel, pm, m);
+        slot_asm<4>(y, sel, pm, m);
+      }
+      tf = tt;
+#pragma unroll
+      for (int k = 0; k < 3; k++) rf[k] = rt[k];
+    }
+    tile_permute16(lds, x, tf, rf, t0, r0);
+    tile_permute16(lds, y, tf, rf, t0, r0);
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      st4<true>(pair_ptr(psi, ub | wave_off | reg_off[r], lane_bytes),
+                make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
+      st4<true>(pair_ptr(lam, ub | wave_off | reg_off[r], lane_bytes),
+                make_float4(y[2 * r].x, y[2 * r].y, y[2 * r + 1].x, y[2 * r + 1].y));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < a.nrot * 2) {
+    const uint32_t r = threadIdx.x >> 1, c = threadIdx.x & 1;
+    double t = 0.0;
+    for (int w = 0; w < W; w++) t += acc[r][w][c];
+    part[(size_t(r) * gridDim.x + blockIdx.x) * 2 + c] = t;
   }
 }
 
@@ -1482,7 +1699,13 @@ void adj_pass_nl(dim3 grid, hipStream_t s, const AdjPassArgs &a, double *partial
 
 void launch_adj_pass(hipStream_t s, const AdjPassArgs &a, double *partials, double *out) {
   adj_pass_nl<4>(dim3(kAdjBlocks), s, a, partials);
-  if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, kAdjBlocks, out, 0);
+  if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, kAdjBlocks, out, 0, kAdjMaxRot);
+}
+
+void launch_adj_tile(hipStream_t s, const AdjTileArgs &a, double *partials, double *out) {
+  const unsigned blocks = unsigned(std::min<uint64_t>(a.lay.nthreads, kAdjTileBlocks));
+  k_adj_tile<<<blocks, kAdjTileThreads, 0, s>>>(a, partials);
+  if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, int(blocks), out, 0, kAdjTileMaxRot);
 }
 
 void launch_block_sums(hipStream_t s, const float2 *a, uint64_t dim, double *sums) {
@@ -1557,6 +1780,19 @@ unsigned pauli_batched_blocks(uint64_t dim) { return grid_for(dim / 2, kRedBlock
 
 unsigned adj_batched_blocks(const Layout &lay) { return grid_for(lay.nthreads, kAdjBlocks); }
 
+unsigned adj_tile_batched_blocks(const Layout &lay) {
+  return unsigned(std::min<uint64_t>(lay.nthreads, kAdjTileBlocks));
+}
+
+void launch_adj_tile_batched(hipStream_t s, const AdjTileArgs &a, double *partials, double *out,
+                             uint64_t ostride, uint32_t count) {
+  const unsigned blocks = adj_tile_batched_blocks(a.lay);
+  k_adj_tile<<<dim3(blocks, count), kAdjTileThreads, 0, s>>>(a, partials);
+  if (a.nrot)
+    k_finish_rot<<<dim3(a.nrot, count), kT, 0, s>>>(partials, int(blocks), out, ostride,
+                                                   kAdjTileMaxRot);
+}
+
 void launch_pauli_batched(hipStream_t s, const float2 *bra, const float2 *ket, uint64_t dim,
                           uint64_t flip, uint64_t sign, double *partials, double *out,
                           uint32_t count) {
@@ -1570,7 +1806,9 @@ void launch_adj_pass_batched(hipStream_t s, const AdjPassArgs &a, double *partia
   const unsigned blocks = adj_batched_blocks(a.lay);
   const dim3 grid(blocks, count);
   adj_pass_nl<4>(grid, s, a, partials);
-  if (a.nrot) k_finish_rot<<<dim3(a.nrot, count), kT, 0, s>>>(partials, int(blocks), out, ostride);
+  if (a.nrot)
+    k_finish_rot<<<dim3(a.nrot, count), kT, 0, s>>>(partials, int(blocks), out, ostride,
+                                                   kAdjMaxRot);
 }
 
 void launch_batched_results(hipStream_t s, const double *eslots, const double *rslots,

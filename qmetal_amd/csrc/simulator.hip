@@ -1197,11 +1197,25 @@ struct Simulator::Impl {
   // One merged backward pass: inverse gates (indices into the circuit, in
   // execution order) whose targets fit wire 0 plus `reg` (<= kAdjMaxReg wires), with
   // `nrot` rotations, the first at sweep index rot0.
+  // A unit of a backward tile pass: gate g, or (len 3) the sandwich
+  // CX(a,b) RZ_b CX(a,b), undone as the diagonal exp(i t/2 Z_a Z_b) whose
+  // overlap <l|Z_a Z_b|psi> is the RZ's (CX Z_b CX = Z_a Z_b).
+  struct AUnit {
+    uint32_t g;
+    uint32_t len;
+  };
   struct APass {
     std::vector<size_t> gates;
     std::vector<uint32_t> reg;
     uint32_t nrot = 0;
     size_t rot0 = 0;
+    // Tile pass (k_adj_tile) when set: units in sweep order, each segment's
+    // register wires (<= 3) and the end of its units, the five high wires.
+    bool tile = false;
+    std::vector<AUnit> units;
+    std::vector<std::vector<uint32_t>> seg_reg;
+    std::vector<size_t> seg_end;
+    uint32_t hpos[5] = {};
   };
 
   // Backward sweep in merged passes: inverse gates whose target wires fit one
@@ -1214,6 +1228,7 @@ struct Simulator::Impl {
   // gate-by-gate sweep would see (U'P U'^dagger = P). rot_index gets each
   // rotation's sweep index.
   static std::vector<APass> adjoint_schedule(const Circuit &c, std::vector<size_t> &rot_index) {
+    if (c.n >= kAdjTileMinQubits) return adjoint_tile_schedule(c, rot_index);
     const size_t G = c.gates.size();
     auto targets = [](const Gate &g) {
       std::vector<uint32_t> t;
@@ -1302,11 +1317,391 @@ struct Simulator::Impl {
     return passes;
   }
 
-  // Staging bytes of a backward pass's AdjOps (256-aligned).
+  // Units of the tile sweep: every gate, except that CX(a,b) RZ_b CX(a,b)
+  // (three consecutive gates) is one diagonal unit.
+  static std::vector<AUnit> adjoint_units(const Circuit &c) {
+    std::vector<AUnit> u;
+    const size_t G = c.gates.size();
+    for (size_t k = 0; k < G;) {
+      const Gate &g = c.gates[k];
+      if (g.kind == K_CX && k + 2 < G) {
+        const Gate &r = c.gates[k + 1], &h = c.gates[k + 2];
+        if (r.kind == K_RZ && r.q[0] == g.q[1] && h.kind == K_CX && h.q[0] == g.q[0] &&
+            h.q[1] == g.q[1]) {
+          u.push_back(AUnit{uint32_t(k), 3});
+          k += 3;
+          continue;
+        }
+      }
+      u.push_back(AUnit{uint32_t(k), 1});
+      k++;
+    }
+    return u;
+  }
+  static bool unit_diagonal(const Circuit &c, const AUnit &u) {
+    return u.len == 3 || is_diagonal(c.gates[u.g]);
+  }
+  // Wires that must be tile-local when the unit is undone (none for a
+  // diagonal unit, whose phase and Z-type overlap read index bits anywhere).
+  static std::vector<uint32_t> unit_targets(const Circuit &c, const AUnit &u) {
+    if (unit_diagonal(c, u)) return {};
+    const Gate &g = c.gates[u.g];
+    if (g.nq == 1) return {g.q[0]};
+    if (g.kind == K_SWAP) return {g.q[0], g.q[1]};
+    return {g.q[g.nq - 1]};
+  }
+  // Gate whose overlap the unit carries, or -1.
+  static int64_t unit_rotation(const Circuit &c, const AUnit &u) {
+    if (u.len == 3) return int64_t(u.g) + 1;
+    return is_rotation(c.gates[u.g].kind) ? int64_t(u.g) : -1;
+  }
+
+  // Backward sweep in tile passes (k_adj_tile, states of kAdjTileMinQubits
+  // or more): units from the dependency graph of the reversed circuit (two
+  // units keep their order when their wires overlap, unless both are
+  // diagonal), each pass a tile over wires 0..6 and five more, in segments of
+  // three register wires. Greedily: the ready unit adding the fewest register
+  // wires to the current segment (then the earliest) whose targets fit the
+  // tile (wires >= 7 join the tile while fewer than five have); a new segment
+  // when none fits; the pass ends when a new segment would not help, at
+  // kAdjTileMaxSeg segments or kAdjTileMaxRot rotations.
+  static std::vector<APass> adjoint_tile_schedule(const Circuit &c,
+                                                  std::vector<size_t> &rot_index) {
+    const std::vector<AUnit> units = adjoint_units(c);
+    const size_t U = units.size();
+    std::vector<std::vector<uint32_t>> succ(U);
+    std::vector<uint32_t> ndeps(U, 0);
+    {
+      std::vector<int64_t> last(64, -1);
+      std::vector<std::vector<uint32_t>> diags(64);
+      std::vector<uint32_t> deps;
+      for (size_t u = 0; u < U; u++) {
+        const AUnit &un = units[U - 1 - u];
+        const Gate &g = c.gates[un.g];
+        const bool dg = unit_diagonal(c, un);
+        const uint32_t nw = un.len == 3 ? 2u : g.nq;
+        deps.clear();
+        for (uint32_t j = 0; j < nw; j++) {
+          const uint32_t q = g.q[j];
+          if (last[q] >= 0) deps.push_back(uint32_t(last[q]));
+          if (!dg) deps.insert(deps.end(), diags[q].begin(), diags[q].end());
+        }
+        std::sort(deps.begin(), deps.end());
+        deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
+        for (uint32_t d : deps) succ[d].push_back(uint32_t(u));
+        ndeps[u] = uint32_t(deps.size());
+        for (uint32_t j = 0; j < nw; j++) {
+          const uint32_t q = g.q[j];
+          if (dg) {
+            diags[q].push_back(uint32_t(u));
+          } else {
+            last[q] = int64_t(u);
+            diags[q].clear();
+          }
+        }
+      }
+    }
+    std::vector<std::vector<uint32_t>> tg(U);
+    std::vector<uint32_t> isrot(U);
+    for (size_t u = 0; u < U; u++) {
+      tg[u] = unit_targets(c, units[U - 1 - u]);
+      isrot[u] = unit_rotation(c, units[U - 1 - u]) >= 0 ? 1u : 0u;
+    }
+    std::vector<uint32_t> ready;
+    for (uint32_t u = 0; u < U; u++)
+      if (!ndeps[u]) ready.push_back(u);
+    std::vector<APass> passes;
+    rot_index.assign(c.gates.size(), 0);
+    size_t sweep_rot = 0, done = 0;
+    const uint32_t n = c.n;
+    while (done < U) {
+      APass cur;
+      cur.tile = true;
+      cur.rot0 = sweep_rot;
+      std::vector<uint32_t> hw;  // high wires taken so far
+      cur.seg_reg.emplace_back();
+      for (;;) {
+        int best = -1;
+        size_t best_j = 0, best_add = 0;
+        for (size_t j = 0; j < ready.size(); j++) {
+          const uint32_t u = ready[j];
+          if (cur.nrot + isrot[u] > uint32_t(kAdjTileMaxRot)) continue;
+          size_t newh = 0, add = 0;
+          const std::vector<uint32_t> &sr = cur.seg_reg.back();
+          for (size_t i = 0; i < tg[u].size(); i++) {
+            const uint32_t q = tg[u][i];
+            if (q >= 7 && std::find(hw.begin(), hw.end(), q) == hw.end() &&
+                std::find(tg[u].begin(), tg[u].begin() + long(i), q) == tg[u].begin() + long(i))
+              newh++;
+            if (q != 0 && std::find(sr.begin(), sr.end(), q) == sr.end()) add++;
+          }
+          if (hw.size() + newh > 5 || sr.size() + add > 3) continue;
+          if (best < 0 || add < best_add || (add == best_add && u < uint32_t(best))) {
+            best = int(u);
+            best_j = j;
+            best_add = add;
+          }
+        }
+        if (best < 0) {
+          if (cur.seg_reg.size() < size_t(kAdjTileMaxSeg) && !cur.seg_reg.back().empty()) {
+            cur.seg_end.push_back(cur.units.size());
+            cur.seg_reg.emplace_back();
+            continue;
+          }
+          break;
+        }
+        const AUnit &un = units[U - 1 - size_t(best)];
+        for (uint32_t q : tg[best]) {
+          if (q >= 7 && std::find(hw.begin(), hw.end(), q) == hw.end()) hw.push_back(q);
+          std::vector<uint32_t> &sr = cur.seg_reg.back();
+          if (q != 0 && std::find(sr.begin(), sr.end(), q) == sr.end()) sr.push_back(q);
+        }
+        const int64_t rg = unit_rotation(c, un);
+        if (rg >= 0) {
+          rot_index[size_t(rg)] = sweep_rot++;
+          cur.nrot++;
+        }
+        cur.units.push_back(un);
+        ready.erase(ready.begin() + long(best_j));
+        for (uint32_t t : succ[size_t(best)])
+          if (--ndeps[t] == 0) ready.push_back(t);
+        done++;
+      }
+      if (cur.units.empty()) fail("internal: adjoint tile schedule");
+      // The last segment ends with the pass (an empty trailing one is dropped).
+      if (!cur.seg_end.empty() && cur.seg_end.back() == cur.units.size()) cur.seg_reg.pop_back();
+      else cur.seg_end.push_back(cur.units.size());
+      if (cur.seg_end.size() != cur.seg_reg.size()) fail("internal: adjoint tile segments");
+      // Fill the tile to five high wires with the lowest unused ones.
+      for (uint32_t q = 7; hw.size() < 5 && q < n; q++)
+        if (std::find(hw.begin(), hw.end(), q) == hw.end()) hw.push_back(q);
+      if (hw.size() != 5) fail("internal: adjoint tile wires");
+      std::sort(hw.begin(), hw.end());
+      for (int k = 0; k < 5; k++) cur.hpos[k] = hw[size_t(k)];
+      passes.push_back(std::move(cur));
+    }
+    return passes;
+  }
+
+  // Staging bytes of a backward pass (256-aligned): AdjOps, or for a tile
+  // pass its segments then at most three AdjOps per unit.
   static size_t adj_pass_bytes(const Circuit &c, const APass &ps) {
+    if (ps.tile) {
+      const size_t segb = (ps.seg_reg.size() * sizeof(AdjSeg) + 255) & ~size_t(255);
+      return segb + ((3 * ps.units.size() * sizeof(AdjOp) + 255) & ~size_t(255));
+    }
     size_t nops = 0;
     for (size_t k : ps.gates) nops += c.gates[k].kind == K_SWAP ? 3 : 1;
     return (nops * sizeof(AdjOp) + 255) & ~size_t(255);
+  }
+
+  // Segments and AdjOps of tile pass `ps` into host block `dst`, whose
+  // device copy is `dev`; sets every field of `a` but the states. Returns
+  // the number of AdjOps.
+  size_t emit_adj_tile(const Circuit &c, const APass &ps, uint8_t *dst, const uint8_t *dev,
+                       AdjTileArgs &a) const {
+    const size_t nseg = ps.seg_reg.size();
+    const size_t segb = (nseg * sizeof(AdjSeg) + 255) & ~size_t(255);
+    AdjSeg *sg = reinterpret_cast<AdjSeg *>(dst);
+    AdjOp *ops = reinterpret_cast<AdjOp *>(dst + segb);
+    a = AdjTileArgs{};
+    // Tile bit b: wire b (b <= 6), hpos[b - 7] (b >= 7).
+    uint32_t tw[12];
+    for (uint32_t b = 0; b < 7; b++) tw[b] = b;
+    for (int k = 0; k < 5; k++) {
+      tw[7 + k] = ps.hpos[k];
+      a.hpos[k] = uint8_t(ps.hpos[k]);
+    }
+    auto tile_bit = [&](uint32_t q) {
+      for (uint32_t b = 0; b < 12; b++)
+        if (tw[b] == q) return int(b);
+      return -1;
+    };
+    size_t no = 0, u0 = 0;
+    uint32_t rs = 0;
+    const auto bit = [](uint32_t q) { return uint64_t(1) << q; };
+    for (size_t si = 0; si < nseg; si++) {
+      // Register tile bits: the segment's wires, then the lowest unused.
+      uint32_t rb[3];
+      int nr = 0;
+      uint32_t used = 1u;
+      for (uint32_t q : ps.seg_reg[si]) {
+        const int b = tile_bit(q);
+        if (b < 1 || nr == 3) fail("internal: adjoint tile register wire");
+        rb[nr++] = uint32_t(b);
+        used |= 1u << b;
+      }
+      for (uint32_t b = 1; nr < 3; b++)
+        if (!((used >> b) & 1)) {
+          rb[nr++] = b;
+          used |= 1u << b;
+        }
+      AdjSeg &g = sg[si];
+      g = AdjSeg{};
+      int nt = 0;
+      for (uint32_t b = 1; b < 12; b++)
+        if (!((used >> b) & 1)) {
+          g.tslot[nt] = tile_swizzle(1u << (b - 1));
+          g.tpos[nt] = uint8_t(tw[b]);
+          nt++;
+        }
+      if (nt != 8) fail("internal: adjoint tile thread bits");
+      for (int k = 0; k < 3; k++) g.rslot[k] = tile_swizzle(1u << (rb[k] - 1));
+      auto local_of = [&](uint32_t q) {
+        if (q == 0) return 0;
+        for (int k = 0; k < 3; k++)
+          if (tw[rb[k]] == q) return k + 1;
+        return -1;
+      };
+      auto op = [&](int lt, const cd *m, const cd *malt, uint32_t cm, uint64_t gmask,
+                    uint64_t pmask) -> AdjOp & {
+        if (lt < 0) fail("internal: adjoint tile target not local");
+        AdjOp &o = ops[no++];
+        o = AdjOp{};
+        o.lbit = uint32_t(lt);
+        o.gmask = gmask;
+        o.pmask = pmask;
+        narrow(m, o.m);
+        narrow(malt ? malt : m, o.malt);
+        bool real = true;
+        for (int i = 1; i < 8; i += 2) real = real && o.m[i] == 0.0f && o.malt[i] == 0.0f;
+        o.sel = o.lbit + 8u * slot_kind(o.m, real, cm != 0, gmask != 0 || pmask != 0);
+        uint32_t pidx = 0;
+        for (uint32_t l = 0; l < 16u; l++) {
+          if ((l >> o.lbit) & 1) continue;
+          if ((l & cm) == cm) o.pm |= 1u << pidx;
+          pidx++;
+        }
+        return o;
+      };
+      auto overlap = [&](AdjOp &o, uint32_t axis, uint32_t lb, uint32_t lb2) {
+        o.type = 1;
+        o.axis = axis;
+        o.lbit = lb == 99 ? o.lbit : lb;
+        o.lbit2 = lb2;
+        o.rslot = rs++;
+      };
+      g.op0 = uint32_t(no);
+      for (size_t ui = u0; ui < ps.seg_end[si]; ui++) {
+        const AUnit &un = ps.units[ui];
+        const Gate &gt = c.gates[un.g];
+        if (un.len == 3) {
+          const uint32_t qa = gt.q[0], qb = gt.q[1];
+          cd r[4];
+          const double p[3] = {-c.gates[un.g + 1].p[0], 0.0, 0.0};
+          gate_matrix(K_RZ, p, r);
+          const cd d0 = r[0], d1 = r[3];
+          const int la = local_of(qa), lb = local_of(qb);
+          if (la < 0 && lb < 0) {
+            const cd m[4] = {d0, 0.0, 0.0, d0}, ma[4] = {d1, 0.0, 0.0, d1};
+            overlap(op(0, m, ma, 0, 0, bit(qa) | bit(qb)), 3, 99, 0);
+          } else if (la < 0 || lb < 0) {
+            const int l = la < 0 ? lb : la;
+            const uint32_t other = la < 0 ? qa : qb;
+            const cd m[4] = {d0, 0.0, 0.0, d1}, ma[4] = {d1, 0.0, 0.0, d0};
+            overlap(op(l, m, ma, 0, 0, bit(other)), 2, 99, 0);
+          } else {
+            const cd m[4] = {d0, 0.0, 0.0, d1};
+            overlap(op(lb, m, nullptr, 0, 0, 0), 4, uint32_t(std::min(la, lb)),
+                    uint32_t(std::max(la, lb)));
+            const cd m2[4] = {d1 * std::conj(d0), 0.0, 0.0, d0 * std::conj(d1)};
+            op(lb, m2, nullptr, 1u << la, 0, 0);
+          }
+          continue;
+        }
+        const Gate inv = inverse(gt);
+        cd m[4];
+        if (gt.nq == 1 && is_diagonal(gt)) {
+          gate_matrix(inv.kind, inv.p, m);
+          const int l = local_of(gt.q[0]);
+          const bool rz = gt.kind == K_RZ;
+          if (l >= 0) {
+            AdjOp &o = op(l, m, nullptr, 0, 0, 0);
+            if (rz) overlap(o, 2, 99, 0);
+          } else {
+            const cd mm[4] = {m[0], 0.0, 0.0, m[0]}, ma[4] = {m[3], 0.0, 0.0, m[3]};
+            AdjOp &o = op(0, mm, ma, 0, 0, bit(gt.q[0]));
+            if (rz) overlap(o, 3, 99, 0);
+          }
+          continue;
+        }
+        if (gt.kind == K_CZ || gt.kind == K_CP) {
+          const cd ph = inv.kind == K_CZ ? cd(-1, 0) : cd(std::cos(inv.p[0]), std::sin(inv.p[0]));
+          const uint32_t qa = gt.q[0], qb = gt.q[1];
+          const int la = local_of(qa), lb = local_of(qb);
+          const cd mm[4] = {1.0, 0.0, 0.0, ph};
+          if (lb >= 0) op(lb, mm, nullptr, la >= 0 ? 1u << la : 0u, la >= 0 ? 0 : bit(qa), 0);
+          else if (la >= 0) op(la, mm, nullptr, 0, bit(qb), 0);
+          else {
+            const cd md[4] = {ph, 0.0, 0.0, ph};
+            op(0, md, nullptr, 0, bit(qa) | bit(qb), 0);
+          }
+          continue;
+        }
+        // Other gates: targets are tile-local; controls local or base bits.
+        auto ctrl = [&](std::initializer_list<uint32_t> qs, uint32_t &cm, uint64_t &gm) {
+          cm = 0;
+          gm = 0;
+          for (uint32_t q : qs) {
+            const int l = local_of(q);
+            if (l >= 0) cm |= 1u << l;
+            else gm |= bit(q);
+          }
+        };
+        uint32_t cm = 0;
+        uint64_t gm = 0;
+        if (gt.nq == 1) {
+          gate_matrix(inv.kind, inv.p, m);
+          AdjOp &o = op(local_of(gt.q[0]), m, nullptr, 0, 0, 0);
+          if (is_rotation(gt.kind)) overlap(o, gt.kind == K_RX ? 0 : 1, 99, 0);
+          continue;
+        }
+        switch (gt.kind) {
+          case K_CX:
+          case K_CY:
+            gate_matrix(gt.kind == K_CX ? K_X : K_Y, nullptr, m);
+            ctrl({gt.q[0]}, cm, gm);
+            op(local_of(gt.q[1]), m, nullptr, cm, gm, 0);
+            break;
+          case K_SWAP: {
+            gate_matrix(K_X, nullptr, m);
+            ctrl({gt.q[0]}, cm, gm);
+            op(local_of(gt.q[1]), m, nullptr, cm, gm, 0);
+            ctrl({gt.q[1]}, cm, gm);
+            op(local_of(gt.q[0]), m, nullptr, cm, gm, 0);
+            ctrl({gt.q[0]}, cm, gm);
+            op(local_of(gt.q[1]), m, nullptr, cm, gm, 0);
+            break;
+          }
+          case K_CCX:
+            gate_matrix(K_X, nullptr, m);
+            ctrl({gt.q[0], gt.q[1]}, cm, gm);
+            op(local_of(gt.q[2]), m, nullptr, cm, gm, 0);
+            break;
+          default:
+            fail(std::string("unsupported gate: ") + kind_name(gt.kind));
+        }
+      }
+      g.op1 = uint32_t(no);
+      u0 = ps.seg_end[si];
+    }
+    if (no > 3 * ps.units.size() || rs != ps.nrot) fail("internal: adjoint tile op count");
+    Layout &L = a.lay;
+    uint32_t ins[12];
+    for (uint32_t b = 0; b < 12; b++) ins[b] = tw[b];
+    std::sort(ins, ins + 12);
+    for (uint32_t b = 0; b < 12; b++) {
+      L.ins[L.nins] = uint8_t(ins[b]);
+      L.hm[L.nins] = ~((uint64_t(1) << ins[b]) - 1);
+      L.nins++;
+    }
+    L.nthreads = dim >> 12;
+    L.streaming = 1;
+    a.segs = reinterpret_cast<const AdjSeg *>(dev);
+    a.ops = reinterpret_cast<const AdjOp *>(dev + segb);
+    a.nsegs = uint32_t(nseg);
+    a.nrot = ps.nrot;
+    return no;
   }
 
   // AdjOps of backward pass `ps` into `ops`; sets a.lay, a.nops, a.nrot.
@@ -2219,7 +2614,15 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
     }
     s->stage(total);
     std::vector<AdjPassArgs> args(passes.size());
+    std::vector<AdjTileArgs> targs(passes.size());
     for (size_t i = 0; i < passes.size(); i++) {
+      if (passes[i].tile) {
+        AdjTileArgs &t = targs[i];
+        s->emit_adj_tile(c, passes[i], s->staging + offs[i], s->arena + offs[i], t);
+        t.psi = s->state;
+        t.lam = s->costate;
+        continue;
+      }
       AdjPassArgs &a = args[i];
       s->emit_adj_pass(c, passes[i], reinterpret_cast<AdjOp *>(s->staging + offs[i]), a);
       a.psi = s->state;
@@ -2227,12 +2630,16 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       a.ops = reinterpret_cast<const AdjOp *>(s->arena + offs[i]);
     }
     s->upload(total);
-    s->grow(s->adj_partials, s->adj_partials_cap, size_t(kAdjMaxRot) * kAdjBlocks * 2,
+    s->grow(s->adj_partials, s->adj_partials_cap,
+            std::max(size_t(kAdjMaxRot) * kAdjBlocks, size_t(kAdjTileMaxRot) * kAdjTileBlocks) * 2,
             "hipMalloc(adjoint partials)");
     double *rot_slots = slots + 2 * (G + 1);
     for (size_t i = 0; i < passes.size(); i++) {
       s->timed(kTimingAdjPass, 4 * amp_bytes, [&] {
-        launch_adj_pass(s->stream, args[i], s->adj_partials, rot_slots + 2 * passes[i].rot0);
+        if (passes[i].tile)
+          launch_adj_tile(s->stream, targs[i], s->adj_partials, rot_slots + 2 * passes[i].rot0);
+        else
+          launch_adj_pass(s->stream, args[i], s->adj_partials, rot_slots + 2 * passes[i].rot0);
       });
       s->launched();
       s->dispatches++;
@@ -2409,7 +2816,13 @@ struct BatchedAdjoint::Data {
   uint32_t *rgate = nullptr, *inst = nullptr, *gbase = nullptr;
   std::vector<BatchedPass> fwd;
   std::vector<BatchedAccum> costate;
-  std::vector<AdjPassArgs> bwd;
+  struct Bwd {
+    bool tile = false;
+    AdjPassArgs a{};
+    AdjTileArgs t{};
+    size_t nops = 0;
+  };
+  std::vector<Bwd> bwd;
   std::vector<size_t> bwd_rot0;
 
   ~Data() {
@@ -2450,9 +2863,11 @@ void BatchedAdjoint::enqueue(void *stream, double *energy_out, double *grad_out)
     else launch_pauli_accum_batched(st, b.g, d.count);
   }
   launch_pauli_batched(st, d.psi, d.lam, d.dim, 0, 0, d.red, d.eslots, d.count);
-  for (size_t j = 0; j < d.bwd.size(); j++)
-    launch_adj_pass_batched(st, d.bwd[j], d.adj, d.rslots + 2 * d.bwd_rot0[j], 2ull * d.nrot,
-                            d.count);
+  for (size_t j = 0; j < d.bwd.size(); j++) {
+    double *out = d.rslots + 2 * d.bwd_rot0[j];
+    if (d.bwd[j].tile) launch_adj_tile_batched(st, d.bwd[j].t, d.adj, out, 2ull * d.nrot, d.count);
+    else launch_adj_pass_batched(st, d.bwd[j].a, d.adj, out, 2ull * d.nrot, d.count);
+  }
   launch_batched_results(st, d.eslots, d.rslots, d.count, d.nrot, d.rgate, d.inst, d.gbase,
                          energy_out, grad_out);
   check(hipGetLastError(), "batched adjoint launch");
@@ -2565,19 +2980,30 @@ std::unique_ptr<BatchedAdjoint> Simulator::compile_batched(
         else if (same) same = same_layout(pl.a.lay, t.a.lay) && pl.a.nops == t.a.nops;
         if (!same) return nullptr;
       }
-      std::vector<AdjPassArgs> bw(apasses.size());
       for (size_t j = 0; j < apasses.size(); j++) {
-        AdjPassArgs a{};
-        s->emit_adj_pass(*circuits[i], apasses[j],
-                         reinterpret_cast<AdjOp *>(host.data() + i * stride + boff[j]), a);
-        if (i == 0) {
-          a.ops = reinterpret_cast<const AdjOp *>(d.arena + boff[j]);
-          d.bwd.push_back(a);
-          d.bwd_rot0.push_back(apasses[j].rot0);
-        } else if (!same_layout(a.lay, d.bwd[j].lay) || a.nops != d.bwd[j].nops ||
-                   a.nrot != d.bwd[j].nrot) {
-          return nullptr;
+        BatchedAdjoint::Data::Bwd b;
+        b.tile = apasses[j].tile;
+        if (b.tile) {
+          b.nops = s->emit_adj_tile(*circuits[i], apasses[j], host.data() + i * stride + boff[j],
+                                    d.arena + boff[j], b.t);
+        } else {
+          s->emit_adj_pass(*circuits[i], apasses[j],
+                           reinterpret_cast<AdjOp *>(host.data() + i * stride + boff[j]), b.a);
+          b.a.ops = reinterpret_cast<const AdjOp *>(d.arena + boff[j]);
+          b.nops = b.a.nops;
         }
+        if (i == 0) {
+          d.bwd.push_back(b);
+          d.bwd_rot0.push_back(apasses[j].rot0);
+          continue;
+        }
+        const BatchedAdjoint::Data::Bwd &t = d.bwd[j];
+        const bool same = b.tile ? same_layout(b.t.lay, t.t.lay) && b.t.nsegs == t.t.nsegs &&
+                                       b.t.nrot == t.t.nrot && b.nops == t.nops &&
+                                       std::memcmp(b.t.hpos, t.t.hpos, sizeof(b.t.hpos)) == 0
+                                 : same_layout(b.a.lay, t.a.lay) && b.nops == t.nops &&
+                                       b.a.nrot == t.a.nrot;
+        if (!same) return nullptr;
       }
     }
     for (size_t k = 0; k < passes.size(); k++) {
@@ -2605,11 +3031,11 @@ std::unique_ptr<BatchedAdjoint> Simulator::compile_batched(
     b.a.bstate = b.t.bstate = b.g.bstate = d.dim;
     b.a.barena = b.t.barena = b.g.barena = stride;
   }
-  for (AdjPassArgs &a : d.bwd) {
-    a.psi = d.psi;
-    a.lam = d.lam;
-    a.bstate = d.dim;
-    a.barena = stride;
+  for (BatchedAdjoint::Data::Bwd &b : d.bwd) {
+    b.a.psi = b.t.psi = d.psi;
+    b.a.lam = b.t.lam = d.lam;
+    b.a.bstate = b.t.bstate = d.dim;
+    b.a.barena = b.t.barena = stride;
   }
   for (Impl::AccumLaunch &L : Impl::costate_launches(h)) {
     BatchedAccum b;
@@ -2637,10 +3063,11 @@ std::unique_ptr<BatchedAdjoint> Simulator::compile_batched(
   check(hipMalloc(reinterpret_cast<void **>(&d.red),
                   size_t(m) * pauli_batched_blocks(d.dim) * 2 * sizeof(double)),
         "hipMalloc(batched partials)");
-  unsigned ablocks = 1;
-  for (const AdjPassArgs &a : d.bwd) ablocks = std::max(ablocks, adj_batched_blocks(a.lay));
-  check(hipMalloc(reinterpret_cast<void **>(&d.adj),
-                  size_t(m) * kAdjMaxRot * ablocks * 2 * sizeof(double)),
+  size_t aparts = 1;
+  for (const BatchedAdjoint::Data::Bwd &b : d.bwd)
+    aparts = std::max(aparts, b.tile ? size_t(kAdjTileMaxRot) * adj_tile_batched_blocks(b.t.lay)
+                                     : size_t(kAdjMaxRot) * adj_batched_blocks(b.a.lay));
+  check(hipMalloc(reinterpret_cast<void **>(&d.adj), size_t(m) * aparts * 2 * sizeof(double)),
         "hipMalloc(batched adjoint partials)");
   check(hipMalloc(reinterpret_cast<void **>(&d.eslots), size_t(m) * 2 * sizeof(double)),
         "hipMalloc(batched energies)");

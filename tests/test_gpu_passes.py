@@ -296,6 +296,69 @@ def test_merged_adjoint_against_oracle_and_per_gate_sweep():
                     assert g_m[k] == 0.0
 
 
+def tile_adjoint_stream(n, count, seed):
+    """adjoint_stream's kinds plus CX(a,b) RZ_b CX(a,b) sandwiches (undone as
+    one diagonal unit by the tile sweep), on neighbouring and random pairs,
+    so their wires land on register, lane, wave, high-tile and base bits."""
+    r = C.SplitMix64(seed)
+    base = adjoint_stream(n, count, seed)
+    out = []
+    for g in base:
+        out.append(g)
+        if r.next_u64() % 3 == 0:
+            a = r.next_u64() % n
+            b = (a + 1) % n if r.next_u64() % 2 else (a + 1 + r.next_u64() % (n - 1)) % n
+            out += C.zz(a, b, r.uniform(-3.2, 3.2))
+    return out
+
+
+@pytest.mark.parametrize("n", (12, 13, 16))
+def test_tile_adjoint_against_oracle_and_per_gate_sweep(n):
+    """Backward sweep in LDS tile passes (twelve wires per pass, segments of
+    three register wires, diagonal units anywhere) against the complex128
+    adjoint and the one-launch-per-gate sweep."""
+    terms = C.tfim_ring_hamiltonian(n) + [(0.3, 0, 1 << (n // 2), 0), (0.2, 1 << (n - 1), 0, 1)]
+    h = C.to_hamiltonian(terms)
+    for seed in range(2):
+        gates = tile_adjoint_stream(n, 90, 31 * n + seed)
+        c = C.to_circuit(n, gates)
+        with qm.Simulator(n) as s:
+            e_m, g_m = s.energy_and_gradient(c, h)
+            s.set_pass_merging(False)
+            e_p, g_p = s.energy_and_gradient(c, h)
+        e_ref, g_ref = O.adjoint(n, gates, terms)
+        d_ref = float(np.abs(np.array(g_m) - g_ref).max())
+        d_per = float(np.abs(np.array(g_m) - np.array(g_p)).max())
+        print(f"n={n} seed={seed} gates={len(gates)}: |tile - oracle| {d_ref:.2e}, "
+              f"|tile - per-gate| {d_per:.2e}")
+        assert abs(e_m - e_ref) <= 1e-5 and abs(e_m - e_p) <= 1e-5
+        assert d_ref <= 1e-5 and d_per <= 1e-5
+        for k, (kind, _, _) in enumerate(gates):
+            if kind not in (C.RX, C.RY, C.RZ):
+                assert g_m[k] == 0.0
+
+
+@pytest.mark.parametrize("name,n", (("qaoa", 14), ("qaoa", 19), ("hea", 15), ("hea", 20)))
+def test_tile_adjoint_structured_circuits(name, n):
+    """QAOA ring (its ZZ sandwiches run as diagonal units) and HEA (CX ladder)
+    through the tile sweep, against the complex128 adjoint."""
+    r = C.SplitMix64(7 * n)
+    if name == "qaoa":
+        gates = C.qaoa_ring(n, 4, [r.uniform(0, 1) for _ in range(4)],
+                            [r.uniform(0, 1) for _ in range(4)])
+        terms = [(1.0, 0, 0, (1 << q) | (1 << ((q + 1) % n))) for q in range(n)]
+    else:
+        gates = C.hardware_efficient(n, 3, seed=0x51 + n)
+        terms = C.tfim_ring_hamiltonian(n)
+    with qm.Simulator(n) as s:
+        e, g = s.energy_and_gradient(C.to_circuit(n, gates), C.to_hamiltonian(terms))
+    e_ref, g_ref = O.adjoint(n, gates, terms)
+    d = float(np.abs(np.array(g) - g_ref).max())
+    print(f"{name} n={n}: energy {e:.6f} (oracle {e_ref:.6f}), max |grad diff| {d:.2e}")
+    assert abs(e - e_ref) <= 1e-5 * max(1.0, abs(e_ref))
+    assert d <= 1e-5
+
+
 def step_gate(r, w, kind):
     """A chain step's 2x2: H ("h": the butterfly mode), RY ("real": a real
     matrix), U3 ("cplx")."""
