@@ -440,7 +440,7 @@ struct AdjTileArgs {
   const AdjOp *ops;
   uint32_t nsegs;
   uint32_t nrot;
-  uint8_t hpos[5];
+  uint32_t hpos[5];
   uint64_t bstate;  // batched launches, as in PassArgs
   uint64_t barena;
 };

@@ -1143,7 +1143,8 @@ __device__ __forceinline__ void tile_overlap_parity(const float2 (&x)[16], const
                                                     double &re, double &im) {
 #pragma unroll
   for (int l = 0; l < 16; l++) {
-    const int par = (LA >= 0 ? (l >> LA) & 1 : 0) ^ (LB >= 0 ? (l >> LB) & 1 : 0);
+    const int par = (LA >= 0 ? (l >> (LA < 0 ? 0 : LA)) & 1 : 0) ^
+                    (LB >= 0 ? (l >> (LB < 0 ? 0 : LB)) & 1 : 0);
     if (par) acc_cj_fma<true>(re, im, y[l], x[l]);
     else acc_cj_fma<false>(re, im, y[l], x[l]);
   }
@@ -1196,11 +1197,86 @@ __device__ __forceinline__ void tile_permute16(float4 *lds, float2 (&x)[16], uin
   }
 }
 
+// Wavefront sum from the device library (a fixed sequence of cross-lane
+// steps: the same order on every run).
+extern "C" __device__ __attribute__((const)) double __ockl_wfred_add_f64(double);
+
+// AdjOps [o0, o1) on the thread's 16 amplitudes of each state. Each op's
+// fields are loaded as one block into scalar registers (a lane-dependent
+// choice between two fields read from memory would otherwise become a
+// per-lane load; loading the next op's block ahead spills scalar registers). Overlaps go to acc[rslot][wave][re, im]
+// (wave sums, lane 0 adds them in program order).
+__device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
+                                             const_ptr<AdjOp> ops, uint32_t o0, uint32_t o1,
+                                             uint64_t base, uint32_t lane, uint32_t wave,
+                                             double *acc) {
+  struct Hdr {
+    uint32_t type, lbit, sel, pm, axis, rslot, lbit2;
+    uint64_t gmask, pmask;
+    float m[8], malt[8];
+  };
+  auto load_hdr = [&](uint32_t o, Hdr &h) {
+    h.type = ops[o].type;
+    h.lbit = ops[o].lbit;
+    h.sel = ops[o].sel;
+    h.pm = ops[o].pm;
+    h.axis = ops[o].axis;
+    h.rslot = ops[o].rslot;
+    h.lbit2 = ops[o].lbit2;
+    h.gmask = ops[o].gmask;
+    h.pmask = ops[o].pmask;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      h.m[i] = ops[o].m[i];
+      h.malt[i] = ops[o].malt[i];
+    }
+  };
+  auto pin = [](Hdr &h) {
+    asm volatile("" : "+s"(h.type), "+s"(h.lbit), "+s"(h.sel), "+s"(h.pm), "+s"(h.axis),
+                 "+s"(h.rslot), "+s"(h.lbit2), "+s"(h.gmask), "+s"(h.pmask));
+    asm volatile("" : "+s"(h.m[0]), "+s"(h.m[1]), "+s"(h.m[2]), "+s"(h.m[3]), "+s"(h.m[4]),
+                 "+s"(h.m[5]), "+s"(h.m[6]), "+s"(h.m[7]));
+    asm volatile("" : "+s"(h.malt[0]), "+s"(h.malt[1]), "+s"(h.malt[2]), "+s"(h.malt[3]),
+                 "+s"(h.malt[4]), "+s"(h.malt[5]), "+s"(h.malt[6]), "+s"(h.malt[7]));
+  };
+  for (uint32_t o = o0; o < o1; o++) {
+    Hdr h;
+    load_hdr(o, h);
+    pin(h);
+    const bool odd = (__builtin_popcountll(base & h.pmask) & 1) != 0;
+    if (h.type == 1) {
+      double re = 0.0, im = 0.0;
+      adj_op_overlap(x, y, h.axis, h.lbit, h.lbit2, re, im);
+      re = __ockl_wfred_add_f64(odd ? -re : re);
+      im = __ockl_wfred_add_f64(odd ? -im : im);
+      if (lane == 0) {
+        acc[(h.rslot * (kAdjTileThreads / 64) + wave) * 2] += re;
+        acc[(h.rslot * (kAdjTileThreads / 64) + wave) * 2 + 1] += im;
+      }
+    }
+    float m[8];
+    if ((h.gmask | h.pmask) == 0) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) m[i] = h.m[i];
+    } else {
+      const bool gon = (base & h.gmask) == h.gmask;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float mi = odd ? h.malt[i] : h.m[i];
+        m[i] = gon ? mi : (i == 0 || i == 6 ? 1.0f : 0.0f);
+      }
+    }
+    slot_asm<4>(x, h.sel, h.pm, m);
+    slot_asm<4>(y, h.sel, h.pm, m);
+  }
+}
+
 // Backward tile pass (AdjTileArgs). Every workgroup works on whole tiles, so
 // all its threads take part in every op; overlaps are reduced as in
 // k_adj_pass (wave sums into per-wave LDS slots in program order, per-block
 // partials, a fixed-order finish).
-__global__ __launch_bounds__(kAdjTileThreads) void k_adj_tile(AdjTileArgs a, double *part) {
+__global__ __launch_bounds__(kAdjTileThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void k_adj_tile(AdjTileArgs a, double *part) {
   constexpr int W = kAdjTileThreads / 64;
   __shared__ float4 lds[2048];
   __shared__ double acc[kAdjTileMaxRot][W][2];
@@ -1249,48 +1325,32 @@ __global__ __launch_bounds__(kAdjTileThreads) void k_adj_tile(AdjTileArgs a, dou
     }
     uint32_t tf = t0, rf[3] = {r0[0], r0[1], r0[2]};
     for (uint32_t s = 0; s < a.nsegs; s++) {
-      const AdjSeg __attribute__((address_space(4))) &sg = segs[s];
+      // The segment's fields as scalars (each loaded once, whatever the
+      // lane), then combined per lane.
+      uint32_t ts[8], tp[8], rt[3];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        ts[j] = segs[s].tslot[j];
+        tp[j] = segs[s].tpos[j];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) rt[k] = segs[s].rslot[k];
+      const uint32_t op0 = segs[s].op0, op1 = segs[s].op1;
+      asm volatile("" : "+s"(ts[0]), "+s"(ts[1]), "+s"(ts[2]), "+s"(ts[3]), "+s"(ts[4]),
+                   "+s"(ts[5]), "+s"(ts[6]), "+s"(ts[7]));
+      asm volatile("" : "+s"(tp[0]), "+s"(tp[1]), "+s"(tp[2]), "+s"(tp[3]), "+s"(tp[4]),
+                   "+s"(tp[5]), "+s"(tp[6]), "+s"(tp[7]), "+s"(rt[0]), "+s"(rt[1]), "+s"(rt[2]));
       uint32_t tt = 0;
       uint64_t base = ub;
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const bool on = j < 6 ? ((lane >> j) & 1) != 0 : ((wave >> (j - 6)) & 1) != 0;
-        if (on) {
-          tt ^= sg.tslot[j];
-          base |= 1ull << sg.tpos[j];
-        }
+        tt ^= on ? ts[j] : 0u;
+        base |= on ? 1ull << tp[j] : 0ull;
       }
-      uint32_t rt[3];
-#pragma unroll
-      for (int k = 0; k < 3; k++) rt[k] = sg.rslot[k];
       tile_permute16(lds, x, tf, rf, tt, rt);
       tile_permute16(lds, y, tf, rf, tt, rt);
-      for (uint32_t o = sg.op0; o < sg.op1; o++) {
-        const bool odd = (__builtin_popcountll(base & ops[o].pmask) & 1) != 0;
-        if (ops[o].type == 1) {
-          double re = 0.0, im = 0.0;
-          adj_op_overlap(x, y, ops[o].axis, ops[o].lbit, ops[o].lbit2, re, im);
-          re = wave_allsum(odd ? -re : re);
-          im = wave_allsum(odd ? -im : im);
-          if (lane == 0) {
-            const uint32_t r = ops[o].rslot;
-            acc[r][wave][0] += re;
-            acc[r][wave][1] += im;
-          }
-        }
-        const uint64_t gmask = ops[o].gmask;
-        const bool gon = (base & gmask) == gmask;
-        float m[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-          const float mi = odd ? ops[o].malt[i] : ops[o].m[i];
-          m[i] = gon ? mi : (i == 0 || i == 6 ? 1.0f : 0.0f);
-        }
-        const uint32_t sel = __builtin_amdgcn_readfirstlane(ops[o].sel);
-        const uint32_t pm = __builtin_amdgcn_readfirstlane(ops[o].pm);
-        slot_asm<4>(x, sel, pm, m);
-        slot_asm<4>(y, sel, pm, m);
-      }
+      adj_tile_ops(x, y, ops, op0, op1, base, lane, wave, &acc[0][0][0]);
       tf = tt;
 #pragma unroll
       for (int k = 0; k < 3; k++) rf[k] = rt[k];

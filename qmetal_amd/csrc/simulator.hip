@@ -1510,7 +1510,7 @@ struct Simulator::Impl {
     for (uint32_t b = 0; b < 7; b++) tw[b] = b;
     for (int k = 0; k < 5; k++) {
       tw[7 + k] = ps.hpos[k];
-      a.hpos[k] = uint8_t(ps.hpos[k]);
+      a.hpos[k] = ps.hpos[k];
     }
     auto tile_bit = [&](uint32_t q) {
       for (uint32_t b = 0; b < 12; b++)
