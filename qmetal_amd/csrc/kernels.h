@@ -344,6 +344,48 @@ struct AccumTileArgs {
 };
 void launch_pauli_accum_tile(hipStream_t s, const AccumTileArgs &a);
 
+// Co-state pass over an LDS tile (states of kAdjTileMinQubits or more): the
+// tile is 2^12 amplitudes over wires 0..6 and hpos[0..4], loaded as by
+// k_adj_tile (16 per thread: bit 0, register bits on hpos[2..4]) and also
+// written to LDS. lam_i (+)= sum over terms of coef_t s_t(j) psi_j, j = i ^ f:
+//  - diagonal terms (f = 0; real coefficients): sorted by their local mask m
+//    (bit 0 -> 1, hpos[2 + k] -> 2 << k), [doff[m], doff[m + 1]); each thread
+//    sums the coefficients by local mask in FP64, signed by the parity of
+//    its base index over the term's mask, and a Walsh-Hadamard transform over
+//    its four local bits gives each amplitude's factor;
+//  - flip groups g < ngroups: flip gflip[g] (within the tile), terms
+//    [goff[g], goff[g + 1]), partner amplitudes read from LDS at slot offset
+//    gslot[g]; coefficients already carry the sign of the flip's own parity
+//    (s_t(j) = s_t(i) s_t(f)); `lmask` is each term's local mask.
+// `first`: lam is written without being read. With `epart`, the pass also
+// leaves per-block FP64 partials of <psi|lam> (after its update) for
+// k_finish (2 doubles per block, blocks of instance y after y * gridDim).
+constexpr int kCoTerms = 48;
+constexpr int kCoGroups = 24;
+struct CoTileArgs {
+  float2 *lam;
+  const float2 *psi;
+  Layout lay;
+  uint32_t hpos[5];
+  uint32_t first;
+  uint32_t ngroups;
+  uint32_t doff[17];
+  uint32_t goff[kCoGroups + 1];
+  uint32_t gslot[kCoGroups];
+  uint64_t gflip[kCoGroups];
+  uint64_t sign[kCoTerms];
+  uint32_t lmask[kCoTerms];
+  double cre[kCoTerms];
+  double cim[kCoTerms];
+  uint64_t bstate;  // batched launches: per-instance state stride
+};
+// Single state; with `energy` (2 doubles) the fused <psi|lam> lands there.
+void launch_costate_tile(hipStream_t s, const CoTileArgs &a, double *partials, double *energy);
+// Batched: instance y's energy to energy[2 y] (when non-null).
+unsigned costate_tile_blocks(const Layout &lay);
+void launch_costate_tile_batched(hipStream_t s, const CoTileArgs &a, double *partials,
+                                 double *energy, uint32_t count);
+
 // Adjoint step for a rotation exp(-i t P / 2), P in {X, Y, Z} on wire q:
 // accumulates <lam|P|psi> (FP64, per-block partials reduced into
 // out[2*slot], out[2*slot+1]) and then applies the inverse rotation to both

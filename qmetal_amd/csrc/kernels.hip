@@ -1374,6 +1374,141 @@ void k_adj_tile(AdjTileArgs a, double *part) {
   }
 }
 
+// Co-state tile pass (CoTileArgs).
+__global__ __launch_bounds__(kAdjTileThreads) void k_costate_tile(CoTileArgs a, double *epart) {
+  constexpr int W = kAdjTileThreads / 64;
+  __shared__ float4 lds[2048];
+  __shared__ double red[W][2];
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t inst = blockIdx.y;
+  const float2 *const psi = a.psi + inst * a.bstate;
+  float2 *const lam = a.lam + inst * a.bstate;
+  const uint32_t lane_bytes = lane * 16u;
+  const uint64_t wave_off = ((wave & 1u) ? 1ull << a.hpos[0] : 0ull) |
+                            ((wave & 2u) ? 1ull << a.hpos[1] : 0ull);
+  uint64_t reg_off[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++)
+    reg_off[r] = ((r & 1) ? 1ull << a.hpos[2] : 0ull) | ((r & 2) ? 1ull << a.hpos[3] : 0ull) |
+                 ((r & 4) ? 1ull << a.hpos[4] : 0ull);
+  // The thread's slots (load arrangement of k_adj_tile) and its base index
+  // (lane and wave bits; local bits are 0).
+  uint32_t t0 = 0;
+#pragma unroll
+  for (int j = 0; j < 6; j++)
+    if ((lane >> j) & 1) t0 ^= tile_swizzle(1u << j);
+#pragma unroll
+  for (int w = 0; w < 2; w++)
+    if ((wave >> w) & 1) t0 ^= tile_swizzle(1u << (6 + w));
+  uint32_t slot[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++)
+    slot[r] = t0 ^ ((r & 1) ? tile_swizzle(1u << 8) : 0u) ^ ((r & 2) ? tile_swizzle(1u << 9) : 0u) ^
+              ((r & 4) ? tile_swizzle(1u << 10) : 0u);
+  const uint64_t thr_bits = (uint64_t(lane) << 1) | wave_off;
+  double ere = 0.0, eim = 0.0;
+  for (uint64_t tile = blockIdx.x; tile < a.lay.nthreads; tile += gridDim.x) {
+    const uint64_t ub = scatter(tile, a.lay) | a.lay.fixed;
+    const uint64_t base = ub | thr_bits;
+    float2 x[16], y[16];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const float4 v = ld4<true>(pair_ptr(psi, ub | wave_off | reg_off[r], lane_bytes));
+      x[2 * r] = make_float2(v.x, v.y);
+      x[2 * r + 1] = make_float2(v.z, v.w);
+    }
+    if (a.first) {
+#pragma unroll
+      for (int l = 0; l < 16; l++) y[l] = make_float2(0.0f, 0.0f);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const float4 w = ld4<true>(pair_ptr(lam, ub | wave_off | reg_off[r], lane_bytes));
+        y[2 * r] = make_float2(w.x, w.y);
+        y[2 * r + 1] = make_float2(w.z, w.w);
+      }
+    }
+    __syncthreads();  // the previous tile's reads of the buffer are done
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+      lds[slot[r]] = make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
+    __syncthreads();
+    if (a.doff[16] > 0) {
+      // Coefficients by local mask, then a Walsh-Hadamard transform: factor
+      // of local combination l = sum_m w[m] (-1)^popcount(l & m).
+      double w[16];
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        double acc = 0.0;
+        for (uint32_t t = a.doff[m]; t < a.doff[m + 1]; t++)
+          acc += (__popcll(base & a.sign[t]) & 1) ? -a.cre[t] : a.cre[t];
+        w[m] = acc;
+      }
+#pragma unroll
+      for (int b = 1; b < 16; b <<= 1)
+#pragma unroll
+        for (int l = 0; l < 16; l++)
+          if (!(l & b)) {
+            const double u = w[l], v = w[l | b];
+            w[l] = u + v;
+            w[l | b] = u - v;
+          }
+#pragma unroll
+      for (int l = 0; l < 16; l++) {
+        const float f = float(w[l]);
+        y[l].x = fmaf(f, x[l].x, y[l].x);
+        y[l].y = fmaf(f, x[l].y, y[l].y);
+      }
+    }
+    for (uint32_t g = 0; g < a.ngroups; g++) {
+      const uint32_t gs = a.gslot[g];
+      const bool swap = (a.gflip[g] & 1ull) != 0;
+      float2 p[16];
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const float4 v = lds[slot[r] ^ gs];
+        p[2 * r] = swap ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+        p[2 * r + 1] = swap ? make_float2(v.x, v.y) : make_float2(v.z, v.w);
+      }
+      for (uint32_t t = a.goff[g]; t < a.goff[g + 1]; t++) {
+        const bool neg = (__popcll(base & a.sign[t]) & 1) != 0;
+        const float cr = float(a.cre[t]), ci = float(a.cim[t]);
+        const float sr = neg ? -cr : cr, si = neg ? -ci : ci;
+        const uint32_t lm = a.lmask[t];
+#pragma unroll
+        for (int l = 0; l < 16; l++) {
+          const bool fl = (__popc(uint32_t(l) & lm) & 1) != 0;
+          const float qr = fl ? -sr : sr, qi = fl ? -si : si;
+          y[l].x = fmaf(qr, p[l].x, fmaf(-qi, p[l].y, y[l].x));
+          y[l].y = fmaf(qr, p[l].y, fmaf(qi, p[l].x, y[l].y));
+        }
+      }
+    }
+    if (epart) {
+#pragma unroll
+      for (int l = 0; l < 16; l++) acc_cj_fma<false>(ere, eim, x[l], y[l]);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+      st4<true>(pair_ptr(lam, ub | wave_off | reg_off[r], lane_bytes),
+                make_float4(y[2 * r].x, y[2 * r].y, y[2 * r + 1].x, y[2 * r + 1].y));
+  }
+  if (epart) {
+    ere = __ockl_wfred_add_f64(ere);
+    eim = __ockl_wfred_add_f64(eim);
+    if (lane == 0) {
+      red[wave][0] = ere;
+      red[wave][1] = eim;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      double t = 0.0;
+      for (int w = 0; w < W; w++) t += red[w][threadIdx.x];
+      epart[(size_t(inst) * gridDim.x + blockIdx.x) * 2 + threadIdx.x] = t;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kT) void k_adjoint_rot(AdjArgs a, double *part) {
   __shared__ double sh[kT / 64];
   const uint64_t stride = uint64_t(gridDim.x) * kT;
@@ -1766,6 +1901,23 @@ void launch_adj_tile(hipStream_t s, const AdjTileArgs &a, double *partials, doub
   const unsigned blocks = unsigned(std::min<uint64_t>(a.lay.nthreads, kAdjTileBlocks));
   k_adj_tile<<<blocks, kAdjTileThreads, 0, s>>>(a, partials);
   if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, int(blocks), out, 0, kAdjTileMaxRot);
+}
+
+unsigned costate_tile_blocks(const Layout &lay) {
+  return unsigned(std::min<uint64_t>(lay.nthreads, kAdjTileBlocks));
+}
+
+void launch_costate_tile(hipStream_t s, const CoTileArgs &a, double *partials, double *energy) {
+  const unsigned blocks = costate_tile_blocks(a.lay);
+  k_costate_tile<<<blocks, kAdjTileThreads, 0, s>>>(a, energy ? partials : nullptr);
+  if (energy) k_finish<<<1, kT, 0, s>>>(partials, int(blocks), 2, energy);
+}
+
+void launch_costate_tile_batched(hipStream_t s, const CoTileArgs &a, double *partials,
+                                 double *energy, uint32_t count) {
+  const unsigned blocks = costate_tile_blocks(a.lay);
+  k_costate_tile<<<dim3(blocks, count), kAdjTileThreads, 0, s>>>(a, energy ? partials : nullptr);
+  if (energy) k_finish<<<count, kT, 0, s>>>(partials, int(blocks), 2, energy);
 }
 
 void launch_block_sums(hipStream_t s, const float2 *a, uint64_t dim, double *sums) {

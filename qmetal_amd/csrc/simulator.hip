@@ -1795,6 +1795,187 @@ struct Simulator::Impl {
     AccumTileArgs t{};
     AccumArgs g{};
   };
+  // Co-state in LDS tile passes (k_costate_tile, states of kAdjTileMinQubits
+  // or more): terms grouped by flip; a flip whose bits >= 7 number at most
+  // five is served inside a tile over wires 0..6 and five more (its partner
+  // amplitudes read from LDS), flip groups packed first-fit into as few tiles
+  // as their high wires allow; the diagonal terms (flip 0) go with the first
+  // tile (in chunks of kCoTerms across tiles when there are more). Flips that
+  // fit no tile keep k_pauli_accum passes, after the tiles. The first tile
+  // writes lambda without reading it. States are left for the caller.
+  struct CoPlan {
+    std::vector<CoTileArgs> tiles;
+    std::vector<AccumLaunch> generic;
+  };
+  static CoPlan costate_tile_plan(const Hamiltonian &h, uint32_t n) {
+    CoPlan out;
+    if (n < kAdjTileMinQubits || h.empty()) return out;
+    std::map<uint64_t, std::vector<size_t>> by_flip;
+    for (size_t k = 0; k < h.size(); k++) by_flip[h[k].p.x | h[k].p.y].push_back(k);
+    auto coef = [&](const Term &t, double &re, double &im) {  // c * i^ny
+      const uint32_t ny = uint32_t(__builtin_popcountll(t.p.y));
+      re = t.c;
+      im = 0.0;
+      switch (ny & 3u) {
+        case 1: re = 0.0; im = t.c; break;
+        case 2: re = -t.c; break;
+        case 3: re = 0.0; im = -t.c; break;
+        default: break;
+      }
+    };
+    const uint64_t low = (uint64_t(1) << 7) - 1;
+    struct Group {
+      uint64_t flip;
+      std::vector<size_t> terms;
+    };
+    struct Tile {
+      uint64_t hw = 0;  // high wires taken
+      std::vector<Group> groups;
+      std::vector<size_t> diag;
+      size_t nterms = 0;
+    };
+    std::vector<Tile> tiles;
+    std::vector<size_t> diag = by_flip.count(0) ? by_flip[0] : std::vector<size_t>{};
+    std::vector<Group> groups;
+    for (const auto &kv : by_flip) {
+      if (kv.first == 0) continue;
+      if (__builtin_popcountll(kv.first & ~low) > 5) {
+        const std::vector<size_t> &ts = kv.second;
+        for (size_t b = 0; b < ts.size(); b += kAccumTerms) {
+          AccumLaunch L;
+          AccumArgs &a = L.g;
+          a.flip = kv.first;
+          for (size_t j = b; j < ts.size() && j < b + kAccumTerms; j++) {
+            const Term &t = h[ts[j]];
+            coef(t, a.cre[a.nterms], a.cim[a.nterms]);
+            a.sign[a.nterms] = t.p.y | t.p.z;
+            a.nterms++;
+          }
+          out.generic.push_back(L);
+        }
+        continue;
+      }
+      for (size_t b = 0; b < kv.second.size(); b += kCoTerms) {
+        Group g;
+        g.flip = kv.first;
+        g.terms.assign(kv.second.begin() + long(b),
+                       kv.second.begin() + long(std::min(kv.second.size(), b + kCoTerms)));
+        groups.push_back(std::move(g));
+      }
+    }
+    // Lowest high wire first, so neighbouring wires share tiles.
+    std::stable_sort(groups.begin(), groups.end(), [&](const Group &a, const Group &b) {
+      const uint64_t ha = a.flip & ~low, hb = b.flip & ~low;
+      const int la = ha ? __builtin_ctzll(ha) : -1, lb2 = hb ? __builtin_ctzll(hb) : -1;
+      return la < lb2;
+    });
+    for (size_t b = 0; b < diag.size() || (b == 0 && tiles.empty()); b += kCoTerms) {
+      Tile t;
+      t.diag.assign(diag.begin() + long(std::min(b, diag.size())),
+                    diag.begin() + long(std::min(diag.size(), b + kCoTerms)));
+      t.nterms = t.diag.size();
+      tiles.push_back(std::move(t));
+      if (diag.empty()) break;
+    }
+    for (Group &g : groups) {
+      const uint64_t gh = g.flip & ~low;
+      Tile *dst = nullptr;
+      for (Tile &t : tiles)
+        if (__builtin_popcountll(t.hw | gh) <= 5 && t.groups.size() < size_t(kCoGroups) &&
+            t.nterms + g.terms.size() <= size_t(kCoTerms)) {
+          dst = &t;
+          break;
+        }
+      if (!dst) {
+        tiles.emplace_back();
+        dst = &tiles.back();
+      }
+      dst->hw |= gh;
+      dst->nterms += g.terms.size();
+      dst->groups.push_back(std::move(g));
+    }
+    if (tiles.size() == 1 && tiles[0].diag.empty() && tiles[0].groups.empty()) tiles.clear();
+    for (size_t ti = 0; ti < tiles.size(); ti++) {
+      Tile &t = tiles[ti];
+      for (uint32_t q = 7; __builtin_popcountll(t.hw) < 5 && q < n; q++) t.hw |= uint64_t(1) << q;
+      if (__builtin_popcountll(t.hw) != 5) fail("internal: co-state tile wires");
+      CoTileArgs a{};
+      uint32_t hp[5];
+      {
+        int k = 0;
+        for (uint32_t q = 7; q < n; q++)
+          if ((t.hw >> q) & 1) hp[k++] = q;
+      }
+      for (int k = 0; k < 5; k++) a.hpos[k] = hp[k];
+      auto lmask = [&](uint64_t sgn) {
+        uint32_t m = uint32_t(sgn & 1);
+        for (int k = 0; k < 3; k++) m |= uint32_t((sgn >> hp[2 + k]) & 1) << (1 + k);
+        return m;
+      };
+      auto slot_bits = [&](uint64_t f) {
+        uint32_t sl = 0;
+        for (uint32_t w = 1; w < 7; w++)
+          if ((f >> w) & 1) sl ^= tile_swizzle(1u << (w - 1));
+        for (int k = 0; k < 5; k++)
+          if ((f >> hp[k]) & 1) sl ^= tile_swizzle(1u << (6 + k));
+        return sl;
+      };
+      uint32_t cnt = 0;
+      // Diagonal terms by local mask (real coefficients: no Y without a flip).
+      std::vector<std::vector<size_t>> bym(16);
+      for (size_t k : t.diag) bym[lmask(h[k].p.z)].push_back(k);
+      for (uint32_t m = 0; m < 16; m++) {
+        a.doff[m] = cnt;
+        for (size_t k : bym[m]) {
+          a.sign[cnt] = h[k].p.z;
+          a.lmask[cnt] = m;
+          a.cre[cnt] = h[k].c;
+          a.cim[cnt] = 0.0;
+          cnt++;
+        }
+      }
+      a.doff[16] = cnt;
+      for (size_t gi = 0; gi < t.groups.size(); gi++) {
+        const Group &g = t.groups[gi];
+        a.gflip[gi] = g.flip;
+        a.gslot[gi] = slot_bits(g.flip);
+        a.goff[gi] = cnt;
+        for (size_t k : g.terms) {
+          const Term &tm = h[k];
+          const uint64_t sg = tm.p.y | tm.p.z;
+          double re, im;
+          coef(tm, re, im);
+          if (__builtin_popcountll(g.flip & sg) & 1) {
+            re = -re;
+            im = -im;
+          }
+          a.sign[cnt] = sg;
+          a.lmask[cnt] = lmask(sg);
+          a.cre[cnt] = re;
+          a.cim[cnt] = im;
+          cnt++;
+        }
+      }
+      a.ngroups = uint32_t(t.groups.size());
+      a.goff[a.ngroups] = cnt;
+      if (cnt > uint32_t(kCoTerms)) fail("internal: co-state tile terms");
+      a.first = ti == 0 ? 1u : 0u;
+      Layout &L = a.lay;
+      uint32_t ins[12];
+      for (uint32_t b = 0; b < 7; b++) ins[b] = b;
+      for (int k = 0; k < 5; k++) ins[7 + k] = hp[k];
+      for (uint32_t b = 0; b < 12; b++) {
+        L.ins[L.nins] = uint8_t(ins[b]);
+        L.hm[L.nins] = ~((uint64_t(1) << ins[b]) - 1);
+        L.nins++;
+      }
+      L.nthreads = (uint64_t(1) << n) >> 12;
+      L.streaming = 1;
+      out.tiles.push_back(a);
+    }
+    return out;
+  }
+
   static std::vector<AccumLaunch> costate_launches(const Hamiltonian &h) {
     std::vector<AccumLaunch> out;
     std::map<uint64_t, std::vector<size_t>> by_flip;
@@ -2571,6 +2752,44 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
 
   // lambda = H psi, terms grouped by flip mask.
   const double amp_bytes = 8.0 * double(s->dim);
+  const size_t G = c.gates.size();
+  size_t nrot_total = 0;
+  for (const Gate &g : c.gates) nrot_total += is_rotation(g.kind) ? 1 : 0;
+  // Slots: [0, 2G) per gate (one launch per gate), [2G, 2G + 2) energy,
+  // then per rotation in sweep order (merged passes).
+  double *slots = s->result_slots(2 * (G + 1) + 2 * nrot_total);
+  s->grow(s->adj_partials, s->adj_partials_cap,
+          std::max(size_t(kAdjMaxRot) * kAdjBlocks, size_t(kAdjTileMaxRot) * kAdjTileBlocks) * 2,
+          "hipMalloc(adjoint partials)");
+  Impl::CoPlan cp;
+  if (s->merge) cp = Impl::costate_tile_plan(h, s->n);
+  bool energy_done = false;
+  if (!cp.tiles.empty()) {
+    // Tiles first (the first writes lambda), then the flips no tile holds;
+    // the energy rides with the last tile when it is the last co-state pass.
+    for (size_t k = 0; k < cp.tiles.size(); k++) {
+      CoTileArgs &a = cp.tiles[k];
+      a.psi = s->state;
+      a.lam = s->costate;
+      const bool fuse = k + 1 == cp.tiles.size() && cp.generic.empty();
+      s->timed(kTimingCoState, (a.first ? 2 : 3) * amp_bytes, [&] {
+        launch_costate_tile(s->stream, a, s->adj_partials, fuse ? slots + 2 * G : nullptr);
+      });
+      energy_done = fuse;
+      s->launched();
+      s->dispatches++;
+      s->launches++;
+    }
+    for (Impl::AccumLaunch &L : cp.generic) {
+      L.g.lam = s->costate;
+      L.g.psi = s->state;
+      L.g.dim = s->dim;
+      s->timed(kTimingCoState, 3 * amp_bytes, [&] { launch_pauli_accum(s->stream, L.g); });
+      s->launched();
+      s->dispatches++;
+      s->launches++;
+    }
+  } else {
   s->timed(kTimingCoState, amp_bytes, [&] { launch_zero(s->stream, s->costate, s->dim); });
   s->launched();
   for (Impl::AccumLaunch &L : Impl::costate_launches(h)) {
@@ -2589,17 +2808,14 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
     s->dispatches++;
     s->launches++;
   }
-  const size_t G = c.gates.size();
-  size_t nrot_total = 0;
-  for (const Gate &g : c.gates) nrot_total += is_rotation(g.kind) ? 1 : 0;
-  // Slots: [0, 2G) per gate (one launch per gate), [2G, 2G + 2) energy,
-  // then per rotation in sweep order (merged passes).
-  double *slots = s->result_slots(2 * (G + 1) + 2 * nrot_total);
-  // E = Re<psi|lambda>
-  s->timed(kTimingOverlap, 2 * amp_bytes,
-           [&] { s->pauli_sum(s->state, s->costate, Pauli{}, slots + 2 * G); });
-  s->dispatches++;
-  s->launches++;
+  }
+  if (!energy_done) {
+    // E = Re<psi|lambda>
+    s->timed(kTimingOverlap, 2 * amp_bytes,
+             [&] { s->pauli_sum(s->state, s->costate, Pauli{}, slots + 2 * G); });
+    s->dispatches++;
+    s->launches++;
+  }
 
   if (s->merge && s->n >= 6) {
     // Backward sweep in merged passes (adjoint_schedule); the ops of every
@@ -2630,9 +2846,6 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
       a.ops = reinterpret_cast<const AdjOp *>(s->arena + offs[i]);
     }
     s->upload(total);
-    s->grow(s->adj_partials, s->adj_partials_cap,
-            std::max(size_t(kAdjMaxRot) * kAdjBlocks, size_t(kAdjTileMaxRot) * kAdjTileBlocks) * 2,
-            "hipMalloc(adjoint partials)");
     double *rot_slots = slots + 2 * (G + 1);
     for (size_t i = 0; i < passes.size(); i++) {
       s->timed(kTimingAdjPass, 4 * amp_bytes, [&] {
@@ -2771,8 +2984,11 @@ struct BatchedPass {
 };
 struct BatchedAccum {
   bool tile = false;
+  bool cotile = false;  // k_costate_tile (c); `fuse`: it leaves the energies
+  bool fuse = false;
   AccumTileArgs t{};
   AccumArgs g{};
+  CoTileArgs c{};
 };
 
 bool same_layout(const Layout &x, const Layout &y) {
@@ -2857,12 +3073,20 @@ void BatchedAdjoint::enqueue(void *stream, double *energy_out, double *grad_out)
     else if (b.kind == 2) launch_gwide_batched(st, b.wb, b.g, d.count);
     else launch_pass_batched(st, b.nl, b.a, d.count);
   }
-  launch_zero_batched(st, d.lam, d.dim, d.count);
+  const bool cotile = !d.costate.empty() && d.costate[0].cotile;
+  if (!cotile) launch_zero_batched(st, d.lam, d.dim, d.count);
+  bool fused = false;
   for (const BatchedAccum &b : d.costate) {
-    if (b.tile) launch_pauli_accum_tile_batched(st, b.t, d.count);
-    else launch_pauli_accum_batched(st, b.g, d.count);
+    if (b.cotile) {
+      launch_costate_tile_batched(st, b.c, d.red, b.fuse ? d.eslots : nullptr, d.count);
+      fused = fused || b.fuse;
+    } else if (b.tile) {
+      launch_pauli_accum_tile_batched(st, b.t, d.count);
+    } else {
+      launch_pauli_accum_batched(st, b.g, d.count);
+    }
   }
-  launch_pauli_batched(st, d.psi, d.lam, d.dim, 0, 0, d.red, d.eslots, d.count);
+  if (!fused) launch_pauli_batched(st, d.psi, d.lam, d.dim, 0, 0, d.red, d.eslots, d.count);
   for (size_t j = 0; j < d.bwd.size(); j++) {
     double *out = d.rslots + 2 * d.bwd_rot0[j];
     if (d.bwd[j].tile) launch_adj_tile_batched(st, d.bwd[j].t, d.adj, out, 2ull * d.nrot, d.count);
@@ -3037,7 +3261,18 @@ std::unique_ptr<BatchedAdjoint> Simulator::compile_batched(
     b.a.bstate = b.t.bstate = d.dim;
     b.a.barena = b.t.barena = stride;
   }
-  for (Impl::AccumLaunch &L : Impl::costate_launches(h)) {
+  Impl::CoPlan cp = Impl::costate_tile_plan(h, d.n);
+  for (size_t k = 0; k < cp.tiles.size(); k++) {
+    BatchedAccum b;
+    b.cotile = true;
+    b.fuse = k + 1 == cp.tiles.size() && cp.generic.empty();
+    b.c = cp.tiles[k];
+    b.c.lam = d.lam;
+    b.c.psi = d.psi;
+    b.c.bstate = d.dim;
+    d.costate.push_back(b);
+  }
+  for (Impl::AccumLaunch &L : cp.tiles.empty() ? Impl::costate_launches(h) : cp.generic) {
     BatchedAccum b;
     b.tile = L.tile;
     b.t = L.t;
@@ -3060,8 +3295,10 @@ std::unique_ptr<BatchedAdjoint> Simulator::compile_batched(
   d.rgate = device_copy(rgate, "batched rotation map");
   d.inst = device_copy(inst, "batched instances");
   d.gbase = device_copy(gbase, "batched gradient bases");
-  check(hipMalloc(reinterpret_cast<void **>(&d.red),
-                  size_t(m) * pauli_batched_blocks(d.dim) * 2 * sizeof(double)),
+  size_t rblocks = pauli_batched_blocks(d.dim);
+  for (const BatchedAccum &b : d.costate)
+    if (b.cotile) rblocks = std::max<size_t>(rblocks, costate_tile_blocks(b.c.lay));
+  check(hipMalloc(reinterpret_cast<void **>(&d.red), size_t(m) * rblocks * 2 * sizeof(double)),
         "hipMalloc(batched partials)");
   size_t aparts = 1;
   for (const BatchedAdjoint::Data::Bwd &b : d.bwd)

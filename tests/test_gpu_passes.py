@@ -338,6 +338,53 @@ def test_tile_adjoint_against_oracle_and_per_gate_sweep(n):
                 assert g_m[k] == 0.0
 
 
+def random_hamiltonian(n, count, seed):
+    """Random Pauli strings of weight 1..4 (X/Y/Z on random wires, so flips
+    land on wire 0, lane, wave, register and base bits), 60 Z-only strings
+    (more diagonal terms than one co-state tile takes), a flip over seven high
+    wires (no tile holds it) and one over wires 0..6."""
+    r = C.SplitMix64(seed)
+    terms = []
+    for _ in range(count):
+        x = y = z = 0
+        for _ in range(1 + r.next_u64() % 4):
+            q = r.next_u64() % n
+            k = r.next_u64() % 3
+            x &= ~(1 << q)
+            y &= ~(1 << q)
+            z &= ~(1 << q)
+            if k == 0:
+                x |= 1 << q
+            elif k == 1:
+                y |= 1 << q
+            else:
+                z |= 1 << q
+        terms.append((r.uniform(-1, 1), x, y, z))
+    for _ in range(60):
+        z = (1 << (r.next_u64() % n)) | (1 << (r.next_u64() % n))
+        terms.append((r.uniform(-1, 1), 0, 0, z))
+    terms.append((0.3, sum(1 << q for q in range(n - 7, n)), 0, 1))
+    terms.append((-0.2, 0, 0b1111110, 1 << (n - 1)))
+    return terms
+
+
+@pytest.mark.parametrize("n", (12, 14, 17))
+def test_costate_tiles_against_oracle(n):
+    """lambda = H psi in LDS tile passes (flips served from LDS, diagonal
+    terms by a per-thread Walsh-Hadamard transform, the energy fused into
+    the last pass) for Hamiltonians of every flip shape, through the energy
+    and the gradient, against the complex128 adjoint."""
+    terms = random_hamiltonian(n, 90, 13 * n)
+    gates = tile_adjoint_stream(n, 60, 5 * n)
+    with qm.Simulator(n) as s:
+        e, g = s.energy_and_gradient(C.to_circuit(n, gates), C.to_hamiltonian(terms))
+    e_ref, g_ref = O.adjoint(n, gates, terms)
+    d = float(np.abs(np.array(g) - g_ref).max())
+    print(f"n={n} terms={len(terms)}: energy {e:.6f} (oracle {e_ref:.6f}), max |grad diff| {d:.2e}")
+    assert abs(e - e_ref) <= 1e-5 * max(1.0, abs(e_ref))
+    assert d <= 2e-5
+
+
 @pytest.mark.parametrize("name,n", (("qaoa", 14), ("qaoa", 19), ("hea", 15), ("hea", 20)))
 def test_tile_adjoint_structured_circuits(name, n):
     """QAOA ring (its ZZ sandwiches run as diagonal units) and HEA (CX ladder)
