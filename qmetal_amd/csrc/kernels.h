@@ -419,7 +419,8 @@ constexpr int kAdjBlocks = 4096;
 // thread), axis 4 = Z Z on tile-local bits lbit and lbit2; a thread whose base
 // index has odd parity over `pmask` applies `malt` instead of `m` and negates
 // its overlap (a diagonal phase or Z generator on a wire that is not
-// tile-local).
+// tile-local). They compute only the overlap's part `comp` (0 real, 1
+// imaginary: the one the gradient reads) and leave the other 0.
 struct AdjOp {
   uint32_t type;
   uint32_t lbit;
@@ -428,7 +429,7 @@ struct AdjOp {
   uint32_t axis;
   uint32_t rslot;
   uint32_t lbit2;
-  uint32_t pad;
+  uint32_t comp;
   uint64_t gmask;
   uint64_t pmask;
   float m[8];

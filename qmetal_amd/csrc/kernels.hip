@@ -1114,62 +1114,83 @@ __device__ __forceinline__ void acc_cj_fma(double &re, double &im, float2 x, flo
   }
 }
 
-// <lam|P|psi> over a 16-amplitude tile for P = X / Y / Z on local bit LB
-// (axis 0..2), as tile_overlap, with fused accumulation.
-template <int LB>
+// v +/-= part COMP (0 real, 1 imaginary) of conj(x) y, in FP64 with fused
+// multiply-adds.
+// The conversions are volatile statements, so they stay in program order:
+// otherwise the compiler converts a whole tile ahead and holds 64 doubles.
+template <bool NEG, int COMP>
+__device__ __forceinline__ void acc_part(double &v, float2 x, float2 y) {
+  double xr, xi, yr, yi;
+  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(xr) : "v"(x.x));
+  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(xi) : "v"(x.y));
+  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(yr) : "v"(y.x));
+  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(yi) : "v"(y.y));
+  if (COMP == 0) {
+    v = fma(NEG ? -xr : xr, yr, v);
+    v = fma(NEG ? -xi : xi, yi, v);
+  } else {
+    v = fma(NEG ? -xr : xr, yi, v);
+    v = fma(NEG ? xi : -xi, yr, v);
+  }
+}
+
+// Part COMP of <lam|P|psi> over a 16-amplitude tile for P = X / Y / Z on
+// local bit LB (axis 0..2; Y before the i^ny factor, as tile_overlap).
+template <int LB, int COMP>
 __device__ __forceinline__ void tile_overlap16(const float2 (&x)[16], const float2 (&y)[16],
-                                               uint32_t axis, double &re, double &im) {
+                                               uint32_t axis, double &v) {
 #pragma unroll
   for (int l = 0; l < 16; l++) {
     if (l & (1 << LB)) continue;
     const int h = l | (1 << LB);
     if (axis == 0) {
-      acc_cj_fma<false>(re, im, y[l], x[h]);
-      acc_cj_fma<false>(re, im, y[h], x[l]);
+      acc_part<false, COMP>(v, y[l], x[h]);
+      acc_part<false, COMP>(v, y[h], x[l]);
     } else if (axis == 1) {
-      acc_cj_fma<true>(re, im, y[l], x[h]);
-      acc_cj_fma<false>(re, im, y[h], x[l]);
+      acc_part<true, COMP>(v, y[l], x[h]);
+      acc_part<false, COMP>(v, y[h], x[l]);
     } else {
-      acc_cj_fma<false>(re, im, y[l], x[l]);
-      acc_cj_fma<true>(re, im, y[h], x[h]);
+      acc_part<false, COMP>(v, y[l], x[l]);
+      acc_part<true, COMP>(v, y[h], x[h]);
     }
   }
 }
 
-// sum_l s(l) conj(y_l) x_l with s(l) = (-1)^(bit LA of l + bit LB of l); LA =
-// LB = -1: s = 1.
-template <int LA, int LB>
+// Part COMP of sum_l s(l) conj(y_l) x_l with s(l) = (-1)^(bit LA of l + bit
+// LB of l); LA = LB = -1: s = 1.
+template <int LA, int LB, int COMP>
 __device__ __forceinline__ void tile_overlap_parity(const float2 (&x)[16], const float2 (&y)[16],
-                                                    double &re, double &im) {
+                                                    double &v) {
 #pragma unroll
   for (int l = 0; l < 16; l++) {
     const int par = (LA >= 0 ? (l >> (LA < 0 ? 0 : LA)) & 1 : 0) ^
                     (LB >= 0 ? (l >> (LB < 0 ? 0 : LB)) & 1 : 0);
-    if (par) acc_cj_fma<true>(re, im, y[l], x[l]);
-    else acc_cj_fma<false>(re, im, y[l], x[l]);
+    if (par) acc_part<true, COMP>(v, y[l], x[l]);
+    else acc_part<false, COMP>(v, y[l], x[l]);
   }
 }
 
+template <int COMP>
 __device__ __forceinline__ void adj_op_overlap(const float2 (&x)[16], const float2 (&y)[16],
                                                uint32_t axis, uint32_t lb, uint32_t lb2,
-                                               double &re, double &im) {
+                                               double &v) {
   if (axis <= 2) {
     switch (lb) {
-      case 0: tile_overlap16<0>(x, y, axis, re, im); break;
-      case 1: tile_overlap16<1>(x, y, axis, re, im); break;
-      case 2: tile_overlap16<2>(x, y, axis, re, im); break;
-      default: tile_overlap16<3>(x, y, axis, re, im); break;
+      case 0: tile_overlap16<0, COMP>(x, y, axis, v); break;
+      case 1: tile_overlap16<1, COMP>(x, y, axis, v); break;
+      case 2: tile_overlap16<2, COMP>(x, y, axis, v); break;
+      default: tile_overlap16<3, COMP>(x, y, axis, v); break;
     }
   } else if (axis == 3) {
-    tile_overlap_parity<-1, -1>(x, y, re, im);
+    tile_overlap_parity<-1, -1, COMP>(x, y, v);
   } else {
     switch (lb * 4 + lb2) {  // lb < lb2
-      case 1: tile_overlap_parity<0, 1>(x, y, re, im); break;
-      case 2: tile_overlap_parity<0, 2>(x, y, re, im); break;
-      case 3: tile_overlap_parity<0, 3>(x, y, re, im); break;
-      case 6: tile_overlap_parity<1, 2>(x, y, re, im); break;
-      case 7: tile_overlap_parity<1, 3>(x, y, re, im); break;
-      default: tile_overlap_parity<2, 3>(x, y, re, im); break;
+      case 1: tile_overlap_parity<0, 1, COMP>(x, y, v); break;
+      case 2: tile_overlap_parity<0, 2, COMP>(x, y, v); break;
+      case 3: tile_overlap_parity<0, 3, COMP>(x, y, v); break;
+      case 6: tile_overlap_parity<1, 2, COMP>(x, y, v); break;
+      case 7: tile_overlap_parity<1, 3, COMP>(x, y, v); break;
+      default: tile_overlap_parity<2, 3, COMP>(x, y, v); break;
     }
   }
 }
@@ -1205,13 +1226,13 @@ extern "C" __device__ __attribute__((const)) double __ockl_wfred_add_f64(double)
 // fields are loaded as one block into scalar registers (a lane-dependent
 // choice between two fields read from memory would otherwise become a
 // per-lane load; loading the next op's block ahead spills scalar registers). Overlaps go to acc[rslot][wave][re, im]
-// (wave sums, lane 0 adds them in program order).
+// (the part `comp` only; wave sums, lane 0 adds them in program order).
 __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
                                              const_ptr<AdjOp> ops, uint32_t o0, uint32_t o1,
                                              uint64_t base, uint32_t lane, uint32_t wave,
                                              double *acc) {
   struct Hdr {
-    uint32_t type, lbit, sel, pm, axis, rslot, lbit2;
+    uint32_t type, lbit, sel, pm, axis, rslot, lbit2, comp;
     uint64_t gmask, pmask;
     float m[8], malt[8];
   };
@@ -1223,6 +1244,7 @@ __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
     h.axis = ops[o].axis;
     h.rslot = ops[o].rslot;
     h.lbit2 = ops[o].lbit2;
+    h.comp = ops[o].comp;
     h.gmask = ops[o].gmask;
     h.pmask = ops[o].pmask;
 #pragma unroll
@@ -1233,7 +1255,7 @@ __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
   };
   auto pin = [](Hdr &h) {
     asm volatile("" : "+s"(h.type), "+s"(h.lbit), "+s"(h.sel), "+s"(h.pm), "+s"(h.axis),
-                 "+s"(h.rslot), "+s"(h.lbit2), "+s"(h.gmask), "+s"(h.pmask));
+                 "+s"(h.rslot), "+s"(h.lbit2), "+s"(h.comp), "+s"(h.gmask), "+s"(h.pmask));
     asm volatile("" : "+s"(h.m[0]), "+s"(h.m[1]), "+s"(h.m[2]), "+s"(h.m[3]), "+s"(h.m[4]),
                  "+s"(h.m[5]), "+s"(h.m[6]), "+s"(h.m[7]));
     asm volatile("" : "+s"(h.malt[0]), "+s"(h.malt[1]), "+s"(h.malt[2]), "+s"(h.malt[3]),
@@ -1245,14 +1267,11 @@ __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
     pin(h);
     const bool odd = (__builtin_popcountll(base & h.pmask) & 1) != 0;
     if (h.type == 1) {
-      double re = 0.0, im = 0.0;
-      adj_op_overlap(x, y, h.axis, h.lbit, h.lbit2, re, im);
-      re = __ockl_wfred_add_f64(odd ? -re : re);
-      im = __ockl_wfred_add_f64(odd ? -im : im);
-      if (lane == 0) {
-        acc[(h.rslot * (kAdjTileThreads / 64) + wave) * 2] += re;
-        acc[(h.rslot * (kAdjTileThreads / 64) + wave) * 2 + 1] += im;
-      }
+      double v = 0.0;
+      if (h.comp) adj_op_overlap<1>(x, y, h.axis, h.lbit, h.lbit2, v);
+      else adj_op_overlap<0>(x, y, h.axis, h.lbit, h.lbit2, v);
+      v = __ockl_wfred_add_f64(odd ? -v : v);
+      if (lane == 0) acc[(h.rslot * (kAdjTileThreads / 64) + wave) * 2 + h.comp] += v;
     }
     float m[8];
     if ((h.gmask | h.pmask) == 0) {

@@ -1574,8 +1574,11 @@ struct Simulator::Impl {
         }
         return o;
       };
+      // The gradient reads the overlap's real part for RY, the imaginary
+      // part otherwise (Im(i^ny S)); only that part is computed.
       auto overlap = [&](AdjOp &o, uint32_t axis, uint32_t lb, uint32_t lb2) {
         o.type = 1;
+        o.comp = axis == 1 ? 0u : 1u;
         o.axis = axis;
         o.lbit = lb == 99 ? o.lbit : lb;
         o.lbit2 = lb2;
