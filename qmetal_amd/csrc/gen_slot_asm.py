@@ -18,6 +18,9 @@ Section `sel` = 8 * kind + lbit:
           the p-th index l with bit lbit clear, ascending)
   kind 5  X: swap the two amplitudes of the pairs in `pm` (2 per pair)
   kind 6  diagonal matrix diag(m00, m11) on the pairs in `pm` (8 per pair)
+  kind 7  real diagonal, imaginary off-diagonal (RX, and products of RX
+          with real phases: m00, m11 = m[0], m[6]; m01, m10 = i m[3], i m[5]),
+          all pairs (8 per pair)
 Any other `sel` leaves the tile unchanged. The matrix is per lane (VGPRs).
 (Kinds 3 and 4, complex matrices on the matrix core, were measured and
 dropped: slower than the VALU in these passes.)
@@ -72,6 +75,22 @@ def bfly_cplx(a, b):
     ]
 
 
+def bfly_rx(a, b):
+    """(a, b) <- (m00 a + i m01i b, i m10i a + m11 b) with m00, m11 real: the
+    four cross products into temporaries, then fused in-place updates."""
+    ra, ia, rb, ib = f"%[r{a}]", f"%[i{a}]", f"%[r{b}]", f"%[i{b}]"
+    return [
+        f"v_mul_f32 %[t0], %[m1i], {ib}",
+        f"v_mul_f32 %[t1], %[m1i], {rb}",
+        f"v_mul_f32 %[t2], %[m2i], {ia}",
+        f"v_mul_f32 %[t3], %[m2i], {ra}",
+        f"v_fma_f32 {ra}, %[m0r], {ra}, -%[t0]",
+        f"v_fma_f32 {ia}, %[m0r], {ia}, %[t1]",
+        f"v_fma_f32 {rb}, %[m3r], {rb}, -%[t2]",
+        f"v_fma_f32 {ib}, %[m3r], {ib}, %[t3]",
+    ]
+
+
 def pairs(nl, h):
     return [(l, l | (1 << h)) for l in range(1 << nl) if not (l >> h) & 1]
 
@@ -101,7 +120,7 @@ def diag_section(nl, h):
     return out
 
 
-KIND_ORDER = (1, 6, 5, 0, 2)  # dispatch order: the common kinds first
+KIND_ORDER = (1, 6, 7, 5, 0, 2)  # dispatch order: the common kinds first
 
 
 def statement(nl):
@@ -126,6 +145,8 @@ def statement(nl):
                         lines += bfly_real(a, b)
                     elif kind == 1:
                         lines += bfly_cplx(a, b)
+                    elif kind == 7:
+                        lines += bfly_rx(a, b)
                     else:
                         lines += [f"s_bitcmp1_b32 %[pm], {p}", f"s_cbranch_scc0 .Lqs%=_{kind}_{h}_{p}"]
                         lines += bfly_cplx(a, b)

@@ -151,12 +151,17 @@ const float kX[8] = {0, 0, 1, 0, 1, 0, 0, 0};
 // instructions per pair instead of 16); a diagonal matrix multiplies each
 // amplitude (6, eight per pair: RZ, and the phases of CZ and CP, on the
 // locally controlled pairs; a control on a base-index bit turns into the
-// identity matrix per lane, which is diagonal too); otherwise real (0),
-// complex (1), or complex over the locally controlled pairs (2).
+// identity matrix per lane, which is diagonal too); otherwise complex over
+// the locally controlled pairs (2), real (0), real on the diagonal and
+// imaginary off it (7: RX and its products with real factors, eight per
+// pair), or complex (1).
 uint32_t slot_kind(const float *m8, bool real, bool local_ctrl, bool lane_ctrl) {
   if (!lane_ctrl && std::memcmp(m8, kX, sizeof(kX)) == 0) return 5u;
   if (m8[2] == 0.0f && m8[3] == 0.0f && m8[4] == 0.0f && m8[5] == 0.0f) return 6u;
-  return local_ctrl ? 2u : (real ? 0u : 1u);
+  if (local_ctrl) return 2u;
+  if (real) return 0u;
+  if (m8[1] == 0.0f && m8[2] == 0.0f && m8[4] == 0.0f && m8[7] == 0.0f) return 7u;
+  return 1u;
 }
 
 // U^dagger (reference simulator.mm:786-799); U3(t,p,l)^dagger = U3(-t,-l,-p).
