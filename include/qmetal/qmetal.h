@@ -157,11 +157,16 @@ int qm_plan_get_arrays(qm_plan p, size_t k, uint32_t *group_qubits,
  * passes and the Hamiltonian's reductions: no co-state, no backward sweep).
  * `stream` is the caller's hipStream_t (or NULL for the null stream): work
  * already queued there completes before the parameters are read, and work
- * queued there afterwards sees the results. Host-synchronous: the parameters
- * are copied to the host (the plan's matrices and phase tables are built
- * there), so the call waits for the caller's stream up to that point and
- * cannot be recorded into a graph; the results are written back
- * asynchronously. */
+ * queued there afterwards sees the results.
+ * States of 12 qubits or more (with pass merging on and a non-empty
+ * Hamiltonian): the first call for a circuit template (gate kinds and
+ * wires; angles come from `params`) and Hamiltonian compiles a program and
+ * waits; every later call only launches kernels on `stream` (the rotation
+ * matrices are written on the device from `params`): no host
+ * synchronisation, no allocation, so it can be captured in a graph. Calls on
+ * one simulator must be ordered (one stream, or synchronised), as they share
+ * its states and program. Smaller states take the host path: the parameters
+ * are copied to the host and the call waits for `stream` up to that point. */
 int qm_energy_and_gradient_device(qm_simulator s, qm_circuit c,
                                   const qm_hamiltonian *h, const double *params,
                                   size_t num_params, double *out_energy,

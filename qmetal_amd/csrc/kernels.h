@@ -489,6 +489,25 @@ struct AdjTileArgs {
 };
 // Per-block partials: kAdjTileMaxRot * kAdjTileBlocks * 2 doubles.
 void launch_adj_tile(hipStream_t s, const AdjTileArgs &a, double *partials, double *out);
+// The same tile passes on one state (a.psi; lam and overlaps unused): the
+// forward sweep of a device-resident gradient program.
+void launch_fwd_tile(hipStream_t s, const AdjTileArgs &a);
+
+// Angles of a device-resident gradient program, written on the device: the
+// AdjOp at byte offset `off` of the arena takes m / malt for angle a = +t or
+// -t (bit 8 of `code`: the inverse), t = params[param], by recipe code & 15:
+//   0: the rotation's own 2x2 (code >> 4 & 15: 0 RX, 1 RY, 2 RZ), malt = m;
+//   with d0 = exp(-i a / 2), d1 = exp(i a / 2):
+//   1: m = diag(d0, d0), malt = diag(d1, d1);   2: m = diag(d0, d1), malt =
+//   diag(d1, d0);   3: m = malt = diag(d1 conj(d0), d0 conj(d1)).
+struct AnglePatch {
+  uint32_t off;
+  uint32_t param;
+  uint32_t code;
+  uint32_t pad;
+};
+void launch_patch_angles(hipStream_t s, uint8_t *arena, const AnglePatch *p, uint32_t count,
+                         const double *params);
 
 // Sampling: FP64 sums of |a|^2 over 1024-amplitude blocks, an inclusive FP64
 // prefix over them, and per-shot resolution (splitmix64 stream, binary search

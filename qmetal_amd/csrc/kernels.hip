@@ -1227,6 +1227,7 @@ extern "C" __device__ __attribute__((const)) double __ockl_wfred_add_f64(double)
 // choice between two fields read from memory would otherwise become a
 // per-lane load; loading the next op's block ahead spills scalar registers). Overlaps go to acc[rslot][wave][re, im]
 // (the part `comp` only; wave sums, lane 0 adds them in program order).
+template <bool ONE>
 __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
                                              const_ptr<AdjOp> ops, uint32_t o0, uint32_t o1,
                                              uint64_t base, uint32_t lane, uint32_t wave,
@@ -1266,7 +1267,7 @@ __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
     load_hdr(o, h);
     pin(h);
     const bool odd = (__builtin_popcountll(base & h.pmask) & 1) != 0;
-    if (h.type == 1) {
+    if (!ONE && h.type == 1) {
       double v = 0.0;
       if (h.comp) adj_op_overlap<1>(x, y, h.axis, h.lbit, h.lbit2, v);
       else adj_op_overlap<0>(x, y, h.axis, h.lbit, h.lbit2, v);
@@ -1286,7 +1287,7 @@ __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
       }
     }
     slot_asm<4>(x, h.sel, h.pm, m);
-    slot_asm<4>(y, h.sel, h.pm, m);
+    if (!ONE) slot_asm<4>(y, h.sel, h.pm, m);
   }
 }
 
@@ -1294,6 +1295,8 @@ __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
 // all its threads take part in every op; overlaps are reduced as in
 // k_adj_pass (wave sums into per-wave LDS slots in program order, per-block
 // partials, a fixed-order finish).
+// ONE: the forward sweep of a device-resident gradient program, on psi only.
+template <bool ONE>
 __global__ __launch_bounds__(kAdjTileThreads) __attribute__((amdgpu_waves_per_eu(4)))
 void k_adj_tile(AdjTileArgs a, double *part) {
   constexpr int W = kAdjTileThreads / 64;
@@ -1336,11 +1339,15 @@ void k_adj_tile(AdjTileArgs a, double *part) {
 #pragma unroll
     for (int r = 0; r < 8; r++) {
       const float4 v = ld4<true>(pair_ptr(psi, ub | wave_off | reg_off[r], lane_bytes));
-      const float4 w = ld4<true>(pair_ptr(lam, ub | wave_off | reg_off[r], lane_bytes));
       x[2 * r] = make_float2(v.x, v.y);
       x[2 * r + 1] = make_float2(v.z, v.w);
-      y[2 * r] = make_float2(w.x, w.y);
-      y[2 * r + 1] = make_float2(w.z, w.w);
+      if (!ONE) {
+        const float4 w = ld4<true>(pair_ptr(lam, ub | wave_off | reg_off[r], lane_bytes));
+        y[2 * r] = make_float2(w.x, w.y);
+        y[2 * r + 1] = make_float2(w.z, w.w);
+      } else {
+        y[2 * r] = y[2 * r + 1] = make_float2(0.0f, 0.0f);
+      }
     }
     uint32_t tf = t0, rf[3] = {r0[0], r0[1], r0[2]};
     for (uint32_t s = 0; s < a.nsegs; s++) {
@@ -1368,22 +1375,24 @@ void k_adj_tile(AdjTileArgs a, double *part) {
         base |= on ? 1ull << tp[j] : 0ull;
       }
       tile_permute16(lds, x, tf, rf, tt, rt);
-      tile_permute16(lds, y, tf, rf, tt, rt);
-      adj_tile_ops(x, y, ops, op0, op1, base, lane, wave, &acc[0][0][0]);
+      if (!ONE) tile_permute16(lds, y, tf, rf, tt, rt);
+      adj_tile_ops<ONE>(x, y, ops, op0, op1, base, lane, wave, &acc[0][0][0]);
       tf = tt;
 #pragma unroll
       for (int k = 0; k < 3; k++) rf[k] = rt[k];
     }
     tile_permute16(lds, x, tf, rf, t0, r0);
-    tile_permute16(lds, y, tf, rf, t0, r0);
+    if (!ONE) tile_permute16(lds, y, tf, rf, t0, r0);
 #pragma unroll
     for (int r = 0; r < 8; r++) {
       st4<true>(pair_ptr(psi, ub | wave_off | reg_off[r], lane_bytes),
                 make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y));
-      st4<true>(pair_ptr(lam, ub | wave_off | reg_off[r], lane_bytes),
-                make_float4(y[2 * r].x, y[2 * r].y, y[2 * r + 1].x, y[2 * r + 1].y));
+      if (!ONE)
+        st4<true>(pair_ptr(lam, ub | wave_off | reg_off[r], lane_bytes),
+                  make_float4(y[2 * r].x, y[2 * r].y, y[2 * r + 1].x, y[2 * r + 1].y));
     }
   }
+  if (ONE) return;
   __syncthreads();
   if (threadIdx.x < a.nrot * 2) {
     const uint32_t r = threadIdx.x >> 1, c = threadIdx.x & 1;
@@ -1918,8 +1927,71 @@ void launch_adj_pass(hipStream_t s, const AdjPassArgs &a, double *partials, doub
 
 void launch_adj_tile(hipStream_t s, const AdjTileArgs &a, double *partials, double *out) {
   const unsigned blocks = unsigned(std::min<uint64_t>(a.lay.nthreads, kAdjTileBlocks));
-  k_adj_tile<<<blocks, kAdjTileThreads, 0, s>>>(a, partials);
+  k_adj_tile<false><<<blocks, kAdjTileThreads, 0, s>>>(a, partials);
   if (a.nrot) k_finish_rot<<<a.nrot, kT, 0, s>>>(partials, int(blocks), out, 0, kAdjTileMaxRot);
+}
+
+void launch_fwd_tile(hipStream_t s, const AdjTileArgs &a) {
+  const unsigned blocks = unsigned(std::min<uint64_t>(a.lay.nthreads, kAdjTileBlocks));
+  k_adj_tile<true><<<blocks, kAdjTileThreads, 0, s>>>(a, nullptr);
+}
+
+// One thread per patch (AnglePatch).
+__global__ __launch_bounds__(kT) void k_patch_angles(uint8_t *arena, const AnglePatch *p,
+                                                     uint32_t count, const double *params) {
+  const uint32_t i = blockIdx.x * kT + threadIdx.x;
+  if (i >= count) return;
+  const AnglePatch q = p[i];
+  const double t = params[q.param];
+  const double a = (q.code & 256u) ? -t : t;
+  double sn, cs;
+  sincos(0.5 * a, &sn, &cs);
+  // d0 = exp(-i a / 2), d1 = exp(i a / 2)
+  const double d0r = cs, d0i = -sn, d1r = cs, d1i = sn;
+  double m[8], ma[8];
+  for (int k = 0; k < 8; k++) m[k] = 0.0;
+  switch (q.code & 15u) {
+    case 0: {
+      const uint32_t kind = (q.code >> 4) & 15u;
+      if (kind == 0) {         // RX: [[c, -i s], [-i s, c]]
+        m[0] = cs; m[3] = -sn; m[5] = -sn; m[6] = cs;
+      } else if (kind == 1) {  // RY: [[c, -s], [s, c]]
+        m[0] = cs; m[2] = -sn; m[4] = sn; m[6] = cs;
+      } else {                 // RZ: diag(d0, d1)
+        m[0] = d0r; m[1] = d0i; m[6] = d1r; m[7] = d1i;
+      }
+      for (int k = 0; k < 8; k++) ma[k] = m[k];
+      break;
+    }
+    case 1:
+      for (int k = 0; k < 8; k++) ma[k] = 0.0;
+      m[0] = m[6] = d0r; m[1] = m[7] = d0i;
+      ma[0] = ma[6] = d1r; ma[1] = ma[7] = d1i;
+      break;
+    case 2:
+      for (int k = 0; k < 8; k++) ma[k] = 0.0;
+      m[0] = d0r; m[1] = d0i; m[6] = d1r; m[7] = d1i;
+      ma[0] = d1r; ma[1] = d1i; ma[6] = d0r; ma[7] = d0i;
+      break;
+    default: {
+      // diag(d1 conj(d0), d0 conj(d1)) = diag(exp(i a), exp(-i a))
+      double s2, c2;
+      sincos(a, &s2, &c2);
+      m[0] = c2; m[1] = s2; m[6] = c2; m[7] = -s2;
+      for (int k = 0; k < 8; k++) ma[k] = m[k];
+      break;
+    }
+  }
+  AdjOp *o = reinterpret_cast<AdjOp *>(arena + q.off);
+  for (int k = 0; k < 8; k++) {
+    o->m[k] = float(m[k]);
+    o->malt[k] = float(ma[k]);
+  }
+}
+
+void launch_patch_angles(hipStream_t s, uint8_t *arena, const AnglePatch *p, uint32_t count,
+                         const double *params) {
+  if (count) k_patch_angles<<<(count + kT - 1) / kT, kT, 0, s>>>(arena, p, count, params);
 }
 
 unsigned costate_tile_blocks(const Layout &lay) {
@@ -2018,7 +2090,7 @@ unsigned adj_tile_batched_blocks(const Layout &lay) {
 void launch_adj_tile_batched(hipStream_t s, const AdjTileArgs &a, double *partials, double *out,
                              uint64_t ostride, uint32_t count) {
   const unsigned blocks = adj_tile_batched_blocks(a.lay);
-  k_adj_tile<<<dim3(blocks, count), kAdjTileThreads, 0, s>>>(a, partials);
+  k_adj_tile<false><<<dim3(blocks, count), kAdjTileThreads, 0, s>>>(a, partials);
   if (a.nrot)
     k_finish_rot<<<dim3(a.nrot, count), kT, 0, s>>>(partials, int(blocks), out, ostride,
                                                    kAdjTileMaxRot);

@@ -106,6 +106,10 @@ class Simulator {
   // Same, with parameters (one per RX/RY/RZ, circuit order) read from and
   // energy / gradient (one per parameter) written to device memory; ordered
   // after and before work on `caller_stream` (a hipStream_t, may be null).
+  // The device-resident program for template c and Hamiltonian h (compiled
+  // on first use, cached); false when the path does not apply (fewer than
+  // 12 qubits, pass merging off, an empty Hamiltonian).
+  bool compile_device_program(const Circuit &c, const Hamiltonian &h);
   void energy_and_gradient_device(const Circuit &c, const Hamiltonian &h,
                                   const double *params, size_t nparams,
                                   double *out_energy, double *out_grad,

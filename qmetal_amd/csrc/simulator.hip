@@ -1197,6 +1197,38 @@ struct Simulator::Impl {
   // Set while compiling a batched program (compile_batched).
   bool batch_compile = false;
 
+  // Device-resident gradient program (energy_and_gradient_device): a
+  // circuit template's forward and backward tile passes and co-state tiles,
+  // compiled once; each call writes the rotations' matrices into the arena
+  // from the device parameters (k_patch_angles) and launches everything on
+  // the caller's stream, with no host synchronisation and no allocation.
+  struct DevProg {
+    std::vector<Gate> shape;  // the template (angles ignored)
+    Hamiltonian h;
+    uint8_t *arena = nullptr;
+    AnglePatch *patch = nullptr;
+    uint32_t npatch = 0;
+    std::vector<AdjTileArgs> fwd, bwd;
+    std::vector<size_t> bwd_rot0;
+    std::vector<CoTileArgs> co;
+    std::vector<AccumArgs> co_generic;
+    uint32_t nrot = 0;
+    double *eslot = nullptr, *rslots = nullptr;
+    uint32_t *rgate = nullptr, *inst = nullptr;
+    ~DevProg() {
+      hipFree(arena);
+      hipFree(patch);
+      hipFree(eslot);
+      hipFree(rslots);
+      hipFree(rgate);
+      hipFree(inst);
+    }
+  };
+  std::unique_ptr<DevProg> dev_prog;
+  // Launch count after the last device-resident call: other work since then
+  // on the simulator's stream must finish before the next one starts.
+  uint64_t dev_launches = ~0ull;
+
   // --- adjoint sweep building blocks (energy_and_gradient, batched programs)
 
   // One merged backward pass: inverse gates (indices into the circuit, in
@@ -1370,10 +1402,15 @@ struct Simulator::Impl {
   // tile (wires >= 7 join the tile while fewer than five have); a new segment
   // when none fits; the pass ends when a new segment would not help, at
   // kAdjTileMaxSeg segments or kAdjTileMaxRot rotations.
+  // forward: the same passes for the circuit itself in gate order (units
+  // taken first-gate-first, no rotation limit: no overlaps).
   static std::vector<APass> adjoint_tile_schedule(const Circuit &c,
-                                                  std::vector<size_t> &rot_index) {
+                                                  std::vector<size_t> &rot_index,
+                                                  bool forward = false) {
     const std::vector<AUnit> units = adjoint_units(c);
     const size_t U = units.size();
+    // Unit u of the sweep order.
+    auto unit_at = [&](size_t u) -> const AUnit & { return units[forward ? u : U - 1 - u]; };
     std::vector<std::vector<uint32_t>> succ(U);
     std::vector<uint32_t> ndeps(U, 0);
     {
@@ -1381,7 +1418,7 @@ struct Simulator::Impl {
       std::vector<std::vector<uint32_t>> diags(64);
       std::vector<uint32_t> deps;
       for (size_t u = 0; u < U; u++) {
-        const AUnit &un = units[U - 1 - u];
+        const AUnit &un = unit_at(u);
         const Gate &g = c.gates[un.g];
         const bool dg = unit_diagonal(c, un);
         const uint32_t nw = un.len == 3 ? 2u : g.nq;
@@ -1409,8 +1446,8 @@ struct Simulator::Impl {
     std::vector<std::vector<uint32_t>> tg(U);
     std::vector<uint32_t> isrot(U);
     for (size_t u = 0; u < U; u++) {
-      tg[u] = unit_targets(c, units[U - 1 - u]);
-      isrot[u] = unit_rotation(c, units[U - 1 - u]) >= 0 ? 1u : 0u;
+      tg[u] = unit_targets(c, unit_at(u));
+      isrot[u] = !forward && unit_rotation(c, unit_at(u)) >= 0 ? 1u : 0u;
     }
     std::vector<uint32_t> ready;
     for (uint32_t u = 0; u < U; u++)
@@ -1455,13 +1492,13 @@ struct Simulator::Impl {
           }
           break;
         }
-        const AUnit &un = units[U - 1 - size_t(best)];
+        const AUnit &un = unit_at(size_t(best));
         for (uint32_t q : tg[best]) {
           if (q >= 7 && std::find(hw.begin(), hw.end(), q) == hw.end()) hw.push_back(q);
           std::vector<uint32_t> &sr = cur.seg_reg.back();
           if (q != 0 && std::find(sr.begin(), sr.end(), q) == sr.end()) sr.push_back(q);
         }
-        const int64_t rg = unit_rotation(c, un);
+        const int64_t rg = forward ? -1 : unit_rotation(c, un);
         if (rg >= 0) {
           rot_index[size_t(rg)] = sweep_rot++;
           cur.nrot++;
@@ -1503,8 +1540,13 @@ struct Simulator::Impl {
   // Segments and AdjOps of tile pass `ps` into host block `dst`, whose
   // device copy is `dev`; sets every field of `a` but the states. Returns
   // the number of AdjOps.
+  // forward: the gates themselves, no overlaps (a forward tile pass). With
+  // `patches`, every op whose matrix depends on a rotation's angle gets a
+  // record (byte offset from `dst`, the rotation's gate index as `param`,
+  // the recipe) and a section fixed by the gate kind, not by the angle.
   size_t emit_adj_tile(const Circuit &c, const APass &ps, uint8_t *dst, const uint8_t *dev,
-                       AdjTileArgs &a) const {
+                       AdjTileArgs &a, bool forward = false,
+                       std::vector<AnglePatch> *patches = nullptr) const {
     const size_t nseg = ps.seg_reg.size();
     const size_t segb = (nseg * sizeof(AdjSeg) + 255) & ~size_t(255);
     AdjSeg *sg = reinterpret_cast<AdjSeg *>(dst);
@@ -1558,6 +1600,9 @@ struct Simulator::Impl {
           if (tw[rb[k]] == q) return k + 1;
         return -1;
       };
+      // Angle-dependent op: record the patch and fix the section (6 for
+      // diagonals, 7 for RX, 0 for RY).
+      uint32_t pcode = ~0u, pgate = 0;
       auto op = [&](int lt, const cd *m, const cd *malt, uint32_t cm, uint64_t gmask,
                     uint64_t pmask) -> AdjOp & {
         if (lt < 0) fail("internal: adjoint tile target not local");
@@ -1570,7 +1615,15 @@ struct Simulator::Impl {
         narrow(malt ? malt : m, o.malt);
         bool real = true;
         for (int i = 1; i < 8; i += 2) real = real && o.m[i] == 0.0f && o.malt[i] == 0.0f;
-        o.sel = o.lbit + 8u * slot_kind(o.m, real, cm != 0, gmask != 0 || pmask != 0);
+        uint32_t kind = slot_kind(o.m, real, cm != 0, gmask != 0 || pmask != 0);
+        if (patches && pcode != ~0u) {
+          const uint32_t rk = (pcode >> 4) & 15u;
+          kind = (pcode & 15u) != 0 || rk == 2 ? 6u : rk == 0 ? 7u : 0u;
+          if (cm != 0 && kind != 6u) fail("internal: controlled rotation patch");
+          patches->push_back(AnglePatch{
+              uint32_t(reinterpret_cast<uint8_t *>(&o) - dst), pgate, pcode, 0});
+        }
+        o.sel = o.lbit + 8u * kind;
         uint32_t pidx = 0;
         for (uint32_t l = 0; l < 16u; l++) {
           if ((l >> o.lbit) & 1) continue;
@@ -1593,44 +1646,60 @@ struct Simulator::Impl {
       for (size_t ui = u0; ui < ps.seg_end[si]; ui++) {
         const AUnit &un = ps.units[ui];
         const Gate &gt = c.gates[un.g];
+        const uint32_t inv_bit = forward ? 0u : 256u;
+        pcode = ~0u;
         if (un.len == 3) {
           const uint32_t qa = gt.q[0], qb = gt.q[1];
           cd r[4];
-          const double p[3] = {-c.gates[un.g + 1].p[0], 0.0, 0.0};
+          const double t = c.gates[un.g + 1].p[0];
+          const double p[3] = {forward ? t : -t, 0.0, 0.0};
           gate_matrix(K_RZ, p, r);
           const cd d0 = r[0], d1 = r[3];
           const int la = local_of(qa), lb = local_of(qb);
+          pgate = un.g + 1;
+          auto ov = [&](AdjOp &o, uint32_t axis, uint32_t l1, uint32_t l2) {
+            if (!forward) overlap(o, axis, l1, l2);
+          };
           if (la < 0 && lb < 0) {
             const cd m[4] = {d0, 0.0, 0.0, d0}, ma[4] = {d1, 0.0, 0.0, d1};
-            overlap(op(0, m, ma, 0, 0, bit(qa) | bit(qb)), 3, 99, 0);
+            pcode = 1u | (2u << 4) | inv_bit;
+            ov(op(0, m, ma, 0, 0, bit(qa) | bit(qb)), 3, 99, 0);
           } else if (la < 0 || lb < 0) {
             const int l = la < 0 ? lb : la;
             const uint32_t other = la < 0 ? qa : qb;
             const cd m[4] = {d0, 0.0, 0.0, d1}, ma[4] = {d1, 0.0, 0.0, d0};
-            overlap(op(l, m, ma, 0, 0, bit(other)), 2, 99, 0);
+            pcode = 2u | (2u << 4) | inv_bit;
+            ov(op(l, m, ma, 0, 0, bit(other)), 2, 99, 0);
           } else {
             const cd m[4] = {d0, 0.0, 0.0, d1};
-            overlap(op(lb, m, nullptr, 0, 0, 0), 4, uint32_t(std::min(la, lb)),
-                    uint32_t(std::max(la, lb)));
+            pcode = 0u | (2u << 4) | inv_bit;
+            ov(op(lb, m, nullptr, 0, 0, 0), 4, uint32_t(std::min(la, lb)),
+               uint32_t(std::max(la, lb)));
             const cd m2[4] = {d1 * std::conj(d0), 0.0, 0.0, d0 * std::conj(d1)};
+            pcode = 3u | (2u << 4) | inv_bit;
             op(lb, m2, nullptr, 1u << la, 0, 0);
           }
+          pcode = ~0u;
           continue;
         }
-        const Gate inv = inverse(gt);
+        const Gate inv = forward ? gt : inverse(gt);
         cd m[4];
         if (gt.nq == 1 && is_diagonal(gt)) {
           gate_matrix(inv.kind, inv.p, m);
           const int l = local_of(gt.q[0]);
           const bool rz = gt.kind == K_RZ;
+          pgate = un.g;
           if (l >= 0) {
+            if (rz) pcode = 0u | (2u << 4) | inv_bit;
             AdjOp &o = op(l, m, nullptr, 0, 0, 0);
-            if (rz) overlap(o, 2, 99, 0);
+            if (rz && !forward) overlap(o, 2, 99, 0);
           } else {
             const cd mm[4] = {m[0], 0.0, 0.0, m[0]}, ma[4] = {m[3], 0.0, 0.0, m[3]};
+            if (rz) pcode = 1u | (2u << 4) | inv_bit;
             AdjOp &o = op(0, mm, ma, 0, 0, bit(gt.q[0]));
-            if (rz) overlap(o, 3, 99, 0);
+            if (rz && !forward) overlap(o, 3, 99, 0);
           }
+          pcode = ~0u;
           continue;
         }
         if (gt.kind == K_CZ || gt.kind == K_CP) {
@@ -1660,8 +1729,13 @@ struct Simulator::Impl {
         uint64_t gm = 0;
         if (gt.nq == 1) {
           gate_matrix(inv.kind, inv.p, m);
+          if (is_rotation(gt.kind)) {
+            pgate = un.g;
+            pcode = 0u | ((gt.kind == K_RX ? 0u : 1u) << 4) | inv_bit;
+          }
           AdjOp &o = op(local_of(gt.q[0]), m, nullptr, 0, 0, 0);
-          if (is_rotation(gt.kind)) overlap(o, gt.kind == K_RX ? 0 : 1, 99, 0);
+          pcode = ~0u;
+          if (is_rotation(gt.kind) && !forward) overlap(o, gt.kind == K_RX ? 0 : 1, 99, 0);
           continue;
         }
         switch (gt.kind) {
@@ -2392,6 +2466,10 @@ Simulator::~Simulator() {
   Impl *s = impl_.get();
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->dev_prog) {
+    (void)hipDeviceSynchronize();  // its kernels ran on the callers' streams
+    s->dev_prog.reset();
+  }
   hipFree(s->state);
   hipFree(s->costate);
   hipFree(s->partials);
@@ -2919,6 +2997,114 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
 // results go back asynchronously. Without out_grad only the energy is
 // computed: the forward passes and one FP64 reduction per term, no co-state
 // and no backward sweep.
+namespace {
+template <typename T>
+T *device_vector(const std::vector<T> &v, const char *what) {
+  T *p = nullptr;
+  check(hipMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(v.size(), 1) * sizeof(T)), what);
+  if (!v.empty())
+    check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+  return p;
+}
+
+bool same_shape(const std::vector<Gate> &a, const std::vector<Gate> &b) {
+  if (a.size() != b.size()) return false;
+  for (size_t k = 0; k < a.size(); k++)
+    if (a[k].kind != b[k].kind || a[k].nq != b[k].nq ||
+        std::memcmp(a[k].q, b[k].q, sizeof(a[k].q)) != 0 ||
+        (!is_rotation(a[k].kind) && std::memcmp(a[k].p, b[k].p, sizeof(a[k].p)) != 0))
+      return false;
+  return true;
+}
+bool same_hamiltonian(const Hamiltonian &a, const Hamiltonian &b) {
+  if (a.size() != b.size()) return false;
+  for (size_t k = 0; k < a.size(); k++)
+    if (a[k].c != b[k].c || a[k].p.x != b[k].p.x || a[k].p.y != b[k].p.y || a[k].p.z != b[k].p.z)
+      return false;
+  return true;
+}
+}  // namespace
+
+// Compiles the device-resident program for template c (every rotation a
+// parameter) and Hamiltonian h; false when the shape is not supported (then
+// the host path serves the call).
+bool Simulator::compile_device_program(const Circuit &c, const Hamiltonian &h) {
+  Impl *s = impl_.get();
+  if (!s->merge || s->n < kAdjTileMinQubits || h.empty()) return false;
+  if (s->dev_prog && same_shape(s->dev_prog->shape, c.gates) &&
+      same_hamiltonian(s->dev_prog->h, h))
+    return true;
+  s->dev_prog.reset();
+  Circuit cc = c;
+  std::vector<uint32_t> pidx(c.gates.size(), 0);
+  uint32_t np = 0;
+  for (size_t k = 0; k < cc.gates.size(); k++)
+    if (is_rotation(cc.gates[k].kind)) {
+      pidx[k] = np++;
+      cc.gates[k].np = std::max<uint8_t>(cc.gates[k].np, 1);
+    }
+  Impl::CoPlan cp = Impl::costate_tile_plan(h, s->n);
+  if (cp.tiles.empty()) return false;
+  std::vector<size_t> rot_index, unused;
+  const std::vector<Impl::APass> fwd = Impl::adjoint_tile_schedule(cc, unused, true);
+  const std::vector<Impl::APass> bwd = Impl::adjoint_tile_schedule(cc, rot_index, false);
+  std::vector<size_t> foff(fwd.size()), boff(bwd.size());
+  size_t total = 0;
+  for (size_t i = 0; i < fwd.size(); i++) {
+    foff[i] = total;
+    total += Impl::adj_pass_bytes(cc, fwd[i]);
+  }
+  for (size_t i = 0; i < bwd.size(); i++) {
+    boff[i] = total;
+    total += Impl::adj_pass_bytes(cc, bwd[i]);
+  }
+  std::unique_ptr<Impl::DevProg> d(new Impl::DevProg);
+  d->shape = c.gates;
+  d->h = h;
+  check(hipMalloc(reinterpret_cast<void **>(&d->arena), std::max<size_t>(total, 256)),
+        "hipMalloc(device program)");
+  std::vector<uint8_t> host(std::max<size_t>(total, 256));
+  std::vector<AnglePatch> patches;
+  auto emit = [&](const Impl::APass &ps, size_t off, bool forward, AdjTileArgs &a) {
+    const size_t before = patches.size();
+    s->emit_adj_tile(cc, ps, host.data() + off, d->arena + off, a, forward, &patches);
+    for (size_t k = before; k < patches.size(); k++) {
+      patches[k].off += uint32_t(off);
+      patches[k].param = pidx[patches[k].param];
+    }
+  };
+  for (size_t i = 0; i < fwd.size(); i++) {
+    AdjTileArgs a{};
+    emit(fwd[i], foff[i], true, a);
+    d->fwd.push_back(a);
+  }
+  for (size_t i = 0; i < bwd.size(); i++) {
+    AdjTileArgs a{};
+    emit(bwd[i], boff[i], false, a);
+    d->bwd.push_back(a);
+    d->bwd_rot0.push_back(bwd[i].rot0);
+  }
+  check(hipMemcpy(d->arena, host.data(), host.size(), hipMemcpyHostToDevice), "device program");
+  d->npatch = uint32_t(patches.size());
+  d->patch = device_vector(patches, "device program patches");
+  for (size_t k = 0; k < cc.gates.size(); k++)
+    if (is_rotation(cc.gates[k].kind)) d->nrot++;
+  std::vector<uint32_t> rgate(std::max<uint32_t>(d->nrot, 1), 0);
+  for (size_t k = 0; k < cc.gates.size(); k++)
+    if (is_rotation(cc.gates[k].kind))
+      rgate[rot_index[k]] = pidx[k] | (cc.gates[k].kind == K_RY ? 0x80000000u : 0u);
+  d->rgate = device_vector(rgate, "device program rotation map");
+  d->inst = device_vector(std::vector<uint32_t>{0u}, "device program instance");
+  check(hipMalloc(reinterpret_cast<void **>(&d->eslot), 2 * sizeof(double)), "hipMalloc");
+  check(hipMalloc(reinterpret_cast<void **>(&d->rslots),
+                  std::max<size_t>(d->nrot, 1) * 2 * sizeof(double)),
+        "hipMalloc");
+  d->co = cp.tiles;
+  for (const Impl::AccumLaunch &L : cp.generic) d->co_generic.push_back(L.g);
+  s->dev_prog = std::move(d);
+  return true;
+}
+
 void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &h,
                                            const double *params, size_t nparams,
                                            double *out_energy, double *out_grad,
@@ -2926,6 +3112,90 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
   Impl *s = impl_.get();
   s->use();
   hipStream_t cs = static_cast<hipStream_t>(caller_stream);
+  {
+    size_t nrot = 0;
+    for (const Gate &g : c.gates) nrot += is_rotation(g.kind) ? 1 : 0;
+    if (nrot != nparams)
+      fail("gradient: " + std::to_string(nparams) + " parameters given, circuit has " +
+           std::to_string(nrot) + " rotations");
+    for (const Gate &g : c.gates)
+      if (g.np > 0 && !is_rotation(g.kind))
+        fail(std::string("gradient: ") + kind_name(g.kind) +
+             " carries a parameter but has no native generator; only rx, ry and rz are "
+             "differentiable");
+    check_circuit(c);
+    for (const Term &t : h)
+      if (!t.p.valid(s->n)) fail("gradient: invalid Pauli term");
+  }
+  const bool fresh = !s->dev_prog || !same_shape(s->dev_prog->shape, c.gates) ||
+                     !same_hamiltonian(s->dev_prog->h, h);
+  if (fresh || s->launches != s->dev_launches) {
+    // Other work on the simulator's stream (or the compile below) comes first.
+    check(hipStreamSynchronize(s->stream), "GPU error");
+  }
+  if (c.n == s->n && compile_device_program(c, h)) {
+    if (!s->costate) {
+      const uint64_t bytes = s->dim * 8ull;
+      if (bytes > free_memory()) fail("gradient: co-state exceeds free device memory");
+      if (hipMalloc(reinterpret_cast<void **>(&s->costate), bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        s->costate = nullptr;
+        fail("gradient: co-state allocation failed");
+      }
+    }
+    s->grow(s->adj_partials, s->adj_partials_cap,
+            std::max(size_t(kAdjMaxRot) * kAdjBlocks, size_t(kAdjTileMaxRot) * kAdjTileBlocks) *
+                2,
+            "hipMalloc(adjoint partials)");
+    Impl::DevProg &d = *s->dev_prog;
+    uint64_t n_launch = 0;
+    launch_patch_angles(cs, d.arena, d.patch, d.npatch, params);
+    launch_fill_basis(cs, s->state, s->dim);
+    n_launch += 2;
+    for (AdjTileArgs &a : d.fwd) {
+      a.psi = s->state;
+      a.lam = s->state;
+      launch_fwd_tile(cs, a);
+      n_launch++;
+    }
+    bool fused = false;
+    for (size_t k = 0; k < d.co.size(); k++) {
+      CoTileArgs &a = d.co[k];
+      a.psi = s->state;
+      a.lam = s->costate;
+      const bool fuse = k + 1 == d.co.size() && d.co_generic.empty();
+      launch_costate_tile(cs, a, s->adj_partials, fuse ? d.eslot : nullptr);
+      fused = fused || fuse;
+      n_launch++;
+    }
+    for (AccumArgs &g : d.co_generic) {
+      g.psi = s->state;
+      g.lam = s->costate;
+      g.dim = s->dim;
+      launch_pauli_accum(cs, g);
+      n_launch++;
+    }
+    if (!fused) {
+      launch_pauli(cs, s->state, s->costate, s->dim, 0, 0, s->partials, d.eslot);
+      n_launch++;
+    }
+    if (out_grad) {
+      for (size_t j = 0; j < d.bwd.size(); j++) {
+        AdjTileArgs &a = d.bwd[j];
+        a.psi = s->state;
+        a.lam = s->costate;
+        launch_adj_tile(cs, a, s->adj_partials, d.rslots + 2 * d.bwd_rot0[j]);
+        n_launch++;
+      }
+    }
+    launch_batched_results(cs, d.eslot, d.rslots, 1, out_grad ? d.nrot : 0, d.rgate, d.inst,
+                           d.inst, out_energy, out_grad);
+    check(hipGetLastError(), "device gradient launch");
+    s->launches += n_launch + 1;
+    s->dispatches += n_launch + 1;
+    s->dev_launches = s->launches;
+    return;
+  }
   hipEvent_t ev;
   check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
   struct EventGuard {
@@ -3021,14 +3291,6 @@ bool same_structure(const Plan &x, const Plan &y) {
   return true;
 }
 
-template <typename T>
-T *device_copy(const std::vector<T> &v, const char *what) {
-  T *p = nullptr;
-  check(hipMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(v.size(), 1) * sizeof(T)), what);
-  if (!v.empty())
-    check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
-  return p;
-}
 }  // namespace
 
 struct BatchedAdjoint::Data {
@@ -3300,9 +3562,9 @@ std::unique_ptr<BatchedAdjoint> Simulator::compile_batched(
   for (size_t k = 0; k < c0.gates.size(); k++)
     if (is_rotation(c0.gates[k].kind))
       rgate[rot_index[k]] = uint32_t(k) | (c0.gates[k].kind == K_RY ? 0x80000000u : 0u);
-  d.rgate = device_copy(rgate, "batched rotation map");
-  d.inst = device_copy(inst, "batched instances");
-  d.gbase = device_copy(gbase, "batched gradient bases");
+  d.rgate = device_vector(rgate, "batched rotation map");
+  d.inst = device_vector(inst, "batched instances");
+  d.gbase = device_vector(gbase, "batched gradient bases");
   size_t rblocks = pauli_batched_blocks(d.dim);
   for (const BatchedAccum &b : d.costate)
     if (b.cotile) rblocks = std::max<size_t>(rblocks, costate_tile_blocks(b.c.lay));

@@ -13,9 +13,11 @@
 call runs the fused forward pass, the co-state build and the backward
 kernels, and stores dE/dtheta for the backward call; otherwise (or through
 `adjoint_energy` with gradients disabled) it runs the forward pass and the
-energy reductions only. The call is host-synchronous: the angles are read
-back to build the plan, so it waits for the caller's stream and cannot be
-captured in a CUDA/HIP graph. Tensors must be contiguous float64 on the
+energy reductions only. From 12 qubits the first call for a circuit template
+compiles a device program and later calls only launch kernels on the current
+stream (angles are written on the device from theta), so they can be captured
+in a CUDA/HIP graph; below 12 qubits the angles are read back to the host and
+the call waits for the stream. Tensors must be contiguous float64 on the
 simulator's device.
 """
 from __future__ import annotations

@@ -112,3 +112,89 @@ def test_energy_only_when_no_gradient_is_needed():
         t_energy = time.perf_counter() - t0
     print(f"energy and gradient {t_full / 5 * 1e3:.2f} ms, energy only {t_energy / 5 * 1e3:.2f} ms")
     assert t_energy < t_full
+
+
+def _with_angles(gates, theta):
+    """The template with its RX/RY/RZ angles replaced, in circuit order."""
+    it = iter(theta)
+    return [(k, q, (next(it),)) if k in (C.RX, C.RY, C.RZ) else (k, q, p) for k, q, p in gates]
+
+
+@pytest.mark.parametrize("name,n", (("hea", 13), ("qaoa", 15), ("mixed", 16)))
+def test_device_program_matches_host_path_and_oracle(name, n):
+    """From 12 qubits the device entry runs a program compiled once per
+    circuit template (tile passes; rotation matrices written on the device
+    from the parameter tensor): against the host path on the same angles and
+    the complex128 adjoint, twice with different angles (the second call
+    reuses the program)."""
+    from qmetal_amd.torch_adjoint import energy_and_grad
+    import test_gpu_passes as TP
+    r = C.SplitMix64(40 + n)
+    if name == "hea":
+        gates = C.hardware_efficient(n, 3, seed=9)
+        terms = C.tfim_ring_hamiltonian(n)
+    elif name == "qaoa":
+        gates = C.qaoa_ring(n, 3, [0.2] * 3, [0.3] * 3)
+        terms = [(1.0, 0, 0, (1 << q) | (1 << ((q + 1) % n))) for q in range(n)]
+    else:
+        gates = TP.tile_adjoint_stream(n, 60, 77 + n)
+        terms = TP.random_hamiltonian(n, 30, 5 + n)
+    nrot = sum(1 for k, _, _ in gates if k in (C.RX, C.RY, C.RZ))
+    circ = C.to_circuit(n, gates)
+    ham = C.to_hamiltonian(terms)
+    with qm.Simulator(n) as sim:
+        for trial in range(2):
+            th = [r.uniform(-3.2, 3.2) for _ in range(nrot)]
+            theta = torch.tensor(th, dtype=torch.float64, device="cuda")
+            e_d, g_d = energy_and_grad(sim, circ, ham, theta)
+            torch.cuda.synchronize()
+            concrete = _with_angles(gates, th)
+            e_h, g_h = sim.energy_and_gradient(C.to_circuit(n, concrete), ham)
+            where = [k for k, (kind, _, _) in enumerate(gates) if kind in (C.RX, C.RY, C.RZ)]
+            g_h = np.array([g_h[k] for k in where])
+            e_o, g_o = O.adjoint(n, concrete, terms)
+            g_o = np.array([g_o[k] for k in where])
+            gd = g_d.cpu().numpy()
+            print(f"{name} n={n} trial {trial}: |dev - host| {np.abs(gd - g_h).max():.2e}, "
+                  f"|dev - oracle| {np.abs(gd - g_o).max():.2e}")
+            assert abs(e_d.item() - e_o) <= 1e-5 * max(1.0, abs(e_o))
+            assert abs(e_d.item() - e_h) <= 1e-5 * max(1.0, abs(e_o))
+            assert np.abs(gd - g_o).max() <= 2e-5
+            assert np.abs(gd - g_h).max() <= 2e-5
+
+
+def test_device_program_under_graph_capture():
+    """The device entry launches only kernels on the caller's stream after its
+    first call, so it can be captured in a graph: replays with new angles in
+    the captured parameter tensor equal eager calls bit for bit (eager work and
+    synchronisation between the replays)."""
+    from qmetal_amd.torch_adjoint import energy_and_grad
+    n, depth = 14, 2
+    P = 2 * n * depth
+    rng = C.SplitMix64(21)
+    circ = C.to_circuit(n, hea_template(n, depth, [0.1] * P))
+    ham = C.to_hamiltonian(C.tfim_ring_hamiltonian(n))
+    with qm.Simulator(n) as sim:
+        theta = torch.tensor([rng.uniform(0, 6.2) for _ in range(P)], dtype=torch.float64,
+                             device="cuda")
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                energy_and_grad(sim, circ, ham, theta)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            e_g, g_g = energy_and_grad(sim, circ, ham, theta)
+        for trial in range(4):
+            theta.copy_(torch.tensor([rng.uniform(0, 6.2) for _ in range(P)],
+                                     dtype=torch.float64, device="cuda"))
+            graph.replay()
+            torch.cuda.synchronize()
+            e_r, g_r = e_g.clone(), g_g.clone()
+            e_e, g_e = energy_and_grad(sim, circ, ham, theta)
+            torch.cuda.synchronize()
+            print(f"replay {trial}: energy {e_r.item():.8f} eager {e_e.item():.8f}")
+            assert e_r.item() == e_e.item()
+            assert torch.equal(g_r, g_e)
