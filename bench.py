@@ -232,6 +232,49 @@ def ref_cpu_n30(n_gpu: int):
     return dt, dt * scale
 
 
+def scale_sweep(qm, C, nmin: int, nmax: int, emit):
+    """The reference's scale bench (bench/scale.mm:30-90) on this build: for
+    every n from nmin to nmax, GHZ (norm, <Z_0>, <Z_0 Z_(n-1)>, 4096 shots on
+    the two poles) and QFT from |0...0> (norm, max |<Z_q>| over up to eight
+    qubits), each with the allocation time and the run time, one JSON line
+    per circuit and width; `ok` applies the reference's thresholds."""
+    for kind in ("ghz", "qft"):
+        for n in range(nmin, nmax + 1):
+            gates = C.ghz(n) if kind == "ghz" else C.qft(n)
+            c = C.to_circuit(n, gates)
+            t0 = time.perf_counter()
+            s = qm.Simulator(n)
+            s.synchronize()
+            t_alloc = time.perf_counter() - t0
+            try:
+                t0 = time.perf_counter()
+                s.run(c)
+                s.synchronize()
+                t_run = time.perf_counter() - t0
+                norm = s.norm()
+                rec = {"config": f"scale_{kind}_n{n}", "gib": (8 << n) / 2**30,
+                       "alloc_s": round(t_alloc, 3), "run_s": round(t_run, 4),
+                       "gates": len(gates), "norm": norm}
+                if kind == "ghz":
+                    z0 = s.expectation_pauli(z=1)
+                    zz = s.expectation_pauli(z=1 | (1 << (n - 1)))
+                    shots = s.sample(4096, 7)
+                    ones = sum(1 for v in shots if v == (1 << n) - 1)
+                    zeros = sum(1 for v in shots if v == 0)
+                    other = len(shots) - ones - zeros
+                    rec.update({"z0": z0, "z0_zlast": zz, "zeros": zeros, "ones": ones,
+                                "other": other,
+                                "ok": abs(norm - 1) < 1e-3 and abs(z0) < 1e-3 and
+                                abs(zz - 1) < 1e-3 and other == 0})
+                else:
+                    step = n // 8 if n > 8 else 1
+                    wz = max(abs(s.expectation_pauli(z=1 << q)) for q in range(0, n, step))
+                    rec.update({"max_abs_z_q": wz, "ok": abs(norm - 1) < 1e-2 and wz < 1e-2})
+                emit(rec)
+            finally:
+                s.close()
+
+
 def suite(qm, C, emit):
     """Every configuration of the measurement plan, one JSON object each:
     GHZ n=20, GHZ n=30..33, QFT scale sweep (merged passes vs one launch per plan op vs the
@@ -687,6 +730,8 @@ def main() -> int:
                     help="skip the reference oracle's n=30 arm of the CPU baseline")
     ap.add_argument("--suite", action="store_true",
                     help="measure every configuration (one JSON line each) instead of the headline")
+    ap.add_argument("--scale", type=int, nargs=2, default=None, metavar=("NMIN", "NMAX"),
+                    help="the reference's scale sweep: GHZ and QFT at every width with invariants")
     ap.add_argument("--frozen", type=int, default=0, metavar="N",
                     help="run the reference's fusion_bench / ablation protocols on its frozen "
                          "benchmark set at n=N (24, 26 or 33) instead of the headline")
@@ -730,6 +775,12 @@ def main() -> int:
     import qmetal_amd as qm
     from qmetal_amd import circuits as C
 
+    if args.scale:
+        print(f"# device {qm.device_name()}  scale {args.scale[0]}..{args.scale[1]}",
+              file=sys.stderr, flush=True)
+        scale_sweep(qm, C, args.scale[0], args.scale[1],
+                    lambda rec: print(json.dumps(rec), flush=True))
+        return 0
     if args.suite:
         print(f"# device {qm.device_name()}  suite", file=sys.stderr, flush=True)
         suite(qm, C, lambda rec: print(json.dumps(rec), flush=True))

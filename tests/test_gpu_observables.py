@@ -260,3 +260,27 @@ def test_gradient_hea_n12_against_oracle_adjoint():
     e_ref, g_ref = O.adjoint(n, gates, terms)
     assert abs(e - e_ref) <= 1e-5
     assert np.abs(np.array(g) - g_ref).max() <= 1e-5
+
+
+def test_scale_sweep_invariants_every_width():
+    """The reference's scale sweep (bench/scale.mm:30-90, results/scale_n33.txt)
+    at every width from 20 to 34, one allocation each: GHZ norm 1, <Z_0> = 0,
+    <Z_0 Z_(n-1)> = 1, 4096 shots only on the two poles; QFT|0...0> norm 1 and
+    <Z_q> = 0 on every qubit (uniform image). Bounds 1e-5, tighter than the
+    reference's 1e-3 / 1e-2."""
+    for n in range(20, 35):
+        top = (1 << n) - 1
+        with sim_with(n, C.ghz(n)) as s:
+            norm = s.norm()
+            z0 = s.expectation_pauli(z=1)
+            zz = s.expectation_pauli(z=1 | (1 << (n - 1)))
+            shots = s.sample(4096, 7)
+            s.reset()
+            s.run(C.to_circuit(n, C.qft(n)))
+            qnorm = s.norm()
+            qz = max(abs(s.expectation_pauli(z=1 << q)) for q in range(n))
+        print(f"n={n}: GHZ norm {norm:.8f} <Z0> {z0:+.1e} <Z0 Zn> {zz:.8f}; "
+              f"QFT norm {qnorm:.8f} max|<Zq>| {qz:.1e}", flush=True)
+        assert abs(norm - 1) <= 1e-5 and abs(z0) <= 1e-5 and abs(zz - 1) <= 1e-5
+        assert set(shots) <= {0, top} and len(set(shots)) == 2
+        assert abs(qnorm - 1) <= 1e-5 and qz <= 1e-5
