@@ -1308,10 +1308,10 @@ void k_adj_tile(AdjTileArgs a, double *part) {
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t inst = blockIdx.y;
   float2 *const psi = a.psi + inst * a.bstate;
-  float2 *const lam = a.lam + inst * a.bstate;
+  float2 *const lam = ONE ? psi : a.lam + inst * a.bstate;
   const const_ptr<AdjSeg> segs = as_const(bshift(a.segs, inst * a.barena));
   const const_ptr<AdjOp> ops = as_const(bshift(a.ops, inst * a.barena));
-  part += size_t(inst) * kAdjTileMaxRot * gridDim.x * 2;
+  if (!ONE) part += size_t(inst) * kAdjTileMaxRot * gridDim.x * 2;
   const uint32_t lane_bytes = lane * 16u;
   // Load arrangement: lane bit j on tile bit j + 1 (wire j + 1), wave bit w
   // on tile bit 7 + w (hpos[w]), register bit k on tile bit 9 + k

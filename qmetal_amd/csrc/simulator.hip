@@ -1606,6 +1606,7 @@ struct Simulator::Impl {
       auto op = [&](int lt, const cd *m, const cd *malt, uint32_t cm, uint64_t gmask,
                     uint64_t pmask) -> AdjOp & {
         if (lt < 0) fail("internal: adjoint tile target not local");
+        if (no >= 3 * ps.units.size()) fail("internal: adjoint tile op count");
         AdjOp &o = ops[no++];
         o = AdjOp{};
         o.lbit = uint32_t(lt);
