@@ -1114,35 +1114,31 @@ __device__ __forceinline__ void acc_cj_fma(double &re, double &im, float2 x, flo
   }
 }
 
-// v +/-= part COMP (0 real, 1 imaginary) of conj(x) y, in FP64 with fused
-// multiply-adds.
-// The conversions are volatile statements, so they stay in program order:
-// otherwise the compiler converts a whole tile ahead and holds 64 doubles.
+// a +/-= part COMP (0 real, 1 imaginary) of conj(x) y with FP32 fused
+// multiply-adds (the negations are free source modifiers).
 template <bool NEG, int COMP>
-__device__ __forceinline__ void acc_part(double &v, float2 x, float2 y) {
-  double xr, xi, yr, yi;
-  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(xr) : "v"(x.x));
-  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(xi) : "v"(x.y));
-  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(yr) : "v"(y.x));
-  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(yi) : "v"(y.y));
+__device__ __forceinline__ void acc_part(float &a, float2 x, float2 y) {
   if (COMP == 0) {
-    v = fma(NEG ? -xr : xr, yr, v);
-    v = fma(NEG ? -xi : xi, yi, v);
+    a = fmaf(NEG ? -x.x : x.x, y.x, a);
+    a = fmaf(NEG ? -x.y : x.y, y.y, a);
   } else {
-    v = fma(NEG ? -xr : xr, yi, v);
-    v = fma(NEG ? xi : -xi, yr, v);
+    a = fmaf(NEG ? -x.x : x.x, y.y, a);
+    a = fmaf(NEG ? x.y : -x.y, y.x, a);
   }
 }
 
 // Part COMP of <lam|P|psi> over a 16-amplitude tile for P = X / Y / Z on
-// local bit LB (axis 0..2; Y before the i^ny factor, as tile_overlap).
+// local bit LB (axis 0..2; Y before the i^ny factor, as tile_overlap), into
+// two FP32 partial sums (even and odd pairs).
 template <int LB, int COMP>
 __device__ __forceinline__ void tile_overlap16(const float2 (&x)[16], const float2 (&y)[16],
-                                               uint32_t axis, double &v) {
+                                               uint32_t axis, float (&a)[2]) {
+  int k = 0;
 #pragma unroll
   for (int l = 0; l < 16; l++) {
     if (l & (1 << LB)) continue;
     const int h = l | (1 << LB);
+    float &v = a[k++ & 1];
     if (axis == 0) {
       acc_part<false, COMP>(v, y[l], x[h]);
       acc_part<false, COMP>(v, y[h], x[l]);
@@ -1160,39 +1156,43 @@ __device__ __forceinline__ void tile_overlap16(const float2 (&x)[16], const floa
 // LB of l); LA = LB = -1: s = 1.
 template <int LA, int LB, int COMP>
 __device__ __forceinline__ void tile_overlap_parity(const float2 (&x)[16], const float2 (&y)[16],
-                                                    double &v) {
+                                                    float (&a)[2]) {
 #pragma unroll
   for (int l = 0; l < 16; l++) {
     const int par = (LA >= 0 ? (l >> (LA < 0 ? 0 : LA)) & 1 : 0) ^
                     (LB >= 0 ? (l >> (LB < 0 ? 0 : LB)) & 1 : 0);
-    if (par) acc_part<true, COMP>(v, y[l], x[l]);
-    else acc_part<false, COMP>(v, y[l], x[l]);
+    if (par) acc_part<true, COMP>(a[l & 1], y[l], x[l]);
+    else acc_part<false, COMP>(a[l & 1], y[l], x[l]);
   }
 }
 
+// The thread's part of one overlap: FP32 over its 16 amplitudes (at most 32
+// products in two sums, relative error ~ 16 * 2^-24 of each sum), FP64 from
+// there on (lanes, waves, workgroups, the finish).
 template <int COMP>
-__device__ __forceinline__ void adj_op_overlap(const float2 (&x)[16], const float2 (&y)[16],
-                                               uint32_t axis, uint32_t lb, uint32_t lb2,
-                                               double &v) {
+__device__ __forceinline__ double adj_op_overlap(const float2 (&x)[16], const float2 (&y)[16],
+                                                 uint32_t axis, uint32_t lb, uint32_t lb2) {
+  float a[2] = {0.0f, 0.0f};
   if (axis <= 2) {
     switch (lb) {
-      case 0: tile_overlap16<0, COMP>(x, y, axis, v); break;
-      case 1: tile_overlap16<1, COMP>(x, y, axis, v); break;
-      case 2: tile_overlap16<2, COMP>(x, y, axis, v); break;
-      default: tile_overlap16<3, COMP>(x, y, axis, v); break;
+      case 0: tile_overlap16<0, COMP>(x, y, axis, a); break;
+      case 1: tile_overlap16<1, COMP>(x, y, axis, a); break;
+      case 2: tile_overlap16<2, COMP>(x, y, axis, a); break;
+      default: tile_overlap16<3, COMP>(x, y, axis, a); break;
     }
   } else if (axis == 3) {
-    tile_overlap_parity<-1, -1, COMP>(x, y, v);
+    tile_overlap_parity<-1, -1, COMP>(x, y, a);
   } else {
     switch (lb * 4 + lb2) {  // lb < lb2
-      case 1: tile_overlap_parity<0, 1, COMP>(x, y, v); break;
-      case 2: tile_overlap_parity<0, 2, COMP>(x, y, v); break;
-      case 3: tile_overlap_parity<0, 3, COMP>(x, y, v); break;
-      case 6: tile_overlap_parity<1, 2, COMP>(x, y, v); break;
-      case 7: tile_overlap_parity<1, 3, COMP>(x, y, v); break;
-      default: tile_overlap_parity<2, 3, COMP>(x, y, v); break;
+      case 1: tile_overlap_parity<0, 1, COMP>(x, y, a); break;
+      case 2: tile_overlap_parity<0, 2, COMP>(x, y, a); break;
+      case 3: tile_overlap_parity<0, 3, COMP>(x, y, a); break;
+      case 6: tile_overlap_parity<1, 2, COMP>(x, y, a); break;
+      case 7: tile_overlap_parity<1, 3, COMP>(x, y, a); break;
+      default: tile_overlap_parity<2, 3, COMP>(x, y, a); break;
     }
   }
+  return double(a[0]) + double(a[1]);
 }
 
 // Moves 16 amplitudes per thread (8 slots) between two arrangements, as
@@ -1268,9 +1268,8 @@ __device__ __forceinline__ void adj_tile_ops(float2 (&x)[16], float2 (&y)[16],
     pin(h);
     const bool odd = (__builtin_popcountll(base & h.pmask) & 1) != 0;
     if (!ONE && h.type == 1) {
-      double v = 0.0;
-      if (h.comp) adj_op_overlap<1>(x, y, h.axis, h.lbit, h.lbit2, v);
-      else adj_op_overlap<0>(x, y, h.axis, h.lbit, h.lbit2, v);
+      double v = h.comp ? adj_op_overlap<1>(x, y, h.axis, h.lbit, h.lbit2)
+                        : adj_op_overlap<0>(x, y, h.axis, h.lbit, h.lbit2);
       v = __ockl_wfred_add_f64(odd ? -v : v);
       if (lane == 0) acc[(h.rslot * (kAdjTileThreads / 64) + wave) * 2 + h.comp] += v;
     }
