@@ -506,6 +506,9 @@ struct AnglePatch {
   uint32_t code;
   uint32_t pad;
 };
+// Patch code flag: the target is the 8 floats of a PassOp's matrix (a
+// merged-pass slot) rather than an AdjOp's m and malt.
+constexpr uint32_t kPatchPassOp = 512;
 void launch_patch_angles(hipStream_t s, uint8_t *arena, const AnglePatch *p, uint32_t count,
                          const double *params);
 

@@ -1981,6 +1981,11 @@ __global__ __launch_bounds__(kT) void k_patch_angles(uint8_t *arena, const Angle
       break;
     }
   }
+  if (q.code & kPatchPassOp) {
+    float *pm = reinterpret_cast<float *>(arena + q.off);
+    for (int k = 0; k < 8; k++) pm[k] = float(m[k]);
+    return;
+  }
   AdjOp *o = reinterpret_cast<AdjOp *>(arena + q.off);
   for (int k = 0; k < 8; k++) {
     o->m[k] = float(m[k]);

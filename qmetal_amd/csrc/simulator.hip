@@ -522,10 +522,13 @@ struct Simulator::Impl {
       const PlanOp &op = p.ops[ks[kk]];
       float m8[8];
       switch (op.kind) {
-        case OpKind::Dense1q:
+        case OpKind::Dense1q: {
           narrow(op.m, m8);
-          slot(op.q[0], m8);
+          PassOp &o = slot(op.q[0], m8);
+          if (slot_log)
+            slot_log->emplace_back(size_t(reinterpret_cast<uint8_t *>(&o) - staging), ks[kk]);
           break;
+        }
         case OpKind::Group:
           for (size_t w = 0; w < op.wires.size(); w++) {
             if (wmask[kk] && !((wmask[kk] >> op.wires[w]) & 1)) continue;
@@ -1196,6 +1199,9 @@ struct Simulator::Impl {
   bool merge = true;
   // Set while compiling a batched program (compile_batched).
   bool batch_compile = false;
+  // While set, emit_ops records every Dense1q slot it writes: the slot's
+  // byte offset from `staging` and its plan op (compile_device_program).
+  std::vector<std::pair<size_t, uint32_t>> *slot_log = nullptr;
 
   // Device-resident gradient program (energy_and_gradient_device): a
   // circuit template's forward and backward tile passes and co-state tiles,
@@ -1209,6 +1215,9 @@ struct Simulator::Impl {
     AnglePatch *patch = nullptr;
     uint32_t npatch = 0;
     std::vector<AdjTileArgs> fwd, bwd;
+    // The forward sweep as merged passes of the template's unfused plan
+    // (when empty, `fwd` holds it as single-state tile passes).
+    std::vector<PassLaunch> mfwd;
     std::vector<size_t> bwd_rot0;
     std::vector<CoTileArgs> co;
     std::vector<AccumArgs> co_generic;
@@ -3046,11 +3055,57 @@ bool Simulator::compile_device_program(const Circuit &c, const Hamiltonian &h) {
     }
   Impl::CoPlan cp = Impl::costate_tile_plan(h, s->n);
   if (cp.tiles.empty()) return false;
+  // Forward sweep: the template's plan without fusion (one op per gate, so
+  // the ops and their matrix structure do not depend on the angles) in the
+  // executor's merged passes. Every rotation is a Dense1q slot whose section
+  // is fixed by its kind (RY real, RZ diagonal, RX real diagonal with an
+  // imaginary off-diagonal, as at the placeholder angle 1) and whose 2x2 is
+  // patched from the parameters. A schedule with chain-shaped passes keeps
+  // the single-state tile passes.
+  Circuit pc = cc;
+  for (Gate &g : pc.gates)
+    if (is_rotation(g.kind)) g.p[0] = 1.0;
+  const Plan fplan = build_plan(pc, FuseOpts::none());
+  std::vector<Pass> mpasses;
+  bool merged_fwd = fplan.ops.size() == pc.gates.size();
+  for (size_t k = 0; merged_fwd && k < pc.gates.size(); k++)
+    if (is_rotation(pc.gates[k].kind))
+      merged_fwd = fplan.ops[k].kind == OpKind::Dense1q && fplan.ops[k].q[0] == pc.gates[k].q[0];
+  if (merged_fwd) {
+    for (Pass &ps : schedule_passes(fplan)) {
+      if (ps.wide || ps.low) {
+        merged_fwd = false;
+        break;
+      }
+      if (!ps.merged) {
+        const PlanOp &op = fplan.ops[ps.ops[0]];
+        if (op.kind == OpKind::Diagonal && op.masks.empty()) continue;  // no work
+        if (!pass_mergeable(op)) {
+          merged_fwd = false;
+          break;
+        }
+        ps.merged = true;
+        ps.reg = pass_wires(op);
+        ps.fixed = pass_fixed(op);
+        if (ps.reg.size() > kPassMaxReg) {
+          merged_fwd = false;
+          break;
+        }
+      }
+      mpasses.push_back(std::move(ps));
+    }
+  }
+  if (!merged_fwd) mpasses.clear();
   std::vector<size_t> rot_index, unused;
-  const std::vector<Impl::APass> fwd = Impl::adjoint_tile_schedule(cc, unused, true);
+  const std::vector<Impl::APass> fwd =
+      merged_fwd ? std::vector<Impl::APass>{} : Impl::adjoint_tile_schedule(cc, unused, true);
   const std::vector<Impl::APass> bwd = Impl::adjoint_tile_schedule(cc, rot_index, false);
-  std::vector<size_t> foff(fwd.size()), boff(bwd.size());
+  std::vector<size_t> moff(mpasses.size()), foff(fwd.size()), boff(bwd.size());
   size_t total = 0;
+  for (size_t k = 0; k < mpasses.size(); k++) {
+    moff[k] = total;
+    total = (total + s->pass_param_bytes(fplan, mpasses[k]) + 255) & ~size_t(255);
+  }
   for (size_t i = 0; i < fwd.size(); i++) {
     foff[i] = total;
     total += Impl::adj_pass_bytes(cc, fwd[i]);
@@ -3074,6 +3129,41 @@ bool Simulator::compile_device_program(const Circuit &c, const Hamiltonian &h) {
       patches[k].param = pidx[patches[k].param];
     }
   };
+  if (merged_fwd) {
+    // Through the executor's own pass preparation, pointed at this
+    // program's host block and arena; chain kernels stay off (their real /
+    // complex choice would depend on the angles).
+    std::vector<std::pair<size_t, uint32_t>> slots;
+    struct Swap {
+      Impl *s;
+      uint8_t *st, *ar;
+      ~Swap() {
+        s->staging = st;
+        s->arena = ar;
+        s->batch_compile = false;
+        s->slot_log = nullptr;
+      }
+    } swap{s, s->staging, s->arena};
+    s->staging = host.data();
+    s->arena = d->arena;
+    s->batch_compile = true;
+    s->slot_log = &slots;
+    for (size_t k = 0; k < mpasses.size(); k++) {
+      Impl::PassLaunch pl{};
+      if (mpasses[k].tile) s->prep_tile(fplan, mpasses[k], moff[k], pl);
+      else if (mpasses[k].gwide) s->prep_gwide(fplan, mpasses[k], moff[k], pl);
+      else s->prep_pass(fplan, mpasses[k], moff[k], pl);
+      if (pl.chain || pl.wide || pl.low) fail("internal: chain pass in a device program");
+      d->mfwd.push_back(pl);
+    }
+    for (const auto &e : slots) {
+      const Gate &g = cc.gates[e.second];
+      if (!is_rotation(g.kind)) continue;
+      const uint32_t rk = g.kind == K_RX ? 0u : g.kind == K_RY ? 1u : 2u;
+      patches.push_back(AnglePatch{uint32_t(e.first + offsetof(PassOp, m)), pidx[e.second],
+                                   0u | (rk << 4) | kPatchPassOp, 0});
+    }
+  }
   for (size_t i = 0; i < fwd.size(); i++) {
     AdjTileArgs a{};
     emit(fwd[i], foff[i], true, a);
@@ -3153,6 +3243,19 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
     launch_patch_angles(cs, d.arena, d.patch, d.npatch, params);
     launch_fill_basis(cs, s->state, s->dim);
     n_launch += 2;
+    for (Impl::PassLaunch &pl : d.mfwd) {
+      if (pl.tile) {
+        pl.t.a = s->state;
+        launch_tile(cs, pl.t);
+      } else if (pl.gwide) {
+        pl.g.a = s->state;
+        launch_gwide(cs, pl.chain_k - 4, pl.g);
+      } else {
+        pl.a.a = s->state;
+        launch_pass(cs, pl.nl, pl.a);
+      }
+      n_launch++;
+    }
     for (AdjTileArgs &a : d.fwd) {
       a.psi = s->state;
       a.lam = s->state;
