@@ -133,20 +133,37 @@ def test_batch_fails_closed():
 
 def test_bench_workload_against_fp64_adjoint():
     """The benchmarked workload (QAOA ring p=6, n ~ U{10..20}, the same
-    splitmix64 draws): the first 256 instances in one autotuned batch; the
-    first instance of every width 10..20 checked against the complex128
-    adjoint on the oracle (itself pinned to parameter shift): every gradient
-    entry within 1e-5 (absolute), the energy within 3e-6 relative to
-    max(1, |E|) -- the state after ~150 complex64 gates carries ~1e-6
-    relative error, which the sum of n ring terms keeps (measured: gradients
-    <= 3.3e-6, energies 1.2e-6 relative at |E| = 9.6)."""
-    inst = C.qaoa_instances(4096)[:256]
+    splitmix64 draws), all 4096 instances in one autotuned batch: every
+    instance against the one-circuit path (qm_energy_and_gradient), and the
+    first instance of every width 10..20 against the complex128 adjoint on
+    the oracle (itself pinned to parameter shift). Every gradient entry within
+    1e-5 (absolute), the energy within 3e-6 relative to max(1, |E|) -- the
+    state after ~150 complex64 gates carries ~1e-6 relative error, which the
+    sum of n ring terms keeps (measured: gradients <= 3.6e-6 against the
+    one-circuit path and <= 3.3e-6 against the oracle, energies 1.2e-6
+    relative at |E| = 9.6)."""
+    inst = C.qaoa_instances(4096)
     gates = [C.qaoa_ring(n, 6, ga, be) for n, ga, be in inst]
     terms = [C.ring_zz_hamiltonian(n) for n, _, _ in inst]
-    b = qm.Batch().prepare([C.to_circuit(n, g) for (n, _, _), g in zip(inst, gates)],
-                           [C.to_hamiltonian(h) for h in terms])
+    circs = [C.to_circuit(n, g) for (n, _, _), g in zip(inst, gates)]
+    hams = [C.to_hamiltonian(h) for h in terms]
+    b = qm.Batch().prepare(circs, hams)
     cfg = b.autotune()
     e, grads, _ = b.run()
+    sims = {n: qm.Simulator(n) for n in sorted({n for n, _, _ in inst})}
+    worst_g, worst_e = 0.0, 0.0
+    try:
+        for i, (n, _, _) in enumerate(inst):
+            e0, g0 = sims[n].energy_and_gradient(circs[i], hams[i])
+            dg = float(np.abs(np.array(g0) - grads[i]).max())
+            de = abs(e0 - e[i]) / max(1.0, abs(e0))
+            assert dg <= 1e-5 and de <= 3e-6, (i, n, dg, de)
+            worst_g, worst_e = max(worst_g, dg), max(worst_e, de)
+    finally:
+        for s in sims.values():
+            s.close()
+    print(f"all {len(inst)} instances vs the one-circuit path: gradients {worst_g:.2e}, "
+          f"energies {worst_e:.2e} relative")
     first = {}
     for i, (n, _, _) in enumerate(inst):
         first.setdefault(n, i)
