@@ -140,8 +140,8 @@ __global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
       e[i] = make_float2(cs, sn);
     }
   }
-  // Arrangement 0: lanes on bits 1..6, the wave bits on 7.., the registers
-  // on the four bits above them (register bit k on bit regpos[0][k]).
+  // Arrangement 0: lanes on bits 1..6 (1 KB runs), wave bit k on bit
+  // lanepos[0][6 + k], register bit k on bit regpos[0][k].
   uint64_t rbit[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) rbit[k] = 1ull << p.regpos[0][k];
@@ -150,7 +150,10 @@ __global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
            ((r & 8) ? rbit[3] : 0ull);
   };
   const uint32_t lane_bytes = lane * 16u;
-  const uint64_t wave_off = uint64_t(wave) << 7;
+  uint64_t wave_off = 0;
+#pragma unroll
+  for (int k = 0; k < WB; k++)
+    if ((wave >> k) & 1) wave_off |= 1ull << p.lanepos[0][6 + k];
   float4 y[16];
   if (blockIdx.x < p.lay.nthreads) {
     const uint64_t ub0 = scatter(blockIdx.x, p.lay) | p.lay.fixed | wave_off;

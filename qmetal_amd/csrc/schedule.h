@@ -72,6 +72,8 @@ constexpr uint32_t kLowMaxWire = 12;
 constexpr uint32_t kLowTwoWaveWire = 11;
 constexpr uint32_t kLowMinSteps = 5;
 constexpr uint32_t kLowMinQubits = 12;
+// From this many qubits a low pass always takes the four-wave tile.
+constexpr uint32_t kLowFourWaveQubits = 24;
 
 // Number of plan ops from `k` that form a low pass (0 if none); `wires` gets
 // the step wires (a final 0 for a step on wire 0).

@@ -908,10 +908,15 @@ struct Simulator::Impl {
     pl.bytes = pass_bytes(n, 0);
   }
 
-  // Low chain pass (LowArgs): WB wave bits (1 when the wires stay within
-  // 1..11, else 2), tile bits 0..10+WB. Arrangement 0 is the coalesced one
-  // (lanes on bits 1..6, the wave bits on 7.., registers on the four above);
-  // arrangement 1 + a holds steps 4a..4a+3 on register bits, filled up with
+  // Low chain pass (LowArgs): WB wave bits, tile bits 0..10+WB: WB = 2 when
+  // a wire reaches 12 or the state has kLowFourWaveQubits qubits or more
+  // (then a tile of 2^13 amplitudes carries wire 12 even without a step on
+  // it: QFT n=33's low pass over wires 11..0 took 26.7 ms in two-wave tiles),
+  // else 1. Arrangement 0 is the coalesced one: lanes on bits 1..6, the
+  // first four steps' wires on the registers when they are all 7 or above
+  // (so that those steps run in place), else the four wires above the wave
+  // bits, and the wave bits on the remaining tile wires of 7 and up.
+  // Arrangement 1 + a holds steps 4a..4a+3 on register bits, filled up with
   // the lowest unused wires, and lanes then the wave bits on the remaining
   // tile wires, ascending. The wire-0 step runs in the last.
   void prep_low(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
@@ -921,14 +926,26 @@ struct Simulator::Impl {
     if (S < int(kLowMinSteps) || S > int(kLowMaxWire) || ps.fixed || n < kLowMinQubits)
       fail("internal: low pass shape");
     const uint32_t wmax = *std::max_element(w.begin(), w.end());
-    const int WB = wmax <= kLowTwoWaveWire ? 1 : 2;
+    const int WB = wmax <= kLowTwoWaveWire && n < kLowFourWaveQubits ? 1 : 2;
     const uint32_t top = uint32_t(10 + WB);  // highest tile wire
     if (wmax > kLowMaxWire || top >= n) fail("internal: low pass tile");
     const int A = (S + 3) / 4;
     LowArgs &la = pl.l;
     la = LowArgs{};
-    for (int j = 0; j < 6 + WB; j++) la.lanepos[0][j] = uint8_t(1 + j);
-    for (int k = 0; k < 4; k++) la.regpos[0][k] = uint8_t(7 + WB + k);
+    {
+      uint32_t regs = 0;
+      bool first = S >= 4;
+      for (int i = 0; first && i < 4; i++) first = w[i] >= 7 && w[i] <= top;
+      for (int k = 0; k < 4; k++) {
+        la.regpos[0][k] = uint8_t(first ? w[k] : uint32_t(7 + WB + k));
+        regs |= 1u << la.regpos[0][k];
+      }
+      for (int j = 0; j < 6; j++) la.lanepos[0][j] = uint8_t(1 + j);
+      int nw = 0;
+      for (uint32_t q = 7; q <= top; q++)
+        if (!((regs >> q) & 1)) la.lanepos[0][6 + nw++] = uint8_t(q);
+      if (nw != WB) fail("internal: low pass wave bits");
+    }
     uint64_t var[kLowMaxArr] = {};
     for (int a = 0; a < A; a++) {
       uint32_t used = 1u;

@@ -479,14 +479,21 @@ def low_stream(n, rounds, seed, kind, top=11):
     a step on wire 0, each step a 2x2 (step_gate) followed by
     phase gates that all hold its wire (onto register, lane, wave and
     workgroup-uniform wires); a CX separates the rounds. With top = 12 every
-    other round holds a step on wire 12 (four-wave tiles). Every third round
-    starts with the four top wires in a scrambled order (the coalesced
-    arrangement then serves the first four steps in place)."""
+    other round holds a step on wire 12 (four-wave tiles). Rounds 1, 4, ...
+    start with four wires drawn from 7..top in any order, rounds 2, 5, ...
+    with the four top wires scrambled: the coalesced arrangement then puts
+    those wires on its registers and serves the first four steps in place."""
     r = C.SplitMix64(seed)
     gates = []
     for i in range(rounds):
         count = 5 + r.next_u64() % (top - 4)
         ws = [12] if top == 12 and i % 2 == 1 else []
+        if i % 3 == 1:
+            ws = []
+            while len(ws) < 4:
+                q = 7 + r.next_u64() % (top - 6)
+                if q not in ws:
+                    ws.append(q)
         if i % 3 == 2:
             ws = [top, top - 2, top - 1, top - 3]
         while len(ws) < count:
@@ -515,9 +522,11 @@ def low_stream(n, rounds, seed, kind, top=11):
 def test_low_passes_against_oracle_and_op_by_op():
     """Chains of 5..11 steps on wires 1..11 (and 0) over two-wave tiles, and
     of 5..12 steps on wires 1..12 over four-wave tiles (states of 13 qubits
-    or more), permuted through LDS, real and complex, with phase terms on
-    every kind of bit."""
-    for n, top in ((12, 11), (14, 11), (17, 11), (13, 12), (14, 12), (17, 12)):
+    or more; from 24 qubits every low pass takes them), permuted through LDS,
+    real and complex, with phase terms on every kind of bit."""
+    O.set_threads(8)
+    for n, top in ((12, 11), (14, 11), (17, 11), (13, 12), (14, 12), (17, 12), (24, 11),
+                   (24, 12)):
         for seed, kind in STEP_KINDS:
             prep = C.product_state_prep(n, seed=seed)
             gates = low_stream(n, 4, 300 * n + 7 * top + seed, kind, top)
@@ -539,6 +548,7 @@ def test_low_passes_against_oracle_and_op_by_op():
                   f"diff {d:.2e}, vs op-by-op {dm:.2e}")
             assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
             assert dm <= TOL_SAME
+    O.set_threads(1)
 
 
 def tile_stream(n, count, seed):
