@@ -816,6 +816,9 @@ def main() -> int:
     print(f"# device {qm.device_name()}  workload {workload}", file=sys.stderr, flush=True)
     sim = qm.Simulator(n)
     sim.set_pass_merging(not args.no_merge)
+    place_attempts, place_ratio = sim.placement
+    print(f"# state placement: {place_attempts} allocation(s), top/mid pass time ratio "
+          f"{place_ratio:.3f}", file=sys.stderr, flush=True)
     sim.run(C.to_circuit(n, C.product_state_prep(n, seed=2024)))
     qft = C.to_circuit(n, C.qft(n))
     plan_ops = qft.plan_size(True)
@@ -997,6 +1000,7 @@ def main() -> int:
         "config": {"workload": workload, "n": n, "gates": n * (n + 1) // 2,
                    "plan_ops": plan_ops, "fused": True, "launches": launches,
                    "pass_merging": not args.no_merge},
+        "state_placement": {"allocations": place_attempts, "top_mid_ratio": round(place_ratio, 4)},
         "hbm_GBps": round(hbm_step / s_per / 1e9, 1),
         "hbm_bytes_per_circuit": hbm_step,
         "hbm_bytes_source": hbm_src,

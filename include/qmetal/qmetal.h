@@ -242,6 +242,19 @@ int qm_read_timing(qm_simulator s, uint32_t *kinds, double *ms, double *bytes,
  * `cap` entries (arrays may be NULL) and setting *count to their number. */
 int qm_set_pass_merging(qm_simulator s, int enable);
 uint64_t qm_launch_count(qm_simulator s);
+
+/* Placement of the state (no counterpart in the reference). On MI355X a
+ * 64 GiB state streams passes whose tiles span rows 4 MB or more apart at
+ * 5.0 TB/s on some allocations and 5.9 TB/s on others, and which one an
+ * allocation gets is up to the allocator. For 31 to 33 qubits the
+ * constructor (and the co-state of the adjoint) therefore times one
+ * value-preserving pass over the top seven wires against one over wires
+ * 18..12, and allocates again while the first takes over 1.05 times the
+ * second, up to 4 allocations, keeping the fastest (one extra state is held
+ * at a time; QMETAL_PLACEMENT_PROBE=0 turns this off). *attempts gets the
+ * number of allocations made for the state (0 when not probed), *ratio that
+ * time ratio for the one kept. */
+int qm_state_placement(qm_simulator s, uint32_t *attempts, double *ratio);
 int qm_plan_passes(qm_plan p, uint32_t *pass_of_op, size_t *num_passes);
 int qm_plan_schedule(qm_plan p, uint32_t *entry_op, uint64_t *entry_wires,
                      uint32_t *entry_pass, size_t cap, size_t *count);

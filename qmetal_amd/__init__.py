@@ -178,6 +178,7 @@ _plan_passes = _decl("qm_plan_passes", c_int, c_void_p, _u32p, POINTER(c_size_t)
 _plan_schedule = _decl("qm_plan_schedule", c_int, c_void_p, _u32p, POINTER(c_uint64), _u32p,
                        c_size_t, POINTER(c_size_t))
 _set_pass_merging = _decl("qm_set_pass_merging", c_int, c_void_p, c_int)
+_state_placement = _decl("qm_state_placement", c_int, c_void_p, _u32p, POINTER(c_double))
 
 
 def _err() -> str:
@@ -489,6 +490,15 @@ class Simulator:
             key = format(v, f"0{n}b")
             out[key] = out.get(key, 0) + 1
         return dict(sorted(out.items(), key=lambda kv: -kv[1]))
+
+    @property
+    def placement(self) -> tuple[int, float]:
+        """(allocations made for the state, time ratio of a pass over the top
+        seven wires to one over wires 18..12 for the one kept); (0, 0.0) when
+        the placement was not probed (outside 31..33 qubits)."""
+        a, r = c_uint32(), c_double()
+        _check(_state_placement(self._h, byref(a), byref(r)))
+        return int(a.value), float(r.value)
 
     def set_pass_merging(self, enable: bool = True):
         """Merge consecutive plan ops into shared HBM passes (default on);

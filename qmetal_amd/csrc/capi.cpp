@@ -564,6 +564,14 @@ int qm_sample_noalloc(qm_simulator s, size_t shots, uint64_t seed, uint64_t *out
   return guard([&] { s->s->sample_noalloc(shots, seed, out); });
 }
 
+int qm_state_placement(qm_simulator s, uint32_t *attempts, double *ratio) {
+  if (!s || !attempts || !ratio) {
+    set_error("null argument");
+    return -1;
+  }
+  return guard([&] { s->s->placement(attempts, ratio); });
+}
+
 int qm_set_pass_merging(qm_simulator s, int enable) {
   if (!s) {
     set_error("null simulator");

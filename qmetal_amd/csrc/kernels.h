@@ -267,6 +267,10 @@ constexpr uint32_t tile_swizzle(uint32_t i) { return i ^ (((i >> 4) ^ (i >> 8)) 
 
 void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim);
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim);
+// Reads and writes back, unchanged, every amplitude of a 2^n state in tiles
+// shaped like a wide chain pass whose seven step bits are p0..p0+6 (needs
+// 7 <= p0, p0 + 7 <= n, n >= 14; otherwise launches nothing).
+void launch_placement_probe(hipStream_t s, float2 *a, uint32_t n, uint32_t p0);
 // out[k] = a[idx[k]] for k < count (indices below the state size).
 void launch_pick(hipStream_t s, const float2 *a, const uint64_t *idx, uint64_t count,
                  float2 *out);

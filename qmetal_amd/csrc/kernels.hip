@@ -721,6 +721,28 @@ __global__ __launch_bounds__(kT) void k_zero(float2 *a, uint64_t dim) {
     v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Placement probe: every amplitude read and written back times `one` (1.0,
+// unknown to the compiler, so the stores stay), over tiles shaped like a
+// wide chain pass: lanes on amplitude bits 1..6, 16 registers on bits
+// p0..p0+3, 8 waves on p0+4..p0+6 (`lay` enumerates the tiles).
+__global__ __launch_bounds__(512) void k_placement_probe(float4 *a, Layout lay, uint32_t p0,
+                                                         float one) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t woff = uint64_t(wave) << (p0 + 4);
+  for (uint64_t tile = blockIdx.x; tile < lay.nthreads; tile += gridDim.x) {
+    const uint64_t ub = scatter(tile, lay) | woff;
+    float4 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = a[((ub | (uint64_t(r) << p0)) >> 1) + lane];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const float4 x = v[r];
+      a[((ub | (uint64_t(r) << p0)) >> 1) + lane] =
+          make_float4(x.x * one, x.y * one, x.z * one, x.w * one);
+    }
+  }
+}
+
 // out[k] = a[idx[k]]; every index was checked against the state size.
 __global__ __launch_bounds__(kT) void k_pick(const float2 *a, const uint64_t *idx, uint64_t count,
                                              float2 *out) {
@@ -1745,6 +1767,21 @@ void launch_fill_basis(hipStream_t s, float2 *a, uint64_t dim) {
 
 void launch_zero(hipStream_t s, float2 *a, uint64_t dim) {
   k_zero<<<grid_for(dim / 2), kT, 0, s>>>(a, dim);
+}
+
+void launch_placement_probe(hipStream_t s, float2 *a, uint32_t n, uint32_t p0) {
+  if (n < 14 || p0 < 7 || p0 + 7 > n) return;
+  Layout L{};
+  const uint64_t tile = 0x7Full | (0x7Full << p0);
+  for (uint32_t q = 0; q < 64; q++)
+    if ((tile >> q) & 1) {
+      L.ins[L.nins] = uint8_t(q);
+      L.hm[L.nins] = ~((uint64_t(1) << q) - 1);
+      L.nins++;
+    }
+  L.nthreads = (uint64_t(1) << n) >> L.nins;
+  const unsigned grid = unsigned(std::min<uint64_t>(L.nthreads, kPassGrid));
+  k_placement_probe<<<grid, 512, 0, s>>>(reinterpret_cast<float4 *>(a), L, p0, 1.0f);
 }
 
 void launch_pick(hipStream_t s, const float2 *a, const uint64_t *idx, uint64_t count,

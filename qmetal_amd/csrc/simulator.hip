@@ -63,6 +63,22 @@ constexpr uint32_t kTimingCoState = 14;
 constexpr uint32_t kTimingOverlap = 15;
 constexpr uint32_t kTimingAdjRot = 16;
 
+// State placement. A 64 GiB state (33 qubits) streams passes over tiles whose
+// rows lie 4 MB or more apart at 5.0 TB/s on some allocations and at 5.9 TB/s
+// on others (rows closer together: 5.6 and 5.9 TB/s); which one an allocation
+// gets was not set by its size, alignment, allocation call or virtual
+// address in any trial, while successive allocations in one process
+// alternate between the two (a 128 GiB state streamed 6.0 TB/s at every
+// stride). From kPlaceMinQubits to kPlaceMaxQubits qubits a new state is
+// therefore probed -- the time of one value-preserving pass with the top
+// seven wires as steps over one with wires 18..12 -- and, while that ratio
+// is above kPlaceSlow, another allocation is made and probed, up to
+// kPlaceAttempts in all; the fastest is kept.
+constexpr uint32_t kPlaceMinQubits = 31;
+constexpr uint32_t kPlaceMaxQubits = 33;
+constexpr int kPlaceAttempts = 4;
+constexpr double kPlaceSlow = 1.05;
+
 int current_device() {
   int d = 0;
   check(hipGetDevice(&d), "hipGetDevice");
@@ -1214,6 +1230,67 @@ struct Simulator::Impl {
   }
 
   bool merge = true;
+  // Placement probe of the state: allocations made, time ratio of the one kept.
+  uint32_t place_attempts = 0;
+  double place_ratio = 0.0;
+
+  // Time of a value-preserving pass with the top seven wires as steps over
+  // one with wires 18..12 (after one warm-up pass), on this stream.
+  double placement_ratio(float2 *p) {
+    hipEvent_t e[3];
+    for (hipEvent_t &x : e) check(hipEventCreate(&x), "hipEventCreate");
+    launch_placement_probe(stream, p, n, 12);
+    check(hipEventRecord(e[0], stream), "hipEventRecord");
+    launch_placement_probe(stream, p, n, n - 7);
+    check(hipEventRecord(e[1], stream), "hipEventRecord");
+    launch_placement_probe(stream, p, n, 12);
+    check(hipEventRecord(e[2], stream), "hipEventRecord");
+    launched();
+    check(hipEventSynchronize(e[2]), "placement probe");
+    float top = 0.0f, mid = 0.0f;
+    check(hipEventElapsedTime(&top, e[0], e[1]), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&mid, e[1], e[2]), "hipEventElapsedTime");
+    for (hipEvent_t &x : e) (void)hipEventDestroy(x);
+    return mid > 0.0f ? double(top) / double(mid) : 0.0;
+  }
+
+  // A state buffer of `bytes` (nullptr if it cannot be allocated), placed as
+  // the constants above describe; QMETAL_PLACEMENT_PROBE=0 turns the probe
+  // off. A candidate is allocated while the best so far is held, so a
+  // failed candidate costs nothing.
+  float2 *alloc_state(uint64_t bytes, uint32_t *attempts, double *ratio) {
+    const char *env = getenv("QMETAL_PLACEMENT_PROBE");
+    const bool probe = n >= kPlaceMinQubits && n <= kPlaceMaxQubits && !(env && env[0] == '0');
+    *attempts = 0;
+    *ratio = 0.0;
+    float2 *best = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&best), bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    if (!probe) return best;
+    *attempts = 1;
+    *ratio = placement_ratio(best);
+    while (*ratio > kPlaceSlow && *attempts < uint32_t(kPlaceAttempts)) {
+      float2 *cand = nullptr;
+      if (free_memory() < bytes ||
+          hipMalloc(reinterpret_cast<void **>(&cand), bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        break;
+      }
+      ++*attempts;
+      const double r = placement_ratio(cand);
+      if (r < *ratio) {
+        check(hipFree(best), "hipFree");
+        best = cand;
+        *ratio = r;
+      } else {
+        check(hipFree(cand), "hipFree");
+      }
+    }
+    return best;
+  }
+
   // Set while compiling a batched program (compile_batched).
   bool batch_compile = false;
   // While set, emit_ops records every Dense1q slot it writes: the slot's
@@ -2479,11 +2556,9 @@ Simulator::Simulator(uint32_t n) : impl_(new Impl) {
          " GiB, only " + std::to_string(freeb >> 30) + " GiB of device memory free");
   check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
   check(hipEventCreateWithFlags(&s->staging_done, hipEventDisableTiming), "hipEventCreate");
-  if (hipMalloc(reinterpret_cast<void **>(&s->state), bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    s->state = nullptr;
+  s->state = s->alloc_state(bytes, &s->place_attempts, &s->place_ratio);
+  if (!s->state)
     fail("Simulator: state allocation failed (" + std::to_string(bytes >> 30) + " GiB)");
-  }
   check(hipMalloc(reinterpret_cast<void **>(&s->partials), 2 * kRedBlocks * sizeof(double)),
         "hipMalloc(partials)");
   reset();
@@ -2610,6 +2685,11 @@ void Simulator::run_plan(const Plan &p) {
 }
 
 void Simulator::set_pass_merging(bool on) { impl_->merge = on; }
+
+void Simulator::placement(uint32_t *attempts, double *ratio) const {
+  *attempts = impl_->place_attempts;
+  *ratio = impl_->place_ratio;
+}
 bool Simulator::pass_merging() const { return impl_->merge; }
 
 void Simulator::set_timing(bool on) { impl_->timing = on; }
@@ -2854,9 +2934,10 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
   if (!s->costate) {
     const uint64_t bytes = s->dim * 8ull;
     if (bytes > free_memory()) fail("gradient: co-state exceeds free device memory");
-    if (hipMalloc(reinterpret_cast<void **>(&s->costate), bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      s->costate = nullptr;
+    uint32_t tries = 0;
+    double ratio = 0.0;
+    s->costate = s->alloc_state(bytes, &tries, &ratio);
+    if (!s->costate) {
       fail("gradient: co-state allocation failed");
     }
   }
@@ -3245,9 +3326,10 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
     if (!s->costate) {
       const uint64_t bytes = s->dim * 8ull;
       if (bytes > free_memory()) fail("gradient: co-state exceeds free device memory");
-      if (hipMalloc(reinterpret_cast<void **>(&s->costate), bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        s->costate = nullptr;
+      uint32_t tries = 0;
+      double ratio = 0.0;
+      s->costate = s->alloc_state(bytes, &tries, &ratio);
+      if (!s->costate) {
         fail("gradient: co-state allocation failed");
       }
     }
