@@ -246,8 +246,8 @@ uint64_t qm_launch_count(qm_simulator s);
 /* Placement of the state (no counterpart in the reference). On MI355X a
  * 64 GiB state streams passes whose tiles span rows 4 MB or more apart at
  * 5.0 TB/s on some allocations and 5.9 TB/s on others, and which one an
- * allocation gets is up to the allocator. For 31 to 33 qubits the
- * constructor (and the co-state of the adjoint) therefore times one
+ * allocation gets is up to the allocator. For 33 qubits the constructor
+ * (and the co-state of the adjoint) therefore times one
  * value-preserving pass over the top seven wires against one over wires
  * 18..12, and allocates again while the first takes over 1.05 times the
  * second, up to 4 allocations, keeping the fastest (one extra state is held

@@ -495,7 +495,7 @@ class Simulator:
     def placement(self) -> tuple[int, float]:
         """(allocations made for the state, time ratio of a pass over the top
         seven wires to one over wires 18..12 for the one kept); (0, 0.0) when
-        the placement was not probed (outside 31..33 qubits)."""
+        the placement was not probed (states other than 33 qubits)."""
         a, r = c_uint32(), c_double()
         _check(_state_placement(self._h, byref(a), byref(r)))
         return int(a.value), float(r.value)

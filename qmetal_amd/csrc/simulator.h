@@ -73,7 +73,7 @@ class Simulator {
   // Off: one kernel launch per plan op.
   void set_pass_merging(bool on);
   bool pass_merging() const;
-  // Placement probe of the state (31..33 qubits): allocations made (0 when
+  // Placement probe of the state (33 qubits): allocations made (0 when
   // not probed) and the kept state's time ratio of a pass over the top seven
   // wires to one over wires 18..12.
   void placement(uint32_t *attempts, double *ratio) const;

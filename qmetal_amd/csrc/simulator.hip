@@ -68,13 +68,13 @@ constexpr uint32_t kTimingAdjRot = 16;
 // on others (rows closer together: 5.6 and 5.9 TB/s); which one an allocation
 // gets was not set by its size, alignment, allocation call or virtual
 // address in any trial, while successive allocations in one process
-// alternate between the two (a 128 GiB state streamed 6.0 TB/s at every
-// stride). From kPlaceMinQubits to kPlaceMaxQubits qubits a new state is
-// therefore probed -- the time of one value-preserving pass with the top
-// seven wires as steps over one with wires 18..12 -- and, while that ratio
-// is above kPlaceSlow, another allocation is made and probed, up to
+// alternate between the two (128 GiB states streamed 6.0 TB/s at every
+// stride; 16 and 32 GiB states were always of the slower kind). A 33-qubit
+// state is therefore probed -- the time of one value-preserving pass with
+// the top seven wires as steps over one with wires 18..12 -- and, while that
+// ratio is above kPlaceSlow, another allocation is made and probed, up to
 // kPlaceAttempts in all; the fastest is kept.
-constexpr uint32_t kPlaceMinQubits = 31;
+constexpr uint32_t kPlaceMinQubits = 33;
 constexpr uint32_t kPlaceMaxQubits = 33;
 constexpr int kPlaceAttempts = 4;
 constexpr double kPlaceSlow = 1.05;

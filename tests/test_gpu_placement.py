@@ -1,4 +1,4 @@
-"""State placement probe (qm_state_placement): from 31 to 33 qubits the
+"""State placement probe (qm_state_placement): for 33 qubits the
 constructor times a value-preserving pass over the top seven wires against
 one over wires 18..12 and allocates again while the first is slower, keeping
 the fastest allocation. The state must still start as |0...0> and run
@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_probed_state_starts_at_zero_and_runs():
-    n = 31
+    n = 33
     with qm.Simulator(n) as s:
         attempts, ratio = s.placement
         print(f"n={n}: {attempts} allocation(s), top/mid pass time ratio {ratio:.3f}")
@@ -32,7 +32,7 @@ def test_probe_off_and_outside_the_range():
     old = os.environ.get("QMETAL_PLACEMENT_PROBE")
     os.environ["QMETAL_PLACEMENT_PROBE"] = "0"
     try:
-        with qm.Simulator(31) as s:
+        with qm.Simulator(33) as s:
             assert s.placement == (0, 0.0)
             assert s.amplitudes_at([0])[0] == 1.0
     finally:
@@ -40,5 +40,6 @@ def test_probe_off_and_outside_the_range():
             del os.environ["QMETAL_PLACEMENT_PROBE"]
         else:
             os.environ["QMETAL_PLACEMENT_PROBE"] = old
-    with qm.Simulator(20) as s:
-        assert s.placement == (0, 0.0)
+    for n in (20, 32):
+        with qm.Simulator(n) as s:
+            assert s.placement == (0, 0.0)
