@@ -2685,12 +2685,12 @@ void Simulator::run_plan(const Plan &p) {
 }
 
 void Simulator::set_pass_merging(bool on) { impl_->merge = on; }
+bool Simulator::pass_merging() const { return impl_->merge; }
 
 void Simulator::placement(uint32_t *attempts, double *ratio) const {
   *attempts = impl_->place_attempts;
   *ratio = impl_->place_ratio;
 }
-bool Simulator::pass_merging() const { return impl_->merge; }
 
 void Simulator::set_timing(bool on) { impl_->timing = on; }
 
@@ -2937,9 +2937,7 @@ std::pair<double, std::vector<double>> Simulator::energy_and_gradient(
     uint32_t tries = 0;
     double ratio = 0.0;
     s->costate = s->alloc_state(bytes, &tries, &ratio);
-    if (!s->costate) {
-      fail("gradient: co-state allocation failed");
-    }
+    if (!s->costate) fail("gradient: co-state allocation failed");
   }
   reset();
   run_plan(build_plan(c, FuseOpts{}));
@@ -3329,9 +3327,7 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
       uint32_t tries = 0;
       double ratio = 0.0;
       s->costate = s->alloc_state(bytes, &tries, &ratio);
-      if (!s->costate) {
-        fail("gradient: co-state allocation failed");
-      }
+      if (!s->costate) fail("gradient: co-state allocation failed");
     }
     s->grow(s->adj_partials, s->adj_partials_cap,
             std::max(size_t(kAdjMaxRot) * kAdjBlocks, size_t(kAdjTileMaxRot) * kAdjTileBlocks) *
