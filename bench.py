@@ -454,9 +454,22 @@ def suite(qm, C, emit):
                     s.run(C.to_circuit(n, g2))
                     vals.append(s.expectation(h))
                 fd_err = max(fd_err, abs((vals[0] - vals[1]) / 2e-3 - gr[j]))
+            # <H> alone: read-only co-state tile passes against one FP64
+            # reduction per term, on the same state
+            s.expectation(h)
+            t0 = time.perf_counter()
+            e_tiles = s.expectation(h)
+            t_tiles = time.perf_counter() - t0
+            s.set_pass_merging(False)
+            t0 = time.perf_counter()
+            e_terms = s.expectation(h)
+            t_terms = time.perf_counter() - t0
+            s.set_pass_merging(True)
         rec = {"config": f"hea_d3_adjoint_n{n}", "gates": len(gates), "params": P,
                "energy": e, "gpu_s": t, "grads_per_s": 1.0 / t,
-               "max_abs_err_vs_central_diff_h1e-3": fd_err}
+               "max_abs_err_vs_central_diff_h1e-3": fd_err,
+               "expectation_s": t_tiles, "expectation_per_term_s": t_terms,
+               "expectation_tiles_vs_per_term_abs_diff": abs(e_tiles - e_terms)}
         if n == 20:
             O.set_threads(th)
             t0 = time.perf_counter()

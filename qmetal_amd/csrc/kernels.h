@@ -364,6 +364,8 @@ void launch_pauli_accum_tile(hipStream_t s, const AccumTileArgs &a);
 // `first`: lam is written without being read. With `epart`, the pass also
 // leaves per-block FP64 partials of <psi|lam> (after its update) for
 // k_finish (2 doubles per block, blocks of instance y after y * gridDim).
+// With lam null the pass reads psi only: the partials then hold <psi|H_t psi>
+// of this tile's terms H_t alone, and nothing is written to HBM.
 constexpr int kCoTerms = 48;
 constexpr int kCoGroups = 24;
 struct CoTileArgs {
@@ -382,7 +384,16 @@ struct CoTileArgs {
   double cre[kCoTerms];
   double cim[kCoTerms];
   uint64_t bstate;  // batched launches: per-instance state stride
+  // Energy-only passes (lam null), per flip group: bit 0 / bit 1 = some term
+  // has a real / imaginary coefficient (the pass needs the real / imaginary
+  // parts of the products conj(x) p); kCoGroupPlain = one term, real
+  // coefficient, local mask 0; kCoGroupLocal (plain groups only) = the flip
+  // is one of the thread's own local bits, bit (gpart >> 4) & 3 of the local
+  // index.
+  uint32_t gpart[kCoGroups];
 };
+constexpr uint32_t kCoGroupPlain = 4;
+constexpr uint32_t kCoGroupLocal = 8;
 // Single state; with `energy` (2 doubles) the fused <psi|lam> lands there.
 void launch_costate_tile(hipStream_t s, const CoTileArgs &a, double *partials, double *energy);
 // Batched: instance y's energy to energy[2 y] (when non-null).

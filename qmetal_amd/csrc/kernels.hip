@@ -1558,6 +1558,246 @@ __global__ __launch_bounds__(kAdjTileThreads) void k_costate_tile(CoTileArgs a, 
   }
 }
 
+// Walsh-signed sum over a thread's 16 amplitudes, sum_l (-1)^popcount(l & m)
+// v[l]: the sign factors over the bits of l, so the sum folds one bit at a
+// time (15 FMAs, the four +-1 factors uniform).
+__device__ __forceinline__ float walsh16(const float (&v)[16], uint32_t m) {
+  const float s0 = __uint_as_float(0x3f800000u | ((m & 1u) << 31));
+  const float s1 = __uint_as_float(0x3f800000u | ((m & 2u) << 30));
+  const float s2 = __uint_as_float(0x3f800000u | ((m & 4u) << 29));
+  const float s3 = __uint_as_float(0x3f800000u | ((m & 8u) << 28));
+  float a[8], b[4];
+#pragma unroll
+  for (int k = 0; k < 8; k++) a[k] = fmaf(s0, v[2 * k + 1], v[2 * k]);
+#pragma unroll
+  for (int k = 0; k < 4; k++) b[k] = fmaf(s1, a[2 * k + 1], a[2 * k]);
+  return fmaf(s3, fmaf(s2, b[3], b[2]), fmaf(s2, b[1], b[0]));
+}
+
+// Sign of term t at the thread's base index: the thread's own part of the
+// parity (bit t of tpar, fixed for the whole pass) and the tile's (upar, the
+// same for every thread of the block).
+__device__ __forceinline__ float term_sign(float v, uint64_t tpar, uint32_t upar, uint32_t t) {
+  return __uint_as_float(__float_as_uint(v) ^ ((uint32_t(tpar >> t) ^ upar) << 31));
+}
+
+// Plain groups (one term, real coefficient, no sign over the thread's own
+// bits): the share before c and the base sign is sum_l Re(conj(x_l) p_l),
+// one FMA chain over the partners.
+template <bool SWAP>
+__device__ __forceinline__ float energy_lds_plain(const float4 *lds, const uint32_t (&slot)[8],
+                                                 uint32_t gs, const float2 (&x)[16]) {
+  float acc0 = 0.0f, acc1 = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const float4 v = lds[slot[r] ^ gs];
+    const float2 p0 = SWAP ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+    const float2 p1 = SWAP ? make_float2(v.x, v.y) : make_float2(v.z, v.w);
+    acc0 = fmaf(x[2 * r].x, p0.x, acc0);
+    acc0 = fmaf(x[2 * r].y, p0.y, acc0);
+    acc1 = fmaf(x[2 * r + 1].x, p1.x, acc1);
+    acc1 = fmaf(x[2 * r + 1].y, p1.y, acc1);
+  }
+  return acc0 + acc1;
+}
+
+// A plain group whose flip is one of the thread's own local bits (bit B of
+// l): both members of every pair (l, l ^ 2^B) sit in this thread, and a
+// Hermitian term's two contributions conj(x_i) (P x)_i and conj(x_j) (P x)_j
+// are complex conjugates, so twice the real part over the half with bit B
+// clear is the whole. k indexes that half (l with bit B removed).
+template <int B>
+__device__ __forceinline__ int half_index(int k) {
+  return ((k >> B) << (B + 1)) | (k & ((1 << B) - 1));
+}
+template <int B>
+__device__ __forceinline__ float energy_local_plain(const float2 (&x)[16]) {
+  float acc0 = 0.0f, acc1 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 8; k += 2) {
+    const float2 u0 = x[half_index<B>(k)], v0 = x[half_index<B>(k) | (1 << B)];
+    const float2 u1 = x[half_index<B>(k + 1)], v1 = x[half_index<B>(k + 1) | (1 << B)];
+    acc0 = fmaf(u0.x, v0.x, acc0);
+    acc0 = fmaf(u0.y, v0.y, acc0);
+    acc1 = fmaf(u1.x, v1.x, acc1);
+    acc1 = fmaf(u1.y, v1.y, acc1);
+  }
+  return 2.0f * (acc0 + acc1);
+}
+// One flip group's share of the energy: partner amplitudes from LDS (slot
+// offset gs; SWAP when the flip holds amplitude bit 0), z_l = conj(x_l) p_l
+// in the parts PART asks for (bit 0 real, bit 1 imaginary), then per term
+// Re(c sum_l (-1)^popcount(l & lm) z_l) with its sign at the base index.
+template <bool SWAP, int PART>
+__device__ __forceinline__ float energy_lds_group(const float4 *lds, const uint32_t (&slot)[8],
+                                                 uint32_t gs, const float2 (&x)[16],
+                                                 const CoTileArgs &a, uint32_t t0, uint32_t t1,
+                                                 uint64_t tpar, uint64_t ub) {
+  float zr[16], zi[16];
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const float4 v = lds[slot[r] ^ gs];
+    const float2 p0 = SWAP ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+    const float2 p1 = SWAP ? make_float2(v.x, v.y) : make_float2(v.z, v.w);
+    const float2 u0 = x[2 * r], u1 = x[2 * r + 1];
+    if (PART & 1) {
+      zr[2 * r] = fmaf(u0.x, p0.x, u0.y * p0.y);
+      zr[2 * r + 1] = fmaf(u1.x, p1.x, u1.y * p1.y);
+    }
+    if (PART & 2) {
+      zi[2 * r] = fmaf(u0.x, p0.y, -(u0.y * p0.x));
+      zi[2 * r + 1] = fmaf(u1.x, p1.y, -(u1.y * p1.x));
+    }
+  }
+  float f = 0.0f;
+  for (uint32_t t = t0; t < t1; t++) {
+    const uint32_t lm = a.lmask[t];
+    const float cr = float(a.cre[t]), ci = float(a.cim[t]);
+    float v = 0.0f;
+    if ((PART & 1) && cr != 0.0f) v = cr * walsh16(zr, lm);
+    if ((PART & 2) && ci != 0.0f) v = fmaf(-ci, walsh16(zi, lm), v);
+    f += term_sign(v, tpar, uint32_t(__popcll(ub & a.sign[t]) & 1), t);
+  }
+  return f;
+}
+template <bool SWAP>
+__device__ __forceinline__ float energy_lds_pick(uint32_t part, const float4 *lds,
+                                                const uint32_t (&slot)[8], uint32_t gs,
+                                                const float2 (&x)[16], const CoTileArgs &a,
+                                                uint32_t t0, uint32_t t1, uint64_t tpar,
+                                                uint64_t ub) {
+  if (part == 1) return energy_lds_group<SWAP, 1>(lds, slot, gs, x, a, t0, t1, tpar, ub);
+  if (part == 2) return energy_lds_group<SWAP, 2>(lds, slot, gs, x, a, t0, t1, tpar, ub);
+  return energy_lds_group<SWAP, 3>(lds, slot, gs, x, a, t0, t1, tpar, ub);
+}
+
+// Energy-only co-state tile pass: <psi|H_t psi> of one tile's terms H_t
+// (CoTileArgs without lam), psi read once and nothing written. With x_l the
+// thread's 16 amplitudes (local combination l) and f_l = sum_m w_m
+// (-1)^popcount(l & m) the diagonal factor, the diagonal part is
+// sum_l f_l |x_l|^2 = sum_m w_m Q_m, Q the Walsh-Hadamard transform of
+// |x_l|^2; a flip term with coefficient c, base sign s and local mask lm
+// gives s c sum_l (-1)^popcount(l & lm) conj(x_l) p_l (p_l its partner
+// amplitude), whose real part is what the energy keeps. Per-tile sums run in
+// FP32 over the thread's 16 amplitudes, FP64 from there on.
+__global__ __launch_bounds__(kAdjTileThreads) void k_costate_energy(CoTileArgs a, double *epart) {
+  constexpr int W = kAdjTileThreads / 64;
+  __shared__ float4 lds[2048];
+  __shared__ double red[W];
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t inst = blockIdx.y;
+  const float2 *const psi = a.psi + inst * a.bstate;
+  const uint32_t lane_bytes = lane * 16u;
+  const uint64_t wave_off = ((wave & 1u) ? 1ull << a.hpos[0] : 0ull) |
+                            ((wave & 2u) ? 1ull << a.hpos[1] : 0ull);
+  uint64_t reg_off[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++)
+    reg_off[r] = ((r & 1) ? 1ull << a.hpos[2] : 0ull) | ((r & 2) ? 1ull << a.hpos[3] : 0ull) |
+                 ((r & 4) ? 1ull << a.hpos[4] : 0ull);
+  uint32_t t0 = 0;
+#pragma unroll
+  for (int j = 0; j < 6; j++)
+    if ((lane >> j) & 1) t0 ^= tile_swizzle(1u << j);
+#pragma unroll
+  for (int w = 0; w < 2; w++)
+    if ((wave >> w) & 1) t0 ^= tile_swizzle(1u << (6 + w));
+  uint32_t slot[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++)
+    slot[r] = t0 ^ ((r & 1) ? tile_swizzle(1u << 8) : 0u) ^ ((r & 2) ? tile_swizzle(1u << 9) : 0u) ^
+              ((r & 4) ? tile_swizzle(1u << 10) : 0u);
+  const uint64_t thr_bits = (uint64_t(lane) << 1) | wave_off;
+  static_assert(kCoTerms <= 64, "term parities are kept in one 64-bit mask");
+  const uint32_t nterms = a.goff[a.ngroups];
+  uint64_t tpar = 0;
+  for (uint32_t t = 0; t < nterms; t++)
+    tpar |= uint64_t(__popcll(thr_bits & a.sign[t]) & 1) << t;
+  bool shared = false;  // some group reads its partners from LDS
+  for (uint32_t g = 0; g < a.ngroups; g++) shared |= !(a.gpart[g] & kCoGroupLocal);
+  const bool diag = a.doff[16] > 0;
+  double e = 0.0;
+  for (uint64_t tile = blockIdx.x; tile < a.lay.nthreads; tile += gridDim.x) {
+    const uint64_t ub = scatter(tile, a.lay) | a.lay.fixed;
+    float2 x[16];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const float4 v = ld4<true>(pair_ptr(psi, ub | wave_off | reg_off[r], lane_bytes));
+      x[2 * r] = make_float2(v.x, v.y);
+      x[2 * r + 1] = make_float2(v.z, v.w);
+    }
+    if (shared) {
+      __syncthreads();  // the previous tile's reads of the buffer are done
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+        lds[slot[r]] = make_float4(x[2 * r].x, x[2 * r].y, x[2 * r + 1].x, x[2 * r + 1].y);
+      __syncthreads();
+    }
+    if (diag) {
+      float q[16];
+#pragma unroll
+      for (int l = 0; l < 16; l++) q[l] = fmaf(x[l].x, x[l].x, x[l].y * x[l].y);
+#pragma unroll
+      for (int b = 1; b < 16; b <<= 1)
+#pragma unroll
+        for (int l = 0; l < 16; l++)
+          if (!(l & b)) {
+            const float u = q[l], v = q[l | b];
+            q[l] = u + v;
+            q[l | b] = u - v;
+          }
+      double d = 0.0;
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        double w = 0.0;
+        for (uint32_t t = a.doff[m]; t < a.doff[m + 1]; t++) {
+          const uint32_t neg = (uint32_t(tpar >> t) ^ uint32_t(__popcll(ub & a.sign[t]))) << 31;
+          const double c = a.cre[t];
+          w += __hiloint2double(__double2hiint(c) ^ int(neg), __double2loint(c));
+        }
+        d = fma(w, double(q[m]), d);
+      }
+      e += d;
+    }
+    float f = 0.0f;
+    for (uint32_t g = 0; g < a.ngroups; g++) {
+      const uint32_t gp = a.gpart[g], part = gp & 3u;
+      if (!part) continue;
+      const uint32_t g0 = a.goff[g], g1 = a.goff[g + 1];
+      const bool swap = (a.gflip[g] & 1ull) != 0;
+      if (gp & kCoGroupPlain) {
+        float v;
+        if (gp & kCoGroupLocal) {
+          switch ((gp >> 4) & 3u) {
+            case 0: v = energy_local_plain<0>(x); break;
+            case 1: v = energy_local_plain<1>(x); break;
+            case 2: v = energy_local_plain<2>(x); break;
+            default: v = energy_local_plain<3>(x); break;
+          }
+        } else {
+          v = swap ? energy_lds_plain<true>(lds, slot, a.gslot[g], x)
+                   : energy_lds_plain<false>(lds, slot, a.gslot[g], x);
+        }
+        f += term_sign(float(a.cre[g0]) * v, tpar, uint32_t(__popcll(ub & a.sign[g0]) & 1), g0);
+      } else if (swap) {
+        f += energy_lds_pick<true>(part, lds, slot, a.gslot[g], x, a, g0, g1, tpar, ub);
+      } else {
+        f += energy_lds_pick<false>(part, lds, slot, a.gslot[g], x, a, g0, g1, tpar, ub);
+      }
+    }
+    e += double(f);
+  }
+  e = __ockl_wfred_add_f64(e);
+  if (lane == 0) red[wave] = e;
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double t = 0.0;
+    if (threadIdx.x == 0)
+      for (int w = 0; w < W; w++) t += red[w];
+    epart[(size_t(inst) * gridDim.x + blockIdx.x) * 2 + threadIdx.x] = t;
+  }
+}
+
 __global__ __launch_bounds__(kT) void k_adjoint_rot(AdjArgs a, double *part) {
   __shared__ double sh[kT / 64];
   const uint64_t stride = uint64_t(gridDim.x) * kT;
@@ -2041,6 +2281,12 @@ unsigned costate_tile_blocks(const Layout &lay) {
 
 void launch_costate_tile(hipStream_t s, const CoTileArgs &a, double *partials, double *energy) {
   const unsigned blocks = costate_tile_blocks(a.lay);
+  if (!a.lam) {
+    if (!energy) return;  // nothing to produce
+    k_costate_energy<<<blocks, kAdjTileThreads, 0, s>>>(a, partials);
+    k_finish<<<1, kT, 0, s>>>(partials, int(blocks), 2, energy);
+    return;
+  }
   k_costate_tile<<<blocks, kAdjTileThreads, 0, s>>>(a, energy ? partials : nullptr);
   if (energy) k_finish<<<1, kT, 0, s>>>(partials, int(blocks), 2, energy);
 }

@@ -1989,11 +1989,15 @@ struct Simulator::Impl {
   // tile (in chunks of kCoTerms across tiles when there are more). Flips that
   // fit no tile keep k_pauli_accum passes, after the tiles. The first tile
   // writes lambda without reading it. States are left for the caller.
+  // `spread` (read-only energy passes, where every tile costs the same read
+  // and its arithmetic decides how well the read is hidden): the tiles are
+  // set by the groups with high wires alone, and the diagonal terms and the
+  // groups on wires 0..6 go one by one to the tile with the least work.
   struct CoPlan {
     std::vector<CoTileArgs> tiles;
     std::vector<AccumLaunch> generic;
   };
-  static CoPlan costate_tile_plan(const Hamiltonian &h, uint32_t n) {
+  static CoPlan costate_tile_plan(const Hamiltonian &h, uint32_t n, bool spread = false) {
     CoPlan out;
     if (n < kAdjTileMinQubits || h.empty()) return out;
     std::map<uint64_t, std::vector<size_t>> by_flip;
@@ -2055,15 +2059,7 @@ struct Simulator::Impl {
       const int la = ha ? __builtin_ctzll(ha) : -1, lb2 = hb ? __builtin_ctzll(hb) : -1;
       return la < lb2;
     });
-    for (size_t b = 0; b < diag.size() || (b == 0 && tiles.empty()); b += kCoTerms) {
-      Tile t;
-      t.diag.assign(diag.begin() + long(std::min(b, diag.size())),
-                    diag.begin() + long(std::min(diag.size(), b + kCoTerms)));
-      t.nterms = t.diag.size();
-      tiles.push_back(std::move(t));
-      if (diag.empty()) break;
-    }
-    for (Group &g : groups) {
+    auto first_fit = [&](Group &g) {
       const uint64_t gh = g.flip & ~low;
       Tile *dst = nullptr;
       for (Tile &t : tiles)
@@ -2079,6 +2075,56 @@ struct Simulator::Impl {
       dst->hw |= gh;
       dst->nterms += g.terms.size();
       dst->groups.push_back(std::move(g));
+    };
+    if (!spread) {
+      for (size_t b = 0; b < diag.size() || (b == 0 && tiles.empty()); b += kCoTerms) {
+        Tile t;
+        t.diag.assign(diag.begin() + long(std::min(b, diag.size())),
+                      diag.begin() + long(std::min(diag.size(), b + kCoTerms)));
+        t.nterms = t.diag.size();
+        tiles.push_back(std::move(t));
+        if (diag.empty()) break;
+      }
+      for (Group &g : groups) first_fit(g);
+    } else {
+      std::vector<Group> lowg;
+      for (Group &g : groups) {
+        if (g.flip & ~low)
+          first_fit(g);
+        else
+          lowg.push_back(std::move(g));
+      }
+      // Work per thread and tile, in VALU operations: a group's products
+      // and per-term sums, a diagonal term's signed coefficient.
+      std::vector<size_t> work(tiles.size(), 0);
+      for (size_t i = 0; i < tiles.size(); i++)
+        for (const Group &g : tiles[i].groups) work[i] += 40 + 20 * g.terms.size();
+      auto least = [&](size_t groups_needed, size_t terms_needed) {
+        size_t best = tiles.size();
+        for (size_t i = 0; i < tiles.size(); i++)
+          if (tiles[i].groups.size() + groups_needed <= size_t(kCoGroups) &&
+              tiles[i].nterms + terms_needed <= size_t(kCoTerms) &&
+              (best == tiles.size() || work[i] < work[best]))
+            best = i;
+        if (best == tiles.size()) {
+          tiles.emplace_back();
+          work.push_back(0);
+        }
+        return best;
+      };
+      for (Group &g : lowg) {
+        const size_t i = least(1, g.terms.size());
+        work[i] += 40 + 20 * g.terms.size();
+        tiles[i].nterms += g.terms.size();
+        tiles[i].groups.push_back(std::move(g));
+      }
+      for (size_t k : diag) {
+        const size_t i = least(0, 1);
+        work[i] += 8;
+        tiles[i].nterms++;
+        tiles[i].diag.push_back(k);
+      }
+      if (tiles.empty()) tiles.emplace_back();
     }
     if (tiles.size() == 1 && tiles[0].diag.empty() && tiles[0].groups.empty()) tiles.clear();
     for (size_t ti = 0; ti < tiles.size(); ti++) {
@@ -2126,6 +2172,20 @@ struct Simulator::Impl {
         a.gflip[gi] = g.flip;
         a.gslot[gi] = slot_bits(g.flip);
         a.goff[gi] = cnt;
+        uint32_t part = 0;
+        for (size_t k : g.terms) {
+          double re, im;
+          coef(h[k], re, im);
+          part |= (re != 0.0 ? 1u : 0u) | (im != 0.0 ? 2u : 0u);
+        }
+        if (g.terms.size() == 1 && part == 1u && lmask(h[g.terms[0]].p.y | h[g.terms[0]].p.z) == 0) {
+          part |= kCoGroupPlain;
+          // A flip of a single local bit: bit 0 or one of the register wires.
+          const uint32_t F = lmask(g.flip);
+          if (__builtin_popcountll(g.flip) == 1 && F != 0)
+            part |= kCoGroupLocal | (uint32_t(__builtin_ctz(F)) << 4);
+        }
+        a.gpart[gi] = part;
         for (size_t k : g.terms) {
           const Term &tm = h[k];
           const uint64_t sg = tm.p.y | tm.p.z;
@@ -2797,16 +2857,53 @@ double Simulator::expectation(const Hamiltonian &h) {
     if (!t.p.valid(s->n)) fail("expectation: invalid Pauli string for this width");
   if (h.empty()) return 0.0;
   s->use();
-  double *slots = s->result_slots(2 * h.size());
-  for (size_t k = 0; k < h.size(); k++) {
-    s->pauli_sum(s->state, s->state, h[k].p, slots + 2 * k);
+  // Terms whose flip holds at most five wires of 7 and up share read-only
+  // co-state tile passes: each tile's terms act on the state in registers and
+  // only <psi|H_t psi> leaves the pass, so a tile costs one read of the state
+  // however many terms it holds. The other terms, narrow states and
+  // Hamiltonians that would not save a pass take one reduction per term.
+  const uint64_t low = (uint64_t(1) << 7) - 1;
+  const bool tiles_ok = s->merge && s->n >= kAdjTileMinQubits && h.size() > 1;
+  auto tileable = [&](const Term &t) {
+    return tiles_ok && __builtin_popcountll((t.p.x | t.p.y) & ~low) <= 5;
+  };
+  Hamiltonian tiled;
+  for (const Term &t : h)
+    if (tileable(t)) tiled.push_back(t);
+  Impl::CoPlan cp;
+  if (!tiled.empty()) cp = Impl::costate_tile_plan(tiled, s->n, true);
+  if (!cp.generic.empty()) fail("internal: expectation tile plan");
+  if (cp.tiles.size() >= tiled.size()) cp.tiles.clear();
+  std::vector<size_t> single;
+  for (size_t k = 0; k < h.size(); k++)
+    if (cp.tiles.empty() || !tileable(h[k])) single.push_back(k);
+  const size_t ns = single.size(), nt = cp.tiles.size();
+  double *slots = s->result_slots(2 * (ns + nt));
+  for (size_t j = 0; j < ns; j++) {
+    s->pauli_sum(s->state, s->state, h[single[j]].p, slots + 2 * j);
     s->dispatches++;
     s->launches++;
   }
-  const double *r = s->fetch_results(2 * h.size());
+  if (nt) {
+    s->grow(s->adj_partials, s->adj_partials_cap, size_t(kAdjTileBlocks) * 2,
+            "hipMalloc(adjoint partials)");
+    for (size_t t = 0; t < nt; t++) {
+      CoTileArgs &a = cp.tiles[t];
+      a.psi = s->state;
+      a.lam = nullptr;
+      launch_costate_tile(s->stream, a, s->adj_partials, slots + 2 * (ns + t));
+      s->launched();
+      s->dispatches++;
+      s->launches++;
+    }
+  }
+  const double *r = s->fetch_results(2 * (ns + nt));
   double total = 0.0;
-  for (size_t k = 0; k < h.size(); k++)
-    total += h[k].c * real_ipow(uint32_t(__builtin_popcountll(h[k].p.y)), r[2 * k], r[2 * k + 1]);
+  for (size_t j = 0; j < ns; j++) {
+    const Term &t = h[single[j]];
+    total += t.c * real_ipow(uint32_t(__builtin_popcountll(t.p.y)), r[2 * j], r[2 * j + 1]);
+  }
+  for (size_t t = 0; t < nt; t++) total += r[2 * (ns + t)];
   return total;
 }
 

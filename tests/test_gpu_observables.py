@@ -79,6 +79,78 @@ def test_hamiltonian_sum_and_norm():
         assert abs(s.norm() - 1.0) <= 1e-5
 
 
+def _mixed_terms(n, seed):
+    """Diagonal, single-flip, two-wire, Y-carrying (i^ny phases), flip-bit-0
+    and many-high-wire terms: more than one tile's worth of diagonal terms
+    and flip groups, plus flips that no tile can hold."""
+    rng = C.SplitMix64(seed)
+    terms = []
+    for q in range(n):
+        terms.append((1.0, 0, 0, (1 << q) | (1 << ((q + 1) % n))))
+        terms.append((0.7, 1 << q, 0, 0))
+        terms.append((-0.4, 0, 1 << q, 0))
+    for q in range(0, n - 1, 2):
+        terms.append((0.25, 1 << q, 1 << (q + 1), 0))
+        terms.append((0.15, 0, 3 << q, 1 << ((q + 5) % n)))
+    for _ in range(40):
+        z = rng.next_u64() & ((1 << n) - 1)
+        terms.append((rng.uniform(-1, 1), 0, 0, z))
+    for _ in range(30):
+        x = rng.next_u64() & ((1 << n) - 1)
+        y = rng.next_u64() & ((1 << n) - 1) & ~x
+        z = rng.next_u64() & ((1 << n) - 1) & ~(x | y)
+        terms.append((rng.uniform(-1, 1), x, y, z))
+    terms.append((0.35, 3, 0, 0))  # X0 X1: flip with amplitude bit 0 across lanes
+    terms.append((-0.45, 1 | (1 << (n - 1)), 0, 1 << 2))
+    terms.append((0.55, 1 << (n - 2), 0, 1 << (n - 3)))
+    top = ((1 << n) - 1) & ~0x7F
+    terms.append((0.5, top, 0, 1))  # every wire of 7 and up (six or more from n = 13)
+    terms.append((-0.3, 1, top & -top, 0))
+    return terms
+
+
+@pytest.mark.parametrize("n", [12, 14, 20])
+def test_hamiltonian_tile_passes_match_per_term_and_oracle(n):
+    """expectation(H) sums tile-held terms in read-only co-state tile passes;
+    it equals the one-reduction-per-term path on the same state and the
+    complex128 oracle on the oracle's state."""
+    gates = C.random_circuit(n, 5, seed=0x5EED + n)
+    ref = O.run(n, gates)
+    terms = _mixed_terms(n, 0xC0FFEE + n)
+    h = qm.Hamiltonian()
+    for t in terms:
+        h.add(*t)
+    scale = sum(abs(t[0]) for t in terms)
+    want = O.expect_h(ref, terms)
+    with sim_with(n, gates) as s:
+        got = s.expectation(h)
+        s.set_pass_merging(False)
+        per_term = s.expectation(h)
+        s.set_pass_merging(True)
+        assert s.expectation(h) == got  # deterministic
+    assert abs(got - per_term) <= 1e-6 * scale, (n, got, per_term)
+    assert abs(got - want) <= 1e-5 * scale, (n, got, want)
+
+
+def test_hamiltonian_single_term_and_diagonal_only():
+    """One term keeps the FP64 reduction; a diagonal-only Hamiltonian (one
+    tile) and an X-only one (flip groups only) match the per-term path."""
+    n = 16
+    gates = C.brickwork(n, 3, seed=11)
+    with sim_with(n, gates) as s:
+        for terms in ([(0.9, 0, 0, 1 << 9)],
+                      [(1.0, 0, 0, 3 << q) for q in range(n - 1)],
+                      [(0.5, 1 << q, 0, 0) for q in range(n)]):
+            h = qm.Hamiltonian()
+            for t in terms:
+                h.add(*t)
+            got = s.expectation(h)
+            s.set_pass_merging(False)
+            per_term = s.expectation(h)
+            s.set_pass_merging(True)
+            assert abs(got - per_term) <= 1e-6 * sum(abs(t[0]) for t in terms), (terms[0], got, per_term)
+
+
 def test_norm_is_fp64_accurate_at_scale():
     """The GPU norm of 2^24 complex64 amplitudes equals the FP64 norm of the
     same amplitudes to 1e-12 (an FP32 accumulation would be off by ~1e-7)."""
