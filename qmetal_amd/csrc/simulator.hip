@@ -2849,7 +2849,8 @@ double Simulator::expectation(const Pauli &p) {
   return expectation(h);
 }
 
-// One FP64 Pauli reduction per term (reference simulator.mm:629-667), all
+// sum_k c_k <psi|P_k|psi> (reference simulator.mm:663-667, each term as in
+// simulator.mm:629-660): tile passes and per-term FP64 reductions, all
 // enqueued before a single read-back.
 double Simulator::expectation(const Hamiltonian &h) {
   Impl *s = impl_.get();
