@@ -389,7 +389,8 @@ struct CoTileArgs {
   // parts of the products conj(x) p); kCoGroupPlain = one term, real
   // coefficient, local mask 0; kCoGroupLocal (plain groups only) = the flip
   // is one of the thread's own local bits, bit (gpart >> 4) & 3 of the local
-  // index.
+  // index. Both passes: bits 8..15 = how many of the group's terms, placed
+  // first, have a real coefficient and local mask 0.
   uint32_t gpart[kCoGroups];
 };
 constexpr uint32_t kCoGroupPlain = 4;

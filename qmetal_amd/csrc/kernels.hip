@@ -1519,7 +1519,19 @@ __global__ __launch_bounds__(kAdjTileThreads) void k_costate_tile(CoTileArgs a, 
         p[2 * r] = swap ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
         p[2 * r + 1] = swap ? make_float2(v.x, v.y) : make_float2(v.z, v.w);
       }
-      for (uint32_t t = a.goff[g]; t < a.goff[g + 1]; t++) {
+      // The group's plain terms first (real coefficient, no sign over the
+      // local bits): y += s c p.
+      const uint32_t tp = a.goff[g] + ((a.gpart[g] >> 8) & 0xFFu);
+      for (uint32_t t = a.goff[g]; t < tp; t++) {
+        const float c = float(a.cre[t]);
+        const float sr = (__popcll(base & a.sign[t]) & 1) ? -c : c;
+#pragma unroll
+        for (int l = 0; l < 16; l++) {
+          y[l].x = fmaf(sr, p[l].x, y[l].x);
+          y[l].y = fmaf(sr, p[l].y, y[l].y);
+        }
+      }
+      for (uint32_t t = tp; t < a.goff[g + 1]; t++) {
         const bool neg = (__popcll(base & a.sign[t]) & 1) != 0;
         const float cr = float(a.cre[t]), ci = float(a.cim[t]);
         const float sr = neg ? -cr : cr, si = neg ? -ci : ci;

@@ -2185,8 +2185,18 @@ struct Simulator::Impl {
           if (__builtin_popcountll(g.flip) == 1 && F != 0)
             part |= kCoGroupLocal | (uint32_t(__builtin_ctz(F)) << 4);
         }
-        a.gpart[gi] = part;
-        for (size_t k : g.terms) {
+        // Plain terms (real coefficient, no sign over the local bits) first:
+        // the co-state pass adds them with two FMAs per amplitude.
+        std::vector<size_t> order(g.terms);
+        auto plain = [&](size_t k) {
+          double re, im;
+          coef(h[k], re, im);
+          return im == 0.0 && lmask(h[k].p.y | h[k].p.z) == 0;
+        };
+        const size_t nplain = size_t(
+            std::stable_partition(order.begin(), order.end(), plain) - order.begin());
+        a.gpart[gi] = part | (uint32_t(nplain) << 8);
+        for (size_t k : order) {
           const Term &tm = h[k];
           const uint64_t sg = tm.p.y | tm.p.z;
           double re, im;
