@@ -772,14 +772,27 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return t;
 }
 
+// The streaming reductions keep kRedUnroll independent 16-byte nontemporal
+// loads in flight per thread (a wave's k-th loads cover 1 KB, the block's
+// kT * kRedUnroll float4 one contiguous run per loop step).
+constexpr int kRedUnroll = 4;
+
 __global__ __launch_bounds__(kT) void k_abs2(const float2 *a, uint64_t dim, double *part) {
   __shared__ double sh[kT / 64];
-  const uint64_t stride = uint64_t(gridDim.x) * kT;
-  const float4 *v = reinterpret_cast<const float4 *>(a);
+  constexpr int U = kRedUnroll;
+  const uint64_t n4 = dim / 2, step = uint64_t(gridDim.x) * kT * U;
   double acc = 0.0;
-  for (uint64_t i = uint64_t(blockIdx.x) * kT + threadIdx.x; i < dim / 2; i += stride) {
-    const float4 x = v[i];
-    acc += double(x.x) * x.x + double(x.y) * x.y + double(x.z) * x.z + double(x.w) * x.w;
+  for (uint64_t c = uint64_t(blockIdx.x) * kT * U + threadIdx.x; c < n4; c += step) {
+    float4 x[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const uint64_t i = c + uint64_t(k) * kT;
+      x[k] = i < n4 ? ld4<true>(a + 2 * i) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+      acc += double(x[k].x) * x[k].x + double(x[k].y) * x[k].y + double(x[k].z) * x[k].z +
+             double(x[k].w) * x[k].w;
   }
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -806,51 +819,76 @@ __global__ __launch_bounds__(kT) void k_pauli(const float2 *bra, const float2 *k
   bra += uint64_t(blockIdx.y) * dim;
   ket += uint64_t(blockIdx.y) * dim;
   part += size_t(blockIdx.y) * gridDim.x * 2;
-  const uint64_t stride = uint64_t(gridDim.x) * kT;
+  // With bra == ket (an expectation) each float4 is loaded once. Two
+  // float4 of each state per thread and step: kRedBlocks workgroups stay
+  // resident at once (more registers would leave a tail of workgroups).
+  const bool same = bra == ket;
+  constexpr int U = 2;
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   double re = 0.0, im = 0.0;
-  if (flip == 0) {
-    for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < dim / 2; u += stride) {
-      const uint64_t i = 2 * u;
-      const float4 b = *reinterpret_cast<const float4 *>(bra + i);
-      const float4 k = *reinterpret_cast<const float4 *>(ket + i);
-      acc_cj(re, im, make_float2(b.x, b.y), make_float2(k.x, k.y), sgn(i, sign));
-      acc_cj(re, im, make_float2(b.z, b.w), make_float2(k.z, k.w), sgn(i + 1, sign));
-    }
-  } else if (flip == 1) {
-    for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < dim / 2; u += stride) {
-      const uint64_t i = 2 * u;
-      const float4 b = *reinterpret_cast<const float4 *>(bra + i);
-      const float4 k = *reinterpret_cast<const float4 *>(ket + i);
-      // i -> j = i + 1, and i + 1 -> i
-      acc_cj(re, im, make_float2(b.x, b.y), make_float2(k.z, k.w), sgn(i + 1, sign));
-      acc_cj(re, im, make_float2(b.z, b.w), make_float2(k.x, k.y), sgn(i, sign));
+  if (flip <= 1) {
+    const uint64_t n4 = dim / 2, step = uint64_t(gridDim.x) * kT * U;
+    for (uint64_t c = uint64_t(blockIdx.x) * kT * U + threadIdx.x; c < n4; c += step) {
+      float4 b[U], k[U];
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        const uint64_t u = c + uint64_t(q) * kT;
+        b[q] = u < n4 ? ld4<true>(bra + 2 * u) : zero;
+        k[q] = same ? b[q] : (u < n4 ? ld4<true>(ket + 2 * u) : zero);
+      }
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        const uint64_t i = 2 * (c + uint64_t(q) * kT);
+        const float2 b0 = make_float2(b[q].x, b[q].y), b1 = make_float2(b[q].z, b[q].w);
+        const float2 k0 = make_float2(k[q].x, k[q].y), k1 = make_float2(k[q].z, k[q].w);
+        if (flip == 0) {
+          acc_cj(re, im, b0, k0, sgn(i, sign));
+          acc_cj(re, im, b1, k1, sgn(i + 1, sign));
+        } else {
+          // i -> j = i + 1, and i + 1 -> i
+          acc_cj(re, im, b0, k1, sgn(i + 1, sign));
+          acc_cj(re, im, b1, k0, sgn(i, sign));
+        }
+      }
     }
   } else {
     const uint32_t h = 63 - __clzll(flip);  // top bit of flip, >= 1
     const uint64_t fe = flip & ~1ull;
     const bool odd = flip & 1ull;
-    for (uint64_t u = uint64_t(blockIdx.x) * kT + threadIdx.x; u < dim / 4; u += stride) {
-      // i: even, bit h clear
-      uint64_t t = u;
-      t = ((t >> (h - 1)) << h) | (t & ((1ull << (h - 1)) - 1ull));
-      const uint64_t i = t << 1;
-      const uint64_t p = i ^ fe;  // even partner base, bit h set
-      const float4 bi = *reinterpret_cast<const float4 *>(bra + i);
-      const float4 ki = *reinterpret_cast<const float4 *>(ket + i);
-      const float4 bp = *reinterpret_cast<const float4 *>(bra + p);
-      const float4 kp = *reinterpret_cast<const float4 *>(ket + p);
-      const float2 bi0 = make_float2(bi.x, bi.y), bi1 = make_float2(bi.z, bi.w);
-      const float2 ki0 = make_float2(ki.x, ki.y), ki1 = make_float2(ki.z, ki.w);
-      const float2 bp0 = make_float2(bp.x, bp.y), bp1 = make_float2(bp.z, bp.w);
-      const float2 kp0 = make_float2(kp.x, kp.y), kp1 = make_float2(kp.z, kp.w);
-      // partner of i is p (+1 if flip is odd); of i+1 is p+1 (-1 if odd)
-      const float2 kj0 = odd ? kp1 : kp0, kj1 = odd ? kp0 : kp1;
-      const float2 bj0 = odd ? bp1 : bp0, bj1 = odd ? bp0 : bp1;
-      const uint64_t j0 = i ^ flip, j1 = (i + 1) ^ flip;
-      acc_cj(re, im, bi0, kj0, sgn(j0, sign));
-      acc_cj(re, im, bi1, kj1, sgn(j1, sign));
-      acc_cj(re, im, bj0, ki0, sgn(i, sign));
-      acc_cj(re, im, bj1, ki1, sgn(i + 1, sign));
+    constexpr int V = U;  // pairs (i, partner) per thread and step
+    const uint64_t n8 = dim / 4, step = uint64_t(gridDim.x) * kT * V;
+    for (uint64_t c = uint64_t(blockIdx.x) * kT * V + threadIdx.x; c < n8; c += step) {
+      float4 bi[V], ki[V], bp[V], kp[V];
+      uint64_t iv[V];
+#pragma unroll
+      for (int q = 0; q < V; q++) {
+        const uint64_t u = c + uint64_t(q) * kT;
+        // i: even, bit h clear; its even partner base p has bit h set
+        uint64_t t = u < n8 ? u : 0;
+        t = ((t >> (h - 1)) << h) | (t & ((1ull << (h - 1)) - 1ull));
+        const uint64_t i = t << 1, p = i ^ fe;
+        iv[q] = i;
+        bi[q] = u < n8 ? ld4<true>(bra + i) : zero;
+        bp[q] = u < n8 ? ld4<true>(bra + p) : zero;
+        ki[q] = same ? bi[q] : (u < n8 ? ld4<true>(ket + i) : zero);
+        kp[q] = same ? bp[q] : (u < n8 ? ld4<true>(ket + p) : zero);
+      }
+#pragma unroll
+      for (int q = 0; q < V; q++) {
+        const uint64_t i = iv[q];
+        const float2 bi0 = make_float2(bi[q].x, bi[q].y), bi1 = make_float2(bi[q].z, bi[q].w);
+        const float2 ki0 = make_float2(ki[q].x, ki[q].y), ki1 = make_float2(ki[q].z, ki[q].w);
+        const float2 bp0 = make_float2(bp[q].x, bp[q].y), bp1 = make_float2(bp[q].z, bp[q].w);
+        const float2 kp0 = make_float2(kp[q].x, kp[q].y), kp1 = make_float2(kp[q].z, kp[q].w);
+        // partner of i is p (+1 if flip is odd); of i+1 is p+1 (-1 if odd)
+        const float2 kj0 = odd ? kp1 : kp0, kj1 = odd ? kp0 : kp1;
+        const float2 bj0 = odd ? bp1 : bp0, bj1 = odd ? bp0 : bp1;
+        const uint64_t j0 = i ^ flip, j1 = (i + 1) ^ flip;
+        acc_cj(re, im, bi0, kj0, sgn(j0, sign));
+        acc_cj(re, im, bi1, kj1, sgn(j1, sign));
+        acc_cj(re, im, bj0, ki0, sgn(i, sign));
+        acc_cj(re, im, bj1, ki1, sgn(i + 1, sign));
+      }
     }
   }
   const double tr = block_sum(re, sh);
