@@ -101,6 +101,9 @@ def _mixed_terms(n, seed):
         z = rng.next_u64() & ((1 << n) - 1) & ~(x | y)
         terms.append((rng.uniform(-1, 1), x, y, z))
     terms.append((0.35, 3, 0, 0))  # X0 X1: flip with amplitude bit 0 across lanes
+    terms.append((0.3, 0, 0, 0))  # identity
+    terms.append((0.7, 1 << (n // 2), 0, 0))  # a duplicate of a single-X term
+    terms.append((-0.25, 0, 0, 3))  # a duplicate of a ZZ term
     terms.append((-0.45, 1 | (1 << (n - 1)), 0, 1 << 2))
     terms.append((0.55, 1 << (n - 2), 0, 1 << (n - 3)))
     top = ((1 << n) - 1) & ~0x7F
@@ -130,6 +133,34 @@ def test_hamiltonian_tile_passes_match_per_term_and_oracle(n):
         assert s.expectation(h) == got  # deterministic
     assert abs(got - per_term) <= 1e-6 * scale, (n, got, per_term)
     assert abs(got - want) <= 1e-5 * scale, (n, got, want)
+
+
+def test_hamiltonian_many_terms_per_flip_and_per_tile():
+    """Groups and diagonal sums larger than one tile holds: 70 diagonal
+    strings, 60 terms sharing one flip, 40 single-wire groups on wires 0..6
+    repeated, against the per-term path and the oracle."""
+    n = 14
+    gates = C.random_circuit(n, 4, seed=0xD1A9)
+    ref = O.run(n, gates)
+    rng = C.SplitMix64(99)
+    m = (1 << n) - 1
+    terms = [(rng.uniform(-1, 1), 0, 0, rng.next_u64() & m) for _ in range(70)]
+    flip = (1 << 2) | (1 << 9)
+    for _ in range(60):
+        y = rng.next_u64() & flip
+        terms.append((rng.uniform(-1, 1), flip & ~y, y, rng.next_u64() & m & ~flip))
+    for k in range(40):
+        terms.append((rng.uniform(-1, 1), 1 << (k % 7), 0, rng.next_u64() & m & ~(1 << (k % 7))))
+    h = qm.Hamiltonian()
+    for t in terms:
+        h.add(*t)
+    scale = sum(abs(t[0]) for t in terms)
+    with sim_with(n, gates) as s:
+        got = s.expectation(h)
+        s.set_pass_merging(False)
+        per_term = s.expectation(h)
+    assert abs(got - per_term) <= 1e-6 * scale, (got, per_term)
+    assert abs(got - O.expect_h(ref, terms)) <= 1e-5 * scale
 
 
 def test_hamiltonian_single_term_and_diagonal_only():
