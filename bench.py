@@ -483,6 +483,39 @@ def suite(qm, C, emit):
         emit(rec)
 
 
+def tfim_expectation(qm, C, n, reps):
+    """<H> of the TFIM ring (2n terms) on the HEA depth-3 state of the
+    adjoint leg: the default path (terms several to a read-only tile pass)
+    against one FP64 reduction per term (pass merging off). Each tile pass
+    reads the state once, so its algorithmic bytes are 8 * 2^n."""
+    rng = C.SplitMix64(0x4EA + n)
+    params = [rng.uniform(0, 2 * math.pi) for _ in range(6 * n)]
+    gates = C.hardware_efficient(n, 3, params=params)
+    terms = C.tfim_ring_hamiltonian(n)
+    h = C.to_hamiltonian(terms)
+    rec = {"workload": f"tfim_ring_expectation_n{n}", "terms": len(terms)}
+    with qm.Simulator(n) as s:
+        s.run(C.to_circuit(n, gates))
+        for key, merge in (("tiles", True), ("per_term", False)):
+            s.set_pass_merging(merge)
+            s.expectation(h)
+            before = s.launch_count
+            e = s.expectation(h)
+            passes = s.launch_count - before
+            best = None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                e = s.expectation(h)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            rec[key] = {"energy": e, "s": best, "passes": passes,
+                        "read_GBps": passes * 8.0 * (1 << n) / best / 1e9}
+        s.set_pass_merging(True)
+    rec["speedup"] = rec["per_term"]["s"] / rec["tiles"]["s"]
+    rec["abs_diff"] = abs(rec["tiles"]["energy"] - rec["per_term"]["energy"])
+    return rec
+
+
 # Published ratios of the reference on its own device (Apple M4 Max, n=26):
 # fusion speedup per circuit (results/fusion_bench_n26.txt:5-13) and the
 # fusion x blocking grid at b=10 (results/ablation_n26_b10_extended.txt:6-15).
@@ -755,6 +788,9 @@ def main() -> int:
     ap.add_argument("--frozen-reps", type=int, default=2)
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the HEA depth-3 adjoint at n=--adjoint-n, per kernel")
+    ap.add_argument("--expectation", type=int, default=0, metavar="N",
+                    help="only time <H> of the TFIM ring on the HEA depth-3 state at n=N: "
+                         "read-only tile passes against one reduction per term")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched QAOA measurement")
     ap.add_argument("--brickwork", type=int, default=0, metavar="N",
@@ -806,6 +842,11 @@ def main() -> int:
         print(f"# device {qm.device_name()}  HEA d3 adjoint n={args.adjoint_n}", file=sys.stderr,
               flush=True)
         print(json.dumps(hea_adjoint(qm, C, args.adjoint_n, 2)), flush=True)
+        return 0
+    if args.expectation:
+        print(f"# device {qm.device_name()}  TFIM ring <H> n={args.expectation}", file=sys.stderr,
+              flush=True)
+        print(json.dumps(tfim_expectation(qm, C, args.expectation, 3)), flush=True)
         return 0
     if args.batch_only:
         print(f"# device {qm.device_name()}  batched QAOA x{args.batch_count}", file=sys.stderr,
