@@ -397,6 +397,9 @@ constexpr uint32_t kCoGroupPlain = 4;
 constexpr uint32_t kCoGroupLocal = 8;
 // Single state; with `energy` (2 doubles) the fused <psi|lam> lands there.
 void launch_costate_tile(hipStream_t s, const CoTileArgs &a, double *partials, double *energy);
+// out[0..1] = sum over k < count of slots[2 k], slots[2 k + 1] (FP64, fixed
+// order): the shares of several energy passes.
+void launch_sum_slots(hipStream_t s, const double *slots, uint32_t count, double *out);
 // Batched: instance y's energy to energy[2 y] (when non-null).
 unsigned costate_tile_blocks(const Layout &lay);
 void launch_costate_tile_batched(hipStream_t s, const CoTileArgs &a, double *partials,

@@ -2341,6 +2341,10 @@ void launch_costate_tile(hipStream_t s, const CoTileArgs &a, double *partials, d
   if (energy) k_finish<<<1, kT, 0, s>>>(partials, int(blocks), 2, energy);
 }
 
+void launch_sum_slots(hipStream_t s, const double *slots, uint32_t count, double *out) {
+  k_finish<<<1, kT, 0, s>>>(slots, int(count), 2, out);
+}
+
 void launch_costate_tile_batched(hipStream_t s, const CoTileArgs &a, double *partials,
                                  double *energy, uint32_t count) {
   const unsigned blocks = costate_tile_blocks(a.lay);

@@ -1315,12 +1315,18 @@ struct Simulator::Impl {
     std::vector<size_t> bwd_rot0;
     std::vector<CoTileArgs> co;
     std::vector<AccumArgs> co_generic;
+    // The energy alone (no gradient asked): read-only tile passes, each
+    // leaving its share in eslots, summed into eslot; empty when a term fits
+    // no tile.
+    std::vector<CoTileArgs> eco;
+    double *eslots = nullptr;
     uint32_t nrot = 0;
     double *eslot = nullptr, *rslots = nullptr;
     uint32_t *rgate = nullptr, *inst = nullptr;
     ~DevProg() {
       hipFree(arena);
       hipFree(patch);
+      hipFree(eslots);
       hipFree(eslot);
       hipFree(rslots);
       hipFree(rgate);
@@ -3396,6 +3402,14 @@ bool Simulator::compile_device_program(const Circuit &c, const Hamiltonian &h) {
         "hipMalloc");
   d->co = cp.tiles;
   for (const Impl::AccumLaunch &L : cp.generic) d->co_generic.push_back(L.g);
+  if (s->merge && h.size() > 1) {
+    Impl::CoPlan ep = Impl::costate_tile_plan(h, s->n, true);
+    if (ep.generic.empty() && !ep.tiles.empty()) {
+      d->eco = ep.tiles;
+      check(hipMalloc(reinterpret_cast<void **>(&d->eslots), d->eco.size() * 2 * sizeof(double)),
+            "hipMalloc");
+    }
+  }
   s->dev_prog = std::move(d);
   return true;
 }
@@ -3429,7 +3443,8 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
     check(hipStreamSynchronize(s->stream), "GPU error");
   }
   if (c.n == s->n && compile_device_program(c, h)) {
-    if (!s->costate) {
+    const bool energy_only = !out_grad && !s->dev_prog->eco.empty();
+    if (!s->costate && !energy_only) {
       const uint64_t bytes = s->dim * 8ull;
       if (bytes > free_memory()) fail("gradient: co-state exceeds free device memory");
       uint32_t tries = 0;
@@ -3465,26 +3480,38 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
       launch_fwd_tile(cs, a);
       n_launch++;
     }
-    bool fused = false;
-    for (size_t k = 0; k < d.co.size(); k++) {
-      CoTileArgs &a = d.co[k];
-      a.psi = s->state;
-      a.lam = s->costate;
-      const bool fuse = k + 1 == d.co.size() && d.co_generic.empty();
-      launch_costate_tile(cs, a, s->adj_partials, fuse ? d.eslot : nullptr);
-      fused = fused || fuse;
+    if (energy_only) {
+      for (size_t k = 0; k < d.eco.size(); k++) {
+        CoTileArgs &a = d.eco[k];
+        a.psi = s->state;
+        a.lam = nullptr;
+        launch_costate_tile(cs, a, s->adj_partials, d.eslots + 2 * k);
+        n_launch++;
+      }
+      launch_sum_slots(cs, d.eslots, uint32_t(d.eco.size()), d.eslot);
       n_launch++;
-    }
-    for (AccumArgs &g : d.co_generic) {
-      g.psi = s->state;
-      g.lam = s->costate;
-      g.dim = s->dim;
-      launch_pauli_accum(cs, g);
-      n_launch++;
-    }
-    if (!fused) {
-      launch_pauli(cs, s->state, s->costate, s->dim, 0, 0, s->partials, d.eslot);
-      n_launch++;
+    } else {
+      bool fused = false;
+      for (size_t k = 0; k < d.co.size(); k++) {
+        CoTileArgs &a = d.co[k];
+        a.psi = s->state;
+        a.lam = s->costate;
+        const bool fuse = k + 1 == d.co.size() && d.co_generic.empty();
+        launch_costate_tile(cs, a, s->adj_partials, fuse ? d.eslot : nullptr);
+        fused = fused || fuse;
+        n_launch++;
+      }
+      for (AccumArgs &g : d.co_generic) {
+        g.psi = s->state;
+        g.lam = s->costate;
+        g.dim = s->dim;
+        launch_pauli_accum(cs, g);
+        n_launch++;
+      }
+      if (!fused) {
+        launch_pauli(cs, s->state, s->costate, s->dim, 0, 0, s->partials, d.eslot);
+        n_launch++;
+      }
     }
     if (out_grad) {
       for (size_t j = 0; j < d.bwd.size(); j++) {

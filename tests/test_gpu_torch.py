@@ -114,6 +114,46 @@ def test_energy_only_when_no_gradient_is_needed():
     assert t_energy < t_full
 
 
+def test_energy_only_program_under_graph_capture():
+    """With no gradient asked, the device program measures the energy in
+    read-only tile passes (no co-state is allocated): captured in a graph,
+    replays with new angles equal eager calls bit for bit and the full
+    adjoint call's energy to 1e-6."""
+    from qmetal_amd.torch_adjoint import energy_and_grad
+    n, depth = 15, 2
+    P = 2 * n * depth
+    rng = C.SplitMix64(33)
+    circ = C.to_circuit(n, hea_template(n, depth, [0.1] * P))
+    ham = C.to_hamiltonian(C.tfim_ring_hamiltonian(n))
+    with qm.Simulator(n) as sim:
+        theta = torch.tensor([rng.uniform(0, 6.2) for _ in range(P)], dtype=torch.float64,
+                             device="cuda")
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                energy_and_grad(sim, circ, ham, theta, need_grad=False)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            e_g, _ = energy_and_grad(sim, circ, ham, theta, need_grad=False)
+        for trial in range(3):
+            theta.copy_(torch.tensor([rng.uniform(0, 6.2) for _ in range(P)],
+                                     dtype=torch.float64, device="cuda"))
+            graph.replay()
+            torch.cuda.synchronize()
+            e_r = e_g.clone()
+            e_e, _ = energy_and_grad(sim, circ, ham, theta, need_grad=False)
+            torch.cuda.synchronize()
+            e_full, _ = energy_and_grad(sim, circ, ham, theta)
+            torch.cuda.synchronize()
+            print(f"replay {trial}: energy {e_r.item():.8f} eager {e_e.item():.8f} "
+                  f"with gradient {e_full.item():.8f}")
+            assert e_r.item() == e_e.item()
+            assert abs(e_r.item() - e_full.item()) <= 1e-6
+
+
 def _with_angles(gates, theta):
     """The template with its RX/RY/RZ angles replaced, in circuit order."""
     it = iter(theta)
