@@ -272,9 +272,11 @@ class Circuit:
         if not self._h:
             raise QMetalError(_err())
 
-    def __del__(self):
+    def __del__(self, _free=_circuit_destroy):
+        # The destroy function is bound here: at interpreter exit the module's
+        # globals may already be gone when the last handles are collected.
         if getattr(self, "_h", None):
-            _circuit_destroy(self._h)
+            _free(self._h)
             self._h = None
 
     def __len__(self) -> int:
@@ -399,9 +401,11 @@ class Simulator:
         if not self._h:
             raise QMetalError(_err())
 
-    def __del__(self):
+    def __del__(self, _free=_destroy):
+        # The destroy function is bound here: at interpreter exit the module's
+        # globals may already be gone when the last handles are collected.
         if getattr(self, "_h", None):
-            _destroy(self._h)
+            _free(self._h)
             self._h = None
 
     def close(self):
@@ -540,9 +544,11 @@ class Batch:
             raise QMetalError(_err())
         self._sizes: list[int] = []
 
-    def __del__(self):
+    def __del__(self, _free=_batch_destroy):
+        # The destroy function is bound here: at interpreter exit the module's
+        # globals may already be gone when the last handles are collected.
         if getattr(self, "_h", None):
-            _batch_destroy(self._h)
+            _free(self._h)
             self._h = None
 
     def close(self):
