@@ -266,8 +266,8 @@ def test_ghz20_invariants_and_1e5_shots():
 @pytest.mark.parametrize("n", [33, 34])
 def test_ghz_full_size_invariants(n):
     """GHZ at 64 and 128 GiB (one allocation), merged passes: norm 1,
-    <Z_q> = 0, <Z_0 Z_(n-1)> = 1, <X...X> = 1 (the two poles' coherence),
-    and shots only on the two poles, both of them, near half each (4000
+    <Z_q> = 0, <Z_0 Z_(n-1)> = 1, <X...X> = 1 (the two poles' coherence), a
+    100-term Hamiltonian's <H> = n + 2 through the tile passes, and shots only on the two poles, both of them, near half each (4000
     shots: 2000 +- 32 sd, the bound is 6 sd)."""
     top = (1 << n) - 1
     with sim_with(n, C.ghz(n)) as s:
@@ -275,12 +275,23 @@ def test_ghz_full_size_invariants(n):
         zq = max(abs(s.expectation_pauli(z=1 << q)) for q in (0, 1, 12, 13, n // 2, n - 1))
         zz = s.expectation_pauli(z=1 | (1 << (n - 1)))
         xx = s.expectation_pauli(x=top)
+        # <H> through the read-only tile passes: every Z_i Z_j is 1 and every
+        # string flipping fewer than n wires is 0 on GHZ; X..X (too wide for a
+        # tile: one reduction) is 1.
+        terms = [(1.0, 0, 0, (1 << q) | (1 << ((q + 1) % n))) for q in range(n)]
+        terms += [(0.5, (1 << q) | (1 << ((q + 1) % n)), 0, 0) for q in range(n)]
+        terms += [(-0.3, 0, 1 << q, 0) for q in range(n)] + [(2.0, top, 0, 0)]
+        h = qm.Hamiltonian()
+        for t in terms:
+            h.add(*t)
+        eh = s.expectation(h)
         shots = s.sample(4000, 2024)
     ones = sum(1 for v in shots if v == top)
     print(f"GHZ{n}: norm {norm:.8f}, max|<Z_q>| {zq:.2e}, <Z0 Z{n - 1}> {zz:.8f}, "
-          f"<X..X> {xx:.8f}, {ones} of 4000 shots on |1..1>")
+          f"<X..X> {xx:.8f}, <H> {eh:.8f} (n + 2 = {n + 2}), {ones} of 4000 shots on |1..1>")
     assert abs(norm - 1) <= 1e-5 and zq <= 1e-5
     assert abs(zz - 1) <= 1e-5 and abs(xx - 1) <= 1e-5
+    assert abs(eh - (n + 2)) <= 1e-5 * sum(abs(t[0]) for t in terms)
     assert set(shots) == {0, top} and abs(ones - 2000) <= 190
 
 
