@@ -98,9 +98,10 @@ const float *qm_amplitudes(qm_simulator s);
 int qm_norm(qm_simulator s, double *out);
 int qm_expectation_pauli(qm_simulator s, uint64_t x_mask, uint64_t y_mask,
                          uint64_t z_mask, double *out);
-/* sum_k c_k <psi|P_k|psi>. From 12 qubits, terms that fit a 12-wire tile are
- * evaluated several per read-only pass over the state; the others, and
- * every term with pass merging off, take one FP64 reduction each. */
+/* sum_k c_k <psi|P_k|psi>. On states where per-term reads would pass 256 MiB
+ * (12 qubits and up), terms that fit a 12-wire tile are evaluated several per
+ * read-only pass over the state; the others, and every term with pass
+ * merging off, take one FP64 reduction each (many terms per launch). */
 int qm_expectation(qm_simulator s, const qm_hamiltonian *h, double *out);
 
 /* Draws `shots` basis-state indices without collapsing the state. `out` must

@@ -302,6 +302,19 @@ void launch_gwide(hipStream_t s, int wb, const GWideArgs &g);
 constexpr int kRedBlocks = 2048;
 constexpr int kRedThreads = 256;
 
+// Terms of one k_pauli_terms launch: flip = x | y, sign = y | z of each.
+constexpr uint32_t kPauliTermsPerLaunch = 128;
+struct PauliTerms {
+  uint64_t flip[kPauliTermsPerLaunch];
+  uint64_t sign[kPauliTermsPerLaunch];
+};
+// out[2 t], out[2 t + 1] = sum_i s_t(i ^ flip_t) conj(a_i) a_{i ^ flip_t} for
+// t < count (<= kPauliTermsPerLaunch) in one launch; `partials` holds
+// count * pauli_terms_blocks(dim) * 2 doubles.
+unsigned pauli_terms_blocks(uint64_t dim);
+void launch_pauli_terms(hipStream_t s, const float2 *a, uint64_t dim, const PauliTerms &t,
+                        uint32_t count, double *partials, double *out);
+
 // out[0] = sum |a_i|^2
 void launch_abs2(hipStream_t s, const float2 *a, uint64_t dim, double *partials,
                  double *out);

@@ -811,24 +811,20 @@ __device__ __forceinline__ void acc_cj(double &re, double &im, float2 x, float2 
 // sum_i s(i ^ flip) conj(bra_i) ket_{i ^ flip}. Each amplitude is read once:
 // a thread takes an even index i with the flip's top bit clear together with
 // its partners, and adds both directions of the pairing.
-__global__ __launch_bounds__(kT) void k_pauli(const float2 *bra, const float2 *ket, uint64_t dim,
-                                              uint64_t flip, uint64_t sign, double *part) {
-  __shared__ double sh[kT / 64];
-  // Batched launches: instance blockIdx.y, states of `dim` amplitudes back to
-  // back, partials per instance.
-  bra += uint64_t(blockIdx.y) * dim;
-  ket += uint64_t(blockIdx.y) * dim;
-  part += size_t(blockIdx.y) * gridDim.x * 2;
-  // With bra == ket (an expectation) each float4 is loaded once. Two
-  // float4 of each state per thread and step: kRedBlocks workgroups stay
-  // resident at once (more registers would leave a tail of workgroups).
+// This workgroup's part of sum_i s(i ^ flip) conj(bra_i) ket_{i ^ flip}
+// (workgroup bx of gx). With bra == ket (an expectation) each float4 is
+// loaded once. Two float4 of each state per thread and step: kRedBlocks
+// workgroups stay resident at once (more registers would leave a tail of
+// workgroups).
+__device__ __forceinline__ void pauli_part(const float2 *bra, const float2 *ket, uint64_t dim,
+                                           uint64_t flip, uint64_t sign, uint32_t bx,
+                                           uint32_t gx, double &re, double &im) {
   const bool same = bra == ket;
   constexpr int U = 2;
   const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  double re = 0.0, im = 0.0;
   if (flip <= 1) {
-    const uint64_t n4 = dim / 2, step = uint64_t(gridDim.x) * kT * U;
-    for (uint64_t c = uint64_t(blockIdx.x) * kT * U + threadIdx.x; c < n4; c += step) {
+    const uint64_t n4 = dim / 2, step = uint64_t(gx) * kT * U;
+    for (uint64_t c = uint64_t(bx) * kT * U + threadIdx.x; c < n4; c += step) {
       float4 b[U], k[U];
 #pragma unroll
       for (int q = 0; q < U; q++) {
@@ -856,8 +852,8 @@ __global__ __launch_bounds__(kT) void k_pauli(const float2 *bra, const float2 *k
     const uint64_t fe = flip & ~1ull;
     const bool odd = flip & 1ull;
     constexpr int V = U;  // pairs (i, partner) per thread and step
-    const uint64_t n8 = dim / 4, step = uint64_t(gridDim.x) * kT * V;
-    for (uint64_t c = uint64_t(blockIdx.x) * kT * V + threadIdx.x; c < n8; c += step) {
+    const uint64_t n8 = dim / 4, step = uint64_t(gx) * kT * V;
+    for (uint64_t c = uint64_t(bx) * kT * V + threadIdx.x; c < n8; c += step) {
       float4 bi[V], ki[V], bp[V], kp[V];
       uint64_t iv[V];
 #pragma unroll
@@ -891,6 +887,34 @@ __global__ __launch_bounds__(kT) void k_pauli(const float2 *bra, const float2 *k
       }
     }
   }
+}
+
+__global__ __launch_bounds__(kT) void k_pauli(const float2 *bra, const float2 *ket, uint64_t dim,
+                                              uint64_t flip, uint64_t sign, double *part) {
+  __shared__ double sh[kT / 64];
+  // Batched launches: instance blockIdx.y, states of `dim` amplitudes back to
+  // back, partials per instance.
+  bra += uint64_t(blockIdx.y) * dim;
+  ket += uint64_t(blockIdx.y) * dim;
+  part += size_t(blockIdx.y) * gridDim.x * 2;
+  double re = 0.0, im = 0.0;
+  pauli_part(bra, ket, dim, flip, sign, blockIdx.x, gridDim.x, re, im);
+  const double tr = block_sum(re, sh);
+  const double ti = block_sum(im, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = tr;
+    part[2 * blockIdx.x + 1] = ti;
+  }
+}
+
+// <a|P_t|a> for the terms of PauliTerms (blockIdx.y = term t): one launch for
+// many terms, partials of term t after t * gridDim.x.
+__global__ __launch_bounds__(kT) void k_pauli_terms(const float2 *a, uint64_t dim, PauliTerms t,
+                                                    double *part) {
+  __shared__ double sh[kT / 64];
+  part += size_t(blockIdx.y) * gridDim.x * 2;
+  double re = 0.0, im = 0.0;
+  pauli_part(a, a, dim, t.flip[blockIdx.y], t.sign[blockIdx.y], blockIdx.x, gridDim.x, re, im);
   const double tr = block_sum(re, sh);
   const double ti = block_sum(im, sh);
   if (threadIdx.x == 0) {
@@ -2206,17 +2230,32 @@ void launch_blocked(hipStream_t s, const BlockedArgs &b) {
   k_blocked<<<unsigned(g), kT, sizeof(float2) << b.bits, s>>>(b);
 }
 
+// Workgroups of the streaming reductions: enough for one loop step per
+// thread (k_abs2: kRedUnroll float4, k_pauli: two float4 of each state),
+// at most kRedBlocks; narrow states reduce fewer partials.
 void launch_abs2(hipStream_t s, const float2 *a, uint64_t dim, double *partials,
                  double *out) {
-  k_abs2<<<kRedBlocks, kT, 0, s>>>(a, dim, partials);
-  k_finish<<<1, kT, 0, s>>>(partials, kRedBlocks, 1, out);
+  const unsigned blocks = grid_for((dim / 2 + kRedUnroll - 1) / kRedUnroll, kRedBlocks);
+  k_abs2<<<blocks, kT, 0, s>>>(a, dim, partials);
+  k_finish<<<1, kT, 0, s>>>(partials, int(blocks), 1, out);
 }
 
 void launch_pauli(hipStream_t s, const float2 *bra, const float2 *ket,
                   uint64_t dim, uint64_t flip, uint64_t sign, double *partials,
                   double *out) {
-  k_pauli<<<kRedBlocks, kT, 0, s>>>(bra, ket, dim, flip, sign, partials);
-  k_finish<<<1, kT, 0, s>>>(partials, kRedBlocks, 2, out);
+  const unsigned blocks = grid_for((dim + 3) / 4, kRedBlocks);
+  k_pauli<<<blocks, kT, 0, s>>>(bra, ket, dim, flip, sign, partials);
+  k_finish<<<1, kT, 0, s>>>(partials, int(blocks), 2, out);
+}
+
+unsigned pauli_terms_blocks(uint64_t dim) { return grid_for((dim + 3) / 4, kRedBlocks); }
+
+void launch_pauli_terms(hipStream_t s, const float2 *a, uint64_t dim, const PauliTerms &t,
+                        uint32_t count, double *partials, double *out) {
+  if (!count) return;
+  const unsigned blocks = pauli_terms_blocks(dim);
+  k_pauli_terms<<<dim3(blocks, count), kT, 0, s>>>(a, dim, t, partials);
+  k_finish<<<count, kT, 0, s>>>(partials, int(blocks), 2, out);
 }
 
 void launch_pauli_accum(hipStream_t s, const AccumArgs &a) {
@@ -2420,7 +2459,7 @@ void launch_pauli_accum_batched(hipStream_t s, const AccumArgs &a, uint32_t coun
   k_pauli_accum<<<dim3(grid_for(a.dim / 2), count), kT, 0, s>>>(a);
 }
 
-unsigned pauli_batched_blocks(uint64_t dim) { return grid_for(dim / 2, kRedBlocks); }
+unsigned pauli_batched_blocks(uint64_t dim) { return grid_for((dim + 3) / 4, kRedBlocks); }
 
 unsigned adj_batched_blocks(const Layout &lay) { return grid_for(lay.nthreads, kAdjBlocks); }
 

@@ -2,6 +2,7 @@
 complex128 oracle and closed forms (reference tests/test_gpu.mm:258-480 and
 tests/test_python.py:34-99, restated)."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -112,8 +113,16 @@ def _mixed_terms(n, seed):
     return terms
 
 
+@pytest.fixture
+def tiles_always(monkeypatch):
+    """Simulators made under it take the tile passes for expectation(H)
+    however small the state (by default, below 256 MiB of per-term reads one
+    launch of per-term reductions is faster)."""
+    monkeypatch.setenv("QMETAL_EXPECT_TILES", "1")
+
+
 @pytest.mark.parametrize("n", [12, 14, 20])
-def test_hamiltonian_tile_passes_match_per_term_and_oracle(n):
+def test_hamiltonian_tile_passes_match_per_term_and_oracle(n, tiles_always):
     """expectation(H) sums tile-held terms in read-only co-state tile passes;
     it equals the one-reduction-per-term path on the same state and the
     complex128 oracle on the oracle's state."""
@@ -135,7 +144,7 @@ def test_hamiltonian_tile_passes_match_per_term_and_oracle(n):
     assert abs(got - want) <= 1e-5 * scale, (n, got, want)
 
 
-def test_hamiltonian_many_terms_per_flip_and_per_tile():
+def test_hamiltonian_many_terms_per_flip_and_per_tile(tiles_always):
     """Groups and diagonal sums larger than one tile holds: 70 diagonal
     strings, 60 terms sharing one flip, 40 single-wire groups on wires 0..6
     repeated, against the per-term path and the oracle."""
@@ -163,7 +172,7 @@ def test_hamiltonian_many_terms_per_flip_and_per_tile():
     assert abs(got - O.expect_h(ref, terms)) <= 1e-5 * scale
 
 
-def test_hamiltonian_single_term_and_diagonal_only():
+def test_hamiltonian_single_term_and_diagonal_only(tiles_always):
     """One term keeps the FP64 reduction; a diagonal-only Hamiltonian (one
     tile) and an X-only one (flip groups only) match the per-term path."""
     n = 16
@@ -180,6 +189,31 @@ def test_hamiltonian_single_term_and_diagonal_only():
             per_term = s.expectation(h)
             s.set_pass_merging(True)
             assert abs(got - per_term) <= 1e-6 * sum(abs(t[0]) for t in terms), (terms[0], got, per_term)
+
+
+def test_hamiltonian_small_state_default_path():
+    """Default path on a state below the tile threshold: all terms in one
+    launch of per-term FP64 reductions, equal to the oracle, and to the tile
+    passes on the same state to 1e-6 of sum |c|."""
+    n = 13
+    gates = C.random_circuit(n, 4, seed=0x13)
+    ref = O.run(n, gates)
+    terms = _mixed_terms(n, 0xABCD)
+    terms += [(0.01 * k, 1 << (k % n), 0, 1 << ((k + 3) % n)) for k in range(150)]
+    h = qm.Hamiltonian()
+    for t in terms:
+        h.add(*t)
+    scale = sum(abs(t[0]) for t in terms)
+    with sim_with(n, gates) as s:
+        got = s.expectation(h)
+    assert abs(got - O.expect_h(ref, terms)) <= 1e-5 * scale
+    os.environ["QMETAL_EXPECT_TILES"] = "1"
+    try:
+        with sim_with(n, gates) as s:
+            tiles = s.expectation(h)
+    finally:
+        del os.environ["QMETAL_EXPECT_TILES"]
+    assert abs(got - tiles) <= 1e-6 * scale, (got, tiles)
 
 
 def test_norm_is_fp64_accurate_at_scale():
