@@ -1927,15 +1927,20 @@ __global__ __launch_bounds__(kT) void k_block_sums(const float2 *a, uint64_t dim
   const int lane = threadIdx.x & 63;
   for (uint64_t b = uint64_t(blockIdx.x) * (kT / 64) + (threadIdx.x >> 6); b < nblocks;
        b += waves) {
-    double acc = 0.0;
+    // 16-byte loads, all eight of the lane's in flight (dim is even, so a
+    // float4 is either wholly inside the state or wholly past its end).
     const uint64_t start = b * kSampleBlock;
-    for (int k = 0; k < 16; k++) {
-      const uint64_t i = start + uint64_t(k) * 64 + lane;
-      if (i < dim) {
-        const float2 x = a[i];
-        acc += double(x.x) * x.x + double(x.y) * x.y;
-      }
+    float4 x[kSampleBlock / 128];
+#pragma unroll
+    for (int k = 0; k < int(kSampleBlock / 128); k++) {
+      const uint64_t i = start + uint64_t(k) * 128 + 2 * uint64_t(lane);
+      x[k] = i < dim ? ld4<true>(a + i) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < int(kSampleBlock / 128); k++)
+      acc += double(x[k].x) * x[k].x + double(x[k].y) * x[k].y + double(x[k].z) * x[k].z +
+             double(x[k].w) * x[k].w;
     acc = wave_sum(acc);
     if (lane == 0) sums[b] = acc;
   }
