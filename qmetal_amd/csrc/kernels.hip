@@ -1698,6 +1698,7 @@ __device__ __forceinline__ float energy_local_plain(const float2 (&x)[16]) {
   }
   return 2.0f * (acc0 + acc1);
 }
+
 // One flip group's share of the energy: partner amplitudes from LDS (slot
 // offset gs; SWAP when the flip holds amplitude bit 0), z_l = conj(x_l) p_l
 // in the parts PART asks for (bit 0 real, bit 1 imaginary), then per term
