@@ -158,7 +158,10 @@ int qm_plan_get_arrays(qm_plan p, size_t k, uint32_t *group_qubits,
  * circuit order and replaces those gates' angles; `out_energy` (1 double) and
  * `out_grad` (num_params doubles, one per rotation) are device pointers;
  * `out_grad` may be NULL, and then only the energy is computed (forward
- * passes and the Hamiltonian's reductions: no co-state, no backward sweep).
+ * passes and the Hamiltonian's reductions, no backward sweep; when every
+ * term fits a tile the energy is read off in read-only passes and no
+ * co-state is allocated, so the first call with a gradient allocates it
+ * and waits).
  * `stream` is the caller's hipStream_t (or NULL for the null stream): work
  * already queued there completes before the parameters are read, and work
  * queued there afterwards sees the results.
@@ -167,10 +170,12 @@ int qm_plan_get_arrays(qm_plan p, size_t k, uint32_t *group_qubits,
  * wires; angles come from `params`) and Hamiltonian compiles a program and
  * waits; every later call only launches kernels on `stream` (the rotation
  * matrices are written on the device from `params`): no host
- * synchronisation, no allocation, so it can be captured in a graph. Calls on
- * one simulator must be ordered (one stream, or synchronised), as they share
- * its states and program. Smaller states take the host path: the parameters
- * are copied to the host and the call waits for `stream` up to that point. */
+ * synchronisation, no allocation, so it can be captured in a graph. Device
+ * calls on one simulator must be ordered among themselves (one stream, or
+ * synchronised), as they share its states and program; the simulator's other
+ * calls (qm_run, qm_expectation, ...) wait on the GPU for an uncaptured
+ * call's kernels. Smaller states take the host path: the parameters are
+ * copied to the host and the call waits for `stream` up to that point. */
 int qm_energy_and_gradient_device(qm_simulator s, qm_circuit c,
                                   const qm_hamiltonian *h, const double *params,
                                   size_t num_params, double *out_energy,

@@ -250,7 +250,21 @@ struct Simulator::Impl {
   float2 *pick_out = nullptr;
   size_t pick_out_cap = 0;
 
-  void use() { check(hipSetDevice(device), "hipSetDevice"); }
+  // After a device-resident call (energy_and_gradient_device): its kernels
+  // ran on the caller's stream, and dev_done marks their end there.
+  hipEvent_t dev_done = nullptr;
+  bool dev_pending = false;
+
+  // Every entry point starts here: the device current and, after a
+  // device-resident call, this stream waiting (on the GPU) for its work on
+  // the caller's stream, which shares the state and co-state.
+  void use() {
+    check(hipSetDevice(device), "hipSetDevice");
+    if (dev_pending) {
+      check(hipStreamWaitEvent(stream, dev_done, 0), "hipStreamWaitEvent");
+      dev_pending = false;
+    }
+  }
 
   void launched() { check(hipGetLastError(), "kernel launch"); }
 
@@ -2638,6 +2652,7 @@ Simulator::Simulator(uint32_t n) : impl_(new Impl) {
          " GiB, only " + std::to_string(freeb >> 30) + " GiB of device memory free");
   check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
   check(hipEventCreateWithFlags(&s->staging_done, hipEventDisableTiming), "hipEventCreate");
+  check(hipEventCreateWithFlags(&s->dev_done, hipEventDisableTiming), "hipEventCreate");
   s->state = s->alloc_state(bytes, &s->place_attempts, &s->place_ratio);
   if (!s->state)
     fail("Simulator: state allocation failed (" + std::to_string(bytes >> 30) + " GiB)");
@@ -2668,6 +2683,7 @@ Simulator::~Simulator() {
   if (s->staging) hipHostFree(s->staging);
   if (s->host_results) hipHostFree(s->host_results);
   if (s->staging_done) hipEventDestroy(s->staging_done);
+  if (s->dev_done) hipEventDestroy(s->dev_done);
   for (auto &t : s->timings) {
     hipEventDestroy(t.a);
     hipEventDestroy(t.b);
@@ -3549,6 +3565,14 @@ void Simulator::energy_and_gradient_device(const Circuit &c, const Hamiltonian &
     launch_batched_results(cs, d.eslot, d.rslots, 1, out_grad ? d.nrot : 0, d.rgate, d.inst,
                            d.inst, out_energy, out_grad);
     check(hipGetLastError(), "device gradient launch");
+    // Later work on this simulator's own stream waits for these kernels (a
+    // captured call is ordered by whoever replays the graph).
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    check(hipStreamIsCapturing(cs, &cap), "hipStreamIsCapturing");
+    if (cap == hipStreamCaptureStatusNone) {
+      check(hipEventRecord(s->dev_done, cs), "hipEventRecord(device program)");
+      s->dev_pending = true;
+    }
     s->launches += n_launch + 1;
     s->dispatches += n_launch + 1;
     s->dev_launches = s->launches;

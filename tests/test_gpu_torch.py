@@ -154,6 +154,58 @@ def test_energy_only_program_under_graph_capture():
             assert abs(e_r.item() - e_full.item()) <= 1e-6
 
 
+def test_energy_only_with_a_term_no_tile_holds():
+    """A Hamiltonian with a string flipping every wire (too wide for a
+    tile): the energy-only device call builds the co-state instead, and
+    agrees with the full call and the host path."""
+    from qmetal_amd.torch_adjoint import energy_and_grad
+    n = 14
+    rng = C.SplitMix64(44)
+    p0 = [rng.uniform(0, 2 * np.pi) for _ in range(4 * n)]
+    gates = hea_template(n, 2, p0)
+    circ = C.to_circuit(n, gates)
+    terms = C.tfim_ring_hamiltonian(n) + [(0.8, (1 << n) - 1, 0, 0)]
+    ham = C.to_hamiltonian(terms)
+    with qm.Simulator(n) as sim:
+        theta = torch.tensor(p0, dtype=torch.float64, device="cuda")
+        e_only, g_none = energy_and_grad(sim, circ, ham, theta, need_grad=False)
+        e_full, _ = energy_and_grad(sim, circ, ham, theta)
+        e_host, _ = sim.energy_and_gradient(circ, ham)
+    assert g_none is None
+    assert abs(e_only.item() - e_full.item()) <= 1e-6
+    assert abs(e_only.item() - e_host) <= 1e-6
+
+
+def test_host_path_after_device_calls_without_synchronising():
+    """Device-resident calls return before their kernels (on the caller's
+    stream) finish; host-path calls that follow at once, on the simulator's
+    own stream, wait for them: gradient, expectation and norm right after
+    device calls equal the complex128 oracle."""
+    from qmetal_amd.torch_adjoint import energy_and_grad
+    n = 14
+    rng = C.SplitMix64(45)
+    p0 = [rng.uniform(0, 2 * np.pi) for _ in range(4 * n)]
+    gates = hea_template(n, 2, p0)
+    circ = C.to_circuit(n, gates)
+    terms = C.tfim_ring_hamiltonian(n) + [(0.8, (1 << n) - 1, 0, 0)]
+    ham = C.to_hamiltonian(terms)
+    want = O.expect_h(O.run(n, gates), terms)
+    with qm.Simulator(n) as sim:
+        theta = torch.tensor(p0, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            e_only, _ = energy_and_grad(sim, circ, ham, theta, need_grad=False)
+            e_full, _ = energy_and_grad(sim, circ, ham, theta)
+            e_host, _ = sim.energy_and_gradient(circ, ham)
+            e_dev2, _ = energy_and_grad(sim, circ, ham, theta)
+            sim.reset()
+            sim.run(circ)
+            e_exp = sim.expectation(ham)
+            nrm = sim.norm()
+            for v in (e_only.item(), e_full.item(), e_host, e_dev2.item(), e_exp):
+                assert abs(v - want) <= 1e-5, (v, want)
+            assert abs(nrm - 1) <= 1e-5
+
+
 def _with_angles(gates, theta):
     """The template with its RX/RY/RZ angles replaced, in circuit order."""
     it = iter(theta)
