@@ -206,6 +206,36 @@ def test_host_path_after_device_calls_without_synchronising():
             assert abs(nrm - 1) <= 1e-5
 
 
+def test_device_program_recompiles_when_template_changes():
+    """One simulator alternating between two circuit templates and two
+    Hamiltonians (each change compiles a new program, the old one retired
+    after its kernels), energy only and with gradients, without
+    synchronising: every call equals the host path on the same angles."""
+    from qmetal_amd.torch_adjoint import energy_and_grad
+    n = 13
+    rng = C.SplitMix64(46)
+    pa = [rng.uniform(0, 2 * np.pi) for _ in range(4 * n)]
+    ga = hea_template(n, 2, pa)
+    pb = [rng.uniform(0, np.pi) for _ in range(4)]
+    gb = C.qaoa_ring(n, 2, gammas=pb[:2], betas=pb[2:])
+    qb = [p[0] for k, _, p in gb if k in (C.RX, C.RY, C.RZ)]
+    hs = [C.to_hamiltonian(C.tfim_ring_hamiltonian(n)),
+          C.to_hamiltonian(C.ring_zz_hamiltonian(n) + [(0.5, (1 << n) - 1, 0, 0)])]
+    with qm.Simulator(n) as sim:
+        for k in range(6):
+            gates, params = (ga, pa) if k % 2 == 0 else (gb, qb)
+            circ = C.to_circuit(n, gates)
+            ham = hs[(k // 2) % 2]
+            theta = torch.tensor(params, dtype=torch.float64, device="cuda")
+            need = k % 3 != 1
+            e_dev, g_dev = energy_and_grad(sim, circ, ham, theta, need_grad=need)
+            e_host, g_host = sim.energy_and_gradient(circ, ham)
+            rot = [j for j, (kind, _, _) in enumerate(gates) if kind in (C.RX, C.RY, C.RZ)]
+            assert abs(e_dev.item() - e_host) <= 1e-5, (k, e_dev.item(), e_host)
+            if need:
+                assert np.abs(g_dev.cpu().numpy() - np.array(g_host)[rot]).max() <= 1e-5, k
+
+
 def _with_angles(gates, theta):
     """The template with its RX/RY/RZ angles replaced, in circuit order."""
     it = iter(theta)
