@@ -214,3 +214,30 @@ def test_batched_classes_run_the_merged_passes():
                     for i, (e0, g0) in enumerate(want))
         print(f"spread from {spread}: {batched} batched instances, worst diff vs plain {worst:.2e}")
         assert worst <= 1e-5, spread
+
+
+def test_batched_classes_with_terms_no_tile_holds():
+    """Batched classes whose Hamiltonian has flip groups on high wires, Y
+    terms and a string flipping every wire (no tile holds it: the batched
+    generic co-state launches), several instances per class at 15 and 16
+    qubits: against the one-circuit path, replays bit-identical."""
+    rng = C.SplitMix64(78)
+    inst = []
+    for n in (15, 16, 15, 16, 15):
+        ga = [rng.uniform(0, 3.14) for _ in range(2)]
+        be = [rng.uniform(0, 1.57) for _ in range(2)]
+        h = (C.tfim_ring_hamiltonian(n) + [(0.3, 0, 1 << q, 0) for q in range(0, n, 3)]
+             + [(0.2, (1 << 9) | (1 << 12), 0, 1)] + [(0.4, (1 << n) - 1, 0, 0)])
+        inst.append((n, C.qaoa_ring(n, 2, ga, be), h))
+    circs = [C.to_circuit(n, g) for n, g, _ in inst]
+    hams = [C.to_hamiltonian(h) for _, _, h in inst]
+    want = [plain(n, g, h) for n, g, h in inst]
+    b = qm.Batch().prepare(circs, hams)
+    b.set_spread(14)
+    assert b.batched_instances() == len(inst)
+    e1, g1, _ = b.run()
+    e2, g2, _ = b.run()
+    assert np.array_equal(e1, e2) and all(np.array_equal(x, y) for x, y in zip(g1, g2))
+    worst = max(max(abs(e1[i] - e0), np.abs(g1[i] - g0).max()) for i, (e0, g0) in enumerate(want))
+    print(f"batched classes with wide terms: worst diff vs plain {worst:.2e}")
+    assert worst <= 1e-5
