@@ -172,6 +172,42 @@ def test_hamiltonian_many_terms_per_flip_and_per_tile(tiles_always):
     assert abs(got - O.expect_h(ref, terms)) <= 1e-5 * scale
 
 
+@pytest.mark.parametrize("n", [16, 20])
+def test_jordan_wigner_hopping_terms(n, tiles_always):
+    """Jordan-Wigner hopping and pairing strings X_a Z..Z X_b, Y_a Z..Z Y_b,
+    X_a Z..Z Y_b (two flipped wires, Z on every wire between: signs over
+    lanes, waves, registers and tile bits) plus number terms Z_a: through the
+    tile passes against the complex128 oracle."""
+    gates = C.random_circuit(n, 4, seed=0x4A + n)
+    ref = O.run(n, gates)
+    rng = C.SplitMix64(0x4A57 + n)
+    terms = []
+    for _ in range(60):
+        a, b = sorted(rng.next_u64() % n for _ in range(2))
+        if a == b:
+            continue
+        zs = ((1 << b) - 1) & ~((1 << (a + 1)) - 1)
+        c = rng.uniform(-1, 1)
+        kind = rng.next_u64() % 3
+        if kind == 0:
+            terms.append((c, (1 << a) | (1 << b), 0, zs))
+        elif kind == 1:
+            terms.append((c, 0, (1 << a) | (1 << b), zs))
+        else:
+            terms.append((c, 1 << a, 1 << b, zs))
+    terms += [(rng.uniform(-1, 1), 0, 0, 1 << q) for q in range(n)]
+    h = qm.Hamiltonian()
+    for t in terms:
+        h.add(*t)
+    scale = sum(abs(t[0]) for t in terms)
+    with sim_with(n, gates) as s:
+        got = s.expectation(h)
+        s.set_pass_merging(False)
+        per_term = s.expectation(h)
+    assert abs(got - per_term) <= 1e-6 * scale, (got, per_term)
+    assert abs(got - O.expect_h(ref, terms)) <= 1e-5 * scale
+
+
 def test_hamiltonian_single_term_and_diagonal_only(tiles_always):
     """One term keeps the FP64 reduction; a diagonal-only Hamiltonian (one
     tile) and an X-only one (flip groups only) match the per-term path."""
