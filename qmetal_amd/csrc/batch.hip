@@ -1235,15 +1235,15 @@ struct Batch::Impl {
   double ms = 0.0;
 
   void free_spread() {
-    hipFree(sp_step_off);
-    hipFree(sp_nsteps);
-    hipFree(sp_steps);
+    (void)hipFree(sp_step_off);
+    (void)hipFree(sp_nsteps);
+    (void)hipFree(sp_steps);
     sp_step_off = sp_nsteps = sp_steps = nullptr;
-    hipFree(sp_inst);
-    hipFree(sp_wg_base);
-    hipFree(sp_wg_inst);
-    hipFree(sp_state);
-    hipFree(sp_scratch);
+    (void)hipFree(sp_inst);
+    (void)hipFree(sp_wg_base);
+    (void)hipFree(sp_wg_inst);
+    (void)hipFree(sp_state);
+    (void)hipFree(sp_scratch);
     hipFree(sp_part);
     sp_inst = sp_wg_base = sp_wg_inst = nullptr;
     sp_state = nullptr;
@@ -1470,15 +1470,15 @@ struct Batch::Impl {
     h_ham.clear();
     h_gbase.clear();
     active.clear();
-    hipFree(ops);
-    hipFree(insts);
-    hipFree(order);
-    hipFree(sample_order);
-    hipFree(terms);
-    hipFree(mats);
-    hipFree(aux);
-    hipFree(energy);
-    hipFree(grad);
+    (void)hipFree(ops);
+    (void)hipFree(insts);
+    (void)hipFree(order);
+    (void)hipFree(sample_order);
+    (void)hipFree(terms);
+    (void)hipFree(mats);
+    (void)hipFree(aux);
+    (void)hipFree(energy);
+    (void)hipFree(grad);
     ops = nullptr;
     insts = nullptr;
     order = nullptr;
@@ -1643,16 +1643,16 @@ Batch::~Batch() {
   Impl *b = impl_.get();
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
   b->free_all();
-  hipFree(b->scratch);
-  hipFree(b->counter);
-  if (b->ev0) hipEventDestroy(b->ev0);
-  if (b->ev1) hipEventDestroy(b->ev1);
-  if (b->fork) hipEventDestroy(b->fork);
-  if (b->join) hipEventDestroy(b->join);
-  if (b->side) hipStreamDestroy(b->side);
-  if (b->stream) hipStreamDestroy(b->stream);
+  (void)hipFree(b->scratch);
+  (void)hipFree(b->counter);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
+  if (b->side) (void)hipStreamDestroy(b->side);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
 void Batch::prepare(const std::vector<const Circuit *> &circuits,

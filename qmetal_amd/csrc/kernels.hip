@@ -1962,7 +1962,7 @@ __global__ __launch_bounds__(kT) void k_scan_chunks(double *v, uint64_t count, d
   __syncthreads();
   // Hillis-Steele over the 256 per-thread totals.
   for (int o = 1; o < kT; o <<= 1) {
-    const double add = threadIdx.x >= o ? sh[threadIdx.x - o] : 0.0;
+    const double add = int(threadIdx.x) >= o ? sh[threadIdx.x - o] : 0.0;
     __syncthreads();
     sh[threadIdx.x] += add;
     __syncthreads();

@@ -348,7 +348,7 @@ struct Simulator::Impl {
   // fixed-one bits so only their half of the state is touched.
   void controlled(const std::vector<uint32_t> &ctrls, uint32_t target,
                   const float *m8, float2 *second) {
-    LayoutBuilder lb{n};
+    LayoutBuilder lb{n, {}};
     const int lt = lb.add(target);
     uint32_t cm = 0;
     for (uint32_t c : ctrls) {
@@ -359,7 +359,7 @@ struct Simulator::Impl {
   }
 
   void swap(uint32_t a, uint32_t b, float2 *second) {
-    LayoutBuilder lb{n};
+    LayoutBuilder lb{n, {}};
     const int la = lb.add(a), lbb = lb.add(b);
     gather(lb,
            {make_slot(lbb, 1u << la, kX), make_slot(la, 1u << lbb, kX),
@@ -369,7 +369,7 @@ struct Simulator::Impl {
 
   void group(const std::vector<uint32_t> &wires, const std::vector<cd> &mats,
              float2 *second) {
-    LayoutBuilder lb{n};
+    LayoutBuilder lb{n, {}};
     std::vector<Slot> slots;
     for (size_t k = 0; k < wires.size(); k++) {
       float m8[8];
@@ -661,7 +661,7 @@ struct Simulator::Impl {
   // Lays out a merged pass: register wires first, fixed bits, fill bits at
   // the top; then one PassOp per 2x2 slot or phase layer, in plan order.
   void prep_pass(const Plan &p, const Pass &ps, size_t off, PassLaunch &pl) {
-    LayoutBuilder lb{n};
+    LayoutBuilder lb{n, {}};
     lb.fixed = ps.fixed;
     for (uint32_t q : ps.reg) lb.add(q);
     const int want = lb.reg.size() + 1 <= 4 ? 4 : 5;
@@ -1344,13 +1344,13 @@ struct Simulator::Impl {
     double *eslot = nullptr, *rslots = nullptr;
     uint32_t *rgate = nullptr, *inst = nullptr;
     ~DevProg() {
-      hipFree(arena);
-      hipFree(patch);
-      hipFree(eslots);
-      hipFree(eslot);
-      hipFree(rslots);
-      hipFree(rgate);
-      hipFree(inst);
+      (void)hipFree(arena);
+      (void)hipFree(patch);
+      (void)hipFree(eslots);
+      (void)hipFree(eslot);
+      (void)hipFree(rslots);
+      (void)hipFree(rgate);
+      (void)hipFree(inst);
     }
   };
   std::unique_ptr<DevProg> dev_prog;
@@ -1918,7 +1918,7 @@ struct Simulator::Impl {
 
   // AdjOps of backward pass `ps` into `ops`; sets a.lay, a.nops, a.nrot.
   void emit_adj_pass(const Circuit &c, const APass &ps, AdjOp *ops, AdjPassArgs &a) const {
-    LayoutBuilder lb{n};
+    LayoutBuilder lb{n, {}};
     for (uint32_t q : ps.reg) lb.add(q);
     int nl = 0;
     a.lay = lb.finish(kAdjMaxReg + 1, &nl);
@@ -2374,7 +2374,7 @@ struct Simulator::Impl {
       pl.g.global = turns(op.global);
       return;
     }
-    LayoutBuilder lb{n};
+    LayoutBuilder lb{n, {}};
     // With no global phase, amplitudes missing any common mask bit keep
     // phase 0 and are never touched.
     if (turns(op.global) == 0.0) lb.fixed = common & ~((1ull << kDiagFixedMin) - 1ull);
@@ -2669,27 +2669,27 @@ Simulator::~Simulator() {
     (void)hipDeviceSynchronize();  // its kernels ran on the callers' streams
     s->dev_prog.reset();
   }
-  hipFree(s->state);
-  hipFree(s->costate);
-  hipFree(s->partials);
-  hipFree(s->results);
-  hipFree(s->arena);
-  hipFree(s->cdf);
-  hipFree(s->adj_partials);
-  hipFree(s->chunks);
-  hipFree(s->shots_dev);
-  hipFree(s->pick_idx);
-  hipFree(s->pick_out);
-  if (s->staging) hipHostFree(s->staging);
-  if (s->host_results) hipHostFree(s->host_results);
-  if (s->staging_done) hipEventDestroy(s->staging_done);
-  if (s->dev_done) hipEventDestroy(s->dev_done);
+  (void)hipFree(s->state);
+  (void)hipFree(s->costate);
+  (void)hipFree(s->partials);
+  (void)hipFree(s->results);
+  (void)hipFree(s->arena);
+  (void)hipFree(s->cdf);
+  (void)hipFree(s->adj_partials);
+  (void)hipFree(s->chunks);
+  (void)hipFree(s->shots_dev);
+  (void)hipFree(s->pick_idx);
+  (void)hipFree(s->pick_out);
+  if (s->staging) (void)hipHostFree(s->staging);
+  if (s->host_results) (void)hipHostFree(s->host_results);
+  if (s->staging_done) (void)hipEventDestroy(s->staging_done);
+  if (s->dev_done) (void)hipEventDestroy(s->dev_done);
   for (auto &t : s->timings) {
-    hipEventDestroy(t.a);
-    hipEventDestroy(t.b);
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
   }
-  for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
-  if (s->stream) hipStreamDestroy(s->stream);
+  for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
 uint32_t Simulator::num_qubits() const { return impl_->n; }
@@ -3694,16 +3694,16 @@ struct BatchedAdjoint::Data {
   std::vector<size_t> bwd_rot0;
 
   ~Data() {
-    hipFree(psi);
-    hipFree(lam);
-    hipFree(arena);
-    hipFree(red);
-    hipFree(adj);
-    hipFree(eslots);
-    hipFree(rslots);
-    hipFree(rgate);
-    hipFree(inst);
-    hipFree(gbase);
+    (void)hipFree(psi);
+    (void)hipFree(lam);
+    (void)hipFree(arena);
+    (void)hipFree(red);
+    (void)hipFree(adj);
+    (void)hipFree(eslots);
+    (void)hipFree(rslots);
+    (void)hipFree(rgate);
+    (void)hipFree(inst);
+    (void)hipFree(gbase);
   }
 };
 
