@@ -277,16 +277,13 @@ __device__ __forceinline__ const_ptr<T> as_const(const T *p) {
   return (const_ptr<T>)p;
 }
 
-// One chain step on tile-local bit BIT: the 2x2 (m) on all pairs, then the
-// phase of the combinations holding BIT, e * lo[low two other bits] *
-// hi[the other bits above them] (other local bits ascending).
+// The 2x2 of a chain step (m) on all pairs of tile-local bit BIT.
 // MODE: 0 complex 2x2, 1 real 2x2, 2 the unscaled butterfly [[1,1],[1,-1]]
 // (the scale of a run of such steps is carried by the run's last step,
 // which uses mode 1).
 template <int NL, int BIT, int MODE>
-__device__ __forceinline__ void chain_step(float2 (&x)[1 << NL],
-                                           const ChainStep __attribute__((address_space(4))) &st,
-                                           float2 e) {
+__device__ __forceinline__ void chain_mix(float2 (&x)[1 << NL],
+                                          const ChainStep __attribute__((address_space(4))) &st) {
   if constexpr (MODE == 2) {
 #pragma unroll
     for (int l = 0; l < (1 << NL); l++) {
@@ -306,6 +303,16 @@ __device__ __forceinline__ void chain_step(float2 (&x)[1 << NL],
       else bfly_cplx_s(m, x[l], x[l | (1 << BIT)]);
     }
   }
+}
+
+// One chain step on tile-local bit BIT: the 2x2, then the phase of the
+// combinations holding BIT, e * lo[low two other bits] * hi[the other bits
+// above them] (other local bits ascending).
+template <int NL, int BIT, int MODE>
+__device__ __forceinline__ void chain_step(float2 (&x)[1 << NL],
+                                           const ChainStep __attribute__((address_space(4))) &st,
+                                           float2 e) {
+  chain_mix<NL, BIT, MODE>(x, st);
   float2 a[4];
   a[0] = e;
 #pragma unroll

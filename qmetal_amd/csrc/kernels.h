@@ -288,6 +288,11 @@ void launch_chain(hipStream_t s, int nl, int k, bool zero, bool real, const Chai
 // steps = 6 or 7.
 void launch_wide(hipStream_t s, int steps, int mode, const WideArgs &w);
 // s in 5..12 steps on wires 1..10+wb, `zero`: a last step on wire 0.
+// Low passes take one more mode, kLowModeTri: mode 2 with triangular phase
+// tables (each step's phase ignores the register bits of its arrangement's
+// earlier steps, and the wire-0 step's ignores all four), which need one
+// complex product per amplitude and step.
+constexpr int kLowModeTri = 3;
 void launch_low(hipStream_t s, int steps, bool zero, int mode, int wb, const LowArgs &l);
 // The low chain kernels for steps S0 and S0 + 1 (S0 = 5, 7, 9, 11).
 template <int S0>

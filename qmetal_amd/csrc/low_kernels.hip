@@ -1,6 +1,6 @@
 // low_kernels.hip -- the low chain pass kernels (k_low) and their launchers.
 // Built once per pair of step counts (QM_LOW_S0 = 5, 7, 9, 11; both counts of
-// a pair in one object) so that the 96 kernel instances compile in parallel;
+// a pair in one object) so that the 128 kernel instances compile in parallel;
 // without QM_LOW_S0 the file holds every pair.
 
 #include <hip/hip_runtime.h>
@@ -74,11 +74,66 @@ __device__ __forceinline__ void low_permute(const LowArgs &p, float4 *lds, float
   }
 }
 
+// Other-bit index c of a step on local bit BIT (the four other local bits,
+// ascending) -> the c bits its phase ignores in a triangular chain: the
+// register bits below BIT, which hold the wires of this arrangement's earlier
+// steps, and for the wire-0 step (BIT 0) all four.
+template <int BIT>
+constexpr uint32_t tri_ignored() {
+  uint32_t m = 0;
+  for (int j = 0; j < 4; j++) {
+    const int q = BIT == 0 ? j + 1 : (j < BIT ? j : j + 1);
+    if (BIT == 0 || (q >= 1 && q < BIT)) m |= 1u << j;
+  }
+  return m;
+}
+
+// The phase of a chain step in a triangular chain (descending wires, each
+// step's phase reading only lower wires, as in the QFT): the factor of
+// combination c is e * hi * lo over the bits the phase reads, formed once per
+// distinct value, then one complex product per amplitude (against two in
+// chain_step). The host checks the step tables before choosing it.
+template <int BIT>
+__device__ __forceinline__ void chain_phase_tri(float2 (&x)[32],
+                                                const ChainStep __attribute__((address_space(4))) &st,
+                                                float2 e) {
+  constexpr uint32_t TM = tri_ignored<BIT>();
+  float2 ah[4], f[16];
+  ah[0] = e;
+#pragma unroll
+  for (int h = 1; h < 4; h++)
+    if (!((uint32_t(h) << 2) & TM)) ah[h] = cmul(e, make_float2(st.hi[2 * h - 2], st.hi[2 * h - 1]));
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    if (uint32_t(c) & TM) continue;
+    const int lo = c & 3, hi = c >> 2;
+    f[c] = lo ? cmul(ah[hi], make_float2(st.lo[2 * lo - 2], st.lo[2 * lo - 1])) : ah[hi];
+  }
+#pragma unroll
+  for (int l = 0; l < 32; l++) {
+    if (!(l & (1 << BIT))) continue;
+    const uint32_t c = uint32_t((l & ((1 << BIT) - 1)) | ((l >> (BIT + 1)) << BIT));
+    cmul_v(x[l], f[c & ~TM]);
+  }
+}
+
+template <int BIT, int M, bool TRI>
+__device__ __forceinline__ void low_step(float2 (&x)[32],
+                                         const ChainStep __attribute__((address_space(4))) &st,
+                                         float2 e) {
+  if constexpr (TRI) {
+    chain_mix<5, BIT, M>(x, st);
+    chain_phase_tri<BIT>(x, st, e);
+  } else {
+    chain_step<5, BIT, M>(x, st, e);
+  }
+}
+
 // Steps I.. of a low chain pass as straight-line code (template recursion,
 // so that every step index is a compile-time constant however long the
 // chain): step i runs in arrangement 1 + i / 4 on register bit 1 + i % 4,
 // the wire-0 step (i == S) on bit 0 of the last arrangement.
-template <int I, int S, int T, int MODE, int WB>
+template <int I, int S, int T, int MODE, int WB, bool TRI>
 __device__ __forceinline__ void low_steps(const LowArgs &p, float4 *lds, float2 (&x)[32],
                                           const float2 (&e)[T], const_ptr<ChainStep> steps,
                                           uint32_t lane, uint32_t wave) {
@@ -87,16 +142,17 @@ __device__ __forceinline__ void low_steps(const LowArgs &p, float4 *lds, float2 
     if constexpr (I == 8 && S > 8) low_permute<2, 3, WB>(p, lds, x, lane, wave);
     const ChainStep __attribute__((address_space(4))) &st = steps[opaque_step(I, x[0].x)];
     constexpr int M = (MODE == 2 && I == T - 1) ? 1 : MODE;
-    if constexpr (I >= S) chain_step<5, 0, M>(x, st, e[I]);
-    else chain_step<5, 1 + I % 4, M>(x, st, e[I]);
-    low_steps<I + 1, S, T, MODE, WB>(p, lds, x, e, steps, lane, wave);
+    if constexpr (I >= S) low_step<0, M, TRI>(x, st, e[I]);
+    else low_step<1 + I % 4, M, TRI>(x, st, e[I]);
+    low_steps<I + 1, S, T, MODE, WB, TRI>(p, lds, x, e, steps, lane, wave);
   }
 }
 
 // Low chain pass (LowArgs): 2^WB waves per workgroup tile of 2^(11+WB)
-// amplitudes, 2^(15+WB) bytes of LDS. The next tile's loads are issued
-// before the current tile is worked on.
-template <int S, bool Z, int MODE, int WB>
+// amplitudes, 2^(14+WB) bytes of LDS. The next tile's loads are issued
+// before the current tile is worked on. TRI: triangular phase tables
+// (chain_phase_tri).
+template <int S, bool Z, int MODE, int WB, bool TRI>
 __global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
   constexpr int T = S + (Z ? 1 : 0);
   constexpr int A = (S + 3) / 4;  // step arrangements
@@ -200,7 +256,7 @@ __global__ __launch_bounds__(64 << WB) void k_low(LowArgs p) {
       }
     }
     if (!p.first_in_place) low_permute<0, 1, WB>(p, lds, x, lane, wave);
-    low_steps<0, S, T, MODE, WB>(p, lds, x, e, steps, lane, wave);
+    low_steps<0, S, T, MODE, WB, TRI>(p, lds, x, e, steps, lane, wave);
     low_permute<A, 0, WB>(p, lds, x, lane, wave);
 #pragma unroll
     for (int r = 0; r < 16; r++)
@@ -220,13 +276,15 @@ void launch_low_s(hipStream_t s, bool zero, int mode, const LowArgs &l) {
   const unsigned grid = tile_grid(l.lay.nthreads, 4, kLowGrid >> (WB - 1));
   constexpr int threads = 64 << WB;
   if (zero) {
-    if (mode == 2) k_low<S, true, 2, WB><<<grid, threads, 0, s>>>(l);
-    else if (mode == 1) k_low<S, true, 1, WB><<<grid, threads, 0, s>>>(l);
-    else k_low<S, true, 0, WB><<<grid, threads, 0, s>>>(l);
+    if (mode == kLowModeTri) k_low<S, true, 2, WB, true><<<grid, threads, 0, s>>>(l);
+    else if (mode == 2) k_low<S, true, 2, WB, false><<<grid, threads, 0, s>>>(l);
+    else if (mode == 1) k_low<S, true, 1, WB, false><<<grid, threads, 0, s>>>(l);
+    else k_low<S, true, 0, WB, false><<<grid, threads, 0, s>>>(l);
   } else {
-    if (mode == 2) k_low<S, false, 2, WB><<<grid, threads, 0, s>>>(l);
-    else if (mode == 1) k_low<S, false, 1, WB><<<grid, threads, 0, s>>>(l);
-    else k_low<S, false, 0, WB><<<grid, threads, 0, s>>>(l);
+    if (mode == kLowModeTri) k_low<S, false, 2, WB, true><<<grid, threads, 0, s>>>(l);
+    else if (mode == 2) k_low<S, false, 2, WB, false><<<grid, threads, 0, s>>>(l);
+    else if (mode == 1) k_low<S, false, 1, WB, false><<<grid, threads, 0, s>>>(l);
+    else k_low<S, false, 0, WB, false><<<grid, threads, 0, s>>>(l);
   }
 }
 

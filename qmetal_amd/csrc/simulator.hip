@@ -822,6 +822,33 @@ struct Simulator::Impl {
     for (int cc = 0; cc < 6; cc++) c.lo[cc] = c.hi[cc] = (cc % 2 == 0) ? 1.0f : 0.0f;
   }
 
+  // True when the phase tables of a low pass's steps are triangular: step i
+  // (local bit 1 + i % 4, or 0 for the wire-0 step i == S) has factors that
+  // do not change with the register bits below its own (the wires of its
+  // arrangement's earlier steps), nor, for the wire-0 step, with any
+  // register bit. Exact float comparisons: the low kernel's triangular form
+  // then skips only factors that are exactly 1.
+  static bool low_tables_triangular(const ChainStep *cs, int S, int T) {
+    for (int i = 0; i < T; i++) {
+      const int b = i < S ? 1 + i % 4 : 0;
+      uint32_t ignored = 0;  // bits of the other-bit index c
+      for (int j = 0; j < 4; j++) {
+        const int q = b == 0 ? j + 1 : (j < b ? j : j + 1);
+        if (b == 0 || (q >= 1 && q < b)) ignored |= 1u << j;
+      }
+      for (int g = 0; g < 2; g++) {
+        const float *t = g ? cs[i].hi : cs[i].lo;
+        const uint32_t gm = (ignored >> (2 * g)) & 3u;
+        for (uint32_t cc = 1; cc < 4; cc++) {
+          const uint32_t r = cc & ~gm;
+          const float re = r ? t[2 * r - 2] : 1.0f, im = r ? t[2 * r - 1] : 0.0f;
+          if (t[2 * cc - 2] != re || t[2 * cc - 1] != im) return false;
+        }
+      }
+    }
+    return true;
+  }
+
   static bool steps_real(const ChainStep *cs, int count) {
     for (int i = 0; i < count; i++)
       if (cs[i].m[1] != 0.0f || cs[i].m[3] != 0.0f || cs[i].m[5] != 0.0f || cs[i].m[7] != 0.0f)
@@ -1069,6 +1096,7 @@ struct Simulator::Impl {
     pl.chain_k = S;
     pl.chain_zero = zero;
     pl.mode = steps_mode(cs, T);
+    if (pl.mode == 2 && low_tables_triangular(cs, S, T)) pl.mode = kLowModeTri;
     pl.nl = 5;
     pl.bytes = pass_bytes(n, 0);
   }

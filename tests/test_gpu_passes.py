@@ -551,6 +551,55 @@ def test_low_passes_against_oracle_and_op_by_op():
     O.set_threads(1)
 
 
+def tri_stream(n, rounds, seed, top):
+    """Rounds of H steps on wires top..1 (descending) then 0, each followed
+    by random-angle CP / CZ / P gates whose other wire is lower than the
+    step's or above the tile: every step's phase then ignores the wires of
+    the earlier steps (the QFT's form, with arbitrary angles), which the low
+    kernel runs with one complex product per amplitude and step. Round
+    lengths vary so that the last arrangement is partly filled."""
+    r = C.SplitMix64(seed)
+    gates = []
+    for i in range(rounds):
+        count = top - (r.next_u64() % 4 if i % 2 else 0)
+        for w in list(range(top, top - count, -1)) + [0]:
+            gates.append((C.H, (w,), ()))
+            # At least two two-qubit phase gates, so that the plan keeps
+            # them as a phase layer after the step's Dense1q.
+            for j in range(2 + r.next_u64() % 3):
+                lower = w > 0 and r.next_u64() % 3 != 0
+                k = r.next_u64() % w if lower else top + 1 + r.next_u64() % (n - top - 1)
+                u = r.next_u64() % (2 if j < 2 else 3)
+                if u == 0:
+                    gates.append((C.CP, (k, w), (r.uniform(-3.2, 3.2),)))
+                elif u == 1:
+                    gates.append((C.CZ, (w, k), ()))
+                else:
+                    gates.append((C.P, (w,), (r.uniform(-3.2, 3.2),)))
+        gates.append((C.CX, (n - 1, 0), ()))
+    return gates
+
+
+@pytest.mark.parametrize("n,top", [(14, 11), (14, 12), (17, 12), (24, 12)])
+def test_triangular_low_chains_against_oracle_and_op_by_op(n, top):
+    """Low passes whose phase tables are triangular (tri_stream) against the
+    complex128 oracle and one launch per plan op, over two- and four-wave
+    tiles."""
+    prep = C.product_state_prep(n, seed=5 * n + top)
+    gates = tri_stream(n, 3, 91 * n + top, top)
+    c = C.to_circuit(n, gates)
+    low = [sh for sh in shapes(c) if sh == "chain"]
+    assert len(low) >= 3, "low passes not formed"
+    want = O.run(n, prep + gates)
+    got, norm, launches = run(n, gates, prep=prep)
+    ref, _, _ = run(n, gates, merge=False, prep=prep)
+    d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
+    print(f"triangular low n={n} top={top}: {len(low)} low passes, {launches} launches, "
+          f"diff {d:.2e}, vs op-by-op {dm:.2e}")
+    assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
+    assert dm <= TOL_SAME
+
+
 def tile_stream(n, count, seed):
     """Every mergeable gate kind with targets on wires 0..11 and controls,
     swap partners and phase wires anywhere, in runs long enough to need
