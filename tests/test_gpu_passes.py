@@ -551,19 +551,24 @@ def test_low_passes_against_oracle_and_op_by_op():
     O.set_threads(1)
 
 
-def tri_stream(n, rounds, seed, top):
+def tri_stream(n, rounds, seed, top, upward=False):
     """Rounds of H steps on wires top..1 (descending) then 0, each followed
     by random-angle CP / CZ / P gates whose other wire is lower than the
     step's or above the tile: every step's phase then ignores the wires of
     the earlier steps (the QFT's form, with arbitrary angles), which the low
     kernel runs with one complex product per amplitude and step. Round
-    lengths vary so that the last arrangement is partly filled."""
+    lengths vary so that the last arrangement is partly filled. `upward`:
+    the third step of every round also takes a CP with the second step's
+    wire (an earlier register wire of its arrangement), so the tables are no
+    longer triangular and the low kernel must take its general form."""
     r = C.SplitMix64(seed)
     gates = []
     for i in range(rounds):
         count = top - (r.next_u64() % 4 if i % 2 else 0)
-        for w in list(range(top, top - count, -1)) + [0]:
+        for p, w in enumerate(list(range(top, top - count, -1)) + [0]):
             gates.append((C.H, (w,), ()))
+            if upward and p == 2:
+                gates.append((C.CP, (w + 1, w), (r.uniform(0.5, 3.0),)))
             # At least two two-qubit phase gates, so that the plan keeps
             # them as a phase layer after the step's Dense1q.
             for j in range(2 + r.next_u64() % 3):
@@ -595,6 +600,26 @@ def test_triangular_low_chains_against_oracle_and_op_by_op(n, top):
     ref, _, _ = run(n, gates, merge=False, prep=prep)
     d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
     print(f"triangular low n={n} top={top}: {len(low)} low passes, {launches} launches, "
+          f"diff {d:.2e}, vs op-by-op {dm:.2e}")
+    assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
+    assert dm <= TOL_SAME
+
+
+@pytest.mark.parametrize("n,top", [(14, 11), (17, 12)])
+def test_nearly_triangular_low_chains_take_the_general_form(n, top):
+    """tri_stream with one upward coupling per round: the host's table check
+    must reject the triangular form (a kernel that skipped the coupling
+    would miss a phase of 0.5..3 rad on a quarter of the state)."""
+    prep = C.product_state_prep(n, seed=7 * n + top)
+    gates = tri_stream(n, 3, 53 * n + top, top, upward=True)
+    c = C.to_circuit(n, gates)
+    low = [sh for sh in shapes(c) if sh == "chain"]
+    assert len(low) >= 3, "low passes not formed"
+    want = O.run(n, prep + gates)
+    got, norm, _ = run(n, gates, prep=prep)
+    ref, _, _ = run(n, gates, merge=False, prep=prep)
+    d, dm = np.abs(got - want).max(), np.abs(got - ref).max()
+    print(f"nearly triangular low n={n} top={top}: {len(low)} low passes, "
           f"diff {d:.2e}, vs op-by-op {dm:.2e}")
     assert d <= TOL_DEEP and abs(norm - 1) <= TOL_DEEP
     assert dm <= TOL_SAME
